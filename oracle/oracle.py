@@ -1,0 +1,203 @@
+"""ctypes front end of the FP64 oracle (vx3_oracle.c).
+
+TEST INFRASTRUCTURE ONLY.  The oracle is the checker for the HIP path; only
+tests/, ``__graft_entry__.smoke()`` and the CPU-baseline leg of ``bench.py``
+may import this module.  It never runs inside the product path.
+
+A problem is a plain dict (see ``SPEC_KEYS``) holding the lattice, palette and
+simulator/environment settings, i.e. the information a VXA file carries.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+
+VXO_CPU_SEMANTICS = 1
+VXO_NO_TEMPERATURE = 2
+
+
+class _Material(C.Structure):
+    _fields_ = [("E", C.c_float), ("rho", C.c_float), ("nu", C.c_float), ("cte", C.c_float),
+                ("mu_s", C.c_float), ("mu_k", C.c_float), ("fail_stress", C.c_float),
+                ("fixed", C.c_int), ("is_measured", C.c_int), ("thermal_on_after", C.c_double)]
+
+
+class _Spec(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("cells", C.POINTER(C.c_int8)), ("phase", C.POINTER(C.c_double)),
+                ("voxel_E", C.POINTER(C.c_float)), ("nmat", C.c_int), ("mats", C.POINTER(_Material)),
+                ("lattice_dim", C.c_double), ("dt_frac", C.c_double),
+                ("bond_damp_z", C.c_float), ("col_damp_z", C.c_float), ("slow_damp_z", C.c_float),
+                ("volume_effects", C.c_int), ("enable_collision", C.c_int), ("enable_expansion", C.c_int),
+                ("grav_acc", C.c_float), ("floor_enabled", C.c_int),
+                ("temp_enabled", C.c_int), ("vary_temp", C.c_int),
+                ("temp_base", C.c_double), ("temp_amplitude", C.c_double), ("temp_period", C.c_double)]
+
+
+SPEC_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=1.0, slow_damp_z=1.0,
+                     volume_effects=0, enable_collision=1, enable_expansion=0, grav_acc=-9.81,
+                     floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
+                     temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None)
+MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
+                         fixed=0, is_measured=1, thermal_on_after=0.0)
+
+
+def build_library(force: bool = False) -> str:
+    """Compile build/liboracle.so with make (gcc)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-s", "-C", _HERE, "all"], check=True)
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        build_library()
+        L = C.CDLL(_LIB_PATH)
+        vp, dp, fp, ip = C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_float), C.POINTER(C.c_int)
+        L.vxo_build.restype = vp
+        L.vxo_build.argtypes = [C.POINTER(_Spec)]
+        L.vxo_free.argtypes = [vp]
+        for f in ("vxo_num_voxels", "vxo_num_links", "vxo_num_surface"):
+            getattr(L, f).restype = C.c_int
+            getattr(L, f).argtypes = [vp]
+        L.vxo_time.restype = C.c_double
+        L.vxo_time.argtypes = [vp]
+        L.vxo_recommended_dt.restype = C.c_double
+        L.vxo_recommended_dt.argtypes = [vp]
+        L.vxo_step_dt.restype = C.c_float
+        L.vxo_step_dt.argtypes = [vp]
+        L.vxo_step.restype = C.c_int
+        L.vxo_step.argtypes = [vp, C.c_float, C.c_int, C.c_int]
+        L.vxo_update_links.argtypes = [vp, dp]
+        L.vxo_get_voxels.argtypes = [vp, dp, dp, dp, dp]
+        L.vxo_set_voxels.argtypes = [vp, dp, dp, dp, dp]
+        L.vxo_get_flags.argtypes = [vp, ip, ip]
+        L.vxo_get_link_state.argtypes = [vp, dp, fp]
+        L.vxo_get_temperature.argtypes = [vp, fp, dp]
+        L.vxo_set_temperature.argtypes = [vp, fp]
+        L.vxo_get_topology.argtypes = [vp, ip, ip, ip, ip, ip]
+        L.vxo_center_of_mass.argtypes = [vp, dp]
+        L.vxo_get_contact.argtypes = [vp, dp]
+        L.vxo_get_voxel_material.argtypes = [vp, C.c_int, fp]
+        L.vxo_get_link_material.argtypes = [vp, C.c_int, fp]
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct)) if a is not None else None
+
+
+class OracleModel:
+    """One model advanced by the FP64 oracle."""
+
+    def __init__(self, problem: dict):
+        p = dict(SPEC_DEFAULTS)
+        p.update(problem)
+        nx, ny, nz = p["dims"]
+        self._cells = np.ascontiguousarray(p["cells"], dtype=np.int8).reshape(-1)
+        assert self._cells.size == nx * ny * nz
+        mats = [dict(MATERIAL_DEFAULTS, **m) for m in p["materials"]]
+        self._mats = (_Material * max(1, len(mats)))()
+        for i, m in enumerate(mats):
+            self._mats[i] = _Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"],
+                                      m["fail_stress"], int(m["fixed"]), int(m["is_measured"]),
+                                      m["thermal_on_after"])
+        self._phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
+        self._E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
+        spec = _Spec(nx, ny, nz, _ptr(self._cells, C.c_int8), _ptr(self._phase, C.c_double),
+                     _ptr(self._E, C.c_float), len(mats), self._mats, p["lattice_dim"], p["dt_frac"],
+                     p["bond_damp_z"], p["col_damp_z"], p["slow_damp_z"], int(p["volume_effects"]),
+                     int(p["enable_collision"]), int(p["enable_expansion"]), p["grav_acc"],
+                     int(p["floor_enabled"]), int(p["temp_enabled"]), int(p["vary_temp"]),
+                     p["temp_base"], p["temp_amplitude"], p["temp_period"])
+        self._L = lib()
+        self._h = self._L.vxo_build(C.byref(spec))
+        self.nv = self._L.vxo_num_voxels(self._h)
+        self.nl = self._L.vxo_num_links(self._h)
+        self.ns = self._L.vxo_num_surface(self._h)
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.vxo_free(self._h)
+            self._h = None
+
+    @property
+    def time(self) -> float:
+        return self._L.vxo_time(self._h)
+
+    def recommended_dt(self) -> float:
+        return self._L.vxo_recommended_dt(self._h)
+
+    def step_dt(self) -> float:
+        return self._L.vxo_step_dt(self._h)
+
+    def step(self, nsteps: int = 1, dt: float = -1.0, flags: int = 0) -> int:
+        return self._L.vxo_step(self._h, dt, nsteps, flags)
+
+    def update_links(self) -> np.ndarray:
+        out = np.zeros((self.nl, 12), dtype=np.float64)
+        self._L.vxo_update_links(self._h, _ptr(out, C.c_double))
+        return out
+
+    def state(self):
+        pos = np.zeros((self.nv, 3)); q = np.zeros((self.nv, 4))
+        lm = np.zeros((self.nv, 3)); am = np.zeros((self.nv, 3))
+        self._L.vxo_get_voxels(self._h, _ptr(pos, C.c_double), _ptr(q, C.c_double),
+                               _ptr(lm, C.c_double), _ptr(am, C.c_double))
+        return dict(pos=pos, quat=q, linmom=lm, angmom=am)
+
+    def set_state(self, pos=None, quat=None, linmom=None, angmom=None):
+        c = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (pos, quat, linmom, angmom)]
+        self._L.vxo_set_voxels(self._h, *[_ptr(a, C.c_double) for a in c])
+
+    def flags(self):
+        vf = np.zeros(self.nv, dtype=np.int32); lf = np.zeros(self.nl, dtype=np.int32)
+        self._L.vxo_get_flags(self._h, _ptr(vf, C.c_int), _ptr(lf, C.c_int))
+        return vf, lf
+
+    def link_state(self):
+        s = np.zeros((self.nl, 9)); st = np.zeros(self.nl, dtype=np.float32)
+        self._L.vxo_get_link_state(self._h, _ptr(s, C.c_double), _ptr(st, C.c_float))
+        return s, st
+
+    def temperature(self):
+        t = np.zeros(self.nv, dtype=np.float32); r = np.zeros(self.nl)
+        self._L.vxo_get_temperature(self._h, _ptr(t, C.c_float), _ptr(r, C.c_double))
+        return t, r
+
+    def set_temperature(self, t):
+        t = np.ascontiguousarray(t, dtype=np.float32)
+        self._L.vxo_set_temperature(self._h, _ptr(t, C.c_float))
+
+    def topology(self):
+        lat = np.zeros((self.nv, 3), dtype=np.int32)
+        ln = np.zeros(self.nl, dtype=np.int32); lp = np.zeros(self.nl, dtype=np.int32)
+        la = np.zeros(self.nl, dtype=np.int32); sf = np.zeros(self.ns, dtype=np.int32)
+        self._L.vxo_get_topology(self._h, *[_ptr(a, C.c_int) for a in (lat, ln, lp, la, sf)])
+        return dict(lattice=lat, link_neg=ln, link_pos=lp, link_axis=la, surface=sf)
+
+    def com(self) -> np.ndarray:
+        c = np.zeros(3)
+        self._L.vxo_center_of_mass(self._h, _ptr(c, C.c_double))
+        return c
+
+    def voxel_material(self, v: int) -> np.ndarray:
+        o = np.zeros(12, dtype=np.float32)
+        self._L.vxo_get_voxel_material(self._h, v, _ptr(o, C.c_float))
+        return o
+
+    def link_material(self, l: int) -> np.ndarray:
+        o = np.zeros(11, dtype=np.float32)
+        self._L.vxo_get_link_material(self._h, l, _ptr(o, C.c_float))
+        return o

@@ -1,0 +1,106 @@
+// ref_driver.cpp -- drives the reference's own CPU Voxelyze (src/old, compiled
+// from the reference sources by oracle/Makefile into oracle/_ref/) so the FP64
+// oracle can be checked against the real reference code.  TEST INFRASTRUCTURE.
+//
+// The model is assembled the way CVX_Sim::Import does it
+// (src/VXA/VX_Sim.cpp:59-153): gravity and floor first, then palette materials
+// (CopyMat :368-420 + damping), voxel size, voxels in lattice order, initial
+// temperatures, volume effects off.  Each step sets the VX3 thermal actuation
+// (src/VX3/VX3_VoxelyzeKernel.cu:625-650) with CVX_Voxel::setTemperature and
+// then calls CVoxelyze::doTimeStep (src/old/Voxelyze.cpp:251-284).
+//
+// stdin (whitespace separated):
+//   nx ny nz  lattice_dim  bond col slow  grav_acc floor  temp_amp temp_period vary enable_expansion t0
+//   nmat  then per material: E rho cte mu_s mu_k
+//   nx*ny*nz cell values (0 = empty)
+//   nv phase offsets (cycles)
+//   dt nsteps record_every
+// stdout (binary): per record, nv * (pos3, quat4, vel3, angvel3) doubles,
+//   then a trailing timing line on stderr "seconds <s> voxel_steps <n>".
+#include "Voxelyze.h"
+#include "VX_Voxel.h"
+#include "VX_Material.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <iostream>
+#include <vector>
+
+int main() {
+    int nx, ny, nz, floorOn, vary, enableExpansion, nmat;
+    double L, tempAmp, tempPeriod;
+    float bond, col, slow, gravAcc, t0;
+    std::cin >> nx >> ny >> nz >> L >> bond >> col >> slow >> gravAcc >> floorOn >> tempAmp >> tempPeriod >> vary >>
+        enableExpansion >> t0 >> nmat;
+    CVoxelyze Vx(0.001);
+    Vx.setGravity((float)(-gravAcc / 9.80665));
+    Vx.enableFloor(floorOn != 0);
+    std::vector<CVX_Material *> mats;
+    for (int i = 0; i < nmat; i++) {
+        float E, rho, cte, mus, muk;
+        std::cin >> E >> rho >> cte >> mus >> muk;
+        CVX_Material *pm = Vx.addMaterial();
+        pm->setModelLinear(E);
+        pm->setPoissonsRatio(0.0f);
+        pm->setDensity(rho);
+        pm->setCte(cte);
+        pm->setStaticFriction(mus);
+        pm->setKineticFriction(muk);
+        pm->setGlobalDamping(slow);
+        pm->setInternalDamping(bond);
+        pm->setCollisionDamping(col);
+        mats.push_back(pm);
+    }
+    Vx.setVoxelSize(L);
+    for (int i = 0; i < nx * ny * nz; i++) {
+        int c;
+        std::cin >> c;
+        if (c <= 0) continue;
+        int z = i / (nx * ny), y = (i - z * nx * ny) / nx, x = i - z * nx * ny - y * nx;
+        Vx.setVoxel(mats[c - 1], x, y, z);
+    }
+    int nv = Vx.voxelCount();
+    std::vector<double> phase(nv);
+    for (int i = 0; i < nv; i++) std::cin >> phase[i];
+    float dt;
+    int nsteps, every;
+    std::cin >> dt >> nsteps >> every;
+    for (int i = 0; i < nv; i++) Vx.voxel(i)->setTemperature(t0);
+
+    std::vector<double> rec((size_t)nv * 13);
+    auto emit = [&]() {
+        for (int i = 0; i < nv; i++) {
+            CVX_Voxel *v = Vx.voxel(i);
+            Vec3D<double> p = v->position(), vel = v->velocity(), av = v->angularVelocity();
+            Quat3D<double> q = v->orientation();
+            double *r = &rec[(size_t)i * 13];
+            r[0] = p.x; r[1] = p.y; r[2] = p.z;
+            r[3] = q.w; r[4] = q.x; r[5] = q.y; r[6] = q.z;
+            r[7] = vel.x; r[8] = vel.y; r[9] = vel.z;
+            r[10] = av.x; r[11] = av.y; r[12] = av.z;
+        }
+        fwrite(rec.data(), sizeof(double), rec.size(), stdout);
+    };
+    const double twoPiF = (double)(2 * 3.1415926f);
+    auto t_start = std::chrono::steady_clock::now();
+    double t = 0.0;  // VX3 keeps time in double (VX3_VoxelyzeKernel.cuh:54); CVoxelyze in float
+    for (int s = 0; s < nsteps; s++) {
+        if (vary && tempPeriod > 0) {
+            for (int i = 0; i < nv; i++) {
+                double T = tempAmp * sin(twoPiF * (t / tempPeriod + phase[i]));
+                if (!enableExpansion && T > 0) T = 0;
+                Vx.voxel(i)->setTemperature((float)T);
+            }
+        }
+        if (!Vx.doTimeStep(dt)) {
+            fprintf(stderr, "diverged at step %d\n", s);
+            return 3;
+        }
+        t += dt;
+        if (every > 0 && (s + 1) % every == 0) emit();
+    }
+    double secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
+    fprintf(stderr, "seconds %.6f voxel_steps %.0f\n", secs, (double)nv * nsteps);
+    return 0;
+}

@@ -1,0 +1,977 @@
+/*
+ * vx3_oracle.c -- FP64 CPU restatement of VX3_VoxelyzeKernel::doTimeStep.
+ *
+ * TEST INFRASTRUCTURE ONLY: this file is the checker for the HIP path.  Only
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load it.
+ *
+ * Every function cites the reference lines it restates.  Types follow the
+ * reference exactly: state is double, material constants, temperature, stress
+ * and dt are float, and every mixed float/double expression is written with the
+ * same operand order and promotions as the reference so that the rounding is
+ * the reference's.  Build with -ffp-contract=off (no FMA contraction).
+ */
+#include "vx3_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+typedef struct { double x, y, z; } v3;
+typedef struct { double w, x, y, z; } q4;
+
+/* src/Utils/VX3_Quat3D.h:30-56 */
+#define Q_PI 3.14159265358979
+#define Q_DBL_EPSILONx24 5.328e-15
+#define Q_DISCARD_ANGLE_RAD 1e-7
+#define Q_SMALL_ANGLE_RAD 1.732e-2
+#define Q_SLTHRESH_ACOS2SQRT 2.4e-3
+/* src/old/types.h:57-59 */
+static const float HYSTERESIS_FACTOR = 1.2f;
+static const float SA_BOND_BEND_RAD = 0.05f;
+static const float SA_BOND_EXT_PERC = 0.50f;
+/* src/VX3/VX3_Collision.h:6 */
+#define COLLISION_ENVELOPE_RADIUS 0.625
+
+/* voxel flag bits, src/old/types.h:34-38 */
+#define VF_FLOOR_ENABLED (1 << 2)
+#define VF_FLOOR_STATIC_FRICTION (1 << 3)
+/* link flag bits: src/old/types.h:6-8 (LOCAL_VELOCITY_VALID) + smallAngle */
+#define LF_LOCAL_VELOCITY_VALID 1
+#define LF_SMALL_ANGLE 2
+
+typedef struct {
+    double nomSize;
+    float E, nu, rho, alphaCTE, muS, muK, zetaI, zetaG, zetaC, eHat, gravMult;
+    float mass, massInv, sqrtMass, inertia, inertiaInv, c2xSqMxExS, c2xSqIxExSxSxS;
+    float sigmaFail, epsFail;
+    int fixed, isMeasured;
+    double thermalOnAfter;
+} vmat;
+
+typedef struct {
+    double nomSize;
+    float E, nu, eHat, epsFail;
+    float a1, a2, b1, b2, b3, sqA1, sqA2xIp, sqB1, sqB2xFMp, sqB3xIp;
+} lmat;
+
+struct vxo_model {
+    int nv, nl, ns;
+    double voxSize;
+    /* voxels */
+    int *ix, *iy, *iz;
+    int *vmat_of;          /* index into vm */
+    int (*vlinks)[6];      /* link index per direction X_POS..Z_NEG, -1 none */
+    v3 *pos, *linMom, *angMom, *contact;
+    q4 *orient;
+    float *temp;
+    float *prevDt;
+    int *vflags;
+    double *phase;
+    int *surface;          /* ns voxel indices, voxel order */
+    int *is_surface;
+    /* links */
+    int *lneg, *lpos, *laxis;
+    lmat *lm;
+    v3 *pos2, *a1v, *a2v;
+    float *strain, *maxStrain, *tArea, *tStrainSum;
+    double *restLen;
+    int *lflags;
+    v3 *fNeg, *fPos, *mNeg, *mPos;
+    /* materials */
+    int nvm;
+    vmat *vm;
+    /* settings */
+    double dtFrac, optimalDt, time;
+    int enableCollision, enableExpansion, varyTemp;
+    double tempAmplitude, tempPeriod;
+};
+
+/* ------------------------------------------------------------------ math */
+static inline v3 v3_(double x, double y, double z) { v3 r = {x, y, z}; return r; }
+static inline v3 vadd(v3 a, v3 b) { return v3_(a.x + b.x, a.y + b.y, a.z + b.z); }
+static inline v3 vsub(v3 a, v3 b) { return v3_(a.x - b.x, a.y - b.y, a.z - b.z); }
+static inline v3 vscale(v3 a, double s) { return v3_(a.x * s, a.y * s, a.z * s); }
+static inline double vlen(v3 a) { return sqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+/* VX3_Quat3D.h:217-222 */
+static inline q4 qmul(q4 a, q4 f) {
+    q4 r;
+    r.w = a.w * f.w - a.x * f.x - a.y * f.y - a.z * f.z;
+    r.x = a.w * f.x + a.x * f.w + a.y * f.z - a.z * f.y;
+    r.y = a.w * f.y - a.x * f.z + a.y * f.w + a.z * f.x;
+    r.z = a.w * f.z + a.x * f.y - a.y * f.x + a.z * f.w;
+    return r;
+}
+static inline q4 qconj(q4 a) { q4 r = {a.w, -a.x, -a.y, -a.z}; return r; }
+/* VX3_Quat3D.h:439-447 */
+static inline v3 qrot(q4 q, v3 f) {
+    double fx = f.x, fy = f.y, fz = f.z;
+    double tw = fx * q.x + fy * q.y + fz * q.z;
+    double tx = fx * q.w - fy * q.z + fz * q.y;
+    double ty = fx * q.z + fy * q.w - fz * q.x;
+    double tz = -fx * q.y + fy * q.x + fz * q.w;
+    return v3_(q.w * tx + q.x * tw + q.y * tz - q.z * ty, q.w * ty - q.x * tz + q.y * tw + q.z * tx,
+               q.w * tz + q.x * ty - q.y * tx + q.z * tw);
+}
+/* VX3_Quat3D.h:465-474 */
+static inline v3 qrotinv(q4 q, v3 f) {
+    double fx = f.x, fy = f.y, fz = f.z;
+    double tw = q.x * fx + q.y * fy + q.z * fz;
+    double tx = q.w * fx - q.y * fz + q.z * fy;
+    double ty = q.w * fy + q.x * fz - q.z * fx;
+    double tz = q.w * fz - q.x * fy + q.y * fx;
+    return v3_(tw * q.x + tx * q.w + ty * q.z - tz * q.y, tw * q.y - tx * q.z + ty * q.w + tz * q.x,
+               tw * q.z + tx * q.y - ty * q.x + tz * q.w);
+}
+/* VX3_Quat3D.h:347-359 */
+static inline v3 q_to_rotvec(q4 q) {
+    if (q.w >= 1.0 || q.w <= -1.0) return v3_(0, 0, 0);
+    double sl = 1.0 - q.w * q.w;
+    if (sl < Q_SLTHRESH_ACOS2SQRT) {
+        double s = sqrt((2 - 2 * q.w) / sl);
+        return v3_(q.x * 2.0 * s, q.y * 2.0 * s, q.z * 2.0 * s);
+    }
+    double a = acos(q.w), d = sqrt(sl);
+    return v3_(q.x * 2.0 * a / d, q.y * 2.0 * a / d, q.z * 2.0 * a / d);
+}
+/* VX3_Quat3D.h:363-378 */
+static inline q4 q_from_rotvec(v3 in) {
+    v3 th = v3_(in.x / 2, in.y / 2, in.z / 2);
+    double s, m2 = th.x * th.x + th.y * th.y + th.z * th.z;
+    q4 r;
+    if (m2 * m2 < Q_DBL_EPSILONx24) {
+        r.w = 1.0 - 0.5 * m2;
+        s = 1.0 - m2 / 6.0;
+    } else {
+        double m = sqrt(m2);
+        r.w = cos(m);
+        s = sin(m) / m;
+    }
+    r.x = th.x * s;
+    r.y = th.y * s;
+    r.z = th.z * s;
+    return r;
+}
+/* VX3_Quat3D.h:384-432; q is the identity on entry (VX3_Link.cu:103,120) */
+static inline q4 q_from_angle_to_posx(v3 rf) {
+    q4 q = {1, 0, 0, 0};
+    if (rf.x == 0 && rf.y == 0 && rf.z == 0) return q;
+    double yx = rf.y / rf.x, zx = rf.z / rf.x;
+    if (yx < Q_SMALL_ANGLE_RAD && yx > -Q_SMALL_ANGLE_RAD && zx < Q_SMALL_ANGLE_RAD && zx > -Q_SMALL_ANGLE_RAD) {
+        q.x = 0;
+        q.y = 0.5 * zx;
+        q.z = -0.5 * yx;
+        q.w = 1 + 0.5 * (-q.y * q.y - q.z * q.z);
+        return q;
+    }
+    v3 n = rf;
+    double l = sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+    if (l > 0) {
+        double li = 1.0 / l;
+        n.x *= li;
+        n.y *= li;
+        n.z *= li;
+    }
+    double theta = acos(n.x);
+    if (theta > Q_PI - Q_DISCARD_ANGLE_RAD) {
+        q.w = 0; q.x = 0; q.y = 1; q.z = 0;
+        return q;
+    }
+    double ami = 1.0 / sqrt(n.z * n.z + n.y * n.y);
+    double a = 0.5 * theta, s = sin(a);
+    q.w = cos(a);
+    q.x = 0;
+    q.y = n.z * ami * s;
+    q.z = -n.y * ami * s;
+    return q;
+}
+/* VX3_Link.h:134-153 (by value) */
+static inline v3 to_axis_x(v3 v, int axis) {
+    if (axis == 1) return v3_(v.y, -v.x, v.z);
+    if (axis == 2) return v3_(v.z, v.y, -v.x);
+    return v;
+}
+static inline q4 to_axis_x_q(q4 q, int axis) {
+    q4 r = q;
+    if (axis == 1) { r.x = q.y; r.y = -q.x; }
+    if (axis == 2) { r.x = q.z; r.z = -q.x; }
+    return r;
+}
+/* VX3_Link.h:156-173 (in place) */
+static inline v3 to_axis_original(v3 v, int axis) {
+    if (axis == 1) return v3_(-v.y, v.x, v.z);
+    if (axis == 2) return v3_(-v.z, v.y, v.x);
+    return v;
+}
+
+/* ------------------------------------------------------------- materials */
+/* CVX_Material::updateDerived src/old/VX_Material.cpp:542-548 and
+ * CVX_MaterialVoxel::updateDerived src/old/VX_MaterialVoxel.cpp:57-79 */
+static void vmat_derive(vmat *m) {
+    m->eHat = m->E / ((1 - 2 * m->nu) * (1 + m->nu));
+    double ns = m->nomSize;
+    double volume = ns * ns * ns;
+    m->mass = (float)(volume * m->rho);
+    m->inertia = (float)(m->mass * ns * ns / 6.0f);
+    if (volume == 0 || m->mass == 0 || m->inertia == 0) {
+        m->massInv = m->sqrtMass = m->inertiaInv = m->c2xSqMxExS = m->c2xSqIxExSxSxS = 0.0f;
+        return;
+    }
+    m->massInv = 1.0f / m->mass;
+    m->sqrtMass = (float)sqrt((double)m->mass);
+    m->inertiaInv = 1.0f / m->inertia;
+    m->c2xSqMxExS = (float)(2.0f * sqrt((double)(m->mass * m->E) * ns));
+    m->c2xSqIxExSxSxS = (float)(2.0f * sqrt((double)(m->inertia * m->E) * ns * ns * ns));
+}
+
+/* CVX_Material::setPoissonsRatio src/old/VX_Material.cpp:488-494 */
+static float clamp_nu(float nu) {
+    if (nu < 0) nu = 0;
+    if (nu >= 0.5) nu = (float)(0.5 - FLT_EPSILON * 2);
+    return nu;
+}
+
+/* CVX_MaterialLink::updateAll/updateDerived src/old/VX_MaterialLink.cpp:45-141 */
+static void lmat_combine(lmat *l, const vmat *a, const vmat *b) {
+    l->nomSize = 0.5 * (a->nomSize + b->nomSize);
+    float f1 = a->sigmaFail, f2 = b->sigmaFail, sf;
+    if (f1 == -1.0f) sf = f2;
+    else if (f2 == -1.0f) sf = f1;
+    else sf = f1 < f2 ? f1 : f2;
+    l->E = 2.0f * a->E * b->E / (a->E + b->E);
+    l->epsFail = (sf == -1.0f) ? -1.0f : sf / l->E;  /* setModelLinear VX_Material.cpp */
+    if (a->nu == 0 && b->nu == 0) l->nu = 0;
+    else {
+        float tmpEHat = 2 * a->eHat * b->eHat / (a->eHat + b->eHat);
+        float tmpE = l->E;
+        float c2 = (tmpEHat - tmpE) / (2 * tmpEHat) + 0.0625;
+        l->nu = (float)(sqrtf(c2) - 0.25);
+    }
+    l->eHat = l->E / ((1 - 2 * l->nu) * (1 + l->nu));
+    float L = (float)l->nomSize;
+    l->a1 = l->E * L;
+    l->a2 = l->E * L * L * L / (12.0f * (1 + l->nu));
+    l->b1 = l->E * L;
+    l->b2 = l->E * L * L / 2.0f;
+    l->b3 = l->E * L * L * L / 6.0f;
+    l->sqA1 = (float)sqrt((double)l->a1);
+    l->sqA2xIp = (float)sqrt((double)(l->a2 * L * L / 6.0f));
+    l->sqB1 = (float)sqrt((double)l->b1);
+    l->sqB2xFMp = (float)sqrt((double)(l->b2 * L / 2.0f));
+    l->sqB3xIp = (float)sqrt((double)(l->b3 * L * L / 6.0f));
+}
+
+/* ----------------------------------------------------------------- build */
+static void *xcalloc(size_t n, size_t s) {
+    void *p = calloc(n ? n : 1, s);
+    if (!p) abort();
+    return p;
+}
+
+/* baseSize(axis): VX3_Voxel.h:96-100.  (1 + tempe*alphaCTE) is float math. */
+static inline double base_size(const vxo_model *m, int v) {
+    const vmat *mt = &m->vm[m->vmat_of[v]];
+    float f = 1 + m->temp[v] * mt->alphaCTE;
+    return mt->nomSize * (double)f;
+}
+/* baseSizeAverage: VX3_Voxel.h:102-105 */
+static inline double base_size_average(const vxo_model *m, int v) {
+    double b = base_size(m, v);
+    return (b + b + b) / 3.0f;
+}
+/* floorPenetration: VX3_Voxel.h:185-188 */
+static inline float floor_penetration(const vxo_model *m, int v) {
+    const vmat *mt = &m->vm[m->vmat_of[v]];
+    return (float)(base_size_average(m, v) / 2 - mt->nomSize / 2 - m->pos[v].z);
+}
+/* updateRestLength: VX3_Link.cu:80-83 */
+static inline void update_rest_length(vxo_model *m, int l) {
+    m->restLen[l] = 0.5 * (base_size(m, m->lneg[l]) + base_size(m, m->lpos[l]));
+}
+
+/* recommendedTimeStep: VX3_VoxelyzeKernel.cu:184-217 (axialStiffness with
+ * nu == 0 returns _a1, VX3_Link.cu:269-278; with nu != 0 the frozen
+ * transverse area and the current rest length are used) */
+double vxo_recommended_dt(const vxo_model *m) {
+    double maxFreq2 = 0.0f;
+    for (int i = 0; i < m->nl; i++) {
+        const lmat *lm = &m->lm[i];
+        double m1 = m->vm[m->vmat_of[m->lneg[i]]].mass, m2 = m->vm[m->vmat_of[m->lpos[i]]].mass;
+        float k;
+        if (lm->nu == 0.0f) k = lm->a1;
+        else k = (float)(lm->eHat * m->tArea[i] / ((m->strain[i] + 1) * m->restLen[i]));
+        double f2 = k / (m1 < m2 ? m1 : m2);
+        if (f2 > maxFreq2) maxFreq2 = f2;
+    }
+    if (maxFreq2 <= 0.0f) {
+        for (int i = 0; i < m->nv; i++) {
+            const vmat *mt = &m->vm[m->vmat_of[i]];
+            double f2 = mt->E * mt->nomSize / mt->mass;
+            if (f2 > maxFreq2) maxFreq2 = f2;
+        }
+    }
+    if (maxFreq2 <= 0.0f) return 0.0f;
+    return 1.0f / (6.283185f * sqrt(maxFreq2));
+}
+
+float vxo_step_dt(vxo_model *m) {
+    /* VX3_VoxelyzeKernel.cu:244-253 */
+    if (!m->optimalDt) m->optimalDt = vxo_recommended_dt(m);
+    if (m->optimalDt < 1e-10) m->optimalDt = 1e-10;
+    return (float)(m->dtFrac * m->optimalDt);
+}
+
+/*
+ * Model construction follows CVX_Sim::Import (src/VXA/VX_Sim.cpp:59-153):
+ * palette materials -> CopyMat, damping, voxel size, voxels in lattice order
+ * (x fastest; CVoxelyze::addVoxel src/old/Voxelyze.cpp:439-461), links created
+ * when the second voxel of a pair is added in the X_POS..Z_NEG loop
+ * (addLink :508-539; CVX_Link ctor src/old/VX_Link.cpp:21-48 puts the lower
+ * lattice coordinate at pVNeg), initial temperatures (UpdateMatTemps
+ * VX_Sim.cpp:467-483 with CVX_Environment::UpdateCurTemp
+ * src/old/VX_Environment.cpp:327-346), volume effects (VX_Sim.cpp:445-458).
+ */
+vxo_model *vxo_build(const vxo_spec *sp) {
+    vxo_model *m = (vxo_model *)xcalloc(1, sizeof(vxo_model));
+    int nx = sp->nx, ny = sp->ny, nz = sp->nz, ncell = nx * ny * nz;
+    m->voxSize = sp->lattice_dim;
+    m->dtFrac = sp->dt_frac;
+    m->enableCollision = sp->enable_collision;
+    m->enableExpansion = sp->enable_expansion;
+    m->varyTemp = sp->vary_temp;
+    m->tempAmplitude = sp->temp_amplitude;
+    m->tempPeriod = sp->temp_period;
+
+    int nv = 0;
+    for (int i = 0; i < ncell; i++)
+        if (sp->cells[i] > 0 && sp->cells[i] <= sp->nmat) nv++;
+    m->nv = nv;
+
+    /* gravity multiplier: CVX_Sim::SetGravityAccel VX_Sim.cpp:485 */
+    float gravMult = (float)(-(float)sp->grav_acc / 9.80665);
+    float zI = sp->bond_damp_z <= 0 ? 0 : sp->bond_damp_z;
+    float zG = sp->slow_damp_z <= 0 ? 0 : sp->slow_damp_z;
+    float zC = sp->col_damp_z <= 0 ? 0 : sp->col_damp_z;
+
+    /* one voxel material per palette entry, plus one per voxel when E is overridden */
+    int npal = sp->nmat;
+    m->nvm = npal + (sp->voxel_E ? nv : 0);
+    m->vm = (vmat *)xcalloc(m->nvm, sizeof(vmat));
+    for (int k = 0; k < npal; k++) {
+        const vxo_material *in = &sp->mats[k];
+        vmat *vm = &m->vm[k];
+        vm->nomSize = sp->lattice_dim;
+        vm->E = in->E;
+        vm->sigmaFail = in->fail_stress;
+        vm->epsFail = (in->fail_stress == -1.0f) ? -1.0f : in->fail_stress / in->E;
+        vm->nu = sp->volume_effects ? clamp_nu(in->nu) : 0.0f;
+        vm->rho = in->rho <= 0 ? FLT_MIN : in->rho;
+        vm->alphaCTE = in->cte;
+        vm->muS = in->mu_s <= 0 ? 0 : in->mu_s;
+        vm->muK = in->mu_k <= 0 ? 0 : in->mu_k;
+        vm->zetaI = zI;
+        vm->zetaG = zG;
+        vm->zetaC = zC;
+        vm->gravMult = gravMult;
+        vm->fixed = in->fixed;
+        vm->isMeasured = in->is_measured;
+        vm->thermalOnAfter = in->thermal_on_after;
+        vmat_derive(vm);
+    }
+
+    m->ix = (int *)xcalloc(nv, sizeof(int));
+    m->iy = (int *)xcalloc(nv, sizeof(int));
+    m->iz = (int *)xcalloc(nv, sizeof(int));
+    m->vmat_of = (int *)xcalloc(nv, sizeof(int));
+    m->vlinks = (int(*)[6])xcalloc(nv, sizeof(int[6]));
+    m->pos = (v3 *)xcalloc(nv, sizeof(v3));
+    m->linMom = (v3 *)xcalloc(nv, sizeof(v3));
+    m->angMom = (v3 *)xcalloc(nv, sizeof(v3));
+    m->contact = (v3 *)xcalloc(nv, sizeof(v3));
+    m->orient = (q4 *)xcalloc(nv, sizeof(q4));
+    m->temp = (float *)xcalloc(nv, sizeof(float));
+    m->prevDt = (float *)xcalloc(nv, sizeof(float));
+    m->vflags = (int *)xcalloc(nv, sizeof(int));
+    m->phase = (double *)xcalloc(nv, sizeof(double));
+    m->is_surface = (int *)xcalloc(nv, sizeof(int));
+
+    int *cellvox = (int *)xcalloc(ncell, sizeof(int));
+    int v = 0;
+    for (int i = 0; i < ncell; i++) {
+        cellvox[i] = -1;
+        if (!(sp->cells[i] > 0 && sp->cells[i] <= sp->nmat)) continue;
+        int z = i / (nx * ny), y = (i - z * nx * ny) / nx, x = i - z * nx * ny - y * nx;
+        cellvox[i] = v;
+        m->ix[v] = x; m->iy[v] = y; m->iz[v] = z;
+        int pal = sp->cells[i] - 1;
+        if (sp->voxel_E) {
+            int k = npal + v;
+            m->vm[k] = m->vm[pal];
+            m->vm[k].E = sp->voxel_E[v];
+            m->vm[k].epsFail = (m->vm[k].sigmaFail == -1.0f) ? -1.0f : m->vm[k].sigmaFail / m->vm[k].E;
+            vmat_derive(&m->vm[k]);
+            m->vmat_of[v] = k;
+        } else {
+            m->vmat_of[v] = pal;
+        }
+        m->pos[v] = v3_(x * m->voxSize, y * m->voxSize, z * m->voxSize);
+        m->orient[v].w = 1;
+        m->vflags[v] = VF_FLOOR_STATIC_FRICTION | (sp->floor_enabled ? VF_FLOOR_ENABLED : 0);
+        m->phase[v] = sp->phase ? sp->phase[v] : 0.0;
+        for (int d = 0; d < 6; d++) m->vlinks[v][d] = -1;
+        v++;
+    }
+
+    /* links: for each new voxel, directions X_NEG, Y_NEG, Z_NEG find an existing neighbour */
+    int nlmax = 3 * nv;
+    m->lneg = (int *)xcalloc(nlmax, sizeof(int));
+    m->lpos = (int *)xcalloc(nlmax, sizeof(int));
+    m->laxis = (int *)xcalloc(nlmax, sizeof(int));
+    int nl = 0;
+    for (v = 0; v < nv; v++) {
+        int x = m->ix[v], y = m->iy[v], z = m->iz[v];
+        int nb[3] = {x > 0 ? cellvox[(x - 1) + nx * (y + ny * z)] : -1,
+                     y > 0 ? cellvox[x + nx * ((y - 1) + ny * z)] : -1,
+                     z > 0 ? cellvox[x + nx * (y + ny * (z - 1))] : -1};
+        for (int a = 0; a < 3; a++) {
+            if (nb[a] < 0) continue;
+            m->lneg[nl] = nb[a];
+            m->lpos[nl] = v;
+            m->laxis[nl] = a;
+            m->vlinks[v][2 * a + 1] = nl;     /* new voxel: link in the NEG direction */
+            m->vlinks[nb[a]][2 * a] = nl;     /* neighbour: link in the POS direction */
+            nl++;
+        }
+    }
+    free(cellvox);
+    m->nl = nl;
+    m->lm = (lmat *)xcalloc(nl, sizeof(lmat));
+    m->pos2 = (v3 *)xcalloc(nl, sizeof(v3));
+    m->a1v = (v3 *)xcalloc(nl, sizeof(v3));
+    m->a2v = (v3 *)xcalloc(nl, sizeof(v3));
+    m->strain = (float *)xcalloc(nl, sizeof(float));
+    m->maxStrain = (float *)xcalloc(nl, sizeof(float));
+    m->tArea = (float *)xcalloc(nl, sizeof(float));
+    m->tStrainSum = (float *)xcalloc(nl, sizeof(float));
+    m->restLen = (double *)xcalloc(nl, sizeof(double));
+    m->lflags = (int *)xcalloc(nl, sizeof(int));
+    m->fNeg = (v3 *)xcalloc(nl, sizeof(v3));
+    m->fPos = (v3 *)xcalloc(nl, sizeof(v3));
+    m->mNeg = (v3 *)xcalloc(nl, sizeof(v3));
+    m->mPos = (v3 *)xcalloc(nl, sizeof(v3));
+    for (int l = 0; l < nl; l++) {
+        /* combinedMaterial(voxel1 = new voxel, voxel2 = neighbour): symmetric */
+        lmat_combine(&m->lm[l], &m->vm[m->vmat_of[m->lpos[l]]], &m->vm[m->vmat_of[m->lneg[l]]]);
+        m->lflags[l] = LF_SMALL_ANGLE;  /* CVX_Link::reset VX_Link.cpp:50-63 */
+        /* transverse info at creation (strains zero): VX3_Voxel.cu:496-513 */
+        float s = (float)m->vm[m->vmat_of[m->lneg[l]]].nomSize;
+        float s2 = (float)m->vm[m->vmat_of[m->lpos[l]]].nomSize;
+        m->tArea[l] = 0.5f * (s * s + s2 * s2);
+        m->tStrainSum[l] = 0.0f;
+    }
+
+    /* surface voxels: fewer than 6 links (VX3_Voxel.cu:515-524) */
+    m->surface = (int *)xcalloc(nv, sizeof(int));
+    int ns = 0;
+    for (v = 0; v < nv; v++) {
+        int c = 0;
+        for (int d = 0; d < 6; d++) c += m->vlinks[v][d] >= 0;
+        if (c < 6) { m->is_surface[v] = 1; m->surface[ns++] = v; }
+    }
+    m->ns = ns;
+
+    /* initial temperature (UpdateMatTemps) */
+    float t0 = 0.0f;
+    if (sp->temp_enabled) {
+        float cur;
+        if (sp->vary_temp) cur = (sp->temp_period == 0) ? 0.0f : (float)(sp->temp_base + sp->temp_amplitude * sin(0.0));
+        else cur = (float)(sp->temp_base + sp->temp_amplitude);
+        t0 = (float)(cur - sp->temp_base);
+    }
+    for (v = 0; v < nv; v++) m->temp[v] = t0;
+    for (int l = 0; l < nl; l++) update_rest_length(m, l);
+    return m;
+}
+
+void vxo_free(vxo_model *m) {
+    if (!m) return;
+    void *ptrs[] = {m->ix, m->iy, m->iz, m->vmat_of, m->vlinks, m->pos, m->linMom, m->angMom, m->contact,
+                    m->orient, m->temp, m->prevDt, m->vflags, m->phase, m->surface, m->is_surface, m->lneg,
+                    m->lpos, m->laxis, m->lm, m->pos2, m->a1v, m->a2v, m->strain, m->maxStrain, m->tArea,
+                    m->tStrainSum, m->restLen, m->lflags, m->fNeg, m->fPos, m->mNeg, m->mPos, m->vm};
+    for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
+    free(m);
+}
+
+int vxo_num_voxels(const vxo_model *m) { return m->nv; }
+int vxo_num_links(const vxo_model *m) { return m->nl; }
+int vxo_num_surface(const vxo_model *m) { return m->ns; }
+double vxo_time(const vxo_model *m) { return m->time; }
+
+/* ------------------------------------------------------------ temperature */
+/* gpu_update_temperature: VX3_VoxelyzeKernel.cu:625-650 */
+static void update_temperature(vxo_model *m) {
+    if (!(m->varyTemp && m->tempPeriod > 0)) return;
+    const double twoPiF = (double)(2 * 3.1415926f);
+    for (int v = 0; v < m->nv; v++) {
+        const vmat *mt = &m->vm[m->vmat_of[v]];
+        if (mt->thermalOnAfter > m->time) continue;
+        if (mt->fixed) continue;
+        double T = m->tempAmplitude * sin(twoPiF * (m->time / m->tempPeriod + m->phase[v]));
+        if (!m->enableExpansion && T > 0) T = 0;
+        m->temp[v] = (float)T;
+    }
+    for (int l = 0; l < m->nl; l++) update_rest_length(m, l);
+}
+
+/* ------------------------------------------------------------------ links */
+/* VX3_Voxel::dampingMultiplier VX3_Voxel.h:206-208 */
+static inline float damping_multiplier(const vxo_model *m, int v) {
+    const vmat *mt = &m->vm[m->vmat_of[v]];
+    return 2 * mt->sqrtMass * mt->zetaI / m->prevDt[v];
+}
+
+/* VX3_Link::updateForces VX3_Link.cu:135-218 with orientLink :90-133 and
+ * updateStrain :220-260 (linear material model, VX3_Material.cu:90-124) */
+static void link_update(vxo_model *m, int l) {
+    const lmat *mat = &m->lm[l];
+    int n = m->lneg[l], p = m->lpos[l], axis = m->laxis[l];
+    v3 oldPos2 = m->pos2[l], oldA1v = m->a1v[l], oldA2v = m->a2v[l];
+    double restLen = m->restLen[l];
+    int flags = m->lflags[l];
+
+    /* orientLink */
+    v3 pos2 = to_axis_x(vsub(m->pos[p], m->pos[n]), axis);
+    q4 angle1 = to_axis_x_q(m->orient[n], axis);
+    q4 angle2 = to_axis_x_q(m->orient[p], axis);
+    q4 totalRot = qconj(angle1);
+    pos2 = qrot(totalRot, pos2);
+    angle2 = qmul(totalRot, angle2);
+    angle1.w = 1; angle1.x = angle1.y = angle1.z = 0;
+
+    float smallTurn = (float)((fabs(pos2.z) + fabs(pos2.y)) / pos2.x);
+    float extendPerc = (float)(fabs(1 - pos2.x / restLen));
+    int smallAngle = (flags & LF_SMALL_ANGLE) != 0;
+    if (!smallAngle && smallTurn < SA_BOND_BEND_RAD && extendPerc < SA_BOND_EXT_PERC) {
+        smallAngle = 1;
+        flags &= ~LF_LOCAL_VELOCITY_VALID;
+    } else if (smallAngle && (smallTurn > HYSTERESIS_FACTOR * SA_BOND_BEND_RAD ||
+                              extendPerc > HYSTERESIS_FACTOR * SA_BOND_EXT_PERC)) {
+        smallAngle = 0;
+        flags &= ~LF_LOCAL_VELOCITY_VALID;
+    }
+    if (smallAngle) {
+        pos2.x -= restLen;
+    } else {
+        angle1 = q_from_angle_to_posx(pos2);
+        totalRot = qmul(angle1, totalRot);
+        angle2 = qmul(angle1, angle2);
+        pos2 = v3_(vlen(pos2) - restLen, 0, 0);
+    }
+    v3 a1v = q_to_rotvec(angle1);
+    v3 a2v = q_to_rotvec(angle2);
+    flags = smallAngle ? (flags | LF_SMALL_ANGLE) : (flags & ~LF_SMALL_ANGLE);
+    m->pos2[l] = pos2;
+    m->a1v[l] = a1v;
+    m->a2v[l] = a2v;
+
+    v3 dPos2 = vscale(vsub(pos2, oldPos2), 0.5);
+    v3 dA1 = vscale(vsub(a1v, oldA1v), 0.5);
+    v3 dA2 = vscale(vsub(a2v, oldA2v), 0.5);
+    /* dPos2 = 0.5*(pos2-oldPos2): the reference multiplies 0.5*vec, identical */
+
+    /* updateStrain (linear): VX3_Link.cu:220-231, VX3_Material.cu:90-101 */
+    float axialStrain = (float)(pos2.x / restLen);
+    m->strain[l] = axialStrain;
+    if (axialStrain > m->maxStrain[l]) m->maxStrain[l] = axialStrain;
+    float stress;
+    if (mat->epsFail != -1.0f && axialStrain > mat->epsFail) stress = 0.0f;
+    else if (mat->nu == 0.0f) stress = mat->E * axialStrain;
+    else stress = mat->eHat * ((1 - mat->nu) * axialStrain + mat->nu * m->tStrainSum[l]);
+
+    if (mat->epsFail != -1.0f && m->maxStrain[l] > mat->epsFail) {
+        m->fNeg[l] = m->fPos[l] = m->mNeg[l] = m->mPos[l] = v3_(0, 0, 0);
+        m->lflags[l] = flags;
+        return;
+    }
+    float b1 = mat->b1, b2 = mat->b2, b3 = mat->b3, a2 = mat->a2;
+    v3 fNeg = v3_((double)(stress * m->tArea[l]), b1 * pos2.y - b2 * (a1v.z + a2v.z), b1 * pos2.z + b2 * (a1v.y + a2v.y));
+    v3 fPos = v3_(-fNeg.x, -fNeg.y, -fNeg.z);
+    v3 mNeg = v3_(a2 * (a2v.x - a1v.x), -b2 * pos2.z - b3 * (2 * a1v.y + a2v.y), b2 * pos2.y - b3 * (2 * a1v.z + a2v.z));
+    v3 mPos = v3_(a2 * (a1v.x - a2v.x), -b2 * pos2.z - b3 * (a1v.y + 2 * a2v.y), b2 * pos2.y - b3 * (a1v.z + 2 * a2v.z));
+
+    if (flags & LF_LOCAL_VELOCITY_VALID) {
+        float sqA1 = mat->sqA1, sqA2xIp = mat->sqA2xIp, sqB1 = mat->sqB1, sqB2xFMp = mat->sqB2xFMp, sqB3xIp = mat->sqB3xIp;
+        v3 posCalc = v3_(sqA1 * dPos2.x, sqB1 * dPos2.y - sqB2xFMp * (dA1.z + dA2.z),
+                         sqB1 * dPos2.z + sqB2xFMp * (dA1.y + dA2.y));
+        float dmN = damping_multiplier(m, n), dmP = damping_multiplier(m, p);
+        fNeg = vadd(fNeg, vscale(posCalc, dmN));
+        fPos = vsub(fPos, vscale(posCalc, dmP));
+        double hN = 0.5 * dmN, hP = 0.5 * dmP;
+        v3 tN = v3_(-sqA2xIp * (dA2.x - dA1.x), sqB2xFMp * dPos2.z + sqB3xIp * (2 * dA1.y + dA2.y),
+                    -sqB2xFMp * dPos2.y + sqB3xIp * (2 * dA1.z + dA2.z));
+        v3 tP = v3_(sqA2xIp * (dA2.x - dA1.x), sqB2xFMp * dPos2.z + sqB3xIp * (dA1.y + 2 * dA2.y),
+                    -sqB2xFMp * dPos2.y + sqB3xIp * (dA1.z + 2 * dA2.z));
+        mNeg = vsub(mNeg, vscale(tN, hN));
+        mPos = vsub(mPos, vscale(tP, hP));
+    } else {
+        flags |= LF_LOCAL_VELOCITY_VALID;
+    }
+    if (!smallAngle) {
+        fNeg = qrotinv(angle1, fNeg);
+        mNeg = qrotinv(angle1, mNeg);
+    }
+    fPos = qrotinv(angle2, fPos);
+    mPos = qrotinv(angle2, mPos);
+    m->fNeg[l] = to_axis_original(fNeg, axis);
+    m->fPos[l] = to_axis_original(fPos, axis);
+    m->mNeg[l] = to_axis_original(mNeg, axis);
+    m->mPos[l] = to_axis_original(mPos, axis);
+    m->lflags[l] = flags;
+}
+
+static void update_links(vxo_model *m) {
+    int nl = m->nl;
+#pragma omp parallel for schedule(static)
+    for (int l = 0; l < nl; l++) {
+        /* gpu_update_links VX3_VoxelyzeKernel.cu:566-581: skip links between two fixed voxels */
+        if (m->vm[m->vmat_of[m->lpos[l]]].fixed && m->vm[m->vmat_of[m->lneg[l]]].fixed) continue;
+        link_update(m, l);
+    }
+}
+
+/* ------------------------------------------------------------- collisions */
+typedef struct { long long key; int s; } cellent;
+static int cellent_cmp(const void *a, const void *b) {
+    const cellent *x = (const cellent *)a, *y = (const cellent *)b;
+    if (x->key != y->key) return x->key < y->key ? -1 : 1;
+    return x->s - y->s;
+}
+static inline long long cell_key(long long cx, long long cy, long long cz) {
+    return ((cx + (1LL << 20)) << 42) | ((cy + (1LL << 20)) << 21) | (cz + (1LL << 20));
+}
+
+/* handle_collision_attachment (collision part) VX3_VoxelyzeKernel.cu:682-729
+ * with VX3_Collision VX3_Collision.cu:3-31, for voxel1 = surface[first],
+ * voxel2 = surface[second], first > second (gpu_update_attach :833-843). */
+static void collide_pair(vxo_model *m, int v1, int v2) {
+    const vmat *m1 = &m->vm[m->vmat_of[v1]], *m2 = &m->vm[m->vmat_of[v2]];
+    if (m1->fixed && m2->fixed) return;
+    v3 diff = vsub(m->pos[v1], m->pos[v2]);
+    double bsa1 = base_size_average(m, v1), bsa2 = base_size_average(m, v2);
+    double watch = (bsa1 + bsa2) * COLLISION_ENVELOPE_RADIUS;
+    if (diff.x > watch || diff.x < -watch) return;
+    if (diff.y > watch || diff.y < -watch) return;
+    if (diff.z > watch || diff.z < -watch) return;
+    if (vlen(diff) > watch) return;
+    /* is_neighbor(voxel1, voxel2, NULL, 1) :651-680 */
+    for (int d = 0; d < 6; d++) {
+        int l = m->vlinks[v1][d];
+        if (l < 0) continue;
+        int other = m->lneg[l] == v1 ? m->lpos[l] : m->lneg[l];
+        if (other == v2) return;
+    }
+    if (!m->enableCollision) return;
+    float pk1 = (float)(2 * m1->E * m1->nomSize), pk2 = (float)(2 * m2->E * m2->nomSize);
+    double penStiff = 2.0f / (1.0f / pk1 + 1.0f / pk2);
+    double dampC = 0.5f * (m1->zetaC * m1->c2xSqMxExS + m2->zetaC * m2->c2xSqMxExS);
+    v3 offset = vsub(m->pos[v2], m->pos[v1]);
+    double nomDist = (bsa1 + bsa2) * COLLISION_ENVELOPE_RADIUS;
+    double relDist = nomDist - vlen(offset);
+    v3 force = v3_(0, 0, 0);
+    if (relDist > 0) {
+        double l = vlen(offset);
+        v3 unit = l > 0 ? v3_(offset.x / l, offset.y / l, offset.z / l) : offset;
+        v3 vel1 = vscale(m->linMom[v1], m1->massInv), vel2 = vscale(m->linMom[v2], m2->massInv);
+        double rv = (vel1.x * unit.x + vel1.y * unit.y + vel1.z * unit.z) - (vel2.x * unit.x + vel2.y * unit.y + vel2.z * unit.z);
+        force = vscale(unit, penStiff * relDist + dampC * rv);
+    }
+    m->contact[v1] = vadd(m->contact[v1], force);
+    m->contact[v2] = vsub(m->contact[v2], force);
+}
+
+/* updateAttach all-pairs branch VX3_VoxelyzeKernel.cu:448-458.  The uniform
+ * grid only prunes pairs farther apart than any watch distance; the pairs are
+ * visited in the same (first, second) order as the nested all-pairs loop. */
+static void update_collisions(vxo_model *m) {
+    int ns = m->ns;
+    if (ns < 2) return;
+    double h = 0;
+    for (int i = 0; i < ns; i++) {
+        double b = base_size_average(m, m->surface[i]);
+        if (b > h) h = b;
+    }
+    h *= 2 * COLLISION_ENVELOPE_RADIUS * 1.0001;
+    if (!(h > 0)) h = m->voxSize;
+    cellent *ent = (cellent *)xcalloc(ns, sizeof(cellent));
+    long long(*cc)[3] = (long long(*)[3])xcalloc(ns, sizeof(long long[3]));
+    for (int i = 0; i < ns; i++) {
+        v3 p = m->pos[m->surface[i]];
+        cc[i][0] = (long long)floor(p.x / h);
+        cc[i][1] = (long long)floor(p.y / h);
+        cc[i][2] = (long long)floor(p.z / h);
+        ent[i].key = cell_key(cc[i][0], cc[i][1], cc[i][2]);
+        ent[i].s = i;
+    }
+    qsort(ent, ns, sizeof(cellent), cellent_cmp);
+    int cap = 64, cnt;
+    int *cand = (int *)xcalloc(cap, sizeof(int));
+    for (int first = 1; first < ns; first++) {
+        cnt = 0;
+        for (int dx = -1; dx <= 1; dx++)
+            for (int dy = -1; dy <= 1; dy++)
+                for (int dz = -1; dz <= 1; dz++) {
+                    long long key = cell_key(cc[first][0] + dx, cc[first][1] + dy, cc[first][2] + dz);
+                    int lo = 0, hi = ns;
+                    while (lo < hi) {
+                        int mid = (lo + hi) / 2;
+                        if (ent[mid].key < key) lo = mid + 1; else hi = mid;
+                    }
+                    for (int k = lo; k < ns && ent[k].key == key; k++) {
+                        if (ent[k].s >= first) continue;
+                        if (cnt == cap) { cap *= 2; cand = (int *)realloc(cand, cap * sizeof(int)); if (!cand) abort(); }
+                        cand[cnt++] = ent[k].s;
+                    }
+                }
+        /* ascending second, as in the all-pairs order */
+        for (int a = 1; a < cnt; a++) {
+            int x = cand[a], b = a - 1;
+            while (b >= 0 && cand[b] > x) { cand[b + 1] = cand[b]; b--; }
+            cand[b + 1] = x;
+        }
+        for (int a = 0; a < cnt; a++) collide_pair(m, m->surface[first], m->surface[cand[a]]);
+    }
+    free(cand);
+    free(cc);
+    free(ent);
+}
+
+/* ----------------------------------------------------------------- voxels */
+/* VX3_Voxel::timeStep VX3_Voxel.cu:162-277 with force() :350-380, moment()
+ * :382-397, floorForce :399-426 (no external fixes, constant-zero force field) */
+static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
+    const vmat *mt = &m->vm[m->vmat_of[v]];
+    double dt = dtf;
+    m->prevDt[v] = dtf;
+    if (dt == 0.0f) return;
+    int flags = m->vflags[v];
+    q4 orient = m->orient[v];
+    v3 pos = m->pos[v], linMom = m->linMom[v], angMom = m->angMom[v];
+
+    /* force() */
+    v3 tf = v3_(0, 0, 0);
+    for (int d = 0; d < 6; d++) {
+        int l = m->vlinks[v][d];
+        if (l < 0) continue;
+        tf = vadd(tf, (d & 1) ? m->fPos[l] : m->fNeg[l]);
+    }
+    tf = qrot(orient, tf);
+    v3 vel = vscale(linMom, mt->massInv);
+    float gdt = mt->zetaG * mt->c2xSqMxExS;
+    tf = vsub(tf, vscale(vel, gdt));
+    tf.z += -mt->mass * 9.80665f * mt->gravMult;
+    tf = vsub(tf, m->contact[v]);
+    m->contact[v] = v3_(0, 0, 0);
+    /* force field: constant-zero trees add +0.0 */
+    tf.x += 0.0; tf.y += 0.0; tf.z += 0.0;
+
+    v3 curForce = tf;
+    if (flags & VF_FLOOR_ENABLED) {
+        float pen = floor_penetration(m, v);
+        if (pen >= 0) {
+            v3 hv = v3_(vel.x, vel.y, 0);
+            float normalForce = (float)(2 * mt->E * mt->nomSize) * pen;
+            curForce.z += normalForce - (mt->zetaC * mt->c2xSqMxExS) * vel.z;
+            if (flags & VF_FLOOR_STATIC_FRICTION) {
+                float sfs = (float)(curForce.x * curForce.x + curForce.y * curForce.y);
+                float ffs = (mt->muS * normalForce) * (mt->muS * normalForce);
+                if (sfs > ffs) flags &= ~VF_FLOOR_STATIC_FRICTION;
+            } else {
+                double l = sqrt(hv.x * hv.x + hv.y * hv.y + hv.z * hv.z);
+                v3 hn = l > 0 ? v3_(hv.x / l, hv.y / l, hv.z / l) : hv;
+                double kf = mt->muK * normalForce;
+                curForce = vsub(curForce, vscale(hn, kf));
+            }
+        } else {
+            flags &= ~VF_FLOOR_STATIC_FRICTION;
+        }
+    }
+    v3 fricForce = vsub(curForce, tf);
+    linMom = vadd(linMom, vscale(curForce, dt));
+    /* CVX_Voxel::timeStep takes a float dt, so its dt*_massInverse product is
+     * float (src/old/VX_Voxel.cpp:182); VX3 passes a double (VX3_Voxel.cu:196) */
+    double dtMinv = cpu_semantics ? (double)(dtf * mt->massInv) : dt * mt->massInv;
+    double dtIinv = cpu_semantics ? (double)(dtf * mt->inertiaInv) : dt * mt->inertiaInv;
+    v3 translate = vscale(linMom, dtMinv);
+    if ((flags & VF_FLOOR_ENABLED) && floor_penetration(m, v) >= 0) {
+        double work = fricForce.x * translate.x + fricForce.y * translate.y;
+        double hKe = 0.5 * mt->massInv * (linMom.x * linMom.x + linMom.y * linMom.y);
+        if (hKe + work <= 0) flags |= VF_FLOOR_STATIC_FRICTION;
+        if (flags & VF_FLOOR_STATIC_FRICTION) {
+            linMom.x = linMom.y = 0;
+            translate.x = translate.y = 0;
+        }
+    } else {
+        flags &= ~VF_FLOOR_STATIC_FRICTION;
+    }
+    pos = vadd(pos, translate);
+    m->pos[v] = pos;
+
+    /* moment() */
+    v3 tm = v3_(0, 0, 0);
+    for (int d = 0; d < 6; d++) {
+        int l = m->vlinks[v][d];
+        if (l < 0) continue;
+        tm = vadd(tm, (d & 1) ? m->mPos[l] : m->mNeg[l]);
+    }
+    tm = qrot(orient, tm);
+    float gdr = mt->zetaG * mt->c2xSqIxExSxSxS;
+    tm = vsub(tm, vscale(vscale(angMom, mt->inertiaInv), gdr));
+    angMom = vadd(angMom, vscale(tm, dt));
+    orient = qmul(q_from_rotvec(vscale(angMom, dtIinv)), orient);
+
+    /* VX3-only: no rotation while held by static floor friction (VX3_Voxel.cu:259-264) */
+    if (!cpu_semantics && (flags & VF_FLOOR_ENABLED) && floor_penetration(m, v) >= 0) {
+        if (flags & VF_FLOOR_STATIC_FRICTION) angMom = v3_(0, 0, 0);
+    }
+    m->orient[v] = orient;
+    m->linMom[v] = linMom;
+    m->angMom[v] = angMom;
+    m->vflags[v] = flags;
+}
+
+static void update_voxels(vxo_model *m, float dt, int cpu_semantics) {
+    int nv = m->nv;
+#pragma omp parallel for schedule(static)
+    for (int v = 0; v < nv; v++) {
+        if (m->vm[m->vmat_of[v]].fixed) continue;  /* gpu_update_voxels :582-589 */
+        voxel_update(m, v, dt, cpu_semantics);
+    }
+}
+
+/* ------------------------------------------------------------------- step */
+/* VX3_VoxelyzeKernel::doTimeStep VX3_VoxelyzeKernel.cu:237-359 */
+int vxo_step(vxo_model *m, float dt, int nsteps, int flags) {
+    for (int s = 0; s < nsteps; s++) {
+        if (!(flags & VXO_NO_TEMPERATURE)) update_temperature(m);
+        float d = dt;
+        if (d < 0) d = vxo_step_dt(m);
+        if (d == 0) continue;
+        update_links(m);
+        int diverged = 0;
+        for (int l = 0; l < m->nl; l++)
+            if (m->strain[l] > 100) { diverged = 1; break; }
+        if (diverged) return s;
+        update_collisions(m);
+        update_voxels(m, d, flags & VXO_CPU_SEMANTICS);
+        m->time += d;
+    }
+    return nsteps;
+}
+
+void vxo_update_links(vxo_model *m, double *out) {
+    update_links(m);
+    if (!out) return;
+    for (int l = 0; l < m->nl; l++) {
+        v3 f[4] = {m->fNeg[l], m->fPos[l], m->mNeg[l], m->mPos[l]};
+        for (int k = 0; k < 4; k++) {
+            out[12 * l + 3 * k + 0] = f[k].x;
+            out[12 * l + 3 * k + 1] = f[k].y;
+            out[12 * l + 3 * k + 2] = f[k].z;
+        }
+    }
+}
+
+/* ----------------------------------------------------------------- access */
+void vxo_get_voxels(const vxo_model *m, double *pos3, double *quat4, double *linmom3, double *angmom3) {
+    for (int v = 0; v < m->nv; v++) {
+        if (pos3) { pos3[3 * v] = m->pos[v].x; pos3[3 * v + 1] = m->pos[v].y; pos3[3 * v + 2] = m->pos[v].z; }
+        if (quat4) {
+            quat4[4 * v] = m->orient[v].w; quat4[4 * v + 1] = m->orient[v].x;
+            quat4[4 * v + 2] = m->orient[v].y; quat4[4 * v + 3] = m->orient[v].z;
+        }
+        if (linmom3) { linmom3[3 * v] = m->linMom[v].x; linmom3[3 * v + 1] = m->linMom[v].y; linmom3[3 * v + 2] = m->linMom[v].z; }
+        if (angmom3) { angmom3[3 * v] = m->angMom[v].x; angmom3[3 * v + 1] = m->angMom[v].y; angmom3[3 * v + 2] = m->angMom[v].z; }
+    }
+}
+
+void vxo_set_voxels(vxo_model *m, const double *pos3, const double *quat4, const double *linmom3, const double *angmom3) {
+    for (int v = 0; v < m->nv; v++) {
+        if (pos3) m->pos[v] = v3_(pos3[3 * v], pos3[3 * v + 1], pos3[3 * v + 2]);
+        if (quat4) { q4 q = {quat4[4 * v], quat4[4 * v + 1], quat4[4 * v + 2], quat4[4 * v + 3]}; m->orient[v] = q; }
+        if (linmom3) m->linMom[v] = v3_(linmom3[3 * v], linmom3[3 * v + 1], linmom3[3 * v + 2]);
+        if (angmom3) m->angMom[v] = v3_(angmom3[3 * v], angmom3[3 * v + 1], angmom3[3 * v + 2]);
+    }
+}
+
+void vxo_get_flags(const vxo_model *m, int *voxel_flags, int *link_flags) {
+    if (voxel_flags) memcpy(voxel_flags, m->vflags, sizeof(int) * m->nv);
+    if (link_flags) memcpy(link_flags, m->lflags, sizeof(int) * m->nl);
+}
+
+void vxo_get_link_state(const vxo_model *m, double *state9, float *strain) {
+    for (int l = 0; l < m->nl; l++) {
+        if (state9) {
+            double *s = state9 + 9 * l;
+            s[0] = m->pos2[l].x; s[1] = m->pos2[l].y; s[2] = m->pos2[l].z;
+            s[3] = m->a1v[l].x; s[4] = m->a1v[l].y; s[5] = m->a1v[l].z;
+            s[6] = m->a2v[l].x; s[7] = m->a2v[l].y; s[8] = m->a2v[l].z;
+        }
+        if (strain) strain[l] = m->strain[l];
+    }
+}
+
+void vxo_get_temperature(const vxo_model *m, float *temp, double *rest_length) {
+    if (temp) memcpy(temp, m->temp, sizeof(float) * m->nv);
+    if (rest_length) memcpy(rest_length, m->restLen, sizeof(double) * m->nl);
+}
+
+void vxo_set_temperature(vxo_model *m, const float *temp) {
+    memcpy(m->temp, temp, sizeof(float) * m->nv);
+    for (int l = 0; l < m->nl; l++) update_rest_length(m, l);
+}
+
+void vxo_get_topology(const vxo_model *m, int *lattice3, int *link_neg, int *link_pos, int *link_axis, int *surface) {
+    for (int v = 0; v < m->nv; v++)
+        if (lattice3) { lattice3[3 * v] = m->ix[v]; lattice3[3 * v + 1] = m->iy[v]; lattice3[3 * v + 2] = m->iz[v]; }
+    if (link_neg) memcpy(link_neg, m->lneg, sizeof(int) * m->nl);
+    if (link_pos) memcpy(link_pos, m->lpos, sizeof(int) * m->nl);
+    if (link_axis) memcpy(link_axis, m->laxis, sizeof(int) * m->nl);
+    if (surface) memcpy(surface, m->surface, sizeof(int) * m->ns);
+}
+
+/* updateCurrentCenterOfMass VX3_VoxelyzeKernel.cu:477-493 */
+void vxo_center_of_mass(const vxo_model *m, double *com3) {
+    double total = 0;
+    v3 sum = v3_(0, 0, 0);
+    for (int v = 0; v < m->nv; v++) {
+        const vmat *mt = &m->vm[m->vmat_of[v]];
+        if (!mt->isMeasured) continue;
+        double w = mt->mass;
+        sum = vadd(sum, vscale(m->pos[v], w));
+        total += w;
+    }
+    if (total == 0) { com3[0] = com3[1] = com3[2] = 0; return; }
+    com3[0] = sum.x / total; com3[1] = sum.y / total; com3[2] = sum.z / total;
+}
+
+void vxo_get_contact(const vxo_model *m, double *contact3) {
+    for (int v = 0; v < m->nv; v++) {
+        contact3[3 * v] = m->contact[v].x; contact3[3 * v + 1] = m->contact[v].y; contact3[3 * v + 2] = m->contact[v].z;
+    }
+}
+
+/* voxel: E nu rho cte muS muK mass massInv sqrtMass inertiaInv 2sqrt(mEL) 2sqrt(IEL^3) */
+void vxo_get_voxel_material(const vxo_model *m, int voxel, float *o) {
+    const vmat *mt = &m->vm[m->vmat_of[voxel]];
+    o[0] = mt->E; o[1] = mt->nu; o[2] = mt->rho; o[3] = mt->alphaCTE; o[4] = mt->muS; o[5] = mt->muK;
+    o[6] = mt->mass; o[7] = mt->massInv; o[8] = mt->sqrtMass; o[9] = mt->inertiaInv;
+    o[10] = mt->c2xSqMxExS; o[11] = mt->c2xSqIxExSxSxS;
+}
+/* link: E a1 a2 b1 b2 b3 sqA1 sqA2xIp sqB1 sqB2xFMp sqB3xIp */
+void vxo_get_link_material(const vxo_model *m, int link, float *o) {
+    const lmat *l = &m->lm[link];
+    o[0] = l->E; o[1] = l->a1; o[2] = l->a2; o[3] = l->b1; o[4] = l->b2; o[5] = l->b3;
+    o[6] = l->sqA1; o[7] = l->sqA2xIp; o[8] = l->sqB1; o[9] = l->sqB2xFMp; o[10] = l->sqB3xIp;
+}

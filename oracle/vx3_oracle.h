@@ -1,0 +1,109 @@
+/*
+ * vx3_oracle.h -- FP64 CPU restatement of the VX3 timestep, used ONLY as the
+ * checker for the HIP implementation (tests, smoke check, CPU baseline leg of
+ * the benchmark).  Nothing in the product links against this file.
+ *
+ * The oracle rebuilds a voxel model from a lattice description exactly the way
+ * the reference does (voxel order, link order, combined link materials,
+ * material constants with the reference's float/double rounding) and advances
+ * it with the reference's update equations:
+ *
+ *   temperature  src/VX3/VX3_VoxelyzeKernel.cu:625-650, VX3_Voxel.cu:107-113,
+ *                VX3_Link.cu:80-83, VX3_Voxel.h:96-100
+ *   dt           src/VX3/VX3_VoxelyzeKernel.cu:184-217, 237-254
+ *   links        src/VX3/VX3_Link.cu:90-260, VX3_Link.h:97-191,
+ *                VX3_Material.cu:90-124, src/Utils/VX3_Quat3D.h
+ *   collisions   src/VX3/VX3_VoxelyzeKernel.cu:682-843, VX3_Collision.cu:3-31
+ *   voxels       src/VX3/VX3_Voxel.cu:162-277, 350-426, VX3_Voxel.h:185-208
+ *   CoM          src/VX3/VX3_VoxelyzeKernel.cu:477-493
+ *
+ * Deliberate (documented) deviations from the racy reference:
+ *   - both endpoint temperatures used for a link rest length are the new ones;
+ *   - contact forces are summed serially over surface pairs (first > second);
+ *   - divergence is "any link axial strain > 100" instead of one random link.
+ */
+#ifndef VX3_ORACLE_H
+#define VX3_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* One palette material (VXA Palette/Material/Mechanical, MatModel 0 or 1). */
+typedef struct {
+    float E;            /* Elastic_Mod, Pa */
+    float rho;          /* Density, kg/m^3 */
+    float nu;           /* Poissons_Ratio (only used with volume effects) */
+    float cte;          /* CTE, 1/degC */
+    float mu_s, mu_k;   /* uStatic, uDynamic */
+    float fail_stress;  /* -1 = no failure (MatModel 0) */
+    int fixed;          /* Fixed */
+    int is_measured;    /* isMeasured */
+    double thermal_on_after; /* TurnOnThermalExpansionAfterThisManySeconds */
+} vxo_material;
+
+/* Lattice + simulator + environment description. */
+typedef struct {
+    int nx, ny, nz;
+    const signed char *cells;   /* nx*ny*nz palette indices, x fastest; <=0 empty */
+    const double *phase;        /* per-voxel phase offset in cycles (voxel order) or NULL */
+    const float *voxel_E;       /* per-voxel Young's modulus override (voxel order) or NULL */
+    int nmat;                   /* palette entries 1..nmat stored at mats[0..nmat-1] */
+    const vxo_material *mats;
+    double lattice_dim;         /* Lattice_Dim (m) */
+    double dt_frac;             /* Integration/DtFrac */
+    float bond_damp_z, col_damp_z, slow_damp_z; /* Damping */
+    int volume_effects;         /* Features/VolumeEffectsEnabled */
+    int enable_collision;       /* AttachDetach/EnableCollision */
+    int enable_expansion;       /* Simulator/EnableExpansion */
+    float grav_acc;             /* Environment/Gravity/GravAcc */
+    int floor_enabled;          /* Environment/Gravity/FloorEnabled */
+    int temp_enabled, vary_temp;                   /* Thermal */
+    double temp_base, temp_amplitude, temp_period; /* Thermal */
+} vxo_spec;
+
+typedef struct vxo_model vxo_model;
+
+/* flags for vxo_step */
+#define VXO_CPU_SEMANTICS 1  /* CVoxelyze (src/old) integrator: no angMom reset under static friction */
+#define VXO_NO_TEMPERATURE 2 /* skip the temperature update (caller drives temperatures) */
+
+vxo_model *vxo_build(const vxo_spec *spec);
+void vxo_free(vxo_model *m);
+int vxo_num_voxels(const vxo_model *m);
+int vxo_num_links(const vxo_model *m);
+int vxo_num_surface(const vxo_model *m);
+double vxo_time(const vxo_model *m);
+double vxo_recommended_dt(const vxo_model *m);
+float vxo_step_dt(vxo_model *m);            /* the float dt doTimeStep(-1) uses */
+
+/* Advance nsteps steps with dt (<0: DtFrac*recommended).  Returns the number of
+ * steps completed; fewer than nsteps means the model diverged. */
+int vxo_step(vxo_model *m, float dt, int nsteps, int flags);
+
+/* Link forces for the current state without advancing: link state (pos2,
+ * angle1v, angle2v, smallAngle, local-velocity flag) IS updated, exactly as
+ * VX3_Link::updateForces does.  out: 12 doubles per link (forceNeg, forcePos,
+ * momentNeg, momentPos), each in its voxel's local frame. */
+void vxo_update_links(vxo_model *m, double *out);
+
+/* state access (voxel order) */
+void vxo_get_voxels(const vxo_model *m, double *pos3, double *quat4, double *linmom3, double *angmom3);
+void vxo_set_voxels(vxo_model *m, const double *pos3, const double *quat4, const double *linmom3, const double *angmom3);
+void vxo_get_flags(const vxo_model *m, int *voxel_flags, int *link_flags);
+void vxo_get_link_state(const vxo_model *m, double *state9, float *strain);
+void vxo_get_temperature(const vxo_model *m, float *temp, double *rest_length);
+void vxo_set_temperature(vxo_model *m, const float *temp);
+void vxo_get_topology(const vxo_model *m, int *lattice3, int *link_neg, int *link_pos, int *link_axis, int *surface);
+void vxo_center_of_mass(const vxo_model *m, double *com3);
+void vxo_get_contact(const vxo_model *m, double *contact3);
+
+/* material constants as the reference stores them (for builder parity tests)
+ * voxel: 12 floats, link: 11 floats (see vx3_oracle.c for the order) */
+void vxo_get_voxel_material(const vxo_model *m, int voxel, float *out12);
+void vxo_get_link_material(const vxo_model *m, int link, float *out11);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
