@@ -1,0 +1,215 @@
+#!/usr/bin/env python3
+"""Benchmark of the VX3 timestep on one MI355X.
+
+Workload (the batched configuration the headline metric is quoted on): 1024
+independent robots, lattice dims U{4..12} per axis, Bernoulli(0.6) occupancy,
+largest face-connected component, two materials, random actuation phases,
+FP32 state, collisions off, seed 0.  A *step* is one timestep of every robot
+of the batch (VX3_VoxelyzeKernel::doTimeStep for all 1024 simulations).
+
+Printed: one JSON line with voxel-steps/s of the batched persistent kernel
+(value), the same workload on the plain path (one robot per launch sequence,
+as the reference launches per simulation) and the ratio, the relative error
+of the FP32 run against the FP64 oracle on a sample of robots, the HBM
+roofline of the dominant kernel, and the reference CPU Voxelyze timed on a
+bounded sample of the same robots on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "voxcraft-sim_amd"))
+
+import numpy as np  # noqa: E402
+
+import vx3mi  # noqa: E402
+from vx3mi import configs  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+PERSISTENT_CHUNK = 1000  # steps per persistent launch
+
+
+def algorithmic_bytes_per_voxel_step(nv: int, nl: int, fp64: bool = False) -> float:
+    """Compulsory bytes of a step that keeps no state on chip across steps:
+    per voxel 13 state words r/w + flags r/w + phase and material + 6 link
+    indices; per link 11 state words r/w + endpoints, axis and material."""
+    if fp64:
+        return 252.0 + 180.0 * nl / nv
+    return 144.0 + 100.0 * nl / nv
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(problems, nsteps: int):
+    """Reference CPU Voxelyze (oracle/_ref, built from the reference sources) on
+    a sample of robots; falls back to the FP64 oracle port when absent."""
+    ref_exe = os.path.join(ROOT, "oracle", "_ref", "vx3_ref_driver")
+    if os.path.exists(ref_exe):
+        secs = vs = 0.0
+        for p in problems:
+            nx, ny, nz = p["dims"]
+            cells = np.asarray(p["cells"]).reshape(-1)
+            nv = int((cells > 0).sum())
+            m0 = p["materials"]
+            lines = [f"{nx} {ny} {nz} {p['lattice_dim']!r} {p['bond_damp_z']!r} {p['col_damp_z']!r} "
+                     f"{p['slow_damp_z']!r} {p['grav_acc']!r} {int(p['floor_enabled'])} {p['temp_amplitude']!r} "
+                     f"{p['temp_period']!r} {int(p['vary_temp'])} {int(p['enable_expansion'])} 0 {len(m0)}"]
+            for m in m0:
+                lines.append(f"{m['E']!r} {m['rho']!r} {m['cte']!r} {m['mu_s']!r} {m['mu_k']!r}")
+            lines.append(" ".join(str(int(c)) for c in cells))
+            lines.append(" ".join(repr(float(x)) for x in p["phase"]))
+            dt = vx3mi.Model.from_problem(p).recommended_dt() * p["dt_frac"]
+            lines.append(f"{float(np.float32(dt)):.9g} {nsteps} 0")
+            r = subprocess.run([ref_exe], input="\n".join(lines).encode(), capture_output=True, timeout=600)
+            if r.returncode != 0:
+                raise RuntimeError(r.stderr.decode())
+            tail = r.stderr.decode().split()
+            secs += float(tail[tail.index("seconds") + 1])
+            vs += nv * nsteps
+        kind = "reference"
+    else:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import oracle as O  # the checker, timed as a port of the reference CPU path
+
+        secs = vs = 0.0
+        for p in problems:
+            m = O.OracleModel(p)
+            t0 = time.perf_counter()
+            m.step(nsteps)
+            secs += time.perf_counter() - t0
+            vs += m.nv * nsteps
+        kind = "port"
+    return vs / secs, kind
+
+
+def fp32_error_vs_oracle(problems, nsteps: int) -> float:
+    """max over robots of e_pos = max_i |p_i - p_i*| / max(|p_i*|, L) after nsteps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in problems], vx3mi.FP32)
+    b.step(nsteps, mode=vx3mi.MODE_PERSISTENT)
+    worst = 0.0
+    for r, p in enumerate(problems):
+        ref = O.OracleModel(p)
+        ref.step(nsteps)
+        s, o = b.state(r)["pos"], ref.state()["pos"]
+        e = np.max(np.linalg.norm(s - o, axis=1) / np.maximum(np.linalg.norm(o, axis=1), p["lattice_dim"]))
+        worst = max(worst, float(e))
+    b.close()
+    return worst
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--robots", type=int, default=1024)
+    ap.add_argument("--plain-steps", type=int, default=20)
+    ap.add_argument("--cpu-robots", type=int, default=48)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--check-robots", type=int, default=16)
+    ap.add_argument("--check-steps", type=int, default=1000)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); robots shard across GPUs "
+              "as independent batches, which this build does not launch.", file=sys.stderr)
+        return 2
+
+    problems = configs.config2(seed=0, n=args.robots)
+    t0 = time.perf_counter()
+    models = [vx3mi.Model.from_problem(p) for p in problems]
+    batch = vx3mi.Batch(models, vx3mi.FP32)
+    nv, nl = batch.nv, batch.nl
+    log(f"bench: {args.robots} robots, {nv} voxels, {nl} links, setup {time.perf_counter() - t0:.2f}s")
+
+    # warmup (also builds nothing lazily: persistent launches need no graph)
+    batch.step(args.warmup, mode=vx3mi.MODE_PERSISTENT)
+    batch.sync()
+    # timed region: inputs resident in HBM, device events around the launches
+    t0 = time.perf_counter()
+    batch.step(args.steps, mode=vx3mi.MODE_PERSISTENT)
+    batch.sync()
+    wall = time.perf_counter() - t0
+    dev_ms = batch.last_step_ms()
+    launches = batch.last_step_launches()
+    div = int(batch.diverged().sum())
+    value = nv * args.steps / (dev_ms * 1e-3)
+    log(f"bench: batched {args.steps} steps: device {dev_ms:.1f} ms, wall {wall * 1e3:.1f} ms, "
+        f"{launches} launches, {value:.3e} voxel-steps/s, diverged robots {div}")
+
+    # plain path: one robot per launch sequence
+    plain = vx3mi.Batch(models, vx3mi.FP32)
+    plain.step(2, mode=vx3mi.MODE_PLAIN)
+    plain.sync()
+    plain.step(args.plain_steps, mode=vx3mi.MODE_PLAIN)
+    plain.sync()
+    plain_ms = plain.last_step_ms()
+    plain_value = nv * args.plain_steps / (plain_ms * 1e-3)
+    plain.close()
+    log(f"bench: plain {args.plain_steps} steps: {plain_ms:.1f} ms, {plain_value:.3e} voxel-steps/s")
+
+    # roofline of the dominant (persistent) kernel
+    bpv = algorithmic_bytes_per_voxel_step(nv, nl)
+    steps_per_launch = args.steps / max(1, launches)
+    bytes_per_launch = bpv * nv * steps_per_launch
+    avg_launch_s = dev_ms * 1e-3 / max(1, launches)
+    achieved = bytes_per_launch / avg_launch_s / 1e9
+
+    err = None
+    if args.check_robots > 0:
+        err = fp32_error_vs_oracle(problems[: args.check_robots], args.check_steps)
+        log(f"bench: FP32 vs FP64 oracle, {args.check_robots} robots x {args.check_steps} steps: e_pos {err:.3e}")
+
+    cpu = None
+    if not args.no_cpu:
+        sample = problems[: args.cpu_robots]
+        cv, kind = cpu_baseline(sample, args.cpu_steps)
+        svox = sum(int((np.asarray(p["cells"]) > 0).sum()) for p in sample)
+        cpu = {"value": cv, "unit": "voxel-steps/s", "cores": 1, "kind": kind,
+               "sample": f"first {len(sample)} robots of the workload ({svox} voxels) x {args.cpu_steps} steps, "
+                         "single thread"}
+        log(f"bench: CPU baseline ({kind}) {cv:.3e} voxel-steps/s")
+
+    out = {
+        "metric": "voxel-steps/s batched vs plain @<=1e-5 rel err to FP64; adjoint/forward cost",
+        "value": value,
+        "unit": "voxel-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": dev_ms / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded random robots)",
+        "config": {"workload": f"batched {args.robots} robots, dims U{{4..12}}^3, p=0.6, largest component, "
+                               "2 materials, random phases, FP32, collisions off",
+                   "robots": args.robots, "voxels": nv, "links": nl, "parallelism": "single GPU"},
+        "plain_value": plain_value,
+        "batched_over_plain": value / plain_value,
+        "fp32_rel_err_vs_fp64": err,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_voxel_step": bpv, "kernel": "k_persistent<float>",
+                     "steps_per_launch": steps_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    batch.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,148 @@
+/*
+ * vx3mi.h -- C ABI of the MI355X-native VX3 timestep.
+ *
+ * This is the drop-in boundary for the reference's VX3 kernel interface
+ * (class VX3_VoxelyzeKernel, src/VX3/VX3_VoxelyzeKernel.cuh:21-49) and its
+ * host-side construction path (VX3_SimulationManager::readVXD,
+ * src/VX3/VX3_SimulationManager.cu:277-381).  Plain pointers and sizes only;
+ * no exception crosses this boundary; every call returns an int status and
+ * vx3mi_last_error() describes the last failure on the calling thread.
+ *
+ * Ownership: a vx3mi_model is a host object; a vx3mi_batch owns every device
+ * buffer it uses (allocated once in vx3mi_batch_create, freed by
+ * vx3mi_batch_destroy) and one HIP stream.  A handle is not re-entrant.
+ */
+#ifndef VX3MI_H
+#define VX3MI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define VX3MI_OK 0
+#define VX3MI_E_ARG (-1)      /* invalid argument */
+#define VX3MI_E_PARSE (-2)    /* VXA/VXD could not be read */
+#define VX3MI_E_HIP (-3)      /* HIP runtime failure */
+#define VX3MI_E_NOMEM (-4)    /* allocation failure */
+#define VX3MI_E_LIMIT (-5)    /* model exceeds a documented limit */
+#define VX3MI_DIVERGED 1      /* at least one robot diverged (see vx3mi_batch_diverged) */
+
+/* precision of the device state */
+#define VX3MI_FP32 32
+#define VX3MI_FP64 64
+
+/* step modes (vx3mi_step) */
+#define VX3MI_MODE_AUTO 0        /* persistent kernel for batches of small robots, else flat */
+#define VX3MI_MODE_PLAIN 1       /* one robot per launch sequence (reference-style) */
+#define VX3MI_MODE_FLAT 2        /* one link launch + one voxel launch per step for the whole batch */
+#define VX3MI_MODE_PERSISTENT 3  /* one workgroup per robot, many steps per launch */
+
+/* One palette material: VXA Palette/Material (src/VXA/VX_Object.cpp:1373-1475). */
+typedef struct {
+    float elastic_mod;   /* Elastic_Mod (Pa) */
+    float density;       /* Density (kg/m^3) */
+    float poissons_ratio;/* Poissons_Ratio (inert unless volume_effects) */
+    float cte;           /* CTE (1/degC) */
+    float u_static;      /* uStatic */
+    float u_dynamic;     /* uDynamic */
+    float fail_stress;   /* Fail_Stress (used when mat_model == 1) */
+    int mat_model;       /* MatModel: 0 linear, 1 linear with failure */
+    int fixed;           /* Fixed */
+    int is_measured;     /* isMeasured */
+    int is_target;       /* isTarget */
+    double thermal_on_after; /* TurnOnThermalExpansionAfterThisManySeconds */
+    float red, green, blue, alpha; /* Display */
+} vx3mi_material;
+
+/* Lattice description: the information content of a merged VXA (base.vxa +
+ * robot.vxd), plus the per-voxel arrays a binding can supply directly. */
+typedef struct {
+    int nx, ny, nz;                /* Structure X_Voxels, Y_Voxels, Z_Voxels */
+    const signed char *cells;      /* nx*ny*nz palette indices, x fastest; <=0 empty */
+    const double *phase;           /* per-voxel PhaseOffset (cycles), voxel order, or NULL */
+    const float *voxel_E;          /* per-voxel Young's modulus override, or NULL */
+    int nmat;                      /* palette size (materials 1..nmat) */
+    const vx3mi_material *mats;
+    double lattice_dim;            /* Lattice/Lattice_Dim */
+    double dt_frac;                /* Simulator/Integration/DtFrac */
+    float bond_damp_z, col_damp_z, slow_damp_z; /* Simulator/Damping */
+    int volume_effects;            /* Simulator/Features/VolumeEffectsEnabled */
+    int enable_collision;          /* Simulator/AttachDetach/EnableCollision */
+    int enable_expansion;          /* Simulator/EnableExpansion */
+    float grav_acc;                /* Environment/Gravity/GravAcc */
+    int floor_enabled;             /* Environment/Gravity/FloorEnabled */
+    int temp_enabled, vary_temp;   /* Environment/Thermal */
+    double temp_base, temp_amplitude, temp_period;
+} vx3mi_lattice;
+
+typedef struct vx3mi_model vx3mi_model;
+typedef struct vx3mi_batch vx3mi_batch;
+
+const char *vx3mi_last_error(void);
+const char *vx3mi_version(void);
+
+/* ---- models (host) -------------------------------------------------------
+ * Replaces VX3_SimulationManager::readVXD (VX3_SimulationManager.cu:277-381)
+ * + CVX_Sim::ReadVXA/Import (src/VXA/VX_Sim.cpp:59-183) + the
+ * VX3_VoxelyzeKernel(CVX_Sim*) constructor (VX3_VoxelyzeKernel.cu:27-105). */
+int vx3mi_model_from_vxa(const char *base_vxa_path, const char *vxd_path, vx3mi_model **out);
+int vx3mi_model_from_xml(const char *base_vxa_text, const char *vxd_text, vx3mi_model **out);
+int vx3mi_model_from_lattice(const vx3mi_lattice *spec, vx3mi_model **out);
+void vx3mi_model_free(vx3mi_model *m);
+/* counts: voxels, links, surface voxels */
+int vx3mi_model_counts(const vx3mi_model *m, int *nvox, int *nlink, int *nsurf);
+/* recommendedTimeStep (VX3_VoxelyzeKernel.cu:184-217), host evaluation */
+int vx3mi_model_recommended_dt(const vx3mi_model *m, double *out);
+/* topology in voxel / link order: lattice (3 per voxel), link ends and axis */
+int vx3mi_model_topology(const vx3mi_model *m, int *lattice3, int *link_neg, int *link_pos, int *link_axis);
+
+/* ---- batches (device) -----------------------------------------------------
+ * A batch packs n models (CSR: voxel and link offsets per robot) into one
+ * set of device arrays.  precision: VX3MI_FP32 or VX3MI_FP64. */
+int vx3mi_batch_create(vx3mi_model *const *models, int n, int precision, vx3mi_batch **out);
+void vx3mi_batch_destroy(vx3mi_batch *b);
+int vx3mi_batch_counts(const vx3mi_batch *b, int *nrobot, int *nvox, int *nlink);
+
+/* recommendedTimeStep of every robot, computed on the device with a
+ * wavefront max-reduction (replaces the serial loop at
+ * VX3_VoxelyzeKernel.cu:195-203).  out: n doubles. */
+int vx3mi_batch_recommended_dt(vx3mi_batch *b, double *out);
+
+/* doTimeStep (VX3_VoxelyzeKernel.cu:237-359) nsteps times for every robot.
+ * dt < 0: each robot uses (float)(DtFrac * recommendedTimeStep), as
+ * doTimeStep(-1).  dt >= 0: every robot uses dt.  Returns VX3MI_OK,
+ * VX3MI_DIVERGED (some robot stopped; it is not advanced further) or an
+ * error.  Asynchronous on the batch stream except for the return code of
+ * launch errors; vx3mi_batch_sync waits. */
+int vx3mi_step(vx3mi_batch *b, int nsteps, float dt, int mode);
+int vx3mi_batch_sync(vx3mi_batch *b);
+
+/* State access in voxel order (absolute positions, as VX3_Voxel::pos). robot
+ * < 0 selects all robots.  Any output pointer may be NULL. */
+int vx3mi_get_state(vx3mi_batch *b, int robot, double *pos3, double *quat4, double *linmom3, double *angmom3);
+int vx3mi_set_state(vx3mi_batch *b, int robot, const double *pos3, const double *quat4, const double *linmom3,
+                    const double *angmom3);
+/* per-voxel flag words (floor / static friction bits as the reference) */
+int vx3mi_get_voxel_flags(vx3mi_batch *b, int robot, int *flags);
+/* per-robot simulated time (currentTime), diverged flag, step count */
+int vx3mi_batch_time(vx3mi_batch *b, double *out);
+int vx3mi_batch_diverged(vx3mi_batch *b, int *out);
+/* updateCurrentCenterOfMass (VX3_VoxelyzeKernel.cu:477-493) per robot: out 3n */
+int vx3mi_center_of_mass(vx3mi_batch *b, double *out);
+
+/* Link forces of the current state (VX3_Link::updateForces) without moving
+ * voxels: updates the link state as the reference does and writes 12 values
+ * per link (forceNeg, forcePos, momentNeg, momentPos in voxel frames). */
+int vx3mi_eval_link_forces(vx3mi_batch *b, double *out12);
+
+/* Device timing of the last vx3mi_step call, from HIP events recorded on the
+ * batch stream around its launches (ms), and the number of kernel launches
+ * that call made. */
+int vx3mi_last_step_ms(vx3mi_batch *b, float *ms);
+int vx3mi_last_step_launches(vx3mi_batch *b, long long *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

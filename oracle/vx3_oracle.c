@@ -871,6 +871,7 @@ int vxo_step(vxo_model *m, float dt, int nsteps, int flags) {
 }
 
 void vxo_update_links(vxo_model *m, double *out) {
+    update_temperature(m);  /* the link phase of the next doTimeStep (:240, :259-270) */
     update_links(m);
     if (!out) return;
     for (int l = 0; l < m->nl; l++) {

@@ -81,7 +81,8 @@ float vxo_step_dt(vxo_model *m);            /* the float dt doTimeStep(-1) uses 
  * steps completed; fewer than nsteps means the model diverged. */
 int vxo_step(vxo_model *m, float dt, int nsteps, int flags);
 
-/* Link forces for the current state without advancing: link state (pos2,
+/* Link forces for the current state without advancing: the temperature
+ * update and the link phase of the next doTimeStep run; link state (pos2,
  * angle1v, angle2v, smallAngle, local-velocity flag) IS updated, exactly as
  * VX3_Link::updateForces does.  out: 12 doubles per link (forceNeg, forcePos,
  * momentNeg, momentPos), each in its voxel's local frame. */

@@ -1,0 +1,154 @@
+"""GPU parity: the HIP timestep (through the C ABI) against the FP64 oracle.
+
+Tolerances (written per test): FP64 device state within 1e-12 relative of the
+oracle, FP32 within 1e-5, over the first steps of each configuration; the
+relative error norms are
+  e_pos = max_i |p_i - p_i*| / max(|p_i*|, L_vox)
+  e_q   = max_i min(|q_i - q_i*|, |q_i + q_i*|).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+import vx3mi
+from vx3mi import configs
+
+pytestmark = pytest.mark.gpu
+
+TOL = {vx3mi.FP64: 1e-12, vx3mi.FP32: 1e-5}
+
+
+def pos_err(p, ref, L=0.01):
+    d = np.linalg.norm(p - ref, axis=1)
+    return float(np.max(d / np.maximum(np.linalg.norm(ref, axis=1), L))) if len(p) else 0.0
+
+
+def quat_err(q, ref):
+    return float(np.max(np.minimum(np.linalg.norm(q - ref, axis=1), np.linalg.norm(q + ref, axis=1)))) if len(q) else 0.0
+
+
+def small_problem(seed=3, dims=(3, 3, 3), p=0.8, **kw):
+    rng = np.random.default_rng(seed)
+    prob = configs.random_robot(rng, dims, p, enable_collision=0)
+    prob.update(kw)
+    return prob
+
+
+def random_pose(nv, rng, disp=2e-4, ang=0.02):
+    pos_noise = rng.normal(scale=disp, size=(nv, 3))
+    axis = rng.normal(size=(nv, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    th = rng.uniform(0, ang, size=nv)
+    q = np.concatenate([np.cos(th / 2)[:, None], axis * np.sin(th / 2)[:, None]], axis=1)
+    return pos_noise, q
+
+
+@pytest.mark.parametrize("precision", [vx3mi.FP64, vx3mi.FP32])
+@pytest.mark.parametrize("ang", [0.01, 0.3])
+def test_link_forces_random_poses(precision, ang):
+    """Per-link forces and moments for random relative poses, small- and large-angle branches."""
+    prob = small_problem()
+    ref = O.OracleModel(prob)
+    batch = vx3mi.Batch([vx3mi.Model.from_problem(prob)], precision)
+    rng = np.random.default_rng(11)
+    # one step first so the voxels carry a previous dt (link damping divides by it)
+    ref.step(1)
+    batch.step(1)
+    s0 = ref.state()
+    noise, q = random_pose(ref.nv, rng, ang=ang)
+    pos = s0["pos"] + noise
+    for k in range(3):  # three evaluations: exercises the velocity-damping path and hysteresis
+        ref.set_state(pos=pos, quat=q)
+        batch.set_state(pos=pos, quat=q)
+        fr = ref.update_links()
+        fg = batch.eval_link_forces()
+        scale = np.max(np.abs(fr))
+        err = np.max(np.abs(fg - fr)) / scale
+        tol = 1e-12 if precision == vx3mi.FP64 else 2e-5
+        assert err <= tol, (k, err)
+        noise, q = random_pose(ref.nv, rng, ang=ang)
+        pos = s0["pos"] + noise
+
+
+@pytest.mark.parametrize("precision", [vx3mi.FP64, vx3mi.FP32])
+def test_config0_trajectory(precision):
+    """Config 0 (4x4x4 cube, thermal actuation, floor, friction): first 1000 steps."""
+    prob = configs.config0()
+    ref = O.OracleModel(prob)
+    batch = vx3mi.Batch([vx3mi.Model.from_problem(prob)], precision)
+    worst_p = worst_q = 0.0
+    for k in range(10):
+        batch.step(100, mode=vx3mi.MODE_FLAT)
+        ref.step(100)
+        s, r = batch.state(), ref.state()
+        worst_p = max(worst_p, pos_err(s["pos"], r["pos"]))
+        worst_q = max(worst_q, quat_err(s["quat"], r["quat"]))
+    print(f"config0 FP{precision}: e_pos {worst_p:.3e} e_q {worst_q:.3e}")
+    assert worst_p <= TOL[precision] and worst_q <= TOL[precision]
+    assert abs(batch.time()[0] - ref.time) <= 1e-15
+
+
+@pytest.mark.parametrize("precision", [vx3mi.FP64, vx3mi.FP32])
+def test_config3_per_voxel_modulus(precision):
+    """Per-voxel E and phase (config 3 robot), collisions off: 500 steps."""
+    prob = configs.config3()
+    ref = O.OracleModel(prob)
+    batch = vx3mi.Batch([vx3mi.Model.from_problem(prob)], precision)
+    batch.step(500)
+    ref.step(500)
+    s, r = batch.state(), ref.state()
+    ep, eq = pos_err(s["pos"], r["pos"]), quat_err(s["quat"], r["quat"])
+    print(f"config3 FP{precision}: e_pos {ep:.3e} e_q {eq:.3e}")
+    assert ep <= TOL[precision] and eq <= TOL[precision]
+
+
+def test_modes_bitwise_identical():
+    """Plain (per-robot launches), flat and persistent kernels give identical bits."""
+    probs = configs.config2(seed=5, n=12)
+    models = [vx3mi.Model.from_problem(p) for p in probs]
+    states = []
+    for mode in (vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT, vx3mi.MODE_PERSISTENT):
+        b = vx3mi.Batch(models, vx3mi.FP32)
+        b.step(130, mode=mode)
+        states.append(b.state())
+        b.close()
+    for s in states[1:]:
+        for k in ("pos", "quat", "linmom", "angmom"):
+            assert np.array_equal(s[k], states[0][k]), k
+
+
+@pytest.mark.parametrize("precision", [vx3mi.FP64, vx3mi.FP32])
+def test_batched_robots_match_oracle(precision):
+    """Uneven robots in one batch, each checked against its own oracle run."""
+    probs = configs.config2(seed=1, n=6)
+    batch = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], precision)
+    batch.step(400, mode=vx3mi.MODE_PERSISTENT)
+    for r, p in enumerate(probs):
+        ref = O.OracleModel(p)
+        ref.step(400)
+        s, o = batch.state(r), ref.state()
+        ep, eq = pos_err(s["pos"], o["pos"]), quat_err(s["quat"], o["quat"])
+        assert ep <= TOL[precision] and eq <= TOL[precision], (r, ep, eq)
+
+
+def test_recommended_dt_reduction():
+    """Device max-reduction equals the host evaluation and the oracle (exact)."""
+    probs = configs.config2(seed=2, n=20) + [configs.config0(), configs.config3()]
+    models = [vx3mi.Model.from_problem(p) for p in probs]
+    b = vx3mi.Batch(models, vx3mi.FP32)
+    dev = b.recommended_dt()
+    host = np.array([m.recommended_dt() for m in models])
+    orc = np.array([O.OracleModel(p).recommended_dt() for p in probs])
+    assert np.array_equal(dev, host)
+    assert np.array_equal(dev, orc)
+
+
+def test_center_of_mass():
+    probs = configs.config2(seed=4, n=5)
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
+    b.step(50)
+    com = b.com()
+    for r, p in enumerate(probs):
+        ref = O.OracleModel(p)
+        ref.step(50)
+        assert np.allclose(com[r], ref.com(), rtol=1e-12, atol=1e-15)

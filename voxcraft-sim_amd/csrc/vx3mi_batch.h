@@ -1,0 +1,89 @@
+// vx3mi_batch.h -- host-side batch interface shared by the FP32 and FP64
+// device translation units and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "vx3mi_internal.h"
+
+namespace vx3mi {
+
+template <typename T> struct alignas(4 * sizeof(T)) T4 { T x, y, z, w; };
+
+struct RobotParams {
+    double tempAmp, tempPeriod;
+    int varyTemp, enableExpansion, collisions, pad;
+    int voff, vcount, loff, lcount;
+};
+
+struct RobotTime {
+    double time[2];
+    float prevDt[2];
+    float dt;
+    int diverged;
+    int steps;
+    int pad;
+};
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (expr);                                                                \
+        if (e_ != hipSuccess) {                                                                \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                      \
+            return VX3MI_E_HIP;                                                                \
+        }                                                                                      \
+    } while (0)
+
+struct BatchBase {
+    int precision = 32;
+    int R = 0, V = 0, L = 0;
+    int parity = 0;
+    int persistentChunk = 1000;  // steps per persistent launch
+    long launches = 0;           // kernel launches of the last step call
+    int lastMode = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::vector<void *> allocs;
+    std::vector<RobotParams> hrp;
+    std::vector<float> defaultDt;  // (float)(DtFrac * recommendedTimeStep) per robot
+    std::vector<float> curDt;      // dt uploaded to the device
+    std::vector<int> hlat, hvmat;
+    std::vector<VoxelMat> hvmats;
+    // graph of a chunk of flat steps
+    hipGraphExec_t graph = nullptr;
+    int graphSteps = 0, graphParity = -1;
+    virtual ~BatchBase() {
+        if (graph) (void)hipGraphExecDestroy(graph);
+        for (void *p : allocs) (void)hipFree(p);
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    template <typename P> int alloc(P **p, size_t n) {
+        *p = nullptr;
+        if (n == 0) n = 1;
+        hipError_t e = hipMalloc((void **)p, n * sizeof(P));
+        if (e != hipSuccess) {
+            set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+            return VX3MI_E_HIP;
+        }
+        allocs.push_back((void *)*p);
+        return VX3MI_OK;
+    }
+    virtual int step(int nsteps, float dt, int mode) = 0;
+    virtual int get_state(int robot, double *pos, double *quat, double *lm, double *am) = 0;
+    virtual int set_state(int robot, const double *pos, const double *quat, const double *lm, const double *am) = 0;
+    virtual int get_flags(int robot, int *flags) = 0;
+    virtual int eval_links(double *out) = 0;
+    virtual int com(double *out) = 0;
+    virtual int recommended(double *out) = 0;
+    RobotTime *d_rt = nullptr;
+};
+
+BatchBase *make_batch_fp32(vx3mi_model *const *models, int n, int *rc);
+BatchBase *make_batch_fp64(vx3mi_model *const *models, int n, int *rc);
+
+}  // namespace vx3mi

@@ -1,0 +1,153 @@
+// vx3mi_batch.hip -- C ABI of device batches (include/vx3mi.h).
+#include <hip/hip_runtime.h>
+
+#include <new>
+#include <string>
+#include <vector>
+
+#include "vx3mi_batch.h"
+
+using namespace vx3mi;
+
+struct vx3mi_batch {
+    BatchBase *b = nullptr;
+};
+
+extern "C" {
+
+int vx3mi_batch_create(vx3mi_model *const *models, int n, int precision, vx3mi_batch **out) {
+    if (!out || !models || n <= 0) { set_error("vx3mi_batch_create: invalid arguments"); return VX3MI_E_ARG; }
+    *out = nullptr;
+    if (precision != VX3MI_FP32 && precision != VX3MI_FP64) { set_error("precision must be 32 or 64"); return VX3MI_E_ARG; }
+    for (int i = 0; i < n; i++)
+        if (!models[i] || models[i]->m.nv() == 0) {
+            // CUDA_Simulation aborts empty models (VX3_SimulationManager.cu:15-19)
+            set_error("model " + std::to_string(i) + " has no voxels");
+            return VX3MI_E_ARG;
+        }
+    int devs = 0;
+    if (hipGetDeviceCount(&devs) != hipSuccess || devs <= 0) {
+        set_error("no HIP device available");
+        return VX3MI_E_HIP;
+    }
+    vx3mi_batch *h = new (std::nothrow) vx3mi_batch();
+    if (!h) { set_error("out of host memory"); return VX3MI_E_NOMEM; }
+    int rc = VX3MI_OK;
+    h->b = precision == VX3MI_FP32 ? make_batch_fp32(models, n, &rc) : make_batch_fp64(models, n, &rc);
+    if (!h->b) {
+        delete h;
+        return rc;
+    }
+    *out = h;
+    return VX3MI_OK;
+}
+
+void vx3mi_batch_destroy(vx3mi_batch *b) {
+    if (!b) return;
+    if (b->b && b->b->stream) (void)hipStreamSynchronize(b->b->stream);
+    delete b->b;
+    delete b;
+}
+
+int vx3mi_batch_counts(const vx3mi_batch *b, int *nrobot, int *nvox, int *nlink) {
+    if (!b) { set_error("batch is NULL"); return VX3MI_E_ARG; }
+    if (nrobot) *nrobot = b->b->R;
+    if (nvox) *nvox = b->b->V;
+    if (nlink) *nlink = b->b->L;
+    return VX3MI_OK;
+}
+
+int vx3mi_batch_recommended_dt(vx3mi_batch *b, double *out) {
+    if (!b || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    return b->b->recommended(out);
+}
+
+int vx3mi_step(vx3mi_batch *b, int nsteps, float dt, int mode) {
+    if (!b) { set_error("batch is NULL"); return VX3MI_E_ARG; }
+    if (mode < 0 || mode > 3 || nsteps < 0) { set_error("invalid step mode or step count"); return VX3MI_E_ARG; }
+    return b->b->step(nsteps, dt, mode);
+}
+
+int vx3mi_batch_sync(vx3mi_batch *b) {
+    if (!b) { set_error("batch is NULL"); return VX3MI_E_ARG; }
+    hipError_t e = hipStreamSynchronize(b->b->stream);
+    if (e != hipSuccess) { set_error(std::string("hipStreamSynchronize: ") + hipGetErrorString(e)); return VX3MI_E_HIP; }
+    return VX3MI_OK;
+}
+
+static int check_robot(vx3mi_batch *b, int robot) {
+    if (!b) { set_error("batch is NULL"); return VX3MI_E_ARG; }
+    if (robot >= b->b->R) { set_error("robot index out of range"); return VX3MI_E_ARG; }
+    return VX3MI_OK;
+}
+
+int vx3mi_get_state(vx3mi_batch *b, int robot, double *pos3, double *quat4, double *linmom3, double *angmom3) {
+    int rc = check_robot(b, robot);
+    return rc ? rc : b->b->get_state(robot, pos3, quat4, linmom3, angmom3);
+}
+
+int vx3mi_set_state(vx3mi_batch *b, int robot, const double *pos3, const double *quat4, const double *linmom3,
+                    const double *angmom3) {
+    int rc = check_robot(b, robot);
+    return rc ? rc : b->b->set_state(robot, pos3, quat4, linmom3, angmom3);
+}
+
+int vx3mi_get_voxel_flags(vx3mi_batch *b, int robot, int *flags) {
+    int rc = check_robot(b, robot);
+    if (rc) return rc;
+    if (!flags) { set_error("flags is NULL"); return VX3MI_E_ARG; }
+    return b->b->get_flags(robot, flags);
+}
+
+static int read_rt(vx3mi_batch *b, std::vector<RobotTime> &rt) {
+    rt.resize(b->b->R);
+    hipError_t e = hipStreamSynchronize(b->b->stream);
+    if (e == hipSuccess) e = hipMemcpy(rt.data(), b->b->d_rt, sizeof(RobotTime) * b->b->R, hipMemcpyDeviceToHost);
+    if (e != hipSuccess) { set_error(std::string("reading robot time: ") + hipGetErrorString(e)); return VX3MI_E_HIP; }
+    return VX3MI_OK;
+}
+
+int vx3mi_batch_time(vx3mi_batch *b, double *out) {
+    if (!b || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    std::vector<RobotTime> rt;
+    int rc = read_rt(b, rt);
+    if (rc) return rc;
+    for (int r = 0; r < b->b->R; r++) out[r] = rt[r].time[rt[r].steps & 1];
+    return VX3MI_OK;
+}
+
+int vx3mi_batch_diverged(vx3mi_batch *b, int *out) {
+    if (!b || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    std::vector<RobotTime> rt;
+    int rc = read_rt(b, rt);
+    if (rc) return rc;
+    int any = 0;
+    for (int r = 0; r < b->b->R; r++) { out[r] = rt[r].diverged; any |= rt[r].diverged; }
+    return any ? VX3MI_DIVERGED : VX3MI_OK;
+}
+
+int vx3mi_center_of_mass(vx3mi_batch *b, double *out) {
+    if (!b || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    return b->b->com(out);
+}
+
+int vx3mi_eval_link_forces(vx3mi_batch *b, double *out12) {
+    if (!b || !out12) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    return b->b->eval_links(out12);
+}
+
+int vx3mi_last_step_ms(vx3mi_batch *b, float *ms) {
+    if (!b || !ms) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    hipError_t e = hipEventSynchronize(b->b->ev1);
+    if (e == hipSuccess) e = hipEventElapsedTime(ms, b->b->ev0, b->b->ev1);
+    if (e != hipSuccess) { set_error(std::string("event timing: ") + hipGetErrorString(e)); return VX3MI_E_HIP; }
+    return VX3MI_OK;
+}
+
+int vx3mi_last_step_launches(vx3mi_batch *b, long long *launches) {
+    if (!b || !launches) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    *launches = b->b->launches;
+    return VX3MI_OK;
+}
+
+}  // extern "C"

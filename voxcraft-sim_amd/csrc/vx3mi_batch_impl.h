@@ -1,0 +1,550 @@
+// vx3mi_batch_impl.h -- device side of the VX3 timestep for gfx950 (templated on
+// the state precision; instantiated by vx3mi_batch_fp32.hip / _fp64.hip).
+//
+// Data layout (structure of arrays, one entry per voxel / link of the whole
+// batch; robots are CSR-packed by voxel and link offsets):
+//   voxel  pq0 = (ux, uy, uz, qw)  pq1 = (qx, qy, qz, T)   u: displacement from
+//          mo0 = (px, py, pz, hx)  mo1 = (hy, hz, flags, 0)  the lattice point
+//          lat (ix | iy<<10 | iz<<20), vmat, vrob, phase, vlinks[6]
+//   link   ls0..2 = pos2, angle1v, angle2v, maxStrain, flags, strain
+//          lf0..2 = forceNeg, momentNeg, forcePos, momentPos (voxel frames)
+//          lends = (pVNeg, pVPos), lmeta = (axis | linkMat<<2, robot)
+// T is the temperature of the voxel for the step about to run; the voxel
+// kernel of step n writes the temperature of step n+1 (gpu_update_temperature,
+// VX3_VoxelyzeKernel.cu:625-650, evaluated one step ahead).  Per-robot time and
+// previous dt are double-buffered by step parity so that no thread reads a
+// value another thread of the same launch writes.
+//
+// Kernels per step: k_links (one work-item per link, writes both end forces;
+// no atomics), k_voxels (one work-item per voxel, gathers its <=6 link ends in
+// direction order X_POS..Z_NEG, integrates).  k_persistent runs many steps with
+// one workgroup per robot and workgroup barriers between the two phases.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "vx3mi_batch.h"
+
+namespace vx3mi {
+
+template <typename T> struct Dev {
+    T4<T> *pq0, *pq1, *mo0, *mo1;
+    const int *lat, *vmat, *vrob;
+    const T *phase;
+    const int *vlinks;
+    const int2 *lends, *lmeta;
+    T4<T> *ls0, *ls1, *ls2;
+    T4<T> *lf0, *lf1, *lf2;
+    const VoxelMat *vmats;
+    const LinkMat *lmats;
+    const RobotParams *rp;
+    RobotTime *rt;
+    const T4<T> *contact;
+};
+
+// ------------------------------------------------------------------ device
+template <typename T> __device__ __forceinline__ void link_update(const Dev<T> &a, int l, int parity) {
+    const int2 meta = a.lmeta[l];
+    RobotTime *rt = a.rt + meta.y;
+    if (rt->diverged) return;
+    const LinkMat &lm = a.lmats[meta.x >> 2];
+    if (lm.fixedBoth) return;  // gpu_update_links VX3_VoxelyzeKernel.cu:572-573
+    const int axis = meta.x & 3;
+    const int2 e = a.lends[l];
+    const T4<T> n0 = a.pq0[e.x], n1 = a.pq1[e.x], p0 = a.pq0[e.y], p1 = a.pq1[e.y];
+    V3<T> du = {p0.x - n0.x, p0.y - n0.y, p0.z - n0.z};
+    Q4<T> qn = {n0.w, n1.x, n1.y, n1.z}, qp = {p0.w, p1.x, p1.y, p1.z};
+    const T4<T> s0 = a.ls0[l], s1 = a.ls1[l], s2 = a.ls2[l];
+    LinkState<T> st;
+    st.pos2 = {s0.x, s0.y, s0.z};
+    st.a1v = {s0.w, s1.x, s1.y};
+    st.a2v = {s1.z, s1.w, s2.x};
+    st.maxStrain = (float)s2.y;
+    st.flags = (int)s2.z;
+    LinkOut<T> o;
+    link_forces<T>(lm, axis, du, qn, qp, (float)n1.w, (float)p1.w, rt->prevDt[parity], st, o);
+    a.ls0[l] = T4<T>{st.pos2.x, st.pos2.y, st.pos2.z, st.a1v.x};
+    a.ls1[l] = T4<T>{st.a1v.y, st.a1v.z, st.a2v.x, st.a2v.y};
+    a.ls2[l] = T4<T>{st.a2v.z, (T)st.maxStrain, (T)st.flags, (T)o.strain};
+    a.lf0[l] = T4<T>{o.fN.x, o.fN.y, o.fN.z, o.mN.x};
+    a.lf1[l] = T4<T>{o.mN.y, o.mN.z, o.fP.x, o.fP.y};
+    a.lf2[l] = T4<T>{o.fP.z, o.mP.x, o.mP.y, o.mP.z};
+    // divergence (VX3_VoxelyzeKernel.cu:271-280): any link, not one sampled link
+    if (o.strain > 100.0f) atomicOr(&rt->diverged, 1);
+}
+
+template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> &a, int v, int parity) {
+    const int r = a.vrob[v];
+    RobotTime *rt = a.rt + r;
+    if (rt->diverged) return;
+    const RobotParams &rp = a.rp[r];
+    const float dt = rt->dt;
+    const double tnext = rt->time[parity] + (double)dt;  // currentTime += dt (:352)
+    if (v == rp.voff) {
+        rt->time[parity ^ 1] = tnext;
+        rt->prevDt[parity ^ 1] = dt;
+        rt->steps += 1;
+    }
+    const VoxelMat &vm = a.vmats[a.vmat[v]];
+    if (vm.fixed) return;  // gpu_update_voxels :588-589
+
+    const int *vl = a.vlinks + 6 * v;
+    V3<T> fs = {0, 0, 0}, ms = {0, 0, 0};
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+        const int l = vl[d];
+        if (l < 0) continue;
+        if (d & 1) {  // link in a NEG direction: this voxel is its pVPos end
+            const T4<T> f1 = a.lf1[l], f2 = a.lf2[l];
+            fs = fs + V3<T>{f1.z, f1.w, f2.x};
+            ms = ms + V3<T>{f2.y, f2.z, f2.w};
+        } else {
+            const T4<T> f0 = a.lf0[l], f1 = a.lf1[l];
+            fs = fs + V3<T>{f0.x, f0.y, f0.z};
+            ms = ms + V3<T>{f0.w, f1.x, f1.y};
+        }
+    }
+    const T4<T> q0 = a.pq0[v], q1 = a.pq1[v], m0 = a.mo0[v], m1 = a.mo1[v];
+    VoxelState<T> s;
+    s.u = {q0.x, q0.y, q0.z};
+    s.q = {q0.w, q1.x, q1.y, q1.z};
+    s.p = {m0.x, m0.y, m0.z};
+    s.h = {m0.w, m1.x, m1.y};
+    s.flags = (int)m1.z;
+    float temp = (float)q1.w;
+    V3<T> contact = {0, 0, 0};
+    if (a.contact) {
+        const T4<T> c = a.contact[v];
+        contact = {c.x, c.y, c.z};
+    }
+    voxel_step<T>(vm, dt, a.lat[v] >> 20, temp, fs, ms, contact, s);
+    if (rp.varyTemp && !(vm.thermalOnAfter > tnext))
+        temp = actuation_temperature<T>(rp.tempAmp, rp.tempPeriod, tnext, (double)a.phase[v], rp.enableExpansion);
+    a.pq0[v] = T4<T>{s.u.x, s.u.y, s.u.z, s.q.w};
+    a.pq1[v] = T4<T>{s.q.x, s.q.y, s.q.z, (T)temp};
+    a.mo0[v] = T4<T>{s.p.x, s.p.y, s.p.z, s.h.x};
+    a.mo1[v] = T4<T>{s.h.y, s.h.z, (T)s.flags, (T)0};
+}
+
+template <typename T> __global__ __launch_bounds__(256) void k_links(Dev<T> a, int l0, int n, int parity) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) link_update(a, l0 + i, parity);
+}
+
+template <typename T> __global__ __launch_bounds__(256) void k_voxels(Dev<T> a, int v0, int n, int parity) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) voxel_update(a, v0 + i, parity);
+}
+
+// One workgroup per robot; nsteps steps with two barrier-separated phases.
+template <typename T> __global__ __launch_bounds__(256) void k_persistent(Dev<T> a, int nsteps, int parity0) {
+    const RobotParams rp = a.rp[blockIdx.x];
+    for (int s = 0; s < nsteps; s++) {
+        const int parity = (parity0 + s) & 1;
+        for (int l = rp.loff + threadIdx.x; l < rp.loff + rp.lcount; l += blockDim.x) link_update(a, l, parity);
+        __syncthreads();
+        for (int v = rp.voff + threadIdx.x; v < rp.voff + rp.vcount; v += blockDim.x) voxel_update(a, v, parity);
+        __syncthreads();
+    }
+}
+
+// recommendedTimeStep (VX3_VoxelyzeKernel.cu:184-217) as a max-reduction:
+// per-lane maxima, a 64-lane butterfly, then one value per wavefront in LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void k_recommended_dt(const int2 *lends, const int2 *lmeta, const int *vmat,
+                                                        const VoxelMat *vmats, const LinkMat *lmats,
+                                                        const float *vmatE, const RobotParams *rp, double *out) {
+    __shared__ double wmax[4];
+    const RobotParams p = rp[blockIdx.x];
+    double mx = 0.0;
+    for (int l = p.loff + threadIdx.x; l < p.loff + p.lcount; l += blockDim.x) {
+        const LinkMat &lm = lmats[lmeta[l].x >> 2];
+        const int2 e = lends[l];
+        const double m1 = vmats[vmat[e.x]].mass, m2 = vmats[vmat[e.y]].mass;
+        const double f2 = lm.a1 / (m1 < m2 ? m1 : m2);
+        mx = f2 > mx ? f2 : mx;
+    }
+    for (int off = 32; off > 0; off >>= 1) {
+        const double o = __shfl_xor(mx, off, 64);
+        mx = o > mx ? o : mx;
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) wmax[wave] = mx;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < (int)(blockDim.x >> 6); w++) mx = wmax[w] > mx ? wmax[w] : mx;
+        if (mx <= 0.0) {  // no links: voxel stiffness E*L/m (:204-211)
+            for (int v = p.voff; v < p.voff + p.vcount; v++) {
+                const VoxelMat &vm = vmats[vmat[v]];
+                const double f2 = vmatE[vmat[v]] * vm.nomSize / vm.mass;
+                mx = f2 > mx ? f2 : mx;
+            }
+        }
+        out[blockIdx.x] = mx <= 0.0 ? 0.0 : 1.0f / (6.283185f * sqrt(mx));
+    }
+}
+
+// mass-weighted centre of mass of measured voxels, one workgroup per robot
+template <typename T>
+__global__ __launch_bounds__(256) void k_com(const T4<T> *pq0, const int *lat, const int *vmat, const VoxelMat *vmats,
+                                             const RobotParams *rp, double *out) {
+    __shared__ double red[4][256];
+    const RobotParams p = rp[blockIdx.x];
+    double sx = 0, sy = 0, sz = 0, sm = 0;
+    for (int v = p.voff + threadIdx.x; v < p.voff + p.vcount; v += blockDim.x) {
+        const VoxelMat &vm = vmats[vmat[v]];
+        if (!vm.isMeasured) continue;
+        const int pk = lat[v];
+        const double L = kAbsolutePositions<T> ? 0.0 : vm.nomSize, w = vm.mass;
+        const T4<T> q = pq0[v];
+        sx += ((pk & 1023) * L + (double)q.x) * w;
+        sy += (((pk >> 10) & 1023) * L + (double)q.y) * w;
+        sz += ((pk >> 20) * L + (double)q.z) * w;
+        sm += w;
+    }
+    red[0][threadIdx.x] = sx; red[1][threadIdx.x] = sy; red[2][threadIdx.x] = sz; red[3][threadIdx.x] = sm;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const double tm = red[3][0];
+        out[3 * blockIdx.x + 0] = tm == 0 ? 0.0 : red[0][0] / tm;
+        out[3 * blockIdx.x + 1] = tm == 0 ? 0.0 : red[1][0] / tm;
+        out[3 * blockIdx.x + 2] = tm == 0 ? 0.0 : red[2][0] / tm;
+    }
+}
+
+// ------------------------------------------------------------------- host
+template <typename T> struct Batch : BatchBase {
+    Dev<T> d{};
+    RobotParams *d_rp = nullptr;
+    VoxelMat *d_vmats = nullptr;
+    LinkMat *d_lmats = nullptr;
+    float *d_vmatE = nullptr;
+    double *d_out = nullptr;
+
+    int create(vx3mi_model *const *models, int n) {
+        R = n;
+        std::vector<int> lat, vmat, vrob, vlinks;
+        std::vector<T> phase;
+        std::vector<int2> lends, lmeta;
+        std::vector<LinkMat> lmats;
+        std::vector<float> vmatE;
+        std::vector<T4<T>> pq0, pq1;
+        hrp.resize(n);
+        for (int r = 0; r < n; r++) {
+            const Model &m = models[r]->m;
+            const int voff = (int)lat.size(), loff = (int)lends.size();
+            const int vmoff = (int)hvmats.size(), lmoff = (int)lmats.size();
+            RobotParams &p = hrp[r];
+            std::memset(&p, 0, sizeof(p));
+            p.tempAmp = m.tempAmplitude;
+            p.tempPeriod = m.tempPeriod;
+            p.varyTemp = m.varyTemp;
+            p.enableExpansion = m.enableExpansion;
+            p.collisions = m.enableCollision;
+            p.voff = voff; p.vcount = m.nv(); p.loff = loff; p.lcount = m.nl();
+            for (const VoxelMat &vm : m.vmats) hvmats.push_back(vm);
+            for (float e : m.vmatE) vmatE.push_back(e);
+            for (const LinkMat &lmt : m.lmats) lmats.push_back(lmt);
+            for (int v = 0; v < m.nv(); v++) {
+                lat.push_back(m.ix[v] | (m.iy[v] << 10) | (m.iz[v] << 20));
+                vmat.push_back(vmoff + m.vmat[v]);
+                vrob.push_back(r);
+                phase.push_back((T)m.phase[v]);
+                for (int k = 0; k < 6; k++) {
+                    int l = m.vlinks[(size_t)v * 6 + k];
+                    vlinks.push_back(l < 0 ? -1 : loff + l);
+                }
+                const VoxelMat &vm = m.vmats[m.vmat[v]];
+                float t0 = m.initialTemp;
+                if (m.varyTemp && !vm.fixed && !(vm.thermalOnAfter > 0.0))
+                    t0 = actuation_temperature<T>(m.tempAmplitude, m.tempPeriod, 0.0, m.phase[v], m.enableExpansion);
+                const double L0 = vm.nomSize;
+                if (kAbsolutePositions<T>) pq0.push_back(T4<T>{(T)(m.ix[v] * L0), (T)(m.iy[v] * L0), (T)(m.iz[v] * L0), 1});
+                else pq0.push_back(T4<T>{0, 0, 0, 1});
+                pq1.push_back(T4<T>{0, 0, 0, (T)t0});
+            }
+            for (int l = 0; l < m.nl(); l++) {
+                lends.push_back(int2{voff + m.lneg[l], voff + m.lpos[l]});
+                lmeta.push_back(int2{m.laxis[l] | ((lmoff + m.lmat[l]) << 2), r});
+            }
+            float dflt = (float)(m.dtFrac * std::max(recommended_dt(m), 1e-10));
+            defaultDt.push_back(dflt);
+            (void)lmoff;
+        }
+        V = (int)lat.size();
+        L = (int)lends.size();
+        hlat = lat;
+        hvmat = vmat;
+
+        HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&ev0));
+        HIP_TRY(hipEventCreate(&ev1));
+        T4<T> *pq0d, *pq1d, *mo0d, *mo1d, *ls0, *ls1, *ls2, *lf0, *lf1, *lf2;
+        int *latd, *vmatd, *vrobd, *vlinksd;
+        T *phased;
+        int2 *lendsd, *lmetad;
+        int rc = VX3MI_OK;
+        if ((rc = alloc(&pq0d, V)) || (rc = alloc(&pq1d, V)) || (rc = alloc(&mo0d, V)) || (rc = alloc(&mo1d, V)) ||
+            (rc = alloc(&latd, V)) || (rc = alloc(&vmatd, V)) || (rc = alloc(&vrobd, V)) || (rc = alloc(&phased, V)) ||
+            (rc = alloc(&vlinksd, (size_t)6 * V)) || (rc = alloc(&lendsd, L)) || (rc = alloc(&lmetad, L)) ||
+            (rc = alloc(&ls0, L)) || (rc = alloc(&ls1, L)) || (rc = alloc(&ls2, L)) || (rc = alloc(&lf0, L)) ||
+            (rc = alloc(&lf1, L)) || (rc = alloc(&lf2, L)) || (rc = alloc(&d_vmats, hvmats.size())) ||
+            (rc = alloc(&d_lmats, lmats.size())) || (rc = alloc(&d_vmatE, vmatE.size())) || (rc = alloc(&d_rp, R)) ||
+            (rc = alloc(&d_rt, R)) || (rc = alloc(&d_out, (size_t)3 * R)))
+            return rc;
+        std::vector<T4<T>> zeros((size_t)std::max(V, L), T4<T>{0, 0, 0, 0});
+        std::vector<T4<T>> mo1((size_t)V, T4<T>{0, 0, 0, 0});
+        for (int r = 0; r < n; r++) {
+            const Model &m = models[r]->m;
+            const int fl = VF_FLOOR_STATIC_FRICTION | (m.floorEnabled ? VF_FLOOR_ENABLED : 0);
+            for (int v = 0; v < m.nv(); v++) mo1[hrp[r].voff + v].z = (T)fl;
+        }
+        std::vector<T4<T>> ls2h((size_t)L, T4<T>{0, 0, (T)LF_SMALL_ANGLE, 0});
+        HIP_TRY(hipMemcpy(pq0d, pq0.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(pq1d, pq1.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(mo0d, zeros.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(mo1d, mo1.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(latd, lat.data(), sizeof(int) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(vmatd, vmat.data(), sizeof(int) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(vrobd, vrob.data(), sizeof(int) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(phased, phase.data(), sizeof(T) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(vlinksd, vlinks.data(), sizeof(int) * 6 * V, hipMemcpyHostToDevice));
+        if (L) {
+            HIP_TRY(hipMemcpy(lendsd, lends.data(), sizeof(int2) * L, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(lmetad, lmeta.data(), sizeof(int2) * L, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(ls0, zeros.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(ls1, zeros.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(ls2, ls2h.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(lf0, zeros.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(lf1, zeros.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(lf2, zeros.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(d_lmats, lmats.data(), sizeof(LinkMat) * lmats.size(), hipMemcpyHostToDevice));
+        }
+        HIP_TRY(hipMemcpy(d_vmats, hvmats.data(), sizeof(VoxelMat) * hvmats.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_vmatE, vmatE.data(), sizeof(float) * vmatE.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_rp, hrp.data(), sizeof(RobotParams) * R, hipMemcpyHostToDevice));
+        std::vector<RobotTime> rt((size_t)R);
+        for (int r = 0; r < R; r++) {
+            std::memset(&rt[r], 0, sizeof(RobotTime));
+            rt[r].dt = defaultDt[r];
+        }
+        curDt = defaultDt;
+        HIP_TRY(hipMemcpy(d_rt, rt.data(), sizeof(RobotTime) * R, hipMemcpyHostToDevice));
+
+        d.pq0 = pq0d; d.pq1 = pq1d; d.mo0 = mo0d; d.mo1 = mo1d;
+        d.lat = latd; d.vmat = vmatd; d.vrob = vrobd; d.phase = phased; d.vlinks = vlinksd;
+        d.lends = lendsd; d.lmeta = lmetad;
+        d.ls0 = ls0; d.ls1 = ls1; d.ls2 = ls2; d.lf0 = lf0; d.lf1 = lf1; d.lf2 = lf2;
+        d.vmats = d_vmats; d.lmats = d_lmats; d.rp = d_rp; d.rt = d_rt; d.contact = nullptr;
+        return VX3MI_OK;
+    }
+
+    // upload per-robot dt if it changed (robot time/prevDt stay on the device)
+    int set_dt(float dt) {
+        std::vector<float> want = defaultDt;
+        if (dt >= 0) std::fill(want.begin(), want.end(), dt);
+        if (want == curDt) return VX3MI_OK;
+        for (int r = 0; r < R; r++)
+            HIP_TRY(hipMemcpyAsync(&d_rt[r].dt, &want[r], sizeof(float), hipMemcpyHostToDevice, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        curDt = want;
+        return VX3MI_OK;
+    }
+
+    void launch_flat_step(int parity) {
+        if (L) k_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, 0, L, parity);
+        k_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
+    }
+
+    int step(int nsteps, float dt, int mode) override {
+        if (nsteps <= 0) return VX3MI_OK;
+        if (dt == 0.0f) return VX3MI_OK;  // doTimeStep(0) only advances the step counter
+        int rc = set_dt(dt);
+        if (rc) return rc;
+        if (mode == VX3MI_MODE_AUTO) mode = (R > 1 && V / R <= 4096) ? VX3MI_MODE_PERSISTENT : VX3MI_MODE_FLAT;
+        HIP_TRY(hipEventRecord(ev0, stream));
+        launches = 0;
+        lastMode = mode;
+        if (mode == VX3MI_MODE_PLAIN) {
+            for (int s = 0; s < nsteps; s++) {
+                for (int r = 0; r < R; r++) {
+                    const RobotParams &p = hrp[r];
+                    if (p.lcount) k_links<T><<<(p.lcount + 255) / 256, 256, 0, stream>>>(d, p.loff, p.lcount, parity);
+                    k_voxels<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
+                    launches += 2;
+                }
+                parity ^= 1;
+            }
+        } else if (mode == VX3MI_MODE_PERSISTENT) {
+            // launches of at most persistentChunk steps keep each kernel short
+            for (int s = 0; s < nsteps; s += persistentChunk) {
+                const int k = std::min(persistentChunk, nsteps - s);
+                k_persistent<T><<<R, 256, 0, stream>>>(d, k, parity);
+                parity = (parity + k) & 1;
+                launches++;
+            }
+        } else {
+            const int chunk = 64;  // steps per captured graph (even: parity returns to its start)
+            int s = 0;
+            if (nsteps >= chunk) {
+                if (!graph || graphParity != parity) {
+                    if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+                    hipGraph_t g;
+                    HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+                    for (int k = 0; k < chunk; k++) launch_flat_step((parity + k) & 1);
+                    HIP_TRY(hipStreamEndCapture(stream, &g));
+                    HIP_TRY(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
+                    HIP_TRY(hipGraphDestroy(g));
+                    graphParity = parity;
+                    graphSteps = chunk;
+                }
+                for (; s + chunk <= nsteps; s += chunk) {
+                    HIP_TRY(hipGraphLaunch(graph, stream));
+                    launches += 2 * chunk;
+                }
+            }
+            for (; s < nsteps; s++) {
+                launch_flat_step(parity);
+                parity ^= 1;
+                launches += 2;
+            }
+        }
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipEventRecord(ev1, stream));
+        return VX3MI_OK;
+    }
+
+    int fetch(std::vector<T4<T>> &a0, std::vector<T4<T>> &a1, std::vector<T4<T>> &b0, std::vector<T4<T>> &b1) {
+        a0.resize(V); a1.resize(V); b0.resize(V); b1.resize(V);
+        HIP_TRY(hipStreamSynchronize(stream));
+        HIP_TRY(hipMemcpy(a0.data(), d.pq0, sizeof(T4<T>) * V, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(a1.data(), d.pq1, sizeof(T4<T>) * V, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(b0.data(), d.mo0, sizeof(T4<T>) * V, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(b1.data(), d.mo1, sizeof(T4<T>) * V, hipMemcpyDeviceToHost));
+        return VX3MI_OK;
+    }
+
+    void range(int robot, int &v0, int &v1) {
+        if (robot < 0) { v0 = 0; v1 = V; }
+        else { v0 = hrp[robot].voff; v1 = v0 + hrp[robot].vcount; }
+    }
+
+    int get_state(int robot, double *pos, double *quat, double *lm, double *am) override {
+        std::vector<T4<T>> a0, a1, b0, b1;
+        int rc = fetch(a0, a1, b0, b1);
+        if (rc) return rc;
+        int v0, v1;
+        range(robot, v0, v1);
+        for (int v = v0; v < v1; v++) {
+            const int i = v - v0, pk = hlat[v];
+            const double Ls = kAbsolutePositions<T> ? 0.0 : hvmats[hvmat[v]].nomSize;
+            if (pos) {
+                pos[3 * i] = (pk & 1023) * Ls + (double)a0[v].x;
+                pos[3 * i + 1] = ((pk >> 10) & 1023) * Ls + (double)a0[v].y;
+                pos[3 * i + 2] = (pk >> 20) * Ls + (double)a0[v].z;
+            }
+            if (quat) { quat[4 * i] = a0[v].w; quat[4 * i + 1] = a1[v].x; quat[4 * i + 2] = a1[v].y; quat[4 * i + 3] = a1[v].z; }
+            if (lm) { lm[3 * i] = b0[v].x; lm[3 * i + 1] = b0[v].y; lm[3 * i + 2] = b0[v].z; }
+            if (am) { am[3 * i] = b0[v].w; am[3 * i + 1] = b1[v].x; am[3 * i + 2] = b1[v].y; }
+        }
+        return VX3MI_OK;
+    }
+
+    int set_state(int robot, const double *pos, const double *quat, const double *lm, const double *am) override {
+        std::vector<T4<T>> a0, a1, b0, b1;
+        int rc = fetch(a0, a1, b0, b1);
+        if (rc) return rc;
+        int v0, v1;
+        range(robot, v0, v1);
+        for (int v = v0; v < v1; v++) {
+            const int i = v - v0, pk = hlat[v];
+            const double Ls = kAbsolutePositions<T> ? 0.0 : hvmats[hvmat[v]].nomSize;
+            if (pos) {
+                a0[v].x = (T)(pos[3 * i] - (pk & 1023) * Ls);
+                a0[v].y = (T)(pos[3 * i + 1] - ((pk >> 10) & 1023) * Ls);
+                a0[v].z = (T)(pos[3 * i + 2] - (pk >> 20) * Ls);
+            }
+            if (quat) { a0[v].w = (T)quat[4 * i]; a1[v].x = (T)quat[4 * i + 1]; a1[v].y = (T)quat[4 * i + 2]; a1[v].z = (T)quat[4 * i + 3]; }
+            if (lm) { b0[v].x = (T)lm[3 * i]; b0[v].y = (T)lm[3 * i + 1]; b0[v].z = (T)lm[3 * i + 2]; }
+            if (am) { b0[v].w = (T)am[3 * i]; b1[v].x = (T)am[3 * i + 1]; b1[v].y = (T)am[3 * i + 2]; }
+        }
+        HIP_TRY(hipMemcpy(d.pq0, a0.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.pq1, a1.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.mo0, b0.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d.mo1, b1.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+        return VX3MI_OK;
+    }
+
+    int get_flags(int robot, int *flags) override {
+        std::vector<T4<T>> a0, a1, b0, b1;
+        int rc = fetch(a0, a1, b0, b1);
+        if (rc) return rc;
+        int v0, v1;
+        range(robot, v0, v1);
+        for (int v = v0; v < v1; v++) flags[v - v0] = (int)b1[v].z;
+        return VX3MI_OK;
+    }
+
+    int eval_links(double *out) override {
+        if (L) k_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, 0, L, parity);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(stream));
+        std::vector<T4<T>> f0(L), f1(L), f2(L);
+        if (L) {
+            HIP_TRY(hipMemcpy(f0.data(), d.lf0, sizeof(T4<T>) * L, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(f1.data(), d.lf1, sizeof(T4<T>) * L, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(f2.data(), d.lf2, sizeof(T4<T>) * L, hipMemcpyDeviceToHost));
+        }
+        for (int l = 0; l < L; l++) {
+            double *o = out + 12 * l;
+            // forceNeg, forcePos, momentNeg, momentPos
+            o[0] = f0[l].x; o[1] = f0[l].y; o[2] = f0[l].z;
+            o[3] = f1[l].z; o[4] = f1[l].w; o[5] = f2[l].x;
+            o[6] = f0[l].w; o[7] = f1[l].x; o[8] = f1[l].y;
+            o[9] = f2[l].y; o[10] = f2[l].z; o[11] = f2[l].w;
+        }
+        return VX3MI_OK;
+    }
+
+    int com(double *out) override {
+        k_com<T><<<R, 256, 0, stream>>>(d.pq0, d.lat, d.vmat, d_vmats, d_rp, d_out);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, d_out, sizeof(double) * 3 * R, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return VX3MI_OK;
+    }
+
+    int recommended(double *out) override {
+        k_recommended_dt<T><<<R, 256, 0, stream>>>(d.lends, d.lmeta, d.vmat, d_vmats, d_lmats, d_vmatE, d_rp, d_out);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipMemcpyAsync(out, d_out, sizeof(double) * R, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        return VX3MI_OK;
+    }
+};
+
+template <typename T> BatchBase *make_batch(vx3mi_model *const *models, int n, int *rc) {
+    Batch<T> *b = new (std::nothrow) Batch<T>();
+    if (!b) { *rc = VX3MI_E_NOMEM; set_error("out of host memory"); return nullptr; }
+    b->precision = (int)(8 * sizeof(T));
+    try {
+        *rc = b->create(models, n);
+    } catch (const std::bad_alloc &) {
+        *rc = VX3MI_E_NOMEM;
+        set_error("out of host memory");
+    }
+    if (*rc != VX3MI_OK) { delete b; return nullptr; }
+    return b;
+}
+
+}  // namespace vx3mi

@@ -1,0 +1,54 @@
+// vx3mi_internal.h -- host-side model representation shared by the VXA reader,
+// the model builder and the batch (device) code.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "vx3mi.h"
+#include "vx3mi_physics.h"
+
+namespace vx3mi {
+
+void set_error(const std::string &msg);
+
+// Host model: topology, materials and initial state of one robot, in the
+// reference's voxel and link order.
+struct Model {
+    int nx = 0, ny = 0, nz = 0;
+    double latticeDim = 0.001;
+    // voxels (voxel order)
+    std::vector<int> ix, iy, iz;
+    std::vector<int> vmat;          // index into vmats
+    std::vector<double> phase;      // cycles
+    std::vector<int> vlinks;        // 6 per voxel, X_POS..Z_NEG, -1 none
+    std::vector<int> surface;       // surface voxel indices
+    // links (link order)
+    std::vector<int> lneg, lpos, laxis, lmat;
+    // materials
+    std::vector<VoxelMat> vmats;
+    std::vector<LinkMat> lmats;
+    std::vector<float> vmatE;       // Young's modulus per voxel material (host only)
+    std::vector<vx3mi_material> palette;  // display/metadata of palette entries
+    // settings
+    double dtFrac = 0.9;
+    int floorEnabled = 1, enableCollision = 1, enableExpansion = 0;
+    int varyTemp = 0;               // VaryTempEnabled && TempPeriod > 0
+    double tempAmplitude = 0, tempPeriod = 0.1, tempBase = 25;
+    float initialTemp = 0;          // UpdateMatTemps result (src/VXA/VX_Sim.cpp:467-483)
+    // settings carried for the drop-in runner (VX3_SimulationManager.cu:323-376)
+    std::string stopFormulaXml, fitnessXml;
+    int recordStepSize = 0, recordVoxel = 1, recordLink = 0;
+    int nv() const { return (int)ix.size(); }
+    int nl() const { return (int)lneg.size(); }
+};
+
+// recommendedTimeStep, VX3_VoxelyzeKernel.cu:184-217 (double arithmetic on
+// the float material constants)
+double recommended_dt(const Model &m);
+
+}  // namespace vx3mi
+
+struct vx3mi_model {
+    vx3mi::Model m;
+};

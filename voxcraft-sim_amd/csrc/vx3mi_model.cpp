@@ -1,0 +1,335 @@
+// vx3mi_model.cpp -- builds a host model from a lattice description the way the
+// reference's CVX_Sim::Import (src/VXA/VX_Sim.cpp:59-153) and CVoxelyze
+// (src/old/Voxelyze.cpp:422-641) do: same voxel order, same link order and
+// orientation, same combined link materials, same float/double rounding of
+// the derived material constants (src/old/VX_Material*.cpp).
+#include <cfloat>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <new>
+#include <utility>
+
+#include "vx3mi_internal.h"
+
+namespace vx3mi {
+
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+
+namespace {
+
+// Derived quantities of a voxel material (CVX_Material::updateDerived
+// src/old/VX_Material.cpp:542-548, CVX_MaterialVoxel::updateDerived
+// src/old/VX_MaterialVoxel.cpp:57-79).
+struct MatDerived {
+    float E, nu, eHat, rho, alpha, muS, muK, zI, zG, zC, gravMult, sigmaFail, epsFail;
+    double nomSize;
+    float mass, massInv, sqrtMass, inertia, inertiaInv, c2M, c2I;
+    int fixed, isMeasured;
+    double thermalOn;
+};
+
+void derive(MatDerived &d) {
+    d.eHat = d.E / ((1 - 2 * d.nu) * (1 + d.nu));
+    double ns = d.nomSize, volume = ns * ns * ns;
+    d.mass = (float)(volume * d.rho);
+    d.inertia = (float)(d.mass * ns * ns / 6.0f);
+    if (volume == 0 || d.mass == 0 || d.inertia == 0) {
+        d.massInv = d.sqrtMass = d.inertiaInv = d.c2M = d.c2I = 0.0f;
+        return;
+    }
+    d.massInv = 1.0f / d.mass;
+    d.sqrtMass = std::sqrt(d.mass);
+    d.inertiaInv = 1.0f / d.inertia;
+    d.c2M = (float)(2.0f * std::sqrt((double)(d.mass * d.E) * ns));
+    d.c2I = (float)(2.0f * std::sqrt((double)(d.inertia * d.E) * ns * ns * ns));
+}
+
+VoxelMat to_voxel_mat(const MatDerived &d) {
+    VoxelMat v;
+    std::memset(&v, 0, sizeof(v));
+    v.nomSize = d.nomSize;
+    v.thermalOnAfter = d.thermalOn;
+    v.alphaCTE = d.alpha;
+    v.mass = d.mass;
+    v.massInv = d.massInv;
+    v.inertiaInv = d.inertiaInv;
+    v.dmNum = 2 * d.sqrtMass * d.zI;
+    v.gDampT = d.zG * d.c2M;
+    v.gDampR = d.zG * d.c2I;
+    v.cDampT = d.zC * d.c2M;
+    v.gravF = -d.mass * 9.80665f * d.gravMult;
+    v.penK = (float)(2 * d.E * d.nomSize);
+    v.muS = d.muS;
+    v.muK = d.muK;
+    v.fixed = d.fixed;
+    v.isMeasured = d.isMeasured;
+    return v;
+}
+
+// CVX_MaterialLink::updateAll / updateDerived, src/old/VX_MaterialLink.cpp:45-141
+LinkMat combine(const MatDerived &a, const MatDerived &b) {
+    LinkMat l;
+    std::memset(&l, 0, sizeof(l));
+    l.nomSize = 0.5 * (a.nomSize + b.nomSize);
+    float f1 = a.sigmaFail, f2 = b.sigmaFail, sf;
+    if (f1 == -1.0f) sf = f2;
+    else if (f2 == -1.0f) sf = f1;
+    else sf = f1 < f2 ? f1 : f2;
+    l.E = 2.0f * a.E * b.E / (a.E + b.E);
+    l.epsFail = (sf == -1.0f) ? -1.0f : sf / l.E;
+    if (a.nu == 0 && b.nu == 0) l.nu = 0;
+    else {
+        float tmpEHat = 2 * a.eHat * b.eHat / (a.eHat + b.eHat);
+        float c2 = (tmpEHat - l.E) / (2 * tmpEHat) + 0.0625;
+        l.nu = (float)(std::sqrt(c2) - 0.25);
+    }
+    l.eHat = l.E / ((1 - 2 * l.nu) * (1 + l.nu));
+    float L = (float)l.nomSize;
+    l.a1 = l.E * L;
+    l.a2 = l.E * L * L * L / (12.0f * (1 + l.nu));
+    l.b1 = l.E * L;
+    l.b2 = l.E * L * L / 2.0f;
+    l.b3 = l.E * L * L * L / 6.0f;
+    l.sqA1 = std::sqrt(l.a1);
+    l.sqA2xIp = std::sqrt(l.a2 * L * L / 6.0f);
+    l.sqB1 = std::sqrt(l.b1);
+    l.sqB2xFMp = std::sqrt(l.b2 * L / 2.0f);
+    l.sqB3xIp = std::sqrt(l.b3 * L * L / 6.0f);
+    return l;
+}
+
+float clamp_nu(float nu) {  // CVX_Material::setPoissonsRatio VX_Material.cpp:488-494
+    if (nu < 0) nu = 0;
+    if (nu >= 0.5) nu = (float)(0.5 - FLT_EPSILON * 2);
+    return nu;
+}
+
+}  // namespace
+
+double recommended_dt(const Model &m) {
+    double maxFreq2 = 0.0f;
+    for (int i = 0; i < m.nl(); i++) {
+        const LinkMat &l = m.lmats[m.lmat[i]];
+        double m1 = m.vmats[m.vmat[m.lneg[i]]].mass, m2 = m.vmats[m.vmat[m.lpos[i]]].mass;
+        float k = l.a1;  // axialStiffness(), VX3_Link.cu:269-278 (nu == 0)
+        if (l.nu != 0.0f) {
+            float fN = thermal_factor(m.initialTemp, l.alphaN), fP = thermal_factor(m.initialTemp, l.alphaP);
+            double rl = 0.5 * (l.nomN * (double)fN + l.nomP * (double)fP);
+            k = (float)(l.eHat * l.tArea / ((0 + 1) * rl));
+        }
+        double f2 = k / (m1 < m2 ? m1 : m2);
+        if (f2 > maxFreq2) maxFreq2 = f2;
+    }
+    if (maxFreq2 <= 0.0f) {
+        for (int v = 0; v < m.nv(); v++) {
+            const VoxelMat &vm = m.vmats[m.vmat[v]];
+            double f2 = m.vmatE[m.vmat[v]] * vm.nomSize / vm.mass;
+            if (f2 > maxFreq2) maxFreq2 = f2;
+        }
+    }
+    if (maxFreq2 <= 0.0f) return 0.0f;
+    return 1.0f / (6.283185f * std::sqrt(maxFreq2));
+}
+
+static int build_model(const vx3mi_lattice *sp, Model &m) {
+    if (!sp || sp->nx <= 0 || sp->ny <= 0 || sp->nz <= 0 || !sp->cells || sp->nmat < 0 || (sp->nmat > 0 && !sp->mats)) {
+        set_error("vx3mi_model_from_lattice: invalid lattice description");
+        return VX3MI_E_ARG;
+    }
+    if (sp->nx > 1023 || sp->ny > 1023 || sp->nz > 1023) {
+        set_error("lattice dimension above 1023 voxels per axis");
+        return VX3MI_E_LIMIT;
+    }
+    const int nx = sp->nx, ny = sp->ny, nz = sp->nz;
+    const long ncell = (long)nx * ny * nz;
+    m.nx = nx; m.ny = ny; m.nz = nz;
+    m.latticeDim = sp->lattice_dim;
+    m.dtFrac = sp->dt_frac;
+    m.floorEnabled = sp->floor_enabled ? 1 : 0;
+    m.enableCollision = sp->enable_collision ? 1 : 0;
+    m.enableExpansion = sp->enable_expansion ? 1 : 0;
+    m.varyTemp = (sp->vary_temp && sp->temp_period > 0) ? 1 : 0;
+    m.tempAmplitude = sp->temp_amplitude;
+    m.tempPeriod = sp->temp_period;
+    m.tempBase = sp->temp_base;
+    m.palette.assign(sp->mats, sp->mats + sp->nmat);
+
+    // UpdateMatTemps (VX_Sim.cpp:467-483) via UpdateCurTemp (VX_Environment.cpp:327-346)
+    float t0 = 0.0f;
+    if (sp->temp_enabled) {
+        float cur;
+        if (sp->vary_temp) cur = (sp->temp_period == 0) ? 0.0f : (float)(sp->temp_base + sp->temp_amplitude * std::sin(0.0));
+        else cur = (float)(sp->temp_base + sp->temp_amplitude);
+        t0 = (float)(cur - sp->temp_base);
+    }
+    m.initialTemp = t0;
+
+    // palette -> derived voxel materials (CopyMat VX_Sim.cpp:368-420, Import :74-86)
+    const float gravMult = (float)(-(float)sp->grav_acc / 9.80665);  // SetGravityAccel VX_Sim.cpp:485
+    const float zI = sp->bond_damp_z <= 0 ? 0 : sp->bond_damp_z;
+    const float zG = sp->slow_damp_z <= 0 ? 0 : sp->slow_damp_z;
+    const float zC = sp->col_damp_z <= 0 ? 0 : sp->col_damp_z;
+    std::vector<MatDerived> pal(sp->nmat);
+    for (int k = 0; k < sp->nmat; k++) {
+        const vx3mi_material &in = sp->mats[k];
+        MatDerived &d = pal[k];
+        std::memset(&d, 0, sizeof(d));
+        d.nomSize = sp->lattice_dim;
+        d.E = in.elastic_mod;
+        bool withFail = in.mat_model == 1 && in.fail_stress > 0;
+        d.sigmaFail = withFail ? in.fail_stress : -1.0f;
+        d.epsFail = withFail ? in.fail_stress / in.elastic_mod : -1.0f;
+        d.nu = sp->volume_effects ? clamp_nu(in.poissons_ratio) : 0.0f;
+        d.rho = in.density <= 0 ? FLT_MIN : in.density;
+        d.alpha = in.cte;
+        d.muS = in.u_static <= 0 ? 0 : in.u_static;
+        d.muK = in.u_dynamic <= 0 ? 0 : in.u_dynamic;
+        d.zI = zI; d.zG = zG; d.zC = zC;
+        d.gravMult = gravMult;
+        d.fixed = in.fixed ? 1 : 0;
+        d.isMeasured = in.is_measured ? 1 : 0;
+        d.thermalOn = in.thermal_on_after;
+        derive(d);
+    }
+
+    // voxels in lattice order (x fastest)
+    std::vector<int> cellvox(ncell, -1);
+    std::vector<MatDerived> vm;  // derived material per voxel-material index
+    std::vector<int> palToVm(sp->nmat, -1);
+    int nv = 0;
+    for (long i = 0; i < ncell; i++) {
+        int c = sp->cells[i];
+        if (c <= 0 || c > sp->nmat) continue;
+        int z = (int)(i / ((long)nx * ny)), y = (int)((i - (long)z * nx * ny) / nx), x = (int)(i - (long)z * nx * ny - (long)y * nx);
+        cellvox[i] = nv;
+        m.ix.push_back(x); m.iy.push_back(y); m.iz.push_back(z);
+        int mi;
+        if (sp->voxel_E) {
+            MatDerived d = pal[c - 1];
+            d.E = sp->voxel_E[nv];
+            d.epsFail = (d.sigmaFail == -1.0f) ? -1.0f : d.sigmaFail / d.E;
+            derive(d);
+            mi = (int)vm.size();
+            vm.push_back(d);
+        } else {
+            if (palToVm[c - 1] < 0) { palToVm[c - 1] = (int)vm.size(); vm.push_back(pal[c - 1]); }
+            mi = palToVm[c - 1];
+        }
+        m.vmat.push_back(mi);
+        m.phase.push_back(sp->phase ? sp->phase[nv] : 0.0);
+        nv++;
+    }
+    m.vlinks.assign((size_t)nv * 6, -1);
+    // links: created when the later voxel of a pair is added (addVoxel's
+    // X_POS..Z_NEG loop only finds lower neighbours), pVNeg = lower coordinate
+    std::map<std::pair<int, int>, int> lmatIndex;
+    for (int v = 0; v < nv; v++) {
+        int x = m.ix[v], y = m.iy[v], z = m.iz[v];
+        int nb[3] = {x > 0 ? cellvox[(x - 1) + (long)nx * (y + (long)ny * z)] : -1,
+                     y > 0 ? cellvox[x + (long)nx * ((y - 1) + (long)ny * z)] : -1,
+                     z > 0 ? cellvox[x + (long)nx * (y + (long)ny * (z - 1))] : -1};
+        for (int a = 0; a < 3; a++) {
+            int n = nb[a];
+            if (n < 0) continue;
+            int li = m.nl();
+            m.lneg.push_back(n);
+            m.lpos.push_back(v);
+            m.laxis.push_back(a);
+            m.vlinks[(size_t)v * 6 + 2 * a + 1] = li;
+            m.vlinks[(size_t)n * 6 + 2 * a] = li;
+            std::pair<int, int> key(m.vmat[n], m.vmat[v]);
+            auto it = lmatIndex.find(key);
+            int lmi;
+            if (it == lmatIndex.end()) {
+                // combinedMaterial(voxel1 = new voxel, voxel2 = neighbour); the result is symmetric
+                LinkMat l = combine(vm[m.vmat[v]], vm[m.vmat[n]]);
+                const MatDerived &dn = vm[m.vmat[n]], &dp = vm[m.vmat[v]];
+                l.nomN = dn.nomSize; l.nomP = dp.nomSize;
+                l.alphaN = dn.alpha; l.alphaP = dp.alpha;
+                l.dmNumN = 2 * dn.sqrtMass * dn.zI;
+                l.dmNumP = 2 * dp.sqrtMass * dp.zI;
+                // transverse info frozen at creation (VX3_Link.cu:147-150; all strains zero)
+                float sN = (float)dn.nomSize, sP = (float)dp.nomSize;
+                l.tArea = 0.5f * (sN * sN + sP * sP);
+                l.tStrainSum = 0.0f;
+                l.fixedBoth = (dn.fixed && dp.fixed) ? 1 : 0;
+                lmi = (int)m.lmats.size();
+                m.lmats.push_back(l);
+                lmatIndex[key] = lmi;
+            } else {
+                lmi = it->second;
+            }
+            m.lmat.push_back(lmi);
+        }
+    }
+    for (const MatDerived &d : vm) {
+        m.vmats.push_back(to_voxel_mat(d));
+        m.vmatE.push_back(d.E);
+    }
+    for (int v = 0; v < nv; v++) {
+        int c = 0;
+        for (int d = 0; d < 6; d++) c += m.vlinks[(size_t)v * 6 + d] >= 0;
+        if (c < 6) m.surface.push_back(v);
+    }
+    return VX3MI_OK;
+}
+
+}  // namespace vx3mi
+
+using namespace vx3mi;
+
+extern "C" {
+
+const char *vx3mi_last_error(void) { return g_error.c_str(); }
+const char *vx3mi_version(void) { return "vx3mi 0.1 (gfx950)"; }
+
+int vx3mi_model_from_lattice(const vx3mi_lattice *spec, vx3mi_model **out) {
+    if (!out) { set_error("out is NULL"); return VX3MI_E_ARG; }
+    *out = nullptr;
+    vx3mi_model *h = new (std::nothrow) vx3mi_model();
+    if (!h) { set_error("out of host memory"); return VX3MI_E_NOMEM; }
+    try {
+        int rc = build_model(spec, h->m);
+        if (rc != VX3MI_OK) { delete h; return rc; }
+    } catch (const std::bad_alloc &) {
+        delete h;
+        set_error("out of host memory");
+        return VX3MI_E_NOMEM;
+    }
+    *out = h;
+    return VX3MI_OK;
+}
+
+void vx3mi_model_free(vx3mi_model *m) { delete m; }
+
+int vx3mi_model_counts(const vx3mi_model *m, int *nvox, int *nlink, int *nsurf) {
+    if (!m) { set_error("model is NULL"); return VX3MI_E_ARG; }
+    if (nvox) *nvox = m->m.nv();
+    if (nlink) *nlink = m->m.nl();
+    if (nsurf) *nsurf = (int)m->m.surface.size();
+    return VX3MI_OK;
+}
+
+int vx3mi_model_recommended_dt(const vx3mi_model *m, double *out) {
+    if (!m || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    *out = recommended_dt(m->m);
+    return VX3MI_OK;
+}
+
+int vx3mi_model_topology(const vx3mi_model *m, int *lattice3, int *link_neg, int *link_pos, int *link_axis) {
+    if (!m) { set_error("model is NULL"); return VX3MI_E_ARG; }
+    const Model &M = m->m;
+    for (int v = 0; v < M.nv(); v++)
+        if (lattice3) { lattice3[3 * v] = M.ix[v]; lattice3[3 * v + 1] = M.iy[v]; lattice3[3 * v + 2] = M.iz[v]; }
+    for (int l = 0; l < M.nl(); l++) {
+        if (link_neg) link_neg[l] = M.lneg[l];
+        if (link_pos) link_pos[l] = M.lpos[l];
+        if (link_axis) link_axis[l] = M.laxis[l];
+    }
+    return VX3MI_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,501 @@
+// vx3mi_physics.h -- per-link and per-voxel update equations of the VX3 timestep,
+// templated on the state precision T (float or double).
+//
+// The equations are those of the reference (file:line given per function).  The
+// double instantiation keeps the reference's operand order and float/double
+// promotions, so it reproduces an FP64 run of the reference to rounding.  The
+// float instantiation evaluates the same mathematical functions in forms that
+// do not cancel catastrophically in single precision:
+//   * positions are stored as displacements u from the nominal lattice point
+//     (ix*L, iy*L, iz*L), so link vectors are L*e_axis + (u_pos - u_neg)
+//     (the double instantiation stores absolute positions, as the reference
+//     does, because the reference's own rounding of ix*L - (ix-1)*L is part
+//     of what an FP64 run of it computes);
+//   * the quaternion -> rotation vector map uses |v|^2 and atan2 instead of
+//     1-w^2 and acos(w); the reference's small-angle approximation
+//     2*sqrt((2-2w)/(1-w^2)) is evaluated as the identical 2*sqrt(2/(1+w));
+//   * the align-to-+x rotation takes its angle from atan2, not acos.
+#pragma once
+
+#include <math.h>
+#include <stdint.h>
+#include <type_traits>
+
+#ifdef __HIPCC__
+#include <hip/hip_runtime.h>
+#define VX_HD __host__ __device__ __forceinline__
+#else
+#define VX_HD inline
+#endif
+
+namespace vx3mi {
+
+// the position field of the state holds absolute positions (double) or
+// displacements from the nominal lattice point (float)
+template <typename T> constexpr bool kAbsolutePositions = std::is_same<T, double>::value;
+
+template <typename T> struct V3 { T x, y, z; };
+template <typename T> struct Q4 { T w, x, y, z; };
+
+VX_HD float msqrt(float a) { return sqrtf(a); }
+VX_HD double msqrt(double a) { return sqrt(a); }
+VX_HD float msin(float a) { return sinf(a); }
+VX_HD double msin(double a) { return sin(a); }
+VX_HD float mcos(float a) { return cosf(a); }
+VX_HD double mcos(double a) { return cos(a); }
+VX_HD float macos(float a) { return acosf(a); }
+VX_HD double macos(double a) { return acos(a); }
+VX_HD float matan2(float a, float b) { return atan2f(a, b); }
+VX_HD double matan2(double a, double b) { return atan2(a, b); }
+VX_HD float mabs(float a) { return fabsf(a); }
+VX_HD double mabs(double a) { return fabs(a); }
+
+template <typename T> VX_HD V3<T> v3(T x, T y, T z) { V3<T> r = {x, y, z}; return r; }
+template <typename T> VX_HD V3<T> operator+(V3<T> a, V3<T> b) { return v3<T>(a.x + b.x, a.y + b.y, a.z + b.z); }
+template <typename T> VX_HD V3<T> operator-(V3<T> a, V3<T> b) { return v3<T>(a.x - b.x, a.y - b.y, a.z - b.z); }
+template <typename T> VX_HD V3<T> operator*(V3<T> a, T s) { return v3<T>(a.x * s, a.y * s, a.z * s); }
+template <typename T> VX_HD T len(V3<T> a) { return msqrt(a.x * a.x + a.y * a.y + a.z * a.z); }
+
+// ---- material constants (float, as the reference stores them) ------------
+// Voxel material: VX3_MaterialVoxel.h:29-44, VX3_Voxel.h:185-208.
+struct VoxelMat {
+    double nomSize;        // nominalSize()
+    double thermalOnAfter; // TurnOnThermalExpansionAfterThisManySeconds
+    float alphaCTE;
+    float mass, massInv, inertiaInv;
+    float dmNum;    // 2*_sqrtMass*zetaInternal   (dampingMultiplier numerator)
+    float gDampT;   // globalDampingTranslateC
+    float gDampR;   // globalDampingRotateC
+    float cDampT;   // collisionDampingTranslateC
+    float gravF;    // gravityForce()
+    float penK;     // penetrationStiffness()
+    float muS, muK;
+    int fixed, isMeasured;
+};
+
+// Link material (VX3_MaterialLink.h:27-36) plus per-end constants the link
+// kernel needs so it never touches the voxel material table.
+struct LinkMat {
+    double nomSize;
+    double nomN, nomP;      // endpoint voxel nominal sizes (rest length)
+    float E, nu, eHat, epsFail;
+    float a1, a2, b1, b2, b3;
+    float sqA1, sqA2xIp, sqB1, sqB2xFMp, sqB3xIp;
+    float tArea, tStrainSum; // frozen transverse info (VX3_Link.cu:147-150)
+    float alphaN, alphaP;    // endpoint CTEs
+    float dmNumN, dmNumP;    // endpoint damping multiplier numerators
+    int fixedBoth, pad;
+};
+
+// ---- constants --------------------------------------------------------------
+// src/Utils/VX3_Quat3D.h:30-56, src/old/types.h:57-59, VX3_Collision.h:6
+constexpr double kPI = 3.14159265358979;
+constexpr double kDBL_EPSILONx24 = 5.328e-15;
+constexpr double kDISCARD_ANGLE_RAD = 1e-7;
+constexpr double kSMALL_ANGLE_RAD = 1.732e-2;
+constexpr double kSLTHRESH_ACOS2SQRT = 2.4e-3;
+constexpr float kHYSTERESIS_FACTOR = 1.2f;
+constexpr float kSA_BOND_BEND_RAD = 0.05f;
+constexpr float kSA_BOND_EXT_PERC = 0.50f;
+constexpr double kCOLLISION_ENVELOPE_RADIUS = 0.625;
+
+// voxel flag bits (src/old/types.h:34-38); link flag bits (types.h:6-8 + smallAngle)
+constexpr int VF_FLOOR_ENABLED = 1 << 2;
+constexpr int VF_FLOOR_STATIC_FRICTION = 1 << 3;
+constexpr int LF_LOCAL_VELOCITY_VALID = 1;
+constexpr int LF_SMALL_ANGLE = 2;
+
+// ---- quaternion algebra (VX3_Quat3D.h:217-222, 439-447, 465-474) -----------
+template <typename T> VX_HD Q4<T> qmul(Q4<T> a, Q4<T> f) {
+    Q4<T> r;
+    r.w = a.w * f.w - a.x * f.x - a.y * f.y - a.z * f.z;
+    r.x = a.w * f.x + a.x * f.w + a.y * f.z - a.z * f.y;
+    r.y = a.w * f.y - a.x * f.z + a.y * f.w + a.z * f.x;
+    r.z = a.w * f.z + a.x * f.y - a.y * f.x + a.z * f.w;
+    return r;
+}
+template <typename T> VX_HD Q4<T> qconj(Q4<T> a) { Q4<T> r = {a.w, -a.x, -a.y, -a.z}; return r; }
+template <typename T> VX_HD V3<T> qrot(Q4<T> q, V3<T> f) {
+    T tw = f.x * q.x + f.y * q.y + f.z * q.z;
+    T tx = f.x * q.w - f.y * q.z + f.z * q.y;
+    T ty = f.x * q.z + f.y * q.w - f.z * q.x;
+    T tz = -f.x * q.y + f.y * q.x + f.z * q.w;
+    return v3<T>(q.w * tx + q.x * tw + q.y * tz - q.z * ty, q.w * ty - q.x * tz + q.y * tw + q.z * tx,
+                 q.w * tz + q.x * ty - q.y * tx + q.z * tw);
+}
+template <typename T> VX_HD V3<T> qrotinv(Q4<T> q, V3<T> f) {
+    T tw = q.x * f.x + q.y * f.y + q.z * f.z;
+    T tx = q.w * f.x - q.y * f.z + q.z * f.y;
+    T ty = q.w * f.y + q.x * f.z - q.z * f.x;
+    T tz = q.w * f.z - q.x * f.y + q.y * f.x;
+    return v3<T>(tw * q.x + tx * q.w + ty * q.z - tz * q.y, tw * q.y - tx * q.z + ty * q.w + tz * q.x,
+                 tw * q.z + tx * q.y - ty * q.x + tz * q.w);
+}
+
+// ToRotationVector, VX3_Quat3D.h:347-359
+template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q) {
+    if constexpr (std::is_same<T, double>::value) {
+        if (q.w >= 1.0 || q.w <= -1.0) return v3<T>(0, 0, 0);
+        double sl = 1.0 - q.w * q.w;
+        if (sl < kSLTHRESH_ACOS2SQRT) {
+            double s = sqrt((2 - 2 * q.w) / sl);
+            return v3<T>(q.x * 2.0 * s, q.y * 2.0 * s, q.z * 2.0 * s);
+        }
+        double a = acos(q.w), d = sqrt(sl);
+        return v3<T>(q.x * 2.0 * a / d, q.y * 2.0 * a / d, q.z * 2.0 * a / d);
+    } else {
+        float s2 = q.x * q.x + q.y * q.y + q.z * q.z;  // == 1 - w^2 for a unit quaternion
+        if (s2 == 0.0f || q.w <= -1.0f) return v3<T>(0, 0, 0);
+        float k;
+        if (s2 < (float)kSLTHRESH_ACOS2SQRT) k = 2.0f * sqrtf(2.0f / (1.0f + q.w));
+        else {
+            float s = sqrtf(s2);
+            k = 2.0f * atan2f(s, q.w) / s;
+        }
+        return v3<T>(q.x * k, q.y * k, q.z * k);
+    }
+}
+
+// FromRotationVector, VX3_Quat3D.h:363-378
+template <typename T> VX_HD Q4<T> from_rotvec(V3<T> in) {
+    V3<T> th = v3<T>(in.x / 2, in.y / 2, in.z / 2);
+    T s, m2 = th.x * th.x + th.y * th.y + th.z * th.z;
+    Q4<T> r;
+    if (m2 * m2 < (T)kDBL_EPSILONx24) {
+        r.w = (T)1.0 - (T)0.5 * m2;
+        s = (T)1.0 - m2 / (T)6.0;
+    } else {
+        T m = msqrt(m2);
+        r.w = mcos(m);
+        s = msin(m) / m;
+    }
+    r.x = th.x * s;
+    r.y = th.y * s;
+    r.z = th.z * s;
+    return r;
+}
+
+// FromAngleToPosX, VX3_Quat3D.h:384-432 (the target starts as the identity,
+// VX3_Link.cu:103)
+template <typename T> VX_HD Q4<T> from_angle_to_posx(V3<T> rf) {
+    Q4<T> q = {1, 0, 0, 0};
+    if (rf.x == 0 && rf.y == 0 && rf.z == 0) return q;
+    T yx = rf.y / rf.x, zx = rf.z / rf.x;
+    const T sa = (T)kSMALL_ANGLE_RAD;
+    if (yx < sa && yx > -sa && zx < sa && zx > -sa) {
+        q.x = 0;
+        q.y = (T)0.5 * zx;
+        q.z = (T)-0.5 * yx;
+        q.w = (T)1 + (T)0.5 * (-q.y * q.y - q.z * q.z);
+        return q;
+    }
+    V3<T> n = rf;
+    T l = msqrt(n.x * n.x + n.y * n.y + n.z * n.z);
+    if (l > 0) {
+        T li = (T)1.0 / l;
+        n.x *= li;
+        n.y *= li;
+        n.z *= li;
+    }
+    T theta;
+    if constexpr (std::is_same<T, double>::value) theta = acos(n.x);
+    else theta = atan2f(sqrtf(n.y * n.y + n.z * n.z), n.x);
+    if (theta > (T)(kPI - kDISCARD_ANGLE_RAD)) {
+        q.w = 0; q.x = 0; q.y = 1; q.z = 0;
+        return q;
+    }
+    T ami = (T)1.0 / msqrt(n.z * n.z + n.y * n.y);
+    T a = (T)0.5 * theta, s = msin(a);
+    q.w = mcos(a);
+    q.x = 0;
+    q.y = n.z * ami * s;
+    q.z = -n.y * ami * s;
+    return q;
+}
+
+// toAxisX / toAxisOriginal, VX3_Link.h:134-191
+template <typename T> VX_HD V3<T> to_axis_x(V3<T> v, int axis) {
+    if (axis == 1) return v3<T>(v.y, -v.x, v.z);
+    if (axis == 2) return v3<T>(v.z, v.y, -v.x);
+    return v;
+}
+template <typename T> VX_HD Q4<T> to_axis_x(Q4<T> q, int axis) {
+    Q4<T> r = q;
+    if (axis == 1) { r.x = q.y; r.y = -q.x; }
+    if (axis == 2) { r.x = q.z; r.z = -q.x; }
+    return r;
+}
+template <typename T> VX_HD V3<T> to_axis_original(V3<T> v, int axis) {
+    if (axis == 1) return v3<T>(-v.y, v.x, v.z);
+    if (axis == 2) return v3<T>(-v.z, v.y, v.x);
+    return v;
+}
+
+// ---- temperature (gpu_update_temperature VX3_VoxelyzeKernel.cu:625-650) -----
+// Returns the new temperature of a voxel at time t (the caller skips fixed
+// voxels and voxels whose TurnOnThermalExpansion time has not come).
+template <typename T> VX_HD float actuation_temperature(double amp, double period, double t, double phase, int enableExpansion) {
+    if constexpr (std::is_same<T, double>::value) {
+        double Tc = amp * sin((double)(2 * 3.1415926f) * (t / period + phase));
+        if (!enableExpansion && Tc > 0) Tc = 0;
+        return (float)Tc;
+    } else {
+        // the cycle count is reduced to [0,1) in double before the float sine
+        double c = t / period + phase;
+        c -= floor(c);
+        float Tc = (float)amp * sinf((2 * 3.1415926f) * (float)c);
+        if (!enableExpansion && Tc > 0) Tc = 0;
+        return Tc;
+    }
+}
+
+// baseSize(axis) thermal factor, VX3_Voxel.h:96-100: (1 + tempe*alphaCTE) is float
+VX_HD float thermal_factor(float temp, float alpha) { return 1 + temp * alpha; }
+
+// rest length, VX3_Link.cu:80-83
+template <typename T> VX_HD T rest_length(const LinkMat &lm, float tN, float tP) {
+    float fN = thermal_factor(tN, lm.alphaN), fP = thermal_factor(tP, lm.alphaP);
+    if constexpr (std::is_same<T, double>::value) return 0.5 * (lm.nomN * (double)fN + lm.nomP * (double)fP);
+    else return 0.5f * ((float)lm.nomN * fN + (float)lm.nomP * fP);
+}
+
+// ---- link state -----------------------------------------------------------------
+template <typename T> struct LinkState {
+    V3<T> pos2, a1v, a2v;
+    float maxStrain;
+    int flags;
+};
+
+template <typename T> struct LinkOut {
+    V3<T> fN, fP, mN, mP;  // forceNeg, forcePos, momentNeg, momentPos (voxel frames)
+    float strain;
+};
+
+// VX3_Link::updateForces VX3_Link.cu:135-218 with orientLink :90-133,
+// updateStrain :220-260, VX3_Material::stress VX3_Material.cu:90-124.
+// du = position field of pVPos minus that of pVNeg: the global link vector in
+// double, the displacement difference (nominal offset nomSize along the link
+// axis) in float.  tN/tP are the end temperatures (rest length,
+// VX3_Link.cu:80-83).
+template <typename T>
+VX_HD void link_forces(const LinkMat &lm, int axis, V3<T> du, Q4<T> qn, Q4<T> qp, float tN, float tP, float prevDt,
+                       LinkState<T> &st, LinkOut<T> &out) {
+    V3<T> oldPos2 = st.pos2, oldA1v = st.a1v, oldA2v = st.a2v;
+    int flags = st.flags;
+
+    Q4<T> angle1 = to_axis_x(qn, axis);
+    Q4<T> angle2 = to_axis_x(qp, axis);
+    Q4<T> totalRot = qconj(angle1);
+    angle2 = qmul(totalRot, angle2);
+    angle1.w = 1; angle1.x = angle1.y = angle1.z = 0;
+
+    // pos2 (the pVPos position in the pVNeg frame, VX3_Link.cu:92-101) and
+    // ext = pos2.x - restLength
+    V3<T> pos2;
+    T restLen, ext;
+    if constexpr (std::is_same<T, double>::value) {
+        restLen = rest_length<T>(lm, tN, tP);
+        pos2 = qrot(totalRot, to_axis_x(du, axis));
+        ext = pos2.x - restLen;
+    } else {
+        // R(L e_x + d) = R d + L (R e_x - e_x) + L e_x with R e_x - e_x written
+        // for a unit quaternion, so the small extension never cancels
+        const float L = (float)lm.nomSize;
+        const float fN = thermal_factor(tN, lm.alphaN), fP = thermal_factor(tP, lm.alphaP);
+        restLen = 0.5f * ((float)lm.nomN * fN + (float)lm.nomP * fP);
+        const float lMinusRest = -0.5f * ((float)lm.nomN * (fN - 1.0f) + (float)lm.nomP * (fP - 1.0f));
+        const Q4<T> R = totalRot;
+        V3<T> rd = qrot(R, to_axis_x(du, axis));
+        V3<T> rel = v3<T>(rd.x - 2.0f * L * (R.y * R.y + R.z * R.z), rd.y + 2.0f * L * (R.x * R.y + R.w * R.z),
+                          rd.z + 2.0f * L * (R.x * R.z - R.w * R.y));
+        pos2 = v3<T>(L + rel.x, rel.y, rel.z);
+        ext = rel.x + lMinusRest;
+    }
+
+    float smallTurn = (float)((mabs(pos2.z) + mabs(pos2.y)) / pos2.x);
+    float extendPerc;
+    if constexpr (std::is_same<T, double>::value) extendPerc = (float)(mabs((T)1 - pos2.x / restLen));
+    else extendPerc = mabs(ext / restLen);
+    bool smallAngle = (flags & LF_SMALL_ANGLE) != 0;
+    if (!smallAngle && smallTurn < kSA_BOND_BEND_RAD && extendPerc < kSA_BOND_EXT_PERC) {
+        smallAngle = true;
+        flags &= ~LF_LOCAL_VELOCITY_VALID;
+    } else if (smallAngle && (smallTurn > kHYSTERESIS_FACTOR * kSA_BOND_BEND_RAD ||
+                              extendPerc > kHYSTERESIS_FACTOR * kSA_BOND_EXT_PERC)) {
+        smallAngle = false;
+        flags &= ~LF_LOCAL_VELOCITY_VALID;
+    }
+    if (smallAngle) {
+        if constexpr (std::is_same<T, double>::value) pos2.x -= restLen;
+        else pos2.x = ext;
+    } else {
+        angle1 = from_angle_to_posx(pos2);
+        angle2 = qmul(angle1, angle2);
+        if constexpr (std::is_same<T, double>::value) {
+            pos2 = v3<T>(len(pos2) - restLen, 0, 0);
+        } else {
+            // |pos2| - restLen = (|pos2|^2 - restLen^2) / (|pos2| + restLen)
+            T l = len(pos2);
+            T num = ext * (pos2.x + restLen) + pos2.y * pos2.y + pos2.z * pos2.z;
+            pos2 = v3<T>(num / (l + restLen), 0, 0);
+        }
+    }
+    V3<T> a1v = to_rotvec(angle1);
+    V3<T> a2v = to_rotvec(angle2);
+    flags = smallAngle ? (flags | LF_SMALL_ANGLE) : (flags & ~LF_SMALL_ANGLE);
+    st.pos2 = pos2;
+    st.a1v = a1v;
+    st.a2v = a2v;
+
+    V3<T> dPos2 = (pos2 - oldPos2) * (T)0.5;
+    V3<T> dA1 = (a1v - oldA1v) * (T)0.5;
+    V3<T> dA2 = (a2v - oldA2v) * (T)0.5;
+
+    float axialStrain = (float)(pos2.x / restLen);
+    out.strain = axialStrain;
+    if (axialStrain > st.maxStrain) st.maxStrain = axialStrain;
+    float stress;
+    if (lm.epsFail != -1.0f && axialStrain > lm.epsFail) stress = 0.0f;
+    else if (lm.nu == 0.0f) stress = lm.E * axialStrain;
+    else stress = lm.eHat * ((1 - lm.nu) * axialStrain + lm.nu * lm.tStrainSum);
+
+    if (lm.epsFail != -1.0f && st.maxStrain > lm.epsFail) {
+        out.fN = out.fP = out.mN = out.mP = v3<T>(0, 0, 0);
+        st.flags = flags;
+        return;
+    }
+    const float b1 = lm.b1, b2 = lm.b2, b3 = lm.b3, a2 = lm.a2;
+    V3<T> fN = v3<T>((T)(stress * lm.tArea), b1 * pos2.y - b2 * (a1v.z + a2v.z), b1 * pos2.z + b2 * (a1v.y + a2v.y));
+    V3<T> fP = v3<T>(-fN.x, -fN.y, -fN.z);
+    V3<T> mN = v3<T>(a2 * (a2v.x - a1v.x), -b2 * pos2.z - b3 * (2 * a1v.y + a2v.y), b2 * pos2.y - b3 * (2 * a1v.z + a2v.z));
+    V3<T> mP = v3<T>(a2 * (a1v.x - a2v.x), -b2 * pos2.z - b3 * (a1v.y + 2 * a2v.y), b2 * pos2.y - b3 * (a1v.z + 2 * a2v.z));
+
+    if (flags & LF_LOCAL_VELOCITY_VALID) {
+        const float sqA1 = lm.sqA1, sqA2xIp = lm.sqA2xIp, sqB1 = lm.sqB1, sqB2xFMp = lm.sqB2xFMp, sqB3xIp = lm.sqB3xIp;
+        V3<T> posCalc = v3<T>(sqA1 * dPos2.x, sqB1 * dPos2.y - sqB2xFMp * (dA1.z + dA2.z),
+                              sqB1 * dPos2.z + sqB2xFMp * (dA1.y + dA2.y));
+        float dmN = lm.dmNumN / prevDt, dmP = lm.dmNumP / prevDt;  // VX3_Voxel.h:206-208
+        fN = fN + posCalc * (T)dmN;
+        fP = fP - posCalc * (T)dmP;
+        T hN = (T)0.5 * dmN, hP = (T)0.5 * dmP;
+        V3<T> tN = v3<T>(-sqA2xIp * (dA2.x - dA1.x), sqB2xFMp * dPos2.z + sqB3xIp * (2 * dA1.y + dA2.y),
+                         -sqB2xFMp * dPos2.y + sqB3xIp * (2 * dA1.z + dA2.z));
+        V3<T> tP = v3<T>(sqA2xIp * (dA2.x - dA1.x), sqB2xFMp * dPos2.z + sqB3xIp * (dA1.y + 2 * dA2.y),
+                         -sqB2xFMp * dPos2.y + sqB3xIp * (dA1.z + 2 * dA2.z));
+        mN = mN - tN * hN;
+        mP = mP - tP * hP;
+    } else {
+        flags |= LF_LOCAL_VELOCITY_VALID;
+    }
+    if (!smallAngle) {
+        fN = qrotinv(angle1, fN);
+        mN = qrotinv(angle1, mN);
+    }
+    fP = qrotinv(angle2, fP);
+    mP = qrotinv(angle2, mP);
+    out.fN = to_axis_original(fN, axis);
+    out.fP = to_axis_original(fP, axis);
+    out.mN = to_axis_original(mN, axis);
+    out.mP = to_axis_original(mP, axis);
+    st.flags = flags;
+}
+
+// ---- voxel integrator -----------------------------------------------------------
+// Floor penetration, VX3_Voxel.h:185-188: (float)(baseSizeAverage()/2 -
+// nominalSize()/2 - pos.z).  uz is the z position field (absolute in double).
+template <typename T> VX_HD float floor_penetration(const VoxelMat &vm, float temp, int iz, T uz) {
+    float f = thermal_factor(temp, vm.alphaCTE);
+    if constexpr (std::is_same<T, double>::value) {
+        (void)iz;
+        double b = vm.nomSize * (double)f;
+        double bsa = (b + b + b) / 3.0f;
+        return (float)(bsa / 2 - vm.nomSize / 2 - uz);
+    } else {
+        float L = (float)vm.nomSize;
+        return (L * (f - 1.0f) * 0.5f - (float)iz * L) - uz;
+    }
+}
+
+template <typename T> struct VoxelState {
+    V3<T> u;     // position field (see kAbsolutePositions)
+    Q4<T> q;     // orientation
+    V3<T> p;     // linear momentum
+    V3<T> h;     // angular momentum
+    int flags;
+};
+
+// VX3_Voxel::timeStep VX3_Voxel.cu:162-277 with force() :350-380, moment()
+// :382-397, floorForce :399-426.  fLink / mLink are the link force / moment
+// sums in the voxel frame (summed in link direction order X_POS..Z_NEG),
+// contact is the collision force to subtract (VX3_Voxel.cu:372-373).
+template <typename T>
+VX_HD void voxel_step(const VoxelMat &vm, float dtf, int iz, float temp, V3<T> fLink, V3<T> mLink, V3<T> contact,
+                      VoxelState<T> &s) {
+    if (dtf == 0.0f) return;
+    const T dt = (T)dtf;
+    int flags = s.flags;
+
+    V3<T> tf = qrot(s.q, fLink);
+    V3<T> vel = s.p * (T)vm.massInv;
+    tf = tf - vel * (T)vm.gDampT;
+    tf.z += vm.gravF;
+    tf = tf - contact;
+    tf.x += (T)0.0; tf.y += (T)0.0; tf.z += (T)0.0;  // constant-zero force field (VX3_ForceField.h:11-20)
+
+    V3<T> cur = tf;
+    if (flags & VF_FLOOR_ENABLED) {
+        float pen = floor_penetration<T>(vm, temp, iz, s.u.z);
+        if (pen >= 0) {
+            float normalForce = vm.penK * pen;
+            cur.z += normalForce - vm.cDampT * vel.z;
+            if (flags & VF_FLOOR_STATIC_FRICTION) {
+                float sfs = (float)(cur.x * cur.x + cur.y * cur.y);
+                float ffs = (vm.muS * normalForce) * (vm.muS * normalForce);
+                if (sfs > ffs) flags &= ~VF_FLOOR_STATIC_FRICTION;
+            } else {
+                V3<T> hv = v3<T>(vel.x, vel.y, 0);
+                T l = len(hv);
+                V3<T> hn = l > 0 ? v3<T>(hv.x / l, hv.y / l, hv.z / l) : hv;
+                cur = cur - hn * (T)(vm.muK * normalForce);
+            }
+        } else {
+            flags &= ~VF_FLOOR_STATIC_FRICTION;
+        }
+    }
+    V3<T> fric = cur - tf;
+    V3<T> p = s.p + cur * dt;
+    V3<T> tr = p * (dt * (T)vm.massInv);
+    if ((flags & VF_FLOOR_ENABLED) && floor_penetration<T>(vm, temp, iz, s.u.z) >= 0) {
+        T work = fric.x * tr.x + fric.y * tr.y;
+        T hKe = (T)0.5 * (T)vm.massInv * (p.x * p.x + p.y * p.y);
+        if (hKe + work <= 0) flags |= VF_FLOOR_STATIC_FRICTION;
+        if (flags & VF_FLOOR_STATIC_FRICTION) {
+            p.x = p.y = 0;
+            tr.x = tr.y = 0;
+        }
+    } else {
+        flags &= ~VF_FLOOR_STATIC_FRICTION;
+    }
+    s.u = s.u + tr;
+    s.p = p;
+
+    V3<T> tm = qrot(s.q, mLink);
+    tm = tm - (s.h * (T)vm.inertiaInv) * (T)vm.gDampR;
+    V3<T> h = s.h + tm * dt;
+    s.q = qmul(from_rotvec(h * (dt * (T)vm.inertiaInv)), s.q);
+    if constexpr (std::is_same<T, float>::value) {
+        // single precision only: keep the orientation a unit quaternion (the
+        // FP64 reference never renormalises, VX3_Voxel.cu:221, and drifts by
+        // ~1e-16 per step; float would drift by ~1e-7 per step)
+        float n = 1.0f / sqrtf(s.q.w * s.q.w + s.q.x * s.q.x + s.q.y * s.q.y + s.q.z * s.q.z);
+        s.q.w *= n; s.q.x *= n; s.q.y *= n; s.q.z *= n;
+    }
+    // VX3-only: no rotation while held by static floor friction (VX3_Voxel.cu:259-264)
+    if ((flags & VF_FLOOR_ENABLED) && floor_penetration<T>(vm, temp, iz, s.u.z) >= 0) {
+        if (flags & VF_FLOOR_STATIC_FRICTION) h = v3<T>(0, 0, 0);
+    }
+    s.h = h;
+    s.flags = flags;
+}
+
+}  // namespace vx3mi

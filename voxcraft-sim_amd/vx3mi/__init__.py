@@ -1,0 +1,273 @@
+"""Python binding of the MI355X-native VX3 timestep (libvx3mi.so, include/vx3mi.h).
+
+The binding is a thin ctypes layer over the C ABI.  It never falls back to a
+CPU implementation: if the HIP library is missing, ``lib()`` raises.
+
+A *problem* is a plain dict describing one robot the way a merged VXA file
+does (lattice dims, palette, per-voxel phase, simulator and environment
+settings); see ``PROBLEM_DEFAULTS`` and ``MATERIAL_DEFAULTS``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Iterable, Sequence
+
+import numpy as np
+
+__all__ = ["lib", "Model", "Batch", "VX3MIError", "FP32", "FP64", "MODE_AUTO", "MODE_PLAIN", "MODE_FLAT",
+           "MODE_PERSISTENT", "PROBLEM_DEFAULTS", "MATERIAL_DEFAULTS", "library_path"]
+
+_PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+_LIB_PATH = os.path.join(_PKG, "lib", "libvx3mi.so")
+
+FP32, FP64 = 32, 64
+MODE_AUTO, MODE_PLAIN, MODE_FLAT, MODE_PERSISTENT = 0, 1, 2, 3
+OK, DIVERGED = 0, 1
+
+PROBLEM_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=1.0, slow_damp_z=1.0,
+                        volume_effects=0, enable_collision=1, enable_expansion=0, grav_acc=-9.81,
+                        floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
+                        temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None)
+MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
+                         fixed=0, is_measured=1, is_target=0, thermal_on_after=0.0,
+                         rgba=(0.5, 0.5, 0.5, 1.0))
+
+
+class VX3MIError(RuntimeError):
+    pass
+
+
+class _Material(C.Structure):
+    _fields_ = [("elastic_mod", C.c_float), ("density", C.c_float), ("poissons_ratio", C.c_float),
+                ("cte", C.c_float), ("u_static", C.c_float), ("u_dynamic", C.c_float),
+                ("fail_stress", C.c_float), ("mat_model", C.c_int), ("fixed", C.c_int),
+                ("is_measured", C.c_int), ("is_target", C.c_int), ("thermal_on_after", C.c_double),
+                ("red", C.c_float), ("green", C.c_float), ("blue", C.c_float), ("alpha", C.c_float)]
+
+
+class _Lattice(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("cells", C.POINTER(C.c_int8)),
+                ("phase", C.POINTER(C.c_double)), ("voxel_E", C.POINTER(C.c_float)), ("nmat", C.c_int),
+                ("mats", C.POINTER(_Material)), ("lattice_dim", C.c_double), ("dt_frac", C.c_double),
+                ("bond_damp_z", C.c_float), ("col_damp_z", C.c_float), ("slow_damp_z", C.c_float),
+                ("volume_effects", C.c_int), ("enable_collision", C.c_int), ("enable_expansion", C.c_int),
+                ("grav_acc", C.c_float), ("floor_enabled", C.c_int), ("temp_enabled", C.c_int),
+                ("vary_temp", C.c_int), ("temp_base", C.c_double), ("temp_amplitude", C.c_double),
+                ("temp_period", C.c_double)]
+
+
+def library_path() -> str:
+    return _LIB_PATH
+
+
+_lib = None
+
+
+def lib():
+    """Load libvx3mi.so (built in-tree by ``make -C voxcraft-sim_amd``)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_LIB_PATH):
+        raise VX3MIError(f"HIP library not built: {_LIB_PATH} (run make -C voxcraft-sim_amd)")
+    L = C.CDLL(_LIB_PATH)
+    vp, ip, dp, fp = C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_float)
+    pvp = C.POINTER(C.c_void_p)
+    sigs = {
+        "vx3mi_last_error": (C.c_char_p, []),
+        "vx3mi_version": (C.c_char_p, []),
+        "vx3mi_model_from_vxa": (C.c_int, [C.c_char_p, C.c_char_p, pvp]),
+        "vx3mi_model_from_xml": (C.c_int, [C.c_char_p, C.c_char_p, pvp]),
+        "vx3mi_model_from_lattice": (C.c_int, [C.POINTER(_Lattice), pvp]),
+        "vx3mi_model_free": (None, [vp]),
+        "vx3mi_model_counts": (C.c_int, [vp, ip, ip, ip]),
+        "vx3mi_model_recommended_dt": (C.c_int, [vp, dp]),
+        "vx3mi_model_topology": (C.c_int, [vp, ip, ip, ip, ip]),
+        "vx3mi_batch_create": (C.c_int, [pvp, C.c_int, C.c_int, pvp]),
+        "vx3mi_batch_destroy": (None, [vp]),
+        "vx3mi_batch_counts": (C.c_int, [vp, ip, ip, ip]),
+        "vx3mi_batch_recommended_dt": (C.c_int, [vp, dp]),
+        "vx3mi_step": (C.c_int, [vp, C.c_int, C.c_float, C.c_int]),
+        "vx3mi_batch_sync": (C.c_int, [vp]),
+        "vx3mi_get_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
+        "vx3mi_set_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
+        "vx3mi_get_voxel_flags": (C.c_int, [vp, C.c_int, ip]),
+        "vx3mi_batch_time": (C.c_int, [vp, dp]),
+        "vx3mi_batch_diverged": (C.c_int, [vp, ip]),
+        "vx3mi_center_of_mass": (C.c_int, [vp, dp]),
+        "vx3mi_eval_link_forces": (C.c_int, [vp, dp]),
+        "vx3mi_last_step_ms": (C.c_int, [vp, fp]),
+        "vx3mi_last_step_launches": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(rc: int, allow=(OK,)) -> int:
+    if rc not in allow:
+        msg = lib().vx3mi_last_error().decode(errors="replace")
+        raise VX3MIError(f"vx3mi error {rc}: {msg}")
+    return rc
+
+
+def _ptr(a, ct):
+    return None if a is None else a.ctypes.data_as(C.POINTER(ct))
+
+
+class Model:
+    """Host model of one robot (topology, materials, initial state)."""
+
+    def __init__(self, handle: int, keep=()):
+        self._h = C.c_void_p(handle)
+        self._keep = keep
+        nv, nl, ns = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().vx3mi_model_counts(self._h, C.byref(nv), C.byref(nl), C.byref(ns)))
+        self.nv, self.nl, self.ns = nv.value, nl.value, ns.value
+
+    @classmethod
+    def from_problem(cls, problem: dict) -> "Model":
+        p = dict(PROBLEM_DEFAULTS)
+        p.update(problem)
+        nx, ny, nz = (int(d) for d in p["dims"])
+        cells = np.ascontiguousarray(p["cells"], dtype=np.int8).reshape(-1)
+        if cells.size != nx * ny * nz:
+            raise VX3MIError("cells size does not match dims")
+        mats = [dict(MATERIAL_DEFAULTS, **m) for m in p["materials"]]
+        marr = (_Material * max(1, len(mats)))()
+        for i, m in enumerate(mats):
+            r, g, b, a = m["rgba"]
+            fail = float(m["fail_stress"])
+            marr[i] = _Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"], fail,
+                                1 if fail > 0 else 0, int(m["fixed"]), int(m["is_measured"]),
+                                int(m["is_target"]), m["thermal_on_after"], r, g, b, a)
+        phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
+        E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
+        spec = _Lattice(nx, ny, nz, _ptr(cells, C.c_int8), _ptr(phase, C.c_double), _ptr(E, C.c_float),
+                        len(mats), marr, p["lattice_dim"], p["dt_frac"], p["bond_damp_z"], p["col_damp_z"],
+                        p["slow_damp_z"], int(p["volume_effects"]), int(p["enable_collision"]),
+                        int(p["enable_expansion"]), p["grav_acc"], int(p["floor_enabled"]),
+                        int(p["temp_enabled"]), int(p["vary_temp"]), p["temp_base"], p["temp_amplitude"],
+                        p["temp_period"])
+        h = C.c_void_p()
+        _check(lib().vx3mi_model_from_lattice(C.byref(spec), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_vxa(cls, base_vxa: str, vxd: str | None = None) -> "Model":
+        h = C.c_void_p()
+        _check(lib().vx3mi_model_from_vxa(base_vxa.encode(), None if vxd is None else vxd.encode(), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_xml(cls, base_text: str, vxd_text: str | None = None) -> "Model":
+        h = C.c_void_p()
+        _check(lib().vx3mi_model_from_xml(base_text.encode(), None if vxd_text is None else vxd_text.encode(),
+                                          C.byref(h)))
+        return cls(h.value)
+
+    def recommended_dt(self) -> float:
+        d = C.c_double()
+        _check(lib().vx3mi_model_recommended_dt(self._h, C.byref(d)))
+        return d.value
+
+    def topology(self) -> dict:
+        lat = np.zeros((self.nv, 3), dtype=np.int32)
+        ln, lp, la = (np.zeros(self.nl, dtype=np.int32) for _ in range(3))
+        _check(lib().vx3mi_model_topology(self._h, _ptr(lat, C.c_int), _ptr(ln, C.c_int), _ptr(lp, C.c_int),
+                                          _ptr(la, C.c_int)))
+        return dict(lattice=lat, link_neg=ln, link_pos=lp, link_axis=la)
+
+    def __del__(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().vx3mi_model_free(self._h)
+            self._h = C.c_void_p()
+
+
+class Batch:
+    """A device-resident batch of robots (CSR-packed)."""
+
+    def __init__(self, models: Sequence[Model] | Model, precision: int = FP32):
+        if isinstance(models, Model):
+            models = [models]
+        self.models = list(models)
+        arr = (C.c_void_p * len(self.models))(*[m._h.value for m in self.models])
+        h = C.c_void_p()
+        _check(lib().vx3mi_batch_create(arr, len(self.models), int(precision), C.byref(h)))
+        self._h = h
+        self.precision = precision
+        r, v, l = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().vx3mi_batch_counts(self._h, C.byref(r), C.byref(v), C.byref(l)))
+        self.nrobot, self.nv, self.nl = r.value, v.value, l.value
+        self.voff = np.concatenate([[0], np.cumsum([m.nv for m in self.models])]).astype(np.int64)
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().vx3mi_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def step(self, nsteps: int = 1, dt: float = -1.0, mode: int = MODE_AUTO) -> int:
+        return _check(lib().vx3mi_step(self._h, int(nsteps), float(dt), int(mode)), allow=(OK, DIVERGED))
+
+    def sync(self):
+        _check(lib().vx3mi_batch_sync(self._h))
+
+    def last_step_ms(self) -> float:
+        ms = C.c_float()
+        _check(lib().vx3mi_last_step_ms(self._h, C.byref(ms)))
+        return ms.value
+
+    def last_step_launches(self) -> int:
+        n = C.c_longlong()
+        _check(lib().vx3mi_last_step_launches(self._h, C.byref(n)))
+        return n.value
+
+    def state(self, robot: int = -1) -> dict:
+        n = self.nv if robot < 0 else self.models[robot].nv
+        pos, lm, am = np.zeros((n, 3)), np.zeros((n, 3)), np.zeros((n, 3))
+        q = np.zeros((n, 4))
+        _check(lib().vx3mi_get_state(self._h, int(robot), _ptr(pos, C.c_double), _ptr(q, C.c_double),
+                                     _ptr(lm, C.c_double), _ptr(am, C.c_double)))
+        return dict(pos=pos, quat=q, linmom=lm, angmom=am)
+
+    def set_state(self, robot: int = -1, pos=None, quat=None, linmom=None, angmom=None):
+        c = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (pos, quat, linmom, angmom)]
+        _check(lib().vx3mi_set_state(self._h, int(robot), *[_ptr(a, C.c_double) for a in c]))
+
+    def voxel_flags(self, robot: int = -1) -> np.ndarray:
+        n = self.nv if robot < 0 else self.models[robot].nv
+        f = np.zeros(n, dtype=np.int32)
+        _check(lib().vx3mi_get_voxel_flags(self._h, int(robot), _ptr(f, C.c_int)))
+        return f
+
+    def time(self) -> np.ndarray:
+        t = np.zeros(self.nrobot)
+        _check(lib().vx3mi_batch_time(self._h, _ptr(t, C.c_double)))
+        return t
+
+    def diverged(self) -> np.ndarray:
+        d = np.zeros(self.nrobot, dtype=np.int32)
+        _check(lib().vx3mi_batch_diverged(self._h, _ptr(d, C.c_int)), allow=(OK, DIVERGED))
+        return d
+
+    def com(self) -> np.ndarray:
+        c = np.zeros((self.nrobot, 3))
+        _check(lib().vx3mi_center_of_mass(self._h, _ptr(c, C.c_double)))
+        return c
+
+    def recommended_dt(self) -> np.ndarray:
+        d = np.zeros(self.nrobot)
+        _check(lib().vx3mi_batch_recommended_dt(self._h, _ptr(d, C.c_double)))
+        return d
+
+    def eval_link_forces(self) -> np.ndarray:
+        out = np.zeros((self.nl, 12))
+        _check(lib().vx3mi_eval_link_forces(self._h, _ptr(out, C.c_double)))
+        return out
