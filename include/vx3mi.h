@@ -96,6 +96,14 @@ int vx3mi_model_counts(const vx3mi_model *m, int *nvox, int *nlink, int *nsurf);
 int vx3mi_model_recommended_dt(const vx3mi_model *m, double *out);
 /* topology in voxel / link order: lattice (3 per voxel), link ends and axis */
 int vx3mi_model_topology(const vx3mi_model *m, int *lattice3, int *link_neg, int *link_pos, int *link_axis);
+/* derived material constants as the device uses them (float, the reference's
+ * rounding; VX3_MaterialVoxel.h:29-44, VX3_MaterialLink.h:27-36).
+ * voxel (12): mass, 1/mass, 1/inertia, 2*sqrt(mass)*zetaInternal,
+ *   globalDampingTranslateC, globalDampingRotateC, collisionDampingTranslateC,
+ *   gravityForce, penetrationStiffness, muStatic, muKinetic, CTE
+ * link (11): E, a1, a2, b1, b2, b3, sqA1, sqA2xIp, sqB1, sqB2xFMp, sqB3xIp */
+int vx3mi_model_voxel_constants(const vx3mi_model *m, int voxel, float *out12);
+int vx3mi_model_link_constants(const vx3mi_model *m, int link, float *out11);
 
 /* ---- batches (device) -----------------------------------------------------
  * A batch packs n models (CSR: voxel and link offsets per robot) into one

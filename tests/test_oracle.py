@@ -1,0 +1,231 @@
+"""Pins the FP64 oracle: golden trajectories of the reference's own CPU Voxelyze,
+closed forms and invariants of the update equations.
+
+Each check is chosen so that a plausible slip in the oracle (a dropped term, a
+wrong sign or axis, a transposed operand) fails it:
+  * golden vectors: every term of the link and voxel update on real robots;
+  * recommendedTimeStep: the closed form 1/(2 pi sqrt(E L / m));
+  * free fall: the exact symplectic-Euler sequence z_n = z_0 - a dt^2 n(n+1)/2;
+  * floor: rest penetration m g / (2 E L);
+  * two-voxel axial oscillation: period 2 pi / sqrt(2 E L / m);
+  * uniform temperature: equilibrium spacing L (1 + alpha T);
+  * internal forces conserve linear momentum;
+  * link forces in voxel frames are objective under a rigid motion;
+  * two overlapping unlinked voxels: penalty force k (R - d) (k = 2 E L).
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import oracle as O
+from vx3mi import configs
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+GOLDEN = os.path.join(HERE, "golden")
+
+
+def quiet(dims, cells=None, **kw):
+    """A robot with no gravity, floor, damping or thermal actuation unless asked."""
+    n = int(np.prod(dims))
+    p = dict(configs.BASE, dims=dims, cells=np.ones(n, dtype=np.int8) if cells is None else np.asarray(cells, np.int8),
+             phase=None, materials=[dict(E=1e6, rho=1000.0, cte=0.01, mu_s=1.0, mu_k=0.5)],
+             grav_acc=0.0, floor_enabled=0, bond_damp_z=0.0, slow_damp_z=0.0, col_damp_z=0.0,
+             vary_temp=0, temp_amplitude=0.0, enable_collision=0)
+    p.update(kw)
+    return p
+
+
+@pytest.mark.parametrize("name", ["config0", "robot5x4x3"])
+def test_golden_reference_cpu(name):
+    """Oracle (CVoxelyze integrator semantics) vs the reference CPU Voxelyze built
+    from src/old: positions and orientations every 50 steps for 600 steps."""
+    g = np.load(os.path.join(GOLDEN, f"ref_cpu_{name}.npz"))
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_golden import golden_problems
+
+    prob = golden_problems()[name]
+    m = O.OracleModel(prob)
+    every = int(g["every"])
+    for k in range(g["pos"].shape[0]):
+        m.step(every, float(g["dt"]), O.VXO_CPU_SEMANTICS)
+        s = m.state()
+        e_pos = np.max(np.linalg.norm(s["pos"] - g["pos"][k], axis=1) / np.maximum(np.linalg.norm(g["pos"][k], axis=1), 0.01))
+        e_q = np.max(np.abs(s["quat"] - g["quat"][k]))
+        assert e_pos <= 1e-12 and e_q <= 1e-12, (k, e_pos, e_q)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "vx3_ref_driver")),
+                    reason="reference CPU driver not built (make -C oracle ref needs the reference tree)")
+def test_live_reference_cpu_config1_subblock():
+    """A 10x10x6 two-material block of config 1 (collisions off) vs the live reference build."""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_golden import run_reference
+
+    prob = configs.config1(seed=3, dims=(10, 10, 6))
+    prob["enable_collision"] = 0
+    dt, rec = run_reference(prob, 300, 100)
+    m = O.OracleModel(prob)
+    for k in range(rec.shape[0]):
+        m.step(100, float(dt), O.VXO_CPU_SEMANTICS)
+        e = np.max(np.abs(m.state()["pos"] - rec[k, :, 0:3])) / 0.01
+        assert e <= 1e-11, (k, e)
+
+
+def test_recommended_dt_closed_form():
+    m = O.OracleModel(configs.config0())
+    E, L, rho = np.float32(1e6), np.float32(0.01), 1000.0
+    mass = np.float32(0.01 ** 3 * rho)
+    k = np.float32(E * L)
+    expect = 1.0 / (np.float32(6.283185) * np.sqrt(float(k) / float(mass)))
+    assert m.recommended_dt() == pytest.approx(expect, rel=1e-15)
+    assert m.recommended_dt() == pytest.approx(1.0 / (2 * np.pi * np.sqrt(1e6 * 0.01 / 1e-3)), rel=1e-6)
+    # two materials: the stiffest link (harmonic mean of 1e7 and 1e7) decides
+    m1 = O.OracleModel(configs.config1(seed=0, dims=(8, 8, 4)))
+    assert m1.recommended_dt() == pytest.approx(1.0 / (2 * np.pi * np.sqrt(1e7 * 0.01 / 1e-3)), rel=1e-6)
+
+
+def test_free_fall_exact_sequence():
+    p = quiet((1, 1, 1), grav_acc=-9.81)
+    m = O.OracleModel(p)
+    dt = np.float32(m.step_dt())
+    mass = np.float32(1e-3)
+    grav_mult = np.float32(-np.float32(-9.81) / 9.80665)
+    gforce = float(np.float32(np.float32(-mass * np.float32(9.80665)) * grav_mult))
+    mass_inv = float(np.float32(1.0) / mass)  # _massInverse is stored as float
+    a = gforce * mass_inv
+    z0 = m.state()["pos"][0, 2]
+    for n in (1, 10, 1000):
+        m2 = O.OracleModel(p)
+        m2.step(n, float(dt))
+        z = m2.state()["pos"][0, 2]
+        expect = z0 + a * float(dt) ** 2 * n * (n + 1) / 2
+        assert z == pytest.approx(expect, rel=1e-12, abs=1e-18)
+
+
+def test_floor_rest_penetration():
+    p = quiet((1, 1, 1), grav_acc=-9.81, floor_enabled=1, slow_damp_z=1.0, col_damp_z=0.8)
+    m = O.OracleModel(p)
+    m.step(3000)
+    z = m.state()["pos"][0, 2]
+    mass = float(np.float32(1e-3))
+    grav = float(np.float32(-9.81)) * -1.0
+    expect = mass * 9.80665 * float(np.float32(grav / 9.80665)) / (2 * 1e6 * 0.01)
+    assert -z == pytest.approx(expect, rel=1e-5)
+
+
+def test_two_voxel_axial_oscillation_period():
+    p = quiet((2, 1, 1))
+    m = O.OracleModel(p)
+    s = m.state()
+    pos = s["pos"].copy()
+    pos[1, 0] += 1e-4
+    m.set_state(pos=pos)
+    dt = m.step_dt()
+    ext = []
+    for _ in range(400):
+        m.step(1)
+        q = m.state()["pos"]
+        ext.append(q[1, 0] - q[0, 0] - 0.01)
+    ext = np.array(ext)
+    crossings = np.where(np.diff(np.sign(ext)) != 0)[0]
+    period = 2 * np.mean(np.diff(crossings)) * dt
+    k, mass = 1e6 * 0.01, 1e-3
+    omega = np.sqrt(2 * k / mass)
+    omega_d = 2 / dt * np.arcsin(omega * dt / 2)  # symplectic Euler dispersion
+    assert period == pytest.approx(2 * np.pi / omega_d, rel=2e-2)
+    assert period == pytest.approx(2 * np.pi / omega, rel=3e-2)
+    # no damping: the amplitude neither grows nor decays
+    assert np.max(np.abs(ext[-100:])) == pytest.approx(np.max(np.abs(ext[:100])), rel=2e-2)
+
+
+def test_uniform_temperature_equilibrium_spacing():
+    p = quiet((3, 3, 3), temp_amplitude=-20.0, vary_temp=0, slow_damp_z=1.0, bond_damp_z=0.1)
+    m = O.OracleModel(p)
+    m.step(4000)
+    pos = m.state()["pos"].reshape(3, 3, 3, 3)  # z, y, x, xyz
+    d = np.linalg.norm(pos[:, :, 1:, :] - pos[:, :, :-1, :], axis=-1)
+    assert np.allclose(d, 0.01 * (1 + 0.01 * -20.0), rtol=1e-6)
+
+
+def test_linear_momentum_conservation():
+    rng = np.random.default_rng(5)
+    p = configs.random_robot(rng, (4, 4, 3), 0.8, enable_collision=0)
+    p.update(grav_acc=0.0, floor_enabled=0, slow_damp_z=0.0)
+    p["materials"] = [dict(mm, rho=1000.0) for mm in p["materials"]]
+    m = O.OracleModel(p)
+    m.step(600)
+    lm = m.state()["linmom"]
+    assert np.linalg.norm(lm.sum(axis=0)) <= 1e-11 * np.abs(lm).sum()
+    assert np.abs(lm).sum() > 0
+
+
+def _quat_mul(a, b):
+    w1, x1, y1, z1 = a
+    w2, x2, y2, z2 = b
+    return np.array([w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2, w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2,
+                     w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2, w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2])
+
+
+def test_link_forces_objective_under_rigid_motion():
+    rng = np.random.default_rng(9)
+    p = configs.random_robot(rng, (4, 3, 3), 0.9, enable_collision=0)
+    a = O.OracleModel(p)
+    s = a.state()
+    pos = s["pos"] + rng.normal(scale=3e-4, size=s["pos"].shape)
+    ang = rng.normal(scale=0.03, size=(a.nv, 3))
+    th = np.linalg.norm(ang, axis=1)
+    q = np.concatenate([np.cos(th / 2)[:, None], (ang / th[:, None]) * np.sin(th / 2)[:, None]], axis=1)
+    a.set_state(pos=pos, quat=q)
+    fa = a.update_links()
+    # rigid motion: rotation R (quaternion r) and translation t
+    r = np.array([np.cos(0.35), *(np.sin(0.35) * np.array([0.3, -0.5, 0.81]) / np.linalg.norm([0.3, -0.5, 0.81]))])
+    w, x, y, z = r
+    R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                  [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                  [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+    b = O.OracleModel(p)
+    b.set_state(pos=pos @ R.T + np.array([0.03, -0.2, 0.5]), quat=np.array([_quat_mul(r, qq) for qq in q]))
+    fb = b.update_links()
+    assert np.max(np.abs(fa - fb)) <= 1e-9 * np.max(np.abs(fa))
+
+
+def test_two_sphere_contact_force():
+    # two unlinked voxels (lattice x = 0 and x = 2), the second moved to x = 1.1 L
+    p = quiet((3, 1, 1), cells=[1, 0, 1], enable_collision=1)
+    m = O.OracleModel(p)
+    pos = m.state()["pos"]
+    pos[1, 0] = 0.011
+    m.set_state(pos=pos)
+    dt = m.step_dt()
+    m.step(1)
+    lm = m.state()["linmom"]
+    pk = np.float32(2 * np.float32(1e6) * 0.01)  # penetrationStiffness, float
+    k = float(np.float32(2.0) / (np.float32(1.0) / pk + np.float32(1.0) / pk))  # VX3_Collision.cu:6
+    force = k * (1.25 * 0.01 - 0.011)
+    assert lm[0, 0] == pytest.approx(-force * dt, rel=1e-12)
+    assert lm[1, 0] == pytest.approx(force * dt, rel=1e-12)
+    # a linked pair pushed to the same distance feels only its link, never a contact
+    q = quiet((2, 1, 1), enable_collision=1)
+    a, b = O.OracleModel(q), O.OracleModel(dict(q, enable_collision=0))
+    for mm in (a, b):
+        ps = mm.state()["pos"]
+        ps[1, 0] = 0.011
+        mm.set_state(pos=ps)
+        mm.step(1)
+    assert np.array_equal(a.state()["linmom"], b.state()["linmom"])
+
+
+def test_cpu_semantics_flag_only_changes_static_friction_rotation():
+    """VX3 zeroes angular momentum of voxels held by static floor friction
+    (VX3_Voxel.cu:259-264); CVoxelyze does not.  Without a floor they agree."""
+    p = dict(configs.config0(), floor_enabled=0)
+    a, b = O.OracleModel(p), O.OracleModel(p)
+    a.step(300)
+    b.step(300, flags=O.VXO_CPU_SEMANTICS)
+    # only the float vs double dt*massInverse product differs without a floor
+    assert np.max(np.abs(a.state()["pos"] - b.state()["pos"])) <= 1e-12

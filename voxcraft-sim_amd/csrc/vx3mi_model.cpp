@@ -332,4 +332,21 @@ int vx3mi_model_topology(const vx3mi_model *m, int *lattice3, int *link_neg, int
     return VX3MI_OK;
 }
 
+int vx3mi_model_voxel_constants(const vx3mi_model *m, int voxel, float *o) {
+    if (!m || !o || voxel < 0 || voxel >= m->m.nv()) { set_error("invalid voxel constants query"); return VX3MI_E_ARG; }
+    const VoxelMat &v = m->m.vmats[m->m.vmat[voxel]];
+    const float vals[12] = {v.mass, v.massInv, v.inertiaInv, v.dmNum, v.gDampT, v.gDampR,
+                            v.cDampT, v.gravF, v.penK, v.muS, v.muK, v.alphaCTE};
+    for (int i = 0; i < 12; i++) o[i] = vals[i];
+    return VX3MI_OK;
+}
+
+int vx3mi_model_link_constants(const vx3mi_model *m, int link, float *o) {
+    if (!m || !o || link < 0 || link >= m->m.nl()) { set_error("invalid link constants query"); return VX3MI_E_ARG; }
+    const LinkMat &l = m->m.lmats[m->m.lmat[link]];
+    const float vals[11] = {l.E, l.a1, l.a2, l.b1, l.b2, l.b3, l.sqA1, l.sqA2xIp, l.sqB1, l.sqB2xFMp, l.sqB3xIp};
+    for (int i = 0; i < 11; i++) o[i] = vals[i];
+    return VX3MI_OK;
+}
+
 }  // extern "C"

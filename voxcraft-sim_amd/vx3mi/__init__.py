@@ -84,6 +84,8 @@ def lib():
         "vx3mi_model_counts": (C.c_int, [vp, ip, ip, ip]),
         "vx3mi_model_recommended_dt": (C.c_int, [vp, dp]),
         "vx3mi_model_topology": (C.c_int, [vp, ip, ip, ip, ip]),
+        "vx3mi_model_voxel_constants": (C.c_int, [vp, C.c_int, fp]),
+        "vx3mi_model_link_constants": (C.c_int, [vp, C.c_int, fp]),
         "vx3mi_batch_create": (C.c_int, [pvp, C.c_int, C.c_int, pvp]),
         "vx3mi_batch_destroy": (None, [vp]),
         "vx3mi_batch_counts": (C.c_int, [vp, ip, ip, ip]),
@@ -181,6 +183,16 @@ class Model:
         _check(lib().vx3mi_model_topology(self._h, _ptr(lat, C.c_int), _ptr(ln, C.c_int), _ptr(lp, C.c_int),
                                           _ptr(la, C.c_int)))
         return dict(lattice=lat, link_neg=ln, link_pos=lp, link_axis=la)
+
+    def voxel_constants(self, v: int) -> np.ndarray:
+        o = np.zeros(12, dtype=np.float32)
+        _check(lib().vx3mi_model_voxel_constants(self._h, int(v), _ptr(o, C.c_float)))
+        return o
+
+    def link_constants(self, l: int) -> np.ndarray:
+        o = np.zeros(11, dtype=np.float32)
+        _check(lib().vx3mi_model_link_constants(self._h, int(l), _ptr(o, C.c_float)))
+        return o
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and self._h.value:
