@@ -90,6 +90,9 @@ def lib():
         L.vxo_get_contact.argtypes = [vp, dp]
         L.vxo_get_voxel_material.argtypes = [vp, C.c_int, fp]
         L.vxo_get_link_material.argtypes = [vp, C.c_int, fp]
+        for f in ("vxo_sin", "vxo_cos", "vxo_acos"):
+            getattr(L, f).restype = C.c_double
+            getattr(L, f).argtypes = [C.c_double]
         _lib = L
     return _lib
 

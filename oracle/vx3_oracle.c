@@ -11,6 +11,7 @@
  * the reference's.  Build with -ffp-contract=off (no FMA contraction).
  */
 #include "vx3_oracle.h"
+#include "vx3mi_math.h" /* sin/cos/acos rounding identically on host and device */
 
 #include <float.h>
 #include <math.h>
@@ -132,7 +133,7 @@ static inline v3 q_to_rotvec(q4 q) {
         double s = sqrt((2 - 2 * q.w) / sl);
         return v3_(q.x * 2.0 * s, q.y * 2.0 * s, q.z * 2.0 * s);
     }
-    double a = acos(q.w), d = sqrt(sl);
+    double a = vxm_acos(q.w), d = sqrt(sl);
     return v3_(q.x * 2.0 * a / d, q.y * 2.0 * a / d, q.z * 2.0 * a / d);
 }
 /* VX3_Quat3D.h:363-378 */
@@ -145,8 +146,8 @@ static inline q4 q_from_rotvec(v3 in) {
         s = 1.0 - m2 / 6.0;
     } else {
         double m = sqrt(m2);
-        r.w = cos(m);
-        s = sin(m) / m;
+        r.w = vxm_cos(m);
+        s = vxm_sin(m) / m;
     }
     r.x = th.x * s;
     r.y = th.y * s;
@@ -173,14 +174,14 @@ static inline q4 q_from_angle_to_posx(v3 rf) {
         n.y *= li;
         n.z *= li;
     }
-    double theta = acos(n.x);
+    double theta = vxm_acos(n.x);
     if (theta > Q_PI - Q_DISCARD_ANGLE_RAD) {
         q.w = 0; q.x = 0; q.y = 1; q.z = 0;
         return q;
     }
     double ami = 1.0 / sqrt(n.z * n.z + n.y * n.y);
-    double a = 0.5 * theta, s = sin(a);
-    q.w = cos(a);
+    double a = 0.5 * theta, s = vxm_sin(a);
+    q.w = vxm_cos(a);
     q.x = 0;
     q.y = n.z * ami * s;
     q.z = -n.y * ami * s;
@@ -485,7 +486,7 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     float t0 = 0.0f;
     if (sp->temp_enabled) {
         float cur;
-        if (sp->vary_temp) cur = (sp->temp_period == 0) ? 0.0f : (float)(sp->temp_base + sp->temp_amplitude * sin(0.0));
+        if (sp->vary_temp) cur = (sp->temp_period == 0) ? 0.0f : (float)(sp->temp_base + sp->temp_amplitude * vxm_sin(0.0));
         else cur = (float)(sp->temp_base + sp->temp_amplitude);
         t0 = (float)(cur - sp->temp_base);
     }
@@ -518,7 +519,7 @@ static void update_temperature(vxo_model *m) {
         const vmat *mt = &m->vm[m->vmat_of[v]];
         if (mt->thermalOnAfter > m->time) continue;
         if (mt->fixed) continue;
-        double T = m->tempAmplitude * sin(twoPiF * (m->time / m->tempPeriod + m->phase[v]));
+        double T = m->tempAmplitude * vxm_sin(twoPiF * (m->time / m->tempPeriod + m->phase[v]));
         if (!m->enableExpansion && T > 0) T = 0;
         m->temp[v] = (float)T;
     }
@@ -976,3 +977,8 @@ void vxo_get_link_material(const vxo_model *m, int link, float *o) {
     o[0] = l->E; o[1] = l->a1; o[2] = l->a2; o[3] = l->b1; o[4] = l->b2; o[5] = l->b3;
     o[6] = l->sqA1; o[7] = l->sqA2xIp; o[8] = l->sqB1; o[9] = l->sqB2xFMp; o[10] = l->sqB3xIp;
 }
+
+/* the elementary functions the oracle uses (for their accuracy test) */
+double vxo_sin(double x) { return vxm_sin(x); }
+double vxo_cos(double x) { return vxm_cos(x); }
+double vxo_acos(double x) { return vxm_acos(x); }

@@ -104,6 +104,11 @@ void vxo_get_contact(const vxo_model *m, double *contact3);
 void vxo_get_voxel_material(const vxo_model *m, int voxel, float *out12);
 void vxo_get_link_material(const vxo_model *m, int link, float *out11);
 
+/* elementary functions used by the oracle (include/vx3mi_math.h) */
+double vxo_sin(double x);
+double vxo_cos(double x);
+double vxo_acos(double x);
+
 #ifdef __cplusplus
 }
 #endif

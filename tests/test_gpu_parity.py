@@ -152,3 +152,72 @@ def test_center_of_mass():
         ref = O.OracleModel(p)
         ref.step(50)
         assert np.allclose(com[r], ref.com(), rtol=1e-12, atol=1e-15)
+
+
+def _perturbed(prob, seed=0, scale=1.6e-3):
+    """Oracle model plus an initial state with voxels jostled by ~0.16 L so that
+    many unlinked surface pairs start inside each other's envelope."""
+    ref = O.OracleModel(prob)
+    rng = np.random.default_rng(seed)
+    s = ref.state()
+    pos = s["pos"] + rng.normal(scale=scale, size=s["pos"].shape)
+    ref.set_state(pos=pos)
+    return ref, pos
+
+
+@pytest.mark.parametrize("precision", [vx3mi.FP64, vx3mi.FP32])
+def test_collisions_config1_block(precision):
+    """Collisions on (config 1 block, 14x12x6): hashed grid + per-voxel contact gather
+    against the oracle's pair loop, 300 steps from a jostled state."""
+    prob = configs.config1(seed=2, dims=(14, 12, 6))
+    ref, pos = _perturbed(prob)
+    off = dict(prob, enable_collision=0)
+    ref_off = O.OracleModel(off)
+    ref_off.set_state(pos=pos)
+    batch = vx3mi.Batch([vx3mi.Model.from_problem(prob)], precision)
+    batch.set_state(pos=pos)
+    for _ in range(3):
+        batch.step(100, mode=vx3mi.MODE_FLAT)
+        ref.step(100)
+        ref_off.step(100)
+    s, r = batch.state(), ref.state()
+    ep, eq = pos_err(s["pos"], r["pos"]), quat_err(s["quat"], r["quat"])
+    # the contacts matter: the run without them is far away
+    assert pos_err(ref_off.state()["pos"], r["pos"]) > 1e-4
+    print(f"collisions FP{precision}: e_pos {ep:.3e} e_q {eq:.3e}")
+    assert ep <= TOL[precision] and eq <= TOL[precision]
+    assert batch.diverged().sum() == 0
+
+
+def test_collisions_config1_full_size_fp32():
+    """Full config 1 (45,800 voxels, collisions on), FP32, first 200 steps."""
+    prob = configs.config1()
+    ref = O.OracleModel(prob)
+    batch = vx3mi.Batch([vx3mi.Model.from_problem(prob)], vx3mi.FP32)
+    batch.step(200, mode=vx3mi.MODE_FLAT)
+    ref.step(200)
+    s, r = batch.state(), ref.state()
+    ep, eq = pos_err(s["pos"], r["pos"]), quat_err(s["quat"], r["quat"])
+    print(f"config1 full FP32 200 steps: e_pos {ep:.3e} e_q {eq:.3e}")
+    assert ep <= 1e-5 and eq <= 1e-5
+
+
+def test_collision_modes_bitwise_identical():
+    """Collisions on: plain, flat and persistent kernels agree bit for bit."""
+    probs = configs.config2(seed=8, n=6, collisions=1)
+    models = [vx3mi.Model.from_problem(p) for p in probs]
+    rng = np.random.default_rng(1)
+    states = []
+    base = vx3mi.Batch(models, vx3mi.FP32)
+    pos = base.state()["pos"] + rng.normal(scale=1.5e-3, size=(base.nv, 3))
+    base.close()
+    for mode in (vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT, vx3mi.MODE_PERSISTENT):
+        b = vx3mi.Batch(models, vx3mi.FP32)
+        b.set_state(pos=pos)
+        b.step(150, mode=mode)
+        states.append(b.state())
+        assert b.diverged().sum() == 0
+        b.close()
+    for s in states[1:]:
+        for k in ("pos", "quat", "linmom", "angmom"):
+            assert np.array_equal(s[k], states[0][k]), k

@@ -55,7 +55,9 @@ def test_golden_reference_cpu(name):
         s = m.state()
         e_pos = np.max(np.linalg.norm(s["pos"] - g["pos"][k], axis=1) / np.maximum(np.linalg.norm(g["pos"][k], axis=1), 0.01))
         e_q = np.max(np.abs(s["quat"] - g["quat"][k]))
-        assert e_pos <= 1e-12 and e_q <= 1e-12, (k, e_pos, e_q)
+        # ~1e-16 through step 500; a floor-friction switch near step 550 then
+        # amplifies the last-ulp difference between glibc's sin and the oracle's
+        assert e_pos <= 1e-11 and e_q <= 1e-11, (k, e_pos, e_q)
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "vx3_ref_driver")),

@@ -17,7 +17,16 @@ struct RobotParams {
     double tempAmp, tempPeriod;
     int varyTemp, enableExpansion, collisions, pad;
     int voff, vcount, loff, lcount;
+    // collision broadphase: surface voxels [soff, soff+scount) of the surface
+    // list, hash table of (mask+1) buckets at bucket offset toff, cell size
+    int soff, scount, toff, mask;
+    double cellSize;
 };
+
+// fixed-capacity buckets of the collision hash grid, and the most contacts one
+// voxel may have in one step (both overflows are reported, never ignored)
+constexpr int kBucketCap = 16;
+constexpr int kMaxContacts = 32;
 
 struct RobotTime {
     double time[2];
@@ -25,7 +34,7 @@ struct RobotTime {
     float dt;
     int diverged;
     int steps;
-    int pad;
+    int overflow;  // collision bucket or contact list overflowed
 };
 
 #define HIP_TRY(expr)                                                                          \
@@ -44,6 +53,9 @@ struct BatchBase {
     int persistentChunk = 1000;  // steps per persistent launch
     long launches = 0;           // kernel launches of the last step call
     int lastMode = 0;
+    int S = 0;                   // surface voxels (collision candidates) of the batch
+    bool anyCollisions = false;
+    long tableBuckets = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void *> allocs;

@@ -44,7 +44,14 @@ template <typename T> struct Dev {
     const LinkMat *lmats;
     const RobotParams *rp;
     RobotTime *rt;
-    const T4<T> *contact;
+    const T4<T> *contact;  // per-voxel contact force, or nullptr without collisions
+    // collision broadphase
+    const int *surf;       // surface voxel ids, CSR by robot
+    int4 *scell;           // per surface voxel: cell coordinates and bucket
+    int *ccnt[2];          // bucket counts, one table per step parity
+    int *ctab[2];          // bucket entries (kBucketCap per bucket): surface indices
+    T4<T> *contactW;
+    long tableBuckets;
 };
 
 // ------------------------------------------------------------------ device
@@ -131,6 +138,134 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
     a.mo1[v] = T4<T>{s.h.y, s.h.z, (T)s.flags, (T)0};
 }
 
+// ---- collisions: hashed uniform grid over surface voxels ----------------------
+// Replaces the all-pairs surface loop (updateAttach VX3_VoxelyzeKernel.cu:448-458,
+// gpu_update_attach :833-843).  The cell edge is the largest envelope
+// diameter, so every pair within an envelope lies in adjacent cells.
+__device__ __forceinline__ unsigned cell_hash(int cx, int cy, int cz) {
+    return ((unsigned)cx * 73856093u) ^ ((unsigned)cy * 19349663u) ^ ((unsigned)cz * 83492791u);
+}
+
+template <typename T> __device__ __forceinline__ V3<T> abs_position(const Dev<T> &a, int v, const T4<T> &q0) {
+    if constexpr (kAbsolutePositions<T>) return V3<T>{q0.x, q0.y, q0.z};
+    const int pk = a.lat[v];
+    const T L = (T)a.vmats[a.vmat[v]].nomSize;
+    return V3<T>{(T)(pk & 1023) * L + q0.x, (T)((pk >> 10) & 1023) * L + q0.y, (T)(pk >> 20) * L + q0.z};
+}
+
+template <typename T> __device__ __forceinline__ void col_insert(const Dev<T> &a, int s, int parity) {
+    const int v = a.surf[s];
+    RobotTime *rt = a.rt + a.vrob[v];
+    const RobotParams &rp = a.rp[a.vrob[v]];
+    if (!rp.collisions || rt->diverged) return;
+    const V3<T> p = abs_position(a, v, a.pq0[v]);
+    const double h = rp.cellSize;
+    const int cx = (int)floor((double)p.x / h), cy = (int)floor((double)p.y / h), cz = (int)floor((double)p.z / h);
+    const int key = (int)(cell_hash(cx, cy, cz) & (unsigned)rp.mask);
+    a.scell[s] = int4{cx, cy, cz, key};
+    const long b = (long)rp.toff + key;
+    const int slot = atomicAdd(&a.ccnt[parity][b], 1);
+    if (slot < kBucketCap) a.ctab[parity][b * kBucketCap + slot] = s;
+    else atomicOr(&rt->overflow, 1);
+}
+
+template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity) {
+    const int v = a.surf[s];
+    RobotTime *rt = a.rt + a.vrob[v];
+    const RobotParams &rp = a.rp[a.vrob[v]];
+    if (!rp.collisions || rt->diverged) return;
+    const VoxelMat &mi = a.vmats[a.vmat[v]];
+    const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v];
+    const V3<T> pi = abs_position(a, v, qi0);
+    const T bsaI = base_size_average<T>(mi.nomSize, (float)qi1.w, mi.alphaCTE);
+    // directly linked voxels never collide (is_neighbor depth 1, :651-680, :701-703)
+    int nb[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+        const int l = a.vlinks[6 * v + d];
+        nb[d] = l < 0 ? -1 : ((d & 1) ? a.lends[l].x : a.lends[l].y);
+    }
+    const int4 c = a.scell[s];
+    int found[kMaxContacts];
+    int nf = 0;
+    for (int dz = -1; dz <= 1; dz++)
+        for (int dy = -1; dy <= 1; dy++)
+            for (int dx = -1; dx <= 1; dx++) {
+                const int key = (int)(cell_hash(c.x + dx, c.y + dy, c.z + dz) & (unsigned)rp.mask);
+                const long b = (long)rp.toff + key;
+                const int n = min(a.ccnt[parity][b], kBucketCap);
+                for (int k = 0; k < n; k++) {
+                    const int sj = a.ctab[parity][b * kBucketCap + k];
+                    if (sj == s) continue;
+                    const int4 cj = a.scell[sj];
+                    if (cj.x != c.x + dx || cj.y != c.y + dy || cj.z != c.z + dz) continue;  // hash alias
+                    const int vj = a.surf[sj];
+                    bool linked = false;
+#pragma unroll
+                    for (int d = 0; d < 6; d++) linked |= nb[d] == vj;
+                    if (linked) continue;
+                    const VoxelMat &mj = a.vmats[a.vmat[vj]];
+                    if (mi.fixed && mj.fixed) continue;
+                    const T4<T> qj0 = a.pq0[vj], qj1 = a.pq1[vj];
+                    const V3<T> pj = abs_position(a, vj, qj0);
+                    V3<T> diff = pi - pj;
+                    if constexpr (!kAbsolutePositions<T>) {
+                        // lattice offset and displacement difference kept apart in float
+                        const int pki = a.lat[v], pkj = a.lat[vj];
+                        const T L = (T)mi.nomSize;
+                        diff = V3<T>{(T)((pki & 1023) - (pkj & 1023)) * L + (qi0.x - qj0.x),
+                                     (T)(((pki >> 10) & 1023) - ((pkj >> 10) & 1023)) * L + (qi0.y - qj0.y),
+                                     (T)((pki >> 20) - (pkj >> 20)) * L + (qi0.z - qj0.z)};
+                    }
+                    const T bsaJ = base_size_average<T>(mj.nomSize, (float)qj1.w, mj.alphaCTE);
+                    if (!in_contact_envelope(diff, bsaI, bsaJ)) continue;
+                    if (nf < kMaxContacts) found[nf++] = sj;
+                    else atomicOr(&rt->overflow, 1);
+                }
+            }
+    // ascending partner index: the order in which the serialized pair loop of
+    // the reference accumulates this voxel's contact force
+    for (int i = 1; i < nf; i++) {
+        const int x = found[i];
+        int j = i - 1;
+        while (j >= 0 && found[j] > x) { found[j + 1] = found[j]; j--; }
+        found[j + 1] = x;
+    }
+    const V3<T> velI = V3<T>{a.mo0[v].x, a.mo0[v].y, a.mo0[v].z} * (T)mi.massInv;
+    V3<T> f = {0, 0, 0};
+    for (int k = 0; k < nf; k++) {
+        const int vj = a.surf[found[k]];
+        const VoxelMat &mj = a.vmats[a.vmat[vj]];
+        const T4<T> qj0 = a.pq0[vj], qj1 = a.pq1[vj];
+        V3<T> diff;
+        if constexpr (kAbsolutePositions<T>) {
+            diff = pi - V3<T>{qj0.x, qj0.y, qj0.z};
+        } else {
+            const int pki = a.lat[v], pkj = a.lat[vj];
+            const T L = (T)mi.nomSize;
+            diff = V3<T>{(T)((pki & 1023) - (pkj & 1023)) * L + (qi0.x - qj0.x),
+                         (T)(((pki >> 10) & 1023) - ((pkj >> 10) & 1023)) * L + (qi0.y - qj0.y),
+                         (T)((pki >> 20) - (pkj >> 20)) * L + (qi0.z - qj0.z)};
+        }
+        const T bsaJ = base_size_average<T>(mj.nomSize, (float)qj1.w, mj.alphaCTE);
+        const V3<T> velJ = V3<T>{a.mo0[vj].x, a.mo0[vj].y, a.mo0[vj].z} * (T)mj.massInv;
+        f = f + contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
+    }
+    a.contactW[v] = T4<T>{f.x, f.y, f.z, 0};
+}
+
+template <typename T> __global__ __launch_bounds__(256) void k_col_insert(Dev<T> a, int s0, int n, int parity) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) col_insert(a, s0 + i, parity);
+}
+
+// contact forces; also empties the other parity's table for the next step
+template <typename T> __global__ __launch_bounds__(256) void k_col_contact(Dev<T> a, int s0, int n, int parity) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (long b = i; b < a.tableBuckets; b += (long)gridDim.x * blockDim.x) a.ccnt[parity ^ 1][b] = 0;
+    if (i < n) col_contact(a, s0 + i, parity);
+}
+
 template <typename T> __global__ __launch_bounds__(256) void k_links(Dev<T> a, int l0, int n, int parity) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) link_update(a, l0 + i, parity);
@@ -144,8 +279,15 @@ template <typename T> __global__ __launch_bounds__(256) void k_voxels(Dev<T> a, 
 // One workgroup per robot; nsteps steps with two barrier-separated phases.
 template <typename T> __global__ __launch_bounds__(256) void k_persistent(Dev<T> a, int nsteps, int parity0) {
     const RobotParams rp = a.rp[blockIdx.x];
+    const bool col = rp.collisions && rp.scount > 0 && a.contact != nullptr;
     for (int s = 0; s < nsteps; s++) {
         const int parity = (parity0 + s) & 1;
+        if (col) {
+            for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_insert(a, i, parity);
+            for (int b = threadIdx.x; b <= rp.mask; b += blockDim.x) a.ccnt[parity ^ 1][(long)rp.toff + b] = 0;
+            __syncthreads();
+            for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_contact(a, i, parity);
+        }
         for (int l = rp.loff + threadIdx.x; l < rp.loff + rp.lcount; l += blockDim.x) link_update(a, l, parity);
         __syncthreads();
         for (int v = rp.voff + threadIdx.x; v < rp.voff + rp.vcount; v += blockDim.x) voxel_update(a, v, parity);
@@ -341,11 +483,58 @@ template <typename T> struct Batch : BatchBase {
         curDt = defaultDt;
         HIP_TRY(hipMemcpy(d_rt, rt.data(), sizeof(RobotTime) * R, hipMemcpyHostToDevice));
 
+        // collision broadphase: surface list, per-robot hash tables
+        std::vector<int> surf;
+        long buckets = 0;
+        for (int r = 0; r < n; r++) {
+            const Model &m = models[r]->m;
+            RobotParams &p = hrp[r];
+            p.soff = (int)surf.size();
+            p.scount = m.enableCollision ? (int)m.surface.size() : 0;
+            if (p.scount) {
+                for (int v : m.surface) surf.push_back(p.voff + v);
+                int hsz = 64;
+                while (hsz < 2 * p.scount) hsz <<= 1;
+                p.toff = (int)buckets;
+                p.mask = hsz - 1;
+                buckets += hsz;
+                // largest envelope diameter: 2 * 0.625 * max base size (thermal growth bounded by
+                // the largest |temperature| times the largest CTE of the robot)
+                float amax = 0.0f;
+                for (const VoxelMat &vm : m.vmats) amax = std::max(amax, std::fabs(vm.alphaCTE));
+                const double tmax = std::max(std::fabs((double)m.initialTemp), std::fabs(m.tempAmplitude));
+                p.cellSize = 2 * kCOLLISION_ENVELOPE_RADIUS * m.latticeDim * (1.0 + tmax * amax) * 1.0001;
+                anyCollisions = true;
+            }
+        }
+        S = (int)surf.size();
+        tableBuckets = buckets;
+        HIP_TRY(hipMemcpy(d_rp, hrp.data(), sizeof(RobotParams) * R, hipMemcpyHostToDevice));
+        d.contact = nullptr;
+        d.tableBuckets = buckets;
+        if (anyCollisions) {
+            int *surfd, *c0, *c1, *t0, *t1;
+            int4 *scd;
+            T4<T> *cw;
+            if ((rc = alloc(&surfd, S)) || (rc = alloc(&scd, S)) || (rc = alloc(&c0, buckets)) ||
+                (rc = alloc(&c1, buckets)) || (rc = alloc(&t0, buckets * kBucketCap)) ||
+                (rc = alloc(&t1, buckets * kBucketCap)) || (rc = alloc(&cw, V)))
+                return rc;
+            HIP_TRY(hipMemcpy(surfd, surf.data(), sizeof(int) * S, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemset(c0, 0, sizeof(int) * buckets));
+            HIP_TRY(hipMemset(c1, 0, sizeof(int) * buckets));
+            HIP_TRY(hipMemset(cw, 0, sizeof(T4<T>) * V));
+            HIP_TRY(hipDeviceSynchronize());
+            d.surf = surfd; d.scell = scd; d.ccnt[0] = c0; d.ccnt[1] = c1; d.ctab[0] = t0; d.ctab[1] = t1;
+            d.contactW = cw;
+            d.contact = cw;
+        }
+
         d.pq0 = pq0d; d.pq1 = pq1d; d.mo0 = mo0d; d.mo1 = mo1d;
         d.lat = latd; d.vmat = vmatd; d.vrob = vrobd; d.phase = phased; d.vlinks = vlinksd;
         d.lends = lendsd; d.lmeta = lmetad;
         d.ls0 = ls0; d.ls1 = ls1; d.ls2 = ls2; d.lf0 = lf0; d.lf1 = lf1; d.lf2 = lf2;
-        d.vmats = d_vmats; d.lmats = d_lmats; d.rp = d_rp; d.rt = d_rt; d.contact = nullptr;
+        d.vmats = d_vmats; d.lmats = d_lmats; d.rp = d_rp; d.rt = d_rt;
         return VX3MI_OK;
     }
 
@@ -362,9 +551,12 @@ template <typename T> struct Batch : BatchBase {
     }
 
     void launch_flat_step(int parity) {
+        if (anyCollisions) k_col_insert<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
         if (L) k_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, 0, L, parity);
+        if (anyCollisions) k_col_contact<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
         k_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
     }
+    int kernels_per_flat_step() const { return 2 + (anyCollisions ? 2 : 0); }
 
     int step(int nsteps, float dt, int mode) override {
         if (nsteps <= 0) return VX3MI_OK;
@@ -379,9 +571,11 @@ template <typename T> struct Batch : BatchBase {
             for (int s = 0; s < nsteps; s++) {
                 for (int r = 0; r < R; r++) {
                     const RobotParams &p = hrp[r];
+                    if (p.scount) k_col_insert<T><<<(p.scount + 255) / 256, 256, 0, stream>>>(d, p.soff, p.scount, parity);
                     if (p.lcount) k_links<T><<<(p.lcount + 255) / 256, 256, 0, stream>>>(d, p.loff, p.lcount, parity);
+                    if (p.scount) k_col_contact<T><<<(p.scount + 255) / 256, 256, 0, stream>>>(d, p.soff, p.scount, parity);
                     k_voxels<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
-                    launches += 2;
+                    launches += 2 + (p.scount ? 2 : 0);
                 }
                 parity ^= 1;
             }
@@ -410,13 +604,13 @@ template <typename T> struct Batch : BatchBase {
                 }
                 for (; s + chunk <= nsteps; s += chunk) {
                     HIP_TRY(hipGraphLaunch(graph, stream));
-                    launches += 2 * chunk;
+                    launches += (long)kernels_per_flat_step() * chunk;
                 }
             }
             for (; s < nsteps; s++) {
                 launch_flat_step(parity);
                 parity ^= 1;
-                launches += 2;
+                launches += kernels_per_flat_step();
             }
         }
         HIP_TRY(hipGetLastError());

@@ -21,6 +21,8 @@
 #include <stdint.h>
 #include <type_traits>
 
+#include "vx3mi_math.h"
+
 #ifdef __HIPCC__
 #include <hip/hip_runtime.h>
 #define VX_HD __host__ __device__ __forceinline__
@@ -39,12 +41,13 @@ template <typename T> struct Q4 { T w, x, y, z; };
 
 VX_HD float msqrt(float a) { return sqrtf(a); }
 VX_HD double msqrt(double a) { return sqrt(a); }
+// double transcendentals round identically on host and device (vx3mi_math.h)
 VX_HD float msin(float a) { return sinf(a); }
-VX_HD double msin(double a) { return sin(a); }
+VX_HD double msin(double a) { return vxm_sin(a); }
 VX_HD float mcos(float a) { return cosf(a); }
-VX_HD double mcos(double a) { return cos(a); }
+VX_HD double mcos(double a) { return vxm_cos(a); }
 VX_HD float macos(float a) { return acosf(a); }
-VX_HD double macos(double a) { return acos(a); }
+VX_HD double macos(double a) { return vxm_acos(a); }
 VX_HD float matan2(float a, float b) { return atan2f(a, b); }
 VX_HD double matan2(double a, double b) { return atan2(a, b); }
 VX_HD float mabs(float a) { return fabsf(a); }
@@ -141,7 +144,7 @@ template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q) {
             double s = sqrt((2 - 2 * q.w) / sl);
             return v3<T>(q.x * 2.0 * s, q.y * 2.0 * s, q.z * 2.0 * s);
         }
-        double a = acos(q.w), d = sqrt(sl);
+        double a = vxm_acos(q.w), d = sqrt(sl);
         return v3<T>(q.x * 2.0 * a / d, q.y * 2.0 * a / d, q.z * 2.0 * a / d);
     } else {
         float s2 = q.x * q.x + q.y * q.y + q.z * q.z;  // == 1 - w^2 for a unit quaternion
@@ -198,7 +201,7 @@ template <typename T> VX_HD Q4<T> from_angle_to_posx(V3<T> rf) {
         n.z *= li;
     }
     T theta;
-    if constexpr (std::is_same<T, double>::value) theta = acos(n.x);
+    if constexpr (std::is_same<T, double>::value) theta = vxm_acos(n.x);
     else theta = atan2f(sqrtf(n.y * n.y + n.z * n.z), n.x);
     if (theta > (T)(kPI - kDISCARD_ANGLE_RAD)) {
         q.w = 0; q.x = 0; q.y = 1; q.z = 0;
@@ -236,7 +239,7 @@ template <typename T> VX_HD V3<T> to_axis_original(V3<T> v, int axis) {
 // voxels and voxels whose TurnOnThermalExpansion time has not come).
 template <typename T> VX_HD float actuation_temperature(double amp, double period, double t, double phase, int enableExpansion) {
     if constexpr (std::is_same<T, double>::value) {
-        double Tc = amp * sin((double)(2 * 3.1415926f) * (t / period + phase));
+        double Tc = amp * vxm_sin((double)(2 * 3.1415926f) * (t / period + phase));
         if (!enableExpansion && Tc > 0) Tc = 0;
         return (float)Tc;
     } else {
@@ -496,6 +499,51 @@ VX_HD void voxel_step(const VoxelMat &vm, float dtf, int iz, float temp, V3<T> f
     }
     s.h = h;
     s.flags = flags;
+}
+
+
+// ---- collisions ---------------------------------------------------------------
+// baseSizeAverage, VX3_Voxel.h:102-105
+template <typename T> VX_HD T base_size_average(double nomSize, float temp, float alpha) {
+    const float f = thermal_factor(temp, alpha);
+    if constexpr (std::is_same<T, double>::value) {
+        const double b = nomSize * (double)f;
+        return (b + b + b) / 3.0f;
+    } else {
+        return (float)nomSize * f;
+    }
+}
+
+// Contact force on voxel i from surface voxel j: handle_collision_attachment
+// (VX3_VoxelyzeKernel.cu:682-729) with voxel1 = i, voxel2 = j and
+// VX3_Collision::updateContactForce (VX3_Collision.cu:17-31); returns false
+// when the pair is outside the envelope.  diff = pos_i - pos_j.  The caller
+// has already excluded directly linked pairs and pairs of two fixed voxels.
+// The force computed with i as voxel1 is exactly the negation of the one the
+// reference adds to voxel2 when i is voxel2, so each voxel can sum its own
+// contacts.
+template <typename T>
+VX_HD bool in_contact_envelope(V3<T> diff, T bsaI, T bsaJ) {
+    const T watch = (bsaI + bsaJ) * (T)kCOLLISION_ENVELOPE_RADIUS;
+    if (diff.x > watch || diff.x < -watch) return false;
+    if (diff.y > watch || diff.y < -watch) return false;
+    if (diff.z > watch || diff.z < -watch) return false;
+    return len(diff) < watch;  // RelDist > 0 (equality contributes a zero force)
+}
+
+template <typename T>
+VX_HD V3<T> contact_force(V3<T> diff, T bsaI, T bsaJ, V3<T> velI, V3<T> velJ, float pkI, float pkJ, float cdI,
+                          float cdJ) {
+    const T penStiff = (T)(2.0f / (1.0f / pkI + 1.0f / pkJ));
+    const T dampC = (T)(0.5f * (cdI + cdJ));
+    const V3<T> offset = v3<T>(-diff.x, -diff.y, -diff.z);
+    const T nomDist = (bsaI + bsaJ) * (T)kCOLLISION_ENVELOPE_RADIUS;
+    const T l = len(offset);
+    const T relDist = nomDist - l;
+    if (!(relDist > 0)) return v3<T>(0, 0, 0);
+    const V3<T> unit = l > 0 ? v3<T>(offset.x / l, offset.y / l, offset.z / l) : offset;
+    const T rv = (velI.x * unit.x + velI.y * unit.y + velI.z * unit.z) - (velJ.x * unit.x + velJ.y * unit.y + velJ.z * unit.z);
+    return unit * (penStiff * relDist + dampC * rv);
 }
 
 }  // namespace vx3mi
