@@ -96,7 +96,7 @@ def fp32_error_vs_oracle(problems, nsteps: int) -> float:
     import oracle as O
 
     b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in problems], vx3mi.FP32)
-    b.step(nsteps, mode=vx3mi.MODE_PERSISTENT)
+    b.step(nsteps, mode=vx3mi.MODE_AUTO)
     worst = 0.0
     for r, p in enumerate(problems):
         ref = O.OracleModel(p)
@@ -120,12 +120,15 @@ def main() -> int:
     ap.add_argument("--check-robots", type=int, default=16)
     ap.add_argument("--check-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--mode", choices=["tiled", "persistent", "flat"], default="tiled")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); robots shard across GPUs "
               "as independent batches, which this build does not launch.", file=sys.stderr)
         return 2
 
+    mode = {"tiled": vx3mi.MODE_TILED, "persistent": vx3mi.MODE_PERSISTENT, "flat": vx3mi.MODE_FLAT}[args.mode]
+    kernel = {"tiled": "k_tile<float>", "persistent": "k_persistent<float>", "flat": "k_links<float> + k_voxels<float>"}[args.mode]
     problems = configs.config2(seed=0, n=args.robots)
     t0 = time.perf_counter()
     models = [vx3mi.Model.from_problem(p) for p in problems]
@@ -134,11 +137,11 @@ def main() -> int:
     log(f"bench: {args.robots} robots, {nv} voxels, {nl} links, setup {time.perf_counter() - t0:.2f}s")
 
     # warmup (also builds nothing lazily: persistent launches need no graph)
-    batch.step(args.warmup, mode=vx3mi.MODE_PERSISTENT)
+    batch.step(args.warmup, mode=mode)
     batch.sync()
     # timed region: inputs resident in HBM, device events around the launches
     t0 = time.perf_counter()
-    batch.step(args.steps, mode=vx3mi.MODE_PERSISTENT)
+    batch.step(args.steps, mode=mode)
     batch.sync()
     wall = time.perf_counter() - t0
     dev_ms = batch.last_step_ms()
@@ -202,7 +205,7 @@ def main() -> int:
         "fp32_rel_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_voxel_step": bpv, "kernel": "k_persistent<float>",
+                     "bytes_per_voxel_step": bpv, "kernel": kernel,
                      "steps_per_launch": steps_per_launch},
         "cpu_baseline": cpu,
     }

@@ -33,10 +33,11 @@ extern "C" {
 #define VX3MI_FP64 64
 
 /* step modes (vx3mi_step) */
-#define VX3MI_MODE_AUTO 0        /* persistent kernel for batches of small robots, else flat */
+#define VX3MI_MODE_AUTO 0        /* tiled if possible, else persistent for small robots, else flat */
 #define VX3MI_MODE_PLAIN 1       /* one robot per launch sequence (reference-style) */
 #define VX3MI_MODE_FLAT 2        /* one link launch + one voxel launch per step for the whole batch */
 #define VX3MI_MODE_PERSISTENT 3  /* one workgroup per robot, many steps per launch */
+#define VX3MI_MODE_TILED 4       /* robots bin-packed into register/LDS-resident tiles (collisions off) */
 
 /* One palette material: VXA Palette/Material (src/VXA/VX_Object.cpp:1373-1475). */
 typedef struct {

@@ -107,7 +107,7 @@ def test_modes_bitwise_identical():
     probs = configs.config2(seed=5, n=12)
     models = [vx3mi.Model.from_problem(p) for p in probs]
     states = []
-    for mode in (vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT, vx3mi.MODE_PERSISTENT):
+    for mode in (vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT, vx3mi.MODE_PERSISTENT, vx3mi.MODE_TILED):
         b = vx3mi.Batch(models, vx3mi.FP32)
         b.step(130, mode=mode)
         states.append(b.state())
@@ -117,12 +117,14 @@ def test_modes_bitwise_identical():
             assert np.array_equal(s[k], states[0][k]), k
 
 
+@pytest.mark.parametrize("mode", [vx3mi.MODE_PERSISTENT, vx3mi.MODE_TILED])
 @pytest.mark.parametrize("precision", [vx3mi.FP64, vx3mi.FP32])
-def test_batched_robots_match_oracle(precision):
+def test_batched_robots_match_oracle(precision, mode):
     """Uneven robots in one batch, each checked against its own oracle run."""
     probs = configs.config2(seed=1, n=6)
     batch = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], precision)
-    batch.step(400, mode=vx3mi.MODE_PERSISTENT)
+    batch.step(150, mode=mode)
+    batch.step(250, mode=mode)
     for r, p in enumerate(probs):
         ref = O.OracleModel(p)
         ref.step(400)
@@ -221,3 +223,18 @@ def test_collision_modes_bitwise_identical():
     for s in states[1:]:
         for k in ("pos", "quat", "linmom", "angmom"):
             assert np.array_equal(s[k], states[0][k]), k
+
+
+def test_tiled_full_batch_matches_flat():
+    """The whole 1024-robot workload: tiled (LPT-packed tiles) and flat kernels agree bit for bit."""
+    probs = configs.config2(seed=0, n=1024)
+    models = [vx3mi.Model.from_problem(p) for p in probs]
+    out = []
+    for mode in (vx3mi.MODE_FLAT, vx3mi.MODE_TILED):
+        b = vx3mi.Batch(models, vx3mi.FP32)
+        b.step(300, mode=mode)
+        out.append((b.state(), b.time()))
+        b.close()
+    for k in ("pos", "quat", "linmom", "angmom"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert np.array_equal(out[0][1], out[1][1])

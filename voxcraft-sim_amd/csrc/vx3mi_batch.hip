@@ -64,7 +64,7 @@ int vx3mi_batch_recommended_dt(vx3mi_batch *b, double *out) {
 
 int vx3mi_step(vx3mi_batch *b, int nsteps, float dt, int mode) {
     if (!b) { set_error("batch is NULL"); return VX3MI_E_ARG; }
-    if (mode < 0 || mode > 3 || nsteps < 0) { set_error("invalid step mode or step count"); return VX3MI_E_ARG; }
+    if (mode < 0 || mode > 4 || nsteps < 0) { set_error("invalid step mode or step count"); return VX3MI_E_ARG; }
     return b->b->step(nsteps, dt, mode);
 }
 

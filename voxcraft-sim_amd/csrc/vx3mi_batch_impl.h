@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <map>
 #include <cstring>
 #include <new>
 #include <string>
@@ -295,6 +296,10 @@ template <typename T> __global__ __launch_bounds__(256) void k_persistent(Dev<T>
     }
 }
 
+}  // namespace vx3mi
+#include "vx3mi_tile.h"
+namespace vx3mi {
+
 // recommendedTimeStep (VX3_VoxelyzeKernel.cu:184-217) as a max-reduction:
 // per-lane maxima, a 64-lane butterfly, then one value per wavefront in LDS.
 template <typename T>
@@ -367,6 +372,10 @@ __global__ __launch_bounds__(256) void k_com(const T4<T> *pq0, const int *lat, c
 // ------------------------------------------------------------------- host
 template <typename T> struct Batch : BatchBase {
     Dev<T> d{};
+    int4 *d_tiles = nullptr;    // robot begin, robot count, voxels, links
+    int *d_tileRobots = nullptr;
+    int ntiles = 0;
+    bool tileOK = false;
     RobotParams *d_rp = nullptr;
     VoxelMat *d_vmats = nullptr;
     LinkMat *d_lmats = nullptr;
@@ -381,11 +390,11 @@ template <typename T> struct Batch : BatchBase {
         std::vector<LinkMat> lmats;
         std::vector<float> vmatE;
         std::vector<T4<T>> pq0, pq1;
+        std::map<std::string, int> vmIndex, lmIndex;
         hrp.resize(n);
         for (int r = 0; r < n; r++) {
             const Model &m = models[r]->m;
             const int voff = (int)lat.size(), loff = (int)lends.size();
-            const int vmoff = (int)hvmats.size(), lmoff = (int)lmats.size();
             RobotParams &p = hrp[r];
             std::memset(&p, 0, sizeof(p));
             p.tempAmp = m.tempAmplitude;
@@ -394,12 +403,31 @@ template <typename T> struct Batch : BatchBase {
             p.enableExpansion = m.enableExpansion;
             p.collisions = m.enableCollision;
             p.voff = voff; p.vcount = m.nv(); p.loff = loff; p.lcount = m.nl();
-            for (const VoxelMat &vm : m.vmats) hvmats.push_back(vm);
-            for (float e : m.vmatE) vmatE.push_back(e);
-            for (const LinkMat &lmt : m.lmats) lmats.push_back(lmt);
+            // materials are deduplicated across robots (byte-identical constants)
+            std::vector<int> vmRemap(m.vmats.size()), lmRemap(m.lmats.size());
+            for (size_t k = 0; k < m.vmats.size(); k++) {
+                std::string key((const char *)&m.vmats[k], sizeof(VoxelMat));
+                key.append((const char *)&m.vmatE[k], sizeof(float));
+                auto it = vmIndex.find(key);
+                if (it == vmIndex.end()) {
+                    it = vmIndex.emplace(key, (int)hvmats.size()).first;
+                    hvmats.push_back(m.vmats[k]);
+                    vmatE.push_back(m.vmatE[k]);
+                }
+                vmRemap[k] = it->second;
+            }
+            for (size_t k = 0; k < m.lmats.size(); k++) {
+                std::string key((const char *)&m.lmats[k], sizeof(LinkMat));
+                auto it = lmIndex.find(key);
+                if (it == lmIndex.end()) {
+                    it = lmIndex.emplace(key, (int)lmats.size()).first;
+                    lmats.push_back(m.lmats[k]);
+                }
+                lmRemap[k] = it->second;
+            }
             for (int v = 0; v < m.nv(); v++) {
                 lat.push_back(m.ix[v] | (m.iy[v] << 10) | (m.iz[v] << 20));
-                vmat.push_back(vmoff + m.vmat[v]);
+                vmat.push_back(vmRemap[m.vmat[v]]);
                 vrob.push_back(r);
                 phase.push_back((T)m.phase[v]);
                 for (int k = 0; k < 6; k++) {
@@ -417,11 +445,10 @@ template <typename T> struct Batch : BatchBase {
             }
             for (int l = 0; l < m.nl(); l++) {
                 lends.push_back(int2{voff + m.lneg[l], voff + m.lpos[l]});
-                lmeta.push_back(int2{m.laxis[l] | ((lmoff + m.lmat[l]) << 2), r});
+                lmeta.push_back(int2{m.laxis[l] | (lmRemap[m.lmat[l]] << 2), r});
             }
             float dflt = (float)(m.dtFrac * std::max(recommended_dt(m), 1e-10));
             defaultDt.push_back(dflt);
-            (void)lmoff;
         }
         V = (int)lat.size();
         L = (int)lends.size();
@@ -535,6 +562,61 @@ template <typename T> struct Batch : BatchBase {
         d.lends = lendsd; d.lmeta = lmetad;
         d.ls0 = ls0; d.ls1 = ls1; d.ls2 = ls2; d.lf0 = lf0; d.lf1 = lf1; d.lf2 = lf2;
         d.vmats = d_vmats; d.lmats = d_lmats; d.rp = d_rp; d.rt = d_rt;
+        return plan_tiles();
+    }
+
+    // Longest-processing-time bin packing of robots into tiles (k_tile): at
+    // least one tile per compute unit, robots sorted by link count, each placed
+    // in the least-loaded tile that still has room.
+    int plan_tiles() {
+        using C = TileCfg<T>;
+        tileOK = false;
+        if (anyCollisions || R < 2) return VX3MI_OK;
+        long totL = 0, totV = 0;
+        for (const RobotParams &p : hrp) {
+            if (p.vcount > C::VMAX || p.lcount > C::LMAX) return VX3MI_OK;
+            totL += p.lcount;
+            totV += p.vcount;
+        }
+        int dev = 0, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            hipDeviceProp_t prop;
+            if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
+        }
+        long want = std::max<long>({(long)cus, (totL + C::LMAX - 1) / C::LMAX, (totV + C::VMAX - 1) / C::VMAX});
+        want = std::min<long>(want, R);
+        std::vector<int> order(R);
+        for (int r = 0; r < R; r++) order[r] = r;
+        std::stable_sort(order.begin(), order.end(), [&](int x, int y) { return hrp[x].lcount > hrp[y].lcount; });
+        struct Bin { long l = 0, v = 0; std::vector<int> robots; };
+        std::vector<Bin> bins((size_t)want);
+        for (int r : order) {
+            const RobotParams &p = hrp[r];
+            int best = -1;
+            for (int b = 0; b < (int)bins.size(); b++) {
+                const Bin &x = bins[b];
+                if (x.l + p.lcount > C::LMAX || x.v + p.vcount > C::VMAX || (int)x.robots.size() >= C::RMAX) continue;
+                if (best < 0 || x.l + x.v < bins[best].l + bins[best].v) best = b;
+            }
+            if (best < 0) { bins.emplace_back(); best = (int)bins.size() - 1; }
+            bins[best].l += p.lcount;
+            bins[best].v += p.vcount;
+            bins[best].robots.push_back(r);
+        }
+        std::vector<int4> tiles;
+        std::vector<int> trobots;
+        for (Bin &b : bins) {
+            if (b.robots.empty()) continue;
+            std::sort(b.robots.begin(), b.robots.end());
+            tiles.push_back(int4{(int)trobots.size(), (int)b.robots.size(), (int)b.v, (int)b.l});
+            for (int r : b.robots) trobots.push_back(r);
+        }
+        ntiles = (int)tiles.size();
+        int rc;
+        if ((rc = alloc(&d_tiles, tiles.size())) || (rc = alloc(&d_tileRobots, trobots.size()))) return rc;
+        HIP_TRY(hipMemcpy(d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(d_tileRobots, trobots.data(), sizeof(int) * trobots.size(), hipMemcpyHostToDevice));
+        tileOK = true;
         return VX3MI_OK;
     }
 
@@ -563,7 +645,12 @@ template <typename T> struct Batch : BatchBase {
         if (dt == 0.0f) return VX3MI_OK;  // doTimeStep(0) only advances the step counter
         int rc = set_dt(dt);
         if (rc) return rc;
-        if (mode == VX3MI_MODE_AUTO) mode = (R > 1 && V / R <= 4096) ? VX3MI_MODE_PERSISTENT : VX3MI_MODE_FLAT;
+        if (mode == VX3MI_MODE_TILED && !tileOK) {
+            set_error("tiled mode needs collisions off and every robot within one tile");
+            return VX3MI_E_ARG;
+        }
+        if (mode == VX3MI_MODE_AUTO)
+            mode = tileOK ? VX3MI_MODE_TILED : ((R > 1 && V / R <= 4096) ? VX3MI_MODE_PERSISTENT : VX3MI_MODE_FLAT);
         HIP_TRY(hipEventRecord(ev0, stream));
         launches = 0;
         lastMode = mode;
@@ -578,6 +665,13 @@ template <typename T> struct Batch : BatchBase {
                     launches += 2 + (p.scount ? 2 : 0);
                 }
                 parity ^= 1;
+            }
+        } else if (mode == VX3MI_MODE_TILED) {
+            for (int s = 0; s < nsteps; s += persistentChunk) {
+                const int k = std::min(persistentChunk, nsteps - s);
+                k_tile<T><<<ntiles, TileCfg<T>::NT, 0, stream>>>(d, d_tiles, d_tileRobots, k, parity);
+                parity = (parity + k) & 1;
+                launches++;
             }
         } else if (mode == VX3MI_MODE_PERSISTENT) {
             // launches of at most persistentChunk steps keep each kernel short

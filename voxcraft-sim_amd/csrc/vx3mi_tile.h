@@ -1,0 +1,243 @@
+// vx3mi_tile.h -- the tiled persistent timestep for batches of small robots.
+//
+// Robots are bin-packed (longest processing time first, by link count) into
+// tiles of at most LMAX links and VMAX voxels; one workgroup of NT threads owns
+// a tile for a whole launch of many steps.  Each thread owns up to KL links and
+// KV voxels and keeps their state in registers across steps; the only data
+// exchanged between threads are the voxel poses (position field, quaternion,
+// temperature) and the link-end forces, both in LDS as structure of arrays.
+// Global memory is read at the start of a launch and written at its end.
+// The per-link and per-voxel arithmetic is the same as k_links / k_voxels
+// (link_forces / voxel_step, link ends summed in direction order), so every
+// launch mode produces the same bits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace vx3mi {
+
+template <typename T> struct TileCfg;
+template <> struct TileCfg<float> {
+    static constexpr int NT = 512, KL = 4, KV = 3, LMAX = 2048, VMAX = 1536, RMAX = 128;
+};
+template <> struct TileCfg<double> {
+    static constexpr int NT = 512, KL = 2, KV = 2, LMAX = 1024, VMAX = 768, RMAX = 64;
+};
+
+// LDS robot table of a tile
+template <int RMAX> struct TileRobots {
+    int rid[RMAX];
+    int voffG[RMAX], loffG[RMAX];
+    int vpre[RMAX + 1], lpre[RMAX + 1];
+    double time[2][RMAX];
+    float prevDt[2][RMAX];
+    float dt[RMAX];
+    int diverged[RMAX];
+    int steps[RMAX];
+};
+
+// largest j with pre[j] <= x (pre ascending, pre[0] = 0, pre[n] = total)
+__device__ __forceinline__ int tile_slot(const int *pre, int n, int x) {
+    int lo = 0, hi = n - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (pre[mid] <= x) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <typename T>
+__global__ __launch_bounds__(TileCfg<T>::NT) void k_tile(Dev<T> a, const int4 *tiles, const int *tileRobots, int nsteps,
+                                                         int parity0) {
+    constexpr int NT = TileCfg<T>::NT, KL = TileCfg<T>::KL, KV = TileCfg<T>::KV;
+    constexpr int LMAX = TileCfg<T>::LMAX, VMAX = TileCfg<T>::VMAX, RMAX = TileCfg<T>::RMAX;
+    __shared__ T sPose[8][VMAX];    // position field xyz, qw, qx, qy, qz, temperature
+    __shared__ T sForce[12][LMAX];  // forceNeg, momentNeg, forcePos, momentPos
+    __shared__ TileRobots<RMAX> tr;
+
+    const int tid = threadIdx.x;
+    const int4 info = tiles[blockIdx.x];  // robot begin, robot count, voxels, links
+    const int nr = info.y, nv = info.z, nl = info.w;
+
+    if (tid < nr) {
+        const int r = tileRobots[info.x + tid];
+        const RobotParams &p = a.rp[r];
+        const RobotTime &t = a.rt[r];
+        tr.rid[tid] = r;
+        tr.voffG[tid] = p.voff;
+        tr.loffG[tid] = p.loff;
+        tr.vpre[tid + 1] = p.vcount;
+        tr.lpre[tid + 1] = p.lcount;
+        tr.time[0][tid] = t.time[0];
+        tr.time[1][tid] = t.time[1];
+        tr.prevDt[0][tid] = t.prevDt[0];
+        tr.prevDt[1][tid] = t.prevDt[1];
+        tr.dt[tid] = t.dt;
+        tr.diverged[tid] = t.diverged;
+        tr.steps[tid] = t.steps;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        tr.vpre[0] = 0;
+        tr.lpre[0] = 0;
+        for (int j = 0; j < nr; j++) {
+            tr.vpre[j + 1] += tr.vpre[j];
+            tr.lpre[j + 1] += tr.lpre[j];
+        }
+    }
+    __syncthreads();
+
+    // ---- owned links: static description and state in registers
+    int lGlob[KL], lEnds[KL], lMeta[KL], lMat[KL];  // global id; neg | pos << 16 (local); axis | slot << 2
+    float lStrain[KL];
+    LinkState<T> ls[KL];
+#pragma unroll
+    for (int k = 0; k < KL; k++) {
+        const int ll = tid + k * NT;
+        lGlob[k] = -1;
+        if (ll < nl) {
+            const int j = tile_slot(tr.lpre, nr, ll);
+            const int gl = tr.loffG[j] + (ll - tr.lpre[j]);
+            const int2 e = a.lends[gl];
+            const int base = tr.vpre[j] - tr.voffG[j];
+            lGlob[k] = gl;
+            lEnds[k] = (e.x + base) | ((e.y + base) << 16);
+            lMeta[k] = (a.lmeta[gl].x & 3) | (j << 2);
+            lMat[k] = a.lmeta[gl].x >> 2;
+            const T4<T> s0 = a.ls0[gl], s1 = a.ls1[gl], s2 = a.ls2[gl];
+            lStrain[k] = (float)s2.w;
+            ls[k].pos2 = {s0.x, s0.y, s0.z};
+            ls[k].a1v = {s0.w, s1.x, s1.y};
+            ls[k].a2v = {s1.z, s1.w, s2.x};
+            ls[k].maxStrain = (float)s2.y;
+            ls[k].flags = (int)s2.z;
+        }
+    }
+    // ---- owned voxels
+    int vGlob[KV], vSlot[KV], vMat[KV], vIz[KV];
+    int vLinks[KV][3];  // local link ids, two 16-bit fields each (0xffff = none)
+    T vPhase[KV];
+    VoxelState<T> vs[KV];
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+        const int lv = tid + k * NT;
+        vGlob[k] = -1;
+        if (lv < nv) {
+            const int j = tile_slot(tr.vpre, nr, lv);
+            const int gv = tr.voffG[j] + (lv - tr.vpre[j]);
+            vGlob[k] = gv;
+            vSlot[k] = j;
+            vMat[k] = a.vmat[gv];
+            vIz[k] = a.lat[gv] >> 20;
+            vPhase[k] = a.phase[gv];
+            const int lbase = tr.lpre[j] - tr.loffG[j];
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                const int l0 = a.vlinks[6 * gv + 2 * d], l1 = a.vlinks[6 * gv + 2 * d + 1];
+                vLinks[k][d] = (l0 < 0 ? 0xffff : l0 + lbase) | ((l1 < 0 ? 0xffff : l1 + lbase) << 16);
+            }
+            const T4<T> q0 = a.pq0[gv], q1 = a.pq1[gv], m0 = a.mo0[gv], m1 = a.mo1[gv];
+            vs[k].u = {q0.x, q0.y, q0.z};
+            vs[k].q = {q0.w, q1.x, q1.y, q1.z};
+            vs[k].p = {m0.x, m0.y, m0.z};
+            vs[k].h = {m0.w, m1.x, m1.y};
+            vs[k].flags = (int)m1.z;
+            sPose[0][lv] = q0.x; sPose[1][lv] = q0.y; sPose[2][lv] = q0.z; sPose[3][lv] = q0.w;
+            sPose[4][lv] = q1.x; sPose[5][lv] = q1.y; sPose[6][lv] = q1.z; sPose[7][lv] = q1.w;
+        }
+    }
+    __syncthreads();
+
+    for (int s = 0; s < nsteps; s++) {
+        const int parity = (parity0 + s) & 1;
+        // ---- links
+#pragma unroll
+        for (int k = 0; k < KL; k++) {
+            const int ll = tid + k * NT;
+            if (lGlob[k] < 0) continue;
+            const int j = lMeta[k] >> 2;
+            if (tr.diverged[j]) continue;
+            const LinkMat &lm = a.lmats[lMat[k]];
+            if (lm.fixedBoth) continue;
+            const int n = lEnds[k] & 0xffff, p = lEnds[k] >> 16;
+            const V3<T> du = {sPose[0][p] - sPose[0][n], sPose[1][p] - sPose[1][n], sPose[2][p] - sPose[2][n]};
+            const Q4<T> qn = {sPose[3][n], sPose[4][n], sPose[5][n], sPose[6][n]};
+            const Q4<T> qp = {sPose[3][p], sPose[4][p], sPose[5][p], sPose[6][p]};
+            LinkOut<T> o;
+            link_forces<T>(lm, lMeta[k] & 3, du, qn, qp, (float)sPose[7][n], (float)sPose[7][p], tr.prevDt[parity][j],
+                           ls[k], o);
+            sForce[0][ll] = o.fN.x; sForce[1][ll] = o.fN.y; sForce[2][ll] = o.fN.z;
+            sForce[3][ll] = o.mN.x; sForce[4][ll] = o.mN.y; sForce[5][ll] = o.mN.z;
+            sForce[6][ll] = o.fP.x; sForce[7][ll] = o.fP.y; sForce[8][ll] = o.fP.z;
+            sForce[9][ll] = o.mP.x; sForce[10][ll] = o.mP.y; sForce[11][ll] = o.mP.z;
+            lStrain[k] = o.strain;
+            if (o.strain > 100.0f) tr.diverged[j] = 1;
+        }
+        __syncthreads();
+        // ---- voxels
+#pragma unroll
+        for (int k = 0; k < KV; k++) {
+            const int lv = tid + k * NT;
+            if (vGlob[k] < 0) continue;
+            const int j = vSlot[k];
+            if (tr.diverged[j]) continue;
+            const float dt = tr.dt[j];
+            const double tnext = tr.time[parity][j] + (double)dt;
+            if (lv == tr.vpre[j]) {
+                tr.time[parity ^ 1][j] = tnext;
+                tr.prevDt[parity ^ 1][j] = dt;
+                tr.steps[j] += 1;
+            }
+            const VoxelMat &vm = a.vmats[vMat[k]];
+            if (vm.fixed) continue;
+            V3<T> fs = {0, 0, 0}, ms = {0, 0, 0};
+#pragma unroll
+            for (int d = 0; d < 6; d++) {
+                const int l = (vLinks[k][d >> 1] >> (16 * (d & 1))) & 0xffff;
+                if (l == 0xffff) continue;
+                const int o = (d & 1) ? 6 : 0;  // NEG direction: this voxel is the pVPos end
+                fs = fs + V3<T>{sForce[o + 0][l], sForce[o + 1][l], sForce[o + 2][l]};
+                ms = ms + V3<T>{sForce[o + 3][l], sForce[o + 4][l], sForce[o + 5][l]};
+            }
+            float temp = (float)sPose[7][lv];
+            voxel_step<T>(vm, dt, vIz[k], temp, fs, ms, V3<T>{0, 0, 0}, vs[k]);
+            const RobotParams &rp = a.rp[tr.rid[j]];
+            if (rp.varyTemp && !(vm.thermalOnAfter > tnext))
+                temp = actuation_temperature<T>(rp.tempAmp, rp.tempPeriod, tnext, (double)vPhase[k], rp.enableExpansion);
+            sPose[0][lv] = vs[k].u.x; sPose[1][lv] = vs[k].u.y; sPose[2][lv] = vs[k].u.z; sPose[3][lv] = vs[k].q.w;
+            sPose[4][lv] = vs[k].q.x; sPose[5][lv] = vs[k].q.y; sPose[6][lv] = vs[k].q.z; sPose[7][lv] = (T)temp;
+        }
+        __syncthreads();
+    }
+
+    // ---- write back
+#pragma unroll
+    for (int k = 0; k < KL; k++) {
+        if (lGlob[k] < 0) continue;
+        const int gl = lGlob[k];
+        a.ls0[gl] = T4<T>{ls[k].pos2.x, ls[k].pos2.y, ls[k].pos2.z, ls[k].a1v.x};
+        a.ls1[gl] = T4<T>{ls[k].a1v.y, ls[k].a1v.z, ls[k].a2v.x, ls[k].a2v.y};
+        a.ls2[gl] = T4<T>{ls[k].a2v.z, (T)ls[k].maxStrain, (T)ls[k].flags, (T)lStrain[k]};
+    }
+#pragma unroll
+    for (int k = 0; k < KV; k++) {
+        if (vGlob[k] < 0) continue;
+        const int gv = vGlob[k], lv = tid + k * NT;
+        a.pq0[gv] = T4<T>{vs[k].u.x, vs[k].u.y, vs[k].u.z, vs[k].q.w};
+        a.pq1[gv] = T4<T>{vs[k].q.x, vs[k].q.y, vs[k].q.z, sPose[7][lv]};
+        a.mo0[gv] = T4<T>{vs[k].p.x, vs[k].p.y, vs[k].p.z, vs[k].h.x};
+        a.mo1[gv] = T4<T>{vs[k].h.y, vs[k].h.z, (T)vs[k].flags, (T)0};
+    }
+    if (tid < nr) {
+        RobotTime &t = a.rt[tr.rid[tid]];
+        t.time[0] = tr.time[0][tid];
+        t.time[1] = tr.time[1][tid];
+        t.prevDt[0] = tr.prevDt[0][tid];
+        t.prevDt[1] = tr.prevDt[1][tid];
+        t.diverged = tr.diverged[tid];
+        t.steps = tr.steps[tid];
+    }
+}
+
+}  // namespace vx3mi

@@ -1,17 +1,553 @@
-// vx3mi_vxa.cpp -- VXA/VXD reader (placeholder until the XML reader lands).
+// vx3mi_vxa.cpp -- VXA/VXD input: a small XML reader, the VXD merge rule and the
+// VXA semantics of the reference's model construction path.
+//
+//   merge      ctool::ptree_merge (src/Utils/ctool.h:49-57): every child of
+//              <VXD> with a replace="VXA.a.b" attribute replaces the subtree at
+//              that path of the base VXA (parents are created when missing)
+//   simulator  CVX_Sim::ReadXML (src/VXA/VX_Sim.cpp:185-294) and the VX3 keys
+//              read by VX3_SimulationManager::readVXD (VX3_SimulationManager.cu:323-376)
+//   environment CVX_Environment::ReadXML (src/old/VX_Environment.cpp:101-171)
+//   palette    CVXC_Material::ReadXML (src/VXA/VX_Object.cpp:1373-1475)
+//   structure  CVXC_Structure::ReadXML (src/VXA/VX_Object.cpp:1760-1958)
+#include <cctype>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <new>
+#include <sstream>
+#include <string>
+#include <vector>
+
 #include "vx3mi_internal.h"
+
+namespace vx3mi {
+namespace {
+
+struct XNode {
+    std::string name, text;
+    bool cdata = false;
+    std::vector<std::pair<std::string, std::string>> attrs;
+    std::vector<XNode> kids;
+    const XNode *child(const std::string &n) const {
+        for (const XNode &k : kids)
+            if (k.name == n) return &k;
+        return nullptr;
+    }
+    const char *attr(const std::string &n) const {
+        for (const auto &a : attrs)
+            if (a.first == n) return a.second.c_str();
+        return nullptr;
+    }
+};
+
+struct XmlError {
+    std::string msg;
+};
+
+class XmlReader {
+  public:
+    explicit XmlReader(const char *s) : p_(s) {}
+    XNode parse_document() {
+        XNode doc;
+        doc.name = "";
+        skip_misc();
+        while (*p_) {
+            if (*p_ != '<') throw XmlError{"text outside the root element"};
+            doc.kids.push_back(element());
+            skip_misc();
+        }
+        if (doc.kids.empty()) throw XmlError{"no root element"};
+        return doc;
+    }
+
+  private:
+    const char *p_;
+
+    static bool name_char(char c) { return std::isalnum((unsigned char)c) || c == '_' || c == '-' || c == '.' || c == ':'; }
+    void ws() {
+        while (*p_ && std::isspace((unsigned char)*p_)) p_++;
+    }
+    bool starts(const char *t) const { return std::strncmp(p_, t, std::strlen(t)) == 0; }
+    void skip_until(const char *t) {
+        const char *e = std::strstr(p_, t);
+        if (!e) throw XmlError{std::string("unterminated construct, expected ") + t};
+        p_ = e + std::strlen(t);
+    }
+    void skip_misc() {
+        for (;;) {
+            ws();
+            if (starts("<?")) skip_until("?>");
+            else if (starts("<!--")) skip_until("-->");
+            else if (starts("<!DOCTYPE")) skip_until(">");
+            else return;
+        }
+    }
+    std::string name() {
+        const char *b = p_;
+        while (*p_ && name_char(*p_)) p_++;
+        if (p_ == b) throw XmlError{"expected a name"};
+        return std::string(b, p_);
+    }
+    static void decode_into(const char *b, const char *e, std::string &out) {
+        while (b < e) {
+            if (*b == '&') {
+                const char *semi = (const char *)std::memchr(b, ';', e - b);
+                if (!semi) { out.push_back(*b++); continue; }
+                std::string ent(b + 1, semi);
+                if (ent == "lt") out.push_back('<');
+                else if (ent == "gt") out.push_back('>');
+                else if (ent == "amp") out.push_back('&');
+                else if (ent == "quot") out.push_back('"');
+                else if (ent == "apos") out.push_back('\'');
+                else if (!ent.empty() && ent[0] == '#') {
+                    long c = ent.size() > 1 && (ent[1] == 'x' || ent[1] == 'X') ? std::strtol(ent.c_str() + 2, nullptr, 16)
+                                                                                : std::strtol(ent.c_str() + 1, nullptr, 10);
+                    out.push_back((char)c);
+                } else {
+                    out.append(b, semi + 1);
+                }
+                b = semi + 1;
+            } else {
+                out.push_back(*b++);
+            }
+        }
+    }
+    XNode element() {
+        XNode n;
+        p_++;  // '<'
+        n.name = name();
+        for (;;) {
+            ws();
+            if (*p_ == '/') {
+                if (p_[1] != '>') throw XmlError{"bad empty-element tag"};
+                p_ += 2;
+                return n;
+            }
+            if (*p_ == '>') { p_++; break; }
+            std::string an = name();
+            ws();
+            if (*p_ != '=') throw XmlError{"attribute without value"};
+            p_++;
+            ws();
+            char q = *p_;
+            if (q != '"' && q != '\'') throw XmlError{"unquoted attribute"};
+            const char *b = ++p_;
+            while (*p_ && *p_ != q) p_++;
+            if (!*p_) throw XmlError{"unterminated attribute"};
+            std::string v;
+            decode_into(b, p_, v);
+            p_++;
+            n.attrs.emplace_back(an, v);
+        }
+        std::string text;
+        for (;;) {
+            if (!*p_) throw XmlError{"unterminated element " + n.name};
+            if (starts("<![CDATA[")) {
+                p_ += 9;
+                const char *e = std::strstr(p_, "]]>");
+                if (!e) throw XmlError{"unterminated CDATA"};
+                text.append(p_, e);
+                n.cdata = true;
+                p_ = e + 3;
+            } else if (starts("<!--")) {
+                skip_until("-->");
+            } else if (starts("<?")) {
+                skip_until("?>");
+            } else if (starts("</")) {
+                p_ += 2;
+                std::string close = name();
+                if (close != n.name) throw XmlError{"mismatched </" + close + "> for <" + n.name + ">"};
+                ws();
+                if (*p_ != '>') throw XmlError{"bad end tag"};
+                p_++;
+                break;
+            } else if (*p_ == '<') {
+                n.kids.push_back(element());
+            } else {
+                const char *b = p_;
+                while (*p_ && *p_ != '<') p_++;
+                decode_into(b, p_, text);
+            }
+        }
+        if (n.cdata) n.text = text;
+        else {
+            size_t a = 0, e = text.size();
+            while (a < e && std::isspace((unsigned char)text[a])) a++;
+            while (e > a && std::isspace((unsigned char)text[e - 1])) e--;
+            n.text = text.substr(a, e - a);
+        }
+        return n;
+    }
+};
+
+// ctool::ptree_merge: put the VXD child at the dotted path of the base tree
+void merge_vxd(XNode &doc, const XNode &vxdDoc) {
+    const XNode *vxd = vxdDoc.child("VXD");
+    if (!vxd) vxd = &vxdDoc;
+    for (const XNode &k : vxd->kids) {
+        const char *rep = k.attr("replace");
+        if (!rep || !*rep) continue;
+        std::string path(rep);
+        XNode *cur = &doc;
+        size_t b = 0;
+        std::string last;
+        for (;;) {
+            size_t e = path.find('.', b);
+            std::string comp = path.substr(b, e == std::string::npos ? std::string::npos : e - b);
+            XNode *next = nullptr;
+            for (XNode &c : cur->kids)
+                if (c.name == comp) { next = &c; break; }
+            if (!next) {
+                cur->kids.emplace_back();
+                cur->kids.back().name = comp;
+                next = &cur->kids.back();
+            }
+            if (e == std::string::npos) {
+                XNode repl = k;
+                repl.name = comp;
+                *next = repl;
+                break;
+            }
+            cur = next;
+            b = e + 1;
+        }
+    }
+}
+
+const XNode *path(const XNode *n, std::initializer_list<const char *> comps) {
+    for (const char *c : comps) {
+        if (!n) return nullptr;
+        n = n->child(c);
+    }
+    return n;
+}
+
+// FindLoadElement conversions (src/old/XML_Rip.h:73-77): atof / atoi, bool = atoi != 0
+bool load(const XNode *parent, const char *key, double &out) {
+    const XNode *n = parent ? parent->child(key) : nullptr;
+    if (!n) return false;
+    out = std::atof(n->text.c_str());
+    return true;
+}
+bool load(const XNode *parent, const char *key, float &out) {
+    double d;
+    if (!load(parent, key, d)) return false;
+    out = (float)d;
+    return true;
+}
+bool load(const XNode *parent, const char *key, int &out) {
+    const XNode *n = parent ? parent->child(key) : nullptr;
+    if (!n) return false;
+    out = std::atoi(n->text.c_str());
+    return true;
+}
+
+std::string base64_decode(const std::string &in) {
+    auto val = [](char c) -> int {
+        if (c >= 'A' && c <= 'Z') return c - 'A';
+        if (c >= 'a' && c <= 'z') return c - 'a' + 26;
+        if (c >= '0' && c <= '9') return c - '0' + 52;
+        if (c == '+') return 62;
+        if (c == '/') return 63;
+        return -1;
+    };
+    std::string out;
+    int buf = 0, bits = 0;
+    for (char c : in) {
+        int v = val(c);
+        if (v < 0) continue;
+        buf = (buf << 6) | v;
+        bits += 6;
+        if (bits >= 8) {
+            bits -= 8;
+            out.push_back((char)((buf >> bits) & 0xff));
+        }
+    }
+    return out;
+}
+
+std::vector<std::string> split_commas(const std::string &s) {
+    std::vector<std::string> out;
+    size_t b = 0;
+    for (;;) {
+        size_t e = s.find(',', b);
+        out.push_back(s.substr(b, e == std::string::npos ? std::string::npos : e - b));
+        if (e == std::string::npos) break;
+        b = e + 1;
+    }
+    return out;
+}
+
+struct VxaContent {
+    vx3mi_lattice lat{};
+    std::vector<signed char> cells;
+    std::vector<double> phase;
+    std::vector<vx3mi_material> mats;
+    std::string stop, fitness;
+    int recordStepSize = 0, recordVoxel = 1, recordLink = 0;
+};
+
+// serialise a MathTree subtree back to text (kept for the runner)
+void dump(const XNode &n, std::string &out) {
+    out += "<" + n.name + ">";
+    out += n.text;
+    for (const XNode &k : n.kids) dump(k, out);
+    out += "</" + n.name + ">";
+}
+
+int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
+    const XNode *vxa = doc.child("VXA");
+    if (!vxa) { err = "no <VXA> root element"; return VX3MI_E_PARSE; }
+    vx3mi_lattice &L = c.lat;
+    std::memset(&L, 0, sizeof(L));
+    // ---- simulator (CVX_Sim::ReadXML; DtFrac defaults to 0.9 in CVX_Sim)
+    L.dt_frac = 0.9;
+    // damping: the reference leaves the zetas uninitialised without <Damping>; the
+    // ReadXML defaults are used here
+    L.bond_damp_z = 0.1f;
+    L.col_damp_z = 1.0f;
+    L.slow_damp_z = 1.0f;
+    const XNode *sim = vxa->child("Simulator");
+    if (sim) {
+        if (const XNode *integ = sim->child("Integration")) {
+            if (!load(integ, "DtFrac", L.dt_frac)) L.dt_frac = 0.9;
+        }
+        if (const XNode *damp = sim->child("Damping")) {
+            if (!load(damp, "BondDampingZ", L.bond_damp_z)) L.bond_damp_z = 0.1f;
+            if (!load(damp, "ColDampingZ", L.col_damp_z)) L.col_damp_z = 1.0f;
+            if (!load(damp, "SlowDampingZ", L.slow_damp_z)) L.slow_damp_z = 1.0f;
+        }
+        if (const XNode *feat = sim->child("Features")) {
+            int v = 0;
+            if (load(feat, "VolumeEffectsEnabled", v)) L.volume_effects = v != 0;
+        }
+        L.enable_collision = 1;  // VX3_SimulationManager.cu:328 default true
+        if (const XNode *ad = sim->child("AttachDetach")) {
+            int v;
+            if (load(ad, "EnableCollision", v)) L.enable_collision = v != 0;
+            if (load(ad, "EnableAttach", v) && v) { err = "EnableAttach is not supported"; return VX3MI_E_PARSE; }
+            if (load(ad, "EnableDetach", v) && v) { err = "EnableDetach is not supported"; return VX3MI_E_PARSE; }
+        }
+        int v;
+        if (load(sim, "EnableExpansion", v)) L.enable_expansion = v;
+        if (load(sim, "EnableCilia", v) && v) { err = "EnableCilia is not supported"; return VX3MI_E_PARSE; }
+        if (load(sim, "EnableSignals", v) && v) { err = "EnableSignals is not supported"; return VX3MI_E_PARSE; }
+        if (const XNode *rh = sim->child("RecordHistory")) {
+            load(rh, "RecordStepSize", c.recordStepSize);
+            load(rh, "RecordVoxel", c.recordVoxel);
+            load(rh, "RecordLink", c.recordLink);
+        }
+        if (const XNode *f = path(sim, {"StopCondition", "StopConditionFormula"})) dump(*f, c.stop);
+        if (const XNode *f = sim->child("FitnessFunction")) dump(*f, c.fitness);
+        if (const XNode *ff = sim->child("ForceField")) {
+            for (const XNode &k : ff->kids) {
+                bool zero = k.kids.size() == 1 && k.kids[0].name == "mtCONST" && std::atof(k.kids[0].text.c_str()) == 0.0;
+                if (!zero && !k.kids.empty()) { err = "non-zero ForceField is not supported"; return VX3MI_E_PARSE; }
+            }
+        }
+    } else {
+        L.enable_collision = 1;
+    }
+    // ---- environment (CVX_Environment ctor defaults, src/old/VX_Environment.cpp:22-35)
+    L.grav_acc = -9.81f;
+    L.floor_enabled = 1;
+    L.temp_enabled = 1;
+    L.vary_temp = 0;
+    L.temp_amplitude = 0;
+    L.temp_base = 25;
+    L.temp_period = 0.1;
+    if (const XNode *env = vxa->child("Environment")) {
+        if (const XNode *bc = env->child("Boundary_Conditions")) {
+            int nbc = 0;
+            if (load(bc, "NumBCs", nbc) && nbc > 0) { err = "boundary conditions are not supported"; return VX3MI_E_PARSE; }
+        }
+        if (const XNode *g = env->child("Gravity")) {
+            int v;
+            double d;
+            // GravEnabled is read but inert: SetGravityAccel always follows (VX_Sim.cpp:67-68, 485)
+            L.grav_acc = load(g, "GravAcc", d) ? (float)d : -9.81f;
+            L.floor_enabled = load(g, "FloorEnabled", v) ? (v != 0) : 0;
+        }
+        if (const XNode *t = env->child("Thermal")) {
+            int v;
+            double d;
+            L.temp_enabled = load(t, "TempEnabled", v) ? (v != 0) : 0;
+            L.temp_base = load(t, "TempBase", d) ? d : 25.0;
+            if (load(t, "TempAmplitude", d)) L.temp_amplitude = d;
+            else if (load(t, "TempAmp", d)) L.temp_amplitude = d - L.temp_base;
+            else L.temp_amplitude = 0;
+            L.vary_temp = load(t, "VaryTempEnabled", v) ? (v != 0) : 0;
+            L.temp_period = load(t, "TempPeriod", d) ? d : 0.1;
+        }
+    }
+    // ---- VXC
+    const XNode *vxc = vxa->child("VXC");
+    if (!vxc) vxc = vxa->child("DMF");
+    if (!vxc) { err = "no <VXC> element"; return VX3MI_E_PARSE; }
+    L.lattice_dim = 0.001;
+    double xadj = 1.0;
+    if (const XNode *lat = vxc->child("Lattice")) {
+        if (!load(lat, "Lattice_Dim", L.lattice_dim)) L.lattice_dim = 0.001;
+        if (!load(lat, "X_Dim_Adj", xadj)) xadj = 1.0;
+    }
+    L.lattice_dim *= xadj;  // Vx.setVoxelSize(LocalVXC.GetLatDimEnv().x) (VX_Sim.cpp:88)
+    if (const XNode *pal = vxc->child("Palette")) {
+        for (const XNode &mn : pal->kids) {
+            if (mn.name != "Material") continue;
+            vx3mi_material m;
+            std::memset(&m, 0, sizeof(m));
+            m.red = m.green = m.blue = 0.5f;
+            m.alpha = 1.0f;
+            m.is_measured = 1;
+            int matType = 0;
+            if (load(&mn, "MatType", matType) && matType != 0) { err = "only MatType 0 (single) materials are supported"; return VX3MI_E_PARSE; }
+            if (const XNode *disp = mn.child("Display")) {
+                load(disp, "Red", m.red);
+                load(disp, "Green", m.green);
+                load(disp, "Blue", m.blue);
+                load(disp, "Alpha", m.alpha);
+            }
+            if (const XNode *mech = mn.child("Mechanical")) {
+                int model = 0, iv;
+                load(mech, "MatModel", model);
+                if (model != 0 && model != 1) { err = "only MatModel 0 and 1 are supported"; return VX3MI_E_PARSE; }
+                m.mat_model = model;
+                load(mech, "Elastic_Mod", m.elastic_mod);
+                load(mech, "Fail_Stress", m.fail_stress);
+                load(mech, "Density", m.density);
+                load(mech, "Poissons_Ratio", m.poissons_ratio);
+                load(mech, "CTE", m.cte);
+                load(mech, "uStatic", m.u_static);
+                load(mech, "uDynamic", m.u_dynamic);
+                if (load(mech, "isTarget", iv)) m.is_target = iv != 0;
+                if (load(mech, "Fixed", iv)) m.fixed = iv != 0;
+                if (load(mech, "isMeasured", iv)) m.is_measured = iv;
+                load(mech, "TurnOnThermalExpansionAfterThisManySeconds", m.thermal_on_after);
+            }
+            c.mats.push_back(m);
+        }
+    }
+    const XNode *st = vxc->child("Structure");
+    if (!st) { err = "no <Structure> element"; return VX3MI_E_PARSE; }
+    int nx = 1, ny = 1, nz = 1;
+    load(st, "X_Voxels", nx);
+    load(st, "Y_Voxels", ny);
+    load(st, "Z_Voxels", nz);
+    if (nx <= 0 || ny <= 0 || nz <= 0) { err = "bad structure dimensions"; return VX3MI_E_PARSE; }
+    const char *comp = st->attr("Compression");
+    std::string compression = comp ? comp : "";
+    const XNode *data = st->child("Data");
+    if (!data) { err = "no <Data> in <Structure>"; return VX3MI_E_PARSE; }
+    c.cells.assign((size_t)nx * ny * nz, 0);
+    int z = 0;
+    for (const XNode &layer : data->kids) {
+        if (layer.name != "Layer") continue;
+        if (z >= nz) break;
+        std::string raw = layer.text, in;
+        if (compression == "ASCII_READABLE") {
+            for (char ch : raw) in.push_back((char)(ch - 48));
+        } else if (compression == "RAW_DATA") {
+            in = raw;
+        } else if (compression == "ZLIB" || compression == "QT_ZLIB") {
+            err = "ZLIB-compressed layers are not supported (nor by the reference build)";
+            return VX3MI_E_PARSE;
+        } else {
+            in = base64_decode(raw);
+        }
+        if ((long)in.size() != (long)nx * ny) { err = "voxel layer data does not match X_Voxels*Y_Voxels"; return VX3MI_E_PARSE; }
+        for (int k = 0; k < nx * ny; k++) c.cells[(size_t)nx * ny * z + k] = (signed char)in[k];
+        z++;
+    }
+    if (z < nz) { err = "fewer voxel layers than Z_Voxels"; return VX3MI_E_PARSE; }
+    // PhaseOffset layers: one value per cell, kept for cells holding a voxel
+    if (const XNode *po = st->child("PhaseOffset")) {
+        int zz = 0;
+        for (const XNode &layer : po->kids) {
+            if (layer.name != "Layer") continue;
+            if (zz >= nz) break;
+            std::vector<std::string> vals = split_commas(layer.text);
+            for (int k = 0; k < nx * ny; k++) {
+                if (c.cells[(size_t)nx * ny * zz + k] > 0)
+                    c.phase.push_back(k < (int)vals.size() ? std::atof(vals[k].c_str()) : 0.0);
+            }
+            zz++;
+        }
+    }
+    L.nx = nx;
+    L.ny = ny;
+    L.nz = nz;
+    return VX3MI_OK;
+}
+
+int model_from_docs(const char *baseText, const char *vxdText, vx3mi_model **out) {
+    if (!out) { set_error("out is NULL"); return VX3MI_E_ARG; }
+    *out = nullptr;
+    if (!baseText) { set_error("base VXA text is NULL"); return VX3MI_E_ARG; }
+    VxaContent c;
+    try {
+        XNode doc = XmlReader(baseText).parse_document();
+        if (vxdText && *vxdText) {
+            XNode vxd = XmlReader(vxdText).parse_document();
+            merge_vxd(doc, vxd);
+        }
+        std::string err;
+        int rc = read_vxa(doc, c, err);
+        if (rc != VX3MI_OK) { set_error(err); return rc; }
+    } catch (const XmlError &e) {
+        set_error("XML: " + e.msg);
+        return VX3MI_E_PARSE;
+    } catch (const std::bad_alloc &) {
+        set_error("out of host memory");
+        return VX3MI_E_NOMEM;
+    }
+    c.lat.cells = c.cells.data();
+    int nv = 0;
+    for (signed char ch : c.cells) nv += (ch > 0 && ch <= (int)c.mats.size());
+    c.lat.phase = (!c.phase.empty() && (int)c.phase.size() >= nv) ? c.phase.data() : nullptr;
+    c.lat.voxel_E = nullptr;
+    c.lat.nmat = (int)c.mats.size();
+    c.lat.mats = c.mats.data();
+    int rc = vx3mi_model_from_lattice(&c.lat, out);
+    if (rc == VX3MI_OK) {
+        Model &m = (*out)->m;
+        m.stopFormulaXml = c.stop;
+        m.fitnessXml = c.fitness;
+        m.recordStepSize = c.recordStepSize;
+        m.recordVoxel = c.recordVoxel;
+        m.recordLink = c.recordLink;
+    }
+    return rc;
+}
+
+bool read_file(const char *path, std::string &out) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) return false;
+    std::ostringstream ss;
+    ss << f.rdbuf();
+    out = ss.str();
+    return true;
+}
+
+}  // namespace
+}  // namespace vx3mi
 
 using namespace vx3mi;
 
 extern "C" {
-int vx3mi_model_from_xml(const char *, const char *, vx3mi_model **out) {
+
+int vx3mi_model_from_xml(const char *base_vxa_text, const char *vxd_text, vx3mi_model **out) {
+    return model_from_docs(base_vxa_text, vxd_text, out);
+}
+
+int vx3mi_model_from_vxa(const char *base_vxa_path, const char *vxd_path, vx3mi_model **out) {
     if (out) *out = nullptr;
-    set_error("VXA reader not built");
-    return VX3MI_E_PARSE;
+    if (!base_vxa_path) { set_error("base VXA path is NULL"); return VX3MI_E_ARG; }
+    std::string base, vxd;
+    if (!read_file(base_vxa_path, base)) { set_error(std::string("cannot read ") + base_vxa_path); return VX3MI_E_PARSE; }
+    if (vxd_path && !read_file(vxd_path, vxd)) { set_error(std::string("cannot read ") + vxd_path); return VX3MI_E_PARSE; }
+    return model_from_docs(base.c_str(), vxd_path ? vxd.c_str() : nullptr, out);
 }
-int vx3mi_model_from_vxa(const char *, const char *, vx3mi_model **out) {
-    if (out) *out = nullptr;
-    set_error("VXA reader not built");
-    return VX3MI_E_PARSE;
-}
-}
+
+}  // extern "C"

@@ -16,13 +16,13 @@ from typing import Iterable, Sequence
 import numpy as np
 
 __all__ = ["lib", "Model", "Batch", "VX3MIError", "FP32", "FP64", "MODE_AUTO", "MODE_PLAIN", "MODE_FLAT",
-           "MODE_PERSISTENT", "PROBLEM_DEFAULTS", "MATERIAL_DEFAULTS", "library_path"]
+           "MODE_PERSISTENT", "MODE_TILED", "PROBLEM_DEFAULTS", "MATERIAL_DEFAULTS", "library_path"]
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 _LIB_PATH = os.path.join(_PKG, "lib", "libvx3mi.so")
 
 FP32, FP64 = 32, 64
-MODE_AUTO, MODE_PLAIN, MODE_FLAT, MODE_PERSISTENT = 0, 1, 2, 3
+MODE_AUTO, MODE_PLAIN, MODE_FLAT, MODE_PERSISTENT, MODE_TILED = 0, 1, 2, 3, 4
 OK, DIVERGED = 0, 1
 
 PROBLEM_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=1.0, slow_damp_z=1.0,
