@@ -77,6 +77,20 @@ typedef struct {
     double temp_base, temp_amplitude, temp_period;
 } vx3mi_lattice;
 
+/* Settings a model carries besides topology and materials, read back by
+ * vx3mi_model_settings (the VXA keys they come from are named). */
+typedef struct {
+    int nx, ny, nz;                /* Structure dims */
+    double lattice_dim;            /* Lattice_Dim * X_Dim_Adj */
+    double dt_frac;                /* Integration/DtFrac */
+    int floor_enabled, enable_collision, enable_expansion;
+    int vary_temp;                 /* VaryTempEnabled && TempPeriod > 0 */
+    double temp_base, temp_amplitude, temp_period;
+    float initial_temp;            /* temperature set at import (UpdateMatTemps) */
+    int record_step_size, record_voxel, record_link; /* Simulator/RecordHistory */
+    int has_stop_formula, has_fitness_formula;
+} vx3mi_settings;
+
 typedef struct vx3mi_model vx3mi_model;
 typedef struct vx3mi_batch vx3mi_batch;
 
@@ -91,6 +105,18 @@ int vx3mi_model_from_vxa(const char *base_vxa_path, const char *vxd_path, vx3mi_
 int vx3mi_model_from_xml(const char *base_vxa_text, const char *vxd_text, vx3mi_model **out);
 int vx3mi_model_from_lattice(const vx3mi_lattice *spec, vx3mi_model **out);
 void vx3mi_model_free(vx3mi_model *m);
+/* settings read back (see vx3mi_settings) */
+int vx3mi_model_settings(const vx3mi_model *m, vx3mi_settings *out);
+/* per-voxel PhaseOffset in cycles, voxel order (nvox doubles) */
+int vx3mi_model_phase(const vx3mi_model *m, double *out);
+/* Evaluate the model's stop-condition or fitness MathTree (VXA
+ * Simulator/StopCondition/StopConditionFormula, Simulator/FitnessFunction;
+ * VX3_MathTree::eval, src/Utils/VX3_MathTree.h:50-190) on the host.
+ * vars (9): x, y, z, hit, t, angle, targetCloseness, numClosePairs, num_voxel.
+ * VX3MI_E_ARG when the model has no such formula. */
+#define VX3MI_FORMULA_STOP 0
+#define VX3MI_FORMULA_FITNESS 1
+int vx3mi_model_eval_formula(const vx3mi_model *m, int which, const double *vars9, double *out);
 /* counts: voxels, links, surface voxels */
 int vx3mi_model_counts(const vx3mi_model *m, int *nvox, int *nlink, int *nsurf);
 /* recommendedTimeStep (VX3_VoxelyzeKernel.cu:184-217), host evaluation */

@@ -1,0 +1,260 @@
+"""VXA/VXD input (host, no GPU): the product's XML reader and merge rule.
+
+A model read from VXA text must equal the model built from the same problem
+through the lattice entry point (itself pinned against the oracle's builder in
+test_model.py): same topology, bit-equal material constants, settings and
+phases.  Formulas are pinned by an independent recursive evaluation of random
+expression trees (the product evaluates a FIFO over the reversed breadth-first
+token order, as VX3_MathTree::eval does).
+"""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import vx3mi
+from vx3mi import configs
+from vx3mi import vxa as W
+
+PROBLEMS = {
+    "config0": configs.config0,
+    "config1_block": lambda: configs.config1(seed=2, dims=(9, 7, 5)),
+    "config2_robot": lambda: configs.config2(seed=4, n=1)[0],
+}
+
+SETTING_KEYS = ("nx", "ny", "nz", "lattice_dim", "dt_frac", "floor_enabled", "enable_collision", "enable_expansion",
+                "vary_temp", "temp_base", "temp_amplitude", "temp_period", "initial_temp")
+
+
+def assert_same_model(a: vx3mi.Model, b: vx3mi.Model):
+    assert (a.nv, a.nl, a.ns) == (b.nv, b.nl, b.ns)
+    ta, tb = a.topology(), b.topology()
+    for k in ta:
+        assert np.array_equal(ta[k], tb[k]), k
+    for v in range(a.nv):
+        assert np.array_equal(a.voxel_constants(v), b.voxel_constants(v)), v
+    for l in range(a.nl):
+        assert np.array_equal(a.link_constants(l), b.link_constants(l)), l
+    sa, sb = a.settings(), b.settings()
+    for k in SETTING_KEYS:
+        assert sa[k] == sb[k], k
+    assert np.array_equal(a.phase(), b.phase())
+    assert a.recommended_dt() == b.recommended_dt()
+
+
+@pytest.mark.parametrize("compression", W.COMPRESSIONS)
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_vxa_equals_lattice_model(name, compression):
+    prob = PROBLEMS[name]()
+    ref = vx3mi.Model.from_problem(prob)
+    assert_same_model(vx3mi.Model.from_xml(W.write_vxa(prob, compression)), ref)
+
+
+@pytest.mark.parametrize("name", list(PROBLEMS))
+def test_base_plus_vxd_merge(name, tmp_path):
+    prob = PROBLEMS[name]()
+    base, vxd = W.write_pair(prob, "BASE64")
+    # the base alone holds an empty 1x1x1 structure
+    assert vx3mi.Model.from_xml(base).nv == 0
+    assert_same_model(vx3mi.Model.from_xml(base, vxd), vx3mi.Model.from_problem(prob))
+    (tmp_path / "base.vxa").write_text(base)
+    (tmp_path / "robot.vxd").write_text(vxd)
+    m = vx3mi.Model.from_vxa(str(tmp_path / "base.vxa"), str(tmp_path / "robot.vxd"))
+    assert_same_model(m, vx3mi.Model.from_problem(prob))
+
+
+def test_vxd_replaces_nested_paths_and_creates_parents():
+    prob = configs.config0()
+    base = W.write_vxa(dict(prob, bond_damp_z=0.3, dt_frac=0.5))
+    vxd = ('<VXD><Damping replace="VXA.Simulator.Damping"><BondDampingZ>0.1</BondDampingZ>'
+           '<ColDampingZ>0.8</ColDampingZ><SlowDampingZ>1.0</SlowDampingZ></Damping>'
+           '<DtFrac replace="VXA.Simulator.Integration.DtFrac">0.9</DtFrac>'
+           '<RecordHistory replace="VXA.Simulator.RecordHistory"><RecordStepSize>7</RecordStepSize></RecordHistory>'
+           '<Ignored>no replace attribute</Ignored></VXD>')
+    m = vx3mi.Model.from_xml(base, vxd)
+    assert_same_model(m, vx3mi.Model.from_problem(prob))
+    s = m.settings()
+    assert (s["record_step_size"], s["record_voxel"], s["record_link"]) == (7, 1, 0)
+
+
+DEMO_LIKE = """<?xml version="1.0"?>
+<!-- one voxel resting one cell above the floor -->
+<VXA Version="1.1">
+<Simulator>
+  <RecordHistory><RecordStepSize>100</RecordStepSize></RecordHistory>
+  <StopCondition><StopConditionFormula>
+    <mtSUB><mtVAR>t</mtVAR><mtCONST>1</mtCONST></mtSUB>
+  </StopConditionFormula></StopCondition>
+</Simulator>
+<VXC Version="0.94">
+  <Lattice><Lattice_Dim>0.01</Lattice_Dim></Lattice>
+  <Palette><Material><Name>A &amp; B</Name>
+    <Display><Red>1</Red><Green>0</Green><Blue>0</Blue><Alpha>1</Alpha></Display>
+    <Mechanical><MatModel>0</MatModel><Elastic_Mod>1e+05</Elastic_Mod><Density>1e+3</Density>
+      <Poissons_Ratio>0.35</Poissons_Ratio><CTE>0</CTE><uStatic>1</uStatic><uDynamic>0.8</uDynamic>
+    </Mechanical></Material></Palette>
+  <Structure Compression="ASCII_READABLE">
+    <X_Voxels>1</X_Voxels><Y_Voxels>1</Y_Voxels><Z_Voxels>2</Z_Voxels>
+    <Data><Layer><![CDATA[0]]></Layer><Layer><![CDATA[1]]></Layer></Data>
+  </Structure>
+</VXC>
+</VXA>
+"""
+
+
+def test_reference_defaults_without_environment():
+    """No <Environment>: CVX_Environment constructor defaults (gravity, floor and
+    temperature on, TempBase 25, amplitude 0, period 0.1); no <Integration>:
+    DtFrac 0.9; EnableCollision defaults to true; RecordVoxel to 1."""
+    m = vx3mi.Model.from_xml(DEMO_LIKE, "<VXD>\n</VXD>")
+    assert (m.nv, m.nl) == (1, 0)
+    assert m.topology()["lattice"].tolist() == [[0, 0, 1]]
+    s = m.settings()
+    assert s["dt_frac"] == 0.9 and s["floor_enabled"] == 1 and s["enable_collision"] == 1
+    assert (s["temp_base"], s["temp_amplitude"], s["temp_period"], s["vary_temp"]) == (25.0, 0.0, 0.1, 0)
+    assert (s["record_step_size"], s["record_voxel"], s["record_link"]) == (100, 1, 0)
+    assert s["initial_temp"] == 0.0
+    c = m.voxel_constants(0)
+    f = np.float32
+    mass = f(f(1000.0) * f(0.01) ** 3)
+    assert c[0] == f(1000.0 * 0.01 ** 3) or c[0] == mass
+    assert c[7] == f(f(-c[0]) * f(9.80665)) * f(-f(-9.81) / 9.80665)  # gravity force, GravAcc -9.81
+    assert c[9] == f(1.0) and c[10] == f(0.8)
+    # no links: dt from the voxel fallback E*L/m
+    assert m.recommended_dt() == pytest.approx(1.0 / (6.283185 * math.sqrt(1e5 * 0.01 / float(c[0]))), rel=1e-6)
+    assert m.eval_formula(vx3mi.FORMULA_STOP, t=0.25) == -0.75
+
+
+def test_environment_missing_key_defaults_inside_blocks():
+    """Inside <Gravity> / <Thermal> the missing-key defaults differ from the
+    constructor's (VX_Environment.cpp:150-171): FloorEnabled and TempEnabled
+    read as off; the legacy TempAmp key is an absolute temperature."""
+    prob = configs.config0()
+    text = W.write_vxa(prob).replace("<FloorEnabled>1</FloorEnabled>", "")
+    text = text.replace("<TempAmplitude>39.0</TempAmplitude>", "<TempAmp>64.0</TempAmp>")
+    s = vx3mi.Model.from_xml(text).settings()
+    assert s["floor_enabled"] == 0
+    assert s["temp_amplitude"] == 64.0 - 25.0
+    t2 = W.write_vxa(prob).replace("<TempEnabled>1</TempEnabled>", "").replace("<VaryTempEnabled>1</VaryTempEnabled>", "")
+    s2 = vx3mi.Model.from_xml(t2).settings()
+    assert s2["vary_temp"] == 0 and s2["initial_temp"] == 0.0
+
+
+def test_static_temperature_sets_initial_temp():
+    """TempEnabled without VaryTemp: every voxel starts at TempBase + amplitude
+    relative to TempBase (UpdateMatTemps)."""
+    prob = dict(configs.config0(), vary_temp=0, temp_amplitude=5.0)
+    s = vx3mi.Model.from_xml(W.write_vxa(prob)).settings()
+    assert s["initial_temp"] == np.float32(5.0)
+
+
+def test_phase_offset_layers_follow_voxel_order():
+    prob = configs.config2(seed=9, n=1)[0]
+    m = vx3mi.Model.from_xml(W.write_vxa(prob))
+    assert np.array_equal(m.phase(), np.asarray(prob["phase"]))
+
+
+# ---- formulas -----------------------------------------------------------------
+BIN = {"mtADD": lambda a, b: a + b, "mtSUB": lambda a, b: a - b, "mtMUL": lambda a, b: a * b,
+       "mtDIV": lambda a, b: a / b if b != 0 else math.copysign(math.inf, a) * math.copysign(1, b) if a else math.nan,
+       "mtGREATERTHAN": lambda a, b: float(a > b), "mtLESSTHAN": lambda a, b: float(a < b),
+       "mtAND": lambda a, b: float(bool(a) and bool(b)), "mtOR": lambda a, b: float(bool(a) or bool(b))}
+UN = {"mtABS": abs, "mtSIN": math.sin, "mtCOS": math.cos, "mtATAN": math.atan, "mtNOT": lambda a: float(a == 0),
+      "mtINT": lambda a: float(round(a)), "mtNORMALCDF": lambda a: 0.5 * math.erfc(-a / math.sqrt(2))}
+VARS = vx3mi.FORMULA_VARS
+
+
+def random_tree(rng, depth):
+    if depth == 0 or rng.random() < 0.25:
+        r = rng.random()
+        if r < 0.45:
+            return ("mtVAR", VARS[rng.integers(len(VARS))])
+        if r < 0.9:
+            return ("mtCONST", float(np.round(rng.normal() * 3, 3)))
+        return ("mtPI",) if r < 0.95 else ("mtE",)
+    if rng.random() < 0.7:
+        op = list(BIN)[rng.integers(len(BIN))]
+        return (op, random_tree(rng, depth - 1), random_tree(rng, depth - 1))
+    op = list(UN)[rng.integers(len(UN))]
+    return (op, random_tree(rng, depth - 1))
+
+
+def eval_tree(e, env):
+    op = e[0]
+    if op == "mtVAR":
+        return env[e[1]]
+    if op == "mtCONST":
+        return e[1]
+    if op == "mtPI":
+        return math.pi
+    if op == "mtE":
+        return math.e
+    if op in BIN:
+        return BIN[op](eval_tree(e[1], env), eval_tree(e[2], env))
+    return UN[op](eval_tree(e[1], env))
+
+
+def test_formulas_match_recursive_evaluation():
+    rng = np.random.default_rng(5)
+    base = configs.config0()
+    checked = 0
+    for _ in range(120):
+        tree = random_tree(rng, 5)
+        if tree[0] in ("mtVAR", "mtCONST", "mtPI", "mtE"):
+            tree = ("mtADD", tree, ("mtCONST", 0.5))
+        m = vx3mi.Model.from_xml(W.write_vxa(base, stop=tree, fitness=("mtMUL", tree, ("mtCONST", 2.0))))
+        env = {k: float(np.round(rng.normal() * 2, 2)) for k in VARS}
+        want = eval_tree(tree, env)
+        got = m.eval_formula(vx3mi.FORMULA_STOP, **env)
+        if math.isnan(want):
+            assert math.isnan(got)
+            continue
+        assert got == pytest.approx(want, rel=1e-12, abs=1e-12), tree
+        fit = m.eval_formula(vx3mi.FORMULA_FITNESS, **env)
+        assert fit == pytest.approx(2.0 * want, rel=1e-12, abs=1e-12) or (math.isinf(want) and math.isinf(fit))
+        checked += 1
+    assert checked > 80
+
+
+def test_operand_order_of_noncommutative_operators():
+    m = vx3mi.Model.from_xml(W.write_vxa(configs.config0(), stop=("mtDIV", ("mtPOW", ("mtVAR", "x"), ("mtCONST", 3)),
+                                                                    ("mtSUB", ("mtVAR", "t"), ("mtVAR", "z")))))
+    assert m.eval_formula(vx3mi.FORMULA_STOP, x=2.0, t=5.0, z=1.0) == 8.0 / 4.0
+
+
+# ---- errors -------------------------------------------------------------------
+BAD = {
+    "unterminated": "<VXA><Simulator></VXA>",
+    "no_vxa": "<Foo/>",
+    "layer_len": DEMO_LIKE.replace("<![CDATA[0]]>", "<![CDATA[00]]>"),
+    "zlib": DEMO_LIKE.replace('Compression="ASCII_READABLE"', 'Compression="ZLIB"'),
+    "few_layers": DEMO_LIKE.replace("<Z_Voxels>2</Z_Voxels>", "<Z_Voxels>3</Z_Voxels>"),
+    "attach": DEMO_LIKE.replace("<Simulator>", "<Simulator><AttachDetach><EnableAttach>true</EnableAttach></AttachDetach>"),
+    "bcs": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs></Boundary_Conditions>"
+                                      "</Environment><VXC "),
+    "bad_token": DEMO_LIKE.replace("<mtSUB>", "<mtFOO>").replace("</mtSUB>", "</mtFOO>"),
+    "bad_var": DEMO_LIKE.replace("<mtVAR>t</mtVAR>", "<mtVAR>speed</mtVAR>"),
+    "missing_operand": DEMO_LIKE.replace("<mtCONST>1</mtCONST>", ""),
+    "force_field": DEMO_LIKE.replace("<Simulator>", "<Simulator><ForceField><x_forcefield><mtCONST>2</mtCONST>"
+                                                    "</x_forcefield></ForceField>"),
+    "target": DEMO_LIKE.replace("<CTE>0</CTE>", "<CTE>0</CTE><isTarget>1</isTarget>"),
+}
+
+
+@pytest.mark.parametrize("case", list(BAD))
+def test_malformed_or_unsupported_input_is_a_parse_error(case):
+    with pytest.raises(vx3mi.VX3MIError, match="error -2"):
+        vx3mi.Model.from_xml(BAD[case])
+
+
+def test_zero_force_field_and_collision_false_are_accepted():
+    text = DEMO_LIKE.replace("<Simulator>", "<Simulator><ForceField><x_forcefield><mtCONST>0</mtCONST></x_forcefield>"
+                                            "</ForceField><AttachDetach><EnableCollision>false</EnableCollision>"
+                                            "</AttachDetach>")
+    assert vx3mi.Model.from_xml(text).settings()["enable_collision"] == 0
+
+
+def test_missing_file_is_a_parse_error(tmp_path):
+    with pytest.raises(vx3mi.VX3MIError, match="cannot read"):
+        vx3mi.Model.from_vxa(str(tmp_path / "nope.vxa"))

@@ -313,6 +313,34 @@ int vx3mi_model_counts(const vx3mi_model *m, int *nvox, int *nlink, int *nsurf) 
     return VX3MI_OK;
 }
 
+int vx3mi_model_settings(const vx3mi_model *m, vx3mi_settings *o) {
+    if (!m || !o) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    const Model &M = m->m;
+    o->nx = M.nx; o->ny = M.ny; o->nz = M.nz;
+    o->lattice_dim = M.latticeDim;
+    o->dt_frac = M.dtFrac;
+    o->floor_enabled = M.floorEnabled;
+    o->enable_collision = M.enableCollision;
+    o->enable_expansion = M.enableExpansion;
+    o->vary_temp = M.varyTemp;
+    o->temp_base = M.tempBase;
+    o->temp_amplitude = M.tempAmplitude;
+    o->temp_period = M.tempPeriod;
+    o->initial_temp = M.initialTemp;
+    o->record_step_size = M.recordStepSize;
+    o->record_voxel = M.recordVoxel;
+    o->record_link = M.recordLink;
+    o->has_stop_formula = !M.stopFormula.empty();
+    o->has_fitness_formula = !M.fitnessFormula.empty();
+    return VX3MI_OK;
+}
+
+int vx3mi_model_phase(const vx3mi_model *m, double *out) {
+    if (!m || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    for (int v = 0; v < m->m.nv(); v++) out[v] = m->m.phase[v];
+    return VX3MI_OK;
+}
+
 int vx3mi_model_recommended_dt(const vx3mi_model *m, double *out) {
     if (!m || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
     *out = recommended_dt(m->m);

@@ -285,16 +285,63 @@ struct VxaContent {
     std::vector<signed char> cells;
     std::vector<double> phase;
     std::vector<vx3mi_material> mats;
-    std::string stop, fitness;
+    std::vector<MtToken> stop, fitness;
     int recordStepSize = 0, recordVoxel = 1, recordLink = 0;
+    int secondaryExperiment = 0;
+    double reinitAfter = 0.0;
 };
 
-// serialise a MathTree subtree back to text (kept for the runner)
-void dump(const XNode &n, std::string &out) {
-    out += "<" + n.name + ">";
-    out += n.text;
-    for (const XNode &k : n.kids) dump(k, out);
-    out += "</" + n.name + ">";
+// boost property_tree get<bool> (VX3_SimulationManager.cu:328-330): "true"/"1" or "false"/"0"
+bool load_bool(const XNode *parent, const char *key, bool dflt) {
+    const XNode *n = parent ? parent->child(key) : nullptr;
+    if (!n) return dflt;
+    if (n->text == "true" || n->text == "1") return true;
+    if (n->text == "false" || n->text == "0") return false;
+    return dflt;
+}
+
+// VX3_SimulationManager::ParseMathTree (VX3_SimulationManager.cu:157-275):
+// breadth-first walk below the formula element, then reversed, MT_END last
+int parse_formula(const XNode *root, std::vector<MtToken> &out, std::string &err) {
+    out.clear();
+    if (!root || root->kids.empty()) return VX3MI_OK;
+    static const struct { const char *name; int op; } ops[] = {
+        {"mtCONST", MT_CONST}, {"mtE", MT_E}, {"mtPI", MT_PI}, {"mtVAR", MT_VAR}, {"mtADD", MT_ADD},
+        {"mtSUB", MT_SUB}, {"mtMUL", MT_MUL}, {"mtDIV", MT_DIV}, {"mtPOW", MT_POW}, {"mtSQRT", MT_SQRT},
+        {"mtSIN", MT_SIN}, {"mtCOS", MT_COS}, {"mtTAN", MT_TAN}, {"mtATAN", MT_ATAN}, {"mtLOG", MT_LOG},
+        {"mtINT", MT_INT}, {"mtABS", MT_ABS}, {"mtNOT", MT_NOT}, {"mtGREATERTHAN", MT_GT},
+        {"mtLESSTHAN", MT_LT}, {"mtAND", MT_AND}, {"mtOR", MT_OR}, {"mtNORMALCDF", MT_NORMALCDF}};
+    static const char *vars[MV_COUNT] = {"x", "y", "z", "hit", "t", "angle", "targetCloseness", "numClosePairs",
+                                         "num_voxel"};
+    std::vector<const XNode *> bfs;
+    for (const XNode &k : root->kids) bfs.push_back(&k);
+    for (size_t head = 0; head < bfs.size(); head++)
+        for (const XNode &k : bfs[head]->kids) bfs.push_back(&k);
+    if (bfs.size() >= 1024) { err = "formula longer than 1023 tokens"; return VX3MI_E_PARSE; }
+    for (auto it = bfs.rbegin(); it != bfs.rend(); ++it) {
+        const XNode &n = **it;
+        MtToken tk{-1, 0.0};
+        for (const auto &o : ops)
+            if (n.name == o.name) tk.op = o.op;
+        if (tk.op < 0) { err = "unknown formula token <" + n.name + ">"; return VX3MI_E_PARSE; }
+        if (tk.op == MT_VAR) {
+            int var = -1;
+            for (int v = 0; v < MV_COUNT; v++)
+                if (n.text == vars[v]) var = v;
+            if (var < 0) { err = "unknown formula variable '" + n.text + "'"; return VX3MI_E_PARSE; }
+            tk.value = var;
+        } else if (tk.op == MT_CONST) {
+            char *end = nullptr;
+            tk.value = std::strtod(n.text.c_str(), &end);
+            if (end == n.text.c_str()) { err = "mtCONST without a number"; return VX3MI_E_PARSE; }
+        }
+        out.push_back(tk);
+    }
+    out.push_back(MtToken{MT_END, 0.0});
+    const double ones[MV_COUNT] = {1, 1, 1, 1, 1, 1, 1, 1, 1};
+    double r;
+    if (!mt_eval(out, ones, &r)) { err = "malformed formula (operator without operands)"; return VX3MI_E_PARSE; }
+    return VX3MI_OK;
 }
 
 int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
@@ -323,28 +370,40 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
             int v = 0;
             if (load(feat, "VolumeEffectsEnabled", v)) L.volume_effects = v != 0;
         }
-        L.enable_collision = 1;  // VX3_SimulationManager.cu:328 default true
-        if (const XNode *ad = sim->child("AttachDetach")) {
-            int v;
-            if (load(ad, "EnableCollision", v)) L.enable_collision = v != 0;
-            if (load(ad, "EnableAttach", v) && v) { err = "EnableAttach is not supported"; return VX3MI_E_PARSE; }
-            if (load(ad, "EnableDetach", v) && v) { err = "EnableDetach is not supported"; return VX3MI_E_PARSE; }
-        }
+        // VX3 keys (VX3_SimulationManager.cu:323-376); EnableCollision defaults to true
+        const XNode *ad = sim->child("AttachDetach");
+        L.enable_collision = load_bool(ad, "EnableCollision", true);
+        if (load_bool(ad, "EnableAttach", false)) { err = "EnableAttach is not supported"; return VX3MI_E_PARSE; }
+        if (load_bool(ad, "EnableDetach", false)) { err = "EnableDetach is not supported"; return VX3MI_E_PARSE; }
         int v;
         if (load(sim, "EnableExpansion", v)) L.enable_expansion = v;
         if (load(sim, "EnableCilia", v) && v) { err = "EnableCilia is not supported"; return VX3MI_E_PARSE; }
         if (load(sim, "EnableSignals", v) && v) { err = "EnableSignals is not supported"; return VX3MI_E_PARSE; }
+        double d;
+        if (load(sim, "MaxDistInVoxelLengthsToCountAsPair", d) && d > 0) {
+            err = "MaxDistInVoxelLengthsToCountAsPair (close-pair counting) is not supported";
+            return VX3MI_E_PARSE;
+        }
+        load(sim, "SecondaryExperiment", c.secondaryExperiment);
+        load(sim, "ReinitializeInitialPositionAfterThisManySeconds", c.reinitAfter);
         if (const XNode *rh = sim->child("RecordHistory")) {
             load(rh, "RecordStepSize", c.recordStepSize);
             load(rh, "RecordVoxel", c.recordVoxel);
             load(rh, "RecordLink", c.recordLink);
         }
-        if (const XNode *f = path(sim, {"StopCondition", "StopConditionFormula"})) dump(*f, c.stop);
-        if (const XNode *f = sim->child("FitnessFunction")) dump(*f, c.fitness);
+        int rc = parse_formula(path(sim, {"StopCondition", "StopConditionFormula"}), c.stop, err);
+        if (rc == VX3MI_OK) rc = parse_formula(sim->child("FitnessFunction"), c.fitness, err);
+        if (rc != VX3MI_OK) return rc;
         if (const XNode *ff = sim->child("ForceField")) {
             for (const XNode &k : ff->kids) {
-                bool zero = k.kids.size() == 1 && k.kids[0].name == "mtCONST" && std::atof(k.kids[0].text.c_str()) == 0.0;
-                if (!zero && !k.kids.empty()) { err = "non-zero ForceField is not supported"; return VX3MI_E_PARSE; }
+                std::vector<MtToken> f;
+                if ((rc = parse_formula(&k, f, err)) != VX3MI_OK) return rc;
+                const double zeros[MV_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+                double r = 0;
+                if (!f.empty() && (f.size() != 2 || f[0].op != MT_CONST || !mt_eval(f, zeros, &r) || r != 0.0)) {
+                    err = "a non-zero ForceField is not supported";
+                    return VX3MI_E_PARSE;
+                }
             }
         }
     } else {
@@ -421,7 +480,7 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
                 load(mech, "CTE", m.cte);
                 load(mech, "uStatic", m.u_static);
                 load(mech, "uDynamic", m.u_dynamic);
-                if (load(mech, "isTarget", iv)) m.is_target = iv != 0;
+                if (load(mech, "isTarget", iv) && iv) { err = "target materials (isTarget) are not supported"; return VX3MI_E_PARSE; }
                 if (load(mech, "Fixed", iv)) m.fixed = iv != 0;
                 if (load(mech, "isMeasured", iv)) m.is_measured = iv;
                 load(mech, "TurnOnThermalExpansionAfterThisManySeconds", m.thermal_on_after);
@@ -512,8 +571,10 @@ int model_from_docs(const char *baseText, const char *vxdText, vx3mi_model **out
     int rc = vx3mi_model_from_lattice(&c.lat, out);
     if (rc == VX3MI_OK) {
         Model &m = (*out)->m;
-        m.stopFormulaXml = c.stop;
-        m.fitnessXml = c.fitness;
+        m.stopFormula = c.stop;
+        m.fitnessFormula = c.fitness;
+        m.secondaryExperiment = c.secondaryExperiment;
+        m.reinitInitialPositionAfter = c.reinitAfter;
         m.recordStepSize = c.recordStepSize;
         m.recordVoxel = c.recordVoxel;
         m.recordLink = c.recordLink;

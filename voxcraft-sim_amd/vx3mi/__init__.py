@@ -16,7 +16,7 @@ from typing import Iterable, Sequence
 import numpy as np
 
 __all__ = ["lib", "Model", "Batch", "VX3MIError", "FP32", "FP64", "MODE_AUTO", "MODE_PLAIN", "MODE_FLAT",
-           "MODE_PERSISTENT", "MODE_TILED", "PROBLEM_DEFAULTS", "MATERIAL_DEFAULTS", "library_path"]
+           "MODE_PERSISTENT", "MODE_TILED", "FORMULA_STOP", "FORMULA_FITNESS", "PROBLEM_DEFAULTS", "MATERIAL_DEFAULTS", "library_path"]
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 _LIB_PATH = os.path.join(_PKG, "lib", "libvx3mi.so")
@@ -44,6 +44,19 @@ class _Material(C.Structure):
                 ("fail_stress", C.c_float), ("mat_model", C.c_int), ("fixed", C.c_int),
                 ("is_measured", C.c_int), ("is_target", C.c_int), ("thermal_on_after", C.c_double),
                 ("red", C.c_float), ("green", C.c_float), ("blue", C.c_float), ("alpha", C.c_float)]
+
+
+class _Settings(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("lattice_dim", C.c_double),
+                ("dt_frac", C.c_double), ("floor_enabled", C.c_int), ("enable_collision", C.c_int),
+                ("enable_expansion", C.c_int), ("vary_temp", C.c_int), ("temp_base", C.c_double),
+                ("temp_amplitude", C.c_double), ("temp_period", C.c_double), ("initial_temp", C.c_float),
+                ("record_step_size", C.c_int), ("record_voxel", C.c_int), ("record_link", C.c_int),
+                ("has_stop_formula", C.c_int), ("has_fitness_formula", C.c_int)]
+
+
+FORMULA_STOP, FORMULA_FITNESS = 0, 1
+FORMULA_VARS = ("x", "y", "z", "hit", "t", "angle", "targetCloseness", "numClosePairs", "num_voxel")
 
 
 class _Lattice(C.Structure):
@@ -82,6 +95,9 @@ def lib():
         "vx3mi_model_from_lattice": (C.c_int, [C.POINTER(_Lattice), pvp]),
         "vx3mi_model_free": (None, [vp]),
         "vx3mi_model_counts": (C.c_int, [vp, ip, ip, ip]),
+        "vx3mi_model_settings": (C.c_int, [vp, C.POINTER(_Settings)]),
+        "vx3mi_model_phase": (C.c_int, [vp, dp]),
+        "vx3mi_model_eval_formula": (C.c_int, [vp, C.c_int, dp, dp]),
         "vx3mi_model_recommended_dt": (C.c_int, [vp, dp]),
         "vx3mi_model_topology": (C.c_int, [vp, ip, ip, ip, ip]),
         "vx3mi_model_voxel_constants": (C.c_int, [vp, C.c_int, fp]),
@@ -176,6 +192,24 @@ class Model:
         d = C.c_double()
         _check(lib().vx3mi_model_recommended_dt(self._h, C.byref(d)))
         return d.value
+
+    def settings(self) -> dict:
+        st = _Settings()
+        _check(lib().vx3mi_model_settings(self._h, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in _Settings._fields_}
+
+    def phase(self) -> np.ndarray:
+        o = np.zeros(self.nv)
+        _check(lib().vx3mi_model_phase(self._h, _ptr(o, C.c_double)))
+        return o
+
+    def eval_formula(self, which: int, **vars_) -> float:
+        """Evaluate the stop (FORMULA_STOP) or fitness (FORMULA_FITNESS) formula;
+        unnamed variables are 0."""
+        v = np.array([float(vars_.get(n, 0.0)) for n in FORMULA_VARS])
+        out = C.c_double()
+        _check(lib().vx3mi_model_eval_formula(self._h, which, _ptr(v, C.c_double), C.byref(out)))
+        return out.value
 
     def topology(self) -> dict:
         lat = np.zeros((self.nv, 3), dtype=np.int32)
