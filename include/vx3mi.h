@@ -177,6 +177,44 @@ int vx3mi_eval_link_forces(vx3mi_batch *b, double *out12);
 int vx3mi_last_step_ms(vx3mi_batch *b, float *ms);
 int vx3mi_last_step_launches(vx3mi_batch *b, long long *launches);
 
+/* ---- differentiable mode ----------------------------------------------------
+ * No counterpart in the reference: a differentiable variant of doTimeStep
+ * (same link equations, VX3_Link.cu:90-218, and integrator,
+ * VX3_Voxel.cu:162-277) with a softplus floor penalty, kinetic-only friction
+ * with a regularised direction, collisions off, per-voxel actuation phase and
+ * Young's modulus as parameters, and a hand-written adjoint with
+ * checkpointing every K steps and recomputation. */
+typedef struct vx3mi_diff vx3mi_diff;
+typedef struct {
+    double floor_stiffness;  /* softplus floor stiffness k (N/m) */
+    double floor_width;      /* softplus width w (m) */
+    double friction_speed;   /* friction regularisation speed (m/s) */
+    double dt;               /* timestep; <= 0: DtFrac x recommendedTimeStep with every modulus at e_max */
+    double e_max;            /* modulus bound for the default timestep (Pa) */
+    int checkpoint_every;    /* K; <= 0: chosen under hbm_budget */
+    double hbm_budget;       /* device bytes for checkpoints + tape when K is chosen */
+} vx3mi_diff_opts;
+/* defaults: k 1e5, w 1e-4, friction speed 1e-3, dt from e_max 1e7, K chosen, budget 16 GiB */
+void vx3mi_diff_default_opts(vx3mi_diff_opts *o);
+/* every model must have one actuation / floor setting, no volume effects and
+ * no TurnOnThermalExpansion delay; collisions are not simulated */
+int vx3mi_diff_create(vx3mi_model *const *models, int n, int precision, const vx3mi_diff_opts *opts, vx3mi_diff **out);
+void vx3mi_diff_destroy(vx3mi_diff *d);
+int vx3mi_diff_counts(const vx3mi_diff *d, int *nrobot, int *nvox, int *nlink);
+/* per-voxel parameters in batch voxel order: phase (cycles) and Young's modulus (Pa) */
+int vx3mi_diff_set_params(vx3mi_diff *d, const double *phase, const double *E);
+/* nsteps from the initial state (voxels at rest on their lattice points),
+ * keeping checkpoints for the adjoint */
+int vx3mi_diff_forward(vx3mi_diff *d, int nsteps);
+/* final state of the last forward; robot -1 = whole batch; absolute positions */
+int vx3mi_diff_get_state(vx3mi_diff *d, int robot, double *pos, double *quat, double *linmom, double *angmom);
+/* adjoint of the last forward: from dL/d(final state) (each may be NULL = 0),
+ * dL/dphase (per cycle) and dL/dE per voxel */
+int vx3mi_diff_backward(vx3mi_diff *d, const double *grad_pos, const double *grad_quat, const double *grad_linmom,
+                        const double *grad_angmom, double *grad_phase, double *grad_E);
+/* out (8): K, checkpoint bytes, tape bytes, dt, forward ms, backward ms, steps, precision */
+int vx3mi_diff_info(const vx3mi_diff *d, double *out8);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,12 @@ class _Spec(C.Structure):
                 ("temp_base", C.c_double), ("temp_amplitude", C.c_double), ("temp_period", C.c_double)]
 
 
+class _DiffOpts(C.Structure):
+    _fields_ = [("floor_k", C.c_double), ("floor_w", C.c_double), ("fric_v", C.c_double),
+                ("floor_enabled", C.c_int), ("vary_temp", C.c_int), ("enable_expansion", C.c_int),
+                ("temp_amplitude", C.c_double), ("temp_period", C.c_double), ("dt", C.c_double)]
+
+
 SPEC_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=1.0, slow_damp_z=1.0,
                      volume_effects=0, enable_collision=1, enable_expansion=0, grav_acc=-9.81,
                      floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
@@ -90,6 +96,14 @@ def lib():
         L.vxo_get_contact.argtypes = [vp, dp]
         L.vxo_get_voxel_material.argtypes = [vp, C.c_int, fp]
         L.vxo_get_link_material.argtypes = [vp, C.c_int, fp]
+        L.vxo_diff_new.restype = vp
+        L.vxo_diff_new.argtypes = [vp, dp, dp, C.POINTER(_DiffOpts)]
+        L.vxo_diff_free.argtypes = [vp]
+        L.vxo_diff_step.argtypes = [vp, C.c_int]
+        L.vxo_diff_get.argtypes = [vp, dp, dp, dp, dp]
+        L.vxo_diff_mass.restype = C.c_double
+        L.vxo_diff_mass.argtypes = [vp, C.c_int]
+        L.vxo_diff_set.argtypes = [vp, dp, dp, dp, dp]
         for f in ("vxo_sin", "vxo_cos", "vxo_acos"):
             getattr(L, f).restype = C.c_double
             getattr(L, f).argtypes = [C.c_double]
@@ -204,3 +218,43 @@ class OracleModel:
         o = np.zeros(11, dtype=np.float32)
         self._L.vxo_get_link_material(self._h, l, _ptr(o, C.c_float))
         return o
+
+
+class OracleDiff:
+    """FP64 oracle of the differentiable mode (softplus floor, per-voxel E and phase)."""
+
+    def __init__(self, problem: dict, phase, E, dt: float, floor_k: float = 1e5, floor_w: float = 1e-4,
+                 fric_v: float = 1e-3):
+        p = dict(SPEC_DEFAULTS)
+        p.update(problem)
+        self.model = OracleModel(problem)
+        self._phase = np.ascontiguousarray(phase, dtype=np.float64)
+        self._E = np.ascontiguousarray(E, dtype=np.float64)
+        vary = int(p["vary_temp"]) and p["temp_period"] > 0
+        o = _DiffOpts(floor_k, floor_w, fric_v, int(p["floor_enabled"]), int(vary), int(p["enable_expansion"]),
+                      p["temp_amplitude"], p["temp_period"], dt)
+        self._L = lib()
+        self._h = self._L.vxo_diff_new(self.model._h, _ptr(self._phase, C.c_double), _ptr(self._E, C.c_double),
+                                       C.byref(o))
+        self.nv = self.model.nv
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            self._L.vxo_diff_free(self._h)
+            self._h = None
+
+    def step(self, n: int = 1):
+        self._L.vxo_diff_step(self._h, int(n))
+
+    def state(self):
+        pos = np.zeros((self.nv, 3)); q = np.zeros((self.nv, 4))
+        lm = np.zeros((self.nv, 3)); am = np.zeros((self.nv, 3))
+        self._L.vxo_diff_get(self._h, *[_ptr(a, C.c_double) for a in (pos, q, lm, am)])
+        return dict(pos=pos, quat=q, linmom=lm, angmom=am)
+
+    def set_state(self, pos=None, quat=None, linmom=None, angmom=None):
+        c = [None if a is None else np.ascontiguousarray(a, dtype=np.float64) for a in (pos, quat, linmom, angmom)]
+        self._L.vxo_diff_set(self._h, *[_ptr(a, C.c_double) for a in c])
+
+    def masses(self) -> np.ndarray:
+        return np.array([self._L.vxo_diff_mass(self._h, v) for v in range(self.nv)])

@@ -982,3 +982,270 @@ void vxo_get_link_material(const vxo_model *m, int link, float *o) {
 double vxo_sin(double x) { return vxm_sin(x); }
 double vxo_cos(double x) { return vxm_cos(x); }
 double vxo_acos(double x) { return vxm_acos(x); }
+
+/* ======================================================================
+ * Differentiable mode (no counterpart in the reference; the variant the
+ * adjoint differentiates).  Same link equations as link_update above and the
+ * same integrator as voxel_update, in double throughout, with:
+ *   - per-voxel Young's modulus E_i and phase phi_i (cycles); link modulus
+ *     2 E_n E_p / (E_n + E_p); Poisson's ratio 0; constants in double from
+ *     the float mass and damping ratios of the voxel's material;
+ *   - floor: normal force k w softplus(pen / w), damping -cDampT vz
+ *     sigmoid(pen / w), kinetic friction -muK N v_xy / sqrt(|v_xy|^2 + vEps^2);
+ *     no static friction;
+ *   - temperature held in double; orientation renormalised after each step.
+ * ====================================================================== */
+typedef struct {
+    double mInv, IInv, gDampT, gDampR, cDampT, grav, L, alpha, muK, dmNum;
+    int fixed;
+} dvox;
+typedef struct {
+    double E, a2, b1, b2, b3, sqA1, sqA2xIp, sqB1, sqB2xFMp, sqB3xIp, tArea, L;
+    int fixedBoth;
+} dlnk;
+
+struct vxo_diff {
+    const vxo_model *m;
+    vxo_diff_opts o;
+    dvox *vc;
+    dlnk *lc;
+    double *phase, *temp;
+    v3 *pos, *linMom, *angMom;
+    q4 *orient;
+    v3 *pos2, *a1v, *a2v;
+    int *lflags;
+    v3 *fN, *fP, *mN, *mP;
+    double time, prevDt;
+};
+
+static double softplus_d(double x) { return x > 0 ? x + log1p(exp(-x)) : log1p(exp(x)); }
+static double sigmoid_d(double x) {
+    if (x >= 0) return 1.0 / (1.0 + exp(-x));
+    double e = exp(x);
+    return e / (1.0 + e);
+}
+
+vxo_diff *vxo_diff_new(const vxo_model *m, const double *phase, const double *E, const vxo_diff_opts *o) {
+    vxo_diff *d = (vxo_diff *)xcalloc(1, sizeof(vxo_diff));
+    int nv = m->nv, nl = m->nl;
+    d->m = m;
+    d->o = *o;
+    d->vc = (dvox *)xcalloc(nv, sizeof(dvox));
+    d->lc = (dlnk *)xcalloc(nl, sizeof(dlnk));
+    d->phase = (double *)xcalloc(nv, sizeof(double));
+    d->temp = (double *)xcalloc(nv, sizeof(double));
+    d->pos = (v3 *)xcalloc(nv, sizeof(v3));
+    d->linMom = (v3 *)xcalloc(nv, sizeof(v3));
+    d->angMom = (v3 *)xcalloc(nv, sizeof(v3));
+    d->orient = (q4 *)xcalloc(nv, sizeof(q4));
+    d->pos2 = (v3 *)xcalloc(nl, sizeof(v3));
+    d->a1v = (v3 *)xcalloc(nl, sizeof(v3));
+    d->a2v = (v3 *)xcalloc(nl, sizeof(v3));
+    d->lflags = (int *)xcalloc(nl, sizeof(int));
+    d->fN = (v3 *)xcalloc(nl, sizeof(v3));
+    d->fP = (v3 *)xcalloc(nl, sizeof(v3));
+    d->mN = (v3 *)xcalloc(nl, sizeof(v3));
+    d->mP = (v3 *)xcalloc(nl, sizeof(v3));
+    for (int v = 0; v < nv; v++) {
+        const vmat *mt = &m->vm[m->vmat_of[v]];
+        dvox *c = &d->vc[v];
+        double mass = mt->mass, L = mt->nomSize, I = mass * L * L / 6.0;
+        c->mInv = 1.0 / mass;
+        c->IInv = 1.0 / I;
+        c->gDampT = mt->zetaG * 2.0 * sqrt(mass * E[v] * L);
+        c->gDampR = mt->zetaG * 2.0 * sqrt(I * E[v] * L * L * L);
+        c->cDampT = mt->zetaC * 2.0 * sqrt(mass * E[v] * L);
+        c->grav = (double)(float)(-mt->mass * 9.80665f * mt->gravMult);
+        c->L = L;
+        c->alpha = mt->alphaCTE;
+        c->muK = mt->muK;
+        c->dmNum = 2.0 * sqrt(mass) * mt->zetaI;
+        c->fixed = mt->fixed;
+        d->phase[v] = phase ? phase[v] : 0.0;
+        d->pos[v] = v3_(m->ix[v] * L, m->iy[v] * L, m->iz[v] * L);
+        d->orient[v].w = 1;
+    }
+    for (int l = 0; l < nl; l++) {
+        dlnk *c = &d->lc[l];
+        double Ea = E[m->lneg[l]], Eb = E[m->lpos[l]];
+        double L = m->lm[l].nomSize;
+        c->E = 2.0 * Ea * Eb / (Ea + Eb);
+        c->L = L;
+        c->a2 = c->E * L * L * L / 12.0;
+        c->b1 = c->E * L;
+        c->b2 = c->E * L * L / 2.0;
+        c->b3 = c->E * L * L * L / 6.0;
+        c->sqA1 = sqrt(c->E * L);
+        c->sqA2xIp = sqrt(c->a2 * L * L / 6.0);
+        c->sqB1 = sqrt(c->b1);
+        c->sqB2xFMp = sqrt(c->b2 * L / 2.0);
+        c->sqB3xIp = sqrt(c->b3 * L * L / 6.0);
+        c->tArea = L * L;
+        c->fixedBoth = d->vc[m->lneg[l]].fixed && d->vc[m->lpos[l]].fixed;
+        d->lflags[l] = LF_SMALL_ANGLE;
+    }
+    return d;
+}
+
+void vxo_diff_free(vxo_diff *d) {
+    if (!d) return;
+    free(d->vc); free(d->lc); free(d->phase); free(d->temp); free(d->pos); free(d->linMom); free(d->angMom);
+    free(d->orient); free(d->pos2); free(d->a1v); free(d->a2v); free(d->lflags);
+    free(d->fN); free(d->fP); free(d->mN); free(d->mP);
+    free(d);
+}
+
+static void diff_link(vxo_diff *d, int l) {
+    const vxo_model *m = d->m;
+    const dlnk *c = &d->lc[l];
+    if (c->fixedBoth) return;
+    int n = m->lneg[l], p = m->lpos[l], axis = m->laxis[l];
+    v3 oldPos2 = d->pos2[l], oldA1v = d->a1v[l], oldA2v = d->a2v[l];
+    int flags = d->lflags[l];
+    const dvox *vn = &d->vc[n], *vp = &d->vc[p];
+    double restLen = 0.5 * (vn->L * (1 + d->temp[n] * vn->alpha) + vp->L * (1 + d->temp[p] * vp->alpha));
+
+    v3 pos2 = to_axis_x(vsub(d->pos[p], d->pos[n]), axis);
+    q4 angle1 = to_axis_x_q(d->orient[n], axis);
+    q4 angle2 = to_axis_x_q(d->orient[p], axis);
+    q4 totalRot = qconj(angle1);
+    pos2 = qrot(totalRot, pos2);
+    angle2 = qmul(totalRot, angle2);
+    angle1.w = 1; angle1.x = angle1.y = angle1.z = 0;
+
+    float smallTurn = (float)((fabs(pos2.z) + fabs(pos2.y)) / pos2.x);
+    float extendPerc = (float)(fabs(1 - pos2.x / restLen));
+    int smallAngle = (flags & LF_SMALL_ANGLE) != 0;
+    if (!smallAngle && smallTurn < SA_BOND_BEND_RAD && extendPerc < SA_BOND_EXT_PERC) {
+        smallAngle = 1;
+        flags &= ~LF_LOCAL_VELOCITY_VALID;
+    } else if (smallAngle && (smallTurn > HYSTERESIS_FACTOR * SA_BOND_BEND_RAD ||
+                              extendPerc > HYSTERESIS_FACTOR * SA_BOND_EXT_PERC)) {
+        smallAngle = 0;
+        flags &= ~LF_LOCAL_VELOCITY_VALID;
+    }
+    if (smallAngle) {
+        pos2.x -= restLen;
+    } else {
+        angle1 = q_from_angle_to_posx(pos2);
+        angle2 = qmul(angle1, angle2);
+        pos2 = v3_(vlen(pos2) - restLen, 0, 0);
+    }
+    v3 a1v = q_to_rotvec(angle1), a2v = q_to_rotvec(angle2);
+    flags = smallAngle ? (flags | LF_SMALL_ANGLE) : (flags & ~LF_SMALL_ANGLE);
+    d->pos2[l] = pos2;
+    d->a1v[l] = a1v;
+    d->a2v[l] = a2v;
+    v3 dP = vscale(vsub(pos2, oldPos2), 0.5), dA1 = vscale(vsub(a1v, oldA1v), 0.5), dA2 = vscale(vsub(a2v, oldA2v), 0.5);
+    double stress = c->E * (pos2.x / restLen);
+    v3 fN = v3_(stress * c->tArea, c->b1 * pos2.y - c->b2 * (a1v.z + a2v.z), c->b1 * pos2.z + c->b2 * (a1v.y + a2v.y));
+    v3 fP = vscale(fN, -1);
+    v3 mN = v3_(c->a2 * (a2v.x - a1v.x), -c->b2 * pos2.z - c->b3 * (2 * a1v.y + a2v.y),
+                c->b2 * pos2.y - c->b3 * (2 * a1v.z + a2v.z));
+    v3 mP = v3_(c->a2 * (a1v.x - a2v.x), -c->b2 * pos2.z - c->b3 * (a1v.y + 2 * a2v.y),
+                c->b2 * pos2.y - c->b3 * (a1v.z + 2 * a2v.z));
+    if (flags & LF_LOCAL_VELOCITY_VALID) {
+        v3 pc = v3_(c->sqA1 * dP.x, c->sqB1 * dP.y - c->sqB2xFMp * (dA1.z + dA2.z),
+                    c->sqB1 * dP.z + c->sqB2xFMp * (dA1.y + dA2.y));
+        double dmN = vn->dmNum / d->prevDt, dmP = vp->dmNum / d->prevDt;
+        fN = vadd(fN, vscale(pc, dmN));
+        fP = vsub(fP, vscale(pc, dmP));
+        v3 tN = v3_(-c->sqA2xIp * (dA2.x - dA1.x), c->sqB2xFMp * dP.z + c->sqB3xIp * (2 * dA1.y + dA2.y),
+                    -c->sqB2xFMp * dP.y + c->sqB3xIp * (2 * dA1.z + dA2.z));
+        v3 tP = v3_(c->sqA2xIp * (dA2.x - dA1.x), c->sqB2xFMp * dP.z + c->sqB3xIp * (dA1.y + 2 * dA2.y),
+                    -c->sqB2xFMp * dP.y + c->sqB3xIp * (dA1.z + 2 * dA2.z));
+        mN = vsub(mN, vscale(tN, 0.5 * dmN));
+        mP = vsub(mP, vscale(tP, 0.5 * dmP));
+    } else {
+        flags |= LF_LOCAL_VELOCITY_VALID;
+    }
+    if (!smallAngle) {
+        fN = qrotinv(angle1, fN);
+        mN = qrotinv(angle1, mN);
+    }
+    fP = qrotinv(angle2, fP);
+    mP = qrotinv(angle2, mP);
+    d->fN[l] = to_axis_original(fN, axis);
+    d->fP[l] = to_axis_original(fP, axis);
+    d->mN[l] = to_axis_original(mN, axis);
+    d->mP[l] = to_axis_original(mP, axis);
+    d->lflags[l] = flags;
+}
+
+static void diff_voxel(vxo_diff *d, int v, double dt) {
+    const vxo_model *m = d->m;
+    const dvox *c = &d->vc[v];
+    if (c->fixed) return;
+    v3 F = v3_(0, 0, 0), M = v3_(0, 0, 0);
+    for (int k = 0; k < 6; k++) {
+        int l = m->vlinks[v][k];
+        if (l < 0) continue;
+        F = vadd(F, (k & 1) ? d->fP[l] : d->fN[l]);
+        M = vadd(M, (k & 1) ? d->mP[l] : d->mN[l]);
+    }
+    q4 q = d->orient[v];
+    v3 p = d->linMom[v], h = d->angMom[v];
+    v3 tf = qrot(q, F);
+    v3 vel = vscale(p, c->mInv);
+    tf = vsub(tf, vscale(vel, c->gDampT));
+    tf.z += c->grav;
+    if (d->o.floor_enabled) {
+        double L = c->L, b = L * (1 + d->temp[v] * c->alpha);
+        double pen = b / 2 - L / 2 - d->pos[v].z;
+        double x = pen / d->o.floor_w;
+        double N = d->o.floor_k * d->o.floor_w * softplus_d(x);
+        tf.z += N - c->cDampT * vel.z * sigmoid_d(x);
+        double r = sqrt(vel.x * vel.x + vel.y * vel.y + d->o.fric_v * d->o.fric_v);
+        tf.x -= c->muK * N * vel.x / r;
+        tf.y -= c->muK * N * vel.y / r;
+    }
+    p = vadd(p, vscale(tf, dt));
+    d->pos[v] = vadd(d->pos[v], vscale(p, dt * c->mInv));
+    v3 tm = vsub(qrot(q, M), vscale(h, c->IInv * c->gDampR));
+    h = vadd(h, vscale(tm, dt));
+    q = qmul(q_from_rotvec(vscale(h, dt * c->IInv)), q);
+    double n = 1.0 / sqrt(q.w * q.w + q.x * q.x + q.y * q.y + q.z * q.z);
+    q.w *= n; q.x *= n; q.y *= n; q.z *= n;
+    d->orient[v] = q;
+    d->linMom[v] = p;
+    d->angMom[v] = h;
+}
+
+void vxo_diff_step(vxo_diff *d, int nsteps) {
+    const vxo_model *m = d->m;
+    const double twoPiF = (double)(2 * 3.1415926f);
+    for (int s = 0; s < nsteps; s++) {
+        for (int v = 0; v < m->nv; v++) {
+            double T = 0;
+            if (d->o.vary_temp && !d->vc[v].fixed) {
+                T = d->o.temp_amplitude * sin(twoPiF * (d->time / d->o.temp_period + d->phase[v]));
+                if (!d->o.enable_expansion && T > 0) T = 0;
+            }
+            d->temp[v] = T;
+        }
+        for (int l = 0; l < m->nl; l++) diff_link(d, l);
+        for (int v = 0; v < m->nv; v++) diff_voxel(d, v, d->o.dt);
+        d->time += d->o.dt;
+        d->prevDt = d->o.dt;
+    }
+}
+
+void vxo_diff_get(const vxo_diff *d, double *pos3, double *quat4, double *linmom3, double *angmom3) {
+    for (int v = 0; v < d->m->nv; v++) {
+        if (pos3) { pos3[3 * v] = d->pos[v].x; pos3[3 * v + 1] = d->pos[v].y; pos3[3 * v + 2] = d->pos[v].z; }
+        if (quat4) { quat4[4 * v] = d->orient[v].w; quat4[4 * v + 1] = d->orient[v].x; quat4[4 * v + 2] = d->orient[v].y; quat4[4 * v + 3] = d->orient[v].z; }
+        if (linmom3) { linmom3[3 * v] = d->linMom[v].x; linmom3[3 * v + 1] = d->linMom[v].y; linmom3[3 * v + 2] = d->linMom[v].z; }
+        if (angmom3) { angmom3[3 * v] = d->angMom[v].x; angmom3[3 * v + 1] = d->angMom[v].y; angmom3[3 * v + 2] = d->angMom[v].z; }
+    }
+}
+
+double vxo_diff_mass(const vxo_diff *d, int v) { return 1.0 / d->vc[v].mInv; }
+
+void vxo_diff_set(vxo_diff *d, const double *pos3, const double *quat4, const double *linmom3, const double *angmom3) {
+    for (int v = 0; v < d->m->nv; v++) {
+        if (pos3) d->pos[v] = v3_(pos3[3 * v], pos3[3 * v + 1], pos3[3 * v + 2]);
+        if (quat4) { q4 q = {quat4[4 * v], quat4[4 * v + 1], quat4[4 * v + 2], quat4[4 * v + 3]}; d->orient[v] = q; }
+        if (linmom3) d->linMom[v] = v3_(linmom3[3 * v], linmom3[3 * v + 1], linmom3[3 * v + 2]);
+        if (angmom3) d->angMom[v] = v3_(angmom3[3 * v], angmom3[3 * v + 1], angmom3[3 * v + 2]);
+    }
+}

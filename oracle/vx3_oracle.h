@@ -109,6 +109,23 @@ double vxo_sin(double x);
 double vxo_cos(double x);
 double vxo_acos(double x);
 
+/* ---- differentiable mode (softplus floor, per-voxel E and phase) ---- */
+typedef struct {
+    double floor_k, floor_w, fric_v;  /* softplus stiffness, width; friction regularisation speed */
+    int floor_enabled, vary_temp, enable_expansion;
+    double temp_amplitude, temp_period;
+    double dt;
+} vxo_diff_opts;
+typedef struct vxo_diff vxo_diff;
+/* the model m supplies topology and materials and must outlive the result;
+ * phase (cycles) and E (Pa) are per voxel */
+vxo_diff *vxo_diff_new(const vxo_model *m, const double *phase, const double *E, const vxo_diff_opts *o);
+void vxo_diff_free(vxo_diff *d);
+void vxo_diff_step(vxo_diff *d, int nsteps);
+void vxo_diff_get(const vxo_diff *d, double *pos3, double *quat4, double *linmom3, double *angmom3);
+double vxo_diff_mass(const vxo_diff *d, int v);
+void vxo_diff_set(vxo_diff *d, const double *pos3, const double *quat4, const double *linmom3, const double *angmom3);
+
 #ifdef __cplusplus
 }
 #endif

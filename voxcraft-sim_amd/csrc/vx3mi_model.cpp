@@ -66,6 +66,9 @@ VoxelMat to_voxel_mat(const MatDerived &d) {
     v.muK = d.muK;
     v.fixed = d.fixed;
     v.isMeasured = d.isMeasured;
+    v.zetaI = d.zI;
+    v.zetaG = d.zG;
+    v.zetaC = d.zC;
     return v;
 }
 

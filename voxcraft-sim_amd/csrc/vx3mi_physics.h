@@ -74,6 +74,8 @@ struct VoxelMat {
     float penK;     // penetrationStiffness()
     float muS, muK;
     int fixed, isMeasured;
+    float zetaI, zetaG, zetaC;  // damping ratios (the differentiable mode derives its own constants)
+    float pad;
 };
 
 // Link material (VX3_MaterialLink.h:27-36) plus per-end constants the link

@@ -16,7 +16,7 @@ from typing import Iterable, Sequence
 import numpy as np
 
 __all__ = ["lib", "Model", "Batch", "VX3MIError", "FP32", "FP64", "MODE_AUTO", "MODE_PLAIN", "MODE_FLAT",
-           "MODE_PERSISTENT", "MODE_TILED", "FORMULA_STOP", "FORMULA_FITNESS", "PROBLEM_DEFAULTS", "MATERIAL_DEFAULTS", "library_path"]
+           "MODE_PERSISTENT", "MODE_TILED", "Diff", "FORMULA_STOP", "FORMULA_FITNESS", "PROBLEM_DEFAULTS", "MATERIAL_DEFAULTS", "library_path"]
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 _LIB_PATH = os.path.join(_PKG, "lib", "libvx3mi.so")
@@ -57,6 +57,12 @@ class _Settings(C.Structure):
 
 FORMULA_STOP, FORMULA_FITNESS = 0, 1
 FORMULA_VARS = ("x", "y", "z", "hit", "t", "angle", "targetCloseness", "numClosePairs", "num_voxel")
+
+
+class _DiffOpts(C.Structure):
+    _fields_ = [("floor_stiffness", C.c_double), ("floor_width", C.c_double), ("friction_speed", C.c_double),
+                ("dt", C.c_double), ("e_max", C.c_double), ("checkpoint_every", C.c_int),
+                ("hbm_budget", C.c_double)]
 
 
 class _Lattice(C.Structure):
@@ -117,6 +123,15 @@ def lib():
         "vx3mi_eval_link_forces": (C.c_int, [vp, dp]),
         "vx3mi_last_step_ms": (C.c_int, [vp, fp]),
         "vx3mi_last_step_launches": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+        "vx3mi_diff_default_opts": (None, [C.POINTER(_DiffOpts)]),
+        "vx3mi_diff_create": (C.c_int, [pvp, C.c_int, C.c_int, C.POINTER(_DiffOpts), pvp]),
+        "vx3mi_diff_destroy": (None, [vp]),
+        "vx3mi_diff_counts": (C.c_int, [vp, ip, ip, ip]),
+        "vx3mi_diff_set_params": (C.c_int, [vp, dp, dp]),
+        "vx3mi_diff_forward": (C.c_int, [vp, C.c_int]),
+        "vx3mi_diff_get_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
+        "vx3mi_diff_backward": (C.c_int, [vp, dp, dp, dp, dp, dp, dp]),
+        "vx3mi_diff_info": (C.c_int, [vp, dp]),
     }
     for name, (res, args) in sigs.items():
         f = getattr(L, name)
@@ -317,3 +332,70 @@ class Batch:
         out = np.zeros((self.nl, 12))
         _check(lib().vx3mi_eval_link_forces(self._h, _ptr(out, C.c_double)))
         return out
+
+
+class Diff:
+    """Differentiable batch (softplus floor, per-voxel phase and modulus, adjoint).
+
+    Options (keyword): floor_stiffness, floor_width, friction_speed, dt, e_max,
+    checkpoint_every, hbm_budget (see vx3mi_diff_opts in include/vx3mi.h).
+    """
+
+    def __init__(self, models: Sequence[Model] | Model, precision: int = FP32, **opts):
+        if isinstance(models, Model):
+            models = [models]
+        self._models = list(models)
+        o = _DiffOpts()
+        lib().vx3mi_diff_default_opts(C.byref(o))
+        for k, v in opts.items():
+            if not hasattr(o, k):
+                raise TypeError(f"unknown option {k}")
+            setattr(o, k, v)
+        arr = (C.c_void_p * len(self._models))(*[m._h.value for m in self._models])
+        h = C.c_void_p()
+        _check(lib().vx3mi_diff_create(arr, len(self._models), precision, C.byref(o), C.byref(h)))
+        self._h = h
+        nr, nv, nl = C.c_int(), C.c_int(), C.c_int()
+        _check(lib().vx3mi_diff_counts(self._h, C.byref(nr), C.byref(nv), C.byref(nl)))
+        self.nr, self.nv, self.nl = nr.value, nv.value, nl.value
+        self.precision = precision
+        self.voxel_offsets = np.cumsum([0] + [m.nv for m in self._models])
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().vx3mi_diff_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        self.close()
+
+    def set_params(self, phase, E):
+        ph = np.ascontiguousarray(phase, dtype=np.float64).reshape(-1)
+        e = np.ascontiguousarray(E, dtype=np.float64).reshape(-1)
+        if ph.size != self.nv or e.size != self.nv:
+            raise VX3MIError("phase and E need one value per voxel of the batch")
+        _check(lib().vx3mi_diff_set_params(self._h, _ptr(ph, C.c_double), _ptr(e, C.c_double)))
+
+    def forward(self, nsteps: int):
+        _check(lib().vx3mi_diff_forward(self._h, int(nsteps)))
+
+    def state(self, robot: int = -1) -> dict:
+        n = self.nv if robot < 0 else self._models[robot].nv
+        pos, q, lm, am = np.zeros((n, 3)), np.zeros((n, 4)), np.zeros((n, 3)), np.zeros((n, 3))
+        _check(lib().vx3mi_diff_get_state(self._h, int(robot), *[_ptr(a, C.c_double) for a in (pos, q, lm, am)]))
+        return dict(pos=pos, quat=q, linmom=lm, angmom=am)
+
+    def backward(self, grad_pos=None, grad_quat=None, grad_linmom=None, grad_angmom=None):
+        """dL/dphase (per cycle) and dL/dE per voxel from dL/d(final state)."""
+        g = [None if a is None else np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+             for a in (grad_pos, grad_quat, grad_linmom, grad_angmom)]
+        gp, ge = np.zeros(self.nv), np.zeros(self.nv)
+        _check(lib().vx3mi_diff_backward(self._h, *[_ptr(a, C.c_double) for a in g], _ptr(gp, C.c_double),
+                                         _ptr(ge, C.c_double)))
+        return gp, ge
+
+    def info(self) -> dict:
+        o = np.zeros(8)
+        _check(lib().vx3mi_diff_info(self._h, _ptr(o, C.c_double)))
+        return dict(checkpoint_every=int(o[0]), checkpoint_bytes=o[1], tape_bytes=o[2], dt=o[3], forward_ms=o[4],
+                    backward_ms=o[5], steps=int(o[6]), precision=int(o[7]))
