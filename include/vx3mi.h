@@ -15,6 +15,8 @@
 #ifndef VX3MI_H
 #define VX3MI_H
 
+#include <stddef.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -176,6 +178,39 @@ int vx3mi_eval_link_forces(vx3mi_batch *b, double *out12);
  * that call made. */
 int vx3mi_last_step_ms(vx3mi_batch *b, float *ms);
 int vx3mi_last_step_launches(vx3mi_batch *b, long long *launches);
+
+/* ---- simulation driver --------------------------------------------------------
+ * CUDA_Simulation (VX3_SimulationManager.cu:11-121) for every robot of a batch:
+ * iterate {stop condition met -> stop; doTimeStep; diverged -> stop; history
+ * frame every real_stepsize iterations} at most max_steps times (<= 0: the
+ * reference's 1000000), then the final centre of mass and fitness.  The
+ * models must be the ones the batch was created from.  history (may be NULL)
+ * receives the .history text of each robot (header and frames, the format of
+ * VX3_SimulationManager.cu:40-111) in order per robot. */
+typedef struct {
+    double current_time;                  /* currentTime when the robot stopped */
+    double fitness_score;                 /* FitnessFunction of the CoM displacement (0 without one) */
+    double initial_com[3], current_com[3];
+    double total_distance_of_all_voxels;  /* sum over measured voxels of |pos - initial pos| */
+    double vox_size;
+    double recommended_dt;                /* recommendedTimeStep() */
+    int num_voxel, num_measured_voxel, num_close_pairs;
+    int diverged;
+    int angle_sample_times;
+    int real_stepsize;
+    long long steps;                      /* doTimeStep calls */
+} vx3mi_result;
+typedef void (*vx3mi_text_fn)(void *user, int robot, const char *text, size_t len);
+int vx3mi_run(vx3mi_batch *b, vx3mi_model *const *models, int n, long long max_steps, int mode, vx3mi_text_fn history,
+              void *user, vx3mi_result *results, double *const *final_positions);
+/* final_positions (may be NULL, or hold NULL entries): per robot 3*nvox doubles
+ * receiving the absolute voxel positions when that robot stopped */
+/* palette index (1-based, the reference's matid) of every voxel, voxel order */
+int vx3mi_model_palette_index(const vx3mi_model *m, int *out);
+/* Simulator/SavePositionOfAllVoxels of a model read from XML (0 otherwise) */
+int vx3mi_model_save_positions(const vx3mi_model *m);
+/* VXA.RawPrint of a model read from XML ("" otherwise); valid while the model lives */
+const char *vx3mi_model_raw_print(const vx3mi_model *m);
 
 /* ---- differentiable mode ----------------------------------------------------
  * No counterpart in the reference: a differentiable variant of doTimeStep

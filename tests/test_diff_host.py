@@ -164,3 +164,67 @@ def test_fd_fixture_reproduces_from_oracle():
     assert np.allclose(g_psi[which], fix["grad_psi"][which], rtol=1e-9, atol=0)
     assert np.allclose(g_l[which], fix["grad_log10E"][which], rtol=1e-9, atol=0)
     assert FD.loss(prob, psi, log10E) == pytest.approx(float(fix["loss"]), rel=1e-12)
+
+
+@pytest.fixture(scope="module")
+def adjoint_host(tmp_path_factory):
+    out = tmp_path_factory.mktemp("adj") / "diff_adjoint_host.so"
+    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off",
+                    "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(ROOT, "voxcraft-sim_amd", "csrc"),
+                    os.path.join(ROOT, "tests", "diff_adjoint_host.cpp"),
+                    os.path.join(ROOT, "voxcraft-sim_amd", "csrc", "vx3mi_model.cpp"), "-o", str(out)], check=True)
+    L = C.CDLL(str(out))
+    dp = C.POINTER(C.c_double)
+    L.harness_adjoint.argtypes = [C.c_void_p, dp, dp, C.c_int, C.c_double, C.c_int, dp, dp, dp, dp]
+    return L
+
+
+def host_adjoint(L, prob, phase, E, nsteps, dt, K, gpos):
+    import vx3mi
+    # the lattice description the product's builder reads (built like Model.from_problem)
+    p = dict(vx3mi.PROBLEM_DEFAULTS)
+    p.update(prob)
+    nx, ny, nz = p["dims"]
+    cells = np.ascontiguousarray(p["cells"], dtype=np.int8).reshape(-1)
+    mats = [dict(vx3mi.MATERIAL_DEFAULTS, **m) for m in p["materials"]]
+    marr = (vx3mi._Material * len(mats))()
+    for i, m in enumerate(mats):
+        r, g, b, a = m["rgba"]
+        marr[i] = vx3mi._Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"], -1.0, 0, int(m["fixed"]),
+                                  int(m["is_measured"]), 0, 0.0, r, g, b, a)
+    spec = vx3mi._Lattice(nx, ny, nz, cells.ctypes.data_as(C.POINTER(C.c_int8)), None, None, len(mats), marr,
+                          p["lattice_dim"], p["dt_frac"], p["bond_damp_z"], p["col_damp_z"], p["slow_damp_z"], 0, 0, 0,
+                          p["grav_acc"], int(p["floor_enabled"]), int(p["temp_enabled"]), int(p["vary_temp"]),
+                          p["temp_base"], p["temp_amplitude"], p["temp_period"])
+    nv = int((cells > 0).sum())
+    ph, e = np.ascontiguousarray(phase, np.float64), np.ascontiguousarray(E, np.float64)
+    g = np.ascontiguousarray(gpos, np.float64)
+    gphase, gE, pos = np.zeros(nv), np.zeros(nv), np.zeros((nv, 3))
+    ptr = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    assert L.harness_adjoint(C.byref(spec), ptr(ph), ptr(e), nsteps, dt, K, ptr(g), ptr(gphase), ptr(gE), ptr(pos)) == 0
+    return gphase, gE, pos
+
+
+@pytest.mark.parametrize("K", [200, 7])
+def test_adjoint_pipeline_on_host_matches_finite_differences(adjoint_host, K):
+    """The GPU kernels' work items, run on the host in double: forward parity
+    with the oracle's differentiable mode and gradients against the FP64
+    finite differences of the fixture, with one segment and with K = 7
+    (checkpoint and recompute)."""
+    import sys
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    import make_diff_fd as FD
+    fix = np.load(os.path.join(ROOT, "tests", "golden", "diff_fd_4x4x4.npz"))
+    prob, psi, log10E = FD.subset_problem()
+    E = 10.0 ** log10E
+    nsteps, dt = int(fix["nsteps"]), float(fix["dt"])
+    gpos = np.zeros((64, 3))
+    gpos[:, 0] = -1.0 / 64
+    gphase, gE, pos = host_adjoint(adjoint_host, prob, psi / (2 * math.pi), E, nsteps, dt, K, gpos)
+    o = O.OracleDiff(prob, psi / (2 * math.pi), E, dt=dt)
+    o.step(nsteps)
+    assert np.max(np.abs(pos - o.state()["pos"])) < 1e-14
+    g_psi, g_l = gphase / (2 * math.pi), gE * E * math.log(10.0)
+    rel_psi = np.linalg.norm(g_psi - fix["grad_psi"]) / np.linalg.norm(fix["grad_psi"])
+    rel_l = np.linalg.norm(g_l - fix["grad_log10E"]) / np.linalg.norm(fix["grad_log10E"])
+    assert rel_psi < 1e-6 and rel_l < 1e-6, (rel_psi, rel_l)

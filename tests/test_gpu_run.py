@@ -1,0 +1,134 @@
+"""The simulation driver (CUDA_Simulation loop) and the command-line drop-in on
+the GPU: stop formula, centre-of-mass and fitness at the stop, .history frames,
+robots stopping at different steps inside one batch, and the report XML.
+
+FP64 batches reproduce the oracle bit for bit, so final centres of mass are
+compared with the oracle run for the same number of steps at 1e-12.
+"""
+import math
+import os
+import re
+import subprocess
+import xml.etree.ElementTree as ET
+
+import numpy as np
+import pytest
+
+import oracle as O
+import vx3mi
+from vx3mi import configs
+from vx3mi import vxa as W
+
+pytestmark = pytest.mark.gpu
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def log(msg):
+    print(msg, flush=True)
+
+
+def steps_until(dt, t_stop):
+    """Iterations the reference runs with stop formula t - t_stop (> 0 stops)."""
+    t, n = 0.0, 0
+    while not (t - t_stop > 0):
+        t += float(np.float32(dt))
+        n += 1
+    return n, t
+
+
+def small_robot(seed, dims=(3, 3, 2)):
+    return configs.random_robot(np.random.default_rng(seed), dims, 0.8, enable_collision=0)
+
+
+def test_stop_formula_fitness_and_history():
+    prob = small_robot(1)
+    t_stop = 0.004
+    text = W.write_vxa(prob, stop=("mtSUB", ("mtVAR", "t"), ("mtCONST", t_stop)),
+                       fitness=("mtVAR", "x"), record=(5, 1, 1))
+    m = vx3mi.Model.from_xml(text)
+    b = vx3mi.Batch([m], vx3mi.FP64)
+    log("runner: single robot FP64")
+    res, hist, finals = vx3mi.run(b, [m])
+    r = res[0]
+    dt = m.settings()["dt_frac"] * m.recommended_dt()
+    n, t = steps_until(dt, t_stop)
+    assert r["steps"] == n and r["current_time"] == t and not r["diverged"]
+    o = O.OracleModel(prob)
+    c0 = o.com()
+    o.step(n)
+    assert np.allclose(r["initial_com"], c0, rtol=0, atol=1e-15)
+    assert np.allclose(r["current_com"], o.com(), rtol=1e-12, atol=0)
+    assert r["fitness_score"] == pytest.approx(o.com()[0] - c0[0], rel=1e-12)
+    assert np.allclose(finals[0], o.state()["pos"], rtol=1e-12, atol=0)
+    # frames every real_stepsize iterations, positions in mm with one decimal
+    real = int(5 / (10000 * m.recommended_dt() * m.settings()["dt_frac"])) + 1
+    assert r["real_stepsize"] == real
+    frames = re.findall(r"<<<Step(\d+) Time:([0-9.]+)>>>(.*?)<<<>>>", hist[0])
+    assert [int(f[0]) for f in frames] == list(range(0, n, real))
+    assert hist[0].startswith("\n{{{setting}}}<rescale>0.001</rescale>\n{{{setting}}}<matcolor><id>1</id>")
+    # the last frame's first surface voxel against the oracle state after that iteration
+    last = int(frames[-1][0])
+    o2 = O.OracleModel(prob)
+    o2.step(last + 1)
+    v = int(o2.topology()["surface"][0])
+    p = o2.state()["pos"][v] * 1000
+    first = frames[-1][2].split(";")[0].split(",")
+    assert first[:3] == ["%.1f" % p[0], "%.1f" % p[1], "%.1f" % p[2]]
+    assert "|[[[" in hist[0] and "[[[]]]" in hist[0]
+
+
+def test_robots_stop_independently_in_one_batch():
+    """Three robots with different stop times run in one batch; each one's
+    results equal those of the robot run alone."""
+    probs = [small_robot(s) for s in (2, 3, 4)]
+    stops = [0.002, 0.0035, 0.0011]
+    models = [vx3mi.Model.from_xml(W.write_vxa(p, stop=("mtSUB", ("mtVAR", "t"), ("mtCONST", ts)),
+                                               fitness=("mtVAR", "x")))
+              for p, ts in zip(probs, stops)]
+    log("runner: three robots in one batch")
+    res, _, fin = vx3mi.run(vx3mi.Batch(models, vx3mi.FP64), models, history=False)
+    for k, mdl in enumerate(models):
+        alone, _, fa = vx3mi.run(vx3mi.Batch([mdl], vx3mi.FP64), [mdl], history=False)
+        assert res[k]["steps"] == alone[0]["steps"]
+        assert res[k]["current_com"] == alone[0]["current_com"]
+        assert res[k]["fitness_score"] == alone[0]["fitness_score"]
+        assert np.array_equal(fin[k], fa[0])
+
+
+def test_max_steps_bound_without_stop_formula():
+    prob = small_robot(5)
+    m = vx3mi.Model.from_xml(W.write_vxa(prob))
+    res, hist, _ = vx3mi.run(vx3mi.Batch([m], vx3mi.FP32), [m], max_steps=37)
+    assert res[0]["steps"] == 37 and hist[0] == ""
+
+
+def test_cli_report(tmp_path):
+    exe = os.path.join(ROOT, "voxcraft-sim_amd", "lib", "vx3mi_run")
+    if not os.path.exists(exe):
+        pytest.fail("vx3mi_run not built (make -C voxcraft-sim_amd cli)")
+    d = tmp_path / "gen"
+    d.mkdir()
+    probs = [small_robot(s) for s in (6, 7)]
+    base, _ = W.write_pair(probs[0], stop=("mtSUB", ("mtVAR", "t"), ("mtCONST", 0.002)), fitness=("mtVAR", "x"),
+                           record=(5, 1, 0))
+    (d / "base.vxa").write_text(base)
+    for k, p in enumerate(probs):
+        (d / f"robot{k}.vxd").write_text(W.write_pair(p, record=(5, 1, 0))[1])
+    out = tmp_path / "report.xml"
+    log("runner: command line")
+    r = subprocess.run([exe, "-i", str(d), "-o", str(out)], capture_output=True, timeout=120)
+    assert r.returncode == 0, r.stdout.decode() + r.stderr.decode()
+    stdout = r.stdout.decode()
+    assert stdout.count("<<<Step0 Time:") == 2 and "real_stepsize:" in stdout
+    rep = ET.parse(out).getroot()
+    assert rep.tag == "report" and rep.find("inputdir").text == "gen"
+    details = list(rep.find("detail"))
+    assert sorted(e.tag for e in details) == ["robot0", "robot1"]
+    fits = [float(e.find("fitness_score").text) for e in details]
+    assert fits == sorted(fits, reverse=True)
+    assert rep.find("bestfit/fitness_score").text == details[0].find("fitness_score").text
+    for e in details:
+        c0 = [float(e.find(f"initialCenterOfMass/{a}").text) for a in "xyz"]
+        c1 = [float(e.find(f"currentCenterOfMass/{a}").text) for a in "xyz"]
+        assert float(e.find("fitness_score").text) == pytest.approx(c1[0] - c0[0], rel=1e-6, abs=1e-12)
+        assert float(e.find("currentTime").text) > 0.002

@@ -11,8 +11,6 @@
 
 namespace vx3mi {
 
-template <typename T> struct alignas(4 * sizeof(T)) T4 { T x, y, z, w; };
-
 struct RobotParams {
     double tempAmp, tempPeriod;
     int varyTemp, enableExpansion, collisions, pad;
@@ -92,6 +90,9 @@ struct BatchBase {
     virtual int eval_links(double *out) = 0;
     virtual int com(double *out) = 0;
     virtual int recommended(double *out) = 0;
+    // one robot's absolute positions, orientations, temperatures and link
+    // strains (robot voxel / link order)
+    virtual int frame(int robot, double *pos, double *quat, float *temp, float *strain) = 0;
     RobotTime *d_rt = nullptr;
 };
 
@@ -99,3 +100,7 @@ BatchBase *make_batch_fp32(vx3mi_model *const *models, int n, int *rc);
 BatchBase *make_batch_fp64(vx3mi_model *const *models, int n, int *rc);
 
 }  // namespace vx3mi
+
+struct vx3mi_batch {
+    vx3mi::BatchBase *b = nullptr;
+};

@@ -9,9 +9,6 @@
 
 using namespace vx3mi;
 
-struct vx3mi_batch {
-    BatchBase *b = nullptr;
-};
 
 extern "C" {
 

@@ -773,6 +773,32 @@ template <typename T> struct Batch : BatchBase {
         return VX3MI_OK;
     }
 
+    int frame(int robot, double *pos, double *quat, float *temp, float *strain) override {
+        if (robot < 0 || robot >= R) { set_error("robot index out of range"); return VX3MI_E_ARG; }
+        const RobotParams &p = hrp[robot];
+        std::vector<T4<T>> a0((size_t)p.vcount), a1((size_t)p.vcount), l2((size_t)p.lcount);
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (p.vcount) {
+            HIP_TRY(hipMemcpy(a0.data(), d.pq0 + p.voff, sizeof(T4<T>) * p.vcount, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(a1.data(), d.pq1 + p.voff, sizeof(T4<T>) * p.vcount, hipMemcpyDeviceToHost));
+        }
+        if (p.lcount) HIP_TRY(hipMemcpy(l2.data(), d.ls2 + p.loff, sizeof(T4<T>) * p.lcount, hipMemcpyDeviceToHost));
+        for (int i = 0; i < p.vcount; i++) {
+            const int v = p.voff + i, pk = hlat[v];
+            const double Ls = kAbsolutePositions<T> ? 0.0 : hvmats[hvmat[v]].nomSize;
+            if (pos) {
+                pos[3 * i] = (pk & 1023) * Ls + (double)a0[i].x;
+                pos[3 * i + 1] = ((pk >> 10) & 1023) * Ls + (double)a0[i].y;
+                pos[3 * i + 2] = (pk >> 20) * Ls + (double)a0[i].z;
+            }
+            if (quat) { quat[4 * i] = a0[i].w; quat[4 * i + 1] = a1[i].x; quat[4 * i + 2] = a1[i].y; quat[4 * i + 3] = a1[i].z; }
+            if (temp) temp[i] = (float)a1[i].w;
+        }
+        for (int l = 0; l < p.lcount; l++)
+            if (strain) strain[l] = (float)l2[l].w;
+        return VX3MI_OK;
+    }
+
     int get_flags(int robot, int *flags) override {
         std::vector<T4<T>> a0, a1, b0, b1;
         int rc = fetch(a0, a1, b0, b1);

@@ -39,6 +39,11 @@ VX_HD double mexp(double a) { return exp(a); }
 VX_HD float mlog1p(float a) { return log1pf(a); }
 VX_HD double mlog1p(double a) { return log1p(a); }
 
+// link end pair (pVNeg, pVPos), the layout of HIP's int2
+struct int2v {
+    int x, y;
+};
+
 // ---- per-voxel / per-link constants of the differentiable mode -------------
 template <typename T> struct DVox {
     T mInv, IInv;      // 1/mass, 1/inertia
@@ -551,7 +556,6 @@ VX_HD void d_voxel_vjp(const DVox<T> &c, const DiffOpts &o, T dt, T temp, V3<T> 
         tf.x -= g * vel.x;
         tf.y -= g * vel.y;
     }
-    const V3<T> p1 = s.p + tf * dt;
     const V3<T> tmM = qrot(s.q, M);
     const V3<T> tm = tmM - s.h * (c.IInv * c.gDampR);
     const V3<T> h1 = s.h + tm * dt;

@@ -38,6 +38,7 @@ struct Model {
     // voxels (voxel order)
     std::vector<int> ix, iy, iz;
     std::vector<int> vmat;          // index into vmats
+    std::vector<int> vpal;          // palette index (1-based, the reference's matid)
     std::vector<double> phase;      // cycles
     std::vector<int> vlinks;        // 6 per voxel, X_POS..Z_NEG, -1 none
     std::vector<int> surface;       // surface voxel indices
@@ -57,6 +58,8 @@ struct Model {
     // settings carried for the drop-in runner (VX3_SimulationManager.cu:323-376)
     std::vector<MtToken> stopFormula, fitnessFormula;
     int recordStepSize = 0, recordVoxel = 1, recordLink = 0;
+    std::string rawPrint;           // VXA.RawPrint
+    int savePositions = 0;          // Simulator/SavePositionOfAllVoxels
     int secondaryExperiment = 0;
     double reinitInitialPositionAfter = 0.0;
     int nv() const { return (int)ix.size(); }

@@ -222,6 +222,7 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
             mi = palToVm[c - 1];
         }
         m.vmat.push_back(mi);
+        m.vpal.push_back(c);
         m.phase.push_back(sp->phase ? sp->phase[nv] : 0.0);
         nv++;
     }
@@ -335,6 +336,15 @@ int vx3mi_model_settings(const vx3mi_model *m, vx3mi_settings *o) {
     o->record_link = M.recordLink;
     o->has_stop_formula = !M.stopFormula.empty();
     o->has_fitness_formula = !M.fitnessFormula.empty();
+    return VX3MI_OK;
+}
+
+const char *vx3mi_model_raw_print(const vx3mi_model *m) { return m ? m->m.rawPrint.c_str() : ""; }
+int vx3mi_model_save_positions(const vx3mi_model *m) { return m ? m->m.savePositions : 0; }
+
+int vx3mi_model_palette_index(const vx3mi_model *m, int *out) {
+    if (!m || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    for (int v = 0; v < m->m.nv(); v++) out[v] = m->m.vpal[v];
     return VX3MI_OK;
 }
 

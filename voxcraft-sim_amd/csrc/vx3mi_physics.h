@@ -37,6 +37,8 @@ namespace vx3mi {
 template <typename T> constexpr bool kAbsolutePositions = std::is_same<T, double>::value;
 
 template <typename T> struct V3 { T x, y, z; };
+// four-component device storage unit (one 16 / 32 byte load)
+template <typename T> struct alignas(4 * sizeof(T)) T4 { T x, y, z, w; };
 template <typename T> struct Q4 { T w, x, y, z; };
 
 VX_HD float msqrt(float a) { return sqrtf(a); }

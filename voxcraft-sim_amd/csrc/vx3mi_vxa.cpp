@@ -286,6 +286,8 @@ struct VxaContent {
     std::vector<double> phase;
     std::vector<vx3mi_material> mats;
     std::vector<MtToken> stop, fitness;
+    std::string rawPrint;
+    int savePositions = 0;
     int recordStepSize = 0, recordVoxel = 1, recordLink = 0;
     int secondaryExperiment = 0;
     double reinitAfter = 0.0;
@@ -349,6 +351,7 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
     if (!vxa) { err = "no <VXA> root element"; return VX3MI_E_PARSE; }
     vx3mi_lattice &L = c.lat;
     std::memset(&L, 0, sizeof(L));
+    if (const XNode *rp = vxa->child("RawPrint")) c.rawPrint = rp->text;  // VX3_SimulationManager.cu:322
     // ---- simulator (CVX_Sim::ReadXML; DtFrac defaults to 0.9 in CVX_Sim)
     L.dt_frac = 0.9;
     // damping: the reference leaves the zetas uninitialised without <Damping>; the
@@ -385,6 +388,7 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
             return VX3MI_E_PARSE;
         }
         load(sim, "SecondaryExperiment", c.secondaryExperiment);
+        load(sim, "SavePositionOfAllVoxels", c.savePositions);
         load(sim, "ReinitializeInitialPositionAfterThisManySeconds", c.reinitAfter);
         if (const XNode *rh = sim->child("RecordHistory")) {
             load(rh, "RecordStepSize", c.recordStepSize);
@@ -571,6 +575,8 @@ int model_from_docs(const char *baseText, const char *vxdText, vx3mi_model **out
     int rc = vx3mi_model_from_lattice(&c.lat, out);
     if (rc == VX3MI_OK) {
         Model &m = (*out)->m;
+        m.rawPrint = c.rawPrint;
+        m.savePositions = c.savePositions;
         m.stopFormula = c.stop;
         m.fitnessFormula = c.fitness;
         m.secondaryExperiment = c.secondaryExperiment;

@@ -16,7 +16,7 @@ from typing import Iterable, Sequence
 import numpy as np
 
 __all__ = ["lib", "Model", "Batch", "VX3MIError", "FP32", "FP64", "MODE_AUTO", "MODE_PLAIN", "MODE_FLAT",
-           "MODE_PERSISTENT", "MODE_TILED", "Diff", "FORMULA_STOP", "FORMULA_FITNESS", "PROBLEM_DEFAULTS", "MATERIAL_DEFAULTS", "library_path"]
+           "MODE_PERSISTENT", "MODE_TILED", "Diff", "run", "FORMULA_STOP", "FORMULA_FITNESS", "PROBLEM_DEFAULTS", "MATERIAL_DEFAULTS", "library_path"]
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 _LIB_PATH = os.path.join(_PKG, "lib", "libvx3mi.so")
@@ -63,6 +63,17 @@ class _DiffOpts(C.Structure):
     _fields_ = [("floor_stiffness", C.c_double), ("floor_width", C.c_double), ("friction_speed", C.c_double),
                 ("dt", C.c_double), ("e_max", C.c_double), ("checkpoint_every", C.c_int),
                 ("hbm_budget", C.c_double)]
+
+
+class _Result(C.Structure):
+    _fields_ = [("current_time", C.c_double), ("fitness_score", C.c_double), ("initial_com", C.c_double * 3),
+                ("current_com", C.c_double * 3), ("total_distance_of_all_voxels", C.c_double),
+                ("vox_size", C.c_double), ("recommended_dt", C.c_double), ("num_voxel", C.c_int),
+                ("num_measured_voxel", C.c_int), ("num_close_pairs", C.c_int), ("diverged", C.c_int),
+                ("angle_sample_times", C.c_int), ("real_stepsize", C.c_int), ("steps", C.c_longlong)]
+
+
+_TEXT_FN = C.CFUNCTYPE(None, C.c_void_p, C.c_int, C.POINTER(C.c_char), C.c_size_t)
 
 
 class _Lattice(C.Structure):
@@ -123,6 +134,11 @@ def lib():
         "vx3mi_eval_link_forces": (C.c_int, [vp, dp]),
         "vx3mi_last_step_ms": (C.c_int, [vp, fp]),
         "vx3mi_last_step_launches": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+        "vx3mi_run": (C.c_int, [vp, pvp, C.c_int, C.c_longlong, C.c_int, _TEXT_FN, vp, C.POINTER(_Result),
+                                C.POINTER(dp)]),
+        "vx3mi_model_raw_print": (C.c_char_p, [vp]),
+        "vx3mi_model_palette_index": (C.c_int, [vp, ip]),
+        "vx3mi_model_save_positions": (C.c_int, [vp]),
         "vx3mi_diff_default_opts": (None, [C.POINTER(_DiffOpts)]),
         "vx3mi_diff_create": (C.c_int, [pvp, C.c_int, C.c_int, C.POINTER(_DiffOpts), pvp]),
         "vx3mi_diff_destroy": (None, [vp]),
@@ -399,3 +415,28 @@ class Diff:
         _check(lib().vx3mi_diff_info(self._h, _ptr(o, C.c_double)))
         return dict(checkpoint_every=int(o[0]), checkpoint_bytes=o[1], tape_bytes=o[2], dt=o[3], forward_ms=o[4],
                     backward_ms=o[5], steps=int(o[6]), precision=int(o[7]))
+
+
+def run(batch: "Batch", models: Sequence[Model], max_steps: int = 0, mode: int = MODE_AUTO, history: bool = True):
+    """The simulation driver (CUDA_Simulation) over a batch: returns per robot a
+    result dict, its .history text (header and frames) and its final positions."""
+    models = list(models)
+    n = len(models)
+    texts = [[] for _ in range(n)]
+
+    def on_text(user, robot, text, length):
+        texts[robot].append(C.string_at(text, length).decode())
+
+    cb = _TEXT_FN(on_text) if history else _TEXT_FN()
+    res = (_Result * n)()
+    finals = [np.zeros((m.nv, 3)) for m in models]
+    fp = (C.POINTER(C.c_double) * n)(*[_ptr(f, C.c_double) for f in finals])
+    arr = (C.c_void_p * n)(*[m._h.value for m in models])
+    _check(lib().vx3mi_run(batch._h, arr, n, int(max_steps), int(mode), cb, None, res, fp))
+    out = []
+    for r in range(n):
+        d = {name: getattr(res[r], name) for name, _ in _Result._fields_}
+        d["initial_com"] = list(res[r].initial_com)
+        d["current_com"] = list(res[r].current_com)
+        out.append(d)
+    return out, ["".join(t) for t in texts], finals
