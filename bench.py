@@ -108,6 +108,34 @@ def fp32_error_vs_oracle(problems, nsteps: int) -> float:
     return worst
 
 
+def adjoint_cost(problems, nsteps: int, checkpoint_every: int, budget: float, precision=None):
+    """Forward (with checkpoints) and backward (recompute + reverse sweep) of the
+    differentiable mode on the given robots; loss = -(x displacement of each
+    robot's centre of mass).  Returns timings and checkpoint sizes."""
+    precision = vx3mi.FP32 if precision is None else precision
+    models = [vx3mi.Model.from_problem(p) for p in problems]
+    d = vx3mi.Diff(models, precision, checkpoint_every=checkpoint_every, hbm_budget=budget)
+    phase = np.concatenate([np.asarray(p["phase"], dtype=np.float64) for p in problems])
+    E = np.concatenate([np.asarray(p["voxel_E"], dtype=np.float64) for p in problems])
+    d.set_params(phase, E)
+    d.forward(min(nsteps, 200))  # warm-up (code load, buffers)
+    d.backward(grad_pos=np.zeros((d.nv, 3)))
+    d.forward(nsteps)
+    g = np.zeros((d.nv, 3))
+    off = d.voxel_offsets
+    for r in range(d.nr):
+        g[off[r]:off[r + 1], 0] = -1.0 / (off[r + 1] - off[r])
+    d.backward(grad_pos=g)
+    info = d.info()
+    d.close()
+    return {"robots": len(problems), "voxels": int(off[-1]), "steps": nsteps,
+            "forward_ms": info["forward_ms"], "backward_ms": info["backward_ms"],
+            "adjoint_over_forward": info["backward_ms"] / info["forward_ms"],
+            "checkpoint_every": info["checkpoint_every"], "checkpoint_bytes": info["checkpoint_bytes"],
+            "tape_bytes": info["tape_bytes"], "hbm_budget_bytes": budget,
+            "forward_voxel_steps_per_s": off[-1] * nsteps / (info["forward_ms"] * 1e-3)}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -121,6 +149,8 @@ def main() -> int:
     ap.add_argument("--check-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--mode", choices=["tiled", "persistent", "flat"], default="tiled")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-budget", type=float, default=16.0 * 2**30)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); robots shard across GPUs "
@@ -184,6 +214,17 @@ def main() -> int:
                          "single thread"}
         log(f"bench: CPU baseline ({kind}) {cv:.3e} voxel-steps/s")
 
+    adjoint = None
+    if not args.no_adjoint:
+        c3 = adjoint_cost([configs.config3()], 2000, 100, args.adjoint_budget)
+        log(f"bench: adjoint 8x8x8 2000 steps K=100: forward {c3['forward_ms']:.1f} ms, backward "
+            f"{c3['backward_ms']:.1f} ms, ratio {c3['adjoint_over_forward']:.2f}")
+        c4 = adjoint_cost(configs.config4(), 5000, 0, args.adjoint_budget)
+        log(f"bench: batched adjoint 256 robots 5000 steps K={c4['checkpoint_every']}: forward {c4['forward_ms']:.1f} ms, "
+            f"backward {c4['backward_ms']:.1f} ms, ratio {c4['adjoint_over_forward']:.2f}")
+        adjoint = {"single_8x8x8": c3, "batched_256": c4,
+                   "note": "backward = recompute from checkpoints + reverse sweep; FP32; softplus floor; collisions off"}
+
     out = {
         "metric": "voxel-steps/s batched vs plain @<=1e-5 rel err to FP64; adjoint/forward cost",
         "value": value,
@@ -208,6 +249,7 @@ def main() -> int:
                      "bytes_per_voxel_step": bpv, "kernel": kernel,
                      "steps_per_launch": steps_per_launch},
         "cpu_baseline": cpu,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
     batch.close()
