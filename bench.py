@@ -148,8 +148,13 @@ def main() -> int:
     ap.add_argument("--check-robots", type=int, default=16)
     ap.add_argument("--check-steps", type=int, default=1000)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--mode", choices=["tiled", "persistent", "flat"], default="tiled")
+    ap.add_argument("--mode", choices=["auto", "tiled", "persistent", "flat"], default="auto",
+                    help="auto: the batch autotuner times every launch configuration and keeps the fastest")
+    ap.add_argument("--tune-steps", type=int, default=200)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--copy-probe-mb", type=int, default=0,
+                    help="after the timed region, copy a device buffer of this size once (a copy kernel of "
+                         "known size, to calibrate HBM byte counters)")
     ap.add_argument("--adjoint-budget", type=float, default=16.0 * 2**30)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -157,8 +162,8 @@ def main() -> int:
               "as independent batches, which this build does not launch.", file=sys.stderr)
         return 2
 
-    mode = {"tiled": vx3mi.MODE_TILED, "persistent": vx3mi.MODE_PERSISTENT, "flat": vx3mi.MODE_FLAT}[args.mode]
-    kernel = {"tiled": "k_tile<float>", "persistent": "k_persistent<float>", "flat": "k_links<float> + k_voxels<float>"}[args.mode]
+    mode = {"auto": vx3mi.MODE_AUTO, "tiled": vx3mi.MODE_TILED, "persistent": vx3mi.MODE_PERSISTENT,
+            "flat": vx3mi.MODE_FLAT}[args.mode]
     problems = configs.config2(seed=0, n=args.robots)
     t0 = time.perf_counter()
     models = [vx3mi.Model.from_problem(p) for p in problems]
@@ -166,7 +171,18 @@ def main() -> int:
     nv, nl = batch.nv, batch.nl
     log(f"bench: {args.robots} robots, {nv} voxels, {nl} links, setup {time.perf_counter() - t0:.2f}s")
 
-    # warmup (also builds nothing lazily: persistent launches need no graph)
+    tune = None
+    if mode == vx3mi.MODE_AUTO:
+        tune = batch.autotune(args.tune_steps)
+        chosen = tune["mode"]
+        log(f"bench: autotune ({args.tune_steps} steps each): {tune['ms']} -> mode {chosen} tile {tune['tile']} "
+            f"threads {tune['threads']}")
+    else:
+        chosen = mode
+    kernel = {vx3mi.MODE_TILED: "k_tile<float, %d>" % (tune["tile"] if tune else 0),
+              vx3mi.MODE_PERSISTENT: "k_persistent<float, %d>" % (tune["threads"] if tune else 256),
+              vx3mi.MODE_FLAT: "k_links<float> + k_voxels<float>"}.get(chosen, "?")
+    # warmup
     batch.step(args.warmup, mode=mode)
     batch.sync()
     # timed region: inputs resident in HBM, device events around the launches
@@ -180,6 +196,17 @@ def main() -> int:
     value = nv * args.steps / (dev_ms * 1e-3)
     log(f"bench: batched {args.steps} steps: device {dev_ms:.1f} ms, wall {wall * 1e3:.1f} ms, "
         f"{launches} launches, {value:.3e} voxel-steps/s, diverged robots {div}")
+
+    if args.copy_probe_mb > 0:
+        import torch
+
+        n = args.copy_probe_mb * 2**20 // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"bench: copy probe {args.copy_probe_mb} MiB device to device")
+        del src, dst
 
     # plain path: one robot per launch sequence
     plain = vx3mi.Batch(models, vx3mi.FP32)
@@ -250,6 +277,7 @@ def main() -> int:
                      "steps_per_launch": steps_per_launch},
         "cpu_baseline": cpu,
         "adjoint": adjoint,
+        "autotune": tune,
     }
     print(json.dumps(out), flush=True)
     batch.close()

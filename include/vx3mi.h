@@ -179,6 +179,19 @@ int vx3mi_eval_link_forces(vx3mi_batch *b, double *out12);
 int vx3mi_last_step_ms(vx3mi_batch *b, float *ms);
 int vx3mi_last_step_launches(vx3mi_batch *b, long long *launches);
 
+/* Autotuner: times trial_steps (<= 0: 100) of each launch configuration on
+ * this batch -- [0] tiled, 512-thread tiles; [1] tiled, 256-thread tiles (two
+ * workgroups per compute unit); [2..4] one workgroup of 128 / 256 / 512
+ * threads per robot; [5] flat launches in captured graphs -- writes the
+ * milliseconds to ms6 (-1: not applicable), keeps the fastest for
+ * VX3MI_MODE_AUTO and reports it in choice3 (mode, tile variant, threads per
+ * robot).  The batch state is restored afterwards. */
+int vx3mi_batch_autotune(vx3mi_batch *b, int trial_steps, double *ms6, int *choice3);
+/* fixes the configuration VX3MI_MODE_AUTO uses: mode (PLAIN..TILED), tile
+ * variant (0 large, 1 small) and threads per robot for PERSISTENT (128, 256,
+ * 512; 0 keeps the current) */
+int vx3mi_batch_configure(vx3mi_batch *b, int mode, int tile, int threads);
+
 /* ---- simulation driver --------------------------------------------------------
  * CUDA_Simulation (VX3_SimulationManager.cu:11-121) for every robot of a batch:
  * iterate {stop condition met -> stop; doTimeStep; diverged -> stop; history

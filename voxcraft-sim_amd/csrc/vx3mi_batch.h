@@ -93,6 +93,11 @@ struct BatchBase {
     // one robot's absolute positions, orientations, temperatures and link
     // strains (robot voxel / link order)
     virtual int frame(int robot, double *pos, double *quat, float *temp, float *strain) = 0;
+    // times the launch configurations on trial steps and keeps the fastest for
+    // VX3MI_MODE_AUTO; the batch state is left as it was
+    virtual int autotune(int trialSteps, double *ms, int *choice) = 0;
+    // fixes the launch configuration VX3MI_MODE_AUTO uses
+    virtual int configure(int mode, int tile, int threads) = 0;
     RobotTime *d_rt = nullptr;
 };
 

@@ -12,6 +12,16 @@ using namespace vx3mi;
 
 extern "C" {
 
+int vx3mi_batch_configure(vx3mi_batch *b, int mode, int tile, int threads) {
+    if (!b || !b->b) { set_error("batch is NULL"); return VX3MI_E_ARG; }
+    return b->b->configure(mode, tile, threads);
+}
+
+int vx3mi_batch_autotune(vx3mi_batch *b, int trial_steps, double *ms6, int *choice3) {
+    if (!b || !b->b || !ms6 || !choice3) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    return b->b->autotune(trial_steps, ms6, choice3);
+}
+
 int vx3mi_batch_create(vx3mi_model *const *models, int n, int precision, vx3mi_batch **out) {
     if (!out || !models || n <= 0) { set_error("vx3mi_batch_create: invalid arguments"); return VX3MI_E_ARG; }
     *out = nullptr;

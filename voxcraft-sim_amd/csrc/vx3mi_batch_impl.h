@@ -278,7 +278,7 @@ template <typename T> __global__ __launch_bounds__(256) void k_voxels(Dev<T> a, 
 }
 
 // One workgroup per robot; nsteps steps with two barrier-separated phases.
-template <typename T> __global__ __launch_bounds__(256) void k_persistent(Dev<T> a, int nsteps, int parity0) {
+template <typename T, int NT> __global__ __launch_bounds__(NT) void k_persistent(Dev<T> a, int nsteps, int parity0) {
     const RobotParams rp = a.rp[blockIdx.x];
     const bool col = rp.collisions && rp.scount > 0 && a.contact != nullptr;
     for (int s = 0; s < nsteps; s++) {
@@ -372,10 +372,15 @@ __global__ __launch_bounds__(256) void k_com(const T4<T> *pq0, const int *lat, c
 // ------------------------------------------------------------------- host
 template <typename T> struct Batch : BatchBase {
     Dev<T> d{};
-    int4 *d_tiles = nullptr;    // robot begin, robot count, voxels, links
-    int *d_tileRobots = nullptr;
-    int ntiles = 0;
-    bool tileOK = false;
+    struct TilePlan {
+        bool ok = false;
+        int ntiles = 0;
+        int4 *d_tiles = nullptr;  // robot begin, robot count, voxels, links
+        int *d_tileRobots = nullptr;
+    } plans[2];                   // large and small tile configuration
+    bool tileOK = false;          // some tile plan exists
+    // launch configuration used by VX3MI_MODE_AUTO (set by the autotuner)
+    int autoMode = 0, tileSel = 0, persistThreads = 256;
     RobotParams *d_rp = nullptr;
     VoxelMat *d_vmats = nullptr;
     LinkMat *d_lmats = nullptr;
@@ -566,11 +571,13 @@ template <typename T> struct Batch : BatchBase {
     }
 
     // Longest-processing-time bin packing of robots into tiles (k_tile): at
-    // least one tile per compute unit, robots sorted by link count, each placed
-    // in the least-loaded tile that still has room.
-    int plan_tiles() {
-        using C = TileCfg<T>;
-        tileOK = false;
+    // least one tile per workgroup slot of the device (compute units x
+    // workgroups per unit), robots sorted by link count, each placed in the
+    // least-loaded tile that still has room.
+    template <int SMALL> int plan_tiles_cfg() {
+        using C = TileCfg<T, SMALL>;
+        TilePlan &P = plans[SMALL];
+        P.ok = false;
         if (anyCollisions || R < 2) return VX3MI_OK;
         long totL = 0, totV = 0;
         for (const RobotParams &p : hrp) {
@@ -583,7 +590,8 @@ template <typename T> struct Batch : BatchBase {
             hipDeviceProp_t prop;
             if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0) cus = prop.multiProcessorCount;
         }
-        long want = std::max<long>({(long)cus, (totL + C::LMAX - 1) / C::LMAX, (totV + C::VMAX - 1) / C::VMAX});
+        const long slots = (long)cus * (SMALL ? 2 : 1);
+        long want = std::max<long>({slots, (totL + C::LMAX - 1) / C::LMAX, (totV + C::VMAX - 1) / C::VMAX});
         want = std::min<long>(want, R);
         std::vector<int> order(R);
         for (int r = 0; r < R; r++) order[r] = r;
@@ -611,13 +619,32 @@ template <typename T> struct Batch : BatchBase {
             tiles.push_back(int4{(int)trobots.size(), (int)b.robots.size(), (int)b.v, (int)b.l});
             for (int r : b.robots) trobots.push_back(r);
         }
-        ntiles = (int)tiles.size();
+        P.ntiles = (int)tiles.size();
         int rc;
-        if ((rc = alloc(&d_tiles, tiles.size())) || (rc = alloc(&d_tileRobots, trobots.size()))) return rc;
-        HIP_TRY(hipMemcpy(d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
-        HIP_TRY(hipMemcpy(d_tileRobots, trobots.data(), sizeof(int) * trobots.size(), hipMemcpyHostToDevice));
-        tileOK = true;
+        if ((rc = alloc(&P.d_tiles, tiles.size())) || (rc = alloc(&P.d_tileRobots, trobots.size()))) return rc;
+        HIP_TRY(hipMemcpy(P.d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
+        HIP_TRY(hipMemcpy(P.d_tileRobots, trobots.data(), sizeof(int) * trobots.size(), hipMemcpyHostToDevice));
+        P.ok = true;
         return VX3MI_OK;
+    }
+
+    int plan_tiles() {
+        int rc;
+        if ((rc = plan_tiles_cfg<0>()) || (rc = plan_tiles_cfg<1>())) return rc;
+        tileOK = plans[0].ok || plans[1].ok;
+        tileSel = plans[0].ok ? 0 : 1;
+        autoMode = tileOK ? VX3MI_MODE_TILED : ((R > 1 && V / R <= 4096) ? VX3MI_MODE_PERSISTENT : VX3MI_MODE_FLAT);
+        return VX3MI_OK;
+    }
+
+    void launch_tiled(int k, int par) {
+        if (tileSel == 0) k_tile<T, 0><<<plans[0].ntiles, TileCfg<T, 0>::NT, 0, stream>>>(d, plans[0].d_tiles, plans[0].d_tileRobots, k, par);
+        else k_tile<T, 1><<<plans[1].ntiles, TileCfg<T, 1>::NT, 0, stream>>>(d, plans[1].d_tiles, plans[1].d_tileRobots, k, par);
+    }
+    void launch_persistent(int k, int par) {
+        if (persistThreads == 128) k_persistent<T, 128><<<R, 128, 0, stream>>>(d, k, par);
+        else if (persistThreads == 512) k_persistent<T, 512><<<R, 512, 0, stream>>>(d, k, par);
+        else k_persistent<T, 256><<<R, 256, 0, stream>>>(d, k, par);
     }
 
     // upload per-robot dt if it changed (robot time/prevDt stay on the device)
@@ -645,12 +672,11 @@ template <typename T> struct Batch : BatchBase {
         if (dt == 0.0f) return VX3MI_OK;  // doTimeStep(0) only advances the step counter
         int rc = set_dt(dt);
         if (rc) return rc;
-        if (mode == VX3MI_MODE_TILED && !tileOK) {
+        if (mode == VX3MI_MODE_TILED && !plans[tileSel].ok) {
             set_error("tiled mode needs collisions off and every robot within one tile");
             return VX3MI_E_ARG;
         }
-        if (mode == VX3MI_MODE_AUTO)
-            mode = tileOK ? VX3MI_MODE_TILED : ((R > 1 && V / R <= 4096) ? VX3MI_MODE_PERSISTENT : VX3MI_MODE_FLAT);
+        if (mode == VX3MI_MODE_AUTO) mode = autoMode;
         HIP_TRY(hipEventRecord(ev0, stream));
         launches = 0;
         lastMode = mode;
@@ -669,7 +695,7 @@ template <typename T> struct Batch : BatchBase {
         } else if (mode == VX3MI_MODE_TILED) {
             for (int s = 0; s < nsteps; s += persistentChunk) {
                 const int k = std::min(persistentChunk, nsteps - s);
-                k_tile<T><<<ntiles, TileCfg<T>::NT, 0, stream>>>(d, d_tiles, d_tileRobots, k, parity);
+                launch_tiled(k, parity);
                 parity = (parity + k) & 1;
                 launches++;
             }
@@ -677,7 +703,7 @@ template <typename T> struct Batch : BatchBase {
             // launches of at most persistentChunk steps keep each kernel short
             for (int s = 0; s < nsteps; s += persistentChunk) {
                 const int k = std::min(persistentChunk, nsteps - s);
-                k_persistent<T><<<R, 256, 0, stream>>>(d, k, parity);
+                launch_persistent(k, parity);
                 parity = (parity + k) & 1;
                 launches++;
             }
@@ -770,6 +796,95 @@ template <typename T> struct Batch : BatchBase {
         HIP_TRY(hipMemcpy(d.pq1, a1.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.mo0, b0.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.mo1, b1.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+        return VX3MI_OK;
+    }
+
+    // ---- autotuner: candidates are the large / small tile, persistent
+    // workgroups of 128 / 256 / 512 threads, and flat graph launches
+    static constexpr int kCandidates = 6;
+    int autotune(int trialSteps, double *ms, int *choice) override {
+        if (trialSteps <= 0) trialSteps = 100;
+        // snapshot of everything a step changes
+        struct Buf { void *p; size_t bytes; };
+        std::vector<Buf> bufs = {{d.pq0, sizeof(T4<T>) * V}, {d.pq1, sizeof(T4<T>) * V}, {d.mo0, sizeof(T4<T>) * V},
+                                 {d.mo1, sizeof(T4<T>) * V}, {d.ls0, sizeof(T4<T>) * L}, {d.ls1, sizeof(T4<T>) * L},
+                                 {d.ls2, sizeof(T4<T>) * L}, {d.lf0, sizeof(T4<T>) * L}, {d.lf1, sizeof(T4<T>) * L},
+                                 {d.lf2, sizeof(T4<T>) * L}, {d_rt, sizeof(RobotTime) * R}};
+        if (anyCollisions) {
+            bufs.push_back({d.ccnt[0], sizeof(int) * (size_t)tableBuckets});
+            bufs.push_back({d.ccnt[1], sizeof(int) * (size_t)tableBuckets});
+            bufs.push_back({d.contactW, sizeof(T4<T>) * V});
+        }
+        size_t total = 0;
+        for (const Buf &b : bufs) total += (b.bytes + 255) & ~(size_t)255;
+        char *snap = nullptr;
+        HIP_TRY(hipMalloc((void **)&snap, total));
+        auto copyAll = [&](bool save) -> int {
+            size_t off = 0;
+            for (const Buf &b : bufs) {
+                if (b.bytes) {
+                    void *dst = save ? (void *)(snap + off) : b.p;
+                    const void *src = save ? b.p : (const void *)(snap + off);
+                    HIP_TRY(hipMemcpyAsync(dst, src, b.bytes, hipMemcpyDeviceToDevice, stream));
+                }
+                off += (b.bytes + 255) & ~(size_t)255;
+            }
+            HIP_TRY(hipStreamSynchronize(stream));
+            return VX3MI_OK;
+        };
+        const int parity0 = parity, tile0 = tileSel, threads0 = persistThreads, mode0 = autoMode;
+        const int modes[kCandidates] = {VX3MI_MODE_TILED, VX3MI_MODE_TILED, VX3MI_MODE_PERSISTENT,
+                                        VX3MI_MODE_PERSISTENT, VX3MI_MODE_PERSISTENT, VX3MI_MODE_FLAT};
+        const int tiles[kCandidates] = {0, 1, 0, 0, 0, 0}, threads[kCandidates] = {0, 0, 128, 256, 512, 0};
+        int rc = copyAll(true), best = -1;
+        double bestMs = 0;
+        for (int c = 0; rc == VX3MI_OK && c < kCandidates; c++) {
+            ms[c] = -1;
+            if (modes[c] == VX3MI_MODE_TILED && !plans[tiles[c]].ok) continue;
+            tileSel = tiles[c];
+            if (threads[c]) persistThreads = threads[c];
+            if ((rc = step(std::min(trialSteps, 64), -1.0f, modes[c])) < 0) break;  // code load, graph capture
+            if ((rc = copyAll(false))) break;
+            parity = parity0;
+            if ((rc = step(trialSteps, -1.0f, modes[c])) < 0) break;
+            HIP_TRY(hipStreamSynchronize(stream));
+            float t = 0;
+            HIP_TRY(hipEventElapsedTime(&t, ev0, ev1));
+            ms[c] = t;
+            if (best < 0 || t < bestMs) { best = c; bestMs = t; }
+            if ((rc = copyAll(false))) break;
+            parity = parity0;
+        }
+        if (rc == VX3MI_DIVERGED) rc = VX3MI_OK;
+        (void)hipFree(snap);
+        if (rc != VX3MI_OK) {
+            tileSel = tile0; persistThreads = threads0; autoMode = mode0; parity = parity0;
+            return rc;
+        }
+        autoMode = modes[best];
+        tileSel = modes[best] == VX3MI_MODE_TILED ? tiles[best] : (plans[0].ok ? 0 : 1);
+        persistThreads = threads[best] ? threads[best] : 256;
+        parity = parity0;
+        if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+        choice[0] = autoMode;
+        choice[1] = tileSel;
+        choice[2] = persistThreads;
+        return VX3MI_OK;
+    }
+
+    int configure(int mode, int tile, int threads) override {
+        if (mode == VX3MI_MODE_TILED && (tile < 0 || tile > 1 || !plans[tile].ok)) {
+            set_error("this tile variant cannot hold every robot of the batch (or collisions are on)");
+            return VX3MI_E_ARG;
+        }
+        if (threads != 0 && threads != 128 && threads != 256 && threads != 512) {
+            set_error("threads per robot must be 128, 256 or 512");
+            return VX3MI_E_ARG;
+        }
+        if (mode < VX3MI_MODE_PLAIN || mode > VX3MI_MODE_TILED) { set_error("invalid mode"); return VX3MI_E_ARG; }
+        autoMode = mode;
+        if (mode == VX3MI_MODE_TILED) tileSel = tile;
+        if (threads) persistThreads = threads;
         return VX3MI_OK;
     }
 

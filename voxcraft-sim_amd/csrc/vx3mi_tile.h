@@ -16,12 +16,22 @@
 
 namespace vx3mi {
 
-template <typename T> struct TileCfg;
-template <> struct TileCfg<float> {
+// Tile configurations: NT threads, up to KL links and KV voxels per thread.
+// The large tile uses ~151 KB of LDS (one workgroup per compute unit); the
+// small one half of that (two workgroups per compute unit, so one can compute
+// while the other waits at a barrier).  The batch's autotuner picks one.
+template <typename T, int SMALL> struct TileCfg;
+template <> struct TileCfg<float, 0> {
     static constexpr int NT = 512, KL = 4, KV = 3, LMAX = 2048, VMAX = 1536, RMAX = 128;
 };
-template <> struct TileCfg<double> {
+template <> struct TileCfg<float, 1> {
+    static constexpr int NT = 256, KL = 4, KV = 3, LMAX = 1024, VMAX = 768, RMAX = 64;
+};
+template <> struct TileCfg<double, 0> {
     static constexpr int NT = 512, KL = 2, KV = 2, LMAX = 1024, VMAX = 768, RMAX = 64;
+};
+template <> struct TileCfg<double, 1> {
+    static constexpr int NT = 256, KL = 2, KV = 2, LMAX = 512, VMAX = 384, RMAX = 32;
 };
 
 // LDS robot table of a tile
@@ -47,11 +57,12 @@ __device__ __forceinline__ int tile_slot(const int *pre, int n, int x) {
     return lo;
 }
 
-template <typename T>
-__global__ __launch_bounds__(TileCfg<T>::NT) void k_tile(Dev<T> a, const int4 *tiles, const int *tileRobots, int nsteps,
-                                                         int parity0) {
-    constexpr int NT = TileCfg<T>::NT, KL = TileCfg<T>::KL, KV = TileCfg<T>::KV;
-    constexpr int LMAX = TileCfg<T>::LMAX, VMAX = TileCfg<T>::VMAX, RMAX = TileCfg<T>::RMAX;
+template <typename T, int SMALL>
+__global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, const int4 *tiles, const int *tileRobots,
+                                                                int nsteps, int parity0) {
+    using C = TileCfg<T, SMALL>;
+    constexpr int NT = C::NT, KL = C::KL, KV = C::KV;
+    constexpr int LMAX = C::LMAX, VMAX = C::VMAX, RMAX = C::RMAX;
     __shared__ T sPose[8][VMAX];    // position field xyz, qw, qx, qy, qz, temperature
     __shared__ T sForce[12][LMAX];  // forceNeg, momentNeg, forcePos, momentPos
     __shared__ TileRobots<RMAX> tr;

@@ -134,6 +134,8 @@ def lib():
         "vx3mi_eval_link_forces": (C.c_int, [vp, dp]),
         "vx3mi_last_step_ms": (C.c_int, [vp, fp]),
         "vx3mi_last_step_launches": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+        "vx3mi_batch_autotune": (C.c_int, [vp, C.c_int, dp, ip]),
+        "vx3mi_batch_configure": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "vx3mi_run": (C.c_int, [vp, pvp, C.c_int, C.c_longlong, C.c_int, _TEXT_FN, vp, C.POINTER(_Result),
                                 C.POINTER(dp)]),
         "vx3mi_model_raw_print": (C.c_char_p, [vp]),
@@ -343,6 +345,20 @@ class Batch:
         d = np.zeros(self.nrobot)
         _check(lib().vx3mi_batch_recommended_dt(self._h, _ptr(d, C.c_double)))
         return d
+
+    CANDIDATES = ("tiled-512", "tiled-256x2", "persistent-128", "persistent-256", "persistent-512", "flat-graph")
+
+    def autotune(self, trial_steps: int = 100) -> dict:
+        """Times every launch configuration on trial steps (state restored after)
+        and keeps the fastest for MODE_AUTO."""
+        ms = np.zeros(6)
+        ch = np.zeros(3, dtype=np.int32)
+        _check(lib().vx3mi_batch_autotune(self._h, int(trial_steps), _ptr(ms, C.c_double), _ptr(ch, C.c_int)))
+        return dict(ms={k: (None if v < 0 else float(v)) for k, v in zip(self.CANDIDATES, ms)},
+                    mode=int(ch[0]), tile=int(ch[1]), threads=int(ch[2]))
+
+    def configure(self, mode: int, tile: int = 0, threads: int = 0):
+        _check(lib().vx3mi_batch_configure(self._h, int(mode), int(tile), int(threads)))
 
     def eval_link_forces(self) -> np.ndarray:
         out = np.zeros((self.nl, 12))
