@@ -131,9 +131,76 @@ def adjoint_cost(problems, nsteps: int, checkpoint_every: int, budget: float, pr
     return {"robots": len(problems), "voxels": int(off[-1]), "steps": nsteps,
             "forward_ms": info["forward_ms"], "backward_ms": info["backward_ms"],
             "adjoint_over_forward": info["backward_ms"] / info["forward_ms"],
+            "recompute_ms": info["recompute_ms"], "reverse_ms": info["reverse_ms"],
             "checkpoint_every": info["checkpoint_every"], "checkpoint_bytes": info["checkpoint_bytes"],
             "tape_bytes": info["tape_bytes"], "hbm_budget_bytes": budget,
             "forward_voxel_steps_per_s": off[-1] * nsteps / (info["forward_ms"] * 1e-3)}
+
+
+def e_pos(p, ref, L=0.01):
+    return float(np.max(np.linalg.norm(p - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), L)))
+
+
+def e_quat(q, ref):
+    return float(np.max(np.minimum(np.linalg.norm(q - ref, axis=1), np.linalg.norm(q + ref, axis=1))))
+
+
+def com_rel(c, c0, ref, ref0):
+    d, dr = np.asarray(c) - np.asarray(c0), np.asarray(ref) - np.asarray(ref0)
+    return float(np.linalg.norm(d - dr) / max(np.linalg.norm(dr), 1e-300))
+
+
+def parity_report(c1_steps: int = 20000, c1_window: int = 1000, every: int = 50):
+    """Relative errors per configuration (position e_pos, orientation e_q, centre-of-mass
+    displacement), against the FP64 oracle or the FP64 device run (bit-identical to the
+    oracle on every configuration the parity tests cover)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+
+    out = {}
+    # config 0: FP64 device vs oracle over 10,000 steps
+    p0 = configs.config0()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p0)], vx3mi.FP64)
+    o = O.OracleModel(p0)
+    c00, r00 = b.com()[0], o.com()
+    b.step(10000)
+    o.step(10000)
+    s, r = b.state(), o.state()
+    out["c0_fp64_vs_oracle_10000"] = {"e_pos": e_pos(s["pos"], r["pos"]), "e_q": e_quat(s["quat"], r["quat"]),
+                                     "com": com_rel(b.com()[0], c00, o.com(), r00)}
+    b.close()
+    # config 1: FP32 vs FP64 device, collisions on
+    p1 = configs.config1()
+    m1 = vx3mi.Model.from_problem(p1)
+    b32, b64 = vx3mi.Batch([m1], vx3mi.FP32), vx3mi.Batch([m1], vx3mi.FP64)
+    c32, c64 = b32.com()[0], b64.com()[0]
+    ep = eq = 0.0
+    for k in range(0, c1_window, every):
+        b32.step(every)
+        b64.step(every)
+        s32, s64 = b32.state(), b64.state()
+        ep, eq = max(ep, e_pos(s32["pos"], s64["pos"])), max(eq, e_quat(s32["quat"], s64["quat"]))
+    b32.step(c1_steps - c1_window)
+    b64.step(c1_steps - c1_window)
+    out["c1_fp32_vs_fp64"] = {"e_pos_first_1000": ep, "e_q_first_1000": eq,
+                              f"com_after_{c1_steps}": com_rel(b32.com()[0], c32, b64.com()[0], c64),
+                              "diverged": [int(b32.diverged()[0]), int(b64.diverged()[0])]}
+    b32.close()
+    b64.close()
+    # config 3: differentiable FP32 vs the oracle's differentiable mode, 2000 steps
+    p3 = configs.config3()
+    d = vx3mi.Diff(vx3mi.Model.from_problem(p3), vx3mi.FP32)
+    ph, E = np.asarray(p3["phase"]), np.asarray(p3["voxel_E"], dtype=np.float64)
+    d.set_params(ph, E)
+    d.forward(2000)
+    od = O.OracleDiff(p3, ph, E, dt=d.info()["dt"])
+    x0 = od.state()["pos"]
+    od.step(2000)
+    s, r = d.state(), od.state()
+    out["c3_diff_fp32_vs_oracle_2000"] = {"e_pos": e_pos(s["pos"], r["pos"]), "e_q": e_quat(s["quat"], r["quat"]),
+                                          "com": com_rel(s["pos"].mean(0), x0.mean(0), r["pos"].mean(0), x0.mean(0))}
+    d.close()
+    return out
 
 
 def main() -> int:
@@ -152,6 +219,7 @@ def main() -> int:
                     help="auto: the batch autotuner times every launch configuration and keeps the fastest")
     ap.add_argument("--tune-steps", type=int, default=200)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-parity", action="store_true")
     ap.add_argument("--copy-probe-mb", type=int, default=0,
                     help="after the timed region, copy a device buffer of this size once (a copy kernel of "
                          "known size, to calibrate HBM byte counters)")
@@ -198,15 +266,20 @@ def main() -> int:
         f"{launches} launches, {value:.3e} voxel-steps/s, diverged robots {div}")
 
     if args.copy_probe_mb > 0:
-        import torch
+        # one device-to-device hipMemcpy (the runtime's copy kernel) of a known size
+        import ctypes as C
 
-        n = args.copy_probe_mb * 2**20 // 4
-        src = torch.ones(n, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
+        hip = C.CDLL("libamdhip64.so")
+        nbytes = C.c_size_t(args.copy_probe_mb * 2**20)
+        src, dst = C.c_void_p(), C.c_void_p()
+        assert hip.hipMalloc(C.byref(src), nbytes) == 0 and hip.hipMalloc(C.byref(dst), nbytes) == 0
+        assert hip.hipMemset(src, 1, nbytes) == 0
+        assert hip.hipDeviceSynchronize() == 0
+        assert hip.hipMemcpy(dst, src, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+        assert hip.hipDeviceSynchronize() == 0
+        hip.hipFree(src)
+        hip.hipFree(dst)
         log(f"bench: copy probe {args.copy_probe_mb} MiB device to device")
-        del src, dst
 
     # plain path: one robot per launch sequence
     plain = vx3mi.Batch(models, vx3mi.FP32)
@@ -240,6 +313,12 @@ def main() -> int:
                "sample": f"first {len(sample)} robots of the workload ({svox} voxels) x {args.cpu_steps} steps, "
                          "single thread"}
         log(f"bench: CPU baseline ({kind}) {cv:.3e} voxel-steps/s")
+
+    parity = None
+    if not args.no_parity:
+        t0 = time.perf_counter()
+        parity = parity_report()
+        log(f"bench: parity per config ({time.perf_counter() - t0:.1f}s): {parity}")
 
     adjoint = None
     if not args.no_adjoint:
@@ -278,6 +357,7 @@ def main() -> int:
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "autotune": tune,
+        "parity": parity,
     }
     print(json.dumps(out), flush=True)
     batch.close()

@@ -260,8 +260,9 @@ int vx3mi_diff_get_state(vx3mi_diff *d, int robot, double *pos, double *quat, do
  * dL/dphase (per cycle) and dL/dE per voxel */
 int vx3mi_diff_backward(vx3mi_diff *d, const double *grad_pos, const double *grad_quat, const double *grad_linmom,
                         const double *grad_angmom, double *grad_phase, double *grad_E);
-/* out (8): K, checkpoint bytes, tape bytes, dt, forward ms, backward ms, steps, precision */
-int vx3mi_diff_info(const vx3mi_diff *d, double *out8);
+/* out (10): K, checkpoint bytes, tape bytes, dt, forward ms, backward ms, steps, precision,
+ * recompute ms and reverse-sweep ms of the backward */
+int vx3mi_diff_info(const vx3mi_diff *d, double *out10);
 
 #ifdef __cplusplus
 }

@@ -77,6 +77,7 @@ int vx3mi_diff_backward(vx3mi_diff *d, const double *grad_pos, const double *gra
 }
 
 int vx3mi_diff_info(const vx3mi_diff *d, double *o) {
+    // (the last two entries: recompute and reverse-sweep parts of the backward)
     if (!d || !o) { set_error("NULL argument"); return VX3MI_E_ARG; }
     const DiffBase &b = *d->b;
     o[0] = b.K;
@@ -87,6 +88,8 @@ int vx3mi_diff_info(const vx3mi_diff *d, double *o) {
     o[5] = b.bwdMs;
     o[6] = b.steps;
     o[7] = b.precision;
+    o[8] = b.recomputeMs;
+    o[9] = b.reverseMs;
     return VX3MI_OK;
 }
 

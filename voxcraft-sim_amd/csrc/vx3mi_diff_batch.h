@@ -22,6 +22,8 @@ struct DiffBase {
     bool tapeValid = false;  // the tape holds the whole last forward (no recompute needed)
     float dt = 0;
     float fwdMs = 0, bwdMs = 0;
+    float recomputeMs = 0, reverseMs = 0;  // parts of the last backward
+    std::vector<hipEvent_t> segEv;         // event pairs around the backward's kernels
     double ckBytes = 0, tapeBytes = 0;
     DiffOpts o{};
     vx3mi_diff_opts optsIn{};
@@ -32,6 +34,7 @@ struct DiffBase {
     std::vector<int> hlat;
     std::vector<double> hphase0, hE0, hphase, hE;
     virtual ~DiffBase() {
+        for (hipEvent_t e : segEv) (void)hipEventDestroy(e);
         for (void *p : allocs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);

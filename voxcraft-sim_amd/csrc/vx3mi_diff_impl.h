@@ -371,8 +371,14 @@ template <typename T> struct DiffBatch : DiffBase {
         HIP_TRY(hipMemsetAsync(d.gPhase, 0, sizeof(T) * V, stream));
         const int nseg = (steps + K - 1) / K;
         const int nb = (std::max(V, L) + 255) / 256;
+        while ((int)segEv.size() < 3 * nseg) {
+            hipEvent_t e;
+            HIP_TRY(hipEventCreate(&e));
+            segEv.push_back(e);
+        }
         for (int seg = nseg - 1; seg >= 0; seg--) {
             const int s0 = seg * K, len = std::min(K, steps - s0);
+            HIP_TRY(hipEventRecord(segEv[3 * seg], stream));
             if (!(tapeValid && nseg == 1)) {
                 k_diff_restore<T><<<std::max(nb, 1), 256, 0, stream>>>(d, ckV, ckL, seg);
                 for (int c = 0; c < len; c += chunk) {
@@ -380,8 +386,10 @@ template <typename T> struct DiffBatch : DiffBase {
                     k_diff_forward<T><<<R, 256, 0, stream>>>(d, d_rp, s0 + c, k, K, nullptr, nullptr, tapeV, tapeL, s0);
                 }
             }
+            HIP_TRY(hipEventRecord(segEv[3 * seg + 1], stream));
             const int gatherFirst = seg != nseg - 1;
             k_diff_reverse<T><<<R, 256, 0, stream>>>(d, d_rp, s0, len, gatherFirst, tapeV, tapeL);
+            HIP_TRY(hipEventRecord(segEv[3 * seg + 2], stream));
         }
         k_diff_finish<T><<<(V + 255) / 256, 256, 0, stream>>>(d, d_gE);
         HIP_TRY(hipEventRecord(ev1, stream));
@@ -391,6 +399,14 @@ template <typename T> struct DiffBatch : DiffBase {
         HIP_TRY(hipStreamSynchronize(stream));
         HIP_TRY(hipGetLastError());
         HIP_TRY(hipEventElapsedTime(&bwdMs, ev0, ev1));
+        recomputeMs = reverseMs = 0;
+        for (int seg = 0; seg < nseg; seg++) {
+            float a = 0, b = 0;
+            HIP_TRY(hipEventElapsedTime(&a, segEv[3 * seg], segEv[3 * seg + 1]));
+            HIP_TRY(hipEventElapsedTime(&b, segEv[3 * seg + 1], segEv[3 * seg + 2]));
+            recomputeMs += a;
+            reverseMs += b;
+        }
         for (int v = 0; v < V; v++) {
             if (gphase) gphase[v] = (double)gp[v];
             if (gE) gE[v] = (double)ge[v];

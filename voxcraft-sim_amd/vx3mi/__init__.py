@@ -427,10 +427,10 @@ class Diff:
         return gp, ge
 
     def info(self) -> dict:
-        o = np.zeros(8)
+        o = np.zeros(10)
         _check(lib().vx3mi_diff_info(self._h, _ptr(o, C.c_double)))
         return dict(checkpoint_every=int(o[0]), checkpoint_bytes=o[1], tape_bytes=o[2], dt=o[3], forward_ms=o[4],
-                    backward_ms=o[5], steps=int(o[6]), precision=int(o[7]))
+                    backward_ms=o[5], steps=int(o[6]), precision=int(o[7]), recompute_ms=o[8], reverse_ms=o[9])
 
 
 def run(batch: "Batch", models: Sequence[Model], max_steps: int = 0, mode: int = MODE_AUTO, history: bool = True):
