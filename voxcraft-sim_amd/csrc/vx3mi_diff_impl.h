@@ -38,8 +38,8 @@ template <typename T> __global__ __launch_bounds__(256) void k_diff_init(DD<T> d
 
 // nsteps steps from step n0 with one workgroup per robot; a checkpoint every
 // ckEvery steps and/or a tape of the steps [tapeStart, tapeStart + nsteps)
-template <typename T>
-__global__ __launch_bounds__(256) void k_diff_forward(DD<T> d, const RobotParams *rps, int n0, int nsteps, int ckEvery,
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_diff_forward(DD<T> d, const RobotParams *rps, int n0, int nsteps, int ckEvery,
                                                       T4<T> *ckV, T4<T> *ckL, T4<T> *tapeV, T4<T> *tapeL, int tapeStart) {
     const RobotParams rp = rps[blockIdx.x];
     const int vb = rp.voff, ve = rp.voff + rp.vcount, lb = rp.loff, le = rp.loff + rp.lcount;
@@ -72,8 +72,8 @@ __global__ __launch_bounds__(256) void k_diff_restore(DD<T> d, const T4<T> *ckV,
 }
 
 // reverse sweep of the steps [segStart, segStart + segLen) from their tape
-template <typename T>
-__global__ __launch_bounds__(256) void k_diff_reverse(DD<T> d, const RobotParams *rps, int segStart, int segLen,
+template <typename T, int NT>
+__global__ __launch_bounds__(NT) void k_diff_reverse(DD<T> d, const RobotParams *rps, int segStart, int segLen,
                                                       int gatherFirst, const T4<T> *tapeV, const T4<T> *tapeL) {
     const RobotParams rp = rps[blockIdx.x];
     const int vb = rp.voff, ve = rp.voff + rp.vcount, lb = rp.loff, le = rp.loff + rp.lcount;
@@ -98,6 +98,13 @@ template <typename T> __global__ __launch_bounds__(256) void k_diff_finish(DD<T>
 }
 
 // ------------------------------------------------------------------- host
+// threads per robot workgroup: as many as the registers of each kernel allow
+// at two waves per SIMD (forward FP32 ~85 VGPRs, FP64 ~230; reverse FP32 ~170,
+// FP64 ~370 with accumulation registers)
+template <typename T> struct DiffThreads;
+template <> struct DiffThreads<float> { static constexpr int FWD = 1024, REV = 512; };
+template <> struct DiffThreads<double> { static constexpr int FWD = 512, REV = 256; };
+
 template <typename T> struct DiffBatch : DiffBase {
     DD<T> d{};
     RobotParams *d_rp = nullptr;
@@ -303,8 +310,8 @@ template <typename T> struct DiffBatch : DiffBase {
         const bool oneSeg = K >= nsteps;
         for (int s0 = 0; s0 < nsteps; s0 += chunk) {
             const int k = std::min(chunk, nsteps - s0);
-            k_diff_forward<T><<<R, 256, 0, stream>>>(d, d_rp, s0, k, K, ckV, ckL, oneSeg ? tapeV : nullptr,
-                                                    oneSeg ? tapeL : nullptr, 0);
+            k_diff_forward<T, DiffThreads<T>::FWD><<<R, DiffThreads<T>::FWD, 0, stream>>>(
+                d, d_rp, s0, k, K, ckV, ckL, oneSeg ? tapeV : nullptr, oneSeg ? tapeL : nullptr, 0);
         }
         HIP_TRY(hipEventRecord(ev1, stream));
         T4<T> *src[4] = {d.su, d.sq, d.sp, d.sh};
@@ -383,12 +390,14 @@ template <typename T> struct DiffBatch : DiffBase {
                 k_diff_restore<T><<<std::max(nb, 1), 256, 0, stream>>>(d, ckV, ckL, seg);
                 for (int c = 0; c < len; c += chunk) {
                     const int k = std::min(chunk, len - c);
-                    k_diff_forward<T><<<R, 256, 0, stream>>>(d, d_rp, s0 + c, k, K, nullptr, nullptr, tapeV, tapeL, s0);
+                    k_diff_forward<T, DiffThreads<T>::FWD><<<R, DiffThreads<T>::FWD, 0, stream>>>(
+                        d, d_rp, s0 + c, k, K, nullptr, nullptr, tapeV, tapeL, s0);
                 }
             }
             HIP_TRY(hipEventRecord(segEv[3 * seg + 1], stream));
             const int gatherFirst = seg != nseg - 1;
-            k_diff_reverse<T><<<R, 256, 0, stream>>>(d, d_rp, s0, len, gatherFirst, tapeV, tapeL);
+            k_diff_reverse<T, DiffThreads<T>::REV><<<R, DiffThreads<T>::REV, 0, stream>>>(d, d_rp, s0, len, gatherFirst,
+                                                                                          tapeV, tapeL);
             HIP_TRY(hipEventRecord(segEv[3 * seg + 2], stream));
         }
         k_diff_finish<T><<<(V + 255) / 256, 256, 0, stream>>>(d, d_gE);
