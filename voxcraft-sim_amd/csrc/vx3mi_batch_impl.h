@@ -381,6 +381,7 @@ template <typename T> struct Batch : BatchBase {
     bool tileOK = false;          // some tile plan exists
     // launch configuration used by VX3MI_MODE_AUTO (set by the autotuner)
     int autoMode = 0, tileSel = 0, persistThreads = 256;
+    int nLinkMats = 0;  // entries of the device link material table
     RobotParams *d_rp = nullptr;
     VoxelMat *d_vmats = nullptr;
     LinkMat *d_lmats = nullptr;
@@ -457,6 +458,7 @@ template <typename T> struct Batch : BatchBase {
         }
         V = (int)lat.size();
         L = (int)lends.size();
+        nLinkMats = (int)lmats.size();
         hlat = lat;
         hvmat = vmat;
 
@@ -638,8 +640,17 @@ template <typename T> struct Batch : BatchBase {
     }
 
     void launch_tiled(int k, int par) {
-        if (tileSel == 0) k_tile<T, 0><<<plans[0].ntiles, TileCfg<T, 0>::NT, 0, stream>>>(d, plans[0].d_tiles, plans[0].d_tileRobots, k, par);
-        else k_tile<T, 1><<<plans[1].ntiles, TileCfg<T, 1>::NT, 0, stream>>>(d, plans[1].d_tiles, plans[1].d_tileRobots, k, par);
+        const int nlm = nLinkMats, nvm = (int)hvmats.size();
+        const int cap = tileSel ? TileMatCache<1>::N : TileMatCache<0>::N;
+        const bool cache = nlm <= cap && nvm <= cap;
+        const TilePlan &P = plans[tileSel];
+        if (tileSel == 0) {
+            if (cache) k_tile<T, 0, 1><<<P.ntiles, TileCfg<T, 0>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
+            else k_tile<T, 0, 0><<<P.ntiles, TileCfg<T, 0>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
+        } else {
+            if (cache) k_tile<T, 1, 1><<<P.ntiles, TileCfg<T, 1>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
+            else k_tile<T, 1, 0><<<P.ntiles, TileCfg<T, 1>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
+        }
     }
     void launch_persistent(int k, int par) {
         if (persistThreads == 128) k_persistent<T, 128><<<R, 128, 0, stream>>>(d, k, par);

@@ -44,7 +44,13 @@ template <int RMAX> struct TileRobots {
     float dt[RMAX];
     int diverged[RMAX];
     int steps[RMAX];
+    double tempAmp[RMAX], tempPeriod[RMAX];  // actuation of each robot
+    int tempFlags[RMAX];                     // varyTemp | enableExpansion << 1
 };
+
+// material tables of at most this many entries (large / small tile) are
+// copied into LDS at launch; the small tile keeps within half the LDS
+template <int SMALL> struct TileMatCache { static constexpr int N = SMALL ? 16 : 24; };
 
 // largest j with pre[j] <= x (pre ascending, pre[0] = 0, pre[n] = total)
 __device__ __forceinline__ int tile_slot(const int *pre, int n, int x) {
@@ -57,15 +63,19 @@ __device__ __forceinline__ int tile_slot(const int *pre, int n, int x) {
     return lo;
 }
 
-template <typename T, int SMALL>
+// LDSMAT: the batch has at most kTileMatCache voxel and link materials, which
+// are read from LDS in the step loop instead of global memory
+template <typename T, int SMALL, int LDSMAT>
 __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, const int4 *tiles, const int *tileRobots,
-                                                                int nsteps, int parity0) {
+                                                                int nsteps, int parity0, int nLinkMats, int nVoxMats) {
     using C = TileCfg<T, SMALL>;
     constexpr int NT = C::NT, KL = C::KL, KV = C::KV;
     constexpr int LMAX = C::LMAX, VMAX = C::VMAX, RMAX = C::RMAX;
     __shared__ T sPose[8][VMAX];    // position field xyz, qw, qx, qy, qz, temperature
     __shared__ T sForce[12][LMAX];  // forceNeg, momentNeg, forcePos, momentPos
     __shared__ TileRobots<RMAX> tr;
+    __shared__ LinkMat sLM[LDSMAT ? TileMatCache<SMALL>::N : 1];
+    __shared__ VoxelMat sVM[LDSMAT ? TileMatCache<SMALL>::N : 1];
 
     const int tid = threadIdx.x;
     const int4 info = tiles[blockIdx.x];  // robot begin, robot count, voxels, links
@@ -87,6 +97,13 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
         tr.dt[tid] = t.dt;
         tr.diverged[tid] = t.diverged;
         tr.steps[tid] = t.steps;
+        tr.tempAmp[tid] = p.tempAmp;
+        tr.tempPeriod[tid] = p.tempPeriod;
+        tr.tempFlags[tid] = (p.varyTemp ? 1 : 0) | (p.enableExpansion ? 2 : 0);
+    }
+    if (LDSMAT) {
+        for (int i = tid; i < nLinkMats; i += NT) sLM[i] = a.lmats[i];
+        for (int i = tid; i < nVoxMats; i += NT) sVM[i] = a.vmats[i];
     }
     __syncthreads();
     if (tid == 0) {
@@ -169,7 +186,7 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
             if (lGlob[k] < 0) continue;
             const int j = lMeta[k] >> 2;
             if (tr.diverged[j]) continue;
-            const LinkMat &lm = a.lmats[lMat[k]];
+            const LinkMat &lm = LDSMAT ? sLM[lMat[k]] : a.lmats[lMat[k]];
             if (lm.fixedBoth) continue;
             const int n = lEnds[k] & 0xffff, p = lEnds[k] >> 16;
             const V3<T> du = {sPose[0][p] - sPose[0][n], sPose[1][p] - sPose[1][n], sPose[2][p] - sPose[2][n]};
@@ -200,7 +217,7 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
                 tr.prevDt[parity ^ 1][j] = dt;
                 tr.steps[j] += 1;
             }
-            const VoxelMat &vm = a.vmats[vMat[k]];
+            const VoxelMat &vm = LDSMAT ? sVM[vMat[k]] : a.vmats[vMat[k]];
             if (vm.fixed) continue;
             V3<T> fs = {0, 0, 0}, ms = {0, 0, 0};
 #pragma unroll
@@ -213,9 +230,9 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
             }
             float temp = (float)sPose[7][lv];
             voxel_step<T>(vm, dt, vIz[k], temp, fs, ms, V3<T>{0, 0, 0}, vs[k]);
-            const RobotParams &rp = a.rp[tr.rid[j]];
-            if (rp.varyTemp && !(vm.thermalOnAfter > tnext))
-                temp = actuation_temperature<T>(rp.tempAmp, rp.tempPeriod, tnext, (double)vPhase[k], rp.enableExpansion);
+            const int tf = tr.tempFlags[j];
+            if ((tf & 1) && !(vm.thermalOnAfter > tnext))
+                temp = actuation_temperature<T>(tr.tempAmp[j], tr.tempPeriod[j], tnext, (double)vPhase[k], tf >> 1);
             sPose[0][lv] = vs[k].u.x; sPose[1][lv] = vs[k].u.y; sPose[2][lv] = vs[k].u.z; sPose[3][lv] = vs[k].q.w;
             sPose[4][lv] = vs[k].q.x; sPose[5][lv] = vs[k].q.y; sPose[6][lv] = vs[k].q.z; sPose[7][lv] = (T)temp;
         }
