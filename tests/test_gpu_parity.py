@@ -238,3 +238,40 @@ def test_tiled_full_batch_matches_flat():
     for k in ("pos", "quat", "linmom", "angmom"):
         assert np.array_equal(out[0][0][k], out[1][0][k]), k
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def _fixed_bottom():
+    p = configs.config0()
+    cells = np.ones(64, dtype=np.int8)
+    cells[:16] = 2  # z = 0 layer
+    return dict(p, cells=cells, materials=[configs.SOFT, dict(configs.SOFT, fixed=1)])
+
+
+FEATURES = {
+    # Poisson's ratio active (VolumeEffectsEnabled): eHat stress with the frozen transverse strain sum
+    "volume_effects": lambda: dict(configs.config0(), volume_effects=1),
+    # a fixed bottom layer: fixed voxels do not move, links between two fixed voxels are skipped
+    "fixed_voxels": _fixed_bottom,
+    # linear-with-failure material: links fail once their strain exceeds Fail_Stress / E
+    # (random phases so that thermal strains differ between neighbours)
+    "failure": lambda: dict(configs.config0(), phase=np.random.default_rng(4).uniform(0, 1, 64),
+                            materials=[dict(configs.SOFT, fail_stress=2e4)]),
+}
+
+
+@pytest.mark.parametrize("name", list(FEATURES))
+def test_material_features_fp64_bit_exact(name):
+    """Material features outside the five workloads, FP64 against the oracle bit for bit."""
+    log(f"feature {name}")
+    prob = FEATURES[name]()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(prob)], vx3mi.FP64)
+    o = O.OracleModel(prob)
+    for _ in range(3):
+        b.step(200)
+        o.step(200)
+        s, r = b.state(), o.state()
+        assert np.array_equal(s["pos"], r["pos"]) and np.array_equal(s["quat"], r["quat"])
+    if name == "failure":  # some links did fail (zero force after exceeding the failure strain)
+        ref = O.OracleModel(dict(prob, materials=[configs.SOFT]))
+        ref.step(600)
+        assert np.abs(ref.state()["pos"] - o.state()["pos"]).max() > 1e-6
