@@ -134,6 +134,23 @@ int vx3mi_model_topology(const vx3mi_model *m, int *lattice3, int *link_neg, int
 int vx3mi_model_voxel_constants(const vx3mi_model *m, int voxel, float *out12);
 int vx3mi_model_link_constants(const vx3mi_model *m, int link, float *out11);
 
+/* External of a voxel: replaces CVX_Voxel::external() with
+ * CVX_External::setDisplacement / setForce / setMoment (src/old/VX_Voxel.h:75,
+ * src/old/VX_External.h:59-80), which the reference copies to the device as
+ * VX3_External (src/VX3/VX3_External.h:27-31, VX3_Voxel.cu:39-45).
+ * dof: bits 1,2,4 fix x,y,z translation and 8,16,32 fix x,y,z rotation;
+ * dof < 0 removes the external (any dof >= 0, even 0, makes one: the force and
+ * moment still apply).  translation3 (m) / rotation3 (rotation vector, rad):
+ * where the fixed degrees of freedom are held, relative to the lattice
+ * position; force3 (N) / moment3 (N m) are stored as float, as the reference
+ * does.  NULL arrays mean zeros.  Boundary-condition regions of a VXA
+ * (Environment/Boundary_Conditions) are applied through the same call. */
+int vx3mi_model_set_external(vx3mi_model *m, int voxel, int dof, const double *translation3,
+                             const double *rotation3, const double *force3, const double *moment3);
+/* the voxel's external (*dof = -1 and zeros when it has none) */
+int vx3mi_model_get_external(const vx3mi_model *m, int voxel, int *dof, double *translation3,
+                             double *rotation3, double *force3, double *moment3);
+
 /* ---- batches (device) -----------------------------------------------------
  * A batch packs n models (CSR: voxel and link offsets per robot) into one
  * set of device arrays.  precision: VX3MI_FP32 or VX3MI_FP64. */

@@ -94,6 +94,7 @@ def lib():
         L.vxo_get_topology.argtypes = [vp, ip, ip, ip, ip, ip]
         L.vxo_center_of_mass.argtypes = [vp, dp]
         L.vxo_get_contact.argtypes = [vp, dp]
+        L.vxo_set_external.argtypes = [vp, C.c_int, C.c_int, dp, dp, fp, fp]
         L.vxo_get_voxel_material.argtypes = [vp, C.c_int, fp]
         L.vxo_get_link_material.argtypes = [vp, C.c_int, fp]
         L.vxo_diff_new.restype = vp
@@ -143,6 +144,20 @@ class OracleModel:
         self.nv = self._L.vxo_num_voxels(self._h)
         self.nl = self._L.vxo_num_links(self._h)
         self.ns = self._L.vxo_num_surface(self._h)
+        for e in p.get("externals") or []:
+            self.set_external(**e)
+
+    def set_external(self, voxel: int, dof: int = 0, translation=(0, 0, 0), rotation=(0, 0, 0),
+                     force=(0, 0, 0), moment=(0, 0, 0)) -> None:
+        """External of a voxel: fixed degrees of freedom (bits 1,2,4 = x,y,z
+        translation, 8,16,32 = x,y,z rotation), prescribed translation / rotation
+        vector, force and moment."""
+        t = np.ascontiguousarray(translation, dtype=np.float64)
+        r = np.ascontiguousarray(rotation, dtype=np.float64)
+        f = np.ascontiguousarray(force, dtype=np.float32)
+        m = np.ascontiguousarray(moment, dtype=np.float32)
+        self._L.vxo_set_external(self._h, int(voxel), int(dof), _ptr(t, C.c_double), _ptr(r, C.c_double),
+                                 _ptr(f, C.c_float), _ptr(m, C.c_float))
 
     def __del__(self):
         if getattr(self, "_h", None):

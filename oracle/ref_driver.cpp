@@ -15,11 +15,16 @@
 //   nx*ny*nz cell values (0 = empty)
 //   nv phase offsets (cycles)
 //   dt nsteps record_every
+//   [next, then per external: voxel dof tx ty tz rx ry rz fx fy fz mx my mz]
+//     applied with CVX_Voxel::external() and CVX_External::setDisplacement /
+//     setForce / setMoment (src/old/VX_External.cpp), as CVX_Sim::Import does
+//     for boundary-condition regions (src/VXA/VX_Sim.cpp:109-143)
 // stdout (binary): per record, nv * (pos3, quat4, vel3, angvel3) doubles,
 //   then a trailing timing line on stderr "seconds <s> voxel_steps <n>".
 #include "Voxelyze.h"
 #include "VX_Voxel.h"
 #include "VX_Material.h"
+#include "VX_External.h"
 
 #include <chrono>
 #include <cmath>
@@ -66,6 +71,21 @@ int main() {
     float dt;
     int nsteps, every;
     std::cin >> dt >> nsteps >> every;
+    int next = 0;
+    if (!(std::cin >> next)) next = 0;
+    for (int k = 0; k < next; k++) {
+        int v, dof;
+        double t[3], r[3];
+        float f[3], mo[3];
+        std::cin >> v >> dof >> t[0] >> t[1] >> t[2] >> r[0] >> r[1] >> r[2] >> f[0] >> f[1] >> f[2] >> mo[0] >> mo[1] >>
+            mo[2];
+        CVX_External *e = Vx.voxel(v)->external();
+        const dofComponent comp[6] = {X_TRANSLATE, Y_TRANSLATE, Z_TRANSLATE, X_ROTATE, Y_ROTATE, Z_ROTATE};
+        for (int b = 0; b < 6; b++)
+            if (dof & (1 << b)) e->setDisplacement(comp[b], b < 3 ? t[b] : r[b - 3]);
+        e->setForce(f[0], f[1], f[2]);
+        e->setMoment(mo[0], mo[1], mo[2]);
+    }
     for (int i = 0; i < nv; i++) Vx.voxel(i)->setTemperature(t0);
 
     std::vector<double> rec((size_t)nv * 13);

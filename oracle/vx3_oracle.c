@@ -69,6 +69,13 @@ struct vxo_model {
     float *prevDt;
     int *vflags;
     double *phase;
+    /* externals (VX3_External.h:27-31): fixed degrees of freedom (bits X, Y, Z
+     * translation, X, Y, Z rotation; -1 = the voxel has no external), prescribed
+     * translation and rotation (and its quaternion), force and moment */
+    int *ext_dof;
+    v3 *ext_t, *ext_r;
+    q4 *ext_q;
+    v3 *ext_f, *ext_m;
     int *surface;          /* ns voxel indices, voxel order */
     int *is_surface;
     /* links */
@@ -396,6 +403,13 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->vflags = (int *)xcalloc(nv, sizeof(int));
     m->phase = (double *)xcalloc(nv, sizeof(double));
     m->is_surface = (int *)xcalloc(nv, sizeof(int));
+    m->ext_dof = (int *)xcalloc(nv, sizeof(int));
+    m->ext_t = (v3 *)xcalloc(nv, sizeof(v3));
+    m->ext_r = (v3 *)xcalloc(nv, sizeof(v3));
+    m->ext_q = (q4 *)xcalloc(nv, sizeof(q4));
+    m->ext_f = (v3 *)xcalloc(nv, sizeof(v3));
+    m->ext_m = (v3 *)xcalloc(nv, sizeof(v3));
+    for (int i = 0; i < nv; i++) m->ext_dof[i] = -1;
 
     int *cellvox = (int *)xcalloc(ncell, sizeof(int));
     int v = 0;
@@ -500,9 +514,27 @@ void vxo_free(vxo_model *m) {
     void *ptrs[] = {m->ix, m->iy, m->iz, m->vmat_of, m->vlinks, m->pos, m->linMom, m->angMom, m->contact,
                     m->orient, m->temp, m->prevDt, m->vflags, m->phase, m->surface, m->is_surface, m->lneg,
                     m->lpos, m->laxis, m->lm, m->pos2, m->a1v, m->a2v, m->strain, m->maxStrain, m->tArea,
-                    m->tStrainSum, m->restLen, m->lflags, m->fNeg, m->fPos, m->mNeg, m->mPos, m->vm};
+                    m->tStrainSum, m->restLen, m->lflags, m->fNeg, m->fPos, m->mNeg, m->mPos, m->vm,
+                    m->ext_dof, m->ext_t, m->ext_r, m->ext_q, m->ext_f, m->ext_m};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(m);
+}
+
+/* CVX_Voxel::external() + CVX_External::setDisplacement / setForce / setMoment
+ * (src/old/VX_External.cpp): the external of a voxel; the rotation quaternion
+ * is cached as CVX_External::rotationChanged does (identity for a zero
+ * rotation vector, else FromRotationVector). */
+void vxo_set_external(vxo_model *m, int v, int dof, const double *t3, const double *r3, const float *f3,
+                      const float *m3) {
+    if (v < 0 || v >= m->nv) return;
+    m->ext_dof[v] = dof < 0 ? -1 : (dof & 63);
+    m->ext_t[v] = t3 ? v3_(t3[0], t3[1], t3[2]) : v3_(0, 0, 0);
+    m->ext_r[v] = r3 ? v3_(r3[0], r3[1], r3[2]) : v3_(0, 0, 0);
+    m->ext_f[v] = f3 ? v3_(f3[0], f3[1], f3[2]) : v3_(0, 0, 0);
+    m->ext_m[v] = m3 ? v3_(m3[0], m3[1], m3[2]) : v3_(0, 0, 0);
+    v3 r = m->ext_r[v];
+    if (r.x != 0 || r.y != 0 || r.z != 0) m->ext_q[v] = q_from_rotvec(r);
+    else { q4 id = {1, 0, 0, 0}; m->ext_q[v] = id; }
 }
 
 int vxo_num_voxels(const vxo_model *m) { return m->nv; }
@@ -750,12 +782,24 @@ static void update_collisions(vxo_model *m) {
 
 /* ----------------------------------------------------------------- voxels */
 /* VX3_Voxel::timeStep VX3_Voxel.cu:162-277 with force() :350-380, moment()
- * :382-397, floorForce :399-426 (no external fixes, constant-zero force field) */
+ * :382-397, floorForce :399-426 (constant-zero force field); externals as at
+ * :167-172 (all six fixed), :363-364 / :393-394 (force, moment) and :222-256
+ * (fixed degrees of freedom) */
 static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
     const vmat *mt = &m->vm[m->vmat_of[v]];
     double dt = dtf;
     m->prevDt[v] = dtf;
     if (dt == 0.0f) return;
+    const int ed = m->ext_dof[v];
+    if (ed == 63) {
+        /* pos = originalPosition() + translation; orient = rotationQuat(); haltMotion() */
+        double s = mt->nomSize;
+        v3 t = m->ext_t[v];
+        m->pos[v] = v3_(m->ix[v] * s + t.x, m->iy[v] * s + t.y, m->iz[v] * s + t.z);
+        m->orient[v] = m->ext_q[v];
+        m->linMom[v] = m->angMom[v] = v3_(0, 0, 0);
+        return;
+    }
     int flags = m->vflags[v];
     q4 orient = m->orient[v];
     v3 pos = m->pos[v], linMom = m->linMom[v], angMom = m->angMom[v];
@@ -768,6 +812,7 @@ static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
         tf = vadd(tf, (d & 1) ? m->fPos[l] : m->fNeg[l]);
     }
     tf = qrot(orient, tf);
+    if (ed >= 0) tf = vadd(tf, m->ext_f[v]);
     v3 vel = vscale(linMom, mt->massInv);
     float gdt = mt->zetaG * mt->c2xSqMxExS;
     tf = vsub(tf, vscale(vel, gdt));
@@ -827,10 +872,31 @@ static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
         tm = vadd(tm, (d & 1) ? m->mPos[l] : m->mNeg[l]);
     }
     tm = qrot(orient, tm);
+    if (ed >= 0) tm = vadd(tm, m->ext_m[v]);
     float gdr = mt->zetaG * mt->c2xSqIxExSxSxS;
     tm = vsub(tm, vscale(vscale(angMom, mt->inertiaInv), gdr));
     angMom = vadd(angMom, vscale(tm, dt));
     orient = qmul(q_from_rotvec(vscale(angMom, dtIinv)), orient);
+    if (ed > 0) {
+        double s = mt->nomSize;
+        v3 t = m->ext_t[v];
+        if (ed & 1) { pos.x = m->ix[v] * s + t.x; linMom.x = 0; }
+        if (ed & 2) { pos.y = m->iy[v] * s + t.y; linMom.y = 0; }
+        if (ed & 4) { pos.z = m->iz[v] * s + t.z; linMom.z = 0; }
+        if (ed & 56) {  /* any rotation fixed */
+            if ((ed & 56) == 56) {
+                orient = m->ext_q[v];
+                angMom = v3_(0, 0, 0);
+            } else {
+                v3 rv = q_to_rotvec(orient);
+                if (ed & 8) { rv.x = 0; angMom.x = 0; }
+                if (ed & 16) { rv.y = 0; angMom.y = 0; }
+                if (ed & 32) { rv.z = 0; angMom.z = 0; }
+                orient = q_from_rotvec(rv);
+            }
+        }
+        m->pos[v] = pos;
+    }
 
     /* VX3-only: no rotation while held by static floor friction (VX3_Voxel.cu:259-264) */
     if (!cpu_semantics && (flags & VF_FLOOR_ENABLED) && floor_penetration(m, v) >= 0) {

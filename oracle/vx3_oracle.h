@@ -88,6 +88,13 @@ int vxo_step(vxo_model *m, float dt, int nsteps, int flags);
  * momentNeg, momentPos), each in its voxel's local frame. */
 void vxo_update_links(vxo_model *m, double *out);
 
+/* external of a voxel (CVX_Voxel::external(), VX3_External.h:27-31): dof bits
+ * 1,2,4 = X,Y,Z translation fixed, 8,16,32 = X,Y,Z rotation fixed (dof < 0
+ * removes the external); translation (m) and rotation vector (rad) of the
+ * fixed degrees of freedom; force (N) and moment (N m).  NULL = zeros. */
+void vxo_set_external(vxo_model *m, int voxel, int dof, const double *t3, const double *r3, const float *f3,
+                      const float *m3);
+
 /* state access (voxel order) */
 void vxo_get_voxels(const vxo_model *m, double *pos3, double *quat4, double *linmom3, double *angmom3);
 void vxo_set_voxels(vxo_model *m, const double *pos3, const double *quat4, const double *linmom3, const double *angmom3);

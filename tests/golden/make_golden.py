@@ -6,7 +6,7 @@ small robots and stores positions and orientations every `every` steps as
 compressed .npz fixtures next to this script.  The fixtures are data (inputs and
 outputs); the reference itself is not needed to read them.
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [name ...]
 """
 import os
 import subprocess
@@ -39,6 +39,15 @@ def driver_input(prob, dt, nsteps, every):
     lines.append(" ".join(str(int(c)) for c in cells))
     lines.append(" ".join(repr(float(p)) for p in phase))
     lines.append(f"{float(np.float32(dt)):.9g} {nsteps} {every}")
+    ext = prob.get("externals") or []
+    lines.append(str(len(ext)))
+    for e in ext:
+        vals = [e["voxel"], e.get("dof", 0)]
+        vals += [repr(float(x)) for x in e.get("translation", (0, 0, 0))]
+        vals += [repr(float(x)) for x in e.get("rotation", (0, 0, 0))]
+        vals += [repr(float(np.float32(x))) for x in e.get("force", (0, 0, 0))]
+        vals += [repr(float(np.float32(x))) for x in e.get("moment", (0, 0, 0))]
+        lines.append(" ".join(str(v) for v in vals))
     return "\n".join(lines).encode()
 
 
@@ -50,14 +59,43 @@ def run_reference(prob, nsteps, every):
     return dt, rec
 
 
+def beam_with_externals():
+    """A 6x2x3 cantilever: the x=0 face fully fixed (one voxel displaced, one
+    rotated), loads and a moment at the tip, a partial rotation fix and a
+    prescribed x translation in the span; gravity on, floor off."""
+    p = dict(configs.config0(), dims=(6, 2, 3), cells=np.ones(36, dtype=np.int8), phase=None, floor_enabled=0,
+             vary_temp=0, temp_amplitude=0.0, enable_collision=0, bond_damp_z=0.1, slow_damp_z=0.01)
+    vid = lambda x, y, z: x + 6 * (y + 2 * z)  # noqa: E731  voxel order of a full lattice
+    ext = []
+    for y in range(2):
+        for z in range(3):
+            e = dict(voxel=vid(0, y, z), dof=63)
+            if (y, z) == (1, 2):
+                e["translation"] = (0.0, 0.0, 5e-4)
+            if (y, z) == (0, 1):
+                e["rotation"] = (0.0, 0.02, 0.0)
+            ext.append(e)
+    for y in range(2):
+        for z in range(3):
+            ext.append(dict(voxel=vid(5, y, z), force=(0.0, 0.002, -0.01)))
+    ext[-1]["moment"] = (1e-4, 0.0, 2e-5)
+    ext.append(dict(voxel=vid(3, 0, 0), dof=8 | 32))
+    ext.append(dict(voxel=vid(2, 1, 1), dof=1, translation=(2e-4, 0.0, 0.0), force=(0.0, 0.0, 0.003)))
+    p["externals"] = ext
+    return p
+
+
 def golden_problems():
     rng = np.random.default_rng(7)
     robot = configs.random_robot(rng, (5, 4, 3), 0.8, enable_collision=0)
-    return {"config0": configs.config0(), "robot5x4x3": robot}
+    return {"config0": configs.config0(), "robot5x4x3": robot, "beam_bc": beam_with_externals()}
 
 
 def main():
+    only = sys.argv[1:]
     for name, prob in golden_problems().items():
+        if only and name not in only:
+            continue
         dt, rec = run_reference(prob, 600, 50)
         np.savez_compressed(os.path.join(HERE, f"ref_cpu_{name}.npz"), dt=dt, every=50,
                             pos=rec[:, :, 0:3], quat=rec[:, :, 3:7])

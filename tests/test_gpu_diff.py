@@ -125,6 +125,24 @@ def test_batched_equals_single():
         assert np.array_equal(go[0], gb[0][off[r]:off[r + 1]]) and np.array_equal(go[1], gb[1][off[r]:off[r + 1]])
 
 
+def test_batched_checkpointed_equals_whole_tape():
+    """Two robots, K = 13 over 120 steps (recompute on a second stream beside
+    the reverse sweep, two tapes in turn) against the whole run's tape."""
+    probs = [configs.config3(seed=3, dims=(4, 4, 3)), configs.config3(seed=4, dims=(5, 3, 2))]
+    models = [vx3mi.Model.from_problem(p) for p in probs]
+    phase = np.concatenate([np.asarray(p["phase"]) for p in probs])
+    E = np.concatenate([np.asarray(p["voxel_E"], np.float64) for p in probs])
+    out = []
+    for K in (0, 13):
+        d = vx3mi.Diff(models, vx3mi.FP32, dt=float(FIX["dt"]), checkpoint_every=K)
+        d.set_params(phase, E)
+        d.forward(120)
+        g = np.random.default_rng(1).normal(size=(d.nv, 3))
+        out.append(d.backward(grad_pos=g))
+        log(f"  K={K} info {d.info()}")
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_torch_function_gradients():
     torch = pytest.importorskip("torch")
     from vx3mi.torch_diff import simulate

@@ -38,7 +38,7 @@ def quiet(dims, cells=None, **kw):
     return p
 
 
-@pytest.mark.parametrize("name", ["config0", "robot5x4x3"])
+@pytest.mark.parametrize("name", ["config0", "robot5x4x3", "beam_bc"])
 def test_golden_reference_cpu(name):
     """Oracle (CVoxelyze integrator semantics) vs the reference CPU Voxelyze built
     from src/old: positions and orientations every 50 steps for 600 steps."""

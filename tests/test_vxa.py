@@ -231,8 +231,12 @@ BAD = {
     "zlib": DEMO_LIKE.replace('Compression="ASCII_READABLE"', 'Compression="ZLIB"'),
     "few_layers": DEMO_LIKE.replace("<Z_Voxels>2</Z_Voxels>", "<Z_Voxels>3</Z_Voxels>"),
     "attach": DEMO_LIKE.replace("<Simulator>", "<Simulator><AttachDetach><EnableAttach>true</EnableAttach></AttachDetach>"),
-    "bcs": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs></Boundary_Conditions>"
-                                      "</Environment><VXC "),
+    "bc_count": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs></Boundary_Conditions>"
+                                          "</Environment><VXC "),
+    "bc_mesh": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs><FRegion><PrimType>3"
+                                         "</PrimType></FRegion></Boundary_Conditions></Environment><VXC "),
+    "bc_no_prim": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs><FRegion><X>0</X>"
+                                            "</FRegion></Boundary_Conditions></Environment><VXC "),
     "bad_token": DEMO_LIKE.replace("<mtSUB>", "<mtFOO>").replace("</mtSUB>", "</mtFOO>"),
     "bad_var": DEMO_LIKE.replace("<mtVAR>t</mtVAR>", "<mtVAR>speed</mtVAR>"),
     "missing_operand": DEMO_LIKE.replace("<mtCONST>1</mtCONST>", ""),
@@ -258,3 +262,90 @@ def test_zero_force_field_and_collision_false_are_accepted():
 def test_missing_file_is_a_parse_error(tmp_path):
     with pytest.raises(vx3mi.VX3MIError, match="cannot read"):
         vx3mi.Model.from_vxa(str(tmp_path / "nope.vxa"))
+
+
+# ---- boundary conditions (Environment/Boundary_Conditions) --------------------
+def _externals(m):
+    return {v: m.external(v) for v in range(m.nv) if m.external(v) is not None}
+
+
+def _cube_with_regions(regions, **kw):
+    return vx3mi.Model.from_xml(W.write_vxa(dict(configs.config0(), bc_regions=regions, **kw)))
+
+
+def test_box_region_fixes_a_face_and_shares_its_force():
+    """A box over the first fifth of x holds the 16 voxels of the x = 0 face
+    (DofFixed 63) and splits ForceZ over them (CVX_Sim::Import divides by the
+    number of voxels the region touches)."""
+    m = _cube_with_regions([dict(PrimType=0, X=0.0, Y=0.0, Z=0.0, dX=0.2, dY=1.0, dZ=1.0, DofFixed=63,
+                                 ForceZ=-1.6)])
+    ext = _externals(m)
+    lat = m.topology()["lattice"]
+    assert sorted(ext) == [v for v in range(m.nv) if lat[v, 0] == 0] and len(ext) == 16
+    for e in ext.values():
+        assert e["dof"] == 63 and e["force"][2] == float(np.float32(-0.1)) and not e["force"][:2].any()
+
+
+def test_sphere_and_cylinder_regions_select_by_distance():
+    """Sphere of radius 0.1 workspace at the centre: the 8 central voxels (their
+    centres lie within radius + half a voxel diagonal); a z cylinder of radius 0.2
+    through the centre: the 2x2 central columns, 16 voxels."""
+    sph = _externals(_cube_with_regions([dict(PrimType=2, X=0.5, Y=0.5, Z=0.5, Radius=0.1, DofFixed=1)]))
+    cyl = _externals(_cube_with_regions([dict(PrimType=1, X=0.5, Y=0.5, Z=0.0, dZ=1.0, Radius=0.2, DofFixed=2)]))
+    lat = vx3mi.Model.from_problem(configs.config0()).topology()["lattice"]
+    centre = lambda v: all(c in (1, 2) for c in lat[v])  # noqa: E731
+    column = lambda v: lat[v, 0] in (1, 2) and lat[v, 1] in (1, 2)  # noqa: E731
+    assert sorted(sph) == [v for v in range(64) if centre(v)] and all(e["dof"] == 1 for e in sph.values())
+    assert sorted(cyl) == [v for v in range(64) if column(v)] and all(e["dof"] == 2 for e in cyl.values())
+
+
+def test_displacements_and_overlapping_regions():
+    """Prescribed displacement / rotation of fixed degrees of freedom, and two
+    overlapping regions: the reference adds external()->force() + F/n to the
+    force already there, so the first region's share counts twice."""
+    r1 = dict(PrimType=0, X=0.0, Y=0.0, Z=0.0, dX=0.2, dY=1.0, dZ=1.0, DofFixed=4 | 8, DisplaceZ=0.001,
+              AngDisplaceX=0.1, ForceX=0.8)
+    r2 = dict(PrimType=0, X=0.0, Y=0.0, Z=0.0, dX=1.0, dY=0.2, dZ=1.0, ForceX=0.4, TorqueY=0.2)
+    ext = _externals(_cube_with_regions([r1, r2]))
+    lat = vx3mi.Model.from_problem(configs.config0()).topology()["lattice"]
+    f1, f2 = np.float32(0.8 / 16), np.float32(0.4 / 16)
+    for v, e in ext.items():
+        in1, in2 = lat[v, 0] == 0, lat[v, 1] == 0
+        assert in1 or in2
+        if in1:
+            assert e["dof"] == 12 and e["translation"][2] == 0.001 and e["rotation"][0] == 0.1
+        else:
+            assert e["dof"] == 0 and not e["translation"].any()
+        expect = f1 if in1 else np.float32(0)
+        if in2:
+            expect = np.float32(expect + np.float32(np.float64(expect) + 0.4 / 16))
+        assert e["force"][0] == float(expect), (v, e["force"][0], expect)
+        assert e["moment"][1] == (float(np.float32(0.2 / 16)) if in2 else 0.0)
+    assert len(ext) == 16 + 16 - 4
+
+
+def test_legacy_fixed_regions_and_binary_fixed_tag():
+    text = W.write_vxa(configs.config0()).replace(
+        "<Environment>", "<Environment><Fixed_Regions><NumFixed>1</NumFixed><FRegion><PrimType>0</PrimType>"
+                         "<X>0</X><Y>0</Y><Z>0</Z><dX>1</dX><dY>1</dY><dZ>0.2</dZ><Fixed>1</Fixed></FRegion>"
+                         "</Fixed_Regions>")
+    ext = _externals(vx3mi.Model.from_xml(text))
+    lat = vx3mi.Model.from_problem(configs.config0()).topology()["lattice"]
+    assert sorted(ext) == [v for v in range(64) if lat[v, 2] == 0] and all(e["dof"] == 63 for e in ext.values())
+
+
+def test_vxa_regions_equal_explicit_externals():
+    """The same externals through a VXA region and through set_external give the
+    same model (same external per voxel)."""
+    reg = dict(PrimType=0, X=0.0, Y=0.0, Z=0.0, dX=0.2, dY=1.0, dZ=1.0, DofFixed=63, DisplaceZ=5e-4)
+    a = _cube_with_regions([reg])
+    b = vx3mi.Model.from_problem(configs.config0())
+    lat = b.topology()["lattice"]
+    for v in range(b.nv):
+        if lat[v, 0] == 0:
+            b.set_external(v, 63, translation=(0, 0, 5e-4))
+    ea, eb = _externals(a), _externals(b)
+    assert sorted(ea) == sorted(eb)
+    for v in ea:
+        for k in ("dof", "translation", "rotation", "force", "moment"):
+            assert np.array_equal(ea[v][k], eb[v][k]), (v, k)

@@ -42,6 +42,7 @@ template <typename T> struct Dev {
     T4<T> *ls0, *ls1, *ls2;
     T4<T> *lf0, *lf1, *lf2;
     const VoxelMat *vmats;
+    const VoxelExt *exts;  // voxel externals (VoxelMat::ext indexes it)
     const LinkMat *lmats;
     const RobotParams *rp;
     RobotTime *rt;
@@ -130,7 +131,7 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
         const T4<T> c = a.contact[v];
         contact = {c.x, c.y, c.z};
     }
-    voxel_step<T>(vm, dt, a.lat[v] >> 20, temp, fs, ms, contact, s);
+    voxel_step<T>(vm, a.exts, dt, a.lat[v], temp, fs, ms, contact, s);
     if (rp.varyTemp && !(vm.thermalOnAfter > tnext))
         temp = actuation_temperature<T>(rp.tempAmp, rp.tempPeriod, tnext, (double)a.phase[v], rp.enableExpansion);
     a.pq0[v] = T4<T>{s.u.x, s.u.y, s.u.z, s.q.w};
@@ -384,6 +385,7 @@ template <typename T> struct Batch : BatchBase {
     int nLinkMats = 0;  // entries of the device link material table
     RobotParams *d_rp = nullptr;
     VoxelMat *d_vmats = nullptr;
+    VoxelExt *d_exts = nullptr;
     LinkMat *d_lmats = nullptr;
     float *d_vmatE = nullptr;
     double *d_out = nullptr;
@@ -395,8 +397,25 @@ template <typename T> struct Batch : BatchBase {
         std::vector<int2> lends, lmeta;
         std::vector<LinkMat> lmats;
         std::vector<float> vmatE;
+        std::vector<VoxelExt> exts;
         std::vector<T4<T>> pq0, pq1;
         std::map<std::string, int> vmIndex, lmIndex;
+        // materials are deduplicated across robots (byte-identical constants);
+        // a voxel with an external gets an entry of its own material whose
+        // ext field points at the external
+        auto vm_index = [&](const VoxelMat &vmIn, float E, int ext) {
+            VoxelMat vm = vmIn;
+            vm.ext = ext;
+            std::string key((const char *)&vm, sizeof(VoxelMat));
+            key.append((const char *)&E, sizeof(float));
+            auto it = vmIndex.find(key);
+            if (it == vmIndex.end()) {
+                it = vmIndex.emplace(key, (int)hvmats.size()).first;
+                hvmats.push_back(vm);
+                vmatE.push_back(E);
+            }
+            return it->second;
+        };
         hrp.resize(n);
         for (int r = 0; r < n; r++) {
             const Model &m = models[r]->m;
@@ -409,19 +428,10 @@ template <typename T> struct Batch : BatchBase {
             p.enableExpansion = m.enableExpansion;
             p.collisions = m.enableCollision;
             p.voff = voff; p.vcount = m.nv(); p.loff = loff; p.lcount = m.nl();
-            // materials are deduplicated across robots (byte-identical constants)
             std::vector<int> vmRemap(m.vmats.size()), lmRemap(m.lmats.size());
-            for (size_t k = 0; k < m.vmats.size(); k++) {
-                std::string key((const char *)&m.vmats[k], sizeof(VoxelMat));
-                key.append((const char *)&m.vmatE[k], sizeof(float));
-                auto it = vmIndex.find(key);
-                if (it == vmIndex.end()) {
-                    it = vmIndex.emplace(key, (int)hvmats.size()).first;
-                    hvmats.push_back(m.vmats[k]);
-                    vmatE.push_back(m.vmatE[k]);
-                }
-                vmRemap[k] = it->second;
-            }
+            for (size_t k = 0; k < m.vmats.size(); k++) vmRemap[k] = vm_index(m.vmats[k], m.vmatE[k], -1);
+            const int extOff = (int)exts.size();
+            exts.insert(exts.end(), m.exts.begin(), m.exts.end());
             for (size_t k = 0; k < m.lmats.size(); k++) {
                 std::string key((const char *)&m.lmats[k], sizeof(LinkMat));
                 auto it = lmIndex.find(key);
@@ -433,7 +443,8 @@ template <typename T> struct Batch : BatchBase {
             }
             for (int v = 0; v < m.nv(); v++) {
                 lat.push_back(m.ix[v] | (m.iy[v] << 10) | (m.iz[v] << 20));
-                vmat.push_back(vmRemap[m.vmat[v]]);
+                const int e = m.ext_of(v);
+                vmat.push_back(e < 0 ? vmRemap[m.vmat[v]] : vm_index(m.vmats[m.vmat[v]], m.vmatE[m.vmat[v]], extOff + e));
                 vrob.push_back(r);
                 phase.push_back((T)m.phase[v]);
                 for (int k = 0; k < 6; k++) {
@@ -475,6 +486,7 @@ template <typename T> struct Batch : BatchBase {
             (rc = alloc(&vlinksd, (size_t)6 * V)) || (rc = alloc(&lendsd, L)) || (rc = alloc(&lmetad, L)) ||
             (rc = alloc(&ls0, L)) || (rc = alloc(&ls1, L)) || (rc = alloc(&ls2, L)) || (rc = alloc(&lf0, L)) ||
             (rc = alloc(&lf1, L)) || (rc = alloc(&lf2, L)) || (rc = alloc(&d_vmats, hvmats.size())) ||
+            (rc = alloc(&d_exts, exts.size())) ||
             (rc = alloc(&d_lmats, lmats.size())) || (rc = alloc(&d_vmatE, vmatE.size())) || (rc = alloc(&d_rp, R)) ||
             (rc = alloc(&d_rt, R)) || (rc = alloc(&d_out, (size_t)3 * R)))
             return rc;
@@ -507,6 +519,8 @@ template <typename T> struct Batch : BatchBase {
             HIP_TRY(hipMemcpy(d_lmats, lmats.data(), sizeof(LinkMat) * lmats.size(), hipMemcpyHostToDevice));
         }
         HIP_TRY(hipMemcpy(d_vmats, hvmats.data(), sizeof(VoxelMat) * hvmats.size(), hipMemcpyHostToDevice));
+        if (!exts.empty())
+            HIP_TRY(hipMemcpy(d_exts, exts.data(), sizeof(VoxelExt) * exts.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d_vmatE, vmatE.data(), sizeof(float) * vmatE.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d_rp, hrp.data(), sizeof(RobotParams) * R, hipMemcpyHostToDevice));
         std::vector<RobotTime> rt((size_t)R);
@@ -568,7 +582,7 @@ template <typename T> struct Batch : BatchBase {
         d.lat = latd; d.vmat = vmatd; d.vrob = vrobd; d.phase = phased; d.vlinks = vlinksd;
         d.lends = lendsd; d.lmeta = lmetad;
         d.ls0 = ls0; d.ls1 = ls1; d.ls2 = ls2; d.lf0 = lf0; d.lf1 = lf1; d.lf2 = lf2;
-        d.vmats = d_vmats; d.lmats = d_lmats; d.rp = d_rp; d.rt = d_rt;
+        d.vmats = d_vmats; d.exts = d_exts; d.lmats = d_lmats; d.rp = d_rp; d.rt = d_rt;
         return plan_tiles();
     }
 

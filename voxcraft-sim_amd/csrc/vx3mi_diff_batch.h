@@ -22,12 +22,13 @@ struct DiffBase {
     bool tapeValid = false;  // the tape holds the whole last forward (no recompute needed)
     float dt = 0;
     float fwdMs = 0, bwdMs = 0;
-    float recomputeMs = 0, reverseMs = 0;  // parts of the last backward
-    std::vector<hipEvent_t> segEv;         // event pairs around the backward's kernels
+    float recomputeMs = 0, reverseMs = 0;  // stream time of the last backward's recompute and reverse kernels
+    std::vector<hipEvent_t> segEv;         // per segment: recompute start/end, reverse start/end
     double ckBytes = 0, tapeBytes = 0;
     DiffOpts o{};
     vx3mi_diff_opts optsIn{};
-    hipStream_t stream = nullptr;
+    hipStream_t stream = nullptr;   // forward, reverse sweep
+    hipStream_t rstream = nullptr;  // recompute of the next segment, beside the reverse sweep
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void *> allocs;
     std::vector<RobotParams> hrp;
@@ -39,6 +40,7 @@ struct DiffBase {
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
+        if (rstream) (void)hipStreamDestroy(rstream);
     }
     template <typename P> int alloc(P **p, size_t n) {
         *p = nullptr;

@@ -6,7 +6,11 @@
 // phase, barrier), writing a checkpoint of the full state every K steps.
 // Backward, per segment of K steps from the last: the segment is recomputed
 // from its checkpoint into a tape (state entering each step and the link force
-// sums of each voxel), then swept in reverse.  A reverse step is two
+// sums of each voxel), then swept in reverse.  The recompute of segment s-1
+// runs on a second stream into the other of two tapes while segment s is swept
+// (the two touch disjoint arrays: forward state vs adjoint state), so the
+// backward costs about one reverse sweep plus one segment's recompute.
+// A reverse step is two
 // barrier-separated phases:
 //   voxel phase  gather the link adjoints of step n+1 (deterministic, <=6 per
 //                voxel, direction order), then the voxel adjoint of step n;
@@ -99,11 +103,14 @@ template <typename T> __global__ __launch_bounds__(256) void k_diff_finish(DD<T>
 
 // ------------------------------------------------------------------- host
 // threads per robot workgroup: as many as the registers of each kernel allow
-// at two waves per SIMD (forward FP32 ~85 VGPRs, FP64 ~230; reverse FP32 ~170,
-// FP64 ~370 with accumulation registers)
+// (forward FP32 84 VGPRs, FP64 ~230; reverse FP32 166, FP64 ~370 with
+// accumulation registers).  The recompute beside the reverse sweep uses fewer
+// threads so that, in FP32, two of its waves and two reverse waves share a SIMD
+// (2 x 88 + 2 x 168 = 512 VGPRs).  The thread count does not change results:
+// every item is computed by one thread between barriers.
 template <typename T> struct DiffThreads;
-template <> struct DiffThreads<float> { static constexpr int FWD = 1024, REV = 512; };
-template <> struct DiffThreads<double> { static constexpr int FWD = 512, REV = 256; };
+template <> struct DiffThreads<float> { static constexpr int FWD = 1024, RECOMP = 512, REV = 512; };
+template <> struct DiffThreads<double> { static constexpr int FWD = 512, RECOMP = 512, REV = 256; };
 
 template <typename T> struct DiffBatch : DiffBase {
     DD<T> d{};
@@ -113,7 +120,8 @@ template <typename T> struct DiffBatch : DiffBase {
     double *d_phase = nullptr, *d_times = nullptr;
     T *d_E = nullptr, *d_gE = nullptr;
     T4<T> *ckV = nullptr, *ckL = nullptr, *tapeV = nullptr, *tapeL = nullptr;
-    int ckSlots = 0, tapeSlots = 0, timesCap = 0;
+    T4<T> *tapeV2 = nullptr, *tapeL2 = nullptr;  // second tape: recompute beside the reverse sweep
+    int ckSlots = 0, tapeSlots = 0, tape2Slots = 0, timesCap = 0;
     std::vector<VoxelMat> hvm;           // material of each voxel
     T4<T> *fin = nullptr;                // final state of the last forward (su, sq, sp, sh)
     std::vector<int> hiz, hvl;           // lattice z; 6 link ids per voxel
@@ -137,6 +145,11 @@ template <typename T> struct DiffBatch : DiffBase {
                 set_error("differentiable batches need one actuation and floor setting for every robot");
                 return VX3MI_E_ARG;
             }
+            for (int v = 0; v < m.nv(); v++)
+                if (m.ext_of(v) >= 0) {
+                    set_error("differentiable mode: voxel externals (boundary conditions) are not supported");
+                    return VX3MI_E_ARG;
+                }
             RobotParams &p = hrp[r];
             std::memset(&p, 0, sizeof(p));
             p.voff = (int)hvm.size();
@@ -192,6 +205,7 @@ template <typename T> struct DiffBatch : DiffBase {
         else dt = (float)(dtFrac * (maxF2 > 0 ? 1.0f / (6.283185f * std::sqrt(maxF2)) : 1e-10));
 
         HIP_TRY(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&rstream, hipStreamNonBlocking));
         HIP_TRY(hipEventCreate(&ev0));
         HIP_TRY(hipEventCreate(&ev1));
         int rc;
@@ -252,16 +266,18 @@ template <typename T> struct DiffBatch : DiffBase {
 
     int ensure_buffers(int nsteps) {
         // checkpoint interval: as given, else the whole run if its tape fits the
-        // budget (no recompute), else the interval minimising checkpoint + tape bytes
+        // budget (no recompute), else the interval minimising checkpoint + tape
+        // bytes (two tapes once there are several segments)
         int kk = optsIn.checkpoint_every;
         if (kk <= 0) {
             const double budget = optsIn.hbm_budget;
             const double full = (double)bytes_tape_slot() * nsteps + (double)bytes_ck_slot();
             if (full <= budget) kk = nsteps;
             else {
-                kk = (int)std::ceil(std::sqrt((double)nsteps * bytes_ck_slot() / (double)bytes_tape_slot()));
+                kk = (int)std::ceil(std::sqrt((double)nsteps * bytes_ck_slot() / (2.0 * bytes_tape_slot())));
                 kk = std::max(1, std::min(kk, nsteps));
-                const double need = (double)bytes_ck_slot() * ((nsteps + kk - 1) / kk) + (double)bytes_tape_slot() * kk;
+                const double need =
+                    (double)bytes_ck_slot() * ((nsteps + kk - 1) / kk) + 2.0 * (double)bytes_tape_slot() * kk;
                 if (need > budget) {
                     set_error("checkpoints and tape of this run exceed the memory budget");
                     return VX3MI_E_LIMIT;
@@ -269,7 +285,7 @@ template <typename T> struct DiffBatch : DiffBase {
             }
         }
         K = std::max(1, std::min(kk, nsteps));
-        const int needCk = (nsteps + K - 1) / K, needTape = K;
+        const int needCk = (nsteps + K - 1) / K, needTape = K, needTape2 = needCk > 1 ? K : 0;
         int rc;
         if (needCk > ckSlots) {
             if (ckV) { free_one(ckV); free_one(ckL); }
@@ -284,13 +300,20 @@ template <typename T> struct DiffBatch : DiffBase {
                 return rc;
             tapeSlots = needTape;
         }
+        if (needTape2 > tape2Slots) {
+            if (tapeV2) { free_one(tapeV2); free_one(tapeL2); }
+            if ((rc = alloc(&tapeV2, (size_t)needTape2 * 5 * V)) ||
+                (rc = alloc(&tapeL2, (size_t)needTape2 * 3 * std::max(L, 1))))
+                return rc;
+            tape2Slots = needTape2;
+        }
         if (nsteps + 1 > timesCap) {
             if (d_times) free_one(d_times);
             if ((rc = alloc(&d_times, (size_t)nsteps + 1))) return rc;
             timesCap = nsteps + 1;
         }
         ckBytes = (double)bytes_ck_slot() * needCk;
-        tapeBytes = (double)bytes_tape_slot() * needTape;
+        tapeBytes = (double)bytes_tape_slot() * (needTape + needTape2);
         return VX3MI_OK;
     }
 
@@ -378,27 +401,40 @@ template <typename T> struct DiffBatch : DiffBase {
         HIP_TRY(hipMemsetAsync(d.gPhase, 0, sizeof(T) * V, stream));
         const int nseg = (steps + K - 1) / K;
         const int nb = (std::max(V, L) + 255) / 256;
-        while ((int)segEv.size() < 3 * nseg) {
+        while ((int)segEv.size() < 4 * nseg) {
             hipEvent_t e;
             HIP_TRY(hipEventCreate(&e));
             segEv.push_back(e);
         }
+        // segment s uses tape s % 2; its recompute (on rstream) waits until the
+        // reverse sweep of segment s + 2, the last reader of that tape, is done
+        T4<T> *bufV[2] = {tapeV, tapeV2}, *bufL[2] = {tapeL, tapeL2};
+        const bool recompute = !(tapeValid && nseg == 1);
+        HIP_TRY(hipStreamWaitEvent(rstream, ev0, 0));
+        auto recompute_seg = [&](int seg) -> int {
+            const int s0 = seg * K, len = std::min(K, steps - s0);
+            if (seg + 2 < nseg) HIP_TRY(hipStreamWaitEvent(rstream, segEv[4 * (seg + 2) + 3], 0));
+            HIP_TRY(hipEventRecord(segEv[4 * seg], rstream));
+            k_diff_restore<T><<<std::max(nb, 1), 256, 0, rstream>>>(d, ckV, ckL, seg);
+            for (int c = 0; c < len; c += chunk) {
+                const int k = std::min(chunk, len - c);
+                k_diff_forward<T, DiffThreads<T>::RECOMP><<<R, DiffThreads<T>::RECOMP, 0, rstream>>>(
+                    d, d_rp, s0 + c, k, K, nullptr, nullptr, bufV[seg & 1], bufL[seg & 1], s0);
+            }
+            HIP_TRY(hipEventRecord(segEv[4 * seg + 1], rstream));
+            return VX3MI_OK;
+        };
+        int rc;
+        if (recompute && (rc = recompute_seg(nseg - 1))) return rc;
         for (int seg = nseg - 1; seg >= 0; seg--) {
             const int s0 = seg * K, len = std::min(K, steps - s0);
-            HIP_TRY(hipEventRecord(segEv[3 * seg], stream));
-            if (!(tapeValid && nseg == 1)) {
-                k_diff_restore<T><<<std::max(nb, 1), 256, 0, stream>>>(d, ckV, ckL, seg);
-                for (int c = 0; c < len; c += chunk) {
-                    const int k = std::min(chunk, len - c);
-                    k_diff_forward<T, DiffThreads<T>::FWD><<<R, DiffThreads<T>::FWD, 0, stream>>>(
-                        d, d_rp, s0 + c, k, K, nullptr, nullptr, tapeV, tapeL, s0);
-                }
-            }
-            HIP_TRY(hipEventRecord(segEv[3 * seg + 1], stream));
+            if (recompute) HIP_TRY(hipStreamWaitEvent(stream, segEv[4 * seg + 1], 0));
+            HIP_TRY(hipEventRecord(segEv[4 * seg + 2], stream));
             const int gatherFirst = seg != nseg - 1;
-            k_diff_reverse<T, DiffThreads<T>::REV><<<R, DiffThreads<T>::REV, 0, stream>>>(d, d_rp, s0, len, gatherFirst,
-                                                                                          tapeV, tapeL);
-            HIP_TRY(hipEventRecord(segEv[3 * seg + 2], stream));
+            k_diff_reverse<T, DiffThreads<T>::REV><<<R, DiffThreads<T>::REV, 0, stream>>>(
+                d, d_rp, s0, len, gatherFirst, bufV[seg & 1], bufL[seg & 1]);
+            HIP_TRY(hipEventRecord(segEv[4 * seg + 3], stream));
+            if (recompute && seg > 0 && (rc = recompute_seg(seg - 1))) return rc;
         }
         k_diff_finish<T><<<(V + 255) / 256, 256, 0, stream>>>(d, d_gE);
         HIP_TRY(hipEventRecord(ev1, stream));
@@ -411,8 +447,8 @@ template <typename T> struct DiffBatch : DiffBase {
         recomputeMs = reverseMs = 0;
         for (int seg = 0; seg < nseg; seg++) {
             float a = 0, b = 0;
-            HIP_TRY(hipEventElapsedTime(&a, segEv[3 * seg], segEv[3 * seg + 1]));
-            HIP_TRY(hipEventElapsedTime(&b, segEv[3 * seg + 1], segEv[3 * seg + 2]));
+            if (recompute) HIP_TRY(hipEventElapsedTime(&a, segEv[4 * seg], segEv[4 * seg + 1]));
+            HIP_TRY(hipEventElapsedTime(&b, segEv[4 * seg + 2], segEv[4 * seg + 3]));
             recomputeMs += a;
             reverseMs += b;
         }

@@ -42,6 +42,8 @@ struct Model {
     std::vector<double> phase;      // cycles
     std::vector<int> vlinks;        // 6 per voxel, X_POS..Z_NEG, -1 none
     std::vector<int> surface;       // surface voxel indices
+    std::vector<int> vext;          // index into exts per voxel, -1 none (empty: no externals)
+    std::vector<VoxelExt> exts;     // externals (CVX_Voxel::external())
     // links (link order)
     std::vector<int> lneg, lpos, laxis, lmat;
     // materials
@@ -63,8 +65,13 @@ struct Model {
     int secondaryExperiment = 0;
     double reinitInitialPositionAfter = 0.0;
     int nv() const { return (int)ix.size(); }
+    int ext_of(int v) const { return vext.empty() ? -1 : vext[v]; }
     int nl() const { return (int)lneg.size(); }
 };
+
+// CVX_Voxel::external() + CVX_External::setDisplacement / setForce / setMoment
+// for voxel v (dof < 0 removes its external)
+void set_external(Model &m, int v, int dof, const double t[3], const double r[3], const float f[3], const float mo[3]);
 
 // recommendedTimeStep, VX3_VoxelyzeKernel.cu:184-217 (double arithmetic on
 // the float material constants)

@@ -69,6 +69,7 @@ VoxelMat to_voxel_mat(const MatDerived &d) {
     v.zetaI = d.zI;
     v.zetaG = d.zG;
     v.zetaC = d.zC;
+    v.ext = -1;
     return v;
 }
 
@@ -281,6 +282,33 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
     return VX3MI_OK;
 }
 
+void set_external(Model &m, int v, int dof, const double t[3], const double r[3], const float f[3], const float mo[3]) {
+    if (m.vext.empty()) m.vext.assign((size_t)m.nv(), -1);
+    if (dof < 0) {
+        m.vext[v] = -1;
+        return;
+    }
+    VoxelExt e;
+    std::memset(&e, 0, sizeof(e));
+    e.dof = dof & DOF_ALL;
+    for (int k = 0; k < 3; k++) {
+        e.t[k] = t ? t[k] : 0.0;
+        e.r[k] = r ? r[k] : 0.0;
+        e.f[k] = f ? f[k] : 0.0f;
+        e.m[k] = mo ? mo[k] : 0.0f;
+    }
+    // rotationChanged(), src/old/VX_External.cpp: identity for a zero rotation
+    // vector, else Quat3D(extRotation) (FromRotationVector)
+    Q4<double> q = {1, 0, 0, 0};
+    if (e.r[0] != 0 || e.r[1] != 0 || e.r[2] != 0) q = from_rotvec(V3<double>{e.r[0], e.r[1], e.r[2]});
+    e.q[0] = q.w; e.q[1] = q.x; e.q[2] = q.y; e.q[3] = q.z;
+    if (m.vext[v] >= 0) m.exts[m.vext[v]] = e;
+    else {
+        m.vext[v] = (int)m.exts.size();
+        m.exts.push_back(e);
+    }
+}
+
 }  // namespace vx3mi
 
 using namespace vx3mi;
@@ -387,6 +415,39 @@ int vx3mi_model_link_constants(const vx3mi_model *m, int link, float *o) {
     const LinkMat &l = m->m.lmats[m->m.lmat[link]];
     const float vals[11] = {l.E, l.a1, l.a2, l.b1, l.b2, l.b3, l.sqA1, l.sqA2xIp, l.sqB1, l.sqB2xFMp, l.sqB3xIp};
     for (int i = 0; i < 11; i++) o[i] = vals[i];
+    return VX3MI_OK;
+}
+
+int vx3mi_model_set_external(vx3mi_model *m, int voxel, int dof, const double *translation3, const double *rotation3,
+                             const double *force3, const double *moment3) {
+    if (!m || voxel < 0 || voxel >= m->m.nv()) {
+        set_error("vx3mi_model_set_external: bad model or voxel index");
+        return VX3MI_E_ARG;
+    }
+    float f[3] = {0, 0, 0}, mo[3] = {0, 0, 0};
+    for (int k = 0; k < 3; k++) {
+        if (force3) f[k] = (float)force3[k];
+        if (moment3) mo[k] = (float)moment3[k];
+    }
+    set_external(m->m, voxel, dof, translation3, rotation3, f, mo);
+    return VX3MI_OK;
+}
+
+int vx3mi_model_get_external(const vx3mi_model *m, int voxel, int *dof, double *translation3, double *rotation3,
+                             double *force3, double *moment3) {
+    if (!m || voxel < 0 || voxel >= m->m.nv()) {
+        set_error("vx3mi_model_get_external: bad model or voxel index");
+        return VX3MI_E_ARG;
+    }
+    const int e = m->m.ext_of(voxel);
+    if (dof) *dof = e < 0 ? -1 : m->m.exts[e].dof;
+    for (int k = 0; k < 3; k++) {
+        const VoxelExt *x = e < 0 ? nullptr : &m->m.exts[e];
+        if (translation3) translation3[k] = x ? x->t[k] : 0.0;
+        if (rotation3) rotation3[k] = x ? x->r[k] : 0.0;
+        if (force3) force3[k] = x ? x->f[k] : 0.0;
+        if (moment3) moment3[k] = x ? x->m[k] : 0.0;
+    }
     return VX3MI_OK;
 }
 

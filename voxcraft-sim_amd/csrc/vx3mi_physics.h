@@ -77,7 +77,21 @@ struct VoxelMat {
     float muS, muK;
     int fixed, isMeasured;
     float zetaI, zetaG, zetaC;  // damping ratios (the differentiable mode derives its own constants)
-    float pad;
+    int ext;        // index of the voxel external (VoxelExt table), -1 none
+};
+
+// External of a voxel (VX3_External.h:27-31, src/old/VX_External.h): fixed
+// degrees of freedom, the prescribed translation / rotation of the fixed ones,
+// an applied force and moment.  Voxels with an external get their own
+// material table entry pointing at it, so voxels without one cost nothing.
+enum { DOF_X = 1, DOF_Y = 2, DOF_Z = 4, DOF_TX = 8, DOF_TY = 16, DOF_TZ = 32, DOF_ALL = 63 };
+struct VoxelExt {
+    double t[3];       // extTranslation (m)
+    double q[4];       // rotationQuat() of extRotation: w, x, y, z
+    double r[3];       // extRotation (rotation vector, rad)
+    float f[3], m[3];  // extForce (N), extMoment (N m)
+    int dof;           // dofFixed
+    int pad;
 };
 
 // Link material (VX3_MaterialLink.h:27-36) plus per-end constants the link
@@ -431,18 +445,40 @@ template <typename T> struct VoxelState {
     int flags;
 };
 
+// position field of a voxel fixed at lattice coordinate c plus translation t
+// (pos = ix * nominalSize() + t, VX3_Voxel.cu:169, 225-233)
+template <typename T> VX_HD T fixed_coordinate(int c, double nomSize, double t) {
+    if constexpr (kAbsolutePositions<T>) return (T)(c * nomSize + t);
+    else { (void)c; (void)nomSize; return (T)t; }
+}
+
 // VX3_Voxel::timeStep VX3_Voxel.cu:162-277 with force() :350-380, moment()
 // :382-397, floorForce :399-426.  fLink / mLink are the link force / moment
 // sums in the voxel frame (summed in link direction order X_POS..Z_NEG),
-// contact is the collision force to subtract (VX3_Voxel.cu:372-373).
+// contact is the collision force to subtract (VX3_Voxel.cu:372-373).  lat is
+// the packed lattice coordinate (ix | iy << 10 | iz << 20); exts the external
+// table (read only when vm.ext >= 0).
 template <typename T>
-VX_HD void voxel_step(const VoxelMat &vm, float dtf, int iz, float temp, V3<T> fLink, V3<T> mLink, V3<T> contact,
-                      VoxelState<T> &s) {
+VX_HD void voxel_step(const VoxelMat &vm, const VoxelExt *exts, float dtf, int lat, float temp, V3<T> fLink,
+                      V3<T> mLink, V3<T> contact, VoxelState<T> &s) {
     if (dtf == 0.0f) return;
     const T dt = (T)dtf;
+    const int iz = lat >> 20;
     int flags = s.flags;
+    const VoxelExt *ext = vm.ext >= 0 ? exts + vm.ext : nullptr;
+    if (ext && ext->dof == DOF_ALL) {
+        // all six fixed: pos = originalPosition() + translation, orient =
+        // rotationQuat(), haltMotion() (VX3_Voxel.cu:167-172)
+        s.u = V3<T>{fixed_coordinate<T>(lat & 1023, vm.nomSize, ext->t[0]),
+                    fixed_coordinate<T>((lat >> 10) & 1023, vm.nomSize, ext->t[1]),
+                    fixed_coordinate<T>(iz, vm.nomSize, ext->t[2])};
+        s.q = Q4<T>{(T)ext->q[0], (T)ext->q[1], (T)ext->q[2], (T)ext->q[3]};
+        s.p = s.h = V3<T>{0, 0, 0};
+        return;
+    }
 
     V3<T> tf = qrot(s.q, fLink);
+    if (ext) tf = tf + V3<T>{(T)ext->f[0], (T)ext->f[1], (T)ext->f[2]};  // external forces (:363-364)
     V3<T> vel = s.p * (T)vm.massInv;
     tf = tf - vel * (T)vm.gDampT;
     tf.z += vm.gravF;
@@ -487,9 +523,29 @@ VX_HD void voxel_step(const VoxelMat &vm, float dtf, int iz, float temp, V3<T> f
     s.p = p;
 
     V3<T> tm = qrot(s.q, mLink);
+    if (ext) tm = tm + V3<T>{(T)ext->m[0], (T)ext->m[1], (T)ext->m[2]};  // external moments (:393-394)
     tm = tm - (s.h * (T)vm.inertiaInv) * (T)vm.gDampR;
     V3<T> h = s.h + tm * dt;
     s.q = qmul(from_rotvec(h * (dt * (T)vm.inertiaInv)), s.q);
+    if (ext && ext->dof) {
+        // fixed degrees of freedom (VX3_Voxel.cu:222-256)
+        const int dof = ext->dof;
+        if (dof & DOF_X) { s.u.x = fixed_coordinate<T>(lat & 1023, vm.nomSize, ext->t[0]); s.p.x = 0; }
+        if (dof & DOF_Y) { s.u.y = fixed_coordinate<T>((lat >> 10) & 1023, vm.nomSize, ext->t[1]); s.p.y = 0; }
+        if (dof & DOF_Z) { s.u.z = fixed_coordinate<T>(iz, vm.nomSize, ext->t[2]); s.p.z = 0; }
+        if (dof & (DOF_TX | DOF_TY | DOF_TZ)) {
+            if ((dof & (DOF_TX | DOF_TY | DOF_TZ)) == (DOF_TX | DOF_TY | DOF_TZ)) {
+                s.q = Q4<T>{(T)ext->q[0], (T)ext->q[1], (T)ext->q[2], (T)ext->q[3]};
+                h = V3<T>{0, 0, 0};
+            } else {  // partial rotation fixes: through the rotation vector
+                V3<T> rv = to_rotvec(s.q);
+                if (dof & DOF_TX) { rv.x = 0; h.x = 0; }
+                if (dof & DOF_TY) { rv.y = 0; h.y = 0; }
+                if (dof & DOF_TZ) { rv.z = 0; h.z = 0; }
+                s.q = from_rotvec(rv);
+            }
+        }
+    }
     if constexpr (std::is_same<T, float>::value) {
         // single precision only: keep the orientation a unit quaternion (the
         // FP64 reference never renormalises, VX3_Voxel.cu:221, and drifts by

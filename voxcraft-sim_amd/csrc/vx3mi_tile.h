@@ -143,7 +143,7 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
         }
     }
     // ---- owned voxels
-    int vGlob[KV], vSlot[KV], vMat[KV], vIz[KV];
+    int vGlob[KV], vSlot[KV], vMat[KV], vLat[KV];
     int vLinks[KV][3];  // local link ids, two 16-bit fields each (0xffff = none)
     T vPhase[KV];
     VoxelState<T> vs[KV];
@@ -157,7 +157,7 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
             vGlob[k] = gv;
             vSlot[k] = j;
             vMat[k] = a.vmat[gv];
-            vIz[k] = a.lat[gv] >> 20;
+            vLat[k] = a.lat[gv];
             vPhase[k] = a.phase[gv];
             const int lbase = tr.lpre[j] - tr.loffG[j];
 #pragma unroll
@@ -229,7 +229,7 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
                 ms = ms + V3<T>{sForce[o + 3][l], sForce[o + 4][l], sForce[o + 5][l]};
             }
             float temp = (float)sPose[7][lv];
-            voxel_step<T>(vm, dt, vIz[k], temp, fs, ms, V3<T>{0, 0, 0}, vs[k]);
+            voxel_step<T>(vm, a.exts, dt, vLat[k], temp, fs, ms, V3<T>{0, 0, 0}, vs[k]);
             const int tf = tr.tempFlags[j];
             if ((tf & 1) && !(vm.thermalOnAfter > tnext))
                 temp = actuation_temperature<T>(tr.tempAmp[j], tr.tempPeriod[j], tnext, (double)vPhase[k], tf >> 1);

@@ -280,8 +280,21 @@ std::vector<std::string> split_commas(const std::string &s) {
     return out;
 }
 
+// One boundary-condition region (CVX_FRegion, src/old/VX_FRegion.cpp:165-219):
+// a box, cylinder or sphere in workspace fractions, the degrees of freedom it
+// fixes and the displacement / force it applies.
+struct BcRegion {
+    int prim = 0;  // PRIM_BOX 0, PRIM_CYLINDER 1, PRIM_SPHERE 2
+    double X = 0, Y = 0, Z = 0, dX = 0, dY = 0, dZ = 0, radius = 0;
+    int dof = 0;
+    double force[3] = {0, 0, 0}, torque[3] = {0, 0, 0}, disp[3] = {0, 0, 0}, angDisp[3] = {0, 0, 0};
+};
+
 struct VxaContent {
     vx3mi_lattice lat{};
+    std::vector<BcRegion> bcs;
+    double latticeDim = 0.001;              // Lattice_Dim
+    double dimAdj[3] = {1.0, 1.0, 1.0};     // X/Y/Z_Dim_Adj
     std::vector<signed char> cells;
     std::vector<double> phase;
     std::vector<vx3mi_material> mats;
@@ -292,6 +305,124 @@ struct VxaContent {
     int secondaryExperiment = 0;
     double reinitAfter = 0.0;
 };
+
+// CVX_FRegion::ReadXML (src/old/VX_FRegion.cpp:165-219)
+int read_region(const XNode *fr, BcRegion &b, std::string &err) {
+    if (!load(fr, "PrimType", b.prim)) { err = "FRegion without PrimType"; return VX3MI_E_PARSE; }
+    if (b.prim < 0 || b.prim > 2) { err = "only box, cylinder and sphere boundary-condition regions are supported"; return VX3MI_E_PARSE; }
+    load(fr, "X", b.X); load(fr, "Y", b.Y); load(fr, "Z", b.Z);
+    load(fr, "dX", b.dX); load(fr, "dY", b.dY); load(fr, "dZ", b.dZ);
+    load(fr, "Radius", b.radius);
+    int v;
+    if (load(fr, "DofFixed", v)) b.dof = (int)(unsigned char)(char)v & 63;
+    else b.dof = (load(fr, "Fixed", v) && v) ? 63 : 0;  // older files: binary Fixed tag
+    const char *fk[3] = {"ForceX", "ForceY", "ForceZ"}, *tk[3] = {"TorqueX", "TorqueY", "TorqueZ"};
+    const char *dk[3] = {"DisplaceX", "DisplaceY", "DisplaceZ"};
+    const char *ak[3] = {"AngDisplaceX", "AngDisplaceY", "AngDisplaceZ"};
+    for (int k = 0; k < 3; k++) {
+        load(fr, fk[k], b.force[k]);
+        load(fr, tk[k], b.torque[k]);
+        load(fr, dk[k], b.disp[k]);
+        load(fr, ak[k], b.angDisp[k]);
+    }
+    return VX3MI_OK;
+}
+
+// the first n FRegion children of a region list (CVX_Environment::ReadXML,
+// src/old/VX_Environment.cpp:105-146)
+int read_regions(const XNode *list, const char *countKey, std::vector<BcRegion> &out, std::string &err) {
+    int n = 0;
+    if (!load(list, countKey, n) || n <= 0) return VX3MI_OK;
+    for (const XNode &k : list->kids) {
+        if ((int)out.size() >= n) break;
+        if (k.name != "FRegion") continue;
+        BcRegion b;
+        int rc = read_region(&k, b, err);
+        if (rc != VX3MI_OK) return rc;
+        out.push_back(b);
+    }
+    if ((int)out.size() < n) { err = "fewer FRegion elements than the region count"; return VX3MI_E_PARSE; }
+    return VX3MI_OK;
+}
+
+// CPrimitive::IsTouching(P, Dist, Envelope) of the box / cylinder / sphere
+// (src/old/VX_FRegion.cpp:420-428, 494-524, 601-605)
+bool region_touches(const BcRegion &b, const double P[3], const double D[3], const double Env[3]) {
+    if (b.prim == 0) {
+        const double B0[3] = {b.X, b.Y, b.Z}, B1[3] = {b.dX, b.dY, b.dZ};
+        for (int k = 0; k < 3; k++) {
+            const double ps = P[k] / Env[k], ds = D[k] / Env[k];
+            if (!(ps + ds > B0[k] && ps - ds < B0[k] + B1[k])) return false;
+        }
+        return true;
+    }
+    const double pos[3] = {b.X * Env[0], b.Y * Env[1], b.Z * Env[2]};
+    const double size[3] = {b.dX * Env[0], b.dY * Env[1], b.dZ * Env[2]};
+    const double m01 = Env[0] > Env[1] ? Env[0] : Env[1], emax = Env[2] > m01 ? Env[2] : m01;  // Vec3D::Max()
+    const double rad = b.radius * emax;
+    if (b.prim == 2) {
+        const double dist = std::sqrt(D[0] * D[0] + D[1] * D[1] + D[2] * D[2]);
+        const double dx = P[0] - pos[0], dy = P[1] - pos[1], dz = P[2] - pos[2];
+        return std::sqrt(dx * dx + dy * dy + dz * dz) < rad + dist;
+    }
+    // cylinder: IsIn along its one non-zero axis
+    const double dim[3] = {b.dX, b.dY, b.dZ};
+    for (int a = 0; a < 3; a++) {
+        if (dim[a] == 0) continue;
+        const int u = (a + 1) % 3, w = (a + 2) % 3;
+        const int lo = u < w ? u : w, hi = u < w ? w : u;
+        return P[a] > pos[a] && P[a] < pos[a] + size[a] &&
+               std::sqrt((P[lo] - pos[lo]) * (P[lo] - pos[lo]) + (P[hi] - pos[hi]) * (P[hi] - pos[hi])) < rad;
+    }
+    return false;
+}
+
+// The boundary conditions of CVX_Sim::Import (src/VXA/VX_Sim.cpp:109-143): each
+// region's voxel count (CVX_Environment::GetNumTouching, VX_Environment.cpp:215-232,
+// from CVX_Object::GetXYZ's cell centres) divides its force and torque; a voxel
+// whose centre touches the region gets the region's fixed degrees of freedom
+// with their displacements, and force / moment added as
+// external()->addForce(external()->force() + Force / count).
+void apply_boundary_conditions(const VxaContent &c, Model &m) {
+    const double Ld = c.latticeDim;
+    const double env[3] = {Ld * c.dimAdj[0], Ld * c.dimAdj[1], Ld * c.dimAdj[2]};  // GetLatDimEnv()
+    const double half[3] = {env[0] / 2.0, env[1] / 2.0, env[2] / 2.0};
+    const double ws[3] = {Ld * c.dimAdj[0] * m.nx, Ld * c.dimAdj[1] * m.ny, Ld * c.dimAdj[2] * m.nz};
+    const int nv = m.nv();
+    std::vector<int> sizes(c.bcs.size(), 0);
+    for (size_t j = 0; j < c.bcs.size(); j++)
+        for (int v = 0; v < nv; v++) {
+            const double eps = 0.000001;  // GetXYZ's offset (VX_Object.cpp:533)
+            const double P[3] = {Ld * (c.dimAdj[0] * (0.5 + m.ix[v]) + eps), Ld * (c.dimAdj[1] * (0.5 + m.iy[v]) + eps),
+                                 Ld * c.dimAdj[2] * (0.5 + m.iz[v])};
+            if (region_touches(c.bcs[j], P, half, ws)) sizes[j]++;
+        }
+    const double s = env[0];  // voxel size (Vx.setVoxelSize(GetLatDimEnv().x))
+    for (int v = 0; v < nv; v++) {
+        const double P[3] = {m.ix[v] * s + half[0], m.iy[v] * s + half[1], m.iz[v] * s + half[2]};
+        bool has = false;
+        int dof = 0;
+        double t[3] = {0, 0, 0}, r[3] = {0, 0, 0};
+        float f[3] = {0, 0, 0}, mo[3] = {0, 0, 0};
+        for (size_t j = 0; j < c.bcs.size(); j++) {
+            const BcRegion &b = c.bcs[j];
+            if (!region_touches(b, P, half, ws)) continue;
+            has = true;
+            for (int k = 0; k < 3; k++) {
+                // setDisplacement: fixes the dof, keeps a zero displacement out
+                if (b.dof & (1 << k)) { dof |= 1 << k; if (b.disp[k] != 0.0f) t[k] = b.disp[k]; }
+                if (b.dof & (8 << k)) { dof |= 8 << k; if (b.angDisp[k] != 0.0f) r[k] = b.angDisp[k]; }
+            }
+            for (int k = 0; k < 3; k++) {
+                const float gf = (float)((double)f[k] + b.force[k] / sizes[j]);
+                const float gm = (float)((double)mo[k] + b.torque[k] / sizes[j]);
+                f[k] += gf;
+                mo[k] += gm;
+            }
+        }
+        if (has) set_external(m, v, dof, t, r, f, mo);
+    }
+}
 
 // boost property_tree get<bool> (VX3_SimulationManager.cu:328-330): "true"/"1" or "false"/"0"
 bool load_bool(const XNode *parent, const char *key, bool dflt) {
@@ -422,10 +553,15 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
     L.temp_base = 25;
     L.temp_period = 0.1;
     if (const XNode *env = vxa->child("Environment")) {
+        int rc = VX3MI_OK;
         if (const XNode *bc = env->child("Boundary_Conditions")) {
-            int nbc = 0;
-            if (load(bc, "NumBCs", nbc) && nbc > 0) { err = "boundary conditions are not supported"; return VX3MI_E_PARSE; }
+            rc = read_regions(bc, "NumBCs", c.bcs, err);
+        } else {  // older files: separate fixed and forced region lists
+            if (const XNode *fx = env->child("Fixed_Regions")) rc = read_regions(fx, "NumFixed", c.bcs, err);
+            if (rc == VX3MI_OK)
+                if (const XNode *fo = env->child("Forced_Regions")) rc = read_regions(fo, "NumForced", c.bcs, err);
         }
+        if (rc != VX3MI_OK) return rc;
         if (const XNode *g = env->child("Gravity")) {
             int v;
             double d;
@@ -454,7 +590,18 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
     if (const XNode *lat = vxc->child("Lattice")) {
         if (!load(lat, "Lattice_Dim", L.lattice_dim)) L.lattice_dim = 0.001;
         if (!load(lat, "X_Dim_Adj", xadj)) xadj = 1.0;
+        if (!load(lat, "Y_Dim_Adj", c.dimAdj[1])) c.dimAdj[1] = 1.0;
+        if (!load(lat, "Z_Dim_Adj", c.dimAdj[2])) c.dimAdj[2] = 1.0;
+        double off = 0;
+        const char *offs[4] = {"X_Line_Offset", "Y_Line_Offset", "X_Layer_Offset", "Y_Layer_Offset"};
+        for (const char *k : offs)
+            if (!c.bcs.empty() && load(lat, k, off) && off != 0.0) {
+                err = "boundary-condition regions on a lattice with line or layer offsets are not supported";
+                return VX3MI_E_PARSE;
+            }
     }
+    c.latticeDim = L.lattice_dim;
+    c.dimAdj[0] = xadj;
     L.lattice_dim *= xadj;  // Vx.setVoxelSize(LocalVXC.GetLatDimEnv().x) (VX_Sim.cpp:88)
     if (const XNode *pal = vxc->child("Palette")) {
         for (const XNode &mn : pal->kids) {
@@ -584,6 +731,7 @@ int model_from_docs(const char *baseText, const char *vxdText, vx3mi_model **out
         m.recordStepSize = c.recordStepSize;
         m.recordVoxel = c.recordVoxel;
         m.recordLink = c.recordLink;
+        if (!c.bcs.empty()) apply_boundary_conditions(c, m);
     }
     return rc;
 }

@@ -23,6 +23,8 @@ _LIB_PATH = os.path.join(_PKG, "lib", "libvx3mi.so")
 
 FP32, FP64 = 32, 64
 MODE_AUTO, MODE_PLAIN, MODE_FLAT, MODE_PERSISTENT, MODE_TILED = 0, 1, 2, 3, 4
+# fixed degrees of freedom of a voxel external (CVX_External dofComponent bits)
+DOF_X, DOF_Y, DOF_Z, DOF_TX, DOF_TY, DOF_TZ, DOF_ALL = 1, 2, 4, 8, 16, 32, 63
 OK, DIVERGED = 0, 1
 
 PROBLEM_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=1.0, slow_damp_z=1.0,
@@ -119,6 +121,8 @@ def lib():
         "vx3mi_model_topology": (C.c_int, [vp, ip, ip, ip, ip]),
         "vx3mi_model_voxel_constants": (C.c_int, [vp, C.c_int, fp]),
         "vx3mi_model_link_constants": (C.c_int, [vp, C.c_int, fp]),
+        "vx3mi_model_set_external": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp]),
+        "vx3mi_model_get_external": (C.c_int, [vp, C.c_int, ip, dp, dp, dp, dp]),
         "vx3mi_batch_create": (C.c_int, [pvp, C.c_int, C.c_int, pvp]),
         "vx3mi_batch_destroy": (None, [vp]),
         "vx3mi_batch_counts": (C.c_int, [vp, ip, ip, ip]),
@@ -206,7 +210,10 @@ class Model:
                         p["temp_period"])
         h = C.c_void_p()
         _check(lib().vx3mi_model_from_lattice(C.byref(spec), C.byref(h)))
-        return cls(h.value)
+        m = cls(h.value)
+        for e in p.get("externals") or []:
+            m.set_external(**e)
+        return m
 
     @classmethod
     def from_vxa(cls, base_vxa: str, vxd: str | None = None) -> "Model":
@@ -260,6 +267,25 @@ class Model:
         o = np.zeros(11, dtype=np.float32)
         _check(lib().vx3mi_model_link_constants(self._h, int(l), _ptr(o, C.c_float)))
         return o
+
+    def set_external(self, voxel: int, dof: int = 0, translation=(0, 0, 0), rotation=(0, 0, 0),
+                     force=(0, 0, 0), moment=(0, 0, 0)) -> None:
+        """External of a voxel (CVX_Voxel::external()): dof bits 1,2,4 fix x,y,z
+        translation and 8,16,32 fix x,y,z rotation (DOF_* constants); translation
+        (m) / rotation vector (rad) where the fixed ones are held; force (N) and
+        moment (N m).  dof < 0 removes the external."""
+        a = [np.ascontiguousarray(x, dtype=np.float64) for x in (translation, rotation, force, moment)]
+        _check(lib().vx3mi_model_set_external(self._h, int(voxel), int(dof), *[_ptr(x, C.c_double) for x in a]))
+
+    def external(self, voxel: int) -> dict | None:
+        """The voxel's external as a dict (None when it has none)."""
+        dof = C.c_int()
+        a = [np.zeros(3) for _ in range(4)]
+        _check(lib().vx3mi_model_get_external(self._h, int(voxel), C.byref(dof),
+                                              *[_ptr(x, C.c_double) for x in a]))
+        if dof.value < 0:
+            return None
+        return dict(dof=dof.value, translation=a[0], rotation=a[1], force=a[2], moment=a[3])
 
     def __del__(self):
         if getattr(self, "_h", None) is not None and self._h.value:

@@ -67,7 +67,27 @@ def _environment(p: dict) -> str:
     therm = (_el("TempEnabled", str(int(p["temp_enabled"]))) + _el("TempBase", _num(p["temp_base"]))
              + _el("TempAmplitude", _num(p["temp_amplitude"])) + _el("VaryTempEnabled", str(int(p["vary_temp"])))
              + _el("TempPeriod", _num(p["temp_period"])))
-    return _el("Environment", _el("Gravity", grav) + _el("Thermal", therm))
+    return _el("Environment", boundary_conditions_xml(p.get("bc_regions") or []) + _el("Gravity", grav)
+               + _el("Thermal", therm))
+
+
+REGION_KEYS = ("PrimType", "X", "Y", "Z", "dX", "dY", "dZ", "Radius", "DofFixed", "ForceX", "ForceY", "ForceZ",
+               "TorqueX", "TorqueY", "TorqueZ", "DisplaceX", "DisplaceY", "DisplaceZ", "AngDisplaceX",
+               "AngDisplaceY", "AngDisplaceZ")
+
+
+def boundary_conditions_xml(regions: Sequence[dict]) -> str:
+    """<Boundary_Conditions> with one <FRegion> per dict (keys as in
+    CVX_FRegion::ReadXML: PrimType 0 box / 1 cylinder / 2 sphere, X..dZ and Radius
+    in workspace fractions, DofFixed bits, Force*, Torque*, Displace*,
+    AngDisplace*); an empty list writes nothing."""
+    if not regions:
+        return ""
+    body = _el("NumBCs", str(len(regions)))
+    for r in regions:
+        body += _el("FRegion", "".join(_el(k, _num(r[k]) if isinstance(r[k], float) else str(r[k]))
+                                       for k in REGION_KEYS if k in r))
+    return _el("Boundary_Conditions", body)
 
 
 def _palette(mats: Sequence[dict]) -> str:
