@@ -49,8 +49,10 @@ typedef struct {
     float cte;           /* CTE (1/degC) */
     float u_static;      /* uStatic */
     float u_dynamic;     /* uDynamic */
-    float fail_stress;   /* Fail_Stress (used when mat_model == 1) */
-    int mat_model;       /* MatModel: 0 linear, 1 linear with failure */
+    float fail_stress;   /* Fail_Stress (mat_model 1, 2; -1 = none) */
+    float plastic_mod;   /* Plastic_Mod (mat_model 2) */
+    float yield_stress;  /* Yield_Stress (mat_model 2) */
+    int mat_model;       /* MatModel: 0 linear, 1 linear with failure, 2 bilinear */
     int fixed;           /* Fixed */
     int is_measured;     /* isMeasured */
     int is_target;       /* isTarget */
@@ -133,6 +135,12 @@ int vx3mi_model_topology(const vx3mi_model *m, int *lattice3, int *link_neg, int
  * link (11): E, a1, a2, b1, b2, b3, sqA1, sqA2xIp, sqB1, sqB2xFMp, sqB3xIp */
 int vx3mi_model_voxel_constants(const vx3mi_model *m, int voxel, float *out12);
 int vx3mi_model_link_constants(const vx3mi_model *m, int link, float *out11);
+/* stress-strain curve of a link's material when it is non-linear (a bilinear
+ * palette material at either end): the springs-in-series combination of
+ * CVX_MaterialLink::updateAll (src/old/VX_MaterialLink.cpp:66-101), starting
+ * at (0, 0).  Returns the number of points (at most 8; 0 for a linear
+ * material) and writes them to strain / stress when not NULL. */
+int vx3mi_model_link_curve(const vx3mi_model *m, int link, float *strain8, float *stress8);
 
 /* External of a voxel: replaces CVX_Voxel::external() with
  * CVX_External::setDisplacement / setForce / setMoment (src/old/VX_Voxel.h:75,

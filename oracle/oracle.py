@@ -25,7 +25,16 @@ VXO_NO_TEMPERATURE = 2
 class _Material(C.Structure):
     _fields_ = [("E", C.c_float), ("rho", C.c_float), ("nu", C.c_float), ("cte", C.c_float),
                 ("mu_s", C.c_float), ("mu_k", C.c_float), ("fail_stress", C.c_float),
+                ("plastic_mod", C.c_float), ("yield_stress", C.c_float), ("model", C.c_int),
                 ("fixed", C.c_int), ("is_measured", C.c_int), ("thermal_on_after", C.c_double)]
+
+
+def material_model(m: dict) -> int:
+    """MatModel of a material dict: 2 (bilinear) with a yield stress, 1 (linear
+    with failure) with a positive failure stress, else 0 (linear)."""
+    if float(m.get("yield_stress", 0.0)) > 0:
+        return 2
+    return 1 if float(m.get("fail_stress", -1.0)) > 0 else 0
 
 
 class _Spec(C.Structure):
@@ -51,6 +60,7 @@ SPEC_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=
                      floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
                      temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
+                         plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, thermal_on_after=0.0)
 
 
@@ -95,6 +105,8 @@ def lib():
         L.vxo_center_of_mass.argtypes = [vp, dp]
         L.vxo_get_contact.argtypes = [vp, dp]
         L.vxo_set_external.argtypes = [vp, C.c_int, C.c_int, dp, dp, fp, fp]
+        L.vxo_get_link_curve.restype = C.c_int
+        L.vxo_get_link_curve.argtypes = [vp, C.c_int, fp, fp]
         L.vxo_get_voxel_material.argtypes = [vp, C.c_int, fp]
         L.vxo_get_link_material.argtypes = [vp, C.c_int, fp]
         L.vxo_diff_new.restype = vp
@@ -128,9 +140,10 @@ class OracleModel:
         mats = [dict(MATERIAL_DEFAULTS, **m) for m in p["materials"]]
         self._mats = (_Material * max(1, len(mats)))()
         for i, m in enumerate(mats):
+            model = material_model(m)
             self._mats[i] = _Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"],
-                                      m["fail_stress"], int(m["fixed"]), int(m["is_measured"]),
-                                      m["thermal_on_after"])
+                                      m["fail_stress"] if model else -1.0, m["plastic_mod"], m["yield_stress"],
+                                      model, int(m["fixed"]), int(m["is_measured"]), m["thermal_on_after"])
         self._phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         self._E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
         spec = _Spec(nx, ny, nz, _ptr(self._cells, C.c_int8), _ptr(self._phase, C.c_double),
@@ -146,6 +159,11 @@ class OracleModel:
         self.ns = self._L.vxo_num_surface(self._h)
         for e in p.get("externals") or []:
             self.set_external(**e)
+
+    def link_curve(self, l: int) -> np.ndarray:
+        st, ss = np.zeros(8, dtype=np.float32), np.zeros(8, dtype=np.float32)
+        n = self._L.vxo_get_link_curve(self._h, int(l), _ptr(st, C.c_float), _ptr(ss, C.c_float))
+        return np.stack([st[:n], ss[:n]], axis=1)
 
     def set_external(self, voxel: int, dof: int = 0, translation=(0, 0, 0), rotation=(0, 0, 0),
                      force=(0, 0, 0), moment=(0, 0, 0)) -> None:

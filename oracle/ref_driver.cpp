@@ -11,7 +11,8 @@
 //
 // stdin (whitespace separated):
 //   nx ny nz  lattice_dim  bond col slow  grav_acc floor  temp_amp temp_period vary enable_expansion t0
-//   nmat  then per material: E rho cte mu_s mu_k
+//   nmat  then per material: E rho cte mu_s mu_k model plastic_mod yield_stress fail_stress
+//     (model 0: setModelLinear(E), 1: setModelLinear(E, fail), 2: setModelBilinear)
 //   nx*ny*nz cell values (0 = empty)
 //   nv phase offsets (cycles)
 //   dt nsteps record_every
@@ -43,10 +44,13 @@ int main() {
     Vx.enableFloor(floorOn != 0);
     std::vector<CVX_Material *> mats;
     for (int i = 0; i < nmat; i++) {
-        float E, rho, cte, mus, muk;
-        std::cin >> E >> rho >> cte >> mus >> muk;
+        float E, rho, cte, mus, muk, plastic, yield, fail;
+        int model;
+        std::cin >> E >> rho >> cte >> mus >> muk >> model >> plastic >> yield >> fail;
         CVX_Material *pm = Vx.addMaterial();
-        pm->setModelLinear(E);
+        if (model == 2) pm->setModelBilinear(E, plastic, yield, fail);
+        else if (model == 1) pm->setModelLinear(E, fail);
+        else pm->setModelLinear(E);
         pm->setPoissonsRatio(0.0f);
         pm->setDensity(rho);
         pm->setCte(cte);

@@ -37,6 +37,8 @@ static const float SA_BOND_EXT_PERC = 0.50f;
 /* voxel flag bits, src/old/types.h:34-38 */
 #define VF_FLOOR_ENABLED (1 << 2)
 #define VF_FLOOR_STATIC_FRICTION (1 << 3)
+#define OM_MAX_POINTS 8  /* stress-strain points (two bilinear materials combine to 5) */
+
 /* link flag bits: src/old/types.h:6-8 (LOCAL_VELOCITY_VALID) + smallAngle */
 #define LF_LOCAL_VELOCITY_VALID 1
 #define LF_SMALL_ANGLE 2
@@ -48,12 +50,17 @@ typedef struct {
     float sigmaFail, epsFail;
     int fixed, isMeasured;
     double thermalOnAfter;
+    /* stress-strain model: CVX_Material linear / strainData / stressData */
+    int linear, npts;
+    float sd[OM_MAX_POINTS], td[OM_MAX_POINTS];
 } vmat;
 
 typedef struct {
     double nomSize;
     float E, nu, eHat, epsFail;
     float a1, a2, b1, b2, b3, sqA1, sqA2xIp, sqB1, sqB2xFMp, sqB3xIp;
+    int linear, npts;
+    float sd[OM_MAX_POINTS], td[OM_MAX_POINTS];
 } lmat;
 
 struct vxo_model {
@@ -82,7 +89,7 @@ struct vxo_model {
     int *lneg, *lpos, *laxis;
     lmat *lm;
     v3 *pos2, *a1v, *a2v;
-    float *strain, *maxStrain, *tArea, *tStrainSum;
+    float *strain, *maxStrain, *strainOffset, *tArea, *tStrainSum;
     double *restLen;
     int *lflags;
     v3 *fNeg, *fPos, *mNeg, *mPos;
@@ -241,14 +248,116 @@ static float clamp_nu(float nu) {
 }
 
 /* CVX_MaterialLink::updateAll/updateDerived src/old/VX_MaterialLink.cpp:45-141 */
+/* CVX_Material::setModelLinear (src/old/VX_Material.cpp:372-401), arguments valid */
+static void om_set_linear(vmat *m, float E, float fail) {
+    float tf = fail == -1 ? 1000000 : fail;
+    float tfs = tf / E;
+    m->npts = 2;
+    m->sd[0] = 0; m->sd[1] = tfs;
+    m->td[0] = 0; m->td[1] = tf;
+    m->linear = 1;
+    m->E = E;
+    m->sigmaFail = fail;
+    m->epsFail = fail == -1 ? -1 : tfs;
+}
+/* CVX_Material::setModelBilinear (src/old/VX_Material.cpp:406-451); 0 when it
+ * rejects the arguments (the model is then left as it was) */
+static int om_set_bilinear(vmat *m, float E, float plastic, float yield, float fail) {
+    if (E <= 0 || plastic <= 0 || plastic >= E || yield <= 0) return 0;
+    if (fail != -1.0f && fail <= yield) return 0;
+    float yieldStrain = yield / E;
+    float tf = fail == -1 ? 3 * yield : fail;
+    float tM = plastic;
+    float tB = yield - tM * yieldStrain;
+    float tfs = (tf - tB) / tM;
+    m->npts = 3;
+    m->sd[0] = 0; m->sd[1] = yieldStrain; m->sd[2] = tfs;
+    m->td[0] = 0; m->td[1] = yield; m->td[2] = tf;
+    m->linear = 0;
+    m->E = E;
+    m->sigmaFail = fail;
+    m->epsFail = fail == -1.0f ? -1.0f : tfs;
+    return 1;
+}
+/* CVX_Material::modulus (src/old/VX_Material.cpp:231-243) */
+static float om_modulus(const vmat *m, float strain) {
+    if (m->epsFail != -1.0f && strain > m->epsFail) return 0.0f;
+    if (strain <= m->sd[1] || m->linear) return m->E;
+    for (int i = 2; i < m->npts; i++)
+        if (strain <= m->sd[i] || i == m->npts - 1) return (m->td[i] - m->td[i - 1]) / (m->sd[i] - m->sd[i - 1]);
+    return 0.0f;
+}
+/* CVX_Material::strain(stress) (src/old/VX_Material.cpp:214-228) of a curve */
+static float om_strain_of(int linear, int n, const float *sd, const float *td, float E, float stress) {
+    if (stress <= td[1] || linear) return stress / E;
+    for (int i = 2; i < n; i++)
+        if (stress <= td[i] || i == n - 1) {
+            float perc = (stress - td[i - 1]) / (td[i] - td[i - 1]);
+            return sd[i - 1] + perc * (sd[i] - sd[i - 1]);
+        }
+    return 0.0f;
+}
+/* VX3_Material::stress (src/VX3/VX3_Material.cu:90-124) of a link material */
+static float om_stress(const lmat *m, float strain, float tss, int forceLinear) {
+    if (m->epsFail != -1.0f && strain > m->epsFail) return 0.0f;
+    if (strain <= m->sd[1] || m->linear || forceLinear) {
+        if (m->nu == 0.0f) return m->E * strain;
+        return m->eHat * ((1 - m->nu) * strain + m->nu * tss);
+    }
+    for (int i = 2; i < m->npts; i++) {
+        if (strain <= m->sd[i] || i == m->npts - 1) {
+            float perc = (strain - m->sd[i - 1]) / (m->sd[i] - m->sd[i - 1]);
+            float basic = m->td[i - 1] + perc * (m->td[i] - m->td[i - 1]);
+            if (m->nu == 0.0f) return basic;
+            float modulus = (m->td[i] - m->td[i - 1]) / (m->sd[i] - m->sd[i - 1]);
+            float modulusHat = modulus / ((1 - 2 * m->nu) * (1 + m->nu));
+            float effStrain = basic / modulus;
+            float effTss = tss * (effStrain / strain);
+            return modulusHat * ((1 - m->nu) * effStrain + m->nu * effTss);
+        }
+    }
+    return 0.0f;
+}
+
+/* CVX_MaterialLink::updateAll (src/old/VX_MaterialLink.cpp:45-125) */
 static void lmat_combine(lmat *l, const vmat *a, const vmat *b) {
     l->nomSize = 0.5 * (a->nomSize + b->nomSize);
     float f1 = a->sigmaFail, f2 = b->sigmaFail, sf;
     if (f1 == -1.0f) sf = f2;
     else if (f2 == -1.0f) sf = f1;
     else sf = f1 < f2 ? f1 : f2;
-    l->E = 2.0f * a->E * b->E / (a->E + b->E);
-    l->epsFail = (sf == -1.0f) ? -1.0f : sf / l->E;  /* setModelLinear VX_Material.cpp */
+    if (a->linear && b->linear) {
+        l->E = 2.0f * a->E * b->E / (a->E + b->E);
+        l->epsFail = (sf == -1.0f) ? -1.0f : sf / l->E;  /* setModelLinear VX_Material.cpp */
+        l->linear = 1;
+        l->npts = 2;
+        l->sd[0] = 0; l->td[0] = 0;
+        l->sd[1] = (sf == -1.0f ? 1000000 : sf) / l->E;
+        l->td[1] = sf == -1.0f ? 1000000 : sf;
+    } else {
+        /* series springs over the union of both materials' strain points */
+        float st[2 * OM_MAX_POINTS], ss[2 * OM_MAX_POINTS];
+        int n = 1, i1 = 1, i2 = 1;
+        st[0] = ss[0] = 0;
+        while (i1 < a->npts && i2 < b->npts) {
+            float e = FLT_MAX;
+            if (i1 < a->npts) e = a->sd[i1];
+            if (i2 < b->npts && b->sd[i2] < e) e = b->sd[i2];
+            if (e == a->sd[i1]) i1++;
+            if (e == b->sd[i2]) i2++;
+            float m1 = om_modulus(a, e - FLT_EPSILON), m2 = om_modulus(b, e - FLT_EPSILON);
+            float mm = 2.0f * m1 * m2 / (m1 + m2);
+            ss[n] = ss[n - 1] + mm * (e - st[n - 1]);
+            st[n] = e;
+            n++;
+        }
+        /* setModel (VX_Material.cpp:300-370): E from the first segment */
+        l->npts = n < OM_MAX_POINTS ? n : OM_MAX_POINTS;
+        for (int k = 0; k < l->npts; k++) { l->sd[k] = st[k]; l->td[k] = ss[k]; }
+        l->linear = (n - 1 == 1);
+        l->E = l->td[1] / l->sd[1];
+        l->epsFail = sf == -1.0f ? -1.0f : om_strain_of(l->linear, l->npts, l->sd, l->td, l->E, sf);
+    }
     if (a->nu == 0 && b->nu == 0) l->nu = 0;
     else {
         float tmpEHat = 2 * a->eHat * b->eHat / (a->eHat + b->eHat);
@@ -370,9 +479,16 @@ vxo_model *vxo_build(const vxo_spec *sp) {
         const vxo_material *in = &sp->mats[k];
         vmat *vm = &m->vm[k];
         vm->nomSize = sp->lattice_dim;
-        vm->E = in->E;
-        vm->sigmaFail = in->fail_stress;
-        vm->epsFail = (in->fail_stress == -1.0f) ? -1.0f : in->fail_stress / in->E;
+        /* CVX_Sim::CopyMat (src/VXA/VX_Sim.cpp:390-405): a setter that rejects its
+         * arguments leaves the addMaterial default, linear at 1e6 Pa */
+        om_set_linear(vm, 1e6f, -1.0f);
+        if (in->model == 0) {
+            if (in->E > 0) om_set_linear(vm, in->E, -1.0f);
+        } else if (in->model == 1) {
+            if (in->E > 0 && (in->fail_stress == -1.0f || in->fail_stress > 0)) om_set_linear(vm, in->E, in->fail_stress);
+        } else {
+            om_set_bilinear(vm, in->E, in->plastic_mod, in->yield_stress, in->fail_stress);
+        }
         vm->nu = sp->volume_effects ? clamp_nu(in->nu) : 0.0f;
         vm->rho = in->rho <= 0 ? FLT_MIN : in->rho;
         vm->alphaCTE = in->cte;
@@ -424,7 +540,7 @@ vxo_model *vxo_build(const vxo_spec *sp) {
             int k = npal + v;
             m->vm[k] = m->vm[pal];
             m->vm[k].E = sp->voxel_E[v];
-            m->vm[k].epsFail = (m->vm[k].sigmaFail == -1.0f) ? -1.0f : m->vm[k].sigmaFail / m->vm[k].E;
+            om_set_linear(&m->vm[k], sp->voxel_E[v], m->vm[k].sigmaFail);
             vmat_derive(&m->vm[k]);
             m->vmat_of[v] = k;
         } else {
@@ -467,6 +583,7 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->a2v = (v3 *)xcalloc(nl, sizeof(v3));
     m->strain = (float *)xcalloc(nl, sizeof(float));
     m->maxStrain = (float *)xcalloc(nl, sizeof(float));
+    m->strainOffset = (float *)xcalloc(nl, sizeof(float));
     m->tArea = (float *)xcalloc(nl, sizeof(float));
     m->tStrainSum = (float *)xcalloc(nl, sizeof(float));
     m->restLen = (double *)xcalloc(nl, sizeof(double));
@@ -514,7 +631,7 @@ void vxo_free(vxo_model *m) {
     void *ptrs[] = {m->ix, m->iy, m->iz, m->vmat_of, m->vlinks, m->pos, m->linMom, m->angMom, m->contact,
                     m->orient, m->temp, m->prevDt, m->vflags, m->phase, m->surface, m->is_surface, m->lneg,
                     m->lpos, m->laxis, m->lm, m->pos2, m->a1v, m->a2v, m->strain, m->maxStrain, m->tArea,
-                    m->tStrainSum, m->restLen, m->lflags, m->fNeg, m->fPos, m->mNeg, m->mPos, m->vm,
+                    m->tStrainSum, m->restLen, m->lflags, m->fNeg, m->fPos, m->mNeg, m->mPos, m->vm, m->strainOffset,
                     m->ext_dof, m->ext_t, m->ext_r, m->ext_q, m->ext_f, m->ext_m};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(m);
@@ -535,6 +652,13 @@ void vxo_set_external(vxo_model *m, int v, int dof, const double *t3, const doub
     v3 r = m->ext_r[v];
     if (r.x != 0 || r.y != 0 || r.z != 0) m->ext_q[v] = q_from_rotvec(r);
     else { q4 id = {1, 0, 0, 0}; m->ext_q[v] = id; }
+}
+
+int vxo_get_link_curve(const vxo_model *m, int link, float *strain, float *stress) {
+    const lmat *l = &m->lm[link];
+    if (l->linear) return 0;
+    for (int k = 0; k < l->npts; k++) { strain[k] = l->sd[k]; stress[k] = l->td[k]; }
+    return l->npts;
 }
 
 int vxo_num_voxels(const vxo_model *m) { return m->nv; }
@@ -614,14 +738,24 @@ static void link_update(vxo_model *m, int l) {
     v3 dA2 = vscale(vsub(a2v, oldA2v), 0.5);
     /* dPos2 = 0.5*(pos2-oldPos2): the reference multiplies 0.5*vec, identical */
 
-    /* updateStrain (linear): VX3_Link.cu:220-231, VX3_Material.cu:90-101 */
+    /* updateStrain: VX3_Link.cu:220-260, VX3_Material.cu:90-124 */
     float axialStrain = (float)(pos2.x / restLen);
     m->strain[l] = axialStrain;
-    if (axialStrain > m->maxStrain[l]) m->maxStrain[l] = axialStrain;
     float stress;
-    if (mat->epsFail != -1.0f && axialStrain > mat->epsFail) stress = 0.0f;
-    else if (mat->nu == 0.0f) stress = mat->E * axialStrain;
-    else stress = mat->eHat * ((1 - mat->nu) * axialStrain + mat->nu * m->tStrainSum[l]);
+    if (mat->linear) {
+        if (axialStrain > m->maxStrain[l]) m->maxStrain[l] = axialStrain;
+        stress = om_stress(mat, axialStrain, m->tStrainSum[l], 0);
+    } else if (axialStrain > m->maxStrain[l]) {  /* new territory on the stress-strain curve */
+        m->maxStrain[l] = axialStrain;
+        stress = om_stress(mat, axialStrain, m->tStrainSum[l], 0);
+        if (mat->nu != 0.0f)
+            m->strainOffset[l] = m->maxStrain[l] - om_stress(mat, axialStrain, 0.0f, 0) / (mat->eHat * (1 - mat->nu));
+        else
+            m->strainOffset[l] = m->maxStrain[l] - stress / mat->E;
+    } else {  /* backed off: linear with the plastic strain offset */
+        float rel = axialStrain - m->strainOffset[l];
+        stress = mat->nu != 0.0f ? om_stress(mat, rel, m->tStrainSum[l], 1) : mat->E * rel;
+    }
 
     if (mat->epsFail != -1.0f && m->maxStrain[l] > mat->epsFail) {
         m->fNeg[l] = m->fPos[l] = m->mNeg[l] = m->mPos[l] = v3_(0, 0, 0);

@@ -36,7 +36,9 @@ typedef struct {
     float nu;           /* Poissons_Ratio (only used with volume effects) */
     float cte;          /* CTE, 1/degC */
     float mu_s, mu_k;   /* uStatic, uDynamic */
-    float fail_stress;  /* -1 = no failure (MatModel 0) */
+    float fail_stress;  /* -1 = no failure */
+    float plastic_mod, yield_stress; /* Plastic_Mod, Yield_Stress (model 2) */
+    int model;          /* MatModel: 0 linear, 1 linear with failure, 2 bilinear */
     int fixed;          /* Fixed */
     int is_measured;    /* isMeasured */
     double thermal_on_after; /* TurnOnThermalExpansionAfterThisManySeconds */
@@ -94,6 +96,9 @@ void vxo_update_links(vxo_model *m, double *out);
  * fixed degrees of freedom; force (N) and moment (N m).  NULL = zeros. */
 void vxo_set_external(vxo_model *m, int voxel, int dof, const double *t3, const double *r3, const float *f3,
                       const float *m3);
+
+/* stress-strain curve of a non-linear link material (n points, 0 when linear) */
+int vxo_get_link_curve(const vxo_model *m, int link, float *strain8, float *stress8);
 
 /* state access (voxel order) */
 void vxo_get_voxels(const vxo_model *m, double *pos3, double *quat4, double *linmom3, double *angmom3);

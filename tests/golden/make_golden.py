@@ -34,8 +34,10 @@ def driver_input(prob, dt, nsteps, every):
              f"{prob['slow_damp_z']!r} {prob['grav_acc']!r} {int(prob['floor_enabled'])} "
              f"{prob['temp_amplitude']!r} {prob['temp_period']!r} {int(prob['vary_temp'])} "
              f"{int(prob['enable_expansion'])} 0 {len(mats)}"]
-    for m in mats:
-        lines.append(f"{m['E']!r} {m['rho']!r} {m['cte']!r} {m['mu_s']!r} {m['mu_k']!r}")
+    for m0 in mats:
+        m = dict(O.MATERIAL_DEFAULTS, **m0)
+        lines.append(f"{m['E']!r} {m['rho']!r} {m['cte']!r} {m['mu_s']!r} {m['mu_k']!r} {O.material_model(m)} "
+                     f"{m['plastic_mod']!r} {m['yield_stress']!r} {m['fail_stress']!r}")
     lines.append(" ".join(str(int(c)) for c in cells))
     lines.append(" ".join(repr(float(p)) for p in phase))
     lines.append(f"{float(np.float32(dt)):.9g} {nsteps} {every}")
@@ -85,10 +87,24 @@ def beam_with_externals():
     return p
 
 
+def bilinear_robot():
+    """A 5x4x3 robot of a bilinear material (yield strain 0.002, plastic modulus
+    E/5) and a linear one, actuated so that links yield, unload and reload;
+    floor off (the reference CPU path has no static-friction angMom reset)."""
+    rng = np.random.default_rng(11)
+    p = configs.random_robot(rng, (5, 4, 3), 0.9, enable_collision=0)
+    p.update(floor_enabled=0, grav_acc=0.0, temp_amplitude=39.0, vary_temp=1, temp_period=0.004,
+             phase=rng.uniform(0, 1, int((np.asarray(p["cells"]) > 0).sum())),
+             materials=[dict(E=1e6, rho=1000.0, cte=0.001, yield_stress=2000.0, plastic_mod=2e5),
+                        dict(E=2e6, rho=1000.0, cte=0.001)])
+    return p
+
+
 def golden_problems():
     rng = np.random.default_rng(7)
     robot = configs.random_robot(rng, (5, 4, 3), 0.8, enable_collision=0)
-    return {"config0": configs.config0(), "robot5x4x3": robot, "beam_bc": beam_with_externals()}
+    return {"config0": configs.config0(), "robot5x4x3": robot, "beam_bc": beam_with_externals(),
+            "bilinear": bilinear_robot()}
 
 
 def main():

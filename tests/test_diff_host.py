@@ -190,8 +190,8 @@ def host_adjoint(L, prob, phase, E, nsteps, dt, K, gpos):
     marr = (vx3mi._Material * len(mats))()
     for i, m in enumerate(mats):
         r, g, b, a = m["rgba"]
-        marr[i] = vx3mi._Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"], -1.0, 0, int(m["fixed"]),
-                                  int(m["is_measured"]), 0, 0.0, r, g, b, a)
+        marr[i] = vx3mi._Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"], -1.0, 0.0, 0.0, 0,
+                                  int(m["fixed"]), int(m["is_measured"]), 0, 0.0, r, g, b, a)
     spec = vx3mi._Lattice(nx, ny, nz, cells.ctypes.data_as(C.POINTER(C.c_int8)), None, None, len(mats), marr,
                           p["lattice_dim"], p["dt_frac"], p["bond_damp_z"], p["col_damp_z"], p["slow_damp_z"], 0, 0, 0,
                           p["grav_acc"], int(p["floor_enabled"]), int(p["temp_enabled"]), int(p["vary_temp"]),

@@ -294,11 +294,12 @@ def _floor_cube_with_externals():
 
 
 def _external_problems():
+    """Voxel externals (two robots), a bilinear-material robot and a plain cube."""
     import os
     import sys
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
-    from make_golden import beam_with_externals
-    return [beam_with_externals(), _floor_cube_with_externals(), configs.config0()]
+    from make_golden import beam_with_externals, bilinear_robot
+    return [beam_with_externals(), _floor_cube_with_externals(), bilinear_robot(), configs.config0()]
 
 
 @pytest.mark.parametrize("mode,tile,threads", [(vx3mi.MODE_PLAIN, 0, 0), (vx3mi.MODE_FLAT, 0, 0),
@@ -306,8 +307,9 @@ def _external_problems():
                                                (vx3mi.MODE_TILED, 1, 0)])
 def test_externals_fp64_bit_exact_every_mode(mode, tile, threads):
     """Voxel externals (Boundary_Conditions: fixed / displaced / rotated degrees
-    of freedom, forces, moments) in every launch mode, FP64 against the oracle
-    bit for bit, robots with and without externals in one batch."""
+    of freedom, forces, moments) and a bilinear material (yield, unload,
+    reload) in every launch mode, FP64 against the oracle bit for bit, in one
+    batch with a plain robot."""
     log(f"externals mode {mode} tile {tile} threads {threads}")
     probs = _external_problems()
     b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
@@ -330,7 +332,8 @@ def test_externals_fp64_bit_exact_every_mode(mode, tile, threads):
 
 
 def test_externals_fp32_tiled():
-    """FP32 tiled kernel with externals within 1e-5 of the oracle over 1000 steps."""
+    """FP32 tiled kernel with externals and a bilinear material within 1e-5 of
+    the oracle over 1000 steps."""
     probs = _external_problems()
     b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP32)
     b.configure(vx3mi.MODE_TILED, 0, 0)

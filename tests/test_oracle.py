@@ -38,7 +38,7 @@ def quiet(dims, cells=None, **kw):
     return p
 
 
-@pytest.mark.parametrize("name", ["config0", "robot5x4x3", "beam_bc"])
+@pytest.mark.parametrize("name", ["config0", "robot5x4x3", "beam_bc", "bilinear"])
 def test_golden_reference_cpu(name):
     """Oracle (CVoxelyze integrator semantics) vs the reference CPU Voxelyze built
     from src/old: positions and orientations every 50 steps for 600 steps."""
@@ -58,6 +58,21 @@ def test_golden_reference_cpu(name):
         # ~1e-16 through step 500; a floor-friction switch near step 550 then
         # amplifies the last-ulp difference between glibc's sin and the oracle's
         assert e_pos <= 1e-11 and e_q <= 1e-11, (k, e_pos, e_q)
+
+
+def test_bilinear_golden_exercises_plasticity():
+    """The bilinear fixture's links go past the yield strain: the same robot with
+    a linear first material ends elsewhere."""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_golden import golden_problems
+
+    prob = golden_problems()["bilinear"]
+    g = np.load(os.path.join(GOLDEN, "ref_cpu_bilinear.npz"))
+    lin = dict(prob, materials=[dict(prob["materials"][0], yield_stress=0.0), prob["materials"][1]])
+    m = O.OracleModel(lin)
+    m.step(600, float(g["dt"]), O.VXO_CPU_SEMANTICS)
+    assert np.max(np.abs(m.state()["pos"] - g["pos"][-1])) > 1e-5
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "vx3_ref_driver")),

@@ -349,3 +349,51 @@ def test_vxa_regions_equal_explicit_externals():
     for v in ea:
         for k in ("dof", "translation", "rotation", "force", "moment"):
             assert np.array_equal(ea[v][k], eb[v][k]), (v, k)
+
+
+# ---- material models -------------------------------------------------------------
+def _bilinear_problem():
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from make_golden import bilinear_robot
+    return bilinear_robot()
+
+
+def test_bilinear_material_from_vxa_and_oracle_curves():
+    """MatModel 2 read from a VXA gives the same link materials and stress-strain
+    curves as the lattice description, and the curves equal the oracle's (which
+    is pinned to the reference CPU Voxelyze by the bilinear golden)."""
+    import oracle as O
+    prob = _bilinear_problem()
+    text = W.write_vxa(prob)
+    assert "<MatModel>2</MatModel>" in text and "<Yield_Stress>2000.0</Yield_Stress>" in text
+    a, b, o = vx3mi.Model.from_problem(prob), vx3mi.Model.from_xml(text), O.OracleModel(prob)
+    n_nonlinear = 0
+    for l in range(a.nl):
+        ca = a.link_curve(l)
+        assert np.array_equal(ca, b.link_curve(l)) and np.array_equal(ca, o.link_curve(l)), l
+        assert np.array_equal(a.link_constants(l), b.link_constants(l))
+        n_nonlinear += len(ca) > 0
+        if len(ca):
+            assert ca[0, 0] == 0 and np.all(np.diff(ca[:, 0]) > 0) and np.all(np.diff(ca[:, 1]) > 0)
+    assert 0 < n_nonlinear < a.nl  # links between two linear voxels stay linear
+
+
+def test_rejected_material_models_keep_the_default():
+    """CVX_Sim::CopyMat: a material setter that rejects its arguments leaves
+    addMaterial's default (linear, 1e6 Pa): MatModel 1 with a zero failure
+    stress, MatModel 2 without a failure stress above the yield stress."""
+    base = configs.config0()
+    soft = dict(configs.SOFT, E=5e6)
+    penK = lambda m: m.voxel_constants(0)[8]  # noqa: E731  penetrationStiffness = 2 E L
+    good = vx3mi.Model.from_xml(W.write_vxa(dict(base, materials=[soft])))
+    assert penK(good) == np.float32(2 * 5e6 * 0.01)
+    t1 = W.write_vxa(dict(base, materials=[dict(soft, fail_stress=1e4)])).replace(
+        "<Fail_Stress>10000.0</Fail_Stress>", "<Fail_Stress>0</Fail_Stress>")
+    t2 = W.write_vxa(dict(base, materials=[dict(soft, yield_stress=2e3, plastic_mod=1e5)])).replace(
+        "<Fail_Stress>-1.0</Fail_Stress>", "")
+    for t in (t1, t2):
+        assert penK(vx3mi.Model.from_xml(t)) == np.float32(2 * 1e6 * 0.01)
+    ok = W.write_vxa(dict(base, materials=[dict(soft, yield_stress=2e3, plastic_mod=1e5)]))
+    m = vx3mi.Model.from_xml(ok)
+    assert penK(m) == np.float32(2 * 5e6 * 0.01) and len(m.link_curve(0)) == 3

@@ -44,6 +44,7 @@ template <typename T> struct Dev {
     const VoxelMat *vmats;
     const VoxelExt *exts;  // voxel externals (VoxelMat::ext indexes it)
     const LinkMat *lmats;
+    const CurvePt *curves; // stress-strain curves of non-linear link materials
     const RobotParams *rp;
     RobotTime *rt;
     const T4<T> *contact;  // per-voxel contact force, or nullptr without collisions
@@ -76,7 +77,7 @@ template <typename T> __device__ __forceinline__ void link_update(const Dev<T> &
     st.maxStrain = (float)s2.y;
     st.flags = (int)s2.z;
     LinkOut<T> o;
-    link_forces<T>(lm, axis, du, qn, qp, (float)n1.w, (float)p1.w, rt->prevDt[parity], st, o);
+    link_forces<T>(lm, a.curves, axis, du, qn, qp, (float)n1.w, (float)p1.w, rt->prevDt[parity], st, o);
     a.ls0[l] = T4<T>{st.pos2.x, st.pos2.y, st.pos2.z, st.a1v.x};
     a.ls1[l] = T4<T>{st.a1v.y, st.a1v.z, st.a2v.x, st.a2v.y};
     a.ls2[l] = T4<T>{st.a2v.z, (T)st.maxStrain, (T)st.flags, (T)o.strain};
@@ -386,6 +387,7 @@ template <typename T> struct Batch : BatchBase {
     RobotParams *d_rp = nullptr;
     VoxelMat *d_vmats = nullptr;
     VoxelExt *d_exts = nullptr;
+    CurvePt *d_curves = nullptr;
     LinkMat *d_lmats = nullptr;
     float *d_vmatE = nullptr;
     double *d_out = nullptr;
@@ -398,8 +400,9 @@ template <typename T> struct Batch : BatchBase {
         std::vector<LinkMat> lmats;
         std::vector<float> vmatE;
         std::vector<VoxelExt> exts;
+        std::vector<CurvePt> curves;
         std::vector<T4<T>> pq0, pq1;
-        std::map<std::string, int> vmIndex, lmIndex;
+        std::map<std::string, int> vmIndex, lmIndex, curveIndex;
         // materials are deduplicated across robots (byte-identical constants);
         // a voxel with an external gets an entry of its own material whose
         // ext field points at the external
@@ -433,11 +436,22 @@ template <typename T> struct Batch : BatchBase {
             const int extOff = (int)exts.size();
             exts.insert(exts.end(), m.exts.begin(), m.exts.end());
             for (size_t k = 0; k < m.lmats.size(); k++) {
-                std::string key((const char *)&m.lmats[k], sizeof(LinkMat));
+                LinkMat lm = m.lmats[k];
+                if (lm.curve) {  // stress-strain curves are shared by content too
+                    const int first = curve_first(lm), np = curve_points(lm);
+                    std::string ck((const char *)&m.curves[first], sizeof(CurvePt) * np);
+                    auto ci = curveIndex.find(ck);
+                    if (ci == curveIndex.end()) {
+                        ci = curveIndex.emplace(ck, (int)curves.size()).first;
+                        curves.insert(curves.end(), m.curves.begin() + first, m.curves.begin() + first + np);
+                    }
+                    lm.curve = (ci->second << 4) | np;
+                }
+                std::string key((const char *)&lm, sizeof(LinkMat));
                 auto it = lmIndex.find(key);
                 if (it == lmIndex.end()) {
                     it = lmIndex.emplace(key, (int)lmats.size()).first;
-                    lmats.push_back(m.lmats[k]);
+                    lmats.push_back(lm);
                 }
                 lmRemap[k] = it->second;
             }
@@ -486,7 +500,7 @@ template <typename T> struct Batch : BatchBase {
             (rc = alloc(&vlinksd, (size_t)6 * V)) || (rc = alloc(&lendsd, L)) || (rc = alloc(&lmetad, L)) ||
             (rc = alloc(&ls0, L)) || (rc = alloc(&ls1, L)) || (rc = alloc(&ls2, L)) || (rc = alloc(&lf0, L)) ||
             (rc = alloc(&lf1, L)) || (rc = alloc(&lf2, L)) || (rc = alloc(&d_vmats, hvmats.size())) ||
-            (rc = alloc(&d_exts, exts.size())) ||
+            (rc = alloc(&d_exts, exts.size())) || (rc = alloc(&d_curves, curves.size())) ||
             (rc = alloc(&d_lmats, lmats.size())) || (rc = alloc(&d_vmatE, vmatE.size())) || (rc = alloc(&d_rp, R)) ||
             (rc = alloc(&d_rt, R)) || (rc = alloc(&d_out, (size_t)3 * R)))
             return rc;
@@ -521,6 +535,8 @@ template <typename T> struct Batch : BatchBase {
         HIP_TRY(hipMemcpy(d_vmats, hvmats.data(), sizeof(VoxelMat) * hvmats.size(), hipMemcpyHostToDevice));
         if (!exts.empty())
             HIP_TRY(hipMemcpy(d_exts, exts.data(), sizeof(VoxelExt) * exts.size(), hipMemcpyHostToDevice));
+        if (!curves.empty())
+            HIP_TRY(hipMemcpy(d_curves, curves.data(), sizeof(CurvePt) * curves.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d_vmatE, vmatE.data(), sizeof(float) * vmatE.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d_rp, hrp.data(), sizeof(RobotParams) * R, hipMemcpyHostToDevice));
         std::vector<RobotTime> rt((size_t)R);
@@ -582,7 +598,7 @@ template <typename T> struct Batch : BatchBase {
         d.lat = latd; d.vmat = vmatd; d.vrob = vrobd; d.phase = phased; d.vlinks = vlinksd;
         d.lends = lendsd; d.lmeta = lmetad;
         d.ls0 = ls0; d.ls1 = ls1; d.ls2 = ls2; d.lf0 = lf0; d.lf1 = lf1; d.lf2 = lf2;
-        d.vmats = d_vmats; d.exts = d_exts; d.lmats = d_lmats; d.rp = d_rp; d.rt = d_rt;
+        d.vmats = d_vmats; d.exts = d_exts; d.lmats = d_lmats; d.curves = d_curves; d.rp = d_rp; d.rt = d_rt;
         return plan_tiles();
     }
 

@@ -177,6 +177,10 @@ template <typename T> struct DiffBatch : DiffBase {
                     set_error("differentiable mode: volume effects (Poisson's ratio) are not supported");
                     return VX3MI_E_ARG;
                 }
+                if (m.lmats[m.lmat[l]].curve) {
+                    set_error("differentiable mode: non-linear (bilinear) materials are not supported");
+                    return VX3MI_E_ARG;
+                }
                 hends.push_back(int2v{p.voff + m.lneg[l], p.voff + m.lpos[l]});
                 haxis.push_back(m.laxis[l]);
                 hnom.push_back(m.lmats[m.lmat[l]].nomSize);

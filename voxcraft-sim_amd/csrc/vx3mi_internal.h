@@ -49,6 +49,7 @@ struct Model {
     // materials
     std::vector<VoxelMat> vmats;
     std::vector<LinkMat> lmats;
+    std::vector<CurvePt> curves;    // stress-strain curves of non-linear link materials (LinkMat::curve)
     std::vector<float> vmatE;       // Young's modulus per voxel material (host only)
     std::vector<vx3mi_material> palette;  // display/metadata of palette entries
     // settings

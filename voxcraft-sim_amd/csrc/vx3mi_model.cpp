@@ -29,7 +29,62 @@ struct MatDerived {
     float mass, massInv, sqrtMass, inertia, inertiaInv, c2M, c2I;
     int fixed, isMeasured;
     double thermalOn;
+    // stress-strain model (CVX_Material strainData / stressData / linear)
+    int linear, npts;
+    float sd[kMaxCurvePoints], td[kMaxCurvePoints];
 };
+
+// CVX_Material::setModelLinear, src/old/VX_Material.cpp:372-401 (arguments
+// already validated)
+void set_linear(MatDerived &d, float E, float fail) {
+    float tf = fail;
+    if (tf == -1) tf = 1000000;
+    const float tfs = tf / E;
+    d.npts = 2;
+    d.sd[0] = 0; d.sd[1] = tfs;
+    d.td[0] = 0; d.td[1] = tf;
+    d.linear = 1;
+    d.E = E;
+    d.sigmaFail = fail;
+    d.epsFail = (fail == -1) ? -1 : tfs;
+}
+
+// CVX_Material::setModelBilinear, src/old/VX_Material.cpp:406-451; false
+// (model unchanged) for the inputs the reference rejects
+bool set_bilinear(MatDerived &d, float E, float plastic, float yield, float fail) {
+    if (E <= 0 || plastic <= 0 || plastic >= E || yield <= 0) return false;
+    if (fail != -1.0f && fail <= yield) return false;
+    const float yieldStrain = yield / E;
+    float tf = fail;
+    if (tf == -1) tf = 3 * yield;
+    const float tM = plastic, tB = yield - tM * yieldStrain, tfs = (tf - tB) / tM;
+    d.npts = 3;
+    d.sd[0] = 0; d.sd[1] = yieldStrain; d.sd[2] = tfs;
+    d.td[0] = 0; d.td[1] = yield; d.td[2] = tf;
+    d.linear = 0;
+    d.E = E;
+    d.sigmaFail = fail;
+    d.epsFail = fail == -1.0f ? -1.0f : tfs;
+    return true;
+}
+
+// CVX_Material::modulus / strain (src/old/VX_Material.cpp:214-243)
+float mat_modulus(const MatDerived &d, float strain) {
+    if (d.epsFail != -1.0f && strain > d.epsFail) return 0.0f;
+    if (strain <= d.sd[1] || d.linear) return d.E;
+    for (int i = 2; i < d.npts; i++)
+        if (strain <= d.sd[i] || i == d.npts - 1) return (d.td[i] - d.td[i - 1]) / (d.sd[i] - d.sd[i - 1]);
+    return 0.0f;
+}
+float mat_strain(const MatDerived &d, float stress) {
+    if (stress <= d.td[1] || d.linear) return stress / d.E;
+    for (int i = 2; i < d.npts; i++)
+        if (stress <= d.td[i] || i == d.npts - 1) {
+            const float perc = (stress - d.td[i - 1]) / (d.td[i] - d.td[i - 1]);
+            return d.sd[i - 1] + perc * (d.sd[i] - d.sd[i - 1]);
+        }
+    return 0.0f;
+}
 
 void derive(MatDerived &d) {
     d.eHat = d.E / ((1 - 2 * d.nu) * (1 + d.nu));
@@ -73,17 +128,52 @@ VoxelMat to_voxel_mat(const MatDerived &d) {
     return v;
 }
 
-// CVX_MaterialLink::updateAll / updateDerived, src/old/VX_MaterialLink.cpp:45-141
-LinkMat combine(const MatDerived &a, const MatDerived &b) {
+// CVX_MaterialLink::updateAll / updateDerived, src/old/VX_MaterialLink.cpp:45-141.
+// A pair with a non-linear material gets the springs-in-series curve of the two
+// (its points in curve; the caller sets LinkMat::curve).
+LinkMat combine(const MatDerived &a, const MatDerived &b, std::vector<CurvePt> &curve) {
     LinkMat l;
     std::memset(&l, 0, sizeof(l));
+    curve.clear();
     l.nomSize = 0.5 * (a.nomSize + b.nomSize);
     float f1 = a.sigmaFail, f2 = b.sigmaFail, sf;
     if (f1 == -1.0f) sf = f2;
     else if (f2 == -1.0f) sf = f1;
     else sf = f1 < f2 ? f1 : f2;
-    l.E = 2.0f * a.E * b.E / (a.E + b.E);
-    l.epsFail = (sf == -1.0f) ? -1.0f : sf / l.E;
+    if (a.linear && b.linear) {
+        l.E = 2.0f * a.E * b.E / (a.E + b.E);
+        l.epsFail = (sf == -1.0f) ? -1.0f : sf / l.E;
+    } else {
+        // step up through the union of both curves' strain points, each new
+        // segment at the series modulus of the two materials just below it
+        MatDerived c;
+        std::memset(&c, 0, sizeof(c));
+        std::vector<float> st(1, 0.0f), ss(1, 0.0f);
+        int i1 = 1, i2 = 1;
+        while (i1 < a.npts && i2 < b.npts) {
+            float e = FLT_MAX;
+            if (i1 < a.npts) e = a.sd[i1];
+            if (i2 < b.npts && b.sd[i2] < e) e = b.sd[i2];
+            if (e == a.sd[i1]) i1++;
+            if (e == b.sd[i2]) i2++;
+            const float m1 = mat_modulus(a, e - FLT_EPSILON), m2 = mat_modulus(b, e - FLT_EPSILON);
+            const float m = 2.0f * m1 * m2 / (m1 + m2);
+            const size_t last = st.size() - 1;
+            ss.push_back(ss[last] + m * (e - st[last]));
+            st.push_back(e);
+        }
+        // CVX_Material::setModel (VX_Material.cpp:300-370) on these points: the
+        // leading (0, 0) is implied; E is the first slope
+        const int n = (int)st.size();
+        c.npts = std::min(n, kMaxCurvePoints);  // two bilinear materials give at most 5 points
+        for (int k = 0; k < c.npts; k++) { c.sd[k] = st[k]; c.td[k] = ss[k]; }
+        c.E = c.td[1] / c.sd[1];
+        c.linear = (n - 1 == 1);
+        l.E = c.E;
+        l.epsFail = (sf == -1.0f) ? -1.0f : mat_strain(c, sf);
+        if (!c.linear)
+            for (int k = 0; k < c.npts; k++) curve.push_back(CurvePt{c.sd[k], c.td[k]});
+    }
     if (a.nu == 0 && b.nu == 0) l.nu = 0;
     else {
         float tmpEHat = 2 * a.eHat * b.eHat / (a.eHat + b.eHat);
@@ -182,10 +272,20 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
         MatDerived &d = pal[k];
         std::memset(&d, 0, sizeof(d));
         d.nomSize = sp->lattice_dim;
-        d.E = in.elastic_mod;
-        bool withFail = in.mat_model == 1 && in.fail_stress > 0;
-        d.sigmaFail = withFail ? in.fail_stress : -1.0f;
-        d.epsFail = withFail ? in.fail_stress / in.elastic_mod : -1.0f;
+        // CVX_Sim::CopyMat (VX_Sim.cpp:390-405): the model of MatModel; inputs
+        // the setter rejects leave the addMaterial default (linear, 1e6 Pa)
+        set_linear(d, 1e6f, -1.0f);
+        const float E = in.elastic_mod, fail = in.fail_stress;
+        if (in.mat_model == 0) {
+            if (E > 0) set_linear(d, E, -1.0f);
+        } else if (in.mat_model == 1) {
+            if (E > 0 && (fail == -1.0f || fail > 0)) set_linear(d, E, fail);
+        } else if (in.mat_model == 2) {
+            set_bilinear(d, E, in.plastic_mod, in.yield_stress, fail);
+        } else {
+            set_error("material model must be 0 (linear), 1 (linear with failure) or 2 (bilinear)");
+            return VX3MI_E_ARG;
+        }
         d.nu = sp->volume_effects ? clamp_nu(in.poissons_ratio) : 0.0f;
         d.rho = in.density <= 0 ? FLT_MIN : in.density;
         d.alpha = in.cte;
@@ -213,8 +313,11 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
         int mi;
         if (sp->voxel_E) {
             MatDerived d = pal[c - 1];
-            d.E = sp->voxel_E[nv];
-            d.epsFail = (d.sigmaFail == -1.0f) ? -1.0f : d.sigmaFail / d.E;
+            if (!d.linear) {
+                set_error("per-voxel Young's modulus needs linear materials");
+                return VX3MI_E_ARG;
+            }
+            set_linear(d, sp->voxel_E[nv], d.sigmaFail);
             derive(d);
             mi = (int)vm.size();
             vm.push_back(d);
@@ -250,7 +353,12 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
             int lmi;
             if (it == lmatIndex.end()) {
                 // combinedMaterial(voxel1 = new voxel, voxel2 = neighbour); the result is symmetric
-                LinkMat l = combine(vm[m.vmat[v]], vm[m.vmat[n]]);
+                std::vector<CurvePt> curve;
+                LinkMat l = combine(vm[m.vmat[v]], vm[m.vmat[n]], curve);
+                if (!curve.empty()) {
+                    l.curve = ((int)m.curves.size() << 4) | (int)curve.size();
+                    m.curves.insert(m.curves.end(), curve.begin(), curve.end());
+                }
                 const MatDerived &dn = vm[m.vmat[n]], &dp = vm[m.vmat[v]];
                 l.nomN = dn.nomSize; l.nomP = dp.nomSize;
                 l.alphaN = dn.alpha; l.alphaP = dp.alpha;
@@ -416,6 +524,20 @@ int vx3mi_model_link_constants(const vx3mi_model *m, int link, float *o) {
     const float vals[11] = {l.E, l.a1, l.a2, l.b1, l.b2, l.b3, l.sqA1, l.sqA2xIp, l.sqB1, l.sqB2xFMp, l.sqB3xIp};
     for (int i = 0; i < 11; i++) o[i] = vals[i];
     return VX3MI_OK;
+}
+
+int vx3mi_model_link_curve(const vx3mi_model *m, int link, float *strain, float *stress) {
+    if (!m || link < 0 || link >= m->m.nl()) {
+        set_error("vx3mi_model_link_curve: bad model or link index");
+        return VX3MI_E_ARG;
+    }
+    const LinkMat &l = m->m.lmats[m->m.lmat[link]];
+    const int n = curve_points(l), first = curve_first(l);
+    for (int k = 0; k < n; k++) {
+        if (strain) strain[k] = m->m.curves[first + k].strain;
+        if (stress) stress[k] = m->m.curves[first + k].stress;
+    }
+    return n;
 }
 
 int vx3mi_model_set_external(vx3mi_model *m, int voxel, int dof, const double *translation3, const double *rotation3,

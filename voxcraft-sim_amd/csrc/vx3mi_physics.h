@@ -105,8 +105,18 @@ struct LinkMat {
     float tArea, tStrainSum; // frozen transverse info (VX3_Link.cu:147-150)
     float alphaN, alphaP;    // endpoint CTEs
     float dmNumN, dmNumP;    // endpoint damping multiplier numerators
-    int fixedBoth, pad;
+    int fixedBoth;
+    int curve;               // stress-strain curve: first point << 4 | points (0: linear material)
 };
+
+// One point of a non-linear stress-strain curve (CVX_Material strainData /
+// stressData); a curve starts at (0, 0).
+struct CurvePt {
+    float strain, stress;
+};
+constexpr int kMaxCurvePoints = 8;
+VX_HD int curve_points(const LinkMat &lm) { return lm.curve & 15; }
+VX_HD int curve_first(const LinkMat &lm) { return lm.curve >> 4; }
 
 // ---- constants --------------------------------------------------------------
 // src/Utils/VX3_Quat3D.h:30-56, src/old/types.h:57-59, VX3_Collision.h:6
@@ -292,15 +302,42 @@ template <typename T> struct LinkOut {
     float strain;
 };
 
+// VX3_Material::stress (VX3_Material.cu:90-124) of a link material with a
+// stress-strain curve c of curve_points(lm) points (at least three: non-linear).
+VX_HD float curve_stress(const LinkMat &lm, const CurvePt *c, float strain, float tss, bool forceLinear) {
+    if (lm.epsFail != -1.0f && strain > lm.epsFail) return 0.0f;
+    if (strain <= c[1].strain || forceLinear) {
+        if (lm.nu == 0.0f) return lm.E * strain;
+        return lm.eHat * ((1 - lm.nu) * strain + lm.nu * tss);
+    }
+    const int n = curve_points(lm);
+    for (int i = 2; i < n; i++) {
+        if (strain <= c[i].strain || i == n - 1) {  // the segment ending at point i (the last one extrapolates)
+            const float perc = (strain - c[i - 1].strain) / (c[i].strain - c[i - 1].strain);
+            const float basic = c[i - 1].stress + perc * (c[i].stress - c[i - 1].stress);
+            if (lm.nu == 0.0f) return basic;
+            const float modulus = (c[i].stress - c[i - 1].stress) / (c[i].strain - c[i - 1].strain);
+            const float modulusHat = modulus / ((1 - 2 * lm.nu) * (1 + lm.nu));
+            const float effStrain = basic / modulus;
+            const float effTss = tss * (effStrain / strain);
+            return modulusHat * ((1 - lm.nu) * effStrain + lm.nu * effTss);
+        }
+    }
+    return 0.0f;
+}
+
 // VX3_Link::updateForces VX3_Link.cu:135-218 with orientLink :90-133,
 // updateStrain :220-260, VX3_Material::stress VX3_Material.cu:90-124.
 // du = position field of pVPos minus that of pVNeg: the global link vector in
 // double, the displacement difference (nominal offset nomSize along the link
 // axis) in float.  tN/tP are the end temperatures (rest length,
-// VX3_Link.cu:80-83).
+// VX3_Link.cu:80-83).  curves: the stress-strain curve table (read only for
+// non-linear materials).  The strain offset of a non-linear link that backed
+// off its curve, maxStrain - stress(maxStrain) / E (:239-242), is a function
+// of maxStrain alone and is recomputed instead of stored.
 template <typename T>
-VX_HD void link_forces(const LinkMat &lm, int axis, V3<T> du, Q4<T> qn, Q4<T> qp, float tN, float tP, float prevDt,
-                       LinkState<T> &st, LinkOut<T> &out) {
+VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T> du, Q4<T> qn, Q4<T> qp, float tN,
+                       float tP, float prevDt, LinkState<T> &st, LinkOut<T> &out) {
     V3<T> oldPos2 = st.pos2, oldA1v = st.a1v, oldA2v = st.a2v;
     int flags = st.flags;
 
@@ -374,11 +411,25 @@ VX_HD void link_forces(const LinkMat &lm, int axis, V3<T> du, Q4<T> qn, Q4<T> qp
 
     float axialStrain = (float)(pos2.x / restLen);
     out.strain = axialStrain;
-    if (axialStrain > st.maxStrain) st.maxStrain = axialStrain;
     float stress;
-    if (lm.epsFail != -1.0f && axialStrain > lm.epsFail) stress = 0.0f;
-    else if (lm.nu == 0.0f) stress = lm.E * axialStrain;
-    else stress = lm.eHat * ((1 - lm.nu) * axialStrain + lm.nu * lm.tStrainSum);
+    if (lm.curve == 0) {  // linear material
+        if (axialStrain > st.maxStrain) st.maxStrain = axialStrain;
+        if (lm.epsFail != -1.0f && axialStrain > lm.epsFail) stress = 0.0f;
+        else if (lm.nu == 0.0f) stress = lm.E * axialStrain;
+        else stress = lm.eHat * ((1 - lm.nu) * axialStrain + lm.nu * lm.tStrainSum);
+    } else {
+        const CurvePt *c = curves + curve_first(lm);
+        if (axialStrain > st.maxStrain) {  // new territory on the curve
+            st.maxStrain = axialStrain;
+            stress = curve_stress(lm, c, axialStrain, lm.tStrainSum, false);
+        } else {  // backed off: linear with the offset of the largest plastic strain
+            const float offset = lm.nu != 0.0f
+                                     ? st.maxStrain - curve_stress(lm, c, st.maxStrain, 0.0f, false) / (lm.eHat * (1 - lm.nu))
+                                     : st.maxStrain - curve_stress(lm, c, st.maxStrain, lm.tStrainSum, false) / lm.E;
+            const float rel = axialStrain - offset;
+            stress = lm.nu != 0.0f ? curve_stress(lm, c, rel, lm.tStrainSum, true) : lm.E * rel;
+        }
+    }
 
     if (lm.epsFail != -1.0f && st.maxStrain > lm.epsFail) {
         out.fN = out.fP = out.mN = out.mP = v3<T>(0, 0, 0);

@@ -193,7 +193,7 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
             const Q4<T> qn = {sPose[3][n], sPose[4][n], sPose[5][n], sPose[6][n]};
             const Q4<T> qp = {sPose[3][p], sPose[4][p], sPose[5][p], sPose[6][p]};
             LinkOut<T> o;
-            link_forces<T>(lm, lMeta[k] & 3, du, qn, qp, (float)sPose[7][n], (float)sPose[7][p], tr.prevDt[parity][j],
+            link_forces<T>(lm, a.curves, lMeta[k] & 3, du, qn, qp, (float)sPose[7][n], (float)sPose[7][p], tr.prevDt[parity][j],
                            ls[k], o);
             sForce[0][ll] = o.fN.x; sForce[1][ll] = o.fN.y; sForce[2][ll] = o.fN.z;
             sForce[3][ll] = o.mN.x; sForce[4][ll] = o.mN.y; sForce[5][ll] = o.mN.z;

@@ -622,10 +622,12 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
             if (const XNode *mech = mn.child("Mechanical")) {
                 int model = 0, iv;
                 load(mech, "MatModel", model);
-                if (model != 0 && model != 1) { err = "only MatModel 0 and 1 are supported"; return VX3MI_E_PARSE; }
+                if (model < 0 || model > 2) { err = "only MatModel 0, 1 and 2 are supported"; return VX3MI_E_PARSE; }
                 m.mat_model = model;
                 load(mech, "Elastic_Mod", m.elastic_mod);
                 load(mech, "Fail_Stress", m.fail_stress);
+                load(mech, "Plastic_Mod", m.plastic_mod);
+                load(mech, "Yield_Stress", m.yield_stress);
                 load(mech, "Density", m.density);
                 load(mech, "Poissons_Ratio", m.poissons_ratio);
                 load(mech, "CTE", m.cte);

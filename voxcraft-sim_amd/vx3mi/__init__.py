@@ -32,6 +32,7 @@ PROBLEM_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp
                         floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
                         temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
+                         plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, is_target=0, thermal_on_after=0.0,
                          rgba=(0.5, 0.5, 0.5, 1.0))
 
@@ -40,10 +41,19 @@ class VX3MIError(RuntimeError):
     pass
 
 
+def material_model(m: dict) -> int:
+    """MatModel of a material dict: 2 (bilinear) with a yield stress, 1 (linear
+    with failure) with a positive failure stress, else 0 (linear)."""
+    if float(m.get("yield_stress", 0.0)) > 0:
+        return 2
+    return 1 if float(m.get("fail_stress", -1.0)) > 0 else 0
+
+
 class _Material(C.Structure):
     _fields_ = [("elastic_mod", C.c_float), ("density", C.c_float), ("poissons_ratio", C.c_float),
                 ("cte", C.c_float), ("u_static", C.c_float), ("u_dynamic", C.c_float),
-                ("fail_stress", C.c_float), ("mat_model", C.c_int), ("fixed", C.c_int),
+                ("fail_stress", C.c_float), ("plastic_mod", C.c_float), ("yield_stress", C.c_float),
+                ("mat_model", C.c_int), ("fixed", C.c_int),
                 ("is_measured", C.c_int), ("is_target", C.c_int), ("thermal_on_after", C.c_double),
                 ("red", C.c_float), ("green", C.c_float), ("blue", C.c_float), ("alpha", C.c_float)]
 
@@ -121,6 +131,7 @@ def lib():
         "vx3mi_model_topology": (C.c_int, [vp, ip, ip, ip, ip]),
         "vx3mi_model_voxel_constants": (C.c_int, [vp, C.c_int, fp]),
         "vx3mi_model_link_constants": (C.c_int, [vp, C.c_int, fp]),
+        "vx3mi_model_link_curve": (C.c_int, [vp, C.c_int, fp, fp]),
         "vx3mi_model_set_external": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp]),
         "vx3mi_model_get_external": (C.c_int, [vp, C.c_int, ip, dp, dp, dp, dp]),
         "vx3mi_batch_create": (C.c_int, [pvp, C.c_int, C.c_int, pvp]),
@@ -196,10 +207,10 @@ class Model:
         marr = (_Material * max(1, len(mats)))()
         for i, m in enumerate(mats):
             r, g, b, a = m["rgba"]
-            fail = float(m["fail_stress"])
-            marr[i] = _Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"], fail,
-                                1 if fail > 0 else 0, int(m["fixed"]), int(m["is_measured"]),
-                                int(m["is_target"]), m["thermal_on_after"], r, g, b, a)
+            marr[i] = _Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"], float(m["fail_stress"]),
+                                float(m["plastic_mod"]), float(m["yield_stress"]), material_model(m),
+                                int(m["fixed"]), int(m["is_measured"]), int(m["is_target"]), m["thermal_on_after"],
+                                r, g, b, a)
         phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
         spec = _Lattice(nx, ny, nz, _ptr(cells, C.c_int8), _ptr(phase, C.c_double), _ptr(E, C.c_float),
@@ -267,6 +278,13 @@ class Model:
         o = np.zeros(11, dtype=np.float32)
         _check(lib().vx3mi_model_link_constants(self._h, int(l), _ptr(o, C.c_float)))
         return o
+
+    def link_curve(self, l: int) -> np.ndarray:
+        """(n, 2) strain / stress points of a non-linear link material (n = 0: linear)."""
+        st, ss = np.zeros(8, dtype=np.float32), np.zeros(8, dtype=np.float32)
+        n = lib().vx3mi_model_link_curve(self._h, int(l), _ptr(st, C.c_float), _ptr(ss, C.c_float))
+        _check(min(n, 0))
+        return np.stack([st[:n], ss[:n]], axis=1)
 
     def set_external(self, voxel: int, dof: int = 0, translation=(0, 0, 0), rotation=(0, 0, 0),
                      force=(0, 0, 0), moment=(0, 0, 0)) -> None:

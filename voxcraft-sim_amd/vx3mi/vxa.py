@@ -17,7 +17,7 @@ from typing import Sequence
 
 import numpy as np
 
-from . import MATERIAL_DEFAULTS, PROBLEM_DEFAULTS
+from . import MATERIAL_DEFAULTS, PROBLEM_DEFAULTS, material_model
 
 COMPRESSIONS = ("ASCII_READABLE", "BASE64")
 
@@ -95,12 +95,14 @@ def _palette(mats: Sequence[dict]) -> str:
     for i, m0 in enumerate(mats):
         m = dict(MATERIAL_DEFAULTS, **m0)
         r, g, b, a = m["rgba"]
-        fail = float(m["fail_stress"])
-        mech = (_el("MatModel", "1" if fail > 0 else "0") + _el("Elastic_Mod", _num(m["E"]))
-                + _el("Fail_Stress", _num(max(fail, 0.0))) + _el("Density", _num(m["rho"]))
+        fail, model = float(m["fail_stress"]), material_model(m)
+        mech = (_el("MatModel", str(model)) + _el("Elastic_Mod", _num(m["E"]))
+                + _el("Fail_Stress", _num(fail if model == 2 else max(fail, 0.0))) + _el("Density", _num(m["rho"]))
                 + _el("Poissons_Ratio", _num(m["nu"])) + _el("CTE", _num(m["cte"]))
                 + _el("uStatic", _num(m["mu_s"])) + _el("uDynamic", _num(m["mu_k"]))
                 + _el("Fixed", str(int(m["fixed"]))) + _el("isMeasured", str(int(m["is_measured"]))))
+        if model == 2:
+            mech += _el("Plastic_Mod", _num(m["plastic_mod"])) + _el("Yield_Stress", _num(m["yield_stress"]))
         if m["thermal_on_after"]:
             mech += _el("TurnOnThermalExpansionAfterThisManySeconds", _num(m["thermal_on_after"]))
         disp = _el("Red", _num(r)) + _el("Green", _num(g)) + _el("Blue", _num(b)) + _el("Alpha", _num(a))
