@@ -62,12 +62,16 @@ def cpu_baseline(problems, nsteps: int):
             lines = [f"{nx} {ny} {nz} {p['lattice_dim']!r} {p['bond_damp_z']!r} {p['col_damp_z']!r} "
                      f"{p['slow_damp_z']!r} {p['grav_acc']!r} {int(p['floor_enabled'])} {p['temp_amplitude']!r} "
                      f"{p['temp_period']!r} {int(p['vary_temp'])} {int(p['enable_expansion'])} 0 {len(m0)}"]
-            for m in m0:
-                lines.append(f"{m['E']!r} {m['rho']!r} {m['cte']!r} {m['mu_s']!r} {m['mu_k']!r}")
+            for mm in m0:
+                m = dict(vx3mi.MATERIAL_DEFAULTS, **mm)
+                lines.append(f"{m['E']!r} {m['rho']!r} {m['cte']!r} {m['mu_s']!r} {m['mu_k']!r} "
+                             f"{vx3mi.material_model(m)} {m['plastic_mod']!r} {m['yield_stress']!r} "
+                             f"{m['fail_stress']!r}")
             lines.append(" ".join(str(int(c)) for c in cells))
             lines.append(" ".join(repr(float(x)) for x in p["phase"]))
             dt = vx3mi.Model.from_problem(p).recommended_dt() * p["dt_frac"]
             lines.append(f"{float(np.float32(dt)):.9g} {nsteps} 0")
+            lines.append("0")  # no externals
             r = subprocess.run([ref_exe], input="\n".join(lines).encode(), capture_output=True, timeout=600)
             if r.returncode != 0:
                 raise RuntimeError(r.stderr.decode())
