@@ -44,6 +44,21 @@ def algorithmic_bytes_per_voxel_step(nv: int, nl: int, fp64: bool = False) -> fl
     return 144.0 + 100.0 * nl / nv
 
 
+def measured_traffic(kernel: str, robots: int):
+    """HBM bytes per launch of the tiled kernel on this workload, from the
+    committed counter summary (FETCH_SIZE + WRITE_SIZE of one launch, calibrated
+    on a copy of known size); None for another kernel or workload.  The tiled
+    kernel reads and writes the batch state once per launch, so the figure does
+    not depend on the steps per launch."""
+    path = os.path.join(ROOT, "profiles", "k_tile_fp32_batched_1024robots_counters.md")
+    if kernel != "k_tile<float, 0>" or robots != 1024 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
@@ -355,7 +370,7 @@ def main() -> int:
         "batched_over_plain": value / plain_value,
         "fp32_rel_err_vs_fp64": err,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, args.robots),
                      "bytes_per_voxel_step": bpv, "kernel": kernel,
                      "steps_per_launch": steps_per_launch},
         "cpu_baseline": cpu,

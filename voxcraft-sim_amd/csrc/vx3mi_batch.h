@@ -13,6 +13,7 @@ namespace vx3mi {
 
 struct RobotParams {
     double tempAmp, tempPeriod;
+    double tempInvPeriod;  // 1 / tempPeriod (the FP32 path multiplies)
     int varyTemp, enableExpansion, collisions, pad;
     int voff, vcount, loff, lcount;
     // collision broadphase: surface voxels [soff, soff+scount) of the surface

@@ -133,8 +133,12 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
         contact = {c.x, c.y, c.z};
     }
     voxel_step<T>(vm, a.exts, dt, a.lat[v], temp, fs, ms, contact, s);
-    if (rp.varyTemp && !(vm.thermalOnAfter > tnext))
-        temp = actuation_temperature<T>(rp.tempAmp, rp.tempPeriod, tnext, (double)a.phase[v], rp.enableExpansion);
+    if (rp.varyTemp && !(vm.thermalOnAfter > tnext)) {
+        if constexpr (std::is_same<T, float>::value)
+            temp = actuation_temperature_f32(rp.tempAmp, rp.tempInvPeriod, tnext, (double)a.phase[v], rp.enableExpansion);
+        else
+            temp = actuation_temperature<T>(rp.tempAmp, rp.tempPeriod, tnext, (double)a.phase[v], rp.enableExpansion);
+    }
     a.pq0[v] = T4<T>{s.u.x, s.u.y, s.u.z, s.q.w};
     a.pq1[v] = T4<T>{s.q.x, s.q.y, s.q.z, (T)temp};
     a.mo0[v] = T4<T>{s.p.x, s.p.y, s.p.z, s.h.x};
@@ -427,6 +431,7 @@ template <typename T> struct Batch : BatchBase {
             std::memset(&p, 0, sizeof(p));
             p.tempAmp = m.tempAmplitude;
             p.tempPeriod = m.tempPeriod;
+            p.tempInvPeriod = 1.0 / m.tempPeriod;
             p.varyTemp = m.varyTemp;
             p.enableExpansion = m.enableExpansion;
             p.collisions = m.enableCollision;

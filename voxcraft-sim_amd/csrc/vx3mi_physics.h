@@ -163,6 +163,39 @@ template <typename T> VX_HD V3<T> qrotinv(Q4<T> q, V3<T> f) {
                  tw * q.z + tx * q.y - ty * q.x + tz * q.w);
 }
 
+// ---- FP32 device arithmetic ------------------------------------------------------
+// The FP32 path is held to the FP64 result by tolerance, not by bits: on the
+// device its quotients and square roots use the hardware reciprocal / square
+// root / reciprocal square root (v_rcp_f32, v_sqrt_f32, v_rsq_f32, about 1 ulp)
+// instead of the IEEE-exact instruction sequences.  FP64 code and host builds
+// keep exact division.
+VX_HD float frcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+VX_HD float fsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_sqrtf(x);
+#else
+    return sqrtf(x);
+#endif
+}
+VX_HD float frsqrt(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rsqf(x);
+#else
+    return 1.0f / sqrtf(x);
+#endif
+}
+// a / b: exact in double, through the reciprocal in float
+template <typename T> VX_HD T qdiv(T a, T b) {
+    if constexpr (std::is_same<T, float>::value) return a * frcp(b);
+    else return a / b;
+}
+
 // ToRotationVector, VX3_Quat3D.h:347-359
 template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q) {
     if constexpr (std::is_same<T, double>::value) {
@@ -178,10 +211,11 @@ template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q) {
         float s2 = q.x * q.x + q.y * q.y + q.z * q.z;  // == 1 - w^2 for a unit quaternion
         if (s2 == 0.0f || q.w <= -1.0f) return v3<T>(0, 0, 0);
         float k;
-        if (s2 < (float)kSLTHRESH_ACOS2SQRT) k = 2.0f * sqrtf(2.0f / (1.0f + q.w));
+        // 2 sqrt((2 - 2w) / (1 - w^2)) = 2 sqrt(2) / sqrt(1 + w)
+        if (s2 < (float)kSLTHRESH_ACOS2SQRT) k = 2.8284271247461903f * frsqrt(1.0f + q.w);
         else {
-            float s = sqrtf(s2);
-            k = 2.0f * atan2f(s, q.w) / s;
+            float s = fsqrt(s2);
+            k = 2.0f * atan2f(s, q.w) * frcp(s);
         }
         return v3<T>(q.x * k, q.y * k, q.z * k);
     }
@@ -192,6 +226,18 @@ template <typename T> VX_HD Q4<T> from_rotvec(V3<T> in) {
     V3<T> th = v3<T>(in.x / 2, in.y / 2, in.z / 2);
     T s, m2 = th.x * th.x + th.y * th.y + th.z * th.z;
     Q4<T> r;
+    if constexpr (std::is_same<T, float>::value) {
+        // half angles below 0.25 rad (every per-step rotation): cos m and
+        // sin(m)/m by their Maclaurin series to m^6 (truncation < 4e-10)
+        if (m2 < 0.0625f) {
+            r.w = 1.0f + m2 * (-0.5f + m2 * (1.0f / 24.0f + m2 * (-1.0f / 720.0f)));
+            s = 1.0f + m2 * (-1.0f / 6.0f + m2 * (1.0f / 120.0f + m2 * (-1.0f / 5040.0f)));
+            r.x = th.x * s;
+            r.y = th.y * s;
+            r.z = th.z * s;
+            return r;
+        }
+    }
     if (m2 * m2 < (T)kDBL_EPSILONx24) {
         r.w = (T)1.0 - (T)0.5 * m2;
         s = (T)1.0 - m2 / (T)6.0;
@@ -211,7 +257,15 @@ template <typename T> VX_HD Q4<T> from_rotvec(V3<T> in) {
 template <typename T> VX_HD Q4<T> from_angle_to_posx(V3<T> rf) {
     Q4<T> q = {1, 0, 0, 0};
     if (rf.x == 0 && rf.y == 0 && rf.z == 0) return q;
-    T yx = rf.y / rf.x, zx = rf.z / rf.x;
+    T yx, zx;
+    if constexpr (std::is_same<T, float>::value) {
+        const float ix = frcp(rf.x);
+        yx = rf.y * ix;
+        zx = rf.z * ix;
+    } else {
+        yx = rf.y / rf.x;
+        zx = rf.z / rf.x;
+    }
     const T sa = (T)kSMALL_ANGLE_RAD;
     if (yx < sa && yx > -sa && zx < sa && zx > -sa) {
         q.x = 0;
@@ -265,18 +319,23 @@ template <typename T> VX_HD V3<T> to_axis_original(V3<T> v, int axis) {
 // ---- temperature (gpu_update_temperature VX3_VoxelyzeKernel.cu:625-650) -----
 // Returns the new temperature of a voxel at time t (the caller skips fixed
 // voxels and voxels whose TurnOnThermalExpansion time has not come).
+// FP32 path: the cycle count t / period + phase is formed in double as
+// t * invPeriod + phase and reduced to [0, 1) before the float sine
+VX_HD float actuation_temperature_f32(double amp, double invPeriod, double t, double phase, int enableExpansion) {
+    double c = t * invPeriod + phase;
+    c -= floor(c);
+    float Tc = (float)amp * sinf((2 * 3.1415926f) * (float)c);
+    if (!enableExpansion && Tc > 0) Tc = 0;
+    return Tc;
+}
+
 template <typename T> VX_HD float actuation_temperature(double amp, double period, double t, double phase, int enableExpansion) {
     if constexpr (std::is_same<T, double>::value) {
         double Tc = amp * vxm_sin((double)(2 * 3.1415926f) * (t / period + phase));
         if (!enableExpansion && Tc > 0) Tc = 0;
         return (float)Tc;
     } else {
-        // the cycle count is reduced to [0,1) in double before the float sine
-        double c = t / period + phase;
-        c -= floor(c);
-        float Tc = (float)amp * sinf((2 * 3.1415926f) * (float)c);
-        if (!enableExpansion && Tc > 0) Tc = 0;
-        return Tc;
+        return actuation_temperature_f32(amp, 1.0 / period, t, phase, enableExpansion);
     }
 }
 
@@ -370,10 +429,10 @@ VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T>
         ext = rel.x + lMinusRest;
     }
 
-    float smallTurn = (float)((mabs(pos2.z) + mabs(pos2.y)) / pos2.x);
+    float smallTurn = (float)qdiv(mabs(pos2.z) + mabs(pos2.y), pos2.x);
     float extendPerc;
     if constexpr (std::is_same<T, double>::value) extendPerc = (float)(mabs((T)1 - pos2.x / restLen));
-    else extendPerc = mabs(ext / restLen);
+    else extendPerc = mabs(ext * frcp(restLen));
     bool smallAngle = (flags & LF_SMALL_ANGLE) != 0;
     if (!smallAngle && smallTurn < kSA_BOND_BEND_RAD && extendPerc < kSA_BOND_EXT_PERC) {
         smallAngle = true;
@@ -393,9 +452,9 @@ VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T>
             pos2 = v3<T>(len(pos2) - restLen, 0, 0);
         } else {
             // |pos2| - restLen = (|pos2|^2 - restLen^2) / (|pos2| + restLen)
-            T l = len(pos2);
+            T l = fsqrt(pos2.x * pos2.x + pos2.y * pos2.y + pos2.z * pos2.z);
             T num = ext * (pos2.x + restLen) + pos2.y * pos2.y + pos2.z * pos2.z;
-            pos2 = v3<T>(num / (l + restLen), 0, 0);
+            pos2 = v3<T>(num * frcp(l + restLen), 0, 0);
         }
     }
     V3<T> a1v = to_rotvec(angle1);
@@ -409,7 +468,7 @@ VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T>
     V3<T> dA1 = (a1v - oldA1v) * (T)0.5;
     V3<T> dA2 = (a2v - oldA2v) * (T)0.5;
 
-    float axialStrain = (float)(pos2.x / restLen);
+    float axialStrain = (float)qdiv(pos2.x, restLen);
     out.strain = axialStrain;
     float stress;
     if (lm.curve == 0) {  // linear material
@@ -446,7 +505,15 @@ VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T>
         const float sqA1 = lm.sqA1, sqA2xIp = lm.sqA2xIp, sqB1 = lm.sqB1, sqB2xFMp = lm.sqB2xFMp, sqB3xIp = lm.sqB3xIp;
         V3<T> posCalc = v3<T>(sqA1 * dPos2.x, sqB1 * dPos2.y - sqB2xFMp * (dA1.z + dA2.z),
                               sqB1 * dPos2.z + sqB2xFMp * (dA1.y + dA2.y));
-        float dmN = lm.dmNumN / prevDt, dmP = lm.dmNumP / prevDt;  // VX3_Voxel.h:206-208
+        float dmN, dmP;  // dampingMultiplier(), VX3_Voxel.h:206-208
+        if constexpr (std::is_same<T, float>::value) {
+            const float ipd = frcp(prevDt);
+            dmN = lm.dmNumN * ipd;
+            dmP = lm.dmNumP * ipd;
+        } else {
+            dmN = lm.dmNumN / prevDt;
+            dmP = lm.dmNumP / prevDt;
+        }
         fN = fN + posCalc * (T)dmN;
         fP = fP - posCalc * (T)dmP;
         T hN = (T)0.5 * dmN, hP = (T)0.5 * dmP;
@@ -549,7 +616,11 @@ VX_HD void voxel_step(const VoxelMat &vm, const VoxelExt *exts, float dtf, int l
             } else {
                 V3<T> hv = v3<T>(vel.x, vel.y, 0);
                 T l = len(hv);
-                V3<T> hn = l > 0 ? v3<T>(hv.x / l, hv.y / l, hv.z / l) : hv;
+                V3<T> hn = hv;
+                if (l > 0) {
+                    if constexpr (std::is_same<T, float>::value) hn = hv * frcp(l);
+                    else hn = v3<T>(hv.x / l, hv.y / l, hv.z / l);
+                }
                 cur = cur - hn * (T)(vm.muK * normalForce);
             }
         } else {
@@ -601,7 +672,7 @@ VX_HD void voxel_step(const VoxelMat &vm, const VoxelExt *exts, float dtf, int l
         // single precision only: keep the orientation a unit quaternion (the
         // FP64 reference never renormalises, VX3_Voxel.cu:221, and drifts by
         // ~1e-16 per step; float would drift by ~1e-7 per step)
-        float n = 1.0f / sqrtf(s.q.w * s.q.w + s.q.x * s.q.x + s.q.y * s.q.y + s.q.z * s.q.z);
+        float n = frsqrt(s.q.w * s.q.w + s.q.x * s.q.x + s.q.y * s.q.y + s.q.z * s.q.z);
         s.q.w *= n; s.q.x *= n; s.q.y *= n; s.q.z *= n;
     }
     // VX3-only: no rotation while held by static floor friction (VX3_Voxel.cu:259-264)

@@ -44,7 +44,7 @@ template <int RMAX> struct TileRobots {
     float dt[RMAX];
     int diverged[RMAX];
     int steps[RMAX];
-    double tempAmp[RMAX], tempPeriod[RMAX];  // actuation of each robot
+    double tempAmp[RMAX], tempPeriod[RMAX];  // actuation of each robot (FP32: 1 / period)
     int tempFlags[RMAX];                     // varyTemp | enableExpansion << 1
 };
 
@@ -98,7 +98,7 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
         tr.diverged[tid] = t.diverged;
         tr.steps[tid] = t.steps;
         tr.tempAmp[tid] = p.tempAmp;
-        tr.tempPeriod[tid] = p.tempPeriod;
+        tr.tempPeriod[tid] = std::is_same<T, float>::value ? p.tempInvPeriod : p.tempPeriod;
         tr.tempFlags[tid] = (p.varyTemp ? 1 : 0) | (p.enableExpansion ? 2 : 0);
     }
     if (LDSMAT) {
@@ -231,8 +231,12 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
             float temp = (float)sPose[7][lv];
             voxel_step<T>(vm, a.exts, dt, vLat[k], temp, fs, ms, V3<T>{0, 0, 0}, vs[k]);
             const int tf = tr.tempFlags[j];
-            if ((tf & 1) && !(vm.thermalOnAfter > tnext))
-                temp = actuation_temperature<T>(tr.tempAmp[j], tr.tempPeriod[j], tnext, (double)vPhase[k], tf >> 1);
+            if ((tf & 1) && !(vm.thermalOnAfter > tnext)) {
+                if constexpr (std::is_same<T, float>::value)
+                    temp = actuation_temperature_f32(tr.tempAmp[j], tr.tempPeriod[j], tnext, (double)vPhase[k], tf >> 1);
+                else
+                    temp = actuation_temperature<T>(tr.tempAmp[j], tr.tempPeriod[j], tnext, (double)vPhase[k], tf >> 1);
+            }
             sPose[0][lv] = vs[k].u.x; sPose[1][lv] = vs[k].u.y; sPose[2][lv] = vs[k].u.z; sPose[3][lv] = vs[k].q.w;
             sPose[4][lv] = vs[k].q.x; sPose[5][lv] = vs[k].q.y; sPose[6][lv] = vs[k].q.z; sPose[7][lv] = (T)temp;
         }
