@@ -127,13 +127,14 @@ def fp32_error_vs_oracle(problems, nsteps: int) -> float:
     return worst
 
 
-def adjoint_cost(problems, nsteps: int, checkpoint_every: int, budget: float, precision=None):
+def adjoint_cost(problems, nsteps: int, checkpoint_every: int, budget: float, precision=None, reverse_threads=0):
     """Forward (with checkpoints) and backward (recompute + reverse sweep) of the
     differentiable mode on the given robots; loss = -(x displacement of each
     robot's centre of mass).  Returns timings and checkpoint sizes."""
     precision = vx3mi.FP32 if precision is None else precision
     models = [vx3mi.Model.from_problem(p) for p in problems]
-    d = vx3mi.Diff(models, precision, checkpoint_every=checkpoint_every, hbm_budget=budget)
+    d = vx3mi.Diff(models, precision, checkpoint_every=checkpoint_every, hbm_budget=budget,
+                   reverse_threads=reverse_threads)
     phase = np.concatenate([np.asarray(p["phase"], dtype=np.float64) for p in problems])
     E = np.concatenate([np.asarray(p["voxel_E"], dtype=np.float64) for p in problems])
     d.set_params(phase, E)
@@ -152,6 +153,7 @@ def adjoint_cost(problems, nsteps: int, checkpoint_every: int, budget: float, pr
             "adjoint_over_forward": info["backward_ms"] / info["forward_ms"],
             "recompute_ms": info["recompute_ms"], "reverse_ms": info["reverse_ms"],
             "checkpoint_every": info["checkpoint_every"], "checkpoint_bytes": info["checkpoint_bytes"],
+            "reverse_threads": reverse_threads or 512,
             "tape_bytes": info["tape_bytes"], "hbm_budget_bytes": budget,
             "forward_voxel_steps_per_s": off[-1] * nsteps / (info["forward_ms"] * 1e-3)}
 
@@ -341,10 +343,19 @@ def main() -> int:
 
     adjoint = None
     if not args.no_adjoint:
-        c3 = adjoint_cost([configs.config3()], 2000, 100, args.adjoint_budget)
+        # the reverse sweep's workgroup size (512 or 1024 threads per robot) is
+        # chosen by timing both on each workload
+        best = {}
+        for name, probs, steps, K in (("c3", [configs.config3()], 2000, 100), ("c4", configs.config4(), 5000, 0)):
+            for rt in (512, 1024):
+                c = adjoint_cost(probs, steps, K, args.adjoint_budget, reverse_threads=rt)
+                log(f"bench: adjoint {name} reverse threads {rt}: forward {c['forward_ms']:.1f} ms, backward "
+                    f"{c['backward_ms']:.1f} ms, ratio {c['adjoint_over_forward']:.2f}")
+                if name not in best or c["backward_ms"] < best[name]["backward_ms"]:
+                    best[name] = c
+        c3, c4 = best["c3"], best["c4"]
         log(f"bench: adjoint 8x8x8 2000 steps K=100: forward {c3['forward_ms']:.1f} ms, backward "
             f"{c3['backward_ms']:.1f} ms, ratio {c3['adjoint_over_forward']:.2f}")
-        c4 = adjoint_cost(configs.config4(), 5000, 0, args.adjoint_budget)
         log(f"bench: batched adjoint 256 robots 5000 steps K={c4['checkpoint_every']}: forward {c4['forward_ms']:.1f} ms, "
             f"backward {c4['backward_ms']:.1f} ms, ratio {c4['adjoint_over_forward']:.2f}")
         adjoint = {"single_8x8x8": c3, "batched_256": c4,

@@ -143,6 +143,20 @@ def test_batched_checkpointed_equals_whole_tape():
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
 
 
+def test_reverse_sweep_width_does_not_change_gradients():
+    """512 and 1024 threads per robot in the reverse sweep give the same bits
+    (each work item is one thread's; the gather order is fixed)."""
+    prob = configs.config3(dims=(5, 4, 4))
+    m = vx3mi.Model.from_problem(prob)
+    out = []
+    for rt in (512, 1024):
+        d = vx3mi.Diff(m, vx3mi.FP32, dt=float(FIX["dt"]), checkpoint_every=17, reverse_threads=rt)
+        d.set_params(np.asarray(prob["phase"]), np.asarray(prob["voxel_E"], np.float64))
+        d.forward(150)
+        out.append(d.backward(grad_pos=np.ones((m.nv, 3))))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+
+
 def test_torch_function_gradients():
     torch = pytest.importorskip("torch")
     from vx3mi.torch_diff import simulate

@@ -16,6 +16,7 @@ void vx3mi_diff_default_opts(vx3mi_diff_opts *o) {
     o->e_max = 1e7;
     o->checkpoint_every = 0;
     o->hbm_budget = 16.0 * 1024 * 1024 * 1024;
+    o->reverse_threads = 0;
 }
 
 int vx3mi_diff_create(vx3mi_model *const *models, int n, int precision, const vx3mi_diff_opts *opts, vx3mi_diff **out) {
@@ -28,6 +29,11 @@ int vx3mi_diff_create(vx3mi_model *const *models, int n, int precision, const vx
     vx3mi_diff_opts o;
     vx3mi_diff_default_opts(&o);
     if (opts) o = *opts;
+    const int rt = o.reverse_threads;
+    if (rt != 0 && !(precision == VX3MI_FP32 ? (rt == 512 || rt == 1024) : rt == 256)) {
+        set_error("reverse_threads must be 0, or 512 / 1024 (FP32), 256 (FP64)");
+        return VX3MI_E_ARG;
+    }
     if (!(o.floor_stiffness > 0) || !(o.floor_width > 0) || !(o.friction_speed > 0) || !(o.e_max > 0)) {
         set_error("floor stiffness, floor width, friction speed and e_max must be positive");
         return VX3MI_E_ARG;

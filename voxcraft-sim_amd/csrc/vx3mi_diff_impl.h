@@ -104,13 +104,14 @@ template <typename T> __global__ __launch_bounds__(256) void k_diff_finish(DD<T>
 // ------------------------------------------------------------------- host
 // threads per robot workgroup: as many as the registers of each kernel allow
 // (forward FP32 84 VGPRs, FP64 ~230; reverse FP32 166, FP64 ~370 with
-// accumulation registers).  The recompute beside the reverse sweep uses fewer
+// accumulation registers; the optional wide FP32 reverse holds 128 with a
+// small spill).  The recompute beside the reverse sweep uses fewer
 // threads so that, in FP32, two of its waves and two reverse waves share a SIMD
 // (2 x 88 + 2 x 168 = 512 VGPRs).  The thread count does not change results:
 // every item is computed by one thread between barriers.
 template <typename T> struct DiffThreads;
-template <> struct DiffThreads<float> { static constexpr int FWD = 1024, RECOMP = 512, REV = 512; };
-template <> struct DiffThreads<double> { static constexpr int FWD = 512, RECOMP = 512, REV = 256; };
+template <> struct DiffThreads<float> { static constexpr int FWD = 1024, RECOMP = 512, REV = 512, REV_WIDE = 1024; };
+template <> struct DiffThreads<double> { static constexpr int FWD = 512, RECOMP = 512, REV = 256, REV_WIDE = 256; };
 
 template <typename T> struct DiffBatch : DiffBase {
     DD<T> d{};
@@ -435,8 +436,12 @@ template <typename T> struct DiffBatch : DiffBase {
             if (recompute) HIP_TRY(hipStreamWaitEvent(stream, segEv[4 * seg + 1], 0));
             HIP_TRY(hipEventRecord(segEv[4 * seg + 2], stream));
             const int gatherFirst = seg != nseg - 1;
-            k_diff_reverse<T, DiffThreads<T>::REV><<<R, DiffThreads<T>::REV, 0, stream>>>(
-                d, d_rp, s0, len, gatherFirst, bufV[seg & 1], bufL[seg & 1]);
+            if (optsIn.reverse_threads == DiffThreads<T>::REV_WIDE && DiffThreads<T>::REV_WIDE != DiffThreads<T>::REV)
+                k_diff_reverse<T, DiffThreads<T>::REV_WIDE><<<R, DiffThreads<T>::REV_WIDE, 0, stream>>>(
+                    d, d_rp, s0, len, gatherFirst, bufV[seg & 1], bufL[seg & 1]);
+            else
+                k_diff_reverse<T, DiffThreads<T>::REV><<<R, DiffThreads<T>::REV, 0, stream>>>(
+                    d, d_rp, s0, len, gatherFirst, bufV[seg & 1], bufL[seg & 1]);
             HIP_TRY(hipEventRecord(segEv[4 * seg + 3], stream));
             if (recompute && seg > 0 && (rc = recompute_seg(seg - 1))) return rc;
         }

@@ -74,7 +74,7 @@ FORMULA_VARS = ("x", "y", "z", "hit", "t", "angle", "targetCloseness", "numClose
 class _DiffOpts(C.Structure):
     _fields_ = [("floor_stiffness", C.c_double), ("floor_width", C.c_double), ("friction_speed", C.c_double),
                 ("dt", C.c_double), ("e_max", C.c_double), ("checkpoint_every", C.c_int),
-                ("hbm_budget", C.c_double)]
+                ("hbm_budget", C.c_double), ("reverse_threads", C.c_int)]
 
 
 class _Result(C.Structure):
@@ -414,7 +414,8 @@ class Diff:
     """Differentiable batch (softplus floor, per-voxel phase and modulus, adjoint).
 
     Options (keyword): floor_stiffness, floor_width, friction_speed, dt, e_max,
-    checkpoint_every, hbm_budget (see vx3mi_diff_opts in include/vx3mi.h).
+    checkpoint_every, hbm_budget, reverse_threads (see vx3mi_diff_opts in
+    include/vx3mi.h).
     """
 
     def __init__(self, models: Sequence[Model] | Model, precision: int = FP32, **opts):
