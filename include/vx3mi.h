@@ -187,6 +187,10 @@ int vx3mi_set_state(vx3mi_batch *b, int robot, const double *pos3, const double 
                     const double *angmom3);
 /* per-voxel flag words (floor / static friction bits as the reference) */
 int vx3mi_get_voxel_flags(vx3mi_batch *b, int robot, int *flags);
+/* per-link flag words (bit 0 local velocity valid, bit 1 small-angle
+ * formulation, VX3_Link.cu:117-131, 172-191) and axial strain of the last step,
+ * link order; either pointer may be NULL */
+int vx3mi_get_link_flags(vx3mi_batch *b, int robot, int *flags, float *strain);
 /* per-robot simulated time (currentTime), diverged flag, step count */
 int vx3mi_batch_time(vx3mi_batch *b, double *out);
 int vx3mi_batch_diverged(vx3mi_batch *b, int *out);

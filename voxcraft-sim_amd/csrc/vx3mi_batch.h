@@ -88,6 +88,7 @@ struct BatchBase {
     virtual int get_state(int robot, double *pos, double *quat, double *lm, double *am) = 0;
     virtual int set_state(int robot, const double *pos, const double *quat, const double *lm, const double *am) = 0;
     virtual int get_flags(int robot, int *flags) = 0;
+    virtual int get_link_flags(int robot, int *flags, float *strain) = 0;
     virtual int eval_links(double *out) = 0;
     virtual int com(double *out) = 0;
     virtual int recommended(double *out) = 0;

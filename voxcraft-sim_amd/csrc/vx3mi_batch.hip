@@ -106,6 +106,12 @@ int vx3mi_get_voxel_flags(vx3mi_batch *b, int robot, int *flags) {
     return b->b->get_flags(robot, flags);
 }
 
+int vx3mi_get_link_flags(vx3mi_batch *b, int robot, int *flags, float *strain) {
+    int rc = check_robot(b, robot);
+    if (rc) return rc;
+    return b->b->get_link_flags(robot, flags, strain);
+}
+
 static int read_rt(vx3mi_batch *b, std::vector<RobotTime> &rt) {
     rt.resize(b->b->R);
     hipError_t e = hipStreamSynchronize(b->b->stream);

@@ -970,6 +970,19 @@ template <typename T> struct Batch : BatchBase {
         return VX3MI_OK;
     }
 
+    int get_link_flags(int robot, int *flags, float *strain) override {
+        int l0 = 0, l1 = L;
+        if (robot >= 0) { l0 = hrp[robot].loff; l1 = l0 + hrp[robot].lcount; }
+        std::vector<T4<T>> s2((size_t)std::max(L, 1));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (L) HIP_TRY(hipMemcpy(s2.data(), d.ls2, sizeof(T4<T>) * L, hipMemcpyDeviceToHost));
+        for (int l = l0; l < l1; l++) {
+            if (flags) flags[l - l0] = (int)s2[l].z;
+            if (strain) strain[l - l0] = (float)s2[l].w;
+        }
+        return VX3MI_OK;
+    }
+
     int eval_links(double *out) override {
         if (L) k_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, 0, L, parity);
         HIP_TRY(hipGetLastError());

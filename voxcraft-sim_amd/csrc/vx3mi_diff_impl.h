@@ -415,6 +415,7 @@ template <typename T> struct DiffBatch : DiffBase {
         // reverse sweep of segment s + 2, the last reader of that tape, is done
         T4<T> *bufV[2] = {tapeV, tapeV2}, *bufL[2] = {tapeL, tapeL2};
         const bool recompute = !(tapeValid && nseg == 1);
+        const bool wide = optsIn.reverse_threads == DiffThreads<T>::REV_WIDE && DiffThreads<T>::REV_WIDE != DiffThreads<T>::REV;
         HIP_TRY(hipStreamWaitEvent(rstream, ev0, 0));
         auto recompute_seg = [&](int seg) -> int {
             const int s0 = seg * K, len = std::min(K, steps - s0);
@@ -423,8 +424,12 @@ template <typename T> struct DiffBatch : DiffBase {
             k_diff_restore<T><<<std::max(nb, 1), 256, 0, rstream>>>(d, ckV, ckL, seg);
             for (int c = 0; c < len; c += chunk) {
                 const int k = std::min(chunk, len - c);
-                k_diff_forward<T, DiffThreads<T>::RECOMP><<<R, DiffThreads<T>::RECOMP, 0, rstream>>>(
-                    d, d_rp, s0 + c, k, K, nullptr, nullptr, bufV[seg & 1], bufL[seg & 1], s0);
+                if (wide)  // a wide reverse fills the compute units: recompute at full width too
+                    k_diff_forward<T, DiffThreads<T>::FWD><<<R, DiffThreads<T>::FWD, 0, rstream>>>(
+                        d, d_rp, s0 + c, k, K, nullptr, nullptr, bufV[seg & 1], bufL[seg & 1], s0);
+                else
+                    k_diff_forward<T, DiffThreads<T>::RECOMP><<<R, DiffThreads<T>::RECOMP, 0, rstream>>>(
+                        d, d_rp, s0 + c, k, K, nullptr, nullptr, bufV[seg & 1], bufL[seg & 1], s0);
             }
             HIP_TRY(hipEventRecord(segEv[4 * seg + 1], rstream));
             return VX3MI_OK;
@@ -436,7 +441,7 @@ template <typename T> struct DiffBatch : DiffBase {
             if (recompute) HIP_TRY(hipStreamWaitEvent(stream, segEv[4 * seg + 1], 0));
             HIP_TRY(hipEventRecord(segEv[4 * seg + 2], stream));
             const int gatherFirst = seg != nseg - 1;
-            if (optsIn.reverse_threads == DiffThreads<T>::REV_WIDE && DiffThreads<T>::REV_WIDE != DiffThreads<T>::REV)
+            if (wide)
                 k_diff_reverse<T, DiffThreads<T>::REV_WIDE><<<R, DiffThreads<T>::REV_WIDE, 0, stream>>>(
                     d, d_rp, s0, len, gatherFirst, bufV[seg & 1], bufL[seg & 1]);
             else

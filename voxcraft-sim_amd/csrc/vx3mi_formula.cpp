@@ -6,7 +6,11 @@
 // unconsumed value, a binary operator the two oldest (left operand = the
 // second of them); MT_END returns the oldest unconsumed value.  Unlike the
 // reference, a token list whose MT_END finds no unconsumed value is reported
-// as malformed instead of returning an uninitialised slot.
+// as malformed instead of returning an uninitialised slot.  sin and cos are
+// the shared vxm_ functions (the device force-field evaluator uses the same).
+//
+// mt_postfix turns the FIFO token list into a postfix program for the
+// bounded-stack evaluator formula_eval (vx3mi_physics.h) used on the device.
 #include <cmath>
 #include <vector>
 
@@ -41,8 +45,8 @@ bool mt_eval(const std::vector<MtToken> &f, const double vars[MV_COUNT], double 
             const double a = val[head++];
             switch (tk.op) {
             case MT_SQRT: r = std::sqrt(a); break;
-            case MT_SIN: r = std::sin(a); break;
-            case MT_COS: r = std::cos(a); break;
+            case MT_SIN: r = vxm_sin(a); break;
+            case MT_COS: r = vxm_cos(a); break;
             case MT_TAN: r = std::tan(a); break;
             case MT_ATAN: r = std::atan(a); break;
             case MT_LOG: r = std::log(a); break;
@@ -77,6 +81,56 @@ bool mt_eval(const std::vector<MtToken> &f, const double vars[MV_COUNT], double 
         val[tail++] = r;
     }
     return false;  // no MT_END
+}
+
+bool mt_postfix(const std::vector<MtToken> &f, std::vector<MtToken> &post) {
+    // run the FIFO machine on expression nodes instead of values, then emit
+    // the tree of the returned value in postfix order
+    struct Node {
+        MtToken tk;
+        int a, b;  // operand nodes (-1 none); a is the left operand
+    };
+    std::vector<Node> nodes;
+    std::vector<int> val;
+    size_t head = 0;
+    int result = -1;
+    for (const MtToken &tk : f) {
+        if (tk.op == MT_END) {
+            if (head >= val.size()) return false;
+            result = val[head];
+            break;
+        }
+        Node nd{tk, -1, -1};
+        if (tk.op == MT_CONST || tk.op == MT_E || tk.op == MT_PI || tk.op == MT_VAR) {
+        } else if ((tk.op >= MT_SQRT && tk.op <= MT_NOT) || tk.op == MT_NORMALCDF) {
+            if (head + 1 > val.size()) return false;
+            nd.a = val[head++];
+        } else if ((tk.op >= MT_ADD && tk.op <= MT_POW) || (tk.op >= MT_GT && tk.op <= MT_OR)) {
+            if (head + 2 > val.size()) return false;
+            nd.b = val[head];
+            nd.a = val[head + 1];
+            head += 2;
+        } else {
+            return false;
+        }
+        val.push_back((int)nodes.size());
+        nodes.push_back(nd);
+    }
+    if (result < 0) return false;
+    post.clear();
+    int depth = 0, maxDepth = 0;
+    std::vector<std::pair<int, int>> todo{{result, 0}};  // (node, operands emitted)
+    while (!todo.empty()) {
+        auto &t = todo.back();
+        const Node &nd = nodes[t.first];
+        if (t.second == 0 && nd.a >= 0) { t.second = 1; todo.push_back({nd.a, 0}); continue; }
+        if (t.second <= 1 && nd.b >= 0) { t.second = 2; todo.push_back({nd.b, 0}); continue; }
+        post.push_back(nd.tk);
+        depth += (nd.a < 0 ? 1 : 0) - (nd.b >= 0 ? 1 : 0);
+        maxDepth = depth > maxDepth ? depth : maxDepth;
+        todo.pop_back();
+    }
+    return maxDepth <= kMaxFormulaStack;
 }
 
 unsigned mt_vars_used(const std::vector<MtToken> &f) {

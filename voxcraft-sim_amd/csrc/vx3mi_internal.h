@@ -14,21 +14,16 @@ void set_error(const std::string &msg);
 
 // Host model: topology, materials and initial state of one robot, in the
 // reference's voxel and link order.
-// MathTree formula of a VXA (src/Utils/VX3_MathTree.h): tokens in reversed
-// breadth-first order of the XML subtree, MT_END last, evaluated as a FIFO of
-// values (an operator consumes the oldest unconsumed values).
-enum MtOp {
-    MT_END, MT_CONST, MT_E, MT_PI, MT_VAR, MT_ADD, MT_SUB, MT_MUL, MT_DIV, MT_POW, MT_SQRT, MT_SIN, MT_COS,
-    MT_TAN, MT_ATAN, MT_LOG, MT_INT, MT_ABS, MT_NOT, MT_GT, MT_LT, MT_AND, MT_OR, MT_NORMALCDF
-};
-// variables in the order of VX3_MathTree::eval's arguments
-enum MtVar { MV_X, MV_Y, MV_Z, MV_HIT, MV_T, MV_ANGLE, MV_CLOSENESS, MV_PAIRS, MV_NUMVOX, MV_COUNT };
-struct MtToken {
-    int op;
-    double value;
-};
+// MathTree formulas (MtOp, MtToken) are declared in vx3mi_physics.h.  A VXA
+// formula is read as tokens in reversed breadth-first order of the XML subtree,
+// MT_END last, evaluated as a FIFO of values (an operator consumes the oldest
+// unconsumed values).
 // false when the token list is malformed (an operator without operands, no MT_END)
 bool mt_eval(const std::vector<MtToken> &f, const double vars[MV_COUNT], double *out);
+// The same formula as a postfix program for formula_eval (operands before
+// their operator, left before right); false when malformed or when evaluating
+// it needs more than kMaxFormulaStack values at once.
+bool mt_postfix(const std::vector<MtToken> &f, std::vector<MtToken> &post);
 // bit v set when variable v occurs in the formula
 unsigned mt_vars_used(const std::vector<MtToken> &f);
 
@@ -60,6 +55,7 @@ struct Model {
     float initialTemp = 0;          // UpdateMatTemps result (src/VXA/VX_Sim.cpp:467-483)
     // settings carried for the drop-in runner (VX3_SimulationManager.cu:323-376)
     std::vector<MtToken> stopFormula, fitnessFormula;
+    std::vector<MtToken> forceField[3];  // Simulator/ForceField/{x,y,z}_forcefield (empty: none)
     int recordStepSize = 0, recordVoxel = 1, recordLink = 0;
     std::string rawPrint;           // VXA.RawPrint
     int savePositions = 0;          // Simulator/SavePositionOfAllVoxels

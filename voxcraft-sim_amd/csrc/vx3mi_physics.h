@@ -163,6 +163,68 @@ template <typename T> VX_HD V3<T> qrotinv(Q4<T> q, V3<T> f) {
                  tw * q.z + tx * q.y - ty * q.x + tz * q.w);
 }
 
+// ---- MathTree formulas (src/Utils/VX3_MathTree.h) ---------------------------------
+enum MtOp {
+    MT_END, MT_CONST, MT_E, MT_PI, MT_VAR, MT_ADD, MT_SUB, MT_MUL, MT_DIV, MT_POW, MT_SQRT, MT_SIN, MT_COS,
+    MT_TAN, MT_ATAN, MT_LOG, MT_INT, MT_ABS, MT_NOT, MT_GT, MT_LT, MT_AND, MT_OR, MT_NORMALCDF
+};
+// variables in the order of VX3_MathTree::eval's arguments
+enum MtVar { MV_X, MV_Y, MV_Z, MV_HIT, MV_T, MV_ANGLE, MV_CLOSENESS, MV_PAIRS, MV_NUMVOX, MV_COUNT };
+struct MtToken {
+    int op;
+    double value;
+};
+constexpr int kMaxFormulaStack = 16;
+
+// A formula compiled to postfix (operands before their operator, left before
+// right) evaluated with a bounded stack instead of VX3_MathTree::eval's
+// 1024-entry value buffer; every operation gets the operands the reference
+// gives it.  sin and cos are the shared vxm_ functions.
+VX_HD double formula_eval(const MtToken *p, int n, const double *vars) {
+    double st[kMaxFormulaStack];
+    int sp = 0;
+    for (int i = 0; i < n; i++) {
+        const int op = p[i].op;
+        if (op == MT_CONST) st[sp++] = p[i].value;
+        else if (op == MT_E) st[sp++] = 2.71828182845904523536;
+        else if (op == MT_PI) st[sp++] = 3.14159265358979323846;
+        else if (op == MT_VAR) st[sp++] = vars[(int)p[i].value];
+        else if ((op >= MT_SQRT && op <= MT_NOT) || op == MT_NORMALCDF) {
+            const double a = st[sp - 1];
+            double r;
+            switch (op) {
+            case MT_SQRT: r = sqrt(a); break;
+            case MT_SIN: r = vxm_sin(a); break;
+            case MT_COS: r = vxm_cos(a); break;
+            case MT_TAN: r = tan(a); break;
+            case MT_ATAN: r = atan(a); break;
+            case MT_LOG: r = log(a); break;
+            case MT_INT: r = rint(a); break;
+            case MT_ABS: r = fabs(a); break;
+            case MT_NOT: r = (a == 0.0) ? 1.0 : 0.0; break;
+            default: r = 0.5 * erfc(-a / sqrt(2.0)); break;  // standard normal CDF
+            }
+            st[sp - 1] = r;
+        } else {
+            const double b = st[--sp], a = st[sp - 1];
+            double r;
+            switch (op) {
+            case MT_ADD: r = a + b; break;
+            case MT_SUB: r = a - b; break;
+            case MT_MUL: r = a * b; break;
+            case MT_DIV: r = a / b; break;
+            case MT_POW: r = pow(a, b); break;
+            case MT_GT: r = a > b; break;
+            case MT_LT: r = a < b; break;
+            case MT_AND: r = (a != 0.0) && (b != 0.0); break;
+            default: r = (a != 0.0) || (b != 0.0); break;
+            }
+            st[sp - 1] = r;
+        }
+    }
+    return sp > 0 ? st[0] : 0.0;
+}
+
 // ---- FP32 device arithmetic ------------------------------------------------------
 // The FP32 path is held to the FP64 result by tolerance, not by bits: on the
 // device its quotients and square roots use the hardware reciprocal / square

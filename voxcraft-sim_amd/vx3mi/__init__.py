@@ -143,6 +143,7 @@ def lib():
         "vx3mi_get_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
         "vx3mi_set_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
         "vx3mi_get_voxel_flags": (C.c_int, [vp, C.c_int, ip]),
+        "vx3mi_get_link_flags": (C.c_int, [vp, C.c_int, ip, fp]),
         "vx3mi_batch_time": (C.c_int, [vp, dp]),
         "vx3mi_batch_diverged": (C.c_int, [vp, ip]),
         "vx3mi_center_of_mass": (C.c_int, [vp, dp]),
@@ -369,6 +370,13 @@ class Batch:
         f = np.zeros(n, dtype=np.int32)
         _check(lib().vx3mi_get_voxel_flags(self._h, int(robot), _ptr(f, C.c_int)))
         return f
+
+    def link_flags(self, robot: int = -1):
+        """(flags, axial strain) per link: bit 0 local velocity valid, bit 1 small angle."""
+        n = self.nl if robot < 0 else self.models[robot].nl
+        f, st = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.float32)
+        _check(lib().vx3mi_get_link_flags(self._h, int(robot), _ptr(f, C.c_int), _ptr(st, C.c_float)))
+        return f, st
 
     def time(self) -> np.ndarray:
         t = np.zeros(self.nrobot)

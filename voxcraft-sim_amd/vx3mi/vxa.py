@@ -59,6 +59,9 @@ def _simulator(p: dict, stop=None, fitness=None, record=None) -> str:
         parts.append(_el("StopCondition", _el("StopConditionFormula", formula_xml(stop))))
     if fitness is not None:
         parts.append(_el("FitnessFunction", formula_xml(fitness)))
+    ff = p.get("force_field") or {}
+    if ff:  # Simulator/ForceField/{x,y,z}_forcefield MathTrees (problem key "force_field": {"x": expr, ...})
+        parts.append(_el("ForceField", "".join(_el(f"{a}_forcefield", formula_xml(ff[a])) for a in "xyz" if a in ff)))
     return _el("Simulator", "".join(parts))
 
 
