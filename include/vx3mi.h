@@ -155,6 +155,15 @@ int vx3mi_model_link_curve(const vx3mi_model *m, int link, float *strain8, float
  * (Environment/Boundary_Conditions) are applied through the same call. */
 int vx3mi_model_set_external(vx3mi_model *m, int voxel, int dof, const double *translation3,
                              const double *rotation3, const double *force3, const double *moment3);
+/* Force field of one axis (0 x, 1 y, 2 z): the MathTree XML a VXA's
+ * Simulator/ForceField/{x,y,z}_forcefield element holds (e.g.
+ * "<mtMUL><mtCONST>0.01</mtCONST><mtVAR>t</mtVAR></mtMUL>"; empty or NULL
+ * removes it), added to every voxel's force as VX3_Voxel::timeStep does
+ * (src/VX3/VX3_Voxel.cu:177-183, parsed as VX3_SimulationManager::ParseMathTree,
+ * VX3_SimulationManager.cu:157-275).  Variables x, y, z, t, num_voxel; hit,
+ * targetCloseness and numClosePairs are 0; formulas using angle are rejected.
+ * Batches with force fields run in the plain and flat modes. */
+int vx3mi_model_set_force_field(vx3mi_model *m, int axis, const char *formula_xml);
 /* the voxel's external (*dof = -1 and zeros when it has none) */
 int vx3mi_model_get_external(const vx3mi_model *m, int voxel, int *dof, double *translation3,
                              double *rotation3, double *force3, double *moment3);

@@ -29,6 +29,14 @@ class _Material(C.Structure):
                 ("fixed", C.c_int), ("is_measured", C.c_int), ("thermal_on_after", C.c_double)]
 
 
+# MathTree token codes (vx3_oracle.h VXO_MT_*) and variable order
+MT_CODES = {name: i for i, name in enumerate(
+    ["mtEND", "mtCONST", "mtE", "mtPI", "mtVAR", "mtADD", "mtSUB", "mtMUL", "mtDIV", "mtPOW", "mtSQRT", "mtSIN",
+     "mtCOS", "mtTAN", "mtATAN", "mtLOG", "mtINT", "mtABS", "mtNOT", "mtGREATERTHAN", "mtLESSTHAN", "mtAND", "mtOR",
+     "mtNORMALCDF"])}
+MT_VARS = ["x", "y", "z", "hit", "t", "angle", "targetCloseness", "numClosePairs", "num_voxel"]
+
+
 def material_model(m: dict) -> int:
     """MatModel of a material dict: 2 (bilinear) with a yield stress, 1 (linear
     with failure) with a positive failure stress, else 0 (linear)."""
@@ -105,6 +113,7 @@ def lib():
         L.vxo_center_of_mass.argtypes = [vp, dp]
         L.vxo_get_contact.argtypes = [vp, dp]
         L.vxo_set_external.argtypes = [vp, C.c_int, C.c_int, dp, dp, fp, fp]
+        L.vxo_set_force_field.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         L.vxo_get_link_curve.restype = C.c_int
         L.vxo_get_link_curve.argtypes = [vp, C.c_int, fp, fp]
         L.vxo_get_voxel_material.argtypes = [vp, C.c_int, fp]
@@ -159,6 +168,32 @@ class OracleModel:
         self.ns = self._L.vxo_num_surface(self._h)
         for e in p.get("externals") or []:
             self.set_external(**e)
+        for axis, expr in (p.get("force_field") or {}).items():
+            self.set_force_field("xyz".index(axis), expr)
+
+    def set_force_field(self, axis: int, expr) -> None:
+        """Force-field formula of one axis as a nested tuple, e.g.
+        ("mtMUL", ("mtCONST", 0.01), ("mtSIN", ("mtVAR", "t"))), linearised in
+        the reference's token order: breadth-first over the formula's subtree,
+        reversed, END last (VX3_SimulationManager::ParseMathTree)."""
+        bfs = [expr]
+        for node in bfs:
+            if node[0] not in ("mtVAR", "mtCONST"):
+                bfs.extend(node[1:])
+        ops, vals = [], []
+        for node in reversed(bfs):
+            ops.append(MT_CODES[node[0]])
+            if node[0] == "mtVAR":
+                vals.append(float(MT_VARS.index(node[1])))
+            elif node[0] == "mtCONST":
+                vals.append(float(node[1]))
+            else:
+                vals.append(0.0)
+        ops.append(MT_CODES["mtEND"])
+        vals.append(0.0)
+        o = np.asarray(ops, dtype=np.int32)
+        v = np.asarray(vals, dtype=np.float64)
+        self._L.vxo_set_force_field(self._h, int(axis), len(o), _ptr(o, C.c_int), _ptr(v, C.c_double))
 
     def link_curve(self, l: int) -> np.ndarray:
         st, ss = np.zeros(8, dtype=np.float32), np.zeros(8, dtype=np.float32)

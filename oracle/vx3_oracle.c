@@ -79,6 +79,9 @@ struct vxo_model {
     /* externals (VX3_External.h:27-31): fixed degrees of freedom (bits X, Y, Z
      * translation, X, Y, Z rotation; -1 = the voxel has no external), prescribed
      * translation and rotation (and its quaternion), force and moment */
+    int ff_n[3];           /* force field tokens per axis (0: none) */
+    int *ff_op[3];
+    double *ff_val[3];
     int *ext_dof;
     v3 *ext_t, *ext_r;
     q4 *ext_q;
@@ -632,9 +635,61 @@ void vxo_free(vxo_model *m) {
                     m->orient, m->temp, m->prevDt, m->vflags, m->phase, m->surface, m->is_surface, m->lneg,
                     m->lpos, m->laxis, m->lm, m->pos2, m->a1v, m->a2v, m->strain, m->maxStrain, m->tArea,
                     m->tStrainSum, m->restLen, m->lflags, m->fNeg, m->fPos, m->mNeg, m->mPos, m->vm, m->strainOffset,
-                    m->ext_dof, m->ext_t, m->ext_r, m->ext_q, m->ext_f, m->ext_m};
+                    m->ext_dof, m->ext_t, m->ext_r, m->ext_q, m->ext_f, m->ext_m,
+                    m->ff_op[0], m->ff_op[1], m->ff_op[2], m->ff_val[0], m->ff_val[1], m->ff_val[2]};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(m);
+}
+
+/* VX3_MathTree::eval (src/Utils/VX3_MathTree.h:50-190): values appended in
+ * token order; an operator consumes the oldest unconsumed values (a binary one
+ * computes values[p + 1] op values[p]); END returns the oldest unconsumed one.
+ * sin and cos are the shared vxm_ functions. */
+static double mt_eval_fifo(const int *op, const double *val, int n, const double *vars) {
+    double v[1024];
+    int pc = 0, vc = 0;
+    for (int i = 0; i < n && vc < 1024; i++) {
+        double r;
+        switch (op[i]) {
+        case VXO_MT_END: return pc < vc ? v[pc] : 0.0;
+        case VXO_MT_CONST: r = val[i]; break;
+        case VXO_MT_E: r = 2.71828182845904523536; break;
+        case VXO_MT_PI: r = 3.14159265358979323846; break;
+        case VXO_MT_VAR: r = vars[(int)val[i]]; break;
+        case VXO_MT_SIN: r = vxm_sin(v[pc++]); break;
+        case VXO_MT_COS: r = vxm_cos(v[pc++]); break;
+        case VXO_MT_TAN: r = tan(v[pc++]); break;
+        case VXO_MT_ATAN: r = atan(v[pc++]); break;
+        case VXO_MT_LOG: r = log(v[pc++]); break;
+        case VXO_MT_INT: r = rint(v[pc++]); break;
+        case VXO_MT_NORMALCDF: r = 0.5 * erfc(-v[pc] / sqrt(2.0)); pc++; break;
+        case VXO_MT_SQRT: r = sqrt(v[pc++]); break;
+        case VXO_MT_ABS: r = fabs(v[pc++]); break;
+        case VXO_MT_NOT: r = !v[pc++]; break;
+        case VXO_MT_ADD: r = v[pc + 1] + v[pc]; pc += 2; break;
+        case VXO_MT_SUB: r = v[pc + 1] - v[pc]; pc += 2; break;
+        case VXO_MT_MUL: r = v[pc + 1] * v[pc]; pc += 2; break;
+        case VXO_MT_DIV: r = v[pc + 1] / v[pc]; pc += 2; break;
+        case VXO_MT_POW: r = pow(v[pc + 1], v[pc]); pc += 2; break;
+        case VXO_MT_GT: r = v[pc + 1] > v[pc]; pc += 2; break;
+        case VXO_MT_LT: r = v[pc + 1] < v[pc]; pc += 2; break;
+        case VXO_MT_AND: r = v[pc + 1] && v[pc]; pc += 2; break;
+        case VXO_MT_OR: r = v[pc + 1] || v[pc]; pc += 2; break;
+        default: return 0.0;
+        }
+        v[vc++] = r;
+    }
+    return 0.0;
+}
+
+void vxo_set_force_field(vxo_model *m, int axis, int n, const int *ops, const double *values) {
+    if (axis < 0 || axis > 2) return;
+    free(m->ff_op[axis]);
+    free(m->ff_val[axis]);
+    m->ff_n[axis] = n > 0 ? n : 0;
+    m->ff_op[axis] = (int *)xcalloc(m->ff_n[axis], sizeof(int));
+    m->ff_val[axis] = (double *)xcalloc(m->ff_n[axis], sizeof(double));
+    for (int i = 0; i < m->ff_n[axis]; i++) { m->ff_op[axis][i] = ops[i]; m->ff_val[axis][i] = values[i]; }
 }
 
 /* CVX_Voxel::external() + CVX_External::setDisplacement / setForce / setMoment
@@ -953,8 +1008,13 @@ static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
     tf.z += -mt->mass * 9.80665f * mt->gravMult;
     tf = vsub(tf, m->contact[v]);
     m->contact[v] = v3_(0, 0, 0);
-    /* force field: constant-zero trees add +0.0 */
-    tf.x += 0.0; tf.y += 0.0; tf.z += 0.0;
+    /* force field (VX3_Voxel.cu:177-183) at the position entering the step and
+     * currentTime; hit, angle, closeness and pairs are 0 (no targets); absent
+     * trees add +0.0 as the reference's constant-zero defaults do */
+    const double vars[9] = {pos.x, pos.y, pos.z, 0.0, m->time, 0.0, 0.0, 0.0, (double)m->nv};
+    tf.x += m->ff_n[0] ? mt_eval_fifo(m->ff_op[0], m->ff_val[0], m->ff_n[0], vars) : 0.0;
+    tf.y += m->ff_n[1] ? mt_eval_fifo(m->ff_op[1], m->ff_val[1], m->ff_n[1], vars) : 0.0;
+    tf.z += m->ff_n[2] ? mt_eval_fifo(m->ff_op[2], m->ff_val[2], m->ff_n[2], vars) : 0.0;
 
     v3 curForce = tf;
     if (flags & VF_FLOOR_ENABLED) {

@@ -97,6 +97,16 @@ void vxo_update_links(vxo_model *m, double *out);
 void vxo_set_external(vxo_model *m, int voxel, int dof, const double *t3, const double *r3, const float *f3,
                       const float *m3);
 
+/* MathTree force field (VXA Simulator/ForceField/{x,y,z}_forcefield): the
+ * tokens in the reference's order (reversed breadth-first order of the XML
+ * subtree, END last) with these op codes (VX3_MathTreeOperator order); a VAR's
+ * value is the variable index (x, y, z, hit, t, angle, targetCloseness,
+ * numClosePairs, num_voxel). */
+enum { VXO_MT_END, VXO_MT_CONST, VXO_MT_E, VXO_MT_PI, VXO_MT_VAR, VXO_MT_ADD, VXO_MT_SUB, VXO_MT_MUL, VXO_MT_DIV,
+       VXO_MT_POW, VXO_MT_SQRT, VXO_MT_SIN, VXO_MT_COS, VXO_MT_TAN, VXO_MT_ATAN, VXO_MT_LOG, VXO_MT_INT, VXO_MT_ABS,
+       VXO_MT_NOT, VXO_MT_GT, VXO_MT_LT, VXO_MT_AND, VXO_MT_OR, VXO_MT_NORMALCDF };
+void vxo_set_force_field(vxo_model *m, int axis, int n, const int *ops, const double *values);
+
 /* stress-strain curve of a non-linear link material (n points, 0 when linear) */
 int vxo_get_link_curve(const vxo_model *m, int link, float *strain8, float *stress8);
 

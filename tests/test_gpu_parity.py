@@ -348,3 +348,57 @@ def test_externals_fp32_tiled():
     b.close()
     log(f"  fp32 externals worst {worst:.3e}")
     assert worst <= 1e-5
+
+
+# force fields: only operations whose results are the same bits on the device
+# and in the oracle (arithmetic, comparisons, sqrt, and the shared sin / cos)
+FORCE_FIELD = {
+    "x": ("mtMUL", ("mtCONST", 0.002), ("mtSIN", ("mtMUL", ("mtCONST", 40.0), ("mtVAR", "t")))),
+    "y": ("mtDIV", ("mtSUB", ("mtVAR", "x"), ("mtVAR", "y")), ("mtCONST", 50.0)),
+    "z": ("mtMUL", ("mtGREATERTHAN", ("mtVAR", "z"), ("mtCONST", 0.02)),
+          ("mtMUL", ("mtCONST", 0.001), ("mtCOS", ("mtSQRT", ("mtVAR", "num_voxel"))))),
+}
+
+
+def _force_field_problems():
+    rng = np.random.default_rng(21)
+    withcol = configs.random_robot(rng, (5, 4, 3), 0.8, enable_collision=1)
+    return [dict(configs.config0(), force_field=FORCE_FIELD), configs.config0(),
+            dict(withcol, force_field={"x": FORCE_FIELD["x"]})]
+
+
+@pytest.mark.parametrize("mode", [vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT, vx3mi.MODE_AUTO])
+def test_force_fields_fp64_bit_exact(mode):
+    """MathTree force fields (Simulator/ForceField) evaluated on the device from
+    postfix programs: FP64 equals the oracle's FIFO evaluation bit for bit, in a
+    batch with a robot without a field and one with collisions."""
+    log(f"force fields mode {mode}")
+    probs = _force_field_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
+    oracles = [O.OracleModel(p) for p in probs]
+    for _ in range(3):
+        b.step(150, mode=mode)
+        for r, o in enumerate(oracles):
+            o.step(150)
+            s, ref = b.state(r), o.state()
+            assert np.array_equal(s["pos"], ref["pos"]) and np.array_equal(s["quat"], ref["quat"]), r
+    b.close()
+    # the field acts: the same cube without it ends elsewhere
+    assert np.abs(oracles[0].state()["pos"] - oracles[1].state()["pos"]).max() > 1e-6
+
+
+def test_force_fields_fp32_and_persistent_refused():
+    probs = _force_field_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP32)
+    with pytest.raises(vx3mi.VX3MIError):
+        b.configure(vx3mi.MODE_PERSISTENT, 0, 256)
+    oracles = [O.OracleModel(p) for p in probs]
+    b.step(450)
+    worst = 0.0
+    for r, o in enumerate(oracles):
+        o.step(450)
+        s, ref = b.state(r), o.state()
+        worst = max(worst, pos_err(s["pos"], ref["pos"]))
+    b.close()
+    log(f"  fp32 force fields worst e_pos {worst:.3e}")
+    assert worst <= 1e-5

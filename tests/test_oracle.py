@@ -246,3 +246,18 @@ def test_cpu_semantics_flag_only_changes_static_friction_rotation():
     b.step(300, flags=O.VXO_CPU_SEMANTICS)
     # only the float vs double dt*massInverse product differs without a floor
     assert np.max(np.abs(a.state()["pos"] - b.state()["pos"])) <= 1e-12
+
+
+def test_force_field_constant_force_and_operand_order():
+    """A force field F_x = 3 - 1 (mtSUB: left minus right) on a free body with
+    no gravity, floor or damping: every voxel follows the exact discrete
+    sequence x_n = x_0 + a dt^2 n(n+1)/2 with a = 2 N / m."""
+    p = quiet((2, 2, 2), force_field={"x": ("mtSUB", ("mtCONST", 3.0), ("mtCONST", 1.0))})
+    m = O.OracleModel(p)
+    dt = m.step_dt()
+    x0 = m.state()["pos"][:, 0].copy()
+    mass_inv = float(np.float32(1.0) / np.float32(1000.0 * 0.01 ** 3))  # the material's float 1/m
+    n = 200
+    m.step(n, dt)
+    expect = x0 + 2.0 * mass_inv * float(dt) ** 2 * n * (n + 1) / 2
+    assert np.allclose(m.state()["pos"][:, 0], expect, rtol=1e-10, atol=0)

@@ -240,8 +240,8 @@ BAD = {
     "bad_token": DEMO_LIKE.replace("<mtSUB>", "<mtFOO>").replace("</mtSUB>", "</mtFOO>"),
     "bad_var": DEMO_LIKE.replace("<mtVAR>t</mtVAR>", "<mtVAR>speed</mtVAR>"),
     "missing_operand": DEMO_LIKE.replace("<mtCONST>1</mtCONST>", ""),
-    "force_field": DEMO_LIKE.replace("<Simulator>", "<Simulator><ForceField><x_forcefield><mtCONST>2</mtCONST>"
-                                                    "</x_forcefield></ForceField>"),
+    "force_field_angle": DEMO_LIKE.replace("<Simulator>", "<Simulator><ForceField><x_forcefield><mtVAR>angle</mtVAR>"
+                                                          "</x_forcefield></ForceField>"),
     "target": DEMO_LIKE.replace("<CTE>0</CTE>", "<CTE>0</CTE><isTarget>1</isTarget>"),
 }
 
@@ -397,3 +397,18 @@ def test_rejected_material_models_keep_the_default():
     ok = W.write_vxa(dict(base, materials=[dict(soft, yield_stress=2e3, plastic_mod=1e5)]))
     m = vx3mi.Model.from_xml(ok)
     assert penK(m) == np.float32(2 * 5e6 * 0.01) and len(m.link_curve(0)) == 3
+
+
+# ---- force fields ----------------------------------------------------------------
+def test_force_fields_are_read_per_axis():
+    """Simulator/ForceField/{x,y,z}_forcefield: a non-zero tree is kept (the
+    reference adds it to each voxel's force), a constant-zero one is the default;
+    a tree deeper than the device evaluator's stack is rejected."""
+    ff = {"x": ("mtMUL", ("mtCONST", 0.01), ("mtSIN", ("mtVAR", "t"))), "z": ("mtCONST", 0.0)}
+    m = vx3mi.Model.from_xml(W.write_vxa(dict(configs.config0(), force_field=ff)))
+    assert m.settings()["nx"] == 4
+    deep = ("mtVAR", "x")
+    for _ in range(20):  # right-leaning chain: each level keeps one more value on the stack
+        deep = ("mtADD", ("mtVAR", "x"), deep)
+    with pytest.raises(vx3mi.VX3MIError, match="error -2"):
+        vx3mi.Model.from_xml(W.write_vxa(dict(configs.config0(), force_field={"y": deep})))

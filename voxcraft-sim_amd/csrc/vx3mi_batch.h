@@ -15,6 +15,7 @@ struct RobotParams {
     double tempAmp, tempPeriod;
     double tempInvPeriod;  // 1 / tempPeriod (the FP32 path multiplies)
     int varyTemp, enableExpansion, collisions, pad;
+    int ffOff[3], ffLen[3];  // force-field postfix programs in the formula table (length 0: none)
     int voff, vcount, loff, lcount;
     // collision broadphase: surface voxels [soff, soff+scount) of the surface
     // list, hash table of (mask+1) buckets at bucket offset toff, cell size

@@ -45,6 +45,8 @@ template <typename T> struct Dev {
     const VoxelExt *exts;  // voxel externals (VoxelMat::ext indexes it)
     const LinkMat *lmats;
     const CurvePt *curves; // stress-strain curves of non-linear link materials
+    const MtToken *formulas; // force-field postfix programs (RobotParams::ffOff / ffLen)
+    T4<T> *field;            // per-voxel force-field force of the step, or nullptr without force fields
     const RobotParams *rp;
     RobotTime *rt;
     const T4<T> *contact;  // per-voxel contact force, or nullptr without collisions
@@ -58,6 +60,13 @@ template <typename T> struct Dev {
 };
 
 // ------------------------------------------------------------------ device
+// absolute position in double of a voxel's position field u
+template <typename T> __device__ __forceinline__ V3<double> position_d(int lat, double nomSize, V3<T> u) {
+    if constexpr (kAbsolutePositions<T>) return V3<double>{u.x, u.y, u.z};
+    return V3<double>{(lat & 1023) * nomSize + (double)u.x, ((lat >> 10) & 1023) * nomSize + (double)u.y,
+                      (lat >> 20) * nomSize + (double)u.z};
+}
+
 template <typename T> __device__ __forceinline__ void link_update(const Dev<T> &a, int l, int parity) {
     const int2 meta = a.lmeta[l];
     RobotTime *rt = a.rt + meta.y;
@@ -132,7 +141,12 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
         const T4<T> c = a.contact[v];
         contact = {c.x, c.y, c.z};
     }
-    voxel_step<T>(vm, a.exts, dt, a.lat[v], temp, fs, ms, contact, s);
+    V3<T> field = {0, 0, 0};
+    if (a.field) {
+        const T4<T> f = a.field[v];
+        field = {f.x, f.y, f.z};
+    }
+    voxel_step<T>(vm, a.exts, dt, a.lat[v], temp, fs, ms, contact, field, s);
     if (rp.varyTemp && !(vm.thermalOnAfter > tnext)) {
         if constexpr (std::is_same<T, float>::value)
             temp = actuation_temperature_f32(rp.tempAmp, rp.tempInvPeriod, tnext, (double)a.phase[v], rp.enableExpansion);
@@ -143,6 +157,25 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
     a.pq1[v] = T4<T>{s.q.x, s.q.y, s.q.z, (T)temp};
     a.mo0[v] = T4<T>{s.p.x, s.p.y, s.p.z, s.h.x};
     a.mo1[v] = T4<T>{s.h.y, s.h.z, (T)s.flags, (T)0};
+}
+
+// ---- force fields (VX3_Voxel.cu:177-183) ---------------------------------------
+// The force-field force of each voxel for the coming voxel update, from the
+// position entering the step and the time before it (a kernel of its own: the
+// formula evaluator stays out of the update kernels).
+template <typename T> __global__ __launch_bounds__(256) void k_force_field(Dev<T> a, int v0, int nv, int parity) {
+    const int v = v0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= v0 + nv) return;
+    const int r = a.vrob[v];
+    const RobotParams &rp = a.rp[r];
+    const RobotTime *rt = a.rt + r;
+    V3<T> f = {0, 0, 0};
+    if (!rt->diverged && (rp.ffLen[0] | rp.ffLen[1] | rp.ffLen[2])) {
+        const T4<T> q0 = a.pq0[v];
+        const V3<double> p = position_d<T>(a.lat[v], a.vmats[a.vmat[v]].nomSize, V3<T>{q0.x, q0.y, q0.z});
+        f = force_field<T>(a.formulas, rp.ffOff, rp.ffLen, p.x, p.y, p.z, rt->time[parity], rp.vcount);
+    }
+    a.field[v] = T4<T>{f.x, f.y, f.z, (T)0};
 }
 
 // ---- collisions: hashed uniform grid over surface voxels ----------------------
@@ -385,6 +418,7 @@ template <typename T> struct Batch : BatchBase {
         int *d_tileRobots = nullptr;
     } plans[2];                   // large and small tile configuration
     bool tileOK = false;          // some tile plan exists
+    bool anyForceField = false;   // some robot has a force field (plain and flat modes only)
     // launch configuration used by VX3MI_MODE_AUTO (set by the autotuner)
     int autoMode = 0, tileSel = 0, persistThreads = 256;
     int nLinkMats = 0;  // entries of the device link material table
@@ -392,6 +426,7 @@ template <typename T> struct Batch : BatchBase {
     VoxelMat *d_vmats = nullptr;
     VoxelExt *d_exts = nullptr;
     CurvePt *d_curves = nullptr;
+    MtToken *d_formulas = nullptr;
     LinkMat *d_lmats = nullptr;
     float *d_vmatE = nullptr;
     double *d_out = nullptr;
@@ -405,6 +440,7 @@ template <typename T> struct Batch : BatchBase {
         std::vector<float> vmatE;
         std::vector<VoxelExt> exts;
         std::vector<CurvePt> curves;
+        std::vector<MtToken> formulas;
         std::vector<T4<T>> pq0, pq1;
         std::map<std::string, int> vmIndex, lmIndex, curveIndex;
         // materials are deduplicated across robots (byte-identical constants);
@@ -436,6 +472,17 @@ template <typename T> struct Batch : BatchBase {
             p.enableExpansion = m.enableExpansion;
             p.collisions = m.enableCollision;
             p.voff = voff; p.vcount = m.nv(); p.loff = loff; p.lcount = m.nl();
+            for (int ax = 0; ax < 3; ax++) {
+                std::vector<MtToken> post;
+                if (!m.forceField[ax].empty() && !mt_postfix(m.forceField[ax], post)) {
+                    set_error("malformed force-field formula");
+                    return VX3MI_E_ARG;
+                }
+                p.ffOff[ax] = (int)formulas.size();
+                p.ffLen[ax] = (int)post.size();
+                if (!post.empty()) anyForceField = true;
+                formulas.insert(formulas.end(), post.begin(), post.end());
+            }
             std::vector<int> vmRemap(m.vmats.size()), lmRemap(m.lmats.size());
             for (size_t k = 0; k < m.vmats.size(); k++) vmRemap[k] = vm_index(m.vmats[k], m.vmatE[k], -1);
             const int extOff = (int)exts.size();
@@ -506,6 +553,7 @@ template <typename T> struct Batch : BatchBase {
             (rc = alloc(&ls0, L)) || (rc = alloc(&ls1, L)) || (rc = alloc(&ls2, L)) || (rc = alloc(&lf0, L)) ||
             (rc = alloc(&lf1, L)) || (rc = alloc(&lf2, L)) || (rc = alloc(&d_vmats, hvmats.size())) ||
             (rc = alloc(&d_exts, exts.size())) || (rc = alloc(&d_curves, curves.size())) ||
+            (rc = alloc(&d_formulas, formulas.size())) ||
             (rc = alloc(&d_lmats, lmats.size())) || (rc = alloc(&d_vmatE, vmatE.size())) || (rc = alloc(&d_rp, R)) ||
             (rc = alloc(&d_rt, R)) || (rc = alloc(&d_out, (size_t)3 * R)))
             return rc;
@@ -542,6 +590,8 @@ template <typename T> struct Batch : BatchBase {
             HIP_TRY(hipMemcpy(d_exts, exts.data(), sizeof(VoxelExt) * exts.size(), hipMemcpyHostToDevice));
         if (!curves.empty())
             HIP_TRY(hipMemcpy(d_curves, curves.data(), sizeof(CurvePt) * curves.size(), hipMemcpyHostToDevice));
+        if (!formulas.empty())
+            HIP_TRY(hipMemcpy(d_formulas, formulas.data(), sizeof(MtToken) * formulas.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d_vmatE, vmatE.data(), sizeof(float) * vmatE.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d_rp, hrp.data(), sizeof(RobotParams) * R, hipMemcpyHostToDevice));
         std::vector<RobotTime> rt((size_t)R);
@@ -603,7 +653,15 @@ template <typename T> struct Batch : BatchBase {
         d.lat = latd; d.vmat = vmatd; d.vrob = vrobd; d.phase = phased; d.vlinks = vlinksd;
         d.lends = lendsd; d.lmeta = lmetad;
         d.ls0 = ls0; d.ls1 = ls1; d.ls2 = ls2; d.lf0 = lf0; d.lf1 = lf1; d.lf2 = lf2;
-        d.vmats = d_vmats; d.exts = d_exts; d.lmats = d_lmats; d.curves = d_curves; d.rp = d_rp; d.rt = d_rt;
+        d.field = nullptr;
+        if (anyForceField) {
+            T4<T> *fd;
+            if ((rc = alloc(&fd, V))) return rc;
+            HIP_TRY(hipMemcpy(fd, zeros.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+            d.field = fd;
+        }
+        d.vmats = d_vmats; d.exts = d_exts; d.lmats = d_lmats; d.curves = d_curves; d.formulas = d_formulas;
+        d.rp = d_rp; d.rt = d_rt;
         return plan_tiles();
     }
 
@@ -615,7 +673,7 @@ template <typename T> struct Batch : BatchBase {
         using C = TileCfg<T, SMALL>;
         TilePlan &P = plans[SMALL];
         P.ok = false;
-        if (anyCollisions || R < 2) return VX3MI_OK;
+        if (anyCollisions || anyForceField || R < 2) return VX3MI_OK;
         long totL = 0, totV = 0;
         for (const RobotParams &p : hrp) {
             if (p.vcount > C::VMAX || p.lcount > C::LMAX) return VX3MI_OK;
@@ -670,7 +728,8 @@ template <typename T> struct Batch : BatchBase {
         if ((rc = plan_tiles_cfg<0>()) || (rc = plan_tiles_cfg<1>())) return rc;
         tileOK = plans[0].ok || plans[1].ok;
         tileSel = plans[0].ok ? 0 : 1;
-        autoMode = tileOK ? VX3MI_MODE_TILED : ((R > 1 && V / R <= 4096) ? VX3MI_MODE_PERSISTENT : VX3MI_MODE_FLAT);
+        autoMode = tileOK ? VX3MI_MODE_TILED
+                          : ((R > 1 && V / R <= 4096 && !anyForceField) ? VX3MI_MODE_PERSISTENT : VX3MI_MODE_FLAT);
         return VX3MI_OK;
     }
 
@@ -709,9 +768,10 @@ template <typename T> struct Batch : BatchBase {
         if (anyCollisions) k_col_insert<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
         if (L) k_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, 0, L, parity);
         if (anyCollisions) k_col_contact<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
+        if (anyForceField) k_force_field<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         k_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
     }
-    int kernels_per_flat_step() const { return 2 + (anyCollisions ? 2 : 0); }
+    int kernels_per_flat_step() const { return 2 + (anyCollisions ? 2 : 0) + (anyForceField ? 1 : 0); }
 
     int step(int nsteps, float dt, int mode) override {
         if (nsteps <= 0) return VX3MI_OK;
@@ -723,6 +783,10 @@ template <typename T> struct Batch : BatchBase {
             return VX3MI_E_ARG;
         }
         if (mode == VX3MI_MODE_AUTO) mode = autoMode;
+        if (mode == VX3MI_MODE_PERSISTENT && anyForceField) {
+            set_error("force fields run in the plain and flat modes only");
+            return VX3MI_E_ARG;
+        }
         HIP_TRY(hipEventRecord(ev0, stream));
         launches = 0;
         lastMode = mode;
@@ -733,8 +797,9 @@ template <typename T> struct Batch : BatchBase {
                     if (p.scount) k_col_insert<T><<<(p.scount + 255) / 256, 256, 0, stream>>>(d, p.soff, p.scount, parity);
                     if (p.lcount) k_links<T><<<(p.lcount + 255) / 256, 256, 0, stream>>>(d, p.loff, p.lcount, parity);
                     if (p.scount) k_col_contact<T><<<(p.scount + 255) / 256, 256, 0, stream>>>(d, p.soff, p.scount, parity);
+                    if (anyForceField) k_force_field<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
                     k_voxels<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
-                    launches += 2 + (p.scount ? 2 : 0);
+                    launches += 2 + (p.scount ? 2 : 0) + (anyForceField ? 1 : 0);
                 }
                 parity ^= 1;
             }
@@ -887,6 +952,7 @@ template <typename T> struct Batch : BatchBase {
         for (int c = 0; rc == VX3MI_OK && c < kCandidates; c++) {
             ms[c] = -1;
             if (modes[c] == VX3MI_MODE_TILED && !plans[tiles[c]].ok) continue;
+            if (modes[c] == VX3MI_MODE_PERSISTENT && anyForceField) continue;
             tileSel = tiles[c];
             if (threads[c]) persistThreads = threads[c];
             if ((rc = step(std::min(trialSteps, 64), -1.0f, modes[c])) < 0) break;  // code load, graph capture
@@ -928,6 +994,10 @@ template <typename T> struct Batch : BatchBase {
             return VX3MI_E_ARG;
         }
         if (mode < VX3MI_MODE_PLAIN || mode > VX3MI_MODE_TILED) { set_error("invalid mode"); return VX3MI_E_ARG; }
+        if (mode == VX3MI_MODE_PERSISTENT && anyForceField) {
+            set_error("force fields run in the plain and flat modes only");
+            return VX3MI_E_ARG;
+        }
         autoMode = mode;
         if (mode == VX3MI_MODE_TILED) tileSel = tile;
         if (threads) persistThreads = threads;

@@ -146,6 +146,10 @@ template <typename T> struct DiffBatch : DiffBase {
                 set_error("differentiable batches need one actuation and floor setting for every robot");
                 return VX3MI_E_ARG;
             }
+            if (!m.forceField[0].empty() || !m.forceField[1].empty() || !m.forceField[2].empty()) {
+                set_error("differentiable mode: force fields are not supported");
+                return VX3MI_E_ARG;
+            }
             for (int v = 0; v < m.nv(); v++)
                 if (m.ext_of(v) >= 0) {
                     set_error("differentiable mode: voxel externals (boundary conditions) are not supported");

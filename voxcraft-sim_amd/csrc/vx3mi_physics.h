@@ -635,12 +635,28 @@ template <typename T> VX_HD T fixed_coordinate(int c, double nomSize, double t) 
 // VX3_Voxel::timeStep VX3_Voxel.cu:162-277 with force() :350-380, moment()
 // :382-397, floorForce :399-426.  fLink / mLink are the link force / moment
 // sums in the voxel frame (summed in link direction order X_POS..Z_NEG),
-// contact is the collision force to subtract (VX3_Voxel.cu:372-373).  lat is
-// the packed lattice coordinate (ix | iy << 10 | iz << 20); exts the external
-// table (read only when vm.ext >= 0).
+// contact is the collision force to subtract (VX3_Voxel.cu:372-373), field the
+// force-field force (force_field).  lat is the packed lattice coordinate
+// (ix | iy << 10 | iz << 20); exts the external table (read only when
+// vm.ext >= 0).
+
+// The force field of a voxel (VX3_Voxel.cu:177-183): formulas f (postfix,
+// off / len per axis, len 0 = none) at the position entering the step and the
+// time before it; hit, angle, targetCloseness and numClosePairs are 0 (no
+// targets or pair counting; formulas using the angle are rejected when read).
+template <typename T>
+VX_HD V3<T> force_field(const MtToken *f, const int *off, const int *len, double x, double y, double z, double t,
+                        int numVoxels) {
+    const double vars[MV_COUNT] = {x, y, z, 0.0, t, 0.0, 0.0, 0.0, (double)numVoxels};
+    V3<T> r = {0, 0, 0};
+    if (len[0]) r.x = (T)formula_eval(f + off[0], len[0], vars);
+    if (len[1]) r.y = (T)formula_eval(f + off[1], len[1], vars);
+    if (len[2]) r.z = (T)formula_eval(f + off[2], len[2], vars);
+    return r;
+}
 template <typename T>
 VX_HD void voxel_step(const VoxelMat &vm, const VoxelExt *exts, float dtf, int lat, float temp, V3<T> fLink,
-                      V3<T> mLink, V3<T> contact, VoxelState<T> &s) {
+                      V3<T> mLink, V3<T> contact, V3<T> field, VoxelState<T> &s) {
     if (dtf == 0.0f) return;
     const T dt = (T)dtf;
     const int iz = lat >> 20;
@@ -663,7 +679,9 @@ VX_HD void voxel_step(const VoxelMat &vm, const VoxelExt *exts, float dtf, int l
     tf = tf - vel * (T)vm.gDampT;
     tf.z += vm.gravF;
     tf = tf - contact;
-    tf.x += (T)0.0; tf.y += (T)0.0; tf.z += (T)0.0;  // constant-zero force field (VX3_ForceField.h:11-20)
+    // force field (VX3_Voxel.cu:177-183); zeros without one, as the reference's
+    // constant-zero default trees (VX3_ForceField.h:11-20) add +0.0
+    tf.x += field.x; tf.y += field.y; tf.z += field.z;
 
     V3<T> cur = tf;
     if (flags & VF_FLOOR_ENABLED) {

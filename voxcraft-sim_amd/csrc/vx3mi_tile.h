@@ -229,8 +229,8 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), 2) void k_tile(Dev<T> a, c
                 ms = ms + V3<T>{sForce[o + 3][l], sForce[o + 4][l], sForce[o + 5][l]};
             }
             float temp = (float)sPose[7][lv];
-            voxel_step<T>(vm, a.exts, dt, vLat[k], temp, fs, ms, V3<T>{0, 0, 0}, vs[k]);
             const int tf = tr.tempFlags[j];
+            voxel_step<T>(vm, a.exts, dt, vLat[k], temp, fs, ms, V3<T>{0, 0, 0}, V3<T>{0, 0, 0}, vs[k]);
             if ((tf & 1) && !(vm.thermalOnAfter > tnext)) {
                 if constexpr (std::is_same<T, float>::value)
                     temp = actuation_temperature_f32(tr.tempAmp[j], tr.tempPeriod[j], tnext, (double)vPhase[k], tf >> 1);

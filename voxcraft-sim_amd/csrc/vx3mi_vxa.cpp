@@ -299,6 +299,7 @@ struct VxaContent {
     std::vector<double> phase;
     std::vector<vx3mi_material> mats;
     std::vector<MtToken> stop, fitness;
+    std::vector<MtToken> forceField[3];
     std::string rawPrint;
     int savePositions = 0;
     int recordStepSize = 0, recordVoxel = 1, recordLink = 0;
@@ -529,16 +530,23 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
         int rc = parse_formula(path(sim, {"StopCondition", "StopConditionFormula"}), c.stop, err);
         if (rc == VX3MI_OK) rc = parse_formula(sim->child("FitnessFunction"), c.fitness, err);
         if (rc != VX3MI_OK) return rc;
+        // ForceField/{x,y,z}_forcefield (VX3_SimulationManager.cu:352-354); a
+        // constant-zero tree is the reference's default and adds +0.0
         if (const XNode *ff = sim->child("ForceField")) {
-            for (const XNode &k : ff->kids) {
-                std::vector<MtToken> f;
-                if ((rc = parse_formula(&k, f, err)) != VX3MI_OK) return rc;
-                const double zeros[MV_COUNT] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-                double r = 0;
-                if (!f.empty() && (f.size() != 2 || f[0].op != MT_CONST || !mt_eval(f, zeros, &r) || r != 0.0)) {
-                    err = "a non-zero ForceField is not supported";
+            static const char *names[3] = {"x_forcefield", "y_forcefield", "z_forcefield"};
+            for (int a = 0; a < 3; a++) {
+                std::vector<MtToken> f, post;
+                if ((rc = parse_formula(ff->child(names[a]), f, err)) != VX3MI_OK) return rc;
+                if (f.empty() || (f.size() == 2 && f[0].op == MT_CONST && f[0].value == 0.0)) continue;
+                if (mt_vars_used(f) & (1u << MV_ANGLE)) {
+                    err = "force fields that use the recent angle are not supported";
                     return VX3MI_E_PARSE;
                 }
+                if (!mt_postfix(f, post)) {
+                    err = "force-field formula nested too deeply";
+                    return VX3MI_E_PARSE;
+                }
+                c.forceField[a] = f;
             }
         }
     } else {
@@ -728,6 +736,7 @@ int model_from_docs(const char *baseText, const char *vxdText, vx3mi_model **out
         m.savePositions = c.savePositions;
         m.stopFormula = c.stop;
         m.fitnessFormula = c.fitness;
+        for (int a = 0; a < 3; a++) m.forceField[a] = c.forceField[a];
         m.secondaryExperiment = c.secondaryExperiment;
         m.reinitInitialPositionAfter = c.reinitAfter;
         m.recordStepSize = c.recordStepSize;
@@ -756,6 +765,31 @@ extern "C" {
 
 int vx3mi_model_from_xml(const char *base_vxa_text, const char *vxd_text, vx3mi_model **out) {
     return model_from_docs(base_vxa_text, vxd_text, out);
+}
+
+int vx3mi_model_set_force_field(vx3mi_model *m, int axis, const char *formula_xml) {
+    if (!m || axis < 0 || axis > 2) { set_error("vx3mi_model_set_force_field: bad model or axis"); return VX3MI_E_ARG; }
+    std::vector<MtToken> f, post;
+    try {
+        if (formula_xml && *formula_xml) {
+            // the formula element's content: the same subtree a VXA's <x_forcefield> holds
+            const std::string wrapped = std::string("<f>") + formula_xml + "</f>";
+            XNode doc = XmlReader(wrapped.c_str()).parse_document();
+            std::string err;
+            const int rc = parse_formula(doc.child("f"), f, err);
+            if (rc != VX3MI_OK) { set_error(err); return rc; }
+        }
+    } catch (const XmlError &e) {
+        set_error("XML: " + e.msg);
+        return VX3MI_E_PARSE;
+    }
+    if (!f.empty() && (mt_vars_used(f) & (1u << MV_ANGLE))) {
+        set_error("force fields that use the recent angle are not supported");
+        return VX3MI_E_PARSE;
+    }
+    if (!f.empty() && !mt_postfix(f, post)) { set_error("force-field formula nested too deeply"); return VX3MI_E_PARSE; }
+    m->m.forceField[axis] = f;
+    return VX3MI_OK;
 }
 
 int vx3mi_model_from_vxa(const char *base_vxa_path, const char *vxd_path, vx3mi_model **out) {

@@ -132,6 +132,7 @@ def lib():
         "vx3mi_model_voxel_constants": (C.c_int, [vp, C.c_int, fp]),
         "vx3mi_model_link_constants": (C.c_int, [vp, C.c_int, fp]),
         "vx3mi_model_link_curve": (C.c_int, [vp, C.c_int, fp, fp]),
+        "vx3mi_model_set_force_field": (C.c_int, [vp, C.c_int, C.c_char_p]),
         "vx3mi_model_set_external": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp]),
         "vx3mi_model_get_external": (C.c_int, [vp, C.c_int, ip, dp, dp, dp, dp]),
         "vx3mi_batch_create": (C.c_int, [pvp, C.c_int, C.c_int, pvp]),
@@ -225,7 +226,17 @@ class Model:
         m = cls(h.value)
         for e in p.get("externals") or []:
             m.set_external(**e)
+        for axis, expr in (p.get("force_field") or {}).items():
+            m.set_force_field(axis, expr)
         return m
+
+    def set_force_field(self, axis, expr) -> None:
+        """Force-field formula of axis ("x", "y", "z" or 0..2) as a nested tuple
+        (the format of vx3mi.vxa.formula_xml) or MathTree XML text; None removes it."""
+        from .vxa import formula_xml
+        a = "xyz".index(axis) if isinstance(axis, str) else int(axis)
+        text = "" if expr is None else (expr if isinstance(expr, str) else formula_xml(expr))
+        _check(lib().vx3mi_model_set_force_field(self._h, a, text.encode()))
 
     @classmethod
     def from_vxa(cls, base_vxa: str, vxd: str | None = None) -> "Model":
