@@ -195,17 +195,26 @@ def parity_report(c1_steps: int = 20000, c1_window: int = 1000, every: int = 50)
     m1 = vx3mi.Model.from_problem(p1)
     b32, b64 = vx3mi.Batch([m1], vx3mi.FP32), vx3mi.Batch([m1], vx3mi.FP64)
     c32, c64 = b32.com()[0], b64.com()[0]
-    ep = eq = 0.0
+    ep = eq = ep999 = 0.0
+    over = 0
     for k in range(0, c1_window, every):
         b32.step(every)
         b64.step(every)
         s32, s64 = b32.state(), b64.state()
-        ep, eq = max(ep, e_pos(s32["pos"], s64["pos"])), max(eq, e_quat(s32["quat"], s64["quat"]))
+        dp = np.linalg.norm(s32["pos"] - s64["pos"], axis=1) / np.maximum(np.linalg.norm(s64["pos"], axis=1), 0.01)
+        dq = np.minimum(np.linalg.norm(s32["quat"] - s64["quat"], axis=1),
+                        np.linalg.norm(s32["quat"] + s64["quat"], axis=1))
+        ep, eq = max(ep, float(dp.max())), max(eq, float(dq.max()))
+        ep999 = max(ep999, float(np.quantile(dp, 0.999)))
+        over = max(over, int(np.sum((dp > 1e-5) | (dq > 1e-5))))
     b32.step(c1_steps - c1_window)
     b64.step(c1_steps - c1_window)
     out["c1_fp32_vs_fp64"] = {"e_pos_first_1000": ep, "e_q_first_1000": eq,
+                              "e_pos_p999_first_1000": ep999,
+                              "most_voxels_over_1e-5_at_a_sample": over, "voxels": int(len(dp)),
                               f"com_after_{c1_steps}": com_rel(b32.com()[0], c32, b64.com()[0], c64),
-                              "diverged": [int(b32.diverged()[0]), int(b64.diverged()[0])]}
+                              "diverged": [int(b32.diverged()[0]), int(b64.diverged()[0])],
+                              "collision_overflow": [int(b32.collision_overflow()[0]), int(b64.collision_overflow()[0])]}
     b32.close()
     b64.close()
     # config 3: differentiable FP32 vs the oracle's differentiable mode, 2000 steps

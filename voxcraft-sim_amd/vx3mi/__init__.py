@@ -145,6 +145,7 @@ def lib():
         "vx3mi_set_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
         "vx3mi_get_voxel_flags": (C.c_int, [vp, C.c_int, ip]),
         "vx3mi_get_link_flags": (C.c_int, [vp, C.c_int, ip, fp]),
+        "vx3mi_batch_collision_overflow": (C.c_int, [vp, ip]),
         "vx3mi_batch_time": (C.c_int, [vp, dp]),
         "vx3mi_batch_diverged": (C.c_int, [vp, ip]),
         "vx3mi_center_of_mass": (C.c_int, [vp, dp]),
@@ -381,6 +382,12 @@ class Batch:
         f = np.zeros(n, dtype=np.int32)
         _check(lib().vx3mi_get_voxel_flags(self._h, int(robot), _ptr(f, C.c_int)))
         return f
+
+    def collision_overflow(self) -> np.ndarray:
+        """Per robot: 1 when the collision grid dropped a contact (see the header)."""
+        out = np.zeros(self.nrobot, dtype=np.int32)
+        _check(min(0, lib().vx3mi_batch_collision_overflow(self._h, _ptr(out, C.c_int))))
+        return out
 
     def link_flags(self, robot: int = -1):
         """(flags, axial strain) per link: bit 0 local velocity valid, bit 1 small angle."""
