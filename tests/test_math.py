@@ -1,9 +1,14 @@
 """The double-precision sin/cos/acos shared by the oracle and the FP64 device
-path (include/vx3mi_math.h) against the platform libm, in ulps."""
+path (include/vx3mi_math.h) against the platform libm, in ulps; the FP32
+rotation primitives of the timestep against their FP64 forms."""
+import os
+
 import numpy as np
 import pytest
 
 import oracle as O
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def ulps(a, b):
@@ -30,3 +35,20 @@ def test_special_values():
     assert L.vxo_acos(1.0) == 0.0 and L.vxo_acos(-1.0) == np.pi
     assert L.vxo_acos(0.0) == np.pi / 2
     assert np.isnan(L.vxo_sin(float("inf"))) and np.isnan(L.vxo_acos(float("nan")))
+
+
+def test_fp32_rotation_primitives_agree_with_fp64(tmp_path):
+    """The FP32 forms of the rotation primitives (Maclaurin series for small
+    rotations, trigonometry-free align-to-x rotation, reciprocal square roots)
+    agree with the FP64 forms, which follow the reference's formulas."""
+    import ctypes as C
+    import subprocess
+    out = tmp_path / "fp32_primitives_check.so"
+    csrc = os.path.join(ROOT, "voxcraft-sim_amd", "csrc")
+    subprocess.run(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", "-ffp-contract=off", "-I" + csrc,
+                    "-I" + os.path.join(ROOT, "include"), os.path.join(ROOT, "tests", "fp32_primitives_check.cpp"),
+                    "-o", str(out)], check=True)
+    lib = C.CDLL(str(out))
+    res = (C.c_double * 3)()
+    lib.check_primitives(C.c_ulonglong(7), 200000, res)
+    assert res[0] < 1e-6 and res[1] < 1e-6 and res[2] < 1e-6, list(res)  # summed over 4 components: ~1 ulp each

@@ -336,6 +336,24 @@ template <typename T> VX_HD Q4<T> from_angle_to_posx(V3<T> rf) {
         q.w = (T)1 + (T)0.5 * (-q.y * q.y - q.z * q.z);
         return q;
     }
+    if constexpr (std::is_same<T, float>::value) {
+        // the same rotation without trigonometry: for the unit vector n = rf/l
+        // at angle theta from +x, cos(theta/2) = sqrt((1 + n.x)/2) and the axis
+        // (0, n.z, -n.y) scaled by sin(theta/2)/sin(theta) = 1/(2 cos(theta/2)):
+        // q = ((l + x), 0, z, -y) / sqrt(2 l (l + x)).  Well conditioned while
+        // l + x is not small against l (theta below ~150 degrees); beyond, the
+        // angle path below.
+        const float l = fsqrt(rf.x * rf.x + rf.y * rf.y + rf.z * rf.z);
+        const float lpx = l + rf.x;
+        if (lpx > 0.125f * l) {
+            const float r = frsqrt(2.0f * l * lpx);
+            q.w = lpx * r;
+            q.x = 0;
+            q.y = rf.z * r;
+            q.z = -rf.y * r;
+            return q;
+        }
+    }
     V3<T> n = rf;
     T l = msqrt(n.x * n.x + n.y * n.y + n.z * n.z);
     if (l > 0) {
