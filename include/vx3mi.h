@@ -203,10 +203,11 @@ int vx3mi_get_link_flags(vx3mi_batch *b, int robot, int *flags, float *strain);
 /* per-robot simulated time (currentTime), diverged flag, step count */
 int vx3mi_batch_time(vx3mi_batch *b, double *out);
 int vx3mi_batch_diverged(vx3mi_batch *b, int *out);
-/* per robot: 1 when the collision grid dropped a contact since creation (a grid
- * bucket held more than 16 surface voxels or a voxel had more than 32 contacts
- * in one step; results then differ from the all-pairs reference).  Returns 1
- * when any robot overflowed, 0 otherwise, negative on error. */
+/* per robot: non-zero when the collision grid dropped a contact since creation
+ * (bit 0: a grid bucket held more than 32 surface voxels; bit 1: a voxel had
+ * more than 64 contacts in one step); results then differ from the all-pairs
+ * reference and are no longer deterministic.  Returns 1 when any robot
+ * overflowed, 0 otherwise, negative on error. */
 int vx3mi_batch_collision_overflow(vx3mi_batch *b, int *out);
 /* updateCurrentCenterOfMass (VX3_VoxelyzeKernel.cu:477-493) per robot: out 3n */
 int vx3mi_center_of_mass(vx3mi_batch *b, double *out);

@@ -25,8 +25,11 @@ struct RobotParams {
 
 // fixed-capacity buckets of the collision hash grid, and the most contacts one
 // voxel may have in one step (both overflows are reported, never ignored)
-constexpr int kBucketCap = 16;
-constexpr int kMaxContacts = 32;
+// (the grid cell is one envelope diameter, up to 1.74 voxel lengths with thermal
+// expansion: a compressed body can put ~15 surface voxels in one cell and ~60
+// within one envelope)
+constexpr int kBucketCap = 32;
+constexpr int kMaxContacts = 64;
 
 struct RobotTime {
     double time[2];
@@ -34,7 +37,7 @@ struct RobotTime {
     float dt;
     int diverged;
     int steps;
-    int overflow;  // collision bucket or contact list overflowed
+    int overflow;  // bit 0: a collision grid bucket overflowed; bit 1: a voxel's contact list did
 };
 
 #define HIP_TRY(expr)                                                                          \

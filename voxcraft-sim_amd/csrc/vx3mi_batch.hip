@@ -146,7 +146,7 @@ int vx3mi_batch_collision_overflow(vx3mi_batch *b, int *out) {
     if (rc) return rc;
     int any = 0;
     for (int r = 0; r < b->b->R; r++) { out[r] = rt[r].overflow; any |= rt[r].overflow; }
-    return any;
+    return any ? 1 : 0;
 }
 
 int vx3mi_center_of_mass(vx3mi_batch *b, double *out) {

@@ -260,7 +260,7 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
                     const T bsaJ = base_size_average<T>(mj.nomSize, (float)qj1.w, mj.alphaCTE);
                     if (!in_contact_envelope(diff, bsaI, bsaJ)) continue;
                     if (nf < kMaxContacts) found[nf++] = sj;
-                    else atomicOr(&rt->overflow, 1);
+                    else atomicOr(&rt->overflow, 2);
                 }
             }
     // ascending partner index: the order in which the serialized pair loop of
