@@ -63,10 +63,14 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(problems, nsteps: int):
+def cpu_baseline(problems, nsteps: int, omp: bool = False):
     """Reference CPU Voxelyze (oracle/_ref, built from the reference sources) on
-    a sample of robots; falls back to the FP64 oracle port when absent."""
-    ref_exe = os.path.join(ROOT, "oracle", "_ref", "vx3_ref_driver")
+    a sample of robots; falls back to the FP64 oracle port when absent.  omp:
+    the reference's own OpenMP build (Voxelyze.cpp:260-279) on every thread
+    OMP_NUM_THREADS allows."""
+    ref_exe = os.path.join(ROOT, "oracle", "_ref", "vx3_ref_driver_omp" if omp else "vx3_ref_driver")
+    if omp and not os.path.exists(ref_exe):
+        return None, None
     if os.path.exists(ref_exe):
         secs = vs = 0.0
         for p in problems:
@@ -343,6 +347,13 @@ def main() -> int:
                "sample": f"first {len(sample)} robots of the workload ({svox} voxels) x {args.cpu_steps} steps, "
                          "single thread"}
         log(f"bench: CPU baseline ({kind}) {cv:.3e} voxel-steps/s")
+        osteps = max(1, args.cpu_steps // 5)
+        ov, _ = cpu_baseline(sample, osteps, omp=True)
+        if ov is not None:
+            threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+            cpu["all_threads"] = {"value": ov, "threads": threads,
+                                  "note": f"reference built with -DUSE_OMP -fopenmp, same robots x {osteps} steps"}
+            log(f"bench: CPU baseline (reference, OpenMP, {threads} threads) {ov:.3e} voxel-steps/s")
 
     parity = None
     if not args.no_parity:
