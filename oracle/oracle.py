@@ -26,7 +26,8 @@ class _Material(C.Structure):
     _fields_ = [("E", C.c_float), ("rho", C.c_float), ("nu", C.c_float), ("cte", C.c_float),
                 ("mu_s", C.c_float), ("mu_k", C.c_float), ("fail_stress", C.c_float),
                 ("plastic_mod", C.c_float), ("yield_stress", C.c_float), ("model", C.c_int),
-                ("fixed", C.c_int), ("is_measured", C.c_int), ("thermal_on_after", C.c_double)]
+                ("fixed", C.c_int), ("is_measured", C.c_int), ("thermal_on_after", C.c_double),
+                ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double)]
 
 
 # MathTree token codes (vx3_oracle.h VXO_MT_*) and variable order
@@ -54,7 +55,9 @@ class _Spec(C.Structure):
                 ("volume_effects", C.c_int), ("enable_collision", C.c_int), ("enable_expansion", C.c_int),
                 ("grav_acc", C.c_float), ("floor_enabled", C.c_int),
                 ("temp_enabled", C.c_int), ("vary_temp", C.c_int),
-                ("temp_base", C.c_double), ("temp_amplitude", C.c_double), ("temp_period", C.c_double)]
+                ("temp_base", C.c_double), ("temp_amplitude", C.c_double), ("temp_period", C.c_double),
+                ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int),
+                ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double))]
 
 
 class _DiffOpts(C.Structure):
@@ -66,10 +69,12 @@ class _DiffOpts(C.Structure):
 SPEC_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=1.0, slow_damp_z=1.0,
                      volume_effects=0, enable_collision=1, enable_expansion=0, grav_acc=-9.81,
                      floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
-                     temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None)
+                     temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None,
+                     enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
-                         fixed=0, is_measured=1, thermal_on_after=0.0)
+                         fixed=0, is_measured=1, thermal_on_after=0.0,
+                         cilia=0.0, cilia_on_after=0.0, remove_after=0.0)
 
 
 def build_library(force: bool = False) -> str:
@@ -106,6 +111,7 @@ def lib():
         L.vxo_get_voxels.argtypes = [vp, dp, dp, dp, dp]
         L.vxo_set_voxels.argtypes = [vp, dp, dp, dp, dp]
         L.vxo_get_flags.argtypes = [vp, ip, ip]
+        L.vxo_get_removed.argtypes = [vp, ip, ip]
         L.vxo_get_link_state.argtypes = [vp, dp, fp]
         L.vxo_get_temperature.argtypes = [vp, fp, dp]
         L.vxo_set_temperature.argtypes = [vp, fp]
@@ -152,15 +158,20 @@ class OracleModel:
             model = material_model(m)
             self._mats[i] = _Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"],
                                       m["fail_stress"] if model else -1.0, m["plastic_mod"], m["yield_stress"],
-                                      model, int(m["fixed"]), int(m["is_measured"]), m["thermal_on_after"])
+                                      model, int(m["fixed"]), int(m["is_measured"]), m["thermal_on_after"],
+                                      m["cilia"], m["cilia_on_after"], m["remove_after"])
         self._phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         self._E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
+        self._cb, self._cs = [None if p[k] is None else np.ascontiguousarray(p[k], dtype=np.float64).reshape(-1)
+                              for k in ("cilia_base", "cilia_shift")]
         spec = _Spec(nx, ny, nz, _ptr(self._cells, C.c_int8), _ptr(self._phase, C.c_double),
                      _ptr(self._E, C.c_float), len(mats), self._mats, p["lattice_dim"], p["dt_frac"],
                      p["bond_damp_z"], p["col_damp_z"], p["slow_damp_z"], int(p["volume_effects"]),
                      int(p["enable_collision"]), int(p["enable_expansion"]), p["grav_acc"],
                      int(p["floor_enabled"]), int(p["temp_enabled"]), int(p["vary_temp"]),
-                     p["temp_base"], p["temp_amplitude"], p["temp_period"])
+                     p["temp_base"], p["temp_amplitude"], p["temp_period"],
+                     int(p["enable_cilia"]), int(p["secondary_experiment"]),
+                     _ptr(self._cb, C.c_double), _ptr(self._cs, C.c_double))
         self._L = lib()
         self._h = self._L.vxo_build(C.byref(spec))
         self.nv = self._L.vxo_num_voxels(self._h)
@@ -251,6 +262,13 @@ class OracleModel:
         self._L.vxo_get_flags(self._h, _ptr(vf, C.c_int), _ptr(lf, C.c_int))
         return vf, lf
 
+    def removed(self):
+        """Per voxel and per link: the step (1-based) at whose end it was
+        removed, 0 while present."""
+        vr = np.zeros(self.nv, dtype=np.int32); lr = np.zeros(max(1, self.nl), dtype=np.int32)
+        self._L.vxo_get_removed(self._h, _ptr(vr, C.c_int), _ptr(lr, C.c_int))
+        return vr, lr[:self.nl]
+
     def link_state(self):
         s = np.zeros((self.nl, 9)); st = np.zeros(self.nl, dtype=np.float32)
         self._L.vxo_get_link_state(self._h, _ptr(s, C.c_double), _ptr(st, C.c_float))
@@ -268,6 +286,7 @@ class OracleModel:
     def topology(self):
         lat = np.zeros((self.nv, 3), dtype=np.int32)
         ln = np.zeros(self.nl, dtype=np.int32); lp = np.zeros(self.nl, dtype=np.int32)
+        self.ns = self._L.vxo_num_surface(self._h)
         la = np.zeros(self.nl, dtype=np.int32); sf = np.zeros(self.ns, dtype=np.int32)
         self._L.vxo_get_topology(self._h, *[_ptr(a, C.c_int) for a in (lat, ln, lp, la, sf)])
         return dict(lattice=lat, link_neg=ln, link_pos=lp, link_axis=la, surface=sf)

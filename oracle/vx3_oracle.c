@@ -50,6 +50,7 @@ typedef struct {
     float sigmaFail, epsFail;
     int fixed, isMeasured;
     double thermalOnAfter;
+    double cilia, ciliaOnAfter, removeAfter;
     /* stress-strain model: CVX_Material linear / strainData / stressData */
     int linear, npts;
     float sd[OM_MAX_POINTS], td[OM_MAX_POINTS];
@@ -88,6 +89,11 @@ struct vxo_model {
     v3 *ext_f, *ext_m;
     int *surface;          /* ns voxel indices, voxel order */
     int *is_surface;
+    /* cilia (BaseCiliaForce / ShiftCiliaForce per voxel) and removal: the step
+     * (1-based) at whose end a voxel or link was removed, 0 while present */
+    v3 *ciliaBase, *ciliaShift;
+    int *vremoved, *lremoved;
+    long steps;
     /* links */
     int *lneg, *lpos, *laxis;
     lmat *lm;
@@ -102,6 +108,7 @@ struct vxo_model {
     /* settings */
     double dtFrac, optimalDt, time;
     int enableCollision, enableExpansion, varyTemp;
+    int enableCilia, secondary;
     double tempAmplitude, tempPeriod;
 };
 
@@ -452,6 +459,20 @@ float vxo_step_dt(vxo_model *m) {
  * VX_Sim.cpp:467-483 with CVX_Environment::UpdateCurTemp
  * src/old/VX_Environment.cpp:327-346), volume effects (VX_Sim.cpp:445-458).
  */
+/* surface voxels: fewer than 6 links (VX3_Voxel::updateSurface
+ * VX3_Voxel.cu:515-524), removed ones left out of the list
+ * (regenerateSurfaceVoxels VX3_VoxelyzeKernel.cu:495-513) */
+static void regenerate_surface(vxo_model *m) {
+    int ns = 0;
+    for (int v = 0; v < m->nv; v++) {
+        int c = 0;
+        for (int d = 0; d < 6; d++) c += m->vlinks[v][d] >= 0;
+        m->is_surface[v] = c < 6;
+        if (c < 6 && !m->vremoved[v]) m->surface[ns++] = v;
+    }
+    m->ns = ns;
+}
+
 vxo_model *vxo_build(const vxo_spec *sp) {
     vxo_model *m = (vxo_model *)xcalloc(1, sizeof(vxo_model));
     int nx = sp->nx, ny = sp->ny, nz = sp->nz, ncell = nx * ny * nz;
@@ -462,6 +483,8 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->varyTemp = sp->vary_temp;
     m->tempAmplitude = sp->temp_amplitude;
     m->tempPeriod = sp->temp_period;
+    m->enableCilia = sp->enable_cilia;
+    m->secondary = sp->secondary_experiment;
 
     int nv = 0;
     for (int i = 0; i < ncell; i++)
@@ -504,6 +527,9 @@ vxo_model *vxo_build(const vxo_spec *sp) {
         vm->fixed = in->fixed;
         vm->isMeasured = in->is_measured;
         vm->thermalOnAfter = in->thermal_on_after;
+        vm->cilia = in->cilia;
+        vm->ciliaOnAfter = in->cilia_on_after;
+        vm->removeAfter = in->remove_after;
         vmat_derive(vm);
     }
 
@@ -529,6 +555,14 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->ext_f = (v3 *)xcalloc(nv, sizeof(v3));
     m->ext_m = (v3 *)xcalloc(nv, sizeof(v3));
     for (int i = 0; i < nv; i++) m->ext_dof[i] = -1;
+    m->ciliaBase = (v3 *)xcalloc(nv, sizeof(v3));
+    m->ciliaShift = (v3 *)xcalloc(nv, sizeof(v3));
+    m->vremoved = (int *)xcalloc(nv, sizeof(int));
+    for (int i = 0; i < nv; i++) {
+        if (sp->cilia_base) m->ciliaBase[i] = v3_(sp->cilia_base[3 * i], sp->cilia_base[3 * i + 1], sp->cilia_base[3 * i + 2]);
+        if (sp->cilia_shift)
+            m->ciliaShift[i] = v3_(sp->cilia_shift[3 * i], sp->cilia_shift[3 * i + 1], sp->cilia_shift[3 * i + 2]);
+    }
 
     int *cellvox = (int *)xcalloc(ncell, sizeof(int));
     int v = 0;
@@ -595,6 +629,7 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->fPos = (v3 *)xcalloc(nl, sizeof(v3));
     m->mNeg = (v3 *)xcalloc(nl, sizeof(v3));
     m->mPos = (v3 *)xcalloc(nl, sizeof(v3));
+    m->lremoved = (int *)xcalloc(nl > 0 ? nl : 1, sizeof(int));
     for (int l = 0; l < nl; l++) {
         /* combinedMaterial(voxel1 = new voxel, voxel2 = neighbour): symmetric */
         lmat_combine(&m->lm[l], &m->vm[m->vmat_of[m->lpos[l]]], &m->vm[m->vmat_of[m->lneg[l]]]);
@@ -606,15 +641,8 @@ vxo_model *vxo_build(const vxo_spec *sp) {
         m->tStrainSum[l] = 0.0f;
     }
 
-    /* surface voxels: fewer than 6 links (VX3_Voxel.cu:515-524) */
     m->surface = (int *)xcalloc(nv, sizeof(int));
-    int ns = 0;
-    for (v = 0; v < nv; v++) {
-        int c = 0;
-        for (int d = 0; d < 6; d++) c += m->vlinks[v][d] >= 0;
-        if (c < 6) { m->is_surface[v] = 1; m->surface[ns++] = v; }
-    }
-    m->ns = ns;
+    regenerate_surface(m);
 
     /* initial temperature (UpdateMatTemps) */
     float t0 = 0.0f;
@@ -636,7 +664,8 @@ void vxo_free(vxo_model *m) {
                     m->lpos, m->laxis, m->lm, m->pos2, m->a1v, m->a2v, m->strain, m->maxStrain, m->tArea,
                     m->tStrainSum, m->restLen, m->lflags, m->fNeg, m->fPos, m->mNeg, m->mPos, m->vm, m->strainOffset,
                     m->ext_dof, m->ext_t, m->ext_r, m->ext_q, m->ext_f, m->ext_m,
-                    m->ff_op[0], m->ff_op[1], m->ff_op[2], m->ff_val[0], m->ff_val[1], m->ff_val[2]};
+                    m->ff_op[0], m->ff_op[1], m->ff_op[2], m->ff_val[0], m->ff_val[1], m->ff_val[2],
+                    m->ciliaBase, m->ciliaShift, m->vremoved, m->lremoved};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(m);
 }
@@ -728,6 +757,7 @@ static void update_temperature(vxo_model *m) {
     const double twoPiF = (double)(2 * 3.1415926f);
     for (int v = 0; v < m->nv; v++) {
         const vmat *mt = &m->vm[m->vmat_of[v]];
+        if (m->vremoved[v]) continue;
         if (mt->thermalOnAfter > m->time) continue;
         if (mt->fixed) continue;
         double T = m->tempAmplitude * vxm_sin(twoPiF * (m->time / m->tempPeriod + m->phase[v]));
@@ -857,7 +887,8 @@ static void update_links(vxo_model *m) {
     int nl = m->nl;
 #pragma omp parallel for schedule(static)
     for (int l = 0; l < nl; l++) {
-        /* gpu_update_links VX3_VoxelyzeKernel.cu:566-581: skip links between two fixed voxels */
+        /* gpu_update_links VX3_VoxelyzeKernel.cu:566-581: skip removed links and links between two fixed voxels */
+        if (m->lremoved[l]) continue;
         if (m->vm[m->vmat_of[m->lpos[l]]].fixed && m->vm[m->vmat_of[m->lneg[l]]].fixed) continue;
         link_update(m, l);
     }
@@ -1008,6 +1039,18 @@ static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
     tf.z += -mt->mass * 9.80665f * mt->gravMult;
     tf = vsub(tf, m->contact[v]);
     m->contact[v] = v3_(0, 0, 0);
+    /* cilia: CiliaForce = orient.RotateVec3D(baseCiliaForce + localSignal *
+     * shiftCiliaForce) for surface voxels of a cilia material once
+     * TurnOnCiliaAfterThisManySeconds has passed (gpu_update_cilia_force
+     * VX3_VoxelyzeKernel.cu:846-858, before the voxel update); force() adds
+     * CiliaForce * mat->Cilia (VX3_Voxel.cu:376-377).  localSignal is 0 (no
+     * signals). */
+    if (m->enableCilia) {
+        v3 cf = v3_(0, 0, 0);
+        if (m->is_surface[v] && mt->cilia != 0 && !(mt->ciliaOnAfter > m->time))
+            cf = qrot(orient, vadd(m->ciliaBase[v], vscale(m->ciliaShift[v], 0.0)));
+        tf = vadd(tf, vscale(cf, mt->cilia));
+    }
     /* force field (VX3_Voxel.cu:177-183) at the position entering the step and
      * currentTime; hit, angle, closeness and pairs are 0 (no targets); absent
      * trees add +0.0 as the reference's constant-zero defaults do */
@@ -1106,9 +1149,35 @@ static void update_voxels(vxo_model *m, float dt, int cpu_semantics) {
     int nv = m->nv;
 #pragma omp parallel for schedule(static)
     for (int v = 0; v < nv; v++) {
-        if (m->vm[m->vmat_of[v]].fixed) continue;  /* gpu_update_voxels :582-589 */
+        if (m->vremoved[v]) continue;              /* gpu_update_voxels :582-589 */
+        if (m->vm[m->vmat_of[v]].fixed) continue;
         voxel_update(m, v, dt, cpu_semantics);
     }
+}
+
+/* VX3_VoxelyzeKernel::removeVoxels VX3_VoxelyzeKernel.cu:365-399, called at
+ * the end of doTimeStep before currentTime advances when SecondaryExperiment
+ * is set (:336-349): every voxel of a material whose
+ * RemoveFromSimulationAfterThisManySeconds is positive and below the current
+ * time is removed with its links (both ends forget the link); the surface
+ * list is regenerated before the next step's collisions and cilia (:282-287). */
+static void remove_voxels(vxo_model *m) {
+    int changed = 0;
+    for (int v = 0; v < m->nv; v++) {
+        const vmat *mt = &m->vm[m->vmat_of[v]];
+        if (m->vremoved[v] || !(mt->removeAfter > 0 && mt->removeAfter < m->time)) continue;
+        m->vremoved[v] = (int)(m->steps + 1);
+        for (int d = 0; d < 6; d++) {
+            int l = m->vlinks[v][d];
+            if (l < 0) continue;
+            m->lremoved[l] = (int)(m->steps + 1);
+            int o = (d & 1) ? m->lneg[l] : m->lpos[l];
+            m->vlinks[o][d ^ 1] = -1;
+            m->vlinks[v][d] = -1;
+        }
+        changed = 1;
+    }
+    if (changed) regenerate_surface(m);
 }
 
 /* ------------------------------------------------------------------- step */
@@ -1126,7 +1195,9 @@ int vxo_step(vxo_model *m, float dt, int nsteps, int flags) {
         if (diverged) return s;
         update_collisions(m);
         update_voxels(m, d, flags & VXO_CPU_SEMANTICS);
+        if (m->secondary) remove_voxels(m);
         m->time += d;
+        m->steps++;
     }
     return nsteps;
 }
@@ -1170,6 +1241,11 @@ void vxo_set_voxels(vxo_model *m, const double *pos3, const double *quat4, const
 void vxo_get_flags(const vxo_model *m, int *voxel_flags, int *link_flags) {
     if (voxel_flags) memcpy(voxel_flags, m->vflags, sizeof(int) * m->nv);
     if (link_flags) memcpy(link_flags, m->lflags, sizeof(int) * m->nl);
+}
+
+void vxo_get_removed(const vxo_model *m, int *voxel_step, int *link_step) {
+    if (voxel_step) memcpy(voxel_step, m->vremoved, sizeof(int) * m->nv);
+    if (link_step) memcpy(link_step, m->lremoved, sizeof(int) * m->nl);
 }
 
 void vxo_get_link_state(const vxo_model *m, double *state9, float *strain) {

@@ -15,6 +15,8 @@
  *                VX3_Material.cu:90-124, src/Utils/VX3_Quat3D.h
  *   collisions   src/VX3/VX3_VoxelyzeKernel.cu:682-843, VX3_Collision.cu:3-31
  *   voxels       src/VX3/VX3_Voxel.cu:162-277, 350-426, VX3_Voxel.h:185-208
+ *   cilia        src/VX3/VX3_VoxelyzeKernel.cu:296-302, 846-858, VX3_Voxel.cu:376-377
+ *   removal      src/VX3/VX3_VoxelyzeKernel.cu:336-399, 495-513 (SecondaryExperiment)
  *   CoM          src/VX3/VX3_VoxelyzeKernel.cu:477-493
  *
  * Deliberate (documented) deviations from the racy reference:
@@ -42,6 +44,9 @@ typedef struct {
     int fixed;          /* Fixed */
     int is_measured;    /* isMeasured */
     double thermal_on_after; /* TurnOnThermalExpansionAfterThisManySeconds */
+    double cilia;            /* Cilia: multiplier of the voxel's cilia force */
+    double cilia_on_after;   /* TurnOnCiliaAfterThisManySeconds */
+    double remove_after;     /* RemoveFromSimulationAfterThisManySeconds (0: never) */
 } vxo_material;
 
 /* Lattice + simulator + environment description. */
@@ -62,6 +67,10 @@ typedef struct {
     int floor_enabled;          /* Environment/Gravity/FloorEnabled */
     int temp_enabled, vary_temp;                   /* Thermal */
     double temp_base, temp_amplitude, temp_period; /* Thermal */
+    int enable_cilia;           /* Simulator/EnableCilia */
+    int secondary_experiment;   /* Simulator/SecondaryExperiment (material removal) */
+    const double *cilia_base;   /* per-voxel BaseCiliaForce (3 per voxel, voxel order) or NULL */
+    const double *cilia_shift;  /* per-voxel ShiftCiliaForce or NULL */
 } vxo_spec;
 
 typedef struct vxo_model vxo_model;
@@ -114,6 +123,10 @@ int vxo_get_link_curve(const vxo_model *m, int link, float *strain8, float *stre
 void vxo_get_voxels(const vxo_model *m, double *pos3, double *quat4, double *linmom3, double *angmom3);
 void vxo_set_voxels(vxo_model *m, const double *pos3, const double *quat4, const double *linmom3, const double *angmom3);
 void vxo_get_flags(const vxo_model *m, int *voxel_flags, int *link_flags);
+/* removal (SecondaryExperiment): per voxel / link the number of the step
+ * (1-based, counted over vxo_step calls) at whose end it was removed, 0 if
+ * present */
+void vxo_get_removed(const vxo_model *m, int *voxel_step, int *link_step);
 void vxo_get_link_state(const vxo_model *m, double *state9, float *strain);
 void vxo_get_temperature(const vxo_model *m, float *temp, double *rest_length);
 void vxo_set_temperature(vxo_model *m, const float *temp);

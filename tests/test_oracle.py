@@ -11,7 +11,11 @@ wrong sign or axis, a transposed operand) fails it:
   * uniform temperature: equilibrium spacing L (1 + alpha T);
   * internal forces conserve linear momentum;
   * link forces in voxel frames are objective under a rigid motion;
-  * two overlapping unlinked voxels: penalty force k (R - d) (k = 2 E L).
+  * two overlapping unlinked voxels: penalty force k (R - d) (k = 2 E L);
+  * cilia: a turned free voxel follows the rotated force exactly; a cube gains
+    momentum from its 26 surface voxels only, from the turn-on time on;
+  * removal: the step it happens, frozen removed voxels, free remainder, and
+    the regenerated surface (the exposed former centre gets cilia force).
 """
 import os
 import subprocess
@@ -261,3 +265,139 @@ def test_force_field_constant_force_and_operand_order():
     m.step(n, dt)
     expect = x0 + 2.0 * mass_inv * float(dt) ** 2 * n * (n + 1) / 2
     assert np.allclose(m.state()["pos"][:, 0], expect, rtol=1e-10, atol=0)
+
+
+def _cilia_problem(dims, base, cilia=1.0, on_after=0.0, remove=None, **kw):
+    """quiet() body with cilia on every voxel: BaseCiliaForce `base` (N), the
+    material's Cilia multiplier, TurnOnCiliaAfterThisManySeconds; `remove`
+    maps cells to a second, removable material (RemoveFromSimulationAfter...)."""
+    n = int(np.prod(dims))
+    cells = np.ones(n, dtype=np.int8)
+    mats = [dict(E=1e6, rho=1000.0, cte=0.01, cilia=cilia, cilia_on_after=on_after)]
+    sec = 0
+    if remove is not None:
+        mask, after = remove
+        cells[np.asarray(mask).reshape(-1)] = 2
+        mats.append(dict(mats[0], remove_after=after))
+        sec = 1
+    p = quiet(dims, cells=cells, enable_cilia=1, secondary_experiment=sec,
+              cilia_base=np.tile(np.asarray(base, dtype=np.float64), (n, 1)))
+    p["materials"] = mats
+    p.update(kw)
+    return p
+
+
+def test_cilia_force_turns_with_the_voxel():
+    """One free voxel turned 90 degrees about z: its cilia force is
+    Cilia * R(q) base, so a base force along x pushes it along +y with the exact
+    sequence y_n = y_0 + a dt^2 n(n+1)/2, a = Cilia |base| / m (a force rotated
+    the wrong way would push along -y)."""
+    p = _cilia_problem((1, 1, 1), (1e-3, 0.0, 0.0), cilia=2.0)
+    m = O.OracleModel(p)
+    c = np.sqrt(0.5)
+    m.set_state(quat=np.array([[c, 0.0, 0.0, c]]))
+    dt = m.step_dt()
+    y0 = m.state()["pos"][0, 1]
+    mass_inv = float(np.float32(1.0) / np.float32(1000.0 * 0.01 ** 3))
+    n = 300
+    m.step(n, dt)
+    s = m.state()
+    expect = y0 + 2.0 * 1e-3 * mass_inv * float(dt) ** 2 * n * (n + 1) / 2
+    assert s["pos"][0, 1] == pytest.approx(expect, rel=1e-12)
+    assert abs(s["pos"][0, 0]) < 1e-12 * abs(expect - y0)
+
+
+def _steps_before(t_on, dt, n):
+    """Steps among the first n whose pre-step time has reached t_on (the
+    reference's test `TurnOn > currentTime` fails), with the time accumulated as
+    the reference does (double += float dt)."""
+    t, k = 0.0, 0
+    for _ in range(n):
+        if not (t_on > t):
+            k += 1
+        t += float(np.float32(dt))
+    return k
+
+
+def test_cilia_only_on_surface_voxels_after_turn_on():
+    """3x3x3 cube, every voxel with the same cilia force: only the 26 surface
+    voxels are pushed (the centre is interior), and only on steps whose start
+    time has reached TurnOnCiliaAfterThisManySeconds.  Links conserve momentum,
+    so the total momentum is (active steps) x dt x 26 x F."""
+    dt = None
+    F = 1e-3
+    for t_on in (0.0, 0.004):
+        p = _cilia_problem((3, 3, 3), (F, 0.0, 0.0), on_after=t_on)
+        m = O.OracleModel(p)
+        dt = m.step_dt()
+        n = 200
+        m.step(n, dt)
+        px = m.state()["linmom"][:, 0].sum()
+        active = _steps_before(t_on, dt, n)
+        assert 0 < active <= n
+        assert px == pytest.approx(active * float(dt) * 26 * F, rel=1e-6)
+
+
+def test_removal_step_frozen_voxels_and_free_remainder():
+    """A 3-voxel bar falling freely; the middle voxel's material is removed
+    after T.  It is removed at the end of the first step whose start time
+    exceeds T (both links with it), then stays where it was, and the two end
+    voxels keep to the exact free-fall sequence (a link left in place would
+    pull them towards the frozen middle)."""
+    T = 0.002
+    p = _cilia_problem((3, 1, 1), (0.0, 0.0, 0.0), cilia=0.0, remove=([False, True, False], T), grav_acc=-9.81)
+    m = O.OracleModel(p)
+    dt = m.step_dt()
+    t, k = 0.0, 0
+    while not (T < t):
+        t += float(np.float32(dt))
+        k += 1
+    k += 1  # the step that starts at t
+    grav_mult = np.float32(-np.float32(-9.81) / 9.80665)
+    mass = np.float32(1000.0 * 0.01 ** 3)
+    gforce = float(np.float32(np.float32(-mass * np.float32(9.80665)) * grav_mult))
+    a = gforce * float(np.float32(1.0) / mass)
+    z0 = m.state()["pos"][:, 2].copy()
+    m.step(k - 1, dt)
+    assert m.removed()[0].tolist() == [0, 0, 0]
+    m.step(1, dt)
+    vr, lr = m.removed()
+    assert vr.tolist() == [0, k, 0] and lr.tolist() == [k, k]
+    z_k = m.state()["pos"][1, 2]
+    assert z_k == pytest.approx(z0[1] + a * float(dt) ** 2 * k * (k + 1) / 2, rel=1e-12)
+    n = k + 150
+    m.step(n - k, dt)
+    z = m.state()["pos"][:, 2]
+    assert z[1] == z_k
+    for v in (0, 2):
+        assert z[v] == pytest.approx(z0[v] + a * float(dt) ** 2 * n * (n + 1) / 2, rel=1e-12)
+
+
+def test_removal_regenerates_the_surface_for_cilia():
+    """3x3x3 cube with cilia everywhere; the x = 0 face (9 voxels, 21 links) is
+    removed after T.  The surface list then holds the 18 remaining voxels (the
+    former centre included), and from the next step on the remaining voxels
+    gain momentum dt x 18 x F per step."""
+    T = 0.002
+    F = 1e-3
+    mask = np.zeros((3, 3, 3), dtype=bool)
+    mask[:, :, 0] = True  # cells are x fastest: [z, y, x]
+    p = _cilia_problem((3, 3, 3), (F, 0.0, 0.0), remove=(mask, T))
+    m = O.OracleModel(p)
+    assert m.topology()["surface"].size == 26
+    dt = m.step_dt()
+    k = 0
+    t = 0.0
+    while not (T < t):
+        t += float(np.float32(dt))
+        k += 1
+    k += 1
+    m.step(k, dt)
+    vr, lr = m.removed()
+    assert (vr > 0).sum() == 9 and (lr > 0).sum() == 21
+    keep = vr == 0
+    assert m.topology()["surface"].size == 18
+    p1 = m.state()["linmom"][keep, 0].sum()
+    m.step(100, dt)
+    p2 = m.state()["linmom"][keep, 0].sum()
+    assert p2 - p1 == pytest.approx(100 * float(dt) * 18 * F, rel=1e-6)
