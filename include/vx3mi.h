@@ -58,6 +58,9 @@ typedef struct {
     int is_target;       /* isTarget */
     double thermal_on_after; /* TurnOnThermalExpansionAfterThisManySeconds */
     float red, green, blue, alpha; /* Display */
+    double cilia;            /* Cilia: multiplier of the voxel's cilia force (with enable_cilia) */
+    double cilia_on_after;   /* TurnOnCiliaAfterThisManySeconds */
+    double remove_after;     /* RemoveFromSimulationAfterThisManySeconds (with secondary_experiment; 0 never) */
 } vx3mi_material;
 
 /* Lattice description: the information content of a merged VXA (base.vxa +
@@ -79,6 +82,10 @@ typedef struct {
     int floor_enabled;             /* Environment/Gravity/FloorEnabled */
     int temp_enabled, vary_temp;   /* Environment/Thermal */
     double temp_base, temp_amplitude, temp_period;
+    int enable_cilia;              /* Simulator/EnableCilia */
+    int secondary_experiment;      /* Simulator/SecondaryExperiment (material removal, re-initialisation) */
+    const double *cilia_base;      /* per-voxel BaseCiliaForce (N, 3 per voxel, voxel order) or NULL */
+    const double *cilia_shift;     /* per-voxel ShiftCiliaForce (scaled by the voxel's signal) or NULL */
 } vx3mi_lattice;
 
 /* Settings a model carries besides topology and materials, read back by
@@ -93,6 +100,7 @@ typedef struct {
     float initial_temp;            /* temperature set at import (UpdateMatTemps) */
     int record_step_size, record_voxel, record_link; /* Simulator/RecordHistory */
     int has_stop_formula, has_fitness_formula;
+    int enable_cilia, secondary_experiment;
 } vx3mi_settings;
 
 typedef struct vx3mi_model vx3mi_model;
@@ -196,6 +204,12 @@ int vx3mi_set_state(vx3mi_batch *b, int robot, const double *pos3, const double 
                     const double *angmom3);
 /* per-voxel flag words (floor / static friction bits as the reference) */
 int vx3mi_get_voxel_flags(vx3mi_batch *b, int robot, int *flags);
+/* Material removal (VX3_VoxelyzeKernel::removeVoxels, VX3_VoxelyzeKernel.cu:365-399,
+ * with SecondaryExperiment): per voxel and link of the robot, the number of the
+ * step (counted from batch creation) at whose end it was removed, 0 while
+ * present.  Removed voxels keep their last state; their links are gone from
+ * both ends.  Batches that can lose voxels run in the plain and flat modes. */
+int vx3mi_batch_removed(vx3mi_batch *b, int robot, int *voxel_step, int *link_step);
 /* per-link flag words (bit 0 local velocity valid, bit 1 small-angle
  * formulation, VX3_Link.cu:117-131, 172-191) and axial strain of the last step,
  * link order; either pointer may be NULL */
@@ -264,6 +278,12 @@ int vx3mi_run(vx3mi_batch *b, vx3mi_model *const *models, int n, long long max_s
  * receiving the absolute voxel positions when that robot stopped */
 /* palette index (1-based, the reference's matid) of every voxel, voxel order */
 int vx3mi_model_palette_index(const vx3mi_model *m, int *out);
+/* palette entry index (1..nmat) as the model holds it */
+int vx3mi_model_material(const vx3mi_model *m, int index, vx3mi_material *out);
+/* per-voxel BaseCiliaForce / ShiftCiliaForce (3 per voxel, voxel order; zeros
+ * when the VXA has none), as CVX_Sim::Import gives them to the voxels
+ * (src/VXA/VX_Sim.cpp:102-105) */
+int vx3mi_model_cilia(const vx3mi_model *m, double *base3, double *shift3);
 /* Simulator/SavePositionOfAllVoxels of a model read from XML (0 otherwise) */
 int vx3mi_model_save_positions(const vx3mi_model *m);
 /* VXA.RawPrint of a model read from XML ("" otherwise); valid while the model lives */

@@ -402,3 +402,81 @@ def test_force_fields_fp32_and_persistent_refused():
     b.close()
     log(f"  fp32 force fields worst e_pos {worst:.3e}")
     assert worst <= 1e-5
+
+
+def _cilia_removal_problems():
+    """Three robots for one batch: cilia and removal together (collisions on, a
+    material removed mid-run, the other with cilia switched on after a delay),
+    cilia only, and a plain robot."""
+    rng = np.random.default_rng(21)
+    a = configs.random_robot(rng, (6, 5, 4), 0.8, enable_collision=1)
+    nv = int((np.asarray(a["cells"]) > 0).sum())
+    a["materials"] = [dict(configs.SOFT, cilia=1.5, cilia_on_after=0.002), dict(configs.STIFF, remove_after=0.003)]
+    a.update(enable_cilia=1, secondary_experiment=1, cilia_base=rng.normal(size=(nv, 3)) * 2e-3,
+             cilia_shift=rng.normal(size=(nv, 3)) * 1e-3)
+    b = configs.random_robot(rng, (4, 4, 3), 0.8, enable_collision=0)
+    nb = int((np.asarray(b["cells"]) > 0).sum())
+    b["materials"] = [dict(configs.SOFT, cilia=-0.7), dict(configs.STIFF, cilia=2.0)]
+    b.update(enable_cilia=1, cilia_base=rng.normal(size=(nb, 3)) * 2e-3)
+    c = configs.random_robot(rng, (4, 3, 3), 0.8, enable_collision=0)
+    return [a, b, c]
+
+
+@pytest.mark.parametrize("mode", [vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT])
+def test_cilia_and_removal_fp64_bit_exact(mode):
+    """Cilia forces (surface voxels, turn-on time, rotated with the voxel) and
+    SecondaryExperiment removal (a material removed mid-run with its links; the
+    exposed voxels join the collision and cilia surface) in one batch with a
+    plain robot: FP64 bit for bit with the oracle, removal steps equal."""
+    log(f"cilia/removal mode {mode}")
+    probs = _cilia_removal_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
+    oracles = [O.OracleModel(p) for p in probs]
+    for _ in range(6):
+        b.step(100, mode=mode)
+        for r, o in enumerate(oracles):
+            o.step(100)
+            s, ref = b.state(r), o.state()
+            for k in ("pos", "quat", "linmom", "angmom"):
+                assert np.array_equal(s[k], ref[k]), (r, k)
+            vr, lr = b.removed(r)
+            ovr, olr = o.removed()
+            assert np.array_equal(vr, ovr) and np.array_equal(lr, olr), r
+    vr, _ = b.removed(0)
+    assert (vr > 0).any() and (vr == 0).any()  # the removal happened, and not to everything
+    assert not b.removed(1)[0].any() and not b.removed(2)[0].any()
+    with pytest.raises(vx3mi.VX3MIError):
+        b.step(1, mode=vx3mi.MODE_PERSISTENT)  # removal needs a kernel between steps
+    b.close()
+
+
+def test_cilia_and_removal_fp32():
+    """The same batch in FP32 (flat mode) within 1e-5 of the oracle over 600 steps."""
+    probs = _cilia_removal_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP32)
+    oracles = [O.OracleModel(p) for p in probs]
+    worst = 0.0
+    for _ in range(6):
+        b.step(100)
+        for r, o in enumerate(oracles):
+            o.step(100)
+            s, ref = b.state(r), o.state()
+            worst = max(worst, pos_err(s["pos"], ref["pos"]), quat_err(s["quat"], ref["quat"]))
+            assert np.array_equal(b.removed(r)[0], o.removed()[0]), r
+    b.close()
+    log(f"  fp32 cilia/removal worst {worst:.3e}")
+    assert worst <= TOL[vx3mi.FP32]
+
+
+def test_cilia_in_persistent_mode_fp64():
+    """Cilia without removal run in the persistent kernel too, bit for bit."""
+    probs = _cilia_removal_problems()[1:]
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
+    b.configure(vx3mi.MODE_PERSISTENT, 0, 256)
+    oracles = [O.OracleModel(p) for p in probs]
+    b.step(400)
+    for r, o in enumerate(oracles):
+        o.step(400)
+        s, ref = b.state(r), o.state()
+        assert np.array_equal(s["pos"], ref["pos"]) and np.array_equal(s["angmom"], ref["angmom"]), r
+    b.close()

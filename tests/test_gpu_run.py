@@ -132,3 +132,47 @@ def test_cli_report(tmp_path):
         c1 = [float(e.find(f"currentCenterOfMass/{a}").text) for a in "xyz"]
         assert float(e.find("fitness_score").text) == pytest.approx(c1[0] - c0[0], rel=1e-6, abs=1e-12)
         assert float(e.find("currentTime").text) > 0.002
+
+
+def test_history_frames_follow_removal():
+    """.history with SecondaryExperiment removal: the frame of the step that
+    removes the x = 0 face of a 3x3x3 cube lists the old surface minus the
+    removed voxels; later frames list the regenerated surface (the former
+    centre included); removed links are left out from that frame on."""
+    n = 27
+    cells = np.ones(n, dtype=np.int8)
+    cells.reshape(3, 3, 3)[:, :, 0] = 2
+    prob = dict(configs.BASE, dims=(3, 3, 3), cells=cells, phase=None, enable_collision=0, vary_temp=0,
+                secondary_experiment=1, materials=[configs.SOFT, dict(configs.SOFT, remove_after=0.0)])
+    m0 = vx3mi.Model.from_problem(prob)
+    dt = float(np.float32(m0.settings()["dt_frac"] * m0.recommended_dt()))
+    rs = int(1 / (10000 * m0.recommended_dt() * m0.settings()["dt_frac"])) + 1  # real_stepsize of RecordStepSize 1
+    k = 4 * rs + 1  # removal at the end of step k (1-based): frame index k - 1 shows it
+    t = [0.0]
+    for _ in range(k):
+        t.append(t[-1] + dt)
+    prob["materials"][1]["remove_after"] = 0.5 * (t[k - 2] + t[k - 1])
+    text = W.write_vxa(prob, stop=("mtSUB", ("mtVAR", "t"), ("mtCONST", t[k] + 6 * rs * dt)), record=(1, 1, 1))
+    m = vx3mi.Model.from_xml(text)
+    b = vx3mi.Batch([m], vx3mi.FP64)
+    log("runner: history with removal")
+    res, hist, _ = vx3mi.run(b, [m])
+    assert res[0]["real_stepsize"] == rs
+    vr, lr = b.removed(0)
+    assert (vr == k).sum() == 9 and (lr == k).sum() == 21
+    frames = re.findall(r"<<<Step(\d+) Time:[^>]*>>>(.*?)<<<>>>\|\[\[\[\d+\]\]\](.*?)\[\[\[\]\]\]", hist[0])
+    assert len(frames) > 6
+    seen = set()
+    for j, vox, links in frames:
+        j = int(j)
+        nvox, nlink = vox.count(";"), links.count(";")
+        if j < k - 1:
+            want = (26, 54)
+        elif j == k - 1:
+            want = (17, 33)
+        else:
+            want = (18, 33)
+        assert (nvox, nlink) == want, j
+        seen.add(want)
+    assert seen == {(26, 54), (17, 33), (18, 33)}
+    b.close()

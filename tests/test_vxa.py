@@ -412,3 +412,48 @@ def test_force_fields_are_read_per_axis():
         deep = ("mtADD", ("mtVAR", "x"), deep)
     with pytest.raises(vx3mi.VX3MIError, match="error -2"):
         vx3mi.Model.from_xml(W.write_vxa(dict(configs.config0(), force_field={"y": deep})))
+
+
+def _cilia_problem(seed=0):
+    p = configs.config1(seed=seed, dims=(5, 4, 3))
+    rng = np.random.default_rng(seed)
+    nv = int((np.asarray(p["cells"]) > 0).sum())
+    mats = [dict(m) for m in p["materials"]]
+    mats[0].update(cilia=2.5, cilia_on_after=0.01)
+    mats[1].update(remove_after=0.02)
+    return dict(p, materials=mats, enable_cilia=1, secondary_experiment=1,
+                cilia_base=rng.normal(size=(nv, 3)) * 1e-3, cilia_shift=rng.normal(size=(nv, 3)) * 1e-4)
+
+
+def test_cilia_and_removal_keys_are_read():
+    """Simulator/EnableCilia and SecondaryExperiment, the materials' Cilia,
+    TurnOnCiliaAfterThisManySeconds and RemoveFromSimulationAfterThisManySeconds,
+    and the BaseCiliaForce / ShiftCiliaForce layers (three values per cell) give
+    the model the lattice entry point builds from the same problem."""
+    p = _cilia_problem()
+    a = vx3mi.Model.from_xml(W.write_vxa(p))
+    b = vx3mi.Model.from_problem(p)
+    assert_same_model(a, b)
+    for m in (a, b):
+        st = m.settings()
+        assert st["enable_cilia"] == 1 and st["secondary_experiment"] == 1
+        assert m.material(1)["cilia"] == 2.5 and m.material(1)["cilia_on_after"] == 0.01
+        assert m.material(2)["remove_after"] == 0.02 and m.material(2)["cilia"] == 0.0
+        base, shift = m.cilia()
+        assert np.array_equal(base, p["cilia_base"]) and np.array_equal(shift, p["cilia_shift"])
+
+
+def test_short_cilia_layers_pad_zeros():
+    """Cells past the end of a BaseCiliaForce / ShiftCiliaForce layer get zeros
+    (VX_Object.cpp:1906-1908 pads them).  Values missing from a cell's partial
+    triple are read as zero (the reference reads past its data there)."""
+    text = DEMO_LIKE.replace("</Data>", "</Data><BaseCiliaForce><Layer><![CDATA[]]></Layer>"
+                                        "<Layer><![CDATA[0.5,-1,2]]></Layer></BaseCiliaForce>"
+                                        "<ShiftCiliaForce><Layer><![CDATA[]]></Layer>"
+                                        "<Layer><![CDATA[3]]></Layer></ShiftCiliaForce>")
+    text = text.replace("<Simulator>", "<Simulator><EnableCilia>1</EnableCilia>")
+    m = vx3mi.Model.from_xml(text)
+    base, shift = m.cilia()
+    assert base.tolist() == [[0.5, -1.0, 2.0]]
+    assert shift.tolist() == [[3.0, 0.0, 0.0]]
+    assert m.settings()["enable_cilia"] == 1

@@ -14,7 +14,8 @@ namespace vx3mi {
 struct RobotParams {
     double tempAmp, tempPeriod;
     double tempInvPeriod;  // 1 / tempPeriod (the FP32 path multiplies)
-    int varyTemp, enableExpansion, collisions, pad;
+    int varyTemp, enableExpansion, collisions;
+    int secondary;           // SecondaryExperiment: materials are removed after their time
     int ffOff[3], ffLen[3];  // force-field postfix programs in the formula table (length 0: none)
     int voff, vcount, loff, lcount;
     // collision broadphase: surface voxels [soff, soff+scount) of the surface
@@ -58,6 +59,7 @@ struct BatchBase {
     int lastMode = 0;
     int S = 0;                   // surface voxels (collision candidates) of the batch
     bool anyCollisions = false;
+    bool anyRemoval = false;      // some robot can lose voxels (SecondaryExperiment)
     long tableBuckets = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -93,6 +95,10 @@ struct BatchBase {
     virtual int set_state(int robot, const double *pos, const double *quat, const double *lm, const double *am) = 0;
     virtual int get_flags(int robot, int *flags) = 0;
     virtual int get_link_flags(int robot, int *flags, float *strain) = 0;
+    // removal state: per voxel / link of the robot the step count (the robot's
+    // steps since the batch was created) at whose end it was removed, 0 while
+    // present
+    virtual int get_removed(int robot, int *vstep, int *lstep) = 0;
     virtual int eval_links(double *out) = 0;
     virtual int com(double *out) = 0;
     virtual int recommended(double *out) = 0;

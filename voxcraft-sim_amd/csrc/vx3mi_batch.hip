@@ -112,6 +112,12 @@ int vx3mi_get_link_flags(vx3mi_batch *b, int robot, int *flags, float *strain) {
     return b->b->get_link_flags(robot, flags, strain);
 }
 
+int vx3mi_batch_removed(vx3mi_batch *b, int robot, int *voxel_step, int *link_step) {
+    int rc = check_robot(b, robot);
+    if (rc) return rc;
+    return b->b->get_removed(robot, voxel_step, link_step);
+}
+
 static int read_rt(vx3mi_batch *b, std::vector<RobotTime> &rt) {
     rt.resize(b->b->R);
     hipError_t e = hipStreamSynchronize(b->b->stream);

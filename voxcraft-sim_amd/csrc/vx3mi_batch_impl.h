@@ -37,7 +37,7 @@ template <typename T> struct Dev {
     T4<T> *pq0, *pq1, *mo0, *mo1;
     const int *lat, *vmat, *vrob;
     const T *phase;
-    const int *vlinks;
+    int *vlinks;           // edited only by removal
     const int2 *lends, *lmeta;
     T4<T> *ls0, *ls1, *ls2;
     T4<T> *lf0, *lf1, *lf2;
@@ -57,6 +57,13 @@ template <typename T> struct Dev {
     int *ctab[2];          // bucket entries (kBucketCap per bucket): surface indices
     T4<T> *contactW;
     long tableBuckets;
+    const VoxelCilia *cilia;  // per-voxel cilia, or nullptr when no robot has EnableCilia
+    // removal (SecondaryExperiment): step count at whose end a voxel / link was
+    // removed (0 present), or nullptr when no robot can lose voxels
+    int *vrem, *lrem;
+    const int *remCand;       // voxels of materials with RemoveFromSimulationAfterThisManySeconds > 0
+    const double *remAt;      // that time, per candidate
+    int nRemCand;
 };
 
 // ------------------------------------------------------------------ device
@@ -71,6 +78,7 @@ template <typename T> __device__ __forceinline__ void link_update(const Dev<T> &
     const int2 meta = a.lmeta[l];
     RobotTime *rt = a.rt + meta.y;
     if (rt->diverged) return;
+    if (a.lrem && a.lrem[l]) return;  // removed link (gpu_update_links :570-571)
     const LinkMat &lm = a.lmats[meta.x >> 2];
     if (lm.fixedBoth) return;  // gpu_update_links VX3_VoxelyzeKernel.cu:572-573
     const int axis = meta.x & 3;
@@ -109,15 +117,17 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
         rt->prevDt[parity ^ 1] = dt;
         rt->steps += 1;
     }
+    if (a.vrem && a.vrem[v]) return;  // removed voxel (gpu_update_voxels :586-587)
     const VoxelMat &vm = a.vmats[a.vmat[v]];
     if (vm.fixed) return;  // gpu_update_voxels :588-589
 
     const int *vl = a.vlinks + 6 * v;
     V3<T> fs = {0, 0, 0}, ms = {0, 0, 0};
+    bool surface = false;  // fewer than six links (VX3_Voxel::updateSurface, VX3_Voxel.cu:515-524)
 #pragma unroll
     for (int d = 0; d < 6; d++) {
         const int l = vl[d];
-        if (l < 0) continue;
+        if (l < 0) { surface = true; continue; }
         if (d & 1) {  // link in a NEG direction: this voxel is its pVPos end
             const T4<T> f1 = a.lf1[l], f2 = a.lf2[l];
             fs = fs + V3<T>{f1.z, f1.w, f2.x};
@@ -146,7 +156,19 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
         const T4<T> f = a.field[v];
         field = {f.x, f.y, f.z};
     }
-    voxel_step<T>(vm, a.exts, dt, a.lat[v], temp, fs, ms, contact, field, s);
+    V3<T> cil = {0, 0, 0};
+    if (a.cilia) {
+        // gpu_update_cilia_force (VX3_VoxelyzeKernel.cu:846-858): surface voxels of a
+        // cilia material once TurnOnCiliaAfterThisManySeconds has passed, with the
+        // orientation entering the step; localSignal is 0 (signals are not built)
+        const VoxelCilia &c = a.cilia[v];
+        if (surface && c.coef != 0 && !(c.onAfter > rt->time[parity])) {
+            const V3<T> b = {(T)(c.base[0] + 0.0 * c.shift[0]), (T)(c.base[1] + 0.0 * c.shift[1]),
+                             (T)(c.base[2] + 0.0 * c.shift[2])};
+            cil = qrot(s.q, b) * (T)c.coef;
+        }
+    }
+    voxel_step<T>(vm, a.exts, dt, a.lat[v], temp, fs, ms, contact, field, s, a.cilia ? &cil : nullptr);
     if (rp.varyTemp && !(vm.thermalOnAfter > tnext)) {
         if constexpr (std::is_same<T, float>::value)
             temp = actuation_temperature_f32(rp.tempAmp, rp.tempInvPeriod, tnext, (double)a.phase[v], rp.enableExpansion);
@@ -193,11 +215,23 @@ template <typename T> __device__ __forceinline__ V3<T> abs_position(const Dev<T>
     return V3<T>{(T)(pk & 1023) * L + q0.x, (T)((pk >> 10) & 1023) * L + q0.y, (T)(pk >> 20) * L + q0.z};
 }
 
+// a voxel of the surface list that currently takes part in collisions: with
+// removal the list also holds voxels that become surface when a neighbour goes
+// (regenerateSurfaceVoxels VX3_VoxelyzeKernel.cu:495-513 keeps surface voxels
+// that are not removed)
+template <typename T> __device__ __forceinline__ bool col_active(const Dev<T> &a, int v) {
+    if (!a.vrem) return true;
+    if (a.vrem[v]) return false;
+    bool surface = false;
+    for (int d = 0; d < 6; d++) surface |= a.vlinks[6 * v + d] < 0;
+    return surface;
+}
+
 template <typename T> __device__ __forceinline__ void col_insert(const Dev<T> &a, int s, int parity) {
     const int v = a.surf[s];
     RobotTime *rt = a.rt + a.vrob[v];
     const RobotParams &rp = a.rp[a.vrob[v]];
-    if (!rp.collisions || rt->diverged) return;
+    if (!rp.collisions || rt->diverged || !col_active(a, v)) return;
     const V3<T> p = abs_position(a, v, a.pq0[v]);
     const double h = rp.cellSize;
     const int cx = (int)floor((double)p.x / h), cy = (int)floor((double)p.y / h), cz = (int)floor((double)p.z / h);
@@ -213,7 +247,7 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
     const int v = a.surf[s];
     RobotTime *rt = a.rt + a.vrob[v];
     const RobotParams &rp = a.rp[a.vrob[v]];
-    if (!rp.collisions || rt->diverged) return;
+    if (!rp.collisions || rt->diverged || !col_active(a, v)) return;
     const VoxelMat &mi = a.vmats[a.vmat[v]];
     const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v];
     const V3<T> pi = abs_position(a, v, qi0);
@@ -314,6 +348,31 @@ template <typename T> __global__ __launch_bounds__(256) void k_links(Dev<T> a, i
 template <typename T> __global__ __launch_bounds__(256) void k_voxels(Dev<T> a, int v0, int n, int parity) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) voxel_update(a, v0 + i, parity);
+}
+
+// VX3_VoxelyzeKernel::removeVoxels (VX3_VoxelyzeKernel.cu:365-399), at the end
+// of a step before currentTime advances (:336-343): a voxel whose material's
+// RemoveFromSimulationAfterThisManySeconds lies below the step's start time is
+// removed with its links, which both ends forget.  Two linked voxels removed in
+// the same step write the same values, in either order.
+template <typename T> __global__ __launch_bounds__(256) void k_removal(Dev<T> a, int parity) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nRemCand) return;
+    const int v = a.remCand[i];
+    const int r = a.vrob[v];
+    const RobotTime *rt = a.rt + r;
+    if (!a.rp[r].secondary || rt->diverged || rt->dt == 0.0f || a.vrem[v]) return;
+    if (!(a.remAt[i] < rt->time[parity])) return;
+    const int stamp = rt->steps;
+    a.vrem[v] = stamp;
+    for (int d = 0; d < 6; d++) {
+        const int l = a.vlinks[6 * v + d];
+        if (l < 0) continue;
+        a.lrem[l] = stamp;
+        const int o = (d & 1) ? a.lends[l].x : a.lends[l].y;
+        a.vlinks[6 * o + (d ^ 1)] = -1;
+        a.vlinks[6 * v + d] = -1;
+    }
 }
 
 // One workgroup per robot; nsteps steps with two barrier-separated phases.
@@ -419,6 +478,10 @@ template <typename T> struct Batch : BatchBase {
     } plans[2];                   // large and small tile configuration
     bool tileOK = false;          // some tile plan exists
     bool anyForceField = false;   // some robot has a force field (plain and flat modes only)
+    bool anyCilia = false;        // some robot has EnableCilia (not in the tiles)
+    // plain and flat modes only: force fields (their own kernel) and removal (a
+    // kernel between steps)
+    bool plainFlatOnly() const { return anyForceField || anyRemoval; }
     // launch configuration used by VX3MI_MODE_AUTO (set by the autotuner)
     int autoMode = 0, tileSel = 0, persistThreads = 256;
     int nLinkMats = 0;  // entries of the device link material table
@@ -442,6 +505,9 @@ template <typename T> struct Batch : BatchBase {
         std::vector<CurvePt> curves;
         std::vector<MtToken> formulas;
         std::vector<T4<T>> pq0, pq1;
+        std::vector<VoxelCilia> cilia;
+        std::vector<int> remCand;
+        std::vector<double> remAt;
         std::map<std::string, int> vmIndex, lmIndex, curveIndex;
         // materials are deduplicated across robots (byte-identical constants);
         // a voxel with an external gets an entry of its own material whose
@@ -471,7 +537,9 @@ template <typename T> struct Batch : BatchBase {
             p.varyTemp = m.varyTemp;
             p.enableExpansion = m.enableExpansion;
             p.collisions = m.enableCollision;
+            p.secondary = m.secondaryExperiment;
             p.voff = voff; p.vcount = m.nv(); p.loff = loff; p.lcount = m.nl();
+            if (m.enableCilia) anyCilia = true;
             for (int ax = 0; ax < 3; ax++) {
                 std::vector<MtToken> post;
                 if (!m.forceField[ax].empty() && !mt_postfix(m.forceField[ax], post)) {
@@ -525,6 +593,19 @@ template <typename T> struct Batch : BatchBase {
                 if (kAbsolutePositions<T>) pq0.push_back(T4<T>{(T)(m.ix[v] * L0), (T)(m.iy[v] * L0), (T)(m.iz[v] * L0), 1});
                 else pq0.push_back(T4<T>{0, 0, 0, 1});
                 pq1.push_back(T4<T>{0, 0, 0, (T)t0});
+                const vx3mi_material &pm = m.palette[m.vpal[v] - 1];
+                VoxelCilia c{};
+                for (int k = 0; k < 3; k++) {
+                    c.base[k] = m.ciliaBase.empty() ? 0.0 : m.ciliaBase[(size_t)3 * v + k];
+                    c.shift[k] = m.ciliaShift.empty() ? 0.0 : m.ciliaShift[(size_t)3 * v + k];
+                }
+                c.coef = m.enableCilia ? pm.cilia : 0.0;
+                c.onAfter = pm.cilia_on_after;
+                cilia.push_back(c);
+                if (m.secondaryExperiment && pm.remove_after > 0) {
+                    remCand.push_back(voff + v);
+                    remAt.push_back(pm.remove_after);
+                }
             }
             for (int l = 0; l < m.nl(); l++) {
                 lends.push_back(int2{voff + m.lneg[l], voff + m.lpos[l]});
@@ -536,6 +617,7 @@ template <typename T> struct Batch : BatchBase {
         V = (int)lat.size();
         L = (int)lends.size();
         nLinkMats = (int)lmats.size();
+        anyRemoval = !remCand.empty();
         hlat = lat;
         hvmat = vmat;
 
@@ -609,9 +691,26 @@ template <typename T> struct Batch : BatchBase {
             const Model &m = models[r]->m;
             RobotParams &p = hrp[r];
             p.soff = (int)surf.size();
-            p.scount = m.enableCollision ? (int)m.surface.size() : 0;
+            // with removal the list also holds the voxels linked to a removable
+            // one (they become surface when it goes), in voxel order
+            std::vector<int> rs;
+            if (m.enableCollision) {
+                std::vector<char> in((size_t)m.nv(), 0);
+                for (int v : m.surface) in[v] = 1;
+                if (m.secondaryExperiment)
+                    for (int v = 0; v < m.nv(); v++) {
+                        if (!(m.palette[m.vpal[v] - 1].remove_after > 0)) continue;
+                        for (int k = 0; k < 6; k++) {
+                            const int l = m.vlinks[(size_t)v * 6 + k];
+                            if (l >= 0) in[(k & 1) ? m.lneg[l] : m.lpos[l]] = 1;
+                        }
+                    }
+                for (int v = 0; v < m.nv(); v++)
+                    if (in[v]) rs.push_back(v);
+            }
+            p.scount = (int)rs.size();
             if (p.scount) {
-                for (int v : m.surface) surf.push_back(p.voff + v);
+                for (int v : rs) surf.push_back(p.voff + v);
                 int hsz = 64;
                 while (hsz < 2 * p.scount) hsz <<= 1;
                 p.toff = (int)buckets;
@@ -662,6 +761,29 @@ template <typename T> struct Batch : BatchBase {
         }
         d.vmats = d_vmats; d.exts = d_exts; d.lmats = d_lmats; d.curves = d_curves; d.formulas = d_formulas;
         d.rp = d_rp; d.rt = d_rt;
+        d.cilia = nullptr;
+        if (anyCilia) {
+            VoxelCilia *cd;
+            if ((rc = alloc(&cd, V))) return rc;
+            HIP_TRY(hipMemcpy(cd, cilia.data(), sizeof(VoxelCilia) * V, hipMemcpyHostToDevice));
+            d.cilia = cd;
+        }
+        d.vrem = d.lrem = nullptr;
+        d.remCand = nullptr;
+        d.remAt = nullptr;
+        d.nRemCand = 0;
+        if (anyRemoval) {
+            int *vr, *lr, *rc0;
+            double *ra;
+            const int nc = (int)remCand.size();
+            if ((rc = alloc(&vr, V)) || (rc = alloc(&lr, L)) || (rc = alloc(&rc0, nc)) || (rc = alloc(&ra, nc))) return rc;
+            HIP_TRY(hipMemset(vr, 0, sizeof(int) * V));
+            HIP_TRY(hipMemset(lr, 0, sizeof(int) * std::max(L, 1)));
+            HIP_TRY(hipMemcpy(rc0, remCand.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(ra, remAt.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+            HIP_TRY(hipDeviceSynchronize());
+            d.vrem = vr; d.lrem = lr; d.remCand = rc0; d.remAt = ra; d.nRemCand = nc;
+        }
         return plan_tiles();
     }
 
@@ -673,7 +795,7 @@ template <typename T> struct Batch : BatchBase {
         using C = TileCfg<T, SMALL>;
         TilePlan &P = plans[SMALL];
         P.ok = false;
-        if (anyCollisions || anyForceField || R < 2) return VX3MI_OK;
+        if (anyCollisions || anyForceField || anyCilia || anyRemoval || R < 2) return VX3MI_OK;
         long totL = 0, totV = 0;
         for (const RobotParams &p : hrp) {
             if (p.vcount > C::VMAX || p.lcount > C::LMAX) return VX3MI_OK;
@@ -729,7 +851,7 @@ template <typename T> struct Batch : BatchBase {
         tileOK = plans[0].ok || plans[1].ok;
         tileSel = plans[0].ok ? 0 : 1;
         autoMode = tileOK ? VX3MI_MODE_TILED
-                          : ((R > 1 && V / R <= 4096 && !anyForceField) ? VX3MI_MODE_PERSISTENT : VX3MI_MODE_FLAT);
+                          : ((R > 1 && V / R <= 4096 && !plainFlatOnly()) ? VX3MI_MODE_PERSISTENT : VX3MI_MODE_FLAT);
         return VX3MI_OK;
     }
 
@@ -770,8 +892,11 @@ template <typename T> struct Batch : BatchBase {
         if (anyCollisions) k_col_contact<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
         if (anyForceField) k_force_field<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         k_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
+        if (anyRemoval) k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
     }
-    int kernels_per_flat_step() const { return 2 + (anyCollisions ? 2 : 0) + (anyForceField ? 1 : 0); }
+    int kernels_per_flat_step() const {
+        return 2 + (anyCollisions ? 2 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0);
+    }
 
     int step(int nsteps, float dt, int mode) override {
         if (nsteps <= 0) return VX3MI_OK;
@@ -779,12 +904,12 @@ template <typename T> struct Batch : BatchBase {
         int rc = set_dt(dt);
         if (rc) return rc;
         if (mode == VX3MI_MODE_TILED && !plans[tileSel].ok) {
-            set_error("tiled mode needs collisions off and every robot within one tile");
+            set_error("tiled mode needs collisions, force fields, cilia and removal off and every robot within one tile");
             return VX3MI_E_ARG;
         }
         if (mode == VX3MI_MODE_AUTO) mode = autoMode;
-        if (mode == VX3MI_MODE_PERSISTENT && anyForceField) {
-            set_error("force fields run in the plain and flat modes only");
+        if (mode == VX3MI_MODE_PERSISTENT && plainFlatOnly()) {
+            set_error("force fields and material removal run in the plain and flat modes only");
             return VX3MI_E_ARG;
         }
         HIP_TRY(hipEventRecord(ev0, stream));
@@ -800,6 +925,10 @@ template <typename T> struct Batch : BatchBase {
                     if (anyForceField) k_force_field<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
                     k_voxels<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
                     launches += 2 + (p.scount ? 2 : 0) + (anyForceField ? 1 : 0);
+                }
+                if (anyRemoval) {
+                    k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
+                    launches++;
                 }
                 parity ^= 1;
             }
@@ -926,6 +1055,11 @@ template <typename T> struct Batch : BatchBase {
             bufs.push_back({d.ccnt[1], sizeof(int) * (size_t)tableBuckets});
             bufs.push_back({d.contactW, sizeof(T4<T>) * V});
         }
+        if (anyRemoval) {
+            bufs.push_back({d.vlinks, sizeof(int) * 6 * (size_t)V});
+            bufs.push_back({d.vrem, sizeof(int) * (size_t)V});
+            bufs.push_back({d.lrem, sizeof(int) * (size_t)L});
+        }
         size_t total = 0;
         for (const Buf &b : bufs) total += (b.bytes + 255) & ~(size_t)255;
         char *snap = nullptr;
@@ -952,7 +1086,7 @@ template <typename T> struct Batch : BatchBase {
         for (int c = 0; rc == VX3MI_OK && c < kCandidates; c++) {
             ms[c] = -1;
             if (modes[c] == VX3MI_MODE_TILED && !plans[tiles[c]].ok) continue;
-            if (modes[c] == VX3MI_MODE_PERSISTENT && anyForceField) continue;
+            if (modes[c] == VX3MI_MODE_PERSISTENT && plainFlatOnly()) continue;
             tileSel = tiles[c];
             if (threads[c]) persistThreads = threads[c];
             if ((rc = step(std::min(trialSteps, 64), -1.0f, modes[c])) < 0) break;  // code load, graph capture
@@ -994,8 +1128,8 @@ template <typename T> struct Batch : BatchBase {
             return VX3MI_E_ARG;
         }
         if (mode < VX3MI_MODE_PLAIN || mode > VX3MI_MODE_TILED) { set_error("invalid mode"); return VX3MI_E_ARG; }
-        if (mode == VX3MI_MODE_PERSISTENT && anyForceField) {
-            set_error("force fields run in the plain and flat modes only");
+        if (mode == VX3MI_MODE_PERSISTENT && plainFlatOnly()) {
+            set_error("force fields and material removal run in the plain and flat modes only");
             return VX3MI_E_ARG;
         }
         autoMode = mode;
@@ -1050,6 +1184,23 @@ template <typename T> struct Batch : BatchBase {
             if (flags) flags[l - l0] = (int)s2[l].z;
             if (strain) strain[l - l0] = (float)s2[l].w;
         }
+        return VX3MI_OK;
+    }
+
+    int get_removed(int robot, int *vstep, int *lstep) override {
+        int v0 = 0, v1 = V, l0 = 0, l1 = L;
+        if (robot >= 0) {
+            v0 = hrp[robot].voff; v1 = v0 + hrp[robot].vcount;
+            l0 = hrp[robot].loff; l1 = l0 + hrp[robot].lcount;
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (!anyRemoval) {
+            if (vstep) std::fill(vstep, vstep + (v1 - v0), 0);
+            if (lstep) std::fill(lstep, lstep + (l1 - l0), 0);
+            return VX3MI_OK;
+        }
+        if (vstep && v1 > v0) HIP_TRY(hipMemcpy(vstep, d.vrem + v0, sizeof(int) * (v1 - v0), hipMemcpyDeviceToHost));
+        if (lstep && l1 > l0) HIP_TRY(hipMemcpy(lstep, d.lrem + l0, sizeof(int) * (l1 - l0), hipMemcpyDeviceToHost));
         return VX3MI_OK;
     }
 

@@ -155,6 +155,12 @@ template <typename T> struct DiffBatch : DiffBase {
                     set_error("differentiable mode: voxel externals (boundary conditions) are not supported");
                     return VX3MI_E_ARG;
                 }
+            bool removal = false;
+            for (const vx3mi_material &pm : m.palette) removal |= m.secondaryExperiment && pm.remove_after > 0;
+            if (m.enableCilia || removal) {
+                set_error("differentiable mode: cilia and material removal are not supported");
+                return VX3MI_E_ARG;
+            }
             RobotParams &p = hrp[r];
             std::memset(&p, 0, sizeof(p));
             p.voff = (int)hvm.size();

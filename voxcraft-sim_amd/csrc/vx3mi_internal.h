@@ -61,6 +61,8 @@ struct Model {
     int savePositions = 0;          // Simulator/SavePositionOfAllVoxels
     int secondaryExperiment = 0;
     double reinitInitialPositionAfter = 0.0;
+    int enableCilia = 0;             // Simulator/EnableCilia
+    std::vector<double> ciliaBase, ciliaShift;  // 3 per voxel (empty: zeros)
     int nv() const { return (int)ix.size(); }
     int ext_of(int v) const { return vext.empty() ? -1 : vext[v]; }
     int nl() const { return (int)lneg.size(); }

@@ -250,6 +250,8 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
     m.tempPeriod = sp->temp_period;
     m.tempBase = sp->temp_base;
     m.palette.assign(sp->mats, sp->mats + sp->nmat);
+    m.enableCilia = sp->enable_cilia ? 1 : 0;
+    m.secondaryExperiment = sp->secondary_experiment ? 1 : 0;
 
     // UpdateMatTemps (VX_Sim.cpp:467-483) via UpdateCurTemp (VX_Environment.cpp:327-346)
     float t0 = 0.0f;
@@ -330,6 +332,9 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
         m.phase.push_back(sp->phase ? sp->phase[nv] : 0.0);
         nv++;
     }
+    // per-voxel cilia forces (CVX_Sim::Import copies them to each voxel, VX_Sim.cpp:102-105)
+    if (sp->cilia_base) m.ciliaBase.assign(sp->cilia_base, sp->cilia_base + (size_t)3 * nv);
+    if (sp->cilia_shift) m.ciliaShift.assign(sp->cilia_shift, sp->cilia_shift + (size_t)3 * nv);
     m.vlinks.assign((size_t)nv * 6, -1);
     // links: created when the later voxel of a pair is added (addVoxel's
     // X_POS..Z_NEG loop only finds lower neighbours), pVNeg = lower coordinate
@@ -472,6 +477,8 @@ int vx3mi_model_settings(const vx3mi_model *m, vx3mi_settings *o) {
     o->record_link = M.recordLink;
     o->has_stop_formula = !M.stopFormula.empty();
     o->has_fitness_formula = !M.fitnessFormula.empty();
+    o->enable_cilia = M.enableCilia;
+    o->secondary_experiment = M.secondaryExperiment;
     return VX3MI_OK;
 }
 
@@ -481,6 +488,24 @@ int vx3mi_model_save_positions(const vx3mi_model *m) { return m ? m->m.savePosit
 int vx3mi_model_palette_index(const vx3mi_model *m, int *out) {
     if (!m || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
     for (int v = 0; v < m->m.nv(); v++) out[v] = m->m.vpal[v];
+    return VX3MI_OK;
+}
+
+int vx3mi_model_cilia(const vx3mi_model *m, double *base3, double *shift3) {
+    if (!m) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    const Model &M = m->m;
+    const size_t n = (size_t)3 * M.nv();
+    for (size_t i = 0; i < n; i++) {
+        if (base3) base3[i] = M.ciliaBase.empty() ? 0.0 : M.ciliaBase[i];
+        if (shift3) shift3[i] = M.ciliaShift.empty() ? 0.0 : M.ciliaShift[i];
+    }
+    return VX3MI_OK;
+}
+
+int vx3mi_model_material(const vx3mi_model *m, int index, vx3mi_material *out) {
+    if (!m || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    if (index < 1 || index > (int)m->m.palette.size()) { set_error("palette index out of range"); return VX3MI_E_ARG; }
+    *out = m->m.palette[index - 1];
     return VX3MI_OK;
 }
 

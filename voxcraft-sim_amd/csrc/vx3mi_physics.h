@@ -94,6 +94,16 @@ struct VoxelExt {
     int pad;
 };
 
+// Cilia of a voxel (VX3_Voxel.h:298-300, read from the VXA's BaseCiliaForce /
+// ShiftCiliaForce layers): the force is coef * R(q) (base + localSignal *
+// shift) on surface voxels once the time reaches onAfter
+// (gpu_update_cilia_force VX3_VoxelyzeKernel.cu:846-858); coef is the
+// material's Cilia, 0 for voxels of robots without EnableCilia.
+struct VoxelCilia {
+    double base[3], shift[3];
+    double coef, onAfter;
+};
+
 // Link material (VX3_MaterialLink.h:27-36) plus per-end constants the link
 // kernel needs so it never touches the voxel material table.
 struct LinkMat {
@@ -674,7 +684,7 @@ VX_HD V3<T> force_field(const MtToken *f, const int *off, const int *len, double
 }
 template <typename T>
 VX_HD void voxel_step(const VoxelMat &vm, const VoxelExt *exts, float dtf, int lat, float temp, V3<T> fLink,
-                      V3<T> mLink, V3<T> contact, V3<T> field, VoxelState<T> &s) {
+                      V3<T> mLink, V3<T> contact, V3<T> field, VoxelState<T> &s, const V3<T> *cilia = nullptr) {
     if (dtf == 0.0f) return;
     const T dt = (T)dtf;
     const int iz = lat >> 20;
@@ -697,6 +707,7 @@ VX_HD void voxel_step(const VoxelMat &vm, const VoxelExt *exts, float dtf, int l
     tf = tf - vel * (T)vm.gDampT;
     tf.z += vm.gravF;
     tf = tf - contact;
+    if (cilia) tf = tf + *cilia;  // CiliaForce * mat->Cilia (VX3_Voxel.cu:376-377)
     // force field (VX3_Voxel.cu:177-183); zeros without one, as the reference's
     // constant-zero default trees (VX3_ForceField.h:11-20) add +0.0
     tf.x += field.x; tf.y += field.y; tf.z += field.z;

@@ -106,14 +106,36 @@ std::string history_header(const Model &m) {
 }
 
 // one .history frame (VX3_SimulationManager.cu:61-111) from the robot's
-// positions, orientations, temperatures and link strains
+// positions, orientations, temperatures and link strains.  With removal
+// (vrem / lrem: the step count at whose end a voxel / link went, 0 present;
+// stampNow: the step count after the frame's step) the voxels listed are those
+// of the surface list regenerated at the start of the frame's step that are
+// still present, and removed links are left out.
 std::string history_frame(const Model &m, long long j, double t, const double *pos, const double *q,
-                          const float *temp, const float *strain) {
+                          const float *temp, const float *strain, const int *vrem = nullptr,
+                          const int *lrem = nullptr, long long stampNow = 0) {
     std::string s;
     const double vs = 1 / 0.001;
+    auto link_at = [&](int v, int d) {  // the voxel's link in direction d now, -1 none
+        const int l = m.vlinks[(size_t)v * 6 + d];
+        return (l >= 0 && lrem && lrem[l]) ? -1 : l;
+    };
+    std::vector<int> listed;
+    if (vrem) {
+        for (int v = 0; v < m.nv(); v++) {
+            if (vrem[v]) continue;
+            bool surface = false;
+            for (int d = 0; d < 6; d++) {
+                const int l = m.vlinks[(size_t)v * 6 + d];
+                surface |= l < 0 || (lrem[l] != 0 && lrem[l] < stampNow);
+            }
+            if (surface) listed.push_back(v);
+        }
+    }
+    const std::vector<int> &voxels = vrem ? listed : m.surface;
     if (m.recordVoxel) {
         appendf(s, "<<<Step%lld Time:%f>>>", j, t);
-        for (int v : m.surface) {
+        for (int v : voxels) {
             appendf(s, "%.1f,%.1f,%.1f,", pos[3 * v] * vs, pos[3 * v + 1] * vs, pos[3 * v + 2] * vs);
             const double w = q[4 * v];
             const double angle = 2.0 * std::acos(w > 1 ? 1 : w) * 57.29577951308232;  // AngleDegrees
@@ -125,7 +147,7 @@ std::string history_frame(const Model &m, long long j, double t, const double *p
             for (int corner = 0; corner < 2; corner++) {
                 const bool posLink = corner == 1;
                 for (int i = 0; i < 3; i++) {
-                    const int l = m.vlinks[(size_t)v * 6 + 2 * i + (posLink ? 0 : 1)];
+                    const int l = link_at(v, 2 * i + (posLink ? 0 : 1));
                     double st;
                     if (l >= 0) {
                         const float ratio = m.vmatE[m.vmat[m.lpos[l]]] / m.vmatE[m.vmat[m.lneg[l]]];
@@ -140,7 +162,7 @@ std::string history_frame(const Model &m, long long j, double t, const double *p
             appendf(s, "%.1f,%.1f,%.1f,%.1f,%.1f,%.1f,", off[0][0] * vs, off[0][1] * vs, off[0][2] * vs, off[1][0] * vs,
                     off[1][1] * vs, off[1][2] * vs);
             appendf(s, "%d,", m.vpal[v]);
-            appendf(s, "%.1f,", 0.0);  // localSignal (signals are not supported)
+            appendf(s, "%.1f,", 0.0);  // localSignal (0: signals are not built)
             s += ";";
         }
         s += "<<<>>>";
@@ -148,6 +170,7 @@ std::string history_frame(const Model &m, long long j, double t, const double *p
     if (m.recordLink) {
         appendf(s, "|[[[%lld]]]", j);
         for (int l = 0; l < m.nl(); l++) {
+            if (lrem && lrem[l]) continue;
             const int a = m.lpos[l], b = m.lneg[l];
             appendf(s, "%.4f,%.4f,%.4f,", pos[3 * a], pos[3 * a + 1], pos[3 * a + 2]);
             appendf(s, "%.4f,%.4f,%.4f,", pos[3 * b], pos[3 * b + 1], pos[3 * b + 2]);
@@ -179,6 +202,7 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
     std::vector<double> com((size_t)3 * n);
     std::vector<double> pos, quat;
     std::vector<float> temp, strain;
+    std::vector<int> vrem, lrem;
     int rc = b.com(com.data());
     if (rc) return rc;
     for (int r = 0; r < n; r++) {
@@ -324,7 +348,16 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
                 temp.resize((size_t)m.nv());
                 strain.resize((size_t)std::max(1, m.nl()));
                 if ((rc = b.frame(r, pos.data(), quat.data(), temp.data(), strain.data()))) return rc;
-                const std::string f = history_frame(m, x.j - 1, x.t, pos.data(), quat.data(), temp.data(), strain.data());
+                const int *vr = nullptr, *lr = nullptr;
+                if (b.anyRemoval && m.secondaryExperiment) {
+                    vrem.resize((size_t)m.nv());
+                    lrem.resize((size_t)std::max(1, m.nl()));
+                    if ((rc = b.get_removed(r, vrem.data(), lrem.data()))) return rc;
+                    vr = vrem.data();
+                    lr = lrem.data();
+                }
+                const std::string f = history_frame(m, x.j - 1, x.t, pos.data(), quat.data(), temp.data(), strain.data(),
+                                                    vr, lr, (long long)steps0[r] + x.j);
                 history(user, r, f.data(), f.size());
             }
         }

@@ -305,6 +305,7 @@ struct VxaContent {
     int recordStepSize = 0, recordVoxel = 1, recordLink = 0;
     int secondaryExperiment = 0;
     double reinitAfter = 0.0;
+    std::vector<double> ciliaBase, ciliaShift;  // 3 per voxel, voxel order
 };
 
 // CVX_FRegion::ReadXML (src/old/VX_FRegion.cpp:165-219)
@@ -512,7 +513,7 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
         if (load_bool(ad, "EnableDetach", false)) { err = "EnableDetach is not supported"; return VX3MI_E_PARSE; }
         int v;
         if (load(sim, "EnableExpansion", v)) L.enable_expansion = v;
-        if (load(sim, "EnableCilia", v) && v) { err = "EnableCilia is not supported"; return VX3MI_E_PARSE; }
+        if (load(sim, "EnableCilia", v)) L.enable_cilia = v != 0;
         if (load(sim, "EnableSignals", v) && v) { err = "EnableSignals is not supported"; return VX3MI_E_PARSE; }
         double d;
         if (load(sim, "MaxDistInVoxelLengthsToCountAsPair", d) && d > 0) {
@@ -520,6 +521,7 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
             return VX3MI_E_PARSE;
         }
         load(sim, "SecondaryExperiment", c.secondaryExperiment);
+        L.secondary_experiment = c.secondaryExperiment != 0;
         load(sim, "SavePositionOfAllVoxels", c.savePositions);
         load(sim, "ReinitializeInitialPositionAfterThisManySeconds", c.reinitAfter);
         if (const XNode *rh = sim->child("RecordHistory")) {
@@ -645,6 +647,9 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
                 if (load(mech, "Fixed", iv)) m.fixed = iv != 0;
                 if (load(mech, "isMeasured", iv)) m.is_measured = iv;
                 load(mech, "TurnOnThermalExpansionAfterThisManySeconds", m.thermal_on_after);
+                load(mech, "Cilia", m.cilia);
+                load(mech, "TurnOnCiliaAfterThisManySeconds", m.cilia_on_after);
+                load(mech, "RemoveFromSimulationAfterThisManySeconds", m.remove_after);
             }
             c.mats.push_back(m);
         }
@@ -695,6 +700,29 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
             zz++;
         }
     }
+    // BaseCiliaForce / ShiftCiliaForce layers (VX_Object.cpp:1886-1958): three
+    // values per cell, kept for cells holding a voxel; a layer too short for a
+    // cell gives that cell zeros (as the reference pads)
+    auto cilia_layers = [&](const char *name, std::vector<double> &out) {
+        const XNode *cf = st->child(name);
+        if (!cf) return;
+        int zz = 0;
+        for (const XNode &layer : cf->kids) {
+            if (layer.name != "Layer") continue;
+            if (zz >= nz) break;
+            std::vector<std::string> vals = split_commas(layer.text);
+            for (int k = 0; k < nx * ny; k++) {
+                if (c.cells[(size_t)nx * ny * zz + k] <= 0) continue;
+                for (int a = 0; a < 3; a++) {
+                    const size_t i = (size_t)3 * k + a;
+                    out.push_back(i < vals.size() ? std::atof(vals[i].c_str()) : 0.0);
+                }
+            }
+            zz++;
+        }
+    };
+    cilia_layers("BaseCiliaForce", c.ciliaBase);
+    cilia_layers("ShiftCiliaForce", c.ciliaShift);
     L.nx = nx;
     L.ny = ny;
     L.nz = nz;
@@ -727,6 +755,11 @@ int model_from_docs(const char *baseText, const char *vxdText, vx3mi_model **out
     for (signed char ch : c.cells) nv += (ch > 0 && ch <= (int)c.mats.size());
     c.lat.phase = (!c.phase.empty() && (int)c.phase.size() >= nv) ? c.phase.data() : nullptr;
     c.lat.voxel_E = nullptr;
+    // missing layers leave zeros (the reference pads short data)
+    if (!c.ciliaBase.empty()) c.ciliaBase.resize((size_t)3 * nv, 0.0);
+    if (!c.ciliaShift.empty()) c.ciliaShift.resize((size_t)3 * nv, 0.0);
+    c.lat.cilia_base = c.ciliaBase.empty() ? nullptr : c.ciliaBase.data();
+    c.lat.cilia_shift = c.ciliaShift.empty() ? nullptr : c.ciliaShift.data();
     c.lat.nmat = (int)c.mats.size();
     c.lat.mats = c.mats.data();
     int rc = vx3mi_model_from_lattice(&c.lat, out);

@@ -30,10 +30,12 @@ OK, DIVERGED = 0, 1
 PROBLEM_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=1.0, slow_damp_z=1.0,
                         volume_effects=0, enable_collision=1, enable_expansion=0, grav_acc=-9.81,
                         floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
-                        temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None)
+                        temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None,
+                        enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, is_target=0, thermal_on_after=0.0,
+                         cilia=0.0, cilia_on_after=0.0, remove_after=0.0,
                          rgba=(0.5, 0.5, 0.5, 1.0))
 
 
@@ -55,7 +57,8 @@ class _Material(C.Structure):
                 ("fail_stress", C.c_float), ("plastic_mod", C.c_float), ("yield_stress", C.c_float),
                 ("mat_model", C.c_int), ("fixed", C.c_int),
                 ("is_measured", C.c_int), ("is_target", C.c_int), ("thermal_on_after", C.c_double),
-                ("red", C.c_float), ("green", C.c_float), ("blue", C.c_float), ("alpha", C.c_float)]
+                ("red", C.c_float), ("green", C.c_float), ("blue", C.c_float), ("alpha", C.c_float),
+                ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double)]
 
 
 class _Settings(C.Structure):
@@ -64,7 +67,8 @@ class _Settings(C.Structure):
                 ("enable_expansion", C.c_int), ("vary_temp", C.c_int), ("temp_base", C.c_double),
                 ("temp_amplitude", C.c_double), ("temp_period", C.c_double), ("initial_temp", C.c_float),
                 ("record_step_size", C.c_int), ("record_voxel", C.c_int), ("record_link", C.c_int),
-                ("has_stop_formula", C.c_int), ("has_fitness_formula", C.c_int)]
+                ("has_stop_formula", C.c_int), ("has_fitness_formula", C.c_int),
+                ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int)]
 
 
 FORMULA_STOP, FORMULA_FITNESS = 0, 1
@@ -96,7 +100,8 @@ class _Lattice(C.Structure):
                 ("volume_effects", C.c_int), ("enable_collision", C.c_int), ("enable_expansion", C.c_int),
                 ("grav_acc", C.c_float), ("floor_enabled", C.c_int), ("temp_enabled", C.c_int),
                 ("vary_temp", C.c_int), ("temp_base", C.c_double), ("temp_amplitude", C.c_double),
-                ("temp_period", C.c_double)]
+                ("temp_period", C.c_double), ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int),
+                ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double))]
 
 
 def library_path() -> str:
@@ -145,6 +150,9 @@ def lib():
         "vx3mi_set_state": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
         "vx3mi_get_voxel_flags": (C.c_int, [vp, C.c_int, ip]),
         "vx3mi_get_link_flags": (C.c_int, [vp, C.c_int, ip, fp]),
+        "vx3mi_batch_removed": (C.c_int, [vp, C.c_int, ip, ip]),
+        "vx3mi_model_cilia": (C.c_int, [vp, dp, dp]),
+        "vx3mi_model_material": (C.c_int, [vp, C.c_int, C.POINTER(_Material)]),
         "vx3mi_batch_collision_overflow": (C.c_int, [vp, ip]),
         "vx3mi_batch_time": (C.c_int, [vp, dp]),
         "vx3mi_batch_diverged": (C.c_int, [vp, ip]),
@@ -213,15 +221,18 @@ class Model:
             marr[i] = _Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"], float(m["fail_stress"]),
                                 float(m["plastic_mod"]), float(m["yield_stress"]), material_model(m),
                                 int(m["fixed"]), int(m["is_measured"]), int(m["is_target"]), m["thermal_on_after"],
-                                r, g, b, a)
+                                r, g, b, a, float(m["cilia"]), float(m["cilia_on_after"]), float(m["remove_after"]))
         phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
+        cb, cs = [None if p[k] is None else np.ascontiguousarray(p[k], dtype=np.float64).reshape(-1)
+                  for k in ("cilia_base", "cilia_shift")]
         spec = _Lattice(nx, ny, nz, _ptr(cells, C.c_int8), _ptr(phase, C.c_double), _ptr(E, C.c_float),
                         len(mats), marr, p["lattice_dim"], p["dt_frac"], p["bond_damp_z"], p["col_damp_z"],
                         p["slow_damp_z"], int(p["volume_effects"]), int(p["enable_collision"]),
                         int(p["enable_expansion"]), p["grav_acc"], int(p["floor_enabled"]),
                         int(p["temp_enabled"]), int(p["vary_temp"]), p["temp_base"], p["temp_amplitude"],
-                        p["temp_period"])
+                        p["temp_period"], int(p["enable_cilia"]), int(p["secondary_experiment"]),
+                        _ptr(cb, C.c_double), _ptr(cs, C.c_double))
         h = C.c_void_p()
         _check(lib().vx3mi_model_from_lattice(C.byref(spec), C.byref(h)))
         m = cls(h.value)
@@ -261,6 +272,18 @@ class Model:
         st = _Settings()
         _check(lib().vx3mi_model_settings(self._h, C.byref(st)))
         return {name: getattr(st, name) for name, _ in _Settings._fields_}
+
+    def cilia(self):
+        """(BaseCiliaForce, ShiftCiliaForce), each voxels x 3."""
+        b, sh = np.zeros((self.nv, 3)), np.zeros((self.nv, 3))
+        _check(lib().vx3mi_model_cilia(self._h, _ptr(b, C.c_double), _ptr(sh, C.c_double)))
+        return b, sh
+
+    def material(self, index: int) -> dict:
+        """Palette entry `index` (1-based) as the model holds it."""
+        o = _Material()
+        _check(lib().vx3mi_model_material(self._h, int(index), C.byref(o)))
+        return {name: getattr(o, name) for name, _ in _Material._fields_}
 
     def phase(self) -> np.ndarray:
         o = np.zeros(self.nv)
@@ -395,6 +418,15 @@ class Batch:
         f, st = np.zeros(n, dtype=np.int32), np.zeros(n, dtype=np.float32)
         _check(lib().vx3mi_get_link_flags(self._h, int(robot), _ptr(f, C.c_int), _ptr(st, C.c_float)))
         return f, st
+
+    def removed(self, robot: int = -1):
+        """(voxel, link) arrays: the step count at whose end each was removed
+        (SecondaryExperiment material removal), 0 while present."""
+        nv = self.nv if robot < 0 else self.models[robot].nv
+        nl = self.nl if robot < 0 else self.models[robot].nl
+        vr, lr = np.zeros(nv, dtype=np.int32), np.zeros(max(1, nl), dtype=np.int32)
+        _check(lib().vx3mi_batch_removed(self._h, int(robot), _ptr(vr, C.c_int), _ptr(lr, C.c_int)))
+        return vr, lr[:nl]
 
     def time(self) -> np.ndarray:
         t = np.zeros(self.nrobot)

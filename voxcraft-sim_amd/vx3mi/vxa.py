@@ -51,6 +51,10 @@ def _simulator(p: dict, stop=None, fitness=None, record=None) -> str:
         _el("AttachDetach", _el("EnableCollision", str(int(p["enable_collision"])))),
         _el("EnableExpansion", str(int(p["enable_expansion"]))),
     ]
+    if p.get("enable_cilia"):
+        parts.append(_el("EnableCilia", "1"))
+    if p.get("secondary_experiment"):
+        parts.append(_el("SecondaryExperiment", "1"))
     if record is not None:
         step, vox, link = record
         parts.append(_el("RecordHistory", _el("RecordStepSize", str(int(step))) + _el("RecordVoxel", str(int(vox)))
@@ -108,6 +112,12 @@ def _palette(mats: Sequence[dict]) -> str:
             mech += _el("Plastic_Mod", _num(m["plastic_mod"])) + _el("Yield_Stress", _num(m["yield_stress"]))
         if m["thermal_on_after"]:
             mech += _el("TurnOnThermalExpansionAfterThisManySeconds", _num(m["thermal_on_after"]))
+        if m["cilia"]:
+            mech += _el("Cilia", _num(m["cilia"]))
+        if m["cilia_on_after"]:
+            mech += _el("TurnOnCiliaAfterThisManySeconds", _num(m["cilia_on_after"]))
+        if m["remove_after"]:
+            mech += _el("RemoveFromSimulationAfterThisManySeconds", _num(m["remove_after"]))
         disp = _el("Red", _num(r)) + _el("Green", _num(g)) + _el("Blue", _num(b)) + _el("Alpha", _num(a))
         out.append(_el("Material", _el("Name", f"material{i + 1}") + _el("Display", disp) + _el("Mechanical", mech),
                        f' ID="{i + 1}"'))
@@ -136,6 +146,14 @@ def _structure(p: dict, compression: str, attrs: str = "") -> str:
         phase = phase.reshape(nz, ny * nx)
         body += _el("PhaseOffset", "".join(
             _el("Layer", f"<![CDATA[{','.join(repr(float(v)) for v in phase[z])}]]>") for z in range(nz)))
+    for key, tag in (("cilia_base", "BaseCiliaForce"), ("cilia_shift", "ShiftCiliaForce")):
+        if p.get(key) is None:
+            continue
+        f = np.zeros((nx * ny * nz, 3))
+        f[np.flatnonzero(cells.reshape(-1) > 0)] = np.asarray(p[key], dtype=np.float64).reshape(-1, 3)
+        f = f.reshape(nz, ny * nx * 3)
+        body += _el(tag, "".join(
+            _el("Layer", f"<![CDATA[{','.join(repr(float(v)) for v in f[z])}]]>") for z in range(nz)))
     return _el("Structure", body, f' Compression="{compression}"{attrs}')
 
 
