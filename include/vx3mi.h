@@ -41,6 +41,9 @@ extern "C" {
 #define VX3MI_MODE_PERSISTENT 3  /* one workgroup per robot, many steps per launch */
 #define VX3MI_MODE_TILED 4       /* robots bin-packed into register/LDS-resident tiles (collisions off) */
 
+#define VX3MI_MAX_SS_POINTS 32    /* stress-strain data points of a MatModel 3 material */
+#define VX3MI_MAX_CURVE_POINTS 72 /* points of a link's combined stress-strain curve */
+
 /* One palette material: VXA Palette/Material (src/VXA/VX_Object.cpp:1373-1475). */
 typedef struct {
     float elastic_mod;   /* Elastic_Mod (Pa) */
@@ -52,7 +55,7 @@ typedef struct {
     float fail_stress;   /* Fail_Stress (mat_model 1, 2; -1 = none) */
     float plastic_mod;   /* Plastic_Mod (mat_model 2) */
     float yield_stress;  /* Yield_Stress (mat_model 2) */
-    int mat_model;       /* MatModel: 0 linear, 1 linear with failure, 2 bilinear */
+    int mat_model;       /* MatModel: 0 linear, 1 linear with failure, 2 bilinear, 3 stress-strain data */
     int fixed;           /* Fixed */
     int is_measured;     /* isMeasured */
     int is_target;       /* isTarget */
@@ -61,6 +64,8 @@ typedef struct {
     double cilia;            /* Cilia: multiplier of the voxel's cilia force (with enable_cilia) */
     double cilia_on_after;   /* TurnOnCiliaAfterThisManySeconds */
     double remove_after;     /* RemoveFromSimulationAfterThisManySeconds (with secondary_experiment; 0 never) */
+    int ss_points;           /* mat_model 3: SSData points as given (a leading 0, 0 is allowed) */
+    float ss_strain[VX3MI_MAX_SS_POINTS], ss_stress[VX3MI_MAX_SS_POINTS]; /* SSData StrainData / StressData */
 } vx3mi_material;
 
 /* Lattice description: the information content of a merged VXA (base.vxa +
@@ -144,11 +149,11 @@ int vx3mi_model_topology(const vx3mi_model *m, int *lattice3, int *link_neg, int
 int vx3mi_model_voxel_constants(const vx3mi_model *m, int voxel, float *out12);
 int vx3mi_model_link_constants(const vx3mi_model *m, int link, float *out11);
 /* stress-strain curve of a link's material when it is non-linear (a bilinear
- * palette material at either end): the springs-in-series combination of
- * CVX_MaterialLink::updateAll (src/old/VX_MaterialLink.cpp:66-101), starting
- * at (0, 0).  Returns the number of points (at most 8; 0 for a linear
- * material) and writes them to strain / stress when not NULL. */
-int vx3mi_model_link_curve(const vx3mi_model *m, int link, float *strain8, float *stress8);
+ * or data palette material at either end): the springs-in-series combination
+ * of CVX_MaterialLink::updateAll (src/old/VX_MaterialLink.cpp:66-101), starting
+ * at (0, 0).  Returns the number of points (at most VX3MI_MAX_CURVE_POINTS; 0
+ * for a linear material) and writes them to strain / stress when not NULL. */
+int vx3mi_model_link_curve(const vx3mi_model *m, int link, float *strain, float *stress);
 
 /* External of a voxel: replaces CVX_Voxel::external() with
  * CVX_External::setDisplacement / setForce / setMoment (src/old/VX_Voxel.h:75,

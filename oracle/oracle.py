@@ -27,7 +27,8 @@ class _Material(C.Structure):
                 ("mu_s", C.c_float), ("mu_k", C.c_float), ("fail_stress", C.c_float),
                 ("plastic_mod", C.c_float), ("yield_stress", C.c_float), ("model", C.c_int),
                 ("fixed", C.c_int), ("is_measured", C.c_int), ("thermal_on_after", C.c_double),
-                ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double)]
+                ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double),
+                ("ss_points", C.c_int), ("ss_strain", C.c_float * 32), ("ss_stress", C.c_float * 32)]
 
 
 # MathTree token codes (vx3_oracle.h VXO_MT_*) and variable order
@@ -39,8 +40,11 @@ MT_VARS = ["x", "y", "z", "hit", "t", "angle", "targetCloseness", "numClosePairs
 
 
 def material_model(m: dict) -> int:
-    """MatModel of a material dict: 2 (bilinear) with a yield stress, 1 (linear
-    with failure) with a positive failure stress, else 0 (linear)."""
+    """MatModel of a material dict: 3 (stress-strain data) with "ss_data"
+    (strains, stresses), 2 (bilinear) with a yield stress, 1 (linear with
+    failure) with a positive failure stress, else 0 (linear)."""
+    if m.get("ss_data") is not None:
+        return 3
     if float(m.get("yield_stress", 0.0)) > 0:
         return 2
     return 1 if float(m.get("fail_stress", -1.0)) > 0 else 0
@@ -74,7 +78,7 @@ SPEC_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, thermal_on_after=0.0,
-                         cilia=0.0, cilia_on_after=0.0, remove_after=0.0)
+                         cilia=0.0, cilia_on_after=0.0, remove_after=0.0, ss_data=None)
 
 
 def build_library(force: bool = False) -> str:
@@ -156,10 +160,13 @@ class OracleModel:
         self._mats = (_Material * max(1, len(mats)))()
         for i, m in enumerate(mats):
             model = material_model(m)
+            ss = m["ss_data"] if m["ss_data"] is not None else ((), ())
+            npts = len(ss[0])
             self._mats[i] = _Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"],
                                       m["fail_stress"] if model else -1.0, m["plastic_mod"], m["yield_stress"],
                                       model, int(m["fixed"]), int(m["is_measured"]), m["thermal_on_after"],
-                                      m["cilia"], m["cilia_on_after"], m["remove_after"])
+                                      m["cilia"], m["cilia_on_after"], m["remove_after"], npts,
+                                      (C.c_float * 32)(*ss[0]), (C.c_float * 32)(*ss[1]))
         self._phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         self._E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
         self._cb, self._cs = [None if p[k] is None else np.ascontiguousarray(p[k], dtype=np.float64).reshape(-1)
@@ -207,7 +214,7 @@ class OracleModel:
         self._L.vxo_set_force_field(self._h, int(axis), len(o), _ptr(o, C.c_int), _ptr(v, C.c_double))
 
     def link_curve(self, l: int) -> np.ndarray:
-        st, ss = np.zeros(8, dtype=np.float32), np.zeros(8, dtype=np.float32)
+        st, ss = np.zeros(72, dtype=np.float32), np.zeros(72, dtype=np.float32)
         n = self._L.vxo_get_link_curve(self._h, int(l), _ptr(st, C.c_float), _ptr(ss, C.c_float))
         return np.stack([st[:n], ss[:n]], axis=1)
 

@@ -12,7 +12,8 @@
 // stdin (whitespace separated):
 //   nx ny nz  lattice_dim  bond col slow  grav_acc floor  temp_amp temp_period vary enable_expansion t0
 //   nmat  then per material: E rho cte mu_s mu_k model plastic_mod yield_stress fail_stress
-//     (model 0: setModelLinear(E), 1: setModelLinear(E, fail), 2: setModelBilinear)
+//     (model 0: setModelLinear(E), 1: setModelLinear(E, fail), 2: setModelBilinear,
+//      3: followed by n and n strain stress pairs for setModel)
 //   nx*ny*nz cell values (0 = empty)
 //   nv phase offsets (cycles)
 //   dt nsteps record_every
@@ -48,7 +49,13 @@ int main() {
         int model;
         std::cin >> E >> rho >> cte >> mus >> muk >> model >> plastic >> yield >> fail;
         CVX_Material *pm = Vx.addMaterial();
-        if (model == 2) pm->setModelBilinear(E, plastic, yield, fail);
+        if (model == 3) {
+            int n = 0;
+            std::cin >> n;
+            std::vector<float> st(n), ss(n);
+            for (int k = 0; k < n; k++) std::cin >> st[k] >> ss[k];
+            if (n > 0) pm->setModel(n, st.data(), ss.data());
+        } else if (model == 2) pm->setModelBilinear(E, plastic, yield, fail);
         else if (model == 1) pm->setModelLinear(E, fail);
         else pm->setModelLinear(E);
         pm->setPoissonsRatio(0.0f);

@@ -37,7 +37,7 @@ static const float SA_BOND_EXT_PERC = 0.50f;
 /* voxel flag bits, src/old/types.h:34-38 */
 #define VF_FLOOR_ENABLED (1 << 2)
 #define VF_FLOOR_STATIC_FRICTION (1 << 3)
-#define OM_MAX_POINTS 8  /* stress-strain points (two bilinear materials combine to 5) */
+#define OM_MAX_POINTS VXO_MAX_CURVE_POINTS  /* stress-strain points of a material or link */
 
 /* link flag bits: src/old/types.h:6-8 (LOCAL_VELOCITY_VALID) + smallAngle */
 #define LF_LOCAL_VELOCITY_VALID 1
@@ -289,6 +289,30 @@ static int om_set_bilinear(vmat *m, float E, float plastic, float yield, float f
     m->epsFail = fail == -1.0f ? -1.0f : tfs;
     return 1;
 }
+/* CVX_Material::setModel (src/old/VX_Material.cpp:300-367) on float data: a
+ * leading (0, 0) is dropped; it rejects no points, a first point not above
+ * zero and strains that do not increase (0: the model is then left as it
+ * was).  Decreasing stresses only leave a message, and its slope test compares
+ * with 0/0 and never fails.  E is the first slope, the last point is the
+ * failure point, one point is a linear material. */
+static int om_set_data(vmat *m, int n, const float *strain, const float *stress) {
+    if (n > 0 && strain[0] == 0 && stress[0] == 0) { strain++; stress++; n--; }
+    if (n <= 0 || n + 1 > OM_MAX_POINTS) return 0;
+    if (strain[0] <= 0 || stress[0] <= 0) return 0;
+    float sweep = 0.0f;
+    for (int i = 0; i < n; i++) {
+        if (strain[i] <= sweep) return 0;
+        sweep = strain[i];
+    }
+    m->npts = n + 1;
+    m->sd[0] = 0; m->td[0] = 0;
+    for (int i = 0; i < n; i++) { m->sd[i + 1] = strain[i]; m->td[i + 1] = stress[i]; }
+    m->E = m->td[1] / m->sd[1];
+    m->sigmaFail = m->td[n];
+    m->epsFail = m->sd[n];
+    m->linear = n == 1;
+    return 1;
+}
 /* CVX_Material::modulus (src/old/VX_Material.cpp:231-243) */
 static float om_modulus(const vmat *m, float strain) {
     if (m->epsFail != -1.0f && strain > m->epsFail) return 0.0f;
@@ -512,8 +536,10 @@ vxo_model *vxo_build(const vxo_spec *sp) {
             if (in->E > 0) om_set_linear(vm, in->E, -1.0f);
         } else if (in->model == 1) {
             if (in->E > 0 && (in->fail_stress == -1.0f || in->fail_stress > 0)) om_set_linear(vm, in->E, in->fail_stress);
-        } else {
+        } else if (in->model == 2) {
             om_set_bilinear(vm, in->E, in->plastic_mod, in->yield_stress, in->fail_stress);
+        } else {
+            om_set_data(vm, in->ss_points, in->ss_strain, in->ss_stress);
         }
         vm->nu = sp->volume_effects ? clamp_nu(in->nu) : 0.0f;
         vm->rho = in->rho <= 0 ? FLT_MIN : in->rho;

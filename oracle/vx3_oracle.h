@@ -31,7 +31,10 @@
 extern "C" {
 #endif
 
-/* One palette material (VXA Palette/Material/Mechanical, MatModel 0 or 1). */
+#define VXO_MAX_SS_POINTS 32   /* stress-strain data points of a MatModel 3 material */
+#define VXO_MAX_CURVE_POINTS 72 /* points of a link's combined curve (two data materials: 65) */
+
+/* One palette material (VXA Palette/Material/Mechanical, MatModel 0-3). */
 typedef struct {
     float E;            /* Elastic_Mod, Pa */
     float rho;          /* Density, kg/m^3 */
@@ -40,13 +43,15 @@ typedef struct {
     float mu_s, mu_k;   /* uStatic, uDynamic */
     float fail_stress;  /* -1 = no failure */
     float plastic_mod, yield_stress; /* Plastic_Mod, Yield_Stress (model 2) */
-    int model;          /* MatModel: 0 linear, 1 linear with failure, 2 bilinear */
+    int model;          /* MatModel: 0 linear, 1 linear with failure, 2 bilinear, 3 data */
     int fixed;          /* Fixed */
     int is_measured;    /* isMeasured */
     double thermal_on_after; /* TurnOnThermalExpansionAfterThisManySeconds */
     double cilia;            /* Cilia: multiplier of the voxel's cilia force */
     double cilia_on_after;   /* TurnOnCiliaAfterThisManySeconds */
     double remove_after;     /* RemoveFromSimulationAfterThisManySeconds (0: never) */
+    int ss_points;           /* model 3: SSData points (strain, stress), as given */
+    float ss_strain[VXO_MAX_SS_POINTS], ss_stress[VXO_MAX_SS_POINTS];
 } vxo_material;
 
 /* Lattice + simulator + environment description. */
@@ -116,8 +121,9 @@ enum { VXO_MT_END, VXO_MT_CONST, VXO_MT_E, VXO_MT_PI, VXO_MT_VAR, VXO_MT_ADD, VX
        VXO_MT_NOT, VXO_MT_GT, VXO_MT_LT, VXO_MT_AND, VXO_MT_OR, VXO_MT_NORMALCDF };
 void vxo_set_force_field(vxo_model *m, int axis, int n, const int *ops, const double *values);
 
-/* stress-strain curve of a non-linear link material (n points, 0 when linear) */
-int vxo_get_link_curve(const vxo_model *m, int link, float *strain8, float *stress8);
+/* stress-strain curve of a non-linear link material (n <= VXO_MAX_CURVE_POINTS
+ * points, 0 when linear) */
+int vxo_get_link_curve(const vxo_model *m, int link, float *strain, float *stress);
 
 /* state access (voxel order) */
 void vxo_get_voxels(const vxo_model *m, double *pos3, double *quat4, double *linmom3, double *angmom3);

@@ -36,8 +36,12 @@ def driver_input(prob, dt, nsteps, every):
              f"{int(prob['enable_expansion'])} 0 {len(mats)}"]
     for m0 in mats:
         m = dict(O.MATERIAL_DEFAULTS, **m0)
-        lines.append(f"{m['E']!r} {m['rho']!r} {m['cte']!r} {m['mu_s']!r} {m['mu_k']!r} {O.material_model(m)} "
-                     f"{m['plastic_mod']!r} {m['yield_stress']!r} {m['fail_stress']!r}")
+        line = (f"{m['E']!r} {m['rho']!r} {m['cte']!r} {m['mu_s']!r} {m['mu_k']!r} {O.material_model(m)} "
+                f"{m['plastic_mod']!r} {m['yield_stress']!r} {m['fail_stress']!r}")
+        if m.get("ss_data") is not None:
+            st, ss = m["ss_data"]
+            line += f" {len(st)} " + " ".join(f"{float(np.float32(a))!r} {float(np.float32(b))!r}" for a, b in zip(st, ss))
+        lines.append(line)
     lines.append(" ".join(str(int(c)) for c in cells))
     lines.append(" ".join(repr(float(p)) for p in phase))
     lines.append(f"{float(np.float32(dt)):.9g} {nsteps} {every}")
@@ -100,11 +104,26 @@ def bilinear_robot():
     return p
 
 
+def data_model_robot():
+    """A 5x4x3 robot of two MatModel 3 materials (stress-strain data: one
+    with a leading (0, 0) point and four segments of falling slope, one of
+    three points), actuated as the bilinear robot so links run along their
+    curves; floor off."""
+    rng = np.random.default_rng(12)
+    p = configs.random_robot(rng, (5, 4, 3), 0.9, enable_collision=0)
+    p.update(floor_enabled=0, grav_acc=0.0, temp_amplitude=39.0, vary_temp=1, temp_period=0.004,
+             phase=rng.uniform(0, 1, int((np.asarray(p["cells"]) > 0).sum())),
+             materials=[dict(rho=1000.0, cte=0.001,
+                             ss_data=([0.0, 0.002, 0.006, 0.015, 0.05], [0.0, 2000.0, 4400.0, 7000.0, 9000.0])),
+                        dict(rho=1000.0, cte=0.001, ss_data=([0.001, 0.004, 0.03], [2000.0, 5000.0, 8000.0]))])
+    return p
+
+
 def golden_problems():
     rng = np.random.default_rng(7)
     robot = configs.random_robot(rng, (5, 4, 3), 0.8, enable_collision=0)
     return {"config0": configs.config0(), "robot5x4x3": robot, "beam_bc": beam_with_externals(),
-            "bilinear": bilinear_robot()}
+            "bilinear": bilinear_robot(), "data_model": data_model_robot()}
 
 
 def main():

@@ -480,3 +480,56 @@ def test_cilia_in_persistent_mode_fp64():
         s, ref = b.state(r), o.state()
         assert np.array_equal(s["pos"], ref["pos"]) and np.array_equal(s["angmom"], ref["angmom"]), r
     b.close()
+
+
+def _data_model_problems():
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from make_golden import golden_problems
+    a = golden_problems()["data_model"]
+    b = dict(a, materials=[dict(m, ss_data=(np.cumsum(np.linspace(1e-4, 2e-4, 32)), np.cumsum(np.linspace(200.0, 20.0, 32))))
+                           for m in a["materials"]])
+    return [a, b, small_problem(seed=5)]
+
+
+@pytest.mark.parametrize("mode,tile,threads", [(vx3mi.MODE_PLAIN, 0, 0), (vx3mi.MODE_FLAT, 0, 0),
+                                               (vx3mi.MODE_PERSISTENT, 0, 256), (vx3mi.MODE_TILED, 0, 0)])
+def test_data_materials_fp64_bit_exact_every_mode(mode, tile, threads):
+    """MatModel 3 stress-strain data (links running along 4- to 63-point series
+    curves, failing past the last point) in every launch mode: FP64 bit for bit
+    with the oracle, which the data-model golden pins to the reference CPU path."""
+    log(f"data materials mode {mode} tile {tile} threads {threads}")
+    probs = _data_model_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
+    run_mode = mode
+    if mode in (vx3mi.MODE_TILED, vx3mi.MODE_PERSISTENT):
+        b.configure(mode, tile, threads)
+        run_mode = vx3mi.MODE_AUTO
+    oracles = [O.OracleModel(p) for p in probs]
+    for _ in range(3):
+        b.step(200, mode=run_mode)
+        for r, o in enumerate(oracles):
+            o.step(200)
+            s, ref = b.state(r), o.state()
+            for k in ("pos", "quat", "linmom", "angmom"):
+                assert np.array_equal(s[k], ref[k]), (r, k)
+    b.close()
+
+
+def test_data_materials_fp32_tiled():
+    """The same batch in FP32 (tiled) within 1e-5 of the oracle over 600 steps."""
+    probs = _data_model_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP32)
+    b.configure(vx3mi.MODE_TILED, 0, 0)
+    oracles = [O.OracleModel(p) for p in probs]
+    worst = 0.0
+    for _ in range(6):
+        b.step(100)
+        for r, o in enumerate(oracles):
+            o.step(100)
+            s, ref = b.state(r), o.state()
+            worst = max(worst, pos_err(s["pos"], ref["pos"]), quat_err(s["quat"], ref["quat"]))
+    b.close()
+    log(f"  fp32 data materials worst {worst:.3e}")
+    assert worst <= TOL[vx3mi.FP32]

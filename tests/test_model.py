@@ -17,6 +17,9 @@ PROBLEMS = {
     "config1_block": lambda: configs.config1(seed=1, dims=(12, 10, 6)),
     "config2_robot": lambda: configs.config2(seed=3, n=1)[0],
     "config3": configs.config3,
+    # MatModel 3 palette (stress-strain data), the data-model golden robot
+    "data_model": lambda: dict(configs.config0(), materials=[
+        dict(rho=1000.0, cte=0.001, ss_data=([0.0, 0.002, 0.006, 0.015, 0.05], [0.0, 2000.0, 4400.0, 7000.0, 9000.0]))]),
 }
 
 

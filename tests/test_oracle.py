@@ -42,7 +42,7 @@ def quiet(dims, cells=None, **kw):
     return p
 
 
-@pytest.mark.parametrize("name", ["config0", "robot5x4x3", "beam_bc", "bilinear"])
+@pytest.mark.parametrize("name", ["config0", "robot5x4x3", "beam_bc", "bilinear", "data_model"])
 def test_golden_reference_cpu(name):
     """Oracle (CVoxelyze integrator semantics) vs the reference CPU Voxelyze built
     from src/old: positions and orientations every 50 steps for 600 steps."""
@@ -77,6 +77,28 @@ def test_bilinear_golden_exercises_plasticity():
     m = O.OracleModel(lin)
     m.step(600, float(g["dt"]), O.VXO_CPU_SEMANTICS)
     assert np.max(np.abs(m.state()["pos"] - g["pos"][-1])) > 1e-5
+
+
+def test_data_model_golden_runs_along_the_curves():
+    """The data-model fixture's links pass the first data point: replacing the
+    curves by their first segments (linear materials of the same modulus) ends
+    elsewhere; its link curves are the series combinations of the data."""
+    import sys
+    sys.path.insert(0, GOLDEN)
+    from make_golden import golden_problems
+
+    prob = golden_problems()["data_model"]
+    g = np.load(os.path.join(GOLDEN, "ref_cpu_data_model.npz"))
+    lin = dict(prob, materials=[dict(m, ss_data=None, E=m["ss_data"][1][1] / m["ss_data"][0][1]) for m in prob["materials"]])
+    m = O.OracleModel(lin)
+    m.step(600, float(g["dt"]), O.VXO_CPU_SEMANTICS)
+    assert np.max(np.abs(m.state()["pos"] - g["pos"][-1])) > 1e-5
+    d = O.OracleModel(prob)
+    curves = {tuple(map(tuple, d.link_curve(l))) for l in range(d.nl)}
+    assert len(curves) == 3  # A-A, A-B, B-B
+    for c in curves:  # strains rise from (0, 0); stresses follow the softer material
+        c = np.array(c)
+        assert c[0].tolist() == [0.0, 0.0] and np.all(np.diff(c[:, 0]) > 0) and np.all(np.diff(c[:, 1]) > 0)
 
 
 @pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "vx3_ref_driver")),

@@ -457,3 +457,69 @@ def test_short_cilia_layers_pad_zeros():
     assert base.tolist() == [[0.5, -1.0, 2.0]]
     assert shift.tolist() == [[3.0, 0.0, 0.0]]
     assert m.settings()["enable_cilia"] == 1
+
+
+def test_data_material_model_from_vxa_and_oracle_curves():
+    """MatModel 3 (SSData) read from a VXA gives the lattice description's link
+    materials; the link curves (series combinations of the data, up to 2 x 32 + 1
+    points) and link constants equal the oracle's, which the data-model golden
+    pins to the reference CPU Voxelyze."""
+    import sys
+    import oracle as O
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    from make_golden import golden_problems
+    prob = golden_problems()["data_model"]
+    text = W.write_vxa(prob)
+    assert "<MatModel>3</MatModel>" in text and "<NumDataPts>5</NumDataPts>" in text
+    a, b, o = vx3mi.Model.from_problem(prob), vx3mi.Model.from_xml(text), O.OracleModel(prob)
+    assert_same_model(a, b)
+    for l in range(a.nl):
+        ca = a.link_curve(l)
+        assert len(ca) > 0 and np.array_equal(ca, b.link_curve(l)) and np.array_equal(ca, o.link_curve(l)), l
+        assert np.array_equal(a.link_constants(l), o.link_material(l)), l
+    # 32 points per material: the longest combination fits
+    st = np.cumsum(np.linspace(1e-4, 2e-4, 32))
+    ss = np.cumsum(np.linspace(200.0, 20.0, 32))
+    st2, ss2 = st * 1.37, ss * 1.5
+    big = dict(prob, materials=[dict(rho=1000.0, ss_data=(st, ss)), dict(rho=1000.0, ss_data=(st2, ss2))])
+    mb, ob = vx3mi.Model.from_xml(W.write_vxa(big)), O.OracleModel(big)
+    lens = {len(mb.link_curve(l)) for l in range(mb.nl)}
+    assert max(lens) > 33
+    for l in range(mb.nl):
+        assert np.array_equal(mb.link_curve(l), ob.link_curve(l)), l
+
+
+DATA_BAD = {
+    "no_points": ("<SSData><NumDataPts>0</NumDataPts></SSData>", "error -2"),
+    "short": ("<SSData><NumDataPts>2</NumDataPts><StrainData><Strain>0.1</Strain></StrainData>"
+              "<StressData><Stress>1</Stress><Stress>2</Stress></StressData></SSData>", "error -2"),
+    "too_many": ("<SSData><NumDataPts>33</NumDataPts></SSData>", "error -5"),
+}
+
+
+@pytest.mark.parametrize("case", list(DATA_BAD))
+def test_data_material_input_errors(case):
+    """MatModel 3 without points (the reference then reads an empty vector),
+    with fewer values than NumDataPts, or with more than 32 points is an error."""
+    body, code = DATA_BAD[case]
+    text = DEMO_LIKE.replace("<MatModel>0</MatModel>", "<MatModel>3</MatModel>" + body)
+    with pytest.raises(vx3mi.VX3MIError, match=code):
+        vx3mi.Model.from_xml(text)
+
+
+def test_rejected_data_keeps_the_default():
+    """CVX_Material::setModel rejects strains that do not increase and a first
+    point at or below zero; the material keeps addMaterial's default (linear,
+    1e6 Pa), as for the other models.  One point is a linear material of that
+    slope; decreasing stresses are accepted."""
+    base = configs.config0()
+    penK = lambda m: m.voxel_constants(0)[8]  # noqa: E731  2 E L
+    for st, ss in (([0.001, 0.001], [1.0, 2.0]), ([-0.001, 0.002], [1.0, 2.0]), ([0.002, 0.001], [1.0, 2.0])):
+        m = vx3mi.Model.from_xml(W.write_vxa(dict(base, materials=[dict(rho=1000.0, ss_data=(st, ss))])))
+        assert penK(m) == np.float32(2 * 1e6 * 0.01) and len(m.link_curve(0)) == 0
+    one = vx3mi.Model.from_xml(W.write_vxa(dict(base, materials=[dict(rho=1000.0, ss_data=([0.002], [6000.0]))])))
+    E1 = np.float32(np.float32(6000.0) / np.float32(0.002))
+    assert penK(one) == np.float32(float(np.float32(2) * E1) * 0.01)  # (float)(2 E L), 2 E in float
+    assert len(one.link_curve(0)) == 0
+    dec = vx3mi.Model.from_xml(W.write_vxa(dict(base, materials=[dict(rho=1000.0, ss_data=([0.001, 0.002], [2000.0, 1500.0]))])))
+    assert len(dec.link_curve(0)) == 3

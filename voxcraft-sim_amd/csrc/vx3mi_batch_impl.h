@@ -565,7 +565,7 @@ template <typename T> struct Batch : BatchBase {
                         ci = curveIndex.emplace(ck, (int)curves.size()).first;
                         curves.insert(curves.end(), m.curves.begin() + first, m.curves.begin() + first + np);
                     }
-                    lm.curve = (ci->second << 4) | np;
+                    lm.curve = (ci->second << 8) | np;
                 }
                 std::string key((const char *)&lm, sizeof(LinkMat));
                 auto it = lmIndex.find(key);

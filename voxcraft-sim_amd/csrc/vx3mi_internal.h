@@ -10,6 +10,8 @@
 
 namespace vx3mi {
 
+static_assert(kMaxCurvePoints == VX3MI_MAX_CURVE_POINTS, "curve capacity of the ABI and the model differ");
+
 void set_error(const std::string &msg);
 
 // Host model: topology, materials and initial state of one robot, in the

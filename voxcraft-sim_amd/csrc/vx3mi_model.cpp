@@ -68,6 +68,31 @@ bool set_bilinear(MatDerived &d, float E, float plastic, float yield, float fail
     return true;
 }
 
+// CVX_Material::setModel (src/old/VX_Material.cpp:300-367) on float data: a
+// leading (0, 0) is dropped; false (model unchanged) for no points, a first
+// point not above zero or strains that do not increase.  Decreasing stresses
+// only leave a message there, and its slope test compares with 0/0 and never
+// fails.  E is the first slope, the last point is the failure point, one point
+// is a linear material.
+bool set_data(MatDerived &d, int n, const float *strain, const float *stress) {
+    if (n > 0 && strain[0] == 0 && stress[0] == 0) { strain++; stress++; n--; }
+    if (n <= 0 || n + 1 > kMaxCurvePoints) return false;
+    if (strain[0] <= 0 || stress[0] <= 0) return false;
+    float sweep = 0.0f;
+    for (int i = 0; i < n; i++) {
+        if (strain[i] <= sweep) return false;
+        sweep = strain[i];
+    }
+    d.npts = n + 1;
+    d.sd[0] = 0; d.td[0] = 0;
+    for (int i = 0; i < n; i++) { d.sd[i + 1] = strain[i]; d.td[i + 1] = stress[i]; }
+    d.E = d.td[1] / d.sd[1];
+    d.sigmaFail = d.td[n];
+    d.epsFail = d.sd[n];
+    d.linear = n == 1;
+    return true;
+}
+
 // CVX_Material::modulus / strain (src/old/VX_Material.cpp:214-243)
 float mat_modulus(const MatDerived &d, float strain) {
     if (d.epsFail != -1.0f && strain > d.epsFail) return 0.0f;
@@ -165,7 +190,7 @@ LinkMat combine(const MatDerived &a, const MatDerived &b, std::vector<CurvePt> &
         // CVX_Material::setModel (VX_Material.cpp:300-370) on these points: the
         // leading (0, 0) is implied; E is the first slope
         const int n = (int)st.size();
-        c.npts = std::min(n, kMaxCurvePoints);  // two bilinear materials give at most 5 points
+        c.npts = std::min(n, kMaxCurvePoints);  // two data materials give at most 2 * 32 + 1 points
         for (int k = 0; k < c.npts; k++) { c.sd[k] = st[k]; c.td[k] = ss[k]; }
         c.E = c.td[1] / c.sd[1];
         c.linear = (n - 1 == 1);
@@ -284,8 +309,14 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
             if (E > 0 && (fail == -1.0f || fail > 0)) set_linear(d, E, fail);
         } else if (in.mat_model == 2) {
             set_bilinear(d, E, in.plastic_mod, in.yield_stress, fail);
+        } else if (in.mat_model == 3) {
+            if (in.ss_points < 0 || in.ss_points > VX3MI_MAX_SS_POINTS) {
+                set_error("stress-strain data: 0 to VX3MI_MAX_SS_POINTS points");
+                return VX3MI_E_LIMIT;
+            }
+            set_data(d, in.ss_points, in.ss_strain, in.ss_stress);
         } else {
-            set_error("material model must be 0 (linear), 1 (linear with failure) or 2 (bilinear)");
+            set_error("material model must be 0 (linear), 1 (linear with failure), 2 (bilinear) or 3 (data)");
             return VX3MI_E_ARG;
         }
         d.nu = sp->volume_effects ? clamp_nu(in.poissons_ratio) : 0.0f;
@@ -361,7 +392,7 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
                 std::vector<CurvePt> curve;
                 LinkMat l = combine(vm[m.vmat[v]], vm[m.vmat[n]], curve);
                 if (!curve.empty()) {
-                    l.curve = ((int)m.curves.size() << 4) | (int)curve.size();
+                    l.curve = ((int)m.curves.size() << 8) | (int)curve.size();
                     m.curves.insert(m.curves.end(), curve.begin(), curve.end());
                 }
                 const MatDerived &dn = vm[m.vmat[n]], &dp = vm[m.vmat[v]];

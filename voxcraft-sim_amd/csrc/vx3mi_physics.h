@@ -116,7 +116,7 @@ struct LinkMat {
     float alphaN, alphaP;    // endpoint CTEs
     float dmNumN, dmNumP;    // endpoint damping multiplier numerators
     int fixedBoth;
-    int curve;               // stress-strain curve: first point << 4 | points (0: linear material)
+    int curve;               // stress-strain curve: first point << 8 | points (0: linear material)
 };
 
 // One point of a non-linear stress-strain curve (CVX_Material strainData /
@@ -124,9 +124,9 @@ struct LinkMat {
 struct CurvePt {
     float strain, stress;
 };
-constexpr int kMaxCurvePoints = 8;
-VX_HD int curve_points(const LinkMat &lm) { return lm.curve & 15; }
-VX_HD int curve_first(const LinkMat &lm) { return lm.curve >> 4; }
+constexpr int kMaxCurvePoints = 72;  // host capacity of a material or link curve (VX3MI_MAX_CURVE_POINTS)
+VX_HD int curve_points(const LinkMat &lm) { return lm.curve & 255; }
+VX_HD int curve_first(const LinkMat &lm) { return lm.curve >> 8; }
 
 // ---- constants --------------------------------------------------------------
 // src/Utils/VX3_Quat3D.h:30-56, src/old/types.h:57-59, VX3_Collision.h:6

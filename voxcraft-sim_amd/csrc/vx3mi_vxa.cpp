@@ -632,8 +632,35 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
             if (const XNode *mech = mn.child("Mechanical")) {
                 int model = 0, iv;
                 load(mech, "MatModel", model);
-                if (model < 0 || model > 2) { err = "only MatModel 0, 1 and 2 are supported"; return VX3MI_E_PARSE; }
+                if (model < 0 || model > 3) { err = "MatModel must be 0, 1, 2 or 3"; return VX3MI_E_PARSE; }
                 m.mat_model = model;
+                // SSData (VX_Object.cpp:1397-1418): NumDataPts values under
+                // StrainData/Strain and StressData/Stress
+                if (const XNode *ss = mech->child("SSData")) {
+                    int np = 0;
+                    load(ss, "NumDataPts", np);
+                    if (np > VX3MI_MAX_SS_POINTS) { err = "more than 32 stress-strain data points"; return VX3MI_E_LIMIT; }
+                    std::vector<double> vals[2];
+                    const char *grp[2] = {"StrainData", "StressData"}, *key[2] = {"Strain", "Stress"};
+                    for (int g = 0; g < 2; g++)
+                        if (const XNode *gd = ss->child(grp[g]))
+                            for (const XNode &e : gd->kids)
+                                if (e.name == key[g] && (int)vals[g].size() < np) vals[g].push_back(std::atof(e.text.c_str()));
+                    if (np > 0 && ((int)vals[0].size() != np || (int)vals[1].size() != np)) {
+                        err = "SSData holds fewer Strain or Stress values than NumDataPts";
+                        return VX3MI_E_PARSE;
+                    }
+                    m.ss_points = np;
+                    for (int i = 0; i < np; i++) {
+                        m.ss_strain[i] = (float)vals[0][i];
+                        m.ss_stress[i] = (float)vals[1][i];
+                    }
+                }
+                if (model == 3 && m.ss_points < 1) {
+                    // without data the reference reads past an empty vector (VX_Sim.cpp:400-407)
+                    err = "MatModel 3 needs SSData points";
+                    return VX3MI_E_PARSE;
+                }
                 load(mech, "Elastic_Mod", m.elastic_mod);
                 load(mech, "Fail_Stress", m.fail_stress);
                 load(mech, "Plastic_Mod", m.plastic_mod);

@@ -35,7 +35,7 @@ PROBLEM_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, is_target=0, thermal_on_after=0.0,
-                         cilia=0.0, cilia_on_after=0.0, remove_after=0.0,
+                         cilia=0.0, cilia_on_after=0.0, remove_after=0.0, ss_data=None,
                          rgba=(0.5, 0.5, 0.5, 1.0))
 
 
@@ -44,8 +44,11 @@ class VX3MIError(RuntimeError):
 
 
 def material_model(m: dict) -> int:
-    """MatModel of a material dict: 2 (bilinear) with a yield stress, 1 (linear
-    with failure) with a positive failure stress, else 0 (linear)."""
+    """MatModel of a material dict: 3 (stress-strain data) with "ss_data"
+    (strains, stresses), 2 (bilinear) with a yield stress, 1 (linear with
+    failure) with a positive failure stress, else 0 (linear)."""
+    if m.get("ss_data") is not None:
+        return 3
     if float(m.get("yield_stress", 0.0)) > 0:
         return 2
     return 1 if float(m.get("fail_stress", -1.0)) > 0 else 0
@@ -58,7 +61,8 @@ class _Material(C.Structure):
                 ("mat_model", C.c_int), ("fixed", C.c_int),
                 ("is_measured", C.c_int), ("is_target", C.c_int), ("thermal_on_after", C.c_double),
                 ("red", C.c_float), ("green", C.c_float), ("blue", C.c_float), ("alpha", C.c_float),
-                ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double)]
+                ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double),
+                ("ss_points", C.c_int), ("ss_strain", C.c_float * 32), ("ss_stress", C.c_float * 32)]
 
 
 class _Settings(C.Structure):
@@ -218,10 +222,14 @@ class Model:
         marr = (_Material * max(1, len(mats)))()
         for i, m in enumerate(mats):
             r, g, b, a = m["rgba"]
+            ss = m["ss_data"] if m["ss_data"] is not None else ((), ())
+            if len(ss[0]) > 32 or len(ss[0]) != len(ss[1]):
+                raise VX3MIError("ss_data: equal numbers of strains and stresses, at most 32")
             marr[i] = _Material(m["E"], m["rho"], m["nu"], m["cte"], m["mu_s"], m["mu_k"], float(m["fail_stress"]),
                                 float(m["plastic_mod"]), float(m["yield_stress"]), material_model(m),
                                 int(m["fixed"]), int(m["is_measured"]), int(m["is_target"]), m["thermal_on_after"],
-                                r, g, b, a, float(m["cilia"]), float(m["cilia_on_after"]), float(m["remove_after"]))
+                                r, g, b, a, float(m["cilia"]), float(m["cilia_on_after"]), float(m["remove_after"]),
+                                len(ss[0]), (C.c_float * 32)(*ss[0]), (C.c_float * 32)(*ss[1]))
         phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
         cb, cs = [None if p[k] is None else np.ascontiguousarray(p[k], dtype=np.float64).reshape(-1)
@@ -317,7 +325,7 @@ class Model:
 
     def link_curve(self, l: int) -> np.ndarray:
         """(n, 2) strain / stress points of a non-linear link material (n = 0: linear)."""
-        st, ss = np.zeros(8, dtype=np.float32), np.zeros(8, dtype=np.float32)
+        st, ss = np.zeros(72, dtype=np.float32), np.zeros(72, dtype=np.float32)
         n = lib().vx3mi_model_link_curve(self._h, int(l), _ptr(st, C.c_float), _ptr(ss, C.c_float))
         _check(min(n, 0))
         return np.stack([st[:n], ss[:n]], axis=1)

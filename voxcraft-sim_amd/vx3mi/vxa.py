@@ -110,6 +110,11 @@ def _palette(mats: Sequence[dict]) -> str:
                 + _el("Fixed", str(int(m["fixed"]))) + _el("isMeasured", str(int(m["is_measured"]))))
         if model == 2:
             mech += _el("Plastic_Mod", _num(m["plastic_mod"])) + _el("Yield_Stress", _num(m["yield_stress"]))
+        if model == 3:  # SSData (VX_Object.cpp:1397-1418)
+            strain, stress = m["ss_data"]
+            mech += _el("SSData", _el("NumDataPts", str(len(strain)))
+                        + _el("StrainData", "".join(_el("Strain", _num(x)) for x in strain))
+                        + _el("StressData", "".join(_el("Stress", _num(x)) for x in stress)))
         if m["thermal_on_after"]:
             mech += _el("TurnOnThermalExpansionAfterThisManySeconds", _num(m["thermal_on_after"]))
         if m["cilia"]:
