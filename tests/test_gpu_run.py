@@ -132,6 +132,16 @@ def test_cli_report(tmp_path):
         c1 = [float(e.find(f"currentCenterOfMass/{a}").text) for a in "xyz"]
         assert float(e.find("fitness_score").text) == pytest.approx(c1[0] - c0[0], rel=1e-6, abs=1e-12)
         assert float(e.find("currentTime").text) > 0.002
+    # the same generation through a vx3_node_worker task file (Boost ptree XML
+    # with several top-level keys), robots in the file's order
+    vxt = tmp_path / "job.vxt"
+    vxt.write_text('<?xml version="1.0" encoding="utf-8"?>\n'
+                   f'<vxa>{d / "base.vxa"}</vxa><input_dir>{d}</input_dir><vxd><f>robot1.vxd</f><f>robot0.vxd</f></vxd>')
+    out2 = tmp_path / "report2.xml"
+    r2 = subprocess.run([exe, "-i", str(vxt), "-o", str(out2)], capture_output=True, timeout=120)
+    assert r2.returncode == 0, r2.stdout.decode() + r2.stderr.decode()
+    assert "Simulation 0 runs: robot1.vxd" in r2.stdout.decode()
+    assert ET.tostring(ET.parse(out2).getroot()) == ET.tostring(rep)
 
 
 def test_history_frames_follow_removal():

@@ -1,9 +1,10 @@
 // vx3mi_run -- command-line drop-in for voxcraft-sim / vx3_node_worker
 // (src/Executables/voxcraft-sim.cpp:25-132, vx3_node_worker.cu:31-141):
 //
-//     vx3mi_run -i <input_dir> [-o <report.xml>] [-f] [--fp64] [--max-steps N]
+//     vx3mi_run -i <input_dir | task.vxt> [-o <report.xml>] [-f] [--fp64] [--max-steps N]
 //
-// <input_dir> holds base.vxa and the robots as *.vxd; every VXD is merged into
+// <input_dir> holds base.vxa and the robots as *.vxd (or a worker task file
+// names the base VXA, the input directory and the VXD files); every VXD is merged into
 // the base (replace= paths) and all robots run as one batch on the GPU.  The
 // .history text (RecordStepSize > 0) and the status lines go to stdout robot
 // by robot; the report is the reference's report XML, results sorted by
@@ -30,7 +31,7 @@ void usage() {
     std::printf(
         "vx3mi_run: runs a generation of VXA/VXD robots on the GPU.\n"
         "Usage: vx3mi_run -i <input_dir> [-o <report.xml>] [-f] [--fp64] [--max-steps N] [--mode auto|tiled|persistent|flat|plain]\n"
-        "  -i  directory with base.vxa and *.vxd files\n"
+        "  -i  directory with base.vxa and *.vxd files, or a vx3_node_worker task file (.vxt)\n"
         "  -o  report file (XML)\n"
         "  -f  overwrite the report if it exists\n");
 }
@@ -42,6 +43,52 @@ bool is_file(const std::string &p) {
 bool is_dir(const std::string &p) {
     struct stat st;
     return stat(p.c_str(), &st) == 0 && S_ISDIR(st.st_mode);
+}
+
+std::string xml_unescape(const std::string &s) {
+    static const char *ent[5][2] = {{"&amp;", "&"}, {"&lt;", "<"}, {"&gt;", ">"}, {"&quot;", "\""}, {"&apos;", "'"}};
+    std::string o;
+    for (size_t i = 0; i < s.size();) {
+        bool done = false;
+        for (auto &e : ent)
+            if (s.compare(i, std::strlen(e[0]), e[0]) == 0) { o += e[1]; i += std::strlen(e[0]); done = true; break; }
+        if (!done) o += s[i++];
+    }
+    return o;
+}
+
+// the text of the first <tag>...</tag> in [from, to) of s; npos-safe
+bool element(const std::string &s, const std::string &tag, size_t from, size_t to, std::string &out, size_t *end = nullptr) {
+    const std::string open = "<" + tag + ">", close = "</" + tag + ">";
+    const size_t a = s.find(open, from);
+    if (a == std::string::npos || a >= to) return false;
+    const size_t b = s.find(close, a + open.size());
+    if (b == std::string::npos || b > to) return false;
+    out = s.substr(a + open.size(), b - a - open.size());
+    if (end) *end = b + close.size();
+    return true;
+}
+
+// A vx3_node_worker task file (.vxt: Boost ptree XML that voxcraft-sim writes,
+// src/Executables/voxcraft-sim.cpp:101-110, and the worker reads,
+// vx3_node_worker.cu:63-75): the base VXA path, the input directory and the VXD
+// file names under vxd/f, in order.
+bool read_task(const std::string &path, std::string &base, std::string &dir, std::vector<std::string> &files) {
+    FILE *f = std::fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::string s;
+    char buf[4096];
+    for (size_t n; (n = std::fread(buf, 1, sizeof(buf), f)) > 0;) s.append(buf, n);
+    std::fclose(f);
+    std::string vxd;
+    if (!element(s, "vxa", 0, s.size(), base) || !element(s, "input_dir", 0, s.size(), dir) ||
+        !element(s, "vxd", 0, s.size(), vxd))
+        return false;
+    base = xml_unescape(base);
+    dir = xml_unescape(dir);
+    std::string name;
+    for (size_t at = 0, end = 0; element(vxd, "f", at, vxd.size(), name, &end); at = end) files.push_back(xml_unescape(name));
+    return true;
 }
 
 std::string xml_escape(const std::string &s) {
@@ -114,29 +161,43 @@ int main(int argc, char **argv) {
         usage();
         return 1;
     }
-    if (!is_dir(input)) { std::printf("Error: input directory not found.\n\n"); usage(); return 1; }
-    std::string dir = input;
-    if (dir.back() != '/') dir += '/';
-    const std::string base = dir + "base.vxa";
-    if (!is_file(base)) { std::printf("No base.vxa found in input directory.\n\n"); usage(); return 1; }
+    std::string dir = input, base, reportDir = input;
+    std::vector<std::string> files;
+    const bool task = is_file(input);
+    if (task) {
+        if (!read_task(input, base, dir, files)) {
+            std::printf("Error: %s is not a task file (vxa, input_dir, vxd/f).\n\n", input.c_str());
+            usage();
+            return 1;
+        }
+        reportDir = dir;
+        if (!dir.empty() && dir.back() != '/') dir += '/';
+        if (!is_file(base)) { std::printf("Error: base VXA %s not found.\n", base.c_str()); return 1; }
+    } else {
+        if (!is_dir(input)) { std::printf("Error: input directory not found.\n\n"); usage(); return 1; }
+        if (dir.back() != '/') dir += '/';
+        base = dir + "base.vxa";
+        if (!is_file(base)) { std::printf("No base.vxa found in input directory.\n\n"); usage(); return 1; }
+    }
     int mode = VX3MI_MODE_AUTO;
     if (modeName == "tiled") mode = VX3MI_MODE_TILED;
     else if (modeName == "persistent") mode = VX3MI_MODE_PERSISTENT;
     else if (modeName == "flat") mode = VX3MI_MODE_FLAT;
     else if (modeName == "plain") mode = VX3MI_MODE_PLAIN;
 
-    std::vector<std::string> files;
-    if (DIR *d = opendir(input.c_str())) {
-        while (dirent *e = readdir(d)) {
-            std::string f = e->d_name;
-            if (f.size() < 4) continue;
-            std::string ext = f.substr(f.size() - 4);
-            for (char &c : ext) c = (char)std::tolower((unsigned char)c);
-            if (ext == ".vxd" && is_file(dir + f)) files.push_back(f);
+    if (!task) {  // a task file keeps its own order
+        if (DIR *d = opendir(input.c_str())) {
+            while (dirent *e = readdir(d)) {
+                std::string f = e->d_name;
+                if (f.size() < 4) continue;
+                std::string ext = f.substr(f.size() - 4);
+                for (char &c : ext) c = (char)std::tolower((unsigned char)c);
+                if (ext == ".vxd" && is_file(dir + f)) files.push_back(f);
+            }
+            closedir(d);
         }
-        closedir(d);
+        std::sort(files.begin(), files.end());
     }
-    std::sort(files.begin(), files.end());
     if (files.empty()) { std::printf("No .vxd files found in input directory.\n"); return 1; }
 
     std::vector<Sim> sims(files.size());
@@ -201,7 +262,7 @@ int main(int argc, char **argv) {
             return fa > fb;
         });
         std::string x = "<?xml version=\"1.0\" encoding=\"utf-8\"?>\n<report>";
-        x += "<inputdir>" + xml_escape(path_filename(input)) + "</inputdir>";
+        x += "<inputdir>" + xml_escape(path_filename(reportDir)) + "</inputdir>";
         const Sim &best = sims[order[0]];
         x += "<bestfit><filename>" + xml_escape(best.file) + "</filename><fitness_score>" + num(best.res.fitness_score) +
              "</fitness_score></bestfit><detail>";
