@@ -28,7 +28,9 @@ class _Material(C.Structure):
                 ("plastic_mod", C.c_float), ("yield_stress", C.c_float), ("model", C.c_int),
                 ("fixed", C.c_int), ("is_measured", C.c_int), ("thermal_on_after", C.c_double),
                 ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double),
-                ("ss_points", C.c_int), ("ss_strain", C.c_float * 32), ("ss_stress", C.c_float * 32)]
+                ("ss_points", C.c_int), ("ss_strain", C.c_float * 32), ("ss_stress", C.c_float * 32),
+                ("is_pacemaker", C.c_int), ("pacemaker_period", C.c_double), ("signal_decay", C.c_double),
+                ("signal_delay", C.c_double), ("inactive_period", C.c_double)]
 
 
 # MathTree token codes (vx3_oracle.h VXO_MT_*) and variable order
@@ -61,7 +63,8 @@ class _Spec(C.Structure):
                 ("temp_enabled", C.c_int), ("vary_temp", C.c_int),
                 ("temp_base", C.c_double), ("temp_amplitude", C.c_double), ("temp_period", C.c_double),
                 ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int),
-                ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double))]
+                ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double)),
+                ("enable_signals", C.c_int)]
 
 
 class _DiffOpts(C.Structure):
@@ -74,11 +77,13 @@ SPEC_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=
                      volume_effects=0, enable_collision=1, enable_expansion=0, grav_acc=-9.81,
                      floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
                      temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None,
-                     enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None)
+                     enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None, enable_signals=0)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, thermal_on_after=0.0,
-                         cilia=0.0, cilia_on_after=0.0, remove_after=0.0, ss_data=None)
+                         cilia=0.0, cilia_on_after=0.0, remove_after=0.0, ss_data=None,
+                         is_pacemaker=0, pacemaker_period=0.0, signal_decay=0.9, signal_delay=0.03,
+                         inactive_period=0.03)
 
 
 def build_library(force: bool = False) -> str:
@@ -116,6 +121,7 @@ def lib():
         L.vxo_set_voxels.argtypes = [vp, dp, dp, dp, dp]
         L.vxo_get_flags.argtypes = [vp, ip, ip]
         L.vxo_get_removed.argtypes = [vp, ip, ip]
+        L.vxo_get_signals.argtypes = [vp, dp]
         L.vxo_get_link_state.argtypes = [vp, dp, fp]
         L.vxo_get_temperature.argtypes = [vp, fp, dp]
         L.vxo_set_temperature.argtypes = [vp, fp]
@@ -166,7 +172,8 @@ class OracleModel:
                                       m["fail_stress"] if model else -1.0, m["plastic_mod"], m["yield_stress"],
                                       model, int(m["fixed"]), int(m["is_measured"]), m["thermal_on_after"],
                                       m["cilia"], m["cilia_on_after"], m["remove_after"], npts,
-                                      (C.c_float * 32)(*ss[0]), (C.c_float * 32)(*ss[1]))
+                                      (C.c_float * 32)(*ss[0]), (C.c_float * 32)(*ss[1]), int(m["is_pacemaker"]),
+                                      m["pacemaker_period"], m["signal_decay"], m["signal_delay"], m["inactive_period"])
         self._phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         self._E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
         self._cb, self._cs = [None if p[k] is None else np.ascontiguousarray(p[k], dtype=np.float64).reshape(-1)
@@ -178,7 +185,7 @@ class OracleModel:
                      int(p["floor_enabled"]), int(p["temp_enabled"]), int(p["vary_temp"]),
                      p["temp_base"], p["temp_amplitude"], p["temp_period"],
                      int(p["enable_cilia"]), int(p["secondary_experiment"]),
-                     _ptr(self._cb, C.c_double), _ptr(self._cs, C.c_double))
+                     _ptr(self._cb, C.c_double), _ptr(self._cs, C.c_double), int(p["enable_signals"]))
         self._L = lib()
         self._h = self._L.vxo_build(C.byref(spec))
         self.nv = self._L.vxo_num_voxels(self._h)
@@ -275,6 +282,12 @@ class OracleModel:
         vr = np.zeros(self.nv, dtype=np.int32); lr = np.zeros(max(1, self.nl), dtype=np.int32)
         self._L.vxo_get_removed(self._h, _ptr(vr, C.c_int), _ptr(lr, C.c_int))
         return vr, lr[:self.nl]
+
+    def signals(self) -> np.ndarray:
+        """Each voxel's localSignal."""
+        o = np.zeros(self.nv)
+        self._L.vxo_get_signals(self._h, _ptr(o, C.c_double))
+        return o
 
     def link_state(self):
         s = np.zeros((self.nl, 9)); st = np.zeros(self.nl, dtype=np.float32)

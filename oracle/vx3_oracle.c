@@ -51,6 +51,8 @@ typedef struct {
     int fixed, isMeasured;
     double thermalOnAfter;
     double cilia, ciliaOnAfter, removeAfter;
+    int isPaceMaker;
+    double paceMakerPeriod, signalDecay, signalDelay, inactivePeriod;
     /* stress-strain model: CVX_Material linear / strainData / stressData */
     int linear, npts;
     float sd[OM_MAX_POINTS], td[OM_MAX_POINTS];
@@ -94,6 +96,10 @@ struct vxo_model {
     v3 *ciliaBase, *ciliaShift;
     int *vremoved, *lremoved;
     long steps;
+    /* signals (VX3_Voxel.h:304-309): localSignal, localSignaldt, d_signal,
+     * inactiveUntil, packmakerNextPulse; the signal a voxel sent this step */
+    double *sLocal, *sLocalDt, *sValue, *sActive, *sInactive, *sNextPulse;
+    double *outValue, *outTime;
     /* links */
     int *lneg, *lpos, *laxis;
     lmat *lm;
@@ -108,7 +114,7 @@ struct vxo_model {
     /* settings */
     double dtFrac, optimalDt, time;
     int enableCollision, enableExpansion, varyTemp;
-    int enableCilia, secondary;
+    int enableCilia, secondary, enableSignals;
     double tempAmplitude, tempPeriod;
 };
 
@@ -509,6 +515,7 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->tempPeriod = sp->temp_period;
     m->enableCilia = sp->enable_cilia;
     m->secondary = sp->secondary_experiment;
+    m->enableSignals = sp->enable_signals;
 
     int nv = 0;
     for (int i = 0; i < ncell; i++)
@@ -556,6 +563,11 @@ vxo_model *vxo_build(const vxo_spec *sp) {
         vm->cilia = in->cilia;
         vm->ciliaOnAfter = in->cilia_on_after;
         vm->removeAfter = in->remove_after;
+        vm->isPaceMaker = in->is_pacemaker;
+        vm->paceMakerPeriod = in->pacemaker_period;
+        vm->signalDecay = in->signal_decay;
+        vm->signalDelay = in->signal_delay;
+        vm->inactivePeriod = in->inactive_period;
         vmat_derive(vm);
     }
 
@@ -584,6 +596,9 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->ciliaBase = (v3 *)xcalloc(nv, sizeof(v3));
     m->ciliaShift = (v3 *)xcalloc(nv, sizeof(v3));
     m->vremoved = (int *)xcalloc(nv, sizeof(int));
+    double **sig[8] = {&m->sLocal, &m->sLocalDt, &m->sValue, &m->sActive, &m->sInactive, &m->sNextPulse,
+                       &m->outValue, &m->outTime};
+    for (int k = 0; k < 8; k++) *sig[k] = (double *)xcalloc(nv, sizeof(double));
     for (int i = 0; i < nv; i++) {
         if (sp->cilia_base) m->ciliaBase[i] = v3_(sp->cilia_base[3 * i], sp->cilia_base[3 * i + 1], sp->cilia_base[3 * i + 2]);
         if (sp->cilia_shift)
@@ -691,7 +706,8 @@ void vxo_free(vxo_model *m) {
                     m->tStrainSum, m->restLen, m->lflags, m->fNeg, m->fPos, m->mNeg, m->mPos, m->vm, m->strainOffset,
                     m->ext_dof, m->ext_t, m->ext_r, m->ext_q, m->ext_f, m->ext_m,
                     m->ff_op[0], m->ff_op[1], m->ff_op[2], m->ff_val[0], m->ff_val[1], m->ff_val[2],
-                    m->ciliaBase, m->ciliaShift, m->vremoved, m->lremoved};
+                    m->ciliaBase, m->ciliaShift, m->vremoved, m->lremoved, m->sLocal, m->sLocalDt, m->sValue,
+                    m->sActive, m->sInactive, m->sNextPulse, m->outValue, m->outTime};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(m);
 }
@@ -1026,6 +1042,59 @@ static void update_collisions(vxo_model *m) {
     free(ent);
 }
 
+/* ---------------------------------------------------------------- signals */
+/* VX3_Voxel::receiveSignal VX3_Voxel.cu:301-322 (mat: the receiver's) */
+static void signal_receive(vxo_model *m, int v, const vmat *mt, double value, double activeTime, int force) {
+    if (!force && m->sInactive[v] > activeTime) return;
+    if (value < 0.1) return;
+    m->sInactive[v] = activeTime + mt->inactivePeriod;
+    m->sLocal[v] = value;
+    m->sValue[v] = value * mt->signalDecay;
+    if (m->sValue[v] < 0.1) m->sValue[v] = 0;
+    m->sActive[v] = activeTime;
+}
+/* the voxel's own signal section at the end of VX3_Voxel::timeStep
+ * (VX3_Voxel.cu:271-276): propagateSignal (:323-348; the send is kept in
+ * outValue / outTime for the receive phase), packMaker (:290-299),
+ * localSignalDecay (:279-288) */
+static void signal_own(vxo_model *m, int v, const vmat *mt) {
+    const double t = m->time;
+    m->outValue[v] = 0;
+    if (!(m->sActive[v] > t) && !(m->sValue[v] < 0.1)) {
+        m->outValue[v] = m->sValue[v];
+        m->outTime[v] = t + mt->signalDelay;
+        m->sValue[v] = 0;
+        m->sActive[v] = 0;
+        m->sInactive[v] = t + 2 * mt->signalDelay + mt->inactivePeriod;
+    }
+    if (mt->isPaceMaker && !(m->sNextPulse[v] > t)) {
+        signal_receive(m, v, mt, 100, t, 1);
+        m->sNextPulse[v] = t + mt->paceMakerPeriod;
+    }
+    if (!(m->sLocalDt[v] > t)) {
+        if (m->sLocal[v] < 0.1) {
+            m->sLocal[v] = 0;
+        } else {
+            m->sLocal[v] = m->sLocal[v] * 0.9;
+            m->sLocalDt[v] = t + 0.01;
+        }
+    }
+}
+/* the receive phase: each present voxel takes the signals its linked
+ * neighbours sent this step, in its link direction order */
+static void signal_deliver(vxo_model *m) {
+    for (int v = 0; v < m->nv; v++) {
+        if (m->vremoved[v]) continue;
+        const vmat *mt = &m->vm[m->vmat_of[v]];
+        for (int d = 0; d < 6; d++) {
+            int l = m->vlinks[v][d];
+            if (l < 0) continue;
+            int o = (d & 1) ? m->lneg[l] : m->lpos[l];
+            if (m->outValue[o] != 0) signal_receive(m, v, mt, m->outValue[o], m->outTime[o], 0);
+        }
+    }
+}
+
 /* ----------------------------------------------------------------- voxels */
 /* VX3_Voxel::timeStep VX3_Voxel.cu:162-277 with force() :350-380, moment()
  * :382-397, floorForce :399-426 (constant-zero force field); externals as at
@@ -1069,12 +1138,11 @@ static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
      * shiftCiliaForce) for surface voxels of a cilia material once
      * TurnOnCiliaAfterThisManySeconds has passed (gpu_update_cilia_force
      * VX3_VoxelyzeKernel.cu:846-858, before the voxel update); force() adds
-     * CiliaForce * mat->Cilia (VX3_Voxel.cu:376-377).  localSignal is 0 (no
-     * signals). */
+     * CiliaForce * mat->Cilia (VX3_Voxel.cu:376-377). */
     if (m->enableCilia) {
         v3 cf = v3_(0, 0, 0);
         if (m->is_surface[v] && mt->cilia != 0 && !(mt->ciliaOnAfter > m->time))
-            cf = qrot(orient, vadd(m->ciliaBase[v], vscale(m->ciliaShift[v], 0.0)));
+            cf = qrot(orient, vadd(m->ciliaBase[v], vscale(m->ciliaShift[v], m->sLocal[v])));
         tf = vadd(tf, vscale(cf, mt->cilia));
     }
     /* force field (VX3_Voxel.cu:177-183) at the position entering the step and
@@ -1169,6 +1237,7 @@ static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
     m->linMom[v] = linMom;
     m->angMom[v] = angMom;
     m->vflags[v] = flags;
+    if (m->enableSignals) signal_own(m, v, mt);
 }
 
 static void update_voxels(vxo_model *m, float dt, int cpu_semantics) {
@@ -1179,6 +1248,7 @@ static void update_voxels(vxo_model *m, float dt, int cpu_semantics) {
         if (m->vm[m->vmat_of[v]].fixed) continue;
         voxel_update(m, v, dt, cpu_semantics);
     }
+    if (m->enableSignals) signal_deliver(m);
 }
 
 /* VX3_VoxelyzeKernel::removeVoxels VX3_VoxelyzeKernel.cu:365-399, called at
@@ -1272,6 +1342,10 @@ void vxo_get_flags(const vxo_model *m, int *voxel_flags, int *link_flags) {
 void vxo_get_removed(const vxo_model *m, int *voxel_step, int *link_step) {
     if (voxel_step) memcpy(voxel_step, m->vremoved, sizeof(int) * m->nv);
     if (link_step) memcpy(link_step, m->lremoved, sizeof(int) * m->nl);
+}
+
+void vxo_get_signals(const vxo_model *m, double *local) {
+    memcpy(local, m->sLocal, sizeof(double) * m->nv);
 }
 
 void vxo_get_link_state(const vxo_model *m, double *state9, float *strain) {

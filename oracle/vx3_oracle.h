@@ -17,12 +17,19 @@
  *   voxels       src/VX3/VX3_Voxel.cu:162-277, 350-426, VX3_Voxel.h:185-208
  *   cilia        src/VX3/VX3_VoxelyzeKernel.cu:296-302, 846-858, VX3_Voxel.cu:376-377
  *   removal      src/VX3/VX3_VoxelyzeKernel.cu:336-399, 495-513 (SecondaryExperiment)
+ *   signals      src/VX3/VX3_Voxel.cu:271-348 (EnableSignals)
  *   CoM          src/VX3/VX3_VoxelyzeKernel.cu:477-493
  *
  * Deliberate (documented) deviations from the racy reference:
  *   - both endpoint temperatures used for a link rest length are the new ones;
  *   - contact forces are summed serially over surface pairs (first > second);
- *   - divergence is "any link axial strain > 100" instead of one random link.
+ *   - divergence is "any link axial strain > 100" instead of one random link;
+ *   - signals: VX3_Voxel::propagateSignal writes a neighbour's signal state from
+ *     inside the parallel voxel update, so the reference's result depends on
+ *     thread order.  Here every voxel first runs its own signal section
+ *     (propagate, pacemaker, decay), then receives the signals its neighbours
+ *     sent in that step, in its link direction order X_POS..Z_NEG: the
+ *     interleaving in which a voxel's own section precedes what it receives.
  */
 #ifndef VX3_ORACLE_H
 #define VX3_ORACLE_H
@@ -52,6 +59,11 @@ typedef struct {
     double remove_after;     /* RemoveFromSimulationAfterThisManySeconds (0: never) */
     int ss_points;           /* model 3: SSData points (strain, stress), as given */
     float ss_strain[VXO_MAX_SS_POINTS], ss_stress[VXO_MAX_SS_POINTS];
+    int is_pacemaker;        /* isPaceMaker */
+    double pacemaker_period; /* PaceMakerPeriod (s) */
+    double signal_decay;     /* signalValueDecay */
+    double signal_delay;     /* signalTimeDelay (s) */
+    double inactive_period;  /* inactivePeriod (s) */
 } vxo_material;
 
 /* Lattice + simulator + environment description. */
@@ -76,6 +88,7 @@ typedef struct {
     int secondary_experiment;   /* Simulator/SecondaryExperiment (material removal) */
     const double *cilia_base;   /* per-voxel BaseCiliaForce (3 per voxel, voxel order) or NULL */
     const double *cilia_shift;  /* per-voxel ShiftCiliaForce or NULL */
+    int enable_signals;         /* Simulator/EnableSignals */
 } vxo_spec;
 
 typedef struct vxo_model vxo_model;
@@ -133,6 +146,8 @@ void vxo_get_flags(const vxo_model *m, int *voxel_flags, int *link_flags);
  * (1-based, counted over vxo_step calls) at whose end it was removed, 0 if
  * present */
 void vxo_get_removed(const vxo_model *m, int *voxel_step, int *link_step);
+/* signals: each voxel's localSignal */
+void vxo_get_signals(const vxo_model *m, double *local);
 void vxo_get_link_state(const vxo_model *m, double *state9, float *strain);
 void vxo_get_temperature(const vxo_model *m, float *temp, double *rest_length);
 void vxo_set_temperature(vxo_model *m, const float *temp);

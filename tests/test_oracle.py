@@ -15,7 +15,9 @@ wrong sign or axis, a transposed operand) fails it:
   * cilia: a turned free voxel follows the rotated force exactly; a cube gains
     momentum from its 26 surface voxels only, from the turn-on time on;
   * removal: the step it happens, frozen removed voxels, free remainder, and
-    the regenerated surface (the exposed former centre gets cilia force).
+    the regenerated surface (the exposed former centre gets cilia force);
+  * signals: a pacemaker pulse runs down a bar at one signalTimeDelay per hop
+    with value 100 rho^k, never echoes; cilia force scales with the signal.
 """
 import os
 import subprocess
@@ -423,3 +425,61 @@ def test_removal_regenerates_the_surface_for_cilia():
     m.step(100, dt)
     p2 = m.state()["linmom"][keep, 0].sum()
     assert p2 - p1 == pytest.approx(100 * float(dt) * 18 * F, rel=1e-6)
+
+
+def _chain(n=10, rho=0.8, delay=0.002, inactive=0.003, period=0.05, **kw):
+    """A free n x 1 x 1 bar whose voxel 0 is a pacemaker (no gravity, floor,
+    actuation or damping)."""
+    cells = np.ones(n, dtype=np.int8)
+    cells[0] = 2
+    sig = dict(signal_decay=rho, signal_delay=delay, inactive_period=inactive)
+    p = quiet((n, 1, 1), cells=cells, enable_signals=1)
+    p["materials"] = [dict(E=1e6, rho=1000.0, **sig), dict(E=1e6, rho=1000.0, is_pacemaker=1, pacemaker_period=period, **sig)]
+    p.update(kw)
+    return p
+
+
+def test_signal_front_speed_value_and_no_echo():
+    """A pacemaker pulse runs down a bar: voxel k first holds 100 rho^k (the
+    value decays by signalValueDecay per hop), one signalTimeDelay (rounded up
+    to a step) after voxel k - 1; the next own section decays it by 0.9; the
+    inactive period keeps the pulse from coming back, so every local signal
+    only falls until the next pacemaker pulse."""
+    rho, delay = 0.8, 0.002
+    m = O.OracleModel(_chain(rho=rho, delay=delay))
+    dt = float(np.float32(m.step_dt()))
+    n = 10
+    hist = []
+    for _ in range(600):
+        m.step(1)
+        hist.append(m.signals().copy())
+    hist = np.array(hist)  # step x voxel
+    t = np.arange(600) * dt  # start time of each step (exact enough for the bounds below)
+    first = [int(np.argmax(hist[:, k] > 0)) for k in range(n)]
+    assert first[0] == 0 and all(hist[first[k], k] > 0 for k in range(n))
+    for k in range(1, n):
+        assert hist[first[k], k] == pytest.approx(100 * rho ** k, rel=1e-12), k
+        assert hist[first[k] + 1, k] == pytest.approx(0.9 * 100 * rho ** k, rel=1e-12), k
+        if k > 1:
+            gap = t[first[k]] - t[first[k - 1]]
+            assert delay - 1e-9 <= gap <= delay + dt + 1e-9, (k, gap)
+        assert np.all(np.diff(hist[first[k]:, k]) <= 0), k
+
+
+def test_cilia_force_scales_with_local_signal():
+    """A lone pacemaker voxel with a ShiftCiliaForce and no base force: its
+    momentum grows by dt Cilia localSignal shift per step (the signal entering
+    the step), checked wherever the signal stays constant."""
+    p = _chain(n=1, period=1.0, enable_cilia=1, cilia_shift=np.array([[1e-3, 0.0, 0.0]]))
+    p["materials"][1].update(cilia=2.0)
+    m = O.OracleModel(p)
+    dt = float(np.float32(m.step_dt()))
+    prev_p, prev_s = m.state()["linmom"][0, 0], m.signals()[0]
+    checked = 0
+    for _ in range(400):
+        m.step(1)
+        px, s = m.state()["linmom"][0, 0], m.signals()[0]
+        assert px - prev_p == pytest.approx(dt * 2.0 * prev_s * 1e-3, rel=1e-9, abs=1e-18)
+        checked += prev_s > 0
+        prev_p, prev_s = px, s
+    assert checked > 300 and 0 < prev_s < 90
