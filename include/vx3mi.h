@@ -66,6 +66,11 @@ typedef struct {
     double remove_after;     /* RemoveFromSimulationAfterThisManySeconds (with secondary_experiment; 0 never) */
     int ss_points;           /* mat_model 3: SSData points as given (a leading 0, 0 is allowed) */
     float ss_strain[VX3MI_MAX_SS_POINTS], ss_stress[VX3MI_MAX_SS_POINTS]; /* SSData StrainData / StressData */
+    int is_pacemaker;        /* isPaceMaker (with enable_signals) */
+    double pacemaker_period; /* PaceMakerPeriod (s) */
+    double signal_value_decay; /* signalValueDecay (a VXA without it: 0.9) */
+    double signal_time_delay;  /* signalTimeDelay (s; a VXA without it: 0.03) */
+    double inactive_period;    /* inactivePeriod (s; a VXA without it: 0.03) */
 } vx3mi_material;
 
 /* Lattice description: the information content of a merged VXA (base.vxa +
@@ -91,6 +96,7 @@ typedef struct {
     int secondary_experiment;      /* Simulator/SecondaryExperiment (material removal, re-initialisation) */
     const double *cilia_base;      /* per-voxel BaseCiliaForce (N, 3 per voxel, voxel order) or NULL */
     const double *cilia_shift;     /* per-voxel ShiftCiliaForce (scaled by the voxel's signal) or NULL */
+    int enable_signals;            /* Simulator/EnableSignals */
 } vx3mi_lattice;
 
 /* Settings a model carries besides topology and materials, read back by
@@ -105,7 +111,7 @@ typedef struct {
     float initial_temp;            /* temperature set at import (UpdateMatTemps) */
     int record_step_size, record_voxel, record_link; /* Simulator/RecordHistory */
     int has_stop_formula, has_fitness_formula;
-    int enable_cilia, secondary_experiment;
+    int enable_cilia, secondary_experiment, enable_signals;
 } vx3mi_settings;
 
 typedef struct vx3mi_model vx3mi_model;
@@ -215,6 +221,14 @@ int vx3mi_get_voxel_flags(vx3mi_batch *b, int robot, int *flags);
  * present.  Removed voxels keep their last state; their links are gone from
  * both ends.  Batches that can lose voxels run in the plain and flat modes. */
 int vx3mi_batch_removed(vx3mi_batch *b, int robot, int *voxel_step, int *link_step);
+/* Signals (EnableSignals; VX3_Voxel::propagateSignal / packMaker /
+ * localSignalDecay / receiveSignal, src/VX3/VX3_Voxel.cu:271-348): each voxel's
+ * localSignal, which scales its ShiftCiliaForce.  The reference writes a
+ * neighbour's signal from inside the parallel voxel update; here each voxel runs
+ * its own signal section and then receives its neighbours' signals of that
+ * step in link direction order (X_POS..Z_NEG).  Batches with signals run in the
+ * plain and flat modes. */
+int vx3mi_batch_signals(vx3mi_batch *b, int robot, double *local);
 /* per-link flag words (bit 0 local velocity valid, bit 1 small-angle
  * formulation, VX3_Link.cu:117-131, 172-191) and axial strain of the last step,
  * link order; either pointer may be NULL */

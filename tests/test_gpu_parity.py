@@ -537,3 +537,62 @@ def test_data_materials_fp32_tiled():
     b.close()
     log(f"  fp32 data materials worst {worst:.3e}")
     assert worst <= TOL[vx3mi.FP32]
+
+
+def _signal_problems():
+    """A robot with signals driving cilia (pacemaker material, ShiftCiliaForce on
+    every voxel, collisions on) and the same robot with removal as well, plus
+    a plain robot."""
+    rng = np.random.default_rng(31)
+    a = configs.random_robot(rng, (6, 5, 3), 0.85, enable_collision=1)
+    nv = int((np.asarray(a["cells"]) > 0).sum())
+    sig = dict(signal_decay=0.85, signal_delay=0.0015, inactive_period=0.002)
+    a["materials"] = [dict(configs.SOFT, cilia=1.0, **sig),
+                      dict(configs.STIFF, cilia=1.0, is_pacemaker=1, pacemaker_period=0.004, **sig)]
+    a.update(enable_signals=1, enable_cilia=1, cilia_base=rng.normal(size=(nv, 3)) * 1e-4,
+             cilia_shift=rng.normal(size=(nv, 3)) * 2e-5)
+    b = dict(a, secondary_experiment=1,
+             materials=[a["materials"][0], dict(a["materials"][1], remove_after=0.005)])
+    return [a, b, small_problem(seed=8)]
+
+
+@pytest.mark.parametrize("mode", [vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT])
+def test_signals_fp64_bit_exact(mode):
+    """Signals (pacemakers, propagation with delay / inactive period / value decay,
+    local-signal decay) modulating cilia forces, with collisions and removal:
+    FP64 state and every voxel's local signal bit for bit with the oracle."""
+    log(f"signals mode {mode}")
+    probs = _signal_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
+    oracles = [O.OracleModel(p) for p in probs]
+    seen = 0
+    for _ in range(8):
+        b.step(100, mode=mode)
+        for r, o in enumerate(oracles):
+            o.step(100)
+            s, ref = b.state(r), o.state()
+            for k in ("pos", "quat", "linmom", "angmom"):
+                assert np.array_equal(s[k], ref[k]), (r, k)
+            assert np.array_equal(b.signals(r), o.signals()), r
+        seen = max(seen, int((b.signals(0) > 0).sum()))
+    assert seen > 10  # the pulses spread
+    b.close()
+
+
+def test_signals_fp32():
+    """The same batch in FP32 within 1e-5 of the oracle over 800 steps; the local
+    signals (a function of time alone) equal the oracle's."""
+    probs = _signal_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP32)
+    oracles = [O.OracleModel(p) for p in probs]
+    worst = 0.0
+    for _ in range(8):
+        b.step(100)
+        for r, o in enumerate(oracles):
+            o.step(100)
+            s, ref = b.state(r), o.state()
+            worst = max(worst, pos_err(s["pos"], ref["pos"]), quat_err(s["quat"], ref["quat"]))
+            assert np.array_equal(b.signals(r), o.signals()), r
+    b.close()
+    log(f"  fp32 signals worst {worst:.3e}")
+    assert worst <= TOL[vx3mi.FP32]

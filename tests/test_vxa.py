@@ -523,3 +523,19 @@ def test_rejected_data_keeps_the_default():
     assert len(one.link_curve(0)) == 0
     dec = vx3mi.Model.from_xml(W.write_vxa(dict(base, materials=[dict(rho=1000.0, ss_data=([0.001, 0.002], [2000.0, 1500.0]))])))
     assert len(dec.link_curve(0)) == 3
+
+
+def test_signal_keys_are_read_with_reference_defaults():
+    """Simulator/EnableSignals and the materials' isPaceMaker, PaceMakerPeriod,
+    signalValueDecay, signalTimeDelay and inactivePeriod; absent keys take
+    ReadXML's defaults (0.9, 0.03, 0.03; VX_Object.cpp:1420-1424)."""
+    p = configs.config0()
+    p = dict(p, enable_signals=1, materials=[dict(configs.SOFT, is_pacemaker=1, pacemaker_period=0.25,
+                                                  signal_decay=0.7, signal_delay=0.004, inactive_period=0.01)])
+    m = vx3mi.Model.from_xml(W.write_vxa(p))
+    assert m.settings()["enable_signals"] == 1
+    mat = m.material(1)
+    assert (mat["is_pacemaker"], mat["pacemaker_period"], mat["signal_value_decay"], mat["signal_time_delay"],
+            mat["inactive_period"]) == (1, 0.25, 0.7, 0.004, 0.01)
+    d = vx3mi.Model.from_xml(DEMO_LIKE).material(1)
+    assert (d["is_pacemaker"], d["signal_value_decay"], d["signal_time_delay"], d["inactive_period"]) == (0, 0.9, 0.03, 0.03)

@@ -99,6 +99,8 @@ struct BatchBase {
     // steps since the batch was created) at whose end it was removed, 0 while
     // present
     virtual int get_removed(int robot, int *vstep, int *lstep) = 0;
+    // each voxel's localSignal (zeros without signals)
+    virtual int get_signals(int robot, double *local) = 0;
     virtual int eval_links(double *out) = 0;
     virtual int com(double *out) = 0;
     virtual int recommended(double *out) = 0;

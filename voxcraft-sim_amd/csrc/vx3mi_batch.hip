@@ -118,6 +118,13 @@ int vx3mi_batch_removed(vx3mi_batch *b, int robot, int *voxel_step, int *link_st
     return b->b->get_removed(robot, voxel_step, link_step);
 }
 
+int vx3mi_batch_signals(vx3mi_batch *b, int robot, double *local) {
+    int rc = check_robot(b, robot);
+    if (rc) return rc;
+    if (!local) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    return b->b->get_signals(robot, local);
+}
+
 static int read_rt(vx3mi_batch *b, std::vector<RobotTime> &rt) {
     rt.resize(b->b->R);
     hipError_t e = hipStreamSynchronize(b->b->stream);

@@ -64,6 +64,8 @@ template <typename T> struct Dev {
     const int *remCand;       // voxels of materials with RemoveFromSimulationAfterThisManySeconds > 0
     const double *remAt;      // that time, per candidate
     int nRemCand;
+    VoxelSignal *sig;         // per-voxel signal state, or nullptr when no robot has EnableSignals
+    const SignalMat *sigMat;  // per-voxel signal constants
 };
 
 // ------------------------------------------------------------------ device
@@ -160,11 +162,12 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
     if (a.cilia) {
         // gpu_update_cilia_force (VX3_VoxelyzeKernel.cu:846-858): surface voxels of a
         // cilia material once TurnOnCiliaAfterThisManySeconds has passed, with the
-        // orientation entering the step; localSignal is 0 (signals are not built)
+        // orientation and the signal entering the step
         const VoxelCilia &c = a.cilia[v];
         if (surface && c.coef != 0 && !(c.onAfter > rt->time[parity])) {
-            const V3<T> b = {(T)(c.base[0] + 0.0 * c.shift[0]), (T)(c.base[1] + 0.0 * c.shift[1]),
-                             (T)(c.base[2] + 0.0 * c.shift[2])};
+            const double ls = a.sig ? a.sig[v].local : 0.0;  // localSignal
+            const V3<T> b = {(T)(c.base[0] + ls * c.shift[0]), (T)(c.base[1] + ls * c.shift[1]),
+                             (T)(c.base[2] + ls * c.shift[2])};
             cil = qrot(s.q, b) * (T)c.coef;
         }
     }
@@ -179,6 +182,33 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
     a.pq1[v] = T4<T>{s.q.x, s.q.y, s.q.z, (T)temp};
     a.mo0[v] = T4<T>{s.p.x, s.p.y, s.p.z, s.h.x};
     a.mo1[v] = T4<T>{s.h.y, s.h.z, (T)s.flags, (T)0};
+    // the voxel's own signal section ends a full timeStep (not one of a voxel
+    // with all six degrees of freedom fixed, VX3_Voxel.cu:166-172)
+    if (a.sig && dt != 0.0f && !(vm.ext >= 0 && a.exts[vm.ext].dof == DOF_ALL)) {
+        VoxelSignal sg = a.sig[v];
+        signal_own(sg, a.sigMat[v], rt->time[parity]);
+        a.sig[v] = sg;
+    }
+}
+
+// the receive phase of signals: each present voxel takes the signals its linked
+// neighbours sent in this step, in its link direction order (the own section of
+// every voxel precedes what it receives)
+template <typename T> __global__ __launch_bounds__(256) void k_signal_deliver(Dev<T> a, int v0, int n) {
+    const int v = v0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= v0 + n) return;
+    if (a.rt[a.vrob[v]].diverged || (a.vrem && a.vrem[v])) return;
+    const SignalMat sm = a.sigMat[v];
+    for (int d = 0; d < 6; d++) {
+        const int l = a.vlinks[6 * v + d];
+        if (l < 0) continue;
+        const int o = (d & 1) ? a.lends[l].x : a.lends[l].y;
+        const double value = a.sig[o].outValue;
+        if (value != 0) {
+            VoxelSignal &me = a.sig[v];
+            signal_receive(me, sm, value, a.sig[o].outTime, false);
+        }
+    }
 }
 
 // ---- force fields (VX3_Voxel.cu:177-183) ---------------------------------------
@@ -479,9 +509,10 @@ template <typename T> struct Batch : BatchBase {
     bool tileOK = false;          // some tile plan exists
     bool anyForceField = false;   // some robot has a force field (plain and flat modes only)
     bool anyCilia = false;        // some robot has EnableCilia (not in the tiles)
-    // plain and flat modes only: force fields (their own kernel) and removal (a
-    // kernel between steps)
-    bool plainFlatOnly() const { return anyForceField || anyRemoval; }
+    bool anySignals = false;      // some robot has EnableSignals
+    // plain and flat modes only: force fields (their own kernel), removal and
+    // signals (kernels between steps)
+    bool plainFlatOnly() const { return anyForceField || anyRemoval || anySignals; }
     // launch configuration used by VX3MI_MODE_AUTO (set by the autotuner)
     int autoMode = 0, tileSel = 0, persistThreads = 256;
     int nLinkMats = 0;  // entries of the device link material table
@@ -506,6 +537,7 @@ template <typename T> struct Batch : BatchBase {
         std::vector<MtToken> formulas;
         std::vector<T4<T>> pq0, pq1;
         std::vector<VoxelCilia> cilia;
+        std::vector<SignalMat> sigMat;
         std::vector<int> remCand;
         std::vector<double> remAt;
         std::map<std::string, int> vmIndex, lmIndex, curveIndex;
@@ -540,6 +572,7 @@ template <typename T> struct Batch : BatchBase {
             p.secondary = m.secondaryExperiment;
             p.voff = voff; p.vcount = m.nv(); p.loff = loff; p.lcount = m.nl();
             if (m.enableCilia) anyCilia = true;
+            if (m.enableSignals) anySignals = true;
             for (int ax = 0; ax < 3; ax++) {
                 std::vector<MtToken> post;
                 if (!m.forceField[ax].empty() && !mt_postfix(m.forceField[ax], post)) {
@@ -602,6 +635,13 @@ template <typename T> struct Batch : BatchBase {
                 c.coef = m.enableCilia ? pm.cilia : 0.0;
                 c.onAfter = pm.cilia_on_after;
                 cilia.push_back(c);
+                SignalMat sm{};
+                sm.period = pm.pacemaker_period;
+                sm.decay = pm.signal_value_decay;
+                sm.delay = pm.signal_time_delay;
+                sm.inactive = pm.inactive_period;
+                sm.pacemaker = m.enableSignals && pm.is_pacemaker;
+                sigMat.push_back(sm);
                 if (m.secondaryExperiment && pm.remove_after > 0) {
                     remCand.push_back(voff + v);
                     remAt.push_back(pm.remove_after);
@@ -784,6 +824,17 @@ template <typename T> struct Batch : BatchBase {
             HIP_TRY(hipDeviceSynchronize());
             d.vrem = vr; d.lrem = lr; d.remCand = rc0; d.remAt = ra; d.nRemCand = nc;
         }
+        d.sig = nullptr;
+        d.sigMat = nullptr;
+        if (anySignals) {
+            VoxelSignal *sd;
+            SignalMat *smd;
+            if ((rc = alloc(&sd, V)) || (rc = alloc(&smd, V))) return rc;
+            HIP_TRY(hipMemset(sd, 0, sizeof(VoxelSignal) * V));
+            HIP_TRY(hipMemcpy(smd, sigMat.data(), sizeof(SignalMat) * V, hipMemcpyHostToDevice));
+            d.sig = sd;
+            d.sigMat = smd;
+        }
         return plan_tiles();
     }
 
@@ -795,7 +846,7 @@ template <typename T> struct Batch : BatchBase {
         using C = TileCfg<T, SMALL>;
         TilePlan &P = plans[SMALL];
         P.ok = false;
-        if (anyCollisions || anyForceField || anyCilia || anyRemoval || R < 2) return VX3MI_OK;
+        if (anyCollisions || anyForceField || anyCilia || anyRemoval || anySignals || R < 2) return VX3MI_OK;
         long totL = 0, totV = 0;
         for (const RobotParams &p : hrp) {
             if (p.vcount > C::VMAX || p.lcount > C::LMAX) return VX3MI_OK;
@@ -892,10 +943,11 @@ template <typename T> struct Batch : BatchBase {
         if (anyCollisions) k_col_contact<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
         if (anyForceField) k_force_field<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         k_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
+        if (anySignals) k_signal_deliver<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V);
         if (anyRemoval) k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
     }
     int kernels_per_flat_step() const {
-        return 2 + (anyCollisions ? 2 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0);
+        return 2 + (anyCollisions ? 2 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0);
     }
 
     int step(int nsteps, float dt, int mode) override {
@@ -909,7 +961,7 @@ template <typename T> struct Batch : BatchBase {
         }
         if (mode == VX3MI_MODE_AUTO) mode = autoMode;
         if (mode == VX3MI_MODE_PERSISTENT && plainFlatOnly()) {
-            set_error("force fields and material removal run in the plain and flat modes only");
+            set_error("force fields, material removal and signals run in the plain and flat modes only");
             return VX3MI_E_ARG;
         }
         HIP_TRY(hipEventRecord(ev0, stream));
@@ -924,7 +976,8 @@ template <typename T> struct Batch : BatchBase {
                     if (p.scount) k_col_contact<T><<<(p.scount + 255) / 256, 256, 0, stream>>>(d, p.soff, p.scount, parity);
                     if (anyForceField) k_force_field<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
                     k_voxels<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
-                    launches += 2 + (p.scount ? 2 : 0) + (anyForceField ? 1 : 0);
+                    if (anySignals) k_signal_deliver<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount);
+                    launches += 2 + (p.scount ? 2 : 0) + (anyForceField ? 1 : 0) + (anySignals ? 1 : 0);
                 }
                 if (anyRemoval) {
                     k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
@@ -1055,6 +1108,7 @@ template <typename T> struct Batch : BatchBase {
             bufs.push_back({d.ccnt[1], sizeof(int) * (size_t)tableBuckets});
             bufs.push_back({d.contactW, sizeof(T4<T>) * V});
         }
+        if (anySignals) bufs.push_back({d.sig, sizeof(VoxelSignal) * (size_t)V});
         if (anyRemoval) {
             bufs.push_back({d.vlinks, sizeof(int) * 6 * (size_t)V});
             bufs.push_back({d.vrem, sizeof(int) * (size_t)V});
@@ -1129,7 +1183,7 @@ template <typename T> struct Batch : BatchBase {
         }
         if (mode < VX3MI_MODE_PLAIN || mode > VX3MI_MODE_TILED) { set_error("invalid mode"); return VX3MI_E_ARG; }
         if (mode == VX3MI_MODE_PERSISTENT && plainFlatOnly()) {
-            set_error("force fields and material removal run in the plain and flat modes only");
+            set_error("force fields, material removal and signals run in the plain and flat modes only");
             return VX3MI_E_ARG;
         }
         autoMode = mode;
@@ -1201,6 +1255,20 @@ template <typename T> struct Batch : BatchBase {
         }
         if (vstep && v1 > v0) HIP_TRY(hipMemcpy(vstep, d.vrem + v0, sizeof(int) * (v1 - v0), hipMemcpyDeviceToHost));
         if (lstep && l1 > l0) HIP_TRY(hipMemcpy(lstep, d.lrem + l0, sizeof(int) * (l1 - l0), hipMemcpyDeviceToHost));
+        return VX3MI_OK;
+    }
+
+    int get_signals(int robot, double *local) override {
+        int v0 = 0, v1 = V;
+        if (robot >= 0) { v0 = hrp[robot].voff; v1 = v0 + hrp[robot].vcount; }
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (!anySignals) {
+            std::fill(local, local + (v1 - v0), 0.0);
+            return VX3MI_OK;
+        }
+        std::vector<VoxelSignal> h((size_t)(v1 - v0));
+        if (v1 > v0) HIP_TRY(hipMemcpy(h.data(), d.sig + v0, sizeof(VoxelSignal) * (v1 - v0), hipMemcpyDeviceToHost));
+        for (int v = 0; v < v1 - v0; v++) local[v] = h[v].local;
         return VX3MI_OK;
     }
 

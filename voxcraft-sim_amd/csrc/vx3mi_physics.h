@@ -104,6 +104,56 @@ struct VoxelCilia {
     double coef, onAfter;
 };
 
+// Signal state of a voxel (VX3_Voxel.h:304-309) and the signal it sent in the
+// current step (value 0: none), with its material's signal constants.
+struct VoxelSignal {
+    double local, localDt;          // localSignal, localSignaldt
+    double value, activeTime;       // d_signal
+    double inactiveUntil, nextPulse;
+    double outValue, outTime;
+};
+struct SignalMat {
+    double period, decay, delay, inactive;  // PaceMakerPeriod, signalValueDecay, signalTimeDelay, inactivePeriod
+    int pacemaker, pad;
+};
+
+// VX3_Voxel::receiveSignal (VX3_Voxel.cu:301-322); sm: the receiver's material
+VX_HD void signal_receive(VoxelSignal &s, const SignalMat &sm, double value, double activeTime, bool force) {
+    if (!force && s.inactiveUntil > activeTime) return;
+    if (value < 0.1) return;
+    s.inactiveUntil = activeTime + sm.inactive;
+    s.local = value;
+    s.value = value * sm.decay;
+    if (s.value < 0.1) s.value = 0;
+    s.activeTime = activeTime;
+}
+// a voxel's own signal section at the end of VX3_Voxel::timeStep
+// (VX3_Voxel.cu:271-276): propagateSignal (:323-348; the send is kept in
+// outValue / outTime for the receive phase), packMaker (:290-299),
+// localSignalDecay (:279-288); t is the time at the start of the step
+VX_HD void signal_own(VoxelSignal &s, const SignalMat &sm, double t) {
+    s.outValue = 0;
+    if (!(s.activeTime > t) && !(s.value < 0.1)) {
+        s.outValue = s.value;
+        s.outTime = t + sm.delay;
+        s.value = 0;
+        s.activeTime = 0;
+        s.inactiveUntil = t + 2 * sm.delay + sm.inactive;
+    }
+    if (sm.pacemaker && !(s.nextPulse > t)) {
+        signal_receive(s, sm, 100, t, true);
+        s.nextPulse = t + sm.period;
+    }
+    if (!(s.localDt > t)) {
+        if (s.local < 0.1) {
+            s.local = 0;
+        } else {
+            s.local = s.local * 0.9;
+            s.localDt = t + 0.01;
+        }
+    }
+}
+
 // Link material (VX3_MaterialLink.h:27-36) plus per-end constants the link
 // kernel needs so it never touches the voxel material table.
 struct LinkMat {

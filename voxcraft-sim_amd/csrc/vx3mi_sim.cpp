@@ -113,7 +113,7 @@ std::string history_header(const Model &m) {
 // still present, and removed links are left out.
 std::string history_frame(const Model &m, long long j, double t, const double *pos, const double *q,
                           const float *temp, const float *strain, const int *vrem = nullptr,
-                          const int *lrem = nullptr, long long stampNow = 0) {
+                          const int *lrem = nullptr, long long stampNow = 0, const double *signal = nullptr) {
     std::string s;
     const double vs = 1 / 0.001;
     auto link_at = [&](int v, int d) {  // the voxel's link in direction d now, -1 none
@@ -162,7 +162,7 @@ std::string history_frame(const Model &m, long long j, double t, const double *p
             appendf(s, "%.1f,%.1f,%.1f,%.1f,%.1f,%.1f,", off[0][0] * vs, off[0][1] * vs, off[0][2] * vs, off[1][0] * vs,
                     off[1][1] * vs, off[1][2] * vs);
             appendf(s, "%d,", m.vpal[v]);
-            appendf(s, "%.1f,", 0.0);  // localSignal (0: signals are not built)
+            appendf(s, "%.1f,", signal ? signal[v] : 0.0);  // localSignal
             s += ";";
         }
         s += "<<<>>>";
@@ -203,6 +203,7 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
     std::vector<double> pos, quat;
     std::vector<float> temp, strain;
     std::vector<int> vrem, lrem;
+    std::vector<double> signal;
     int rc = b.com(com.data());
     if (rc) return rc;
     for (int r = 0; r < n; r++) {
@@ -356,8 +357,14 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
                     vr = vrem.data();
                     lr = lrem.data();
                 }
+                const double *sg = nullptr;
+                if (m.enableSignals) {
+                    signal.resize((size_t)m.nv());
+                    if ((rc = b.get_signals(r, signal.data()))) return rc;
+                    sg = signal.data();
+                }
                 const std::string f = history_frame(m, x.j - 1, x.t, pos.data(), quat.data(), temp.data(), strain.data(),
-                                                    vr, lr, (long long)steps0[r] + x.j);
+                                                    vr, lr, (long long)steps0[r] + x.j, sg);
                 history(user, r, f.data(), f.size());
             }
         }

@@ -514,7 +514,7 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
         int v;
         if (load(sim, "EnableExpansion", v)) L.enable_expansion = v;
         if (load(sim, "EnableCilia", v)) L.enable_cilia = v != 0;
-        if (load(sim, "EnableSignals", v) && v) { err = "EnableSignals is not supported"; return VX3MI_E_PARSE; }
+        if (load(sim, "EnableSignals", v)) L.enable_signals = v != 0;
         double d;
         if (load(sim, "MaxDistInVoxelLengthsToCountAsPair", d) && d > 0) {
             err = "MaxDistInVoxelLengthsToCountAsPair (close-pair counting) is not supported";
@@ -621,6 +621,9 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
             m.red = m.green = m.blue = 0.5f;
             m.alpha = 1.0f;
             m.is_measured = 1;
+            m.signal_value_decay = 0.9;
+            m.signal_time_delay = 0.03;
+            m.inactive_period = 0.03;
             int matType = 0;
             if (load(&mn, "MatType", matType) && matType != 0) { err = "only MatType 0 (single) materials are supported"; return VX3MI_E_PARSE; }
             if (const XNode *disp = mn.child("Display")) {
@@ -677,6 +680,12 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
                 load(mech, "Cilia", m.cilia);
                 load(mech, "TurnOnCiliaAfterThisManySeconds", m.cilia_on_after);
                 load(mech, "RemoveFromSimulationAfterThisManySeconds", m.remove_after);
+                // signal keys and their ReadXML defaults (VX_Object.cpp:1420-1424)
+                if (load(mech, "isPaceMaker", iv)) m.is_pacemaker = iv != 0;
+                load(mech, "PaceMakerPeriod", m.pacemaker_period);
+                load(mech, "signalValueDecay", m.signal_value_decay);
+                load(mech, "signalTimeDelay", m.signal_time_delay);
+                load(mech, "inactivePeriod", m.inactive_period);
             }
             c.mats.push_back(m);
         }

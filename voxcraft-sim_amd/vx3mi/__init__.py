@@ -31,12 +31,13 @@ PROBLEM_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp
                         volume_effects=0, enable_collision=1, enable_expansion=0, grav_acc=-9.81,
                         floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
                         temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None,
-                        enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None)
+                        enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None, enable_signals=0)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, is_target=0, thermal_on_after=0.0,
                          cilia=0.0, cilia_on_after=0.0, remove_after=0.0, ss_data=None,
-                         rgba=(0.5, 0.5, 0.5, 1.0))
+                         is_pacemaker=0, pacemaker_period=0.0, signal_decay=0.9, signal_delay=0.03,
+                         inactive_period=0.03, rgba=(0.5, 0.5, 0.5, 1.0))
 
 
 class VX3MIError(RuntimeError):
@@ -62,7 +63,9 @@ class _Material(C.Structure):
                 ("is_measured", C.c_int), ("is_target", C.c_int), ("thermal_on_after", C.c_double),
                 ("red", C.c_float), ("green", C.c_float), ("blue", C.c_float), ("alpha", C.c_float),
                 ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double),
-                ("ss_points", C.c_int), ("ss_strain", C.c_float * 32), ("ss_stress", C.c_float * 32)]
+                ("ss_points", C.c_int), ("ss_strain", C.c_float * 32), ("ss_stress", C.c_float * 32),
+                ("is_pacemaker", C.c_int), ("pacemaker_period", C.c_double), ("signal_value_decay", C.c_double),
+                ("signal_time_delay", C.c_double), ("inactive_period", C.c_double)]
 
 
 class _Settings(C.Structure):
@@ -72,7 +75,7 @@ class _Settings(C.Structure):
                 ("temp_amplitude", C.c_double), ("temp_period", C.c_double), ("initial_temp", C.c_float),
                 ("record_step_size", C.c_int), ("record_voxel", C.c_int), ("record_link", C.c_int),
                 ("has_stop_formula", C.c_int), ("has_fitness_formula", C.c_int),
-                ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int)]
+                ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int), ("enable_signals", C.c_int)]
 
 
 FORMULA_STOP, FORMULA_FITNESS = 0, 1
@@ -105,7 +108,8 @@ class _Lattice(C.Structure):
                 ("grav_acc", C.c_float), ("floor_enabled", C.c_int), ("temp_enabled", C.c_int),
                 ("vary_temp", C.c_int), ("temp_base", C.c_double), ("temp_amplitude", C.c_double),
                 ("temp_period", C.c_double), ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int),
-                ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double))]
+                ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double)),
+                ("enable_signals", C.c_int)]
 
 
 def library_path() -> str:
@@ -155,6 +159,7 @@ def lib():
         "vx3mi_get_voxel_flags": (C.c_int, [vp, C.c_int, ip]),
         "vx3mi_get_link_flags": (C.c_int, [vp, C.c_int, ip, fp]),
         "vx3mi_batch_removed": (C.c_int, [vp, C.c_int, ip, ip]),
+        "vx3mi_batch_signals": (C.c_int, [vp, C.c_int, dp]),
         "vx3mi_model_cilia": (C.c_int, [vp, dp, dp]),
         "vx3mi_model_material": (C.c_int, [vp, C.c_int, C.POINTER(_Material)]),
         "vx3mi_batch_collision_overflow": (C.c_int, [vp, ip]),
@@ -229,7 +234,9 @@ class Model:
                                 float(m["plastic_mod"]), float(m["yield_stress"]), material_model(m),
                                 int(m["fixed"]), int(m["is_measured"]), int(m["is_target"]), m["thermal_on_after"],
                                 r, g, b, a, float(m["cilia"]), float(m["cilia_on_after"]), float(m["remove_after"]),
-                                len(ss[0]), (C.c_float * 32)(*ss[0]), (C.c_float * 32)(*ss[1]))
+                                len(ss[0]), (C.c_float * 32)(*ss[0]), (C.c_float * 32)(*ss[1]), int(m["is_pacemaker"]),
+                                float(m["pacemaker_period"]), float(m["signal_decay"]), float(m["signal_delay"]),
+                                float(m["inactive_period"]))
         phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
         cb, cs = [None if p[k] is None else np.ascontiguousarray(p[k], dtype=np.float64).reshape(-1)
@@ -240,7 +247,7 @@ class Model:
                         int(p["enable_expansion"]), p["grav_acc"], int(p["floor_enabled"]),
                         int(p["temp_enabled"]), int(p["vary_temp"]), p["temp_base"], p["temp_amplitude"],
                         p["temp_period"], int(p["enable_cilia"]), int(p["secondary_experiment"]),
-                        _ptr(cb, C.c_double), _ptr(cs, C.c_double))
+                        _ptr(cb, C.c_double), _ptr(cs, C.c_double), int(p["enable_signals"]))
         h = C.c_void_p()
         _check(lib().vx3mi_model_from_lattice(C.byref(spec), C.byref(h)))
         m = cls(h.value)
@@ -435,6 +442,12 @@ class Batch:
         vr, lr = np.zeros(nv, dtype=np.int32), np.zeros(max(1, nl), dtype=np.int32)
         _check(lib().vx3mi_batch_removed(self._h, int(robot), _ptr(vr, C.c_int), _ptr(lr, C.c_int)))
         return vr, lr[:nl]
+
+    def signals(self, robot: int = -1) -> np.ndarray:
+        """Each voxel's localSignal (EnableSignals)."""
+        o = np.zeros(self.nv if robot < 0 else self.models[robot].nv)
+        _check(lib().vx3mi_batch_signals(self._h, int(robot), _ptr(o, C.c_double)))
+        return o
 
     def time(self) -> np.ndarray:
         t = np.zeros(self.nrobot)

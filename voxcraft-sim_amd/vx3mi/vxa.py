@@ -55,6 +55,8 @@ def _simulator(p: dict, stop=None, fitness=None, record=None) -> str:
         parts.append(_el("EnableCilia", "1"))
     if p.get("secondary_experiment"):
         parts.append(_el("SecondaryExperiment", "1"))
+    if p.get("enable_signals"):
+        parts.append(_el("EnableSignals", "1"))
     if record is not None:
         step, vox, link = record
         parts.append(_el("RecordHistory", _el("RecordStepSize", str(int(step))) + _el("RecordVoxel", str(int(vox)))
@@ -123,6 +125,10 @@ def _palette(mats: Sequence[dict]) -> str:
             mech += _el("TurnOnCiliaAfterThisManySeconds", _num(m["cilia_on_after"]))
         if m["remove_after"]:
             mech += _el("RemoveFromSimulationAfterThisManySeconds", _num(m["remove_after"]))
+        if m["is_pacemaker"]:
+            mech += _el("isPaceMaker", "1") + _el("PaceMakerPeriod", _num(m["pacemaker_period"]))
+        mech += (_el("signalValueDecay", _num(m["signal_decay"])) + _el("signalTimeDelay", _num(m["signal_delay"]))
+                 + _el("inactivePeriod", _num(m["inactive_period"])))
         disp = _el("Red", _num(r)) + _el("Green", _num(g)) + _el("Blue", _num(b)) + _el("Alpha", _num(a))
         out.append(_el("Material", _el("Name", f"material{i + 1}") + _el("Display", disp) + _el("Mechanical", mech),
                        f' ID="{i + 1}"'))
