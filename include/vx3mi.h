@@ -97,6 +97,7 @@ typedef struct {
     const double *cilia_base;      /* per-voxel BaseCiliaForce (N, 3 per voxel, voxel order) or NULL */
     const double *cilia_shift;     /* per-voxel ShiftCiliaForce (scaled by the voxel's signal) or NULL */
     int enable_signals;            /* Simulator/EnableSignals */
+    int enable_detach;             /* Simulator/AttachDetach/EnableDetach: failed links leave their voxels */
 } vx3mi_lattice;
 
 /* Settings a model carries besides topology and materials, read back by
@@ -111,7 +112,7 @@ typedef struct {
     float initial_temp;            /* temperature set at import (UpdateMatTemps) */
     int record_step_size, record_voxel, record_link; /* Simulator/RecordHistory */
     int has_stop_formula, has_fitness_formula;
-    int enable_cilia, secondary_experiment, enable_signals;
+    int enable_cilia, secondary_experiment, enable_signals, enable_detach;
 } vx3mi_settings;
 
 typedef struct vx3mi_model vx3mi_model;
@@ -216,10 +217,12 @@ int vx3mi_set_state(vx3mi_batch *b, int robot, const double *pos3, const double 
 /* per-voxel flag words (floor / static friction bits as the reference) */
 int vx3mi_get_voxel_flags(vx3mi_batch *b, int robot, int *flags);
 /* Material removal (VX3_VoxelyzeKernel::removeVoxels, VX3_VoxelyzeKernel.cu:365-399,
- * with SecondaryExperiment): per voxel and link of the robot, the number of the
- * step (counted from batch creation) at whose end it was removed, 0 while
- * present.  Removed voxels keep their last state; their links are gone from
- * both ends.  Batches that can lose voxels run in the plain and flat modes. */
+ * with SecondaryExperiment) and detach (gpu_update_detach :946-968, with
+ * EnableDetach: a link past its failure strain leaves both voxels): per voxel
+ * and link of the robot, the number of the step (counted from batch creation)
+ * at whose end it was removed or detached, 0 while present.  Removed voxels
+ * keep their last state; their links are gone from both ends.  Batches that can
+ * lose voxels or links run in the plain and flat modes. */
 int vx3mi_batch_removed(vx3mi_batch *b, int robot, int *voxel_step, int *link_step);
 /* Signals (EnableSignals; VX3_Voxel::propagateSignal / packMaker /
  * localSignalDecay / receiveSignal, src/VX3/VX3_Voxel.cu:271-348): each voxel's

@@ -64,7 +64,7 @@ class _Spec(C.Structure):
                 ("temp_base", C.c_double), ("temp_amplitude", C.c_double), ("temp_period", C.c_double),
                 ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int),
                 ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double)),
-                ("enable_signals", C.c_int)]
+                ("enable_signals", C.c_int), ("enable_detach", C.c_int)]
 
 
 class _DiffOpts(C.Structure):
@@ -77,7 +77,8 @@ SPEC_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=
                      volume_effects=0, enable_collision=1, enable_expansion=0, grav_acc=-9.81,
                      floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
                      temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None,
-                     enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None, enable_signals=0)
+                     enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None, enable_signals=0,
+                     enable_detach=0)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, thermal_on_after=0.0,
@@ -185,7 +186,8 @@ class OracleModel:
                      int(p["floor_enabled"]), int(p["temp_enabled"]), int(p["vary_temp"]),
                      p["temp_base"], p["temp_amplitude"], p["temp_period"],
                      int(p["enable_cilia"]), int(p["secondary_experiment"]),
-                     _ptr(self._cb, C.c_double), _ptr(self._cs, C.c_double), int(p["enable_signals"]))
+                     _ptr(self._cb, C.c_double), _ptr(self._cs, C.c_double), int(p["enable_signals"]),
+                     int(p["enable_detach"]))
         self._L = lib()
         self._h = self._L.vxo_build(C.byref(spec))
         self.nv = self._L.vxo_num_voxels(self._h)

@@ -114,7 +114,7 @@ struct vxo_model {
     /* settings */
     double dtFrac, optimalDt, time;
     int enableCollision, enableExpansion, varyTemp;
-    int enableCilia, secondary, enableSignals;
+    int enableCilia, secondary, enableSignals, enableDetach;
     double tempAmplitude, tempPeriod;
 };
 
@@ -516,6 +516,7 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->enableCilia = sp->enable_cilia;
     m->secondary = sp->secondary_experiment;
     m->enableSignals = sp->enable_signals;
+    m->enableDetach = sp->enable_detach;
 
     int nv = 0;
     for (int i = 0; i < ncell; i++)
@@ -1248,7 +1249,6 @@ static void update_voxels(vxo_model *m, float dt, int cpu_semantics) {
         if (m->vm[m->vmat_of[v]].fixed) continue;
         voxel_update(m, v, dt, cpu_semantics);
     }
-    if (m->enableSignals) signal_deliver(m);
 }
 
 /* VX3_VoxelyzeKernel::removeVoxels VX3_VoxelyzeKernel.cu:365-399, called at
@@ -1256,10 +1256,10 @@ static void update_voxels(vxo_model *m, float dt, int cpu_semantics) {
  * is set (:336-349): every voxel of a material whose
  * RemoveFromSimulationAfterThisManySeconds is positive and below the current
  * time is removed with its links (both ends forget the link); the surface
- * list is regenerated before the next step's collisions and cilia (:282-287). */
-static void remove_voxels(vxo_model *m) {
-    int changed = 0;
-    for (int v = 0; v < m->nv; v++) {
+ * list is regenerated before the next step's collisions and cilia (:282-287)
+ * when this or a detach (changed) altered the topology. */
+static void remove_voxels(vxo_model *m, int changed) {
+    for (int v = 0; m->secondary && v < m->nv; v++) {
         const vmat *mt = &m->vm[m->vmat_of[v]];
         if (m->vremoved[v] || !(mt->removeAfter > 0 && mt->removeAfter < m->time)) continue;
         m->vremoved[v] = (int)(m->steps + 1);
@@ -1274,6 +1274,30 @@ static void remove_voxels(vxo_model *m) {
         changed = 1;
     }
     if (changed) regenerate_surface(m);
+}
+
+/* VX3_VoxelyzeKernel::updateDetach / gpu_update_detach
+ * (VX3_VoxelyzeKernel.cu:461-475, 946-968): a link whose maxStrain has passed its
+ * failure strain (VX3_Link::isFailed, VX3_Link.cu:78) leaves both voxels' link
+ * lists.  The reference does it after the collisions of the step; the surface
+ * list, cilia and collisions see it from the next step (regenerateSurfaceVoxels
+ * runs then), and the voxel update of the step sums the failed link's forces,
+ * which are zero (VX3_Link.cu:152-155), so it is applied here at the end of the
+ * step. */
+static int detach_links(vxo_model *m) {
+    int changed = 0;
+    for (int l = 0; l < m->nl; l++) {
+        if (m->lremoved[l]) continue;
+        const lmat *lm = &m->lm[l];
+        if (!(lm->epsFail != -1.0f && m->maxStrain[l] > lm->epsFail)) continue;
+        m->lremoved[l] = (int)(m->steps + 1);
+        for (int d = 0; d < 6; d++) {
+            if (m->vlinks[m->lneg[l]][d] == l) m->vlinks[m->lneg[l]][d] = -1;
+            if (m->vlinks[m->lpos[l]][d] == l) m->vlinks[m->lpos[l]][d] = -1;
+        }
+        changed = 1;
+    }
+    return changed;
 }
 
 /* ------------------------------------------------------------------- step */
@@ -1291,7 +1315,11 @@ int vxo_step(vxo_model *m, float dt, int nsteps, int flags) {
         if (diverged) return s;
         update_collisions(m);
         update_voxels(m, d, flags & VXO_CPU_SEMANTICS);
-        if (m->secondary) remove_voxels(m);
+        /* the end of the step: detached links no longer carry signals (the
+         * reference detaches before the voxel update), removal comes last */
+        int detached = m->enableDetach ? detach_links(m) : 0;
+        if (m->enableSignals) signal_deliver(m);
+        if (m->secondary || detached) remove_voxels(m, detached);
         m->time += d;
         m->steps++;
     }

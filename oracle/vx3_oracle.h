@@ -18,6 +18,7 @@
  *   cilia        src/VX3/VX3_VoxelyzeKernel.cu:296-302, 846-858, VX3_Voxel.cu:376-377
  *   removal      src/VX3/VX3_VoxelyzeKernel.cu:336-399, 495-513 (SecondaryExperiment)
  *   signals      src/VX3/VX3_Voxel.cu:271-348 (EnableSignals)
+ *   detach       src/VX3/VX3_VoxelyzeKernel.cu:461-475, 946-968 (EnableDetach)
  *   CoM          src/VX3/VX3_VoxelyzeKernel.cu:477-493
  *
  * Deliberate (documented) deviations from the racy reference:
@@ -89,6 +90,7 @@ typedef struct {
     const double *cilia_base;   /* per-voxel BaseCiliaForce (3 per voxel, voxel order) or NULL */
     const double *cilia_shift;  /* per-voxel ShiftCiliaForce or NULL */
     int enable_signals;         /* Simulator/EnableSignals */
+    int enable_detach;          /* Simulator/AttachDetach/EnableDetach */
 } vxo_spec;
 
 typedef struct vxo_model vxo_model;
@@ -142,9 +144,9 @@ int vxo_get_link_curve(const vxo_model *m, int link, float *strain, float *stres
 void vxo_get_voxels(const vxo_model *m, double *pos3, double *quat4, double *linmom3, double *angmom3);
 void vxo_set_voxels(vxo_model *m, const double *pos3, const double *quat4, const double *linmom3, const double *angmom3);
 void vxo_get_flags(const vxo_model *m, int *voxel_flags, int *link_flags);
-/* removal (SecondaryExperiment): per voxel / link the number of the step
- * (1-based, counted over vxo_step calls) at whose end it was removed, 0 if
- * present */
+/* removal (SecondaryExperiment) and detach (EnableDetach): per voxel / link the
+ * number of the step (1-based, counted over vxo_step calls) at whose end it was
+ * removed or detached, 0 if present */
 void vxo_get_removed(const vxo_model *m, int *voxel_step, int *link_step);
 /* signals: each voxel's localSignal */
 void vxo_get_signals(const vxo_model *m, double *local);

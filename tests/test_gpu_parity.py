@@ -596,3 +596,42 @@ def test_signals_fp32():
     b.close()
     log(f"  fp32 signals worst {worst:.3e}")
     assert worst <= TOL[vx3mi.FP32]
+
+
+def _detach_problems():
+    """A robot whose first material fails at a strain of 0.01 under 39 degC
+    actuation (random phases), with EnableDetach and collisions on, signals and
+    cilia as well; and the same robot with detach off (failed links stay, with
+    zero force)."""
+    rng = np.random.default_rng(41)
+    a = configs.random_robot(rng, (5, 4, 3), 0.9, enable_collision=1)
+    nv = int((np.asarray(a["cells"]) > 0).sum())
+    a["materials"] = [dict(configs.SOFT, fail_stress=1e4, cilia=1.0),
+                      dict(configs.STIFF, is_pacemaker=1, pacemaker_period=0.003, signal_delay=0.001,
+                           inactive_period=0.001)]
+    a.update(enable_detach=1, enable_signals=1, enable_cilia=1, cilia_shift=rng.normal(size=(nv, 3)) * 1e-5)
+    return [a, dict(a, enable_detach=0)]
+
+
+@pytest.mark.parametrize("mode", [vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT])
+def test_detach_fp64_bit_exact(mode):
+    """EnableDetach: links past their failure strain leave both voxels (the
+    surface, cilia, signals and collisions follow): FP64 state and detach steps
+    bit for bit with the oracle.  (FP32 is not compared here: failure is a
+    threshold that FP32 crosses at its own step.)"""
+    log(f"detach mode {mode}")
+    probs = _detach_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
+    oracles = [O.OracleModel(p) for p in probs]
+    for _ in range(6):
+        b.step(100, mode=mode)
+        for r, o in enumerate(oracles):
+            o.step(100)
+            s, ref = b.state(r), o.state()
+            for k in ("pos", "quat", "linmom", "angmom"):
+                assert np.array_equal(s[k], ref[k]), (r, k)
+            assert np.array_equal(b.removed(r)[1], o.removed()[1]), r
+            assert np.array_equal(b.signals(r), o.signals()), r
+    lr = b.removed(0)[1]
+    assert (lr > 0).sum() >= 3 and not b.removed(1)[1].any()
+    b.close()

@@ -483,3 +483,40 @@ def test_cilia_force_scales_with_local_signal():
         checked += prev_s > 0
         prev_p, prev_s = px, s
     assert checked > 300 and 0 < prev_s < 90
+
+
+def test_detach_failed_link():
+    """Two voxels pulled apart by opposite external forces through a link that
+    fails at Fail_Stress: with EnableDetach the link leaves both voxels at the end
+    of the first step whose strain passes the failure strain; from then on the
+    voxels are free bodies under their constant forces (exact discrete
+    sequence).  With collisions on, the now-unlinked neighbours overlap the
+    contact envelope (1.25 voxel lengths) and are pushed apart much further."""
+    F, fail, E = 0.5, 1e3, 1e6
+    sep = {}
+    for col in (0, 1):
+        p = quiet((2, 1, 1), enable_detach=1, enable_collision=col,
+                  externals=[dict(voxel=0, force=(-F, 0, 0)), dict(voxel=1, force=(F, 0, 0))])
+        p["materials"] = [dict(E=E, rho=1000.0, fail_stress=fail)]
+        m = O.OracleModel(p)
+        dt = float(np.float32(m.step_dt()))
+        k = 0
+        while True:
+            m.step(1)
+            k += 1
+            _, strain = m.link_state()
+            if strain[0] > np.float32(fail) / np.float32(E):
+                break
+            assert m.removed()[1][0] == 0 and k < 5000
+        assert m.removed()[1].tolist() == [k]
+        s0 = m.state()
+        n = 200
+        m.step(n)
+        s1 = m.state()
+        sep[col] = s1["pos"][1, 0] - s1["pos"][0, 0]
+        if not col:
+            mass_inv = float(np.float32(1.0) / np.float32(1000.0 * 0.01 ** 3))
+            for v, sgn in ((0, -1), (1, 1)):
+                x = s0["pos"][v, 0] + s0["linmom"][v, 0] * mass_inv * dt * n + sgn * F * mass_inv * dt ** 2 * n * (n + 1) / 2
+                assert s1["pos"][v, 0] == pytest.approx(x, rel=1e-12)
+    assert sep[1] > sep[0] + 0.001

@@ -539,3 +539,11 @@ def test_signal_keys_are_read_with_reference_defaults():
             mat["inactive_period"]) == (1, 0.25, 0.7, 0.004, 0.01)
     d = vx3mi.Model.from_xml(DEMO_LIKE).material(1)
     assert (d["is_pacemaker"], d["signal_value_decay"], d["signal_time_delay"], d["inactive_period"]) == (0, 0.9, 0.03, 0.03)
+
+
+def test_detach_key_is_read():
+    """Simulator/AttachDetach/EnableDetach (a bool, VX3_SimulationManager.cu:330)."""
+    m = vx3mi.Model.from_xml(W.write_vxa(dict(configs.config0(), enable_detach=1)))
+    assert m.settings()["enable_detach"] == 1
+    text = DEMO_LIKE.replace("<Simulator>", "<Simulator><AttachDetach><EnableDetach>true</EnableDetach></AttachDetach>")
+    assert vx3mi.Model.from_xml(text).settings()["enable_detach"] == 1

@@ -16,6 +16,7 @@ struct RobotParams {
     double tempInvPeriod;  // 1 / tempPeriod (the FP32 path multiplies)
     int varyTemp, enableExpansion, collisions;
     int secondary;           // SecondaryExperiment: materials are removed after their time
+    int detach;              // EnableDetach: failed links leave both voxels
     int ffOff[3], ffLen[3];  // force-field postfix programs in the formula table (length 0: none)
     int voff, vcount, loff, lcount;
     // collision broadphase: surface voxels [soff, soff+scount) of the surface
@@ -60,6 +61,7 @@ struct BatchBase {
     int S = 0;                   // surface voxels (collision candidates) of the batch
     bool anyCollisions = false;
     bool anyRemoval = false;      // some robot can lose voxels (SecondaryExperiment)
+    bool anyDetach = false;       // some robot can lose links (EnableDetach with failing materials)
     long tableBuckets = 0;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

@@ -64,6 +64,8 @@ template <typename T> struct Dev {
     const int *remCand;       // voxels of materials with RemoveFromSimulationAfterThisManySeconds > 0
     const double *remAt;      // that time, per candidate
     int nRemCand;
+    const int *detCand;       // links of EnableDetach robots that can fail
+    int nDetCand;
     VoxelSignal *sig;         // per-voxel signal state, or nullptr when no robot has EnableSignals
     const SignalMat *sigMat;  // per-voxel signal constants
 };
@@ -405,6 +407,29 @@ template <typename T> __global__ __launch_bounds__(256) void k_removal(Dev<T> a,
     }
 }
 
+// gpu_update_detach (VX3_VoxelyzeKernel.cu:946-968): a link whose maxStrain has
+// passed its failure strain (VX3_Link::isFailed, VX3_Link.cu:78) leaves both
+// voxels' link lists.  The reference detaches after the step's collisions; the
+// surface, cilia and collisions see it from the next step, the step's voxel
+// update sums the failed link's zero forces either way, and the step's signals
+// already go without it: so it runs after the voxel update, before the signal
+// delivery.
+template <typename T> __global__ __launch_bounds__(256) void k_detach(Dev<T> a) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nDetCand) return;
+    const int l = a.detCand[i];
+    const int2 meta = a.lmeta[l];
+    const RobotTime *rt = a.rt + meta.y;
+    if (rt->diverged || rt->dt == 0.0f || a.lrem[l]) return;
+    const float epsFail = a.lmats[meta.x >> 2].epsFail;
+    if (!(epsFail != -1.0f && (float)a.ls2[l].y > epsFail)) return;
+    a.lrem[l] = rt->steps;
+    const int axis = meta.x & 3;
+    const int2 e = a.lends[l];
+    a.vlinks[6 * e.x + 2 * axis] = -1;      // the link is pVNeg's POS-direction link
+    a.vlinks[6 * e.y + 2 * axis + 1] = -1;  // and pVPos's NEG-direction link
+}
+
 // One workgroup per robot; nsteps steps with two barrier-separated phases.
 template <typename T, int NT> __global__ __launch_bounds__(NT) void k_persistent(Dev<T> a, int nsteps, int parity0) {
     const RobotParams rp = a.rp[blockIdx.x];
@@ -512,7 +537,7 @@ template <typename T> struct Batch : BatchBase {
     bool anySignals = false;      // some robot has EnableSignals
     // plain and flat modes only: force fields (their own kernel), removal and
     // signals (kernels between steps)
-    bool plainFlatOnly() const { return anyForceField || anyRemoval || anySignals; }
+    bool plainFlatOnly() const { return anyForceField || anyRemoval || anySignals || anyDetach; }
     // launch configuration used by VX3MI_MODE_AUTO (set by the autotuner)
     int autoMode = 0, tileSel = 0, persistThreads = 256;
     int nLinkMats = 0;  // entries of the device link material table
@@ -538,7 +563,7 @@ template <typename T> struct Batch : BatchBase {
         std::vector<T4<T>> pq0, pq1;
         std::vector<VoxelCilia> cilia;
         std::vector<SignalMat> sigMat;
-        std::vector<int> remCand;
+        std::vector<int> remCand, detCand;
         std::vector<double> remAt;
         std::map<std::string, int> vmIndex, lmIndex, curveIndex;
         // materials are deduplicated across robots (byte-identical constants);
@@ -570,6 +595,7 @@ template <typename T> struct Batch : BatchBase {
             p.enableExpansion = m.enableExpansion;
             p.collisions = m.enableCollision;
             p.secondary = m.secondaryExperiment;
+            p.detach = m.enableDetach;
             p.voff = voff; p.vcount = m.nv(); p.loff = loff; p.lcount = m.nl();
             if (m.enableCilia) anyCilia = true;
             if (m.enableSignals) anySignals = true;
@@ -650,6 +676,7 @@ template <typename T> struct Batch : BatchBase {
             for (int l = 0; l < m.nl(); l++) {
                 lends.push_back(int2{voff + m.lneg[l], voff + m.lpos[l]});
                 lmeta.push_back(int2{m.laxis[l] | (lmRemap[m.lmat[l]] << 2), r});
+                if (m.enableDetach && m.lmats[m.lmat[l]].epsFail != -1.0f) detCand.push_back(loff + l);
             }
             float dflt = (float)(m.dtFrac * std::max(recommended_dt(m), 1e-10));
             defaultDt.push_back(dflt);
@@ -658,6 +685,7 @@ template <typename T> struct Batch : BatchBase {
         L = (int)lends.size();
         nLinkMats = (int)lmats.size();
         anyRemoval = !remCand.empty();
+        anyDetach = !detCand.empty();
         hlat = lat;
         hvmat = vmat;
 
@@ -737,6 +765,9 @@ template <typename T> struct Batch : BatchBase {
             if (m.enableCollision) {
                 std::vector<char> in((size_t)m.nv(), 0);
                 for (int v : m.surface) in[v] = 1;
+                if (m.enableDetach)  // the ends of links that can fail
+                    for (int l = 0; l < m.nl(); l++)
+                        if (m.lmats[m.lmat[l]].epsFail != -1.0f) in[m.lneg[l]] = in[m.lpos[l]] = 1;
                 if (m.secondaryExperiment)
                     for (int v = 0; v < m.nv(); v++) {
                         if (!(m.palette[m.vpal[v] - 1].remove_after > 0)) continue;
@@ -812,17 +843,23 @@ template <typename T> struct Batch : BatchBase {
         d.remCand = nullptr;
         d.remAt = nullptr;
         d.nRemCand = 0;
-        if (anyRemoval) {
-            int *vr, *lr, *rc0;
+        d.detCand = nullptr;
+        d.nDetCand = 0;
+        if (anyRemoval || anyDetach) {
+            int *vr, *lr, *rc0, *dc;
             double *ra;
-            const int nc = (int)remCand.size();
-            if ((rc = alloc(&vr, V)) || (rc = alloc(&lr, L)) || (rc = alloc(&rc0, nc)) || (rc = alloc(&ra, nc))) return rc;
+            const int nc = (int)remCand.size(), nd = (int)detCand.size();
+            if ((rc = alloc(&vr, V)) || (rc = alloc(&lr, L)) || (rc = alloc(&rc0, nc)) || (rc = alloc(&ra, nc)) ||
+                (rc = alloc(&dc, nd)))
+                return rc;
             HIP_TRY(hipMemset(vr, 0, sizeof(int) * V));
             HIP_TRY(hipMemset(lr, 0, sizeof(int) * std::max(L, 1)));
-            HIP_TRY(hipMemcpy(rc0, remCand.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(ra, remAt.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+            if (nc) HIP_TRY(hipMemcpy(rc0, remCand.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
+            if (nc) HIP_TRY(hipMemcpy(ra, remAt.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
+            if (nd) HIP_TRY(hipMemcpy(dc, detCand.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
             HIP_TRY(hipDeviceSynchronize());
             d.vrem = vr; d.lrem = lr; d.remCand = rc0; d.remAt = ra; d.nRemCand = nc;
+            d.detCand = dc; d.nDetCand = nd;
         }
         d.sig = nullptr;
         d.sigMat = nullptr;
@@ -846,7 +883,7 @@ template <typename T> struct Batch : BatchBase {
         using C = TileCfg<T, SMALL>;
         TilePlan &P = plans[SMALL];
         P.ok = false;
-        if (anyCollisions || anyForceField || anyCilia || anyRemoval || anySignals || R < 2) return VX3MI_OK;
+        if (anyCollisions || anyForceField || anyCilia || plainFlatOnly() || R < 2) return VX3MI_OK;
         long totL = 0, totV = 0;
         for (const RobotParams &p : hrp) {
             if (p.vcount > C::VMAX || p.lcount > C::LMAX) return VX3MI_OK;
@@ -943,11 +980,13 @@ template <typename T> struct Batch : BatchBase {
         if (anyCollisions) k_col_contact<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
         if (anyForceField) k_force_field<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         k_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
+        if (anyDetach) k_detach<T><<<(d.nDetCand + 255) / 256, 256, 0, stream>>>(d);
         if (anySignals) k_signal_deliver<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V);
         if (anyRemoval) k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
     }
     int kernels_per_flat_step() const {
-        return 2 + (anyCollisions ? 2 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0);
+        return 2 + (anyCollisions ? 2 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0) +
+               (anyDetach ? 1 : 0);
     }
 
     int step(int nsteps, float dt, int mode) override {
@@ -961,7 +1000,7 @@ template <typename T> struct Batch : BatchBase {
         }
         if (mode == VX3MI_MODE_AUTO) mode = autoMode;
         if (mode == VX3MI_MODE_PERSISTENT && plainFlatOnly()) {
-            set_error("force fields, material removal and signals run in the plain and flat modes only");
+            set_error("force fields, material removal, detach and signals run in the plain and flat modes only");
             return VX3MI_E_ARG;
         }
         HIP_TRY(hipEventRecord(ev0, stream));
@@ -976,8 +1015,17 @@ template <typename T> struct Batch : BatchBase {
                     if (p.scount) k_col_contact<T><<<(p.scount + 255) / 256, 256, 0, stream>>>(d, p.soff, p.scount, parity);
                     if (anyForceField) k_force_field<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
                     k_voxels<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
-                    if (anySignals) k_signal_deliver<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount);
-                    launches += 2 + (p.scount ? 2 : 0) + (anyForceField ? 1 : 0) + (anySignals ? 1 : 0);
+                    launches += 2 + (p.scount ? 2 : 0) + (anyForceField ? 1 : 0);
+                }
+                // the end-of-step phases for the whole batch (each touches only its
+                // robot's voxels and links)
+                if (anyDetach) {
+                    k_detach<T><<<(d.nDetCand + 255) / 256, 256, 0, stream>>>(d);
+                    launches++;
+                }
+                if (anySignals) {
+                    k_signal_deliver<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V);
+                    launches++;
                 }
                 if (anyRemoval) {
                     k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
@@ -1109,7 +1157,7 @@ template <typename T> struct Batch : BatchBase {
             bufs.push_back({d.contactW, sizeof(T4<T>) * V});
         }
         if (anySignals) bufs.push_back({d.sig, sizeof(VoxelSignal) * (size_t)V});
-        if (anyRemoval) {
+        if (anyRemoval || anyDetach) {
             bufs.push_back({d.vlinks, sizeof(int) * 6 * (size_t)V});
             bufs.push_back({d.vrem, sizeof(int) * (size_t)V});
             bufs.push_back({d.lrem, sizeof(int) * (size_t)L});
@@ -1183,7 +1231,7 @@ template <typename T> struct Batch : BatchBase {
         }
         if (mode < VX3MI_MODE_PLAIN || mode > VX3MI_MODE_TILED) { set_error("invalid mode"); return VX3MI_E_ARG; }
         if (mode == VX3MI_MODE_PERSISTENT && plainFlatOnly()) {
-            set_error("force fields, material removal and signals run in the plain and flat modes only");
+            set_error("force fields, material removal, detach and signals run in the plain and flat modes only");
             return VX3MI_E_ARG;
         }
         autoMode = mode;
@@ -1248,7 +1296,7 @@ template <typename T> struct Batch : BatchBase {
             l0 = hrp[robot].loff; l1 = l0 + hrp[robot].lcount;
         }
         HIP_TRY(hipStreamSynchronize(stream));
-        if (!anyRemoval) {
+        if (!anyRemoval && !anyDetach) {
             if (vstep) std::fill(vstep, vstep + (v1 - v0), 0);
             if (lstep) std::fill(lstep, lstep + (l1 - l0), 0);
             return VX3MI_OK;

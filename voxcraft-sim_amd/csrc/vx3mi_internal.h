@@ -65,6 +65,7 @@ struct Model {
     double reinitInitialPositionAfter = 0.0;
     int enableCilia = 0;             // Simulator/EnableCilia
     int enableSignals = 0;           // Simulator/EnableSignals
+    int enableDetach = 0;            // Simulator/AttachDetach/EnableDetach
     std::vector<double> ciliaBase, ciliaShift;  // 3 per voxel (empty: zeros)
     int nv() const { return (int)ix.size(); }
     int ext_of(int v) const { return vext.empty() ? -1 : vext[v]; }

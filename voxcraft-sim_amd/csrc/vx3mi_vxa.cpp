@@ -510,7 +510,7 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
         const XNode *ad = sim->child("AttachDetach");
         L.enable_collision = load_bool(ad, "EnableCollision", true);
         if (load_bool(ad, "EnableAttach", false)) { err = "EnableAttach is not supported"; return VX3MI_E_PARSE; }
-        if (load_bool(ad, "EnableDetach", false)) { err = "EnableDetach is not supported"; return VX3MI_E_PARSE; }
+        L.enable_detach = load_bool(ad, "EnableDetach", false);
         int v;
         if (load(sim, "EnableExpansion", v)) L.enable_expansion = v;
         if (load(sim, "EnableCilia", v)) L.enable_cilia = v != 0;

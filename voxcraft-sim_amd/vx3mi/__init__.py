@@ -31,7 +31,8 @@ PROBLEM_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp
                         volume_effects=0, enable_collision=1, enable_expansion=0, grav_acc=-9.81,
                         floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
                         temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None,
-                        enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None, enable_signals=0)
+                        enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None, enable_signals=0,
+                        enable_detach=0)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, is_target=0, thermal_on_after=0.0,
@@ -75,7 +76,8 @@ class _Settings(C.Structure):
                 ("temp_amplitude", C.c_double), ("temp_period", C.c_double), ("initial_temp", C.c_float),
                 ("record_step_size", C.c_int), ("record_voxel", C.c_int), ("record_link", C.c_int),
                 ("has_stop_formula", C.c_int), ("has_fitness_formula", C.c_int),
-                ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int), ("enable_signals", C.c_int)]
+                ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int), ("enable_signals", C.c_int),
+                ("enable_detach", C.c_int)]
 
 
 FORMULA_STOP, FORMULA_FITNESS = 0, 1
@@ -109,7 +111,7 @@ class _Lattice(C.Structure):
                 ("vary_temp", C.c_int), ("temp_base", C.c_double), ("temp_amplitude", C.c_double),
                 ("temp_period", C.c_double), ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int),
                 ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double)),
-                ("enable_signals", C.c_int)]
+                ("enable_signals", C.c_int), ("enable_detach", C.c_int)]
 
 
 def library_path() -> str:
@@ -247,7 +249,8 @@ class Model:
                         int(p["enable_expansion"]), p["grav_acc"], int(p["floor_enabled"]),
                         int(p["temp_enabled"]), int(p["vary_temp"]), p["temp_base"], p["temp_amplitude"],
                         p["temp_period"], int(p["enable_cilia"]), int(p["secondary_experiment"]),
-                        _ptr(cb, C.c_double), _ptr(cs, C.c_double), int(p["enable_signals"]))
+                        _ptr(cb, C.c_double), _ptr(cs, C.c_double), int(p["enable_signals"]),
+                        int(p["enable_detach"]))
         h = C.c_void_p()
         _check(lib().vx3mi_model_from_lattice(C.byref(spec), C.byref(h)))
         m = cls(h.value)

@@ -48,7 +48,8 @@ def _simulator(p: dict, stop=None, fitness=None, record=None) -> str:
         _el("Damping", _el("BondDampingZ", _num(p["bond_damp_z"])) + _el("ColDampingZ", _num(p["col_damp_z"]))
             + _el("SlowDampingZ", _num(p["slow_damp_z"]))),
         _el("Features", _el("VolumeEffectsEnabled", str(int(p["volume_effects"])))),
-        _el("AttachDetach", _el("EnableCollision", str(int(p["enable_collision"])))),
+        _el("AttachDetach", _el("EnableCollision", str(int(p["enable_collision"])))
+            + (_el("EnableDetach", "1") if p.get("enable_detach") else "")),
         _el("EnableExpansion", str(int(p["enable_expansion"]))),
     ]
     if p.get("enable_cilia"):
