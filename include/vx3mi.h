@@ -58,7 +58,7 @@ typedef struct {
     int mat_model;       /* MatModel: 0 linear, 1 linear with failure, 2 bilinear, 3 stress-strain data */
     int fixed;           /* Fixed */
     int is_measured;     /* isMeasured */
-    int is_target;       /* isTarget */
+    int is_target;       /* isTarget: contacts with non-targets count as hits (collisionCount) */
     double thermal_on_after; /* TurnOnThermalExpansionAfterThisManySeconds */
     float red, green, blue, alpha; /* Display */
     double cilia;            /* Cilia: multiplier of the voxel's cilia force (with enable_cilia) */
@@ -98,6 +98,7 @@ typedef struct {
     const double *cilia_shift;     /* per-voxel ShiftCiliaForce (scaled by the voxel's signal) or NULL */
     int enable_signals;            /* Simulator/EnableSignals */
     int enable_detach;             /* Simulator/AttachDetach/EnableDetach: failed links leave their voxels */
+    double max_dist_pairs;         /* Simulator/MaxDistInVoxelLengthsToCountAsPair (0: no close-pair counting) */
 } vx3mi_lattice;
 
 /* Settings a model carries besides topology and materials, read back by
@@ -113,6 +114,7 @@ typedef struct {
     int record_step_size, record_voxel, record_link; /* Simulator/RecordHistory */
     int has_stop_formula, has_fitness_formula;
     int enable_cilia, secondary_experiment, enable_signals, enable_detach;
+    double max_dist_pairs;         /* MaxDistInVoxelLengthsToCountAsPair */
 } vx3mi_settings;
 
 typedef struct vx3mi_model vx3mi_model;
@@ -232,6 +234,10 @@ int vx3mi_batch_removed(vx3mi_batch *b, int robot, int *voxel_step, int *link_st
  * step in link direction order (X_POS..Z_NEG).  Batches with signals run in the
  * plain and flat modes. */
 int vx3mi_batch_signals(vx3mi_batch *b, int robot, double *local);
+/* collisionCount of every robot: contacts between a target (isTarget) and a
+ * non-target voxel summed over the steps (VX3_VoxelyzeKernel.cu:711-726), the
+ * MathTree variable "hit".  out: n ints. */
+int vx3mi_batch_hits(vx3mi_batch *b, int *out);
 /* per-link flag words (bit 0 local velocity valid, bit 1 small-angle
  * formulation, VX3_Link.cu:117-131, 172-191) and axial strain of the last step,
  * link order; either pointer may be NULL */

@@ -30,7 +30,7 @@ class _Material(C.Structure):
                 ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double),
                 ("ss_points", C.c_int), ("ss_strain", C.c_float * 32), ("ss_stress", C.c_float * 32),
                 ("is_pacemaker", C.c_int), ("pacemaker_period", C.c_double), ("signal_decay", C.c_double),
-                ("signal_delay", C.c_double), ("inactive_period", C.c_double)]
+                ("signal_delay", C.c_double), ("inactive_period", C.c_double), ("is_target", C.c_int)]
 
 
 # MathTree token codes (vx3_oracle.h VXO_MT_*) and variable order
@@ -84,7 +84,7 @@ MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0,
                          fixed=0, is_measured=1, thermal_on_after=0.0,
                          cilia=0.0, cilia_on_after=0.0, remove_after=0.0, ss_data=None,
                          is_pacemaker=0, pacemaker_period=0.0, signal_decay=0.9, signal_delay=0.03,
-                         inactive_period=0.03)
+                         inactive_period=0.03, is_target=0)
 
 
 def build_library(force: bool = False) -> str:
@@ -123,6 +123,8 @@ def lib():
         L.vxo_get_flags.argtypes = [vp, ip, ip]
         L.vxo_get_removed.argtypes = [vp, ip, ip]
         L.vxo_get_signals.argtypes = [vp, dp]
+        L.vxo_collision_count.restype = C.c_long
+        L.vxo_collision_count.argtypes = [vp]
         L.vxo_get_link_state.argtypes = [vp, dp, fp]
         L.vxo_get_temperature.argtypes = [vp, fp, dp]
         L.vxo_set_temperature.argtypes = [vp, fp]
@@ -174,7 +176,8 @@ class OracleModel:
                                       model, int(m["fixed"]), int(m["is_measured"]), m["thermal_on_after"],
                                       m["cilia"], m["cilia_on_after"], m["remove_after"], npts,
                                       (C.c_float * 32)(*ss[0]), (C.c_float * 32)(*ss[1]), int(m["is_pacemaker"]),
-                                      m["pacemaker_period"], m["signal_decay"], m["signal_delay"], m["inactive_period"])
+                                      m["pacemaker_period"], m["signal_decay"], m["signal_delay"], m["inactive_period"],
+                                      int(m["is_target"]))
         self._phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         self._E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
         self._cb, self._cs = [None if p[k] is None else np.ascontiguousarray(p[k], dtype=np.float64).reshape(-1)
@@ -284,6 +287,9 @@ class OracleModel:
         vr = np.zeros(self.nv, dtype=np.int32); lr = np.zeros(max(1, self.nl), dtype=np.int32)
         self._L.vxo_get_removed(self._h, _ptr(vr, C.c_int), _ptr(lr, C.c_int))
         return vr, lr[:self.nl]
+
+    def collision_count(self) -> int:
+        return int(self._L.vxo_collision_count(self._h))
 
     def signals(self) -> np.ndarray:
         """Each voxel's localSignal."""

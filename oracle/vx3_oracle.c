@@ -51,7 +51,7 @@ typedef struct {
     int fixed, isMeasured;
     double thermalOnAfter;
     double cilia, ciliaOnAfter, removeAfter;
-    int isPaceMaker;
+    int isPaceMaker, isTarget;
     double paceMakerPeriod, signalDecay, signalDelay, inactivePeriod;
     /* stress-strain model: CVX_Material linear / strainData / stressData */
     int linear, npts;
@@ -100,6 +100,7 @@ struct vxo_model {
      * inactiveUntil, packmakerNextPulse; the signal a voxel sent this step */
     double *sLocal, *sLocalDt, *sValue, *sActive, *sInactive, *sNextPulse;
     double *outValue, *outTime;
+    long collisionCount;
     /* links */
     int *lneg, *lpos, *laxis;
     lmat *lm;
@@ -569,6 +570,7 @@ vxo_model *vxo_build(const vxo_spec *sp) {
         vm->signalDecay = in->signal_decay;
         vm->signalDelay = in->signal_delay;
         vm->inactivePeriod = in->inactive_period;
+        vm->isTarget = in->is_target;
         vmat_derive(vm);
     }
 
@@ -951,6 +953,8 @@ static inline long long cell_key(long long cx, long long cy, long long cz) {
 /* handle_collision_attachment (collision part) VX3_VoxelyzeKernel.cu:682-729
  * with VX3_Collision VX3_Collision.cu:3-31, for voxel1 = surface[first],
  * voxel2 = surface[second], first > second (gpu_update_attach :833-843). */
+static void signal_receive(vxo_model *m, int v, const vmat *mt, double value, double activeTime, int force);
+
 static void collide_pair(vxo_model *m, int v1, int v2) {
     const vmat *m1 = &m->vm[m->vmat_of[v1]], *m2 = &m->vm[m->vmat_of[v2]];
     if (m1->fixed && m2->fixed) return;
@@ -985,6 +989,15 @@ static void collide_pair(vxo_model *m, int v1, int v2) {
     }
     m->contact[v1] = vadd(m->contact[v1], force);
     m->contact[v2] = vsub(m->contact[v2], force);
+    /* a target touching a non-target: collisionCount, and with signals the
+     * non-target voxel receives a forced 100 (:711-726) */
+    if (m1->isTarget != m2->isTarget) {
+        m->collisionCount++;
+        if (m->enableSignals) {
+            int o = m1->isTarget ? v2 : v1;
+            signal_receive(m, o, &m->vm[m->vmat_of[o]], 100, m->time, 1);
+        }
+    }
 }
 
 /* updateAttach all-pairs branch VX3_VoxelyzeKernel.cu:448-458.  The uniform
@@ -1147,9 +1160,9 @@ static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
         tf = vadd(tf, vscale(cf, mt->cilia));
     }
     /* force field (VX3_Voxel.cu:177-183) at the position entering the step and
-     * currentTime; hit, angle, closeness and pairs are 0 (no targets); absent
+     * currentTime; hit is collisionCount; angle, closeness and pairs are 0; absent
      * trees add +0.0 as the reference's constant-zero defaults do */
-    const double vars[9] = {pos.x, pos.y, pos.z, 0.0, m->time, 0.0, 0.0, 0.0, (double)m->nv};
+    const double vars[9] = {pos.x, pos.y, pos.z, (double)m->collisionCount, m->time, 0.0, 0.0, 0.0, (double)m->nv};
     tf.x += m->ff_n[0] ? mt_eval_fifo(m->ff_op[0], m->ff_val[0], m->ff_n[0], vars) : 0.0;
     tf.y += m->ff_n[1] ? mt_eval_fifo(m->ff_op[1], m->ff_val[1], m->ff_n[1], vars) : 0.0;
     tf.z += m->ff_n[2] ? mt_eval_fifo(m->ff_op[2], m->ff_val[2], m->ff_n[2], vars) : 0.0;
@@ -1371,6 +1384,8 @@ void vxo_get_removed(const vxo_model *m, int *voxel_step, int *link_step) {
     if (voxel_step) memcpy(voxel_step, m->vremoved, sizeof(int) * m->nv);
     if (link_step) memcpy(link_step, m->lremoved, sizeof(int) * m->nl);
 }
+
+long vxo_collision_count(const vxo_model *m) { return m->collisionCount; }
 
 void vxo_get_signals(const vxo_model *m, double *local) {
     memcpy(local, m->sLocal, sizeof(double) * m->nv);

@@ -65,6 +65,7 @@ typedef struct {
     double signal_decay;     /* signalValueDecay */
     double signal_delay;     /* signalTimeDelay (s) */
     double inactive_period;  /* inactivePeriod (s) */
+    int is_target;           /* isTarget */
 } vxo_material;
 
 /* Lattice + simulator + environment description. */
@@ -150,6 +151,9 @@ void vxo_get_flags(const vxo_model *m, int *voxel_flags, int *link_flags);
 void vxo_get_removed(const vxo_model *m, int *voxel_step, int *link_step);
 /* signals: each voxel's localSignal */
 void vxo_get_signals(const vxo_model *m, double *local);
+/* collisionCount: contacts between a target and a non-target voxel, summed over
+ * the steps (VX3_VoxelyzeKernel.cu:711-726) */
+long vxo_collision_count(const vxo_model *m);
 void vxo_get_link_state(const vxo_model *m, double *state9, float *strain);
 void vxo_get_temperature(const vxo_model *m, float *temp, double *rest_length);
 void vxo_set_temperature(vxo_model *m, const float *temp);

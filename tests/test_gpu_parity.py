@@ -635,3 +635,37 @@ def test_detach_fp64_bit_exact(mode):
     lr = b.removed(0)[1]
     assert (lr > 0).sum() >= 3 and not b.removed(1)[1].any()
     b.close()
+
+
+def _target_problems():
+    """A non-target voxel pushed into a linked pair of target voxels (collisions
+    and signals on), and a random robot whose stiff material is a target."""
+    a = dict(configs.BASE, dims=(4, 1, 1), cells=np.array([1, 0, 2, 2], dtype=np.int8), phase=None,
+             enable_collision=1, enable_signals=1, vary_temp=0, floor_enabled=0, grav_acc=0.0,
+             materials=[dict(configs.SOFT), dict(configs.SOFT, is_target=1)],
+             externals=[dict(voxel=0, force=(0.2, 0, 0)), dict(voxel=1, force=(-0.2, 0, 0))], slow_damp_z=0.0)
+    rng = np.random.default_rng(51)
+    b = configs.random_robot(rng, (6, 6, 4), 0.7, enable_collision=1)
+    b["materials"] = [b["materials"][0], dict(b["materials"][1], is_target=1)]
+    return [a, b]
+
+
+@pytest.mark.parametrize("mode", [vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT])
+def test_targets_fp64_bit_exact(mode):
+    """Target contacts: collisionCount (each target / non-target pair in the
+    contact envelope, once per step) and the forced signal of the non-target
+    voxel, FP64 bit for bit with the oracle."""
+    probs = _target_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
+    oracles = [O.OracleModel(p) for p in probs]
+    for _ in range(4):
+        b.step(100, mode=mode)
+        for r, o in enumerate(oracles):
+            o.step(100)
+            s, ref = b.state(r), o.state()
+            for k in ("pos", "quat", "linmom", "angmom"):
+                assert np.array_equal(s[k], ref[k]), (r, k)
+            assert np.array_equal(b.signals(r), o.signals()), r
+        assert b.hits().tolist() == [o.collision_count() for o in oracles]
+    assert b.hits()[0] > 0
+    b.close()

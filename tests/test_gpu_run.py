@@ -186,3 +186,41 @@ def test_history_frames_follow_removal():
         seen.add(want)
     assert seen == {(26, 54), (17, 33), (18, 33)}
     b.close()
+
+
+def test_runner_hits_and_target_closeness():
+    """Stop formula on hit (collisionCount): the run stops at the first
+    iteration whose count exceeds 20; fitness targetCloseness and the report's
+    numClosePairs come from the last CycleStep sample (sum of 1 / distance over
+    target pairs, pairs closer than MaxDist voxel lengths), as the oracle's
+    positions at that step give them."""
+    prob = dict(configs.BASE, dims=(4, 1, 1), cells=np.array([1, 0, 2, 2], dtype=np.int8), phase=None,
+                enable_collision=1, vary_temp=0, temp_period=0.002, floor_enabled=0, grav_acc=0.0,
+                max_dist_pairs=1.5, materials=[dict(configs.SOFT), dict(configs.SOFT, is_target=1)],
+                externals=[dict(voxel=0, force=(0.2, 0, 0)), dict(voxel=1, force=(-0.2, 0, 0))], slow_damp_z=0.0)
+    m = vx3mi.Model.from_problem(prob)
+    text = W.write_vxa(prob, stop=("mtSUB", ("mtVAR", "hit"), ("mtCONST", 20.0)), fitness=("mtVAR", "targetCloseness"))
+    mx = vx3mi.Model.from_xml(text)
+    for mm in (m, mx):
+        for e in prob["externals"]:
+            mm.set_external(**e)
+    b = vx3mi.Batch([mx], vx3mi.FP64)
+    log("runner: hits and target closeness")
+    res, _, _ = vx3mi.run(b, [mx])
+    r = res[0]
+    o = O.OracleModel(prob)
+    dt = float(np.float32(mx.settings()["dt_frac"] * mx.recommended_dt()))
+    cycle = int(0.002 / dt)
+    n, last = 0, None
+    while o.collision_count() <= 20:
+        o.step(1)
+        n += 1
+        if n % cycle == 0:
+            p = o.state()["pos"]
+            d = np.linalg.norm(p[1] - p[2])
+            last = (1 / d, int(d < 1.5 * 0.01))
+        assert n < 5000
+    assert r["steps"] == n and not r["diverged"]
+    assert last is not None
+    assert r["fitness_score"] == last[0] and r["num_close_pairs"] == last[1]
+    b.close()

@@ -520,3 +520,29 @@ def test_detach_failed_link():
                 x = s0["pos"][v, 0] + s0["linmom"][v, 0] * mass_inv * dt * n + sgn * F * mass_inv * dt ** 2 * n * (n + 1) / 2
                 assert s1["pos"][v, 0] == pytest.approx(x, rel=1e-12)
     assert sep[1] > sep[0] + 0.001
+
+
+def test_target_contacts_count_and_signal():
+    """A non-target and a target voxel (not linked) pushed together: every step
+    that starts with the pair inside the contact envelope (1.25 voxel lengths
+    for unheated voxels) adds one to collisionCount; two non-targets add
+    nothing.  With signals on, the non-target voxel's first contact step leaves
+    its local signal at 0.9 x 100 (received forced, then decayed by its own
+    section)."""
+    for target in (1, 0):
+        p = quiet((3, 1, 1), cells=[1, 0, 2], enable_collision=1, enable_signals=1,
+                  externals=[dict(voxel=0, force=(0.05, 0, 0)), dict(voxel=1, force=(-0.05, 0, 0))])
+        p["materials"] = [dict(E=1e6, rho=1000.0), dict(E=1e6, rho=1000.0, is_target=target)]
+        m = O.OracleModel(p)
+        expect, first = 0, None
+        for k in range(400):
+            pos = m.state()["pos"]
+            inside = np.linalg.norm(pos[1] - pos[0]) <= 1.25 * 0.01
+            m.step(1)
+            expect += int(inside and target)
+            if inside and first is None:
+                first = k
+                if target:
+                    assert m.signals()[0] == pytest.approx(90.0, rel=1e-15) and m.signals()[1] == 0
+        assert first is not None and m.collision_count() == expect
+        assert (expect > 5) == bool(target)

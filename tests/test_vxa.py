@@ -242,7 +242,8 @@ BAD = {
     "missing_operand": DEMO_LIKE.replace("<mtCONST>1</mtCONST>", ""),
     "force_field_angle": DEMO_LIKE.replace("<Simulator>", "<Simulator><ForceField><x_forcefield><mtVAR>angle</mtVAR>"
                                                           "</x_forcefield></ForceField>"),
-    "target": DEMO_LIKE.replace("<CTE>0</CTE>", "<CTE>0</CTE><isTarget>1</isTarget>"),
+    "force_field_closeness": DEMO_LIKE.replace("<Simulator>", "<Simulator><ForceField><x_forcefield><mtVAR>"
+                                                              "targetCloseness</mtVAR></x_forcefield></ForceField>"),
 }
 
 
@@ -547,3 +548,10 @@ def test_detach_key_is_read():
     assert m.settings()["enable_detach"] == 1
     text = DEMO_LIKE.replace("<Simulator>", "<Simulator><AttachDetach><EnableDetach>true</EnableDetach></AttachDetach>")
     assert vx3mi.Model.from_xml(text).settings()["enable_detach"] == 1
+
+
+def test_target_and_pair_keys_are_read():
+    """isTarget and Simulator/MaxDistInVoxelLengthsToCountAsPair."""
+    p = dict(configs.config0(), max_dist_pairs=1.5, materials=[dict(configs.SOFT, is_target=1)])
+    m = vx3mi.Model.from_xml(W.write_vxa(p))
+    assert m.material(1)["is_target"] == 1 and m.settings()["max_dist_pairs"] == 1.5

@@ -40,6 +40,8 @@ struct RobotTime {
     int diverged;
     int steps;
     int overflow;  // bit 0: a collision grid bucket overflowed; bit 1: a voxel's contact list did
+    int collisionCount;  // target / non-target contacts summed over the steps (VX3_VoxelyzeKernel.cu:711-726)
+    int pad;
 };
 
 #define HIP_TRY(expr)                                                                          \

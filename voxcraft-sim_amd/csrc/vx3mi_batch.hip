@@ -152,6 +152,15 @@ int vx3mi_batch_diverged(vx3mi_batch *b, int *out) {
     return any ? VX3MI_DIVERGED : VX3MI_OK;
 }
 
+int vx3mi_batch_hits(vx3mi_batch *b, int *out) {
+    if (!b || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
+    std::vector<RobotTime> rt;
+    int rc = read_rt(b, rt);
+    if (rc) return rc;
+    for (int r = 0; r < b->b->R; r++) out[r] = rt[r].collisionCount;
+    return VX3MI_OK;
+}
+
 int vx3mi_batch_collision_overflow(vx3mi_batch *b, int *out) {
     if (!b || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
     std::vector<RobotTime> rt;

@@ -68,6 +68,7 @@ template <typename T> struct Dev {
     int nDetCand;
     VoxelSignal *sig;         // per-voxel signal state, or nullptr when no robot has EnableSignals
     const SignalMat *sigMat;  // per-voxel signal constants
+    const int *vtarget;       // per-voxel isTarget, or nullptr when no voxel is a target
 };
 
 // ------------------------------------------------------------------ device
@@ -227,7 +228,7 @@ template <typename T> __global__ __launch_bounds__(256) void k_force_field(Dev<T
     if (!rt->diverged && (rp.ffLen[0] | rp.ffLen[1] | rp.ffLen[2])) {
         const T4<T> q0 = a.pq0[v];
         const V3<double> p = position_d<T>(a.lat[v], a.vmats[a.vmat[v]].nomSize, V3<T>{q0.x, q0.y, q0.z});
-        f = force_field<T>(a.formulas, rp.ffOff, rp.ffLen, p.x, p.y, p.z, rt->time[parity], rp.vcount);
+        f = force_field<T>(a.formulas, rp.ffOff, rp.ffLen, p.x, p.y, p.z, rt->time[parity], rp.vcount, rt->collisionCount);
     }
     a.field[v] = T4<T>{f.x, f.y, f.z, (T)0};
 }
@@ -336,6 +337,22 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
         int j = i - 1;
         while (j >= 0 && found[j] > x) { found[j + 1] = found[j]; j--; }
         found[j + 1] = x;
+    }
+    if (a.vtarget) {
+        // a target touching a non-target: collisionCount (each pair once), and
+        // with signals the non-target voxel receives a forced 100 (:711-726)
+        const int ti = a.vtarget[v];
+        bool hitTarget = false;
+        for (int k = 0; k < nf; k++) {
+            if (a.vtarget[a.surf[found[k]]] == ti) continue;
+            if (s < found[k]) atomicAdd(&rt->collisionCount, 1);
+            hitTarget = true;
+        }
+        if (hitTarget && !ti && a.sig) {
+            VoxelSignal sg = a.sig[v];
+            signal_receive(sg, a.sigMat[v], 100, rt->time[parity], true);
+            a.sig[v] = sg;
+        }
     }
     const V3<T> velI = V3<T>{a.mo0[v].x, a.mo0[v].y, a.mo0[v].z} * (T)mi.massInv;
     V3<T> f = {0, 0, 0};
@@ -563,6 +580,8 @@ template <typename T> struct Batch : BatchBase {
         std::vector<T4<T>> pq0, pq1;
         std::vector<VoxelCilia> cilia;
         std::vector<SignalMat> sigMat;
+        std::vector<int> vtarget;
+        bool anyTarget = false;
         std::vector<int> remCand, detCand;
         std::vector<double> remAt;
         std::map<std::string, int> vmIndex, lmIndex, curveIndex;
@@ -668,6 +687,8 @@ template <typename T> struct Batch : BatchBase {
                 sm.inactive = pm.inactive_period;
                 sm.pacemaker = m.enableSignals && pm.is_pacemaker;
                 sigMat.push_back(sm);
+                vtarget.push_back(pm.is_target ? 1 : 0);
+                anyTarget |= pm.is_target != 0 && m.enableCollision;
                 if (m.secondaryExperiment && pm.remove_after > 0) {
                     remCand.push_back(voff + v);
                     remAt.push_back(pm.remove_after);
@@ -860,6 +881,13 @@ template <typename T> struct Batch : BatchBase {
             HIP_TRY(hipDeviceSynchronize());
             d.vrem = vr; d.lrem = lr; d.remCand = rc0; d.remAt = ra; d.nRemCand = nc;
             d.detCand = dc; d.nDetCand = nd;
+        }
+        d.vtarget = nullptr;
+        if (anyTarget) {
+            int *td;
+            if ((rc = alloc(&td, V))) return rc;
+            HIP_TRY(hipMemcpy(td, vtarget.data(), sizeof(int) * V, hipMemcpyHostToDevice));
+            d.vtarget = td;
         }
         d.sig = nullptr;
         d.sigMat = nullptr;

@@ -66,6 +66,7 @@ struct Model {
     int enableCilia = 0;             // Simulator/EnableCilia
     int enableSignals = 0;           // Simulator/EnableSignals
     int enableDetach = 0;            // Simulator/AttachDetach/EnableDetach
+    double maxDistPairs = 0.0;       // Simulator/MaxDistInVoxelLengthsToCountAsPair
     std::vector<double> ciliaBase, ciliaShift;  // 3 per voxel (empty: zeros)
     int nv() const { return (int)ix.size(); }
     int ext_of(int v) const { return vext.empty() ? -1 : vext[v]; }

@@ -278,6 +278,7 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
     m.enableCilia = sp->enable_cilia ? 1 : 0;
     m.enableSignals = sp->enable_signals ? 1 : 0;
     m.enableDetach = sp->enable_detach ? 1 : 0;
+    m.maxDistPairs = sp->max_dist_pairs;
     m.secondaryExperiment = sp->secondary_experiment ? 1 : 0;
 
     // UpdateMatTemps (VX_Sim.cpp:467-483) via UpdateCurTemp (VX_Environment.cpp:327-346)
@@ -514,6 +515,7 @@ int vx3mi_model_settings(const vx3mi_model *m, vx3mi_settings *o) {
     o->secondary_experiment = M.secondaryExperiment;
     o->enable_signals = M.enableSignals;
     o->enable_detach = M.enableDetach;
+    o->max_dist_pairs = M.maxDistPairs;
     return VX3MI_OK;
 }
 

@@ -720,12 +720,13 @@ template <typename T> VX_HD T fixed_coordinate(int c, double nomSize, double t) 
 
 // The force field of a voxel (VX3_Voxel.cu:177-183): formulas f (postfix,
 // off / len per axis, len 0 = none) at the position entering the step and the
-// time before it; hit, angle, targetCloseness and numClosePairs are 0 (no
-// targets or pair counting; formulas using the angle are rejected when read).
+// time before it; hit is the robot's collisionCount; angle, targetCloseness
+// and numClosePairs are sampled inside doTimeStep and formulas using them are
+// rejected when read.
 template <typename T>
 VX_HD V3<T> force_field(const MtToken *f, const int *off, const int *len, double x, double y, double z, double t,
-                        int numVoxels) {
-    const double vars[MV_COUNT] = {x, y, z, 0.0, t, 0.0, 0.0, 0.0, (double)numVoxels};
+                        int numVoxels, int hit) {
+    const double vars[MV_COUNT] = {x, y, z, (double)hit, t, 0.0, 0.0, 0.0, (double)numVoxels};
     V3<T> r = {0, 0, 0};
     if (len[0]) r.x = (T)formula_eval(f + off[0], len[0], vars);
     if (len[1]) r.y = (T)formula_eval(f + off[1], len[1], vars);

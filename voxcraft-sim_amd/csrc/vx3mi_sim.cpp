@@ -15,9 +15,10 @@
 // the next frame, the next centre-of-mass sample a formula needs, or the
 // first step at which the stop formula turns positive.  A robot that has
 // stopped is no longer observed (the device keeps integrating it; its results
-// were taken when it stopped).  collisionCount, targetCloseness and
-// numClosePairs stay 0: targets and close-pair counting are rejected when a
-// VXA is read.
+// were taken when it stopped).  collisionCount (hit) comes from the device;
+// targetCloseness and numClosePairs are computed at the CycleStep samples
+// (computeTargetCloseness, VX3_VoxelyzeKernel.cu:545-562) from the target
+// voxels' positions when MaxDistInVoxelLengthsToCountAsPair is set.
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -45,12 +46,17 @@ struct RobotRun {
     double com[3] = {0, 0, 0}, comInit[3] = {0, 0, 0}, hist[2][3] = {{0, 0, 0}, {0, 0, 0}};
     double recentAngle = 0;
     int angleSamples = 0;
+    int hit = 0;               // collisionCount
+    double closeness = 0;      // targetCloseness
+    int pairs = 0;             // numClosePairs
+    std::vector<int> targets;  // target voxels (registerTargets, VX3_VoxelyzeKernel.cu:536-543)
     std::vector<double> initPos;
 };
 
 bool stop_met(const RobotRun &r) {
     if (r.m->stopFormula.empty()) return false;
-    const double vars[MV_COUNT] = {r.com[0], r.com[1], r.com[2], 0, r.t, r.recentAngle, 0, 0, (double)r.m->nv()};
+    const double vars[MV_COUNT] = {r.com[0], r.com[1], r.com[2], (double)r.hit, r.t, r.recentAngle, r.closeness,
+                                   (double)r.pairs, (double)r.m->nv()};
     double v = 0;
     return mt_eval(r.m->stopFormula, vars, &v) && v > 0;
 }
@@ -72,6 +78,23 @@ void cycle_sample(RobotRun &r, const double *com) {
         const double lw = std::sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
         r.recentAngle = std::acos(d / (lu * lw));
     }
+}
+
+// computeTargetCloseness (VX3_VoxelyzeKernel.cu:545-562): over the target pairs,
+// the sum of 1 / distance and the number closer than MaxDist voxel lengths
+void target_closeness(RobotRun &r, const double *pos) {
+    const double R = r.m->maxDistPairs * r.m->latticeDim;
+    double ret = 0;
+    r.pairs = 0;
+    for (size_t i = 0; i < r.targets.size(); i++)
+        for (size_t j = i + 1; j < r.targets.size(); j++) {
+            const double *a = pos + 3 * r.targets[i], *b = pos + 3 * r.targets[j];
+            const double dx = a[0] - b[0], dy = a[1] - b[1], dz = a[2] - b[2];
+            const double d = std::sqrt(dx * dx + dy * dy + dz * dz);
+            if (d < R) r.pairs++;
+            ret += 1 / d;
+        }
+    r.closeness = ret;
 }
 
 void appendf(std::string &s, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
@@ -216,8 +239,11 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
         x.realStep = (long long)(m.recordStepSize / (10000 * rec * m.dtFrac)) + 1;
         const unsigned su = mt_vars_used(m.stopFormula), fu = mt_vars_used(m.fitnessFormula);
         x.reinitPending = m.secondaryExperiment != 0;
-        x.needCycle = (su & ((1u << MV_X) | (1u << MV_Y) | (1u << MV_Z) | (1u << MV_ANGLE))) || (fu & (1u << MV_ANGLE)) ||
-                      x.reinitPending;
+        const unsigned sampled = (1u << MV_ANGLE) | (1u << MV_CLOSENESS) | (1u << MV_PAIRS);
+        x.needCycle = (su & ((1u << MV_X) | (1u << MV_Y) | (1u << MV_Z) | sampled)) || (fu & sampled) ||
+                      x.reinitPending || m.maxDistPairs != 0;
+        for (int v = 0; v < m.nv(); v++)
+            if (m.palette[m.vpal[v] - 1].is_target) x.targets.push_back(v);
         std::memcpy(x.com, &com[3 * r], sizeof(x.com));
         std::memcpy(x.comInit, x.com, sizeof(x.com));
         x.initPos.resize((size_t)3 * m.nv());
@@ -250,6 +276,7 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
         o.diverged = x.diverged;
         o.angle_sample_times = x.needCycle ? x.angleSamples : (int)(x.j / x.cycle);
         o.num_voxel = m.nv();
+        o.num_close_pairs = x.pairs;
         o.vox_size = m.latticeDim;
         for (int k = 0; k < 3; k++) {
             o.initial_com[k] = x.comInit[k];
@@ -265,7 +292,8 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
         o.fitness_score = 0;
         if (!m.fitnessFormula.empty()) {
             const double vars[MV_COUNT] = {o.current_com[0] - x.comInit[0], o.current_com[1] - x.comInit[1],
-                                           o.current_com[2] - x.comInit[2], 0, x.t, x.recentAngle, 0, 0, (double)m.nv()};
+                                           o.current_com[2] - x.comInit[2], (double)x.hit, x.t, x.recentAngle,
+                                           x.closeness, (double)x.pairs, (double)m.nv()};
             mt_eval(m.fitnessFormula, vars, &o.fitness_score);
         }
         return VX3MI_OK;
@@ -302,7 +330,8 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
                     tt += (double)x.dt;
                 }
             }
-            // the stop formula can only change with t between samples
+            // the stop formula can only change with t (and hit, every step) between samples
+            if (!x.m->stopFormula.empty() && (mt_vars_used(x.m->stopFormula) & (1u << MV_HIT))) k = 1;
             if (!x.m->stopFormula.empty() && (mt_vars_used(x.m->stopFormula) & (1u << MV_T))) {
                 RobotRun probe = x;
                 for (long long s = 1; s < k; s++) {
@@ -336,7 +365,15 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
                 x.t += (double)x.dt;
             }
             x.j += seg;
-            if (x.needCycle && x.j % x.cycle == 0) cycle_sample(x, &com[3 * r]);
+            x.hit = rt[r].collisionCount;
+            if (x.needCycle && x.j % x.cycle == 0) {
+                cycle_sample(x, &com[3 * r]);
+                if (x.m->maxDistPairs != 0) {
+                    pos.resize((size_t)3 * x.m->nv());
+                    if ((rc = b.frame(r, pos.data(), nullptr, nullptr, nullptr))) return rc;
+                    target_closeness(x, pos.data());
+                }
+            }
             if (x.reinitPending && x.m->reinitInitialPositionAfter < tLast) {
                 x.reinitPending = false;
                 std::memcpy(x.comInit, x.com, sizeof(x.com));  // InitializeCenterOfMass: the sampled centre of mass

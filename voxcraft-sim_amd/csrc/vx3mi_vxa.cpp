@@ -516,10 +516,7 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
         if (load(sim, "EnableCilia", v)) L.enable_cilia = v != 0;
         if (load(sim, "EnableSignals", v)) L.enable_signals = v != 0;
         double d;
-        if (load(sim, "MaxDistInVoxelLengthsToCountAsPair", d) && d > 0) {
-            err = "MaxDistInVoxelLengthsToCountAsPair (close-pair counting) is not supported";
-            return VX3MI_E_PARSE;
-        }
+        if (load(sim, "MaxDistInVoxelLengthsToCountAsPair", d)) L.max_dist_pairs = d;
         load(sim, "SecondaryExperiment", c.secondaryExperiment);
         L.secondary_experiment = c.secondaryExperiment != 0;
         load(sim, "SavePositionOfAllVoxels", c.savePositions);
@@ -540,8 +537,9 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
                 std::vector<MtToken> f, post;
                 if ((rc = parse_formula(ff->child(names[a]), f, err)) != VX3MI_OK) return rc;
                 if (f.empty() || (f.size() == 2 && f[0].op == MT_CONST && f[0].value == 0.0)) continue;
-                if (mt_vars_used(f) & (1u << MV_ANGLE)) {
-                    err = "force fields that use the recent angle are not supported";
+                if (mt_vars_used(f) & ((1u << MV_ANGLE) | (1u << MV_CLOSENESS) | (1u << MV_PAIRS))) {
+                    // sampled inside doTimeStep every CycleStep (VX3_VoxelyzeKernel.cu:314-334)
+                    err = "force fields that use the recent angle, targetCloseness or numClosePairs are not supported";
                     return VX3MI_E_PARSE;
                 }
                 if (!mt_postfix(f, post)) {
@@ -673,7 +671,7 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
                 load(mech, "CTE", m.cte);
                 load(mech, "uStatic", m.u_static);
                 load(mech, "uDynamic", m.u_dynamic);
-                if (load(mech, "isTarget", iv) && iv) { err = "target materials (isTarget) are not supported"; return VX3MI_E_PARSE; }
+                if (load(mech, "isTarget", iv)) m.is_target = iv != 0;
                 if (load(mech, "Fixed", iv)) m.fixed = iv != 0;
                 if (load(mech, "isMeasured", iv)) m.is_measured = iv;
                 load(mech, "TurnOnThermalExpansionAfterThisManySeconds", m.thermal_on_after);
@@ -852,8 +850,8 @@ int vx3mi_model_set_force_field(vx3mi_model *m, int axis, const char *formula_xm
         set_error("XML: " + e.msg);
         return VX3MI_E_PARSE;
     }
-    if (!f.empty() && (mt_vars_used(f) & (1u << MV_ANGLE))) {
-        set_error("force fields that use the recent angle are not supported");
+    if (!f.empty() && (mt_vars_used(f) & ((1u << MV_ANGLE) | (1u << MV_CLOSENESS) | (1u << MV_PAIRS)))) {
+        set_error("force fields that use the recent angle, targetCloseness or numClosePairs are not supported");
         return VX3MI_E_PARSE;
     }
     if (!f.empty() && !mt_postfix(f, post)) { set_error("force-field formula nested too deeply"); return VX3MI_E_PARSE; }

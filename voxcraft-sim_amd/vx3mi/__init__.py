@@ -32,7 +32,7 @@ PROBLEM_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp
                         floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
                         temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None,
                         enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None, enable_signals=0,
-                        enable_detach=0)
+                        enable_detach=0, max_dist_pairs=0.0)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, is_target=0, thermal_on_after=0.0,
@@ -77,7 +77,7 @@ class _Settings(C.Structure):
                 ("record_step_size", C.c_int), ("record_voxel", C.c_int), ("record_link", C.c_int),
                 ("has_stop_formula", C.c_int), ("has_fitness_formula", C.c_int),
                 ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int), ("enable_signals", C.c_int),
-                ("enable_detach", C.c_int)]
+                ("enable_detach", C.c_int), ("max_dist_pairs", C.c_double)]
 
 
 FORMULA_STOP, FORMULA_FITNESS = 0, 1
@@ -111,7 +111,7 @@ class _Lattice(C.Structure):
                 ("vary_temp", C.c_int), ("temp_base", C.c_double), ("temp_amplitude", C.c_double),
                 ("temp_period", C.c_double), ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int),
                 ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double)),
-                ("enable_signals", C.c_int), ("enable_detach", C.c_int)]
+                ("enable_signals", C.c_int), ("enable_detach", C.c_int), ("max_dist_pairs", C.c_double)]
 
 
 def library_path() -> str:
@@ -162,6 +162,7 @@ def lib():
         "vx3mi_get_link_flags": (C.c_int, [vp, C.c_int, ip, fp]),
         "vx3mi_batch_removed": (C.c_int, [vp, C.c_int, ip, ip]),
         "vx3mi_batch_signals": (C.c_int, [vp, C.c_int, dp]),
+        "vx3mi_batch_hits": (C.c_int, [vp, ip]),
         "vx3mi_model_cilia": (C.c_int, [vp, dp, dp]),
         "vx3mi_model_material": (C.c_int, [vp, C.c_int, C.POINTER(_Material)]),
         "vx3mi_batch_collision_overflow": (C.c_int, [vp, ip]),
@@ -250,7 +251,7 @@ class Model:
                         int(p["temp_enabled"]), int(p["vary_temp"]), p["temp_base"], p["temp_amplitude"],
                         p["temp_period"], int(p["enable_cilia"]), int(p["secondary_experiment"]),
                         _ptr(cb, C.c_double), _ptr(cs, C.c_double), int(p["enable_signals"]),
-                        int(p["enable_detach"]))
+                        int(p["enable_detach"]), float(p["max_dist_pairs"]))
         h = C.c_void_p()
         _check(lib().vx3mi_model_from_lattice(C.byref(spec), C.byref(h)))
         m = cls(h.value)
@@ -445,6 +446,12 @@ class Batch:
         vr, lr = np.zeros(nv, dtype=np.int32), np.zeros(max(1, nl), dtype=np.int32)
         _check(lib().vx3mi_batch_removed(self._h, int(robot), _ptr(vr, C.c_int), _ptr(lr, C.c_int)))
         return vr, lr[:nl]
+
+    def hits(self) -> np.ndarray:
+        """collisionCount per robot (target / non-target contacts over the steps)."""
+        out = np.zeros(self.nrobot, dtype=np.int32)
+        _check(lib().vx3mi_batch_hits(self._h, _ptr(out, C.c_int)))
+        return out
 
     def signals(self, robot: int = -1) -> np.ndarray:
         """Each voxel's localSignal (EnableSignals)."""

@@ -58,6 +58,8 @@ def _simulator(p: dict, stop=None, fitness=None, record=None) -> str:
         parts.append(_el("SecondaryExperiment", "1"))
     if p.get("enable_signals"):
         parts.append(_el("EnableSignals", "1"))
+    if p.get("max_dist_pairs"):
+        parts.append(_el("MaxDistInVoxelLengthsToCountAsPair", _num(p["max_dist_pairs"])))
     if record is not None:
         step, vox, link = record
         parts.append(_el("RecordHistory", _el("RecordStepSize", str(int(step))) + _el("RecordVoxel", str(int(vox)))
@@ -122,6 +124,8 @@ def _palette(mats: Sequence[dict]) -> str:
             mech += _el("TurnOnThermalExpansionAfterThisManySeconds", _num(m["thermal_on_after"]))
         if m["cilia"]:
             mech += _el("Cilia", _num(m["cilia"]))
+        if m["is_target"]:
+            mech += _el("isTarget", "1")
         if m["cilia_on_after"]:
             mech += _el("TurnOnCiliaAfterThisManySeconds", _num(m["cilia_on_after"]))
         if m["remove_after"]:
