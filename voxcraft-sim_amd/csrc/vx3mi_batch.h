@@ -17,6 +17,7 @@ struct RobotParams {
     int varyTemp, enableExpansion, collisions;
     int secondary;           // SecondaryExperiment: materials are removed after their time
     int detach;              // EnableDetach: failed links leave both voxels
+    int signals;             // EnableSignals (the signal arrays are the batch's)
     int ffOff[3], ffLen[3];  // force-field postfix programs in the formula table (length 0: none)
     int voff, vcount, loff, lcount;
     // collision broadphase: surface voxels [soff, soff+scount) of the surface

@@ -187,7 +187,7 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
     a.mo1[v] = T4<T>{s.h.y, s.h.z, (T)s.flags, (T)0};
     // the voxel's own signal section ends a full timeStep (not one of a voxel
     // with all six degrees of freedom fixed, VX3_Voxel.cu:166-172)
-    if (a.sig && dt != 0.0f && !(vm.ext >= 0 && a.exts[vm.ext].dof == DOF_ALL)) {
+    if (a.sig && rp.signals && dt != 0.0f && !(vm.ext >= 0 && a.exts[vm.ext].dof == DOF_ALL)) {
         VoxelSignal sg = a.sig[v];
         signal_own(sg, a.sigMat[v], rt->time[parity]);
         a.sig[v] = sg;
@@ -200,7 +200,7 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
 template <typename T> __global__ __launch_bounds__(256) void k_signal_deliver(Dev<T> a, int v0, int n) {
     const int v = v0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= v0 + n) return;
-    if (a.rt[a.vrob[v]].diverged || (a.vrem && a.vrem[v])) return;
+    if (!a.rp[a.vrob[v]].signals || a.rt[a.vrob[v]].diverged || (a.vrem && a.vrem[v])) return;
     const SignalMat sm = a.sigMat[v];
     for (int d = 0; d < 6; d++) {
         const int l = a.vlinks[6 * v + d];
@@ -348,7 +348,7 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
             if (s < found[k]) atomicAdd(&rt->collisionCount, 1);
             hitTarget = true;
         }
-        if (hitTarget && !ti && a.sig) {
+        if (hitTarget && !ti && a.sig && rp.signals) {
             VoxelSignal sg = a.sig[v];
             signal_receive(sg, a.sigMat[v], 100, rt->time[parity], true);
             a.sig[v] = sg;
@@ -615,6 +615,7 @@ template <typename T> struct Batch : BatchBase {
             p.collisions = m.enableCollision;
             p.secondary = m.secondaryExperiment;
             p.detach = m.enableDetach;
+            p.signals = m.enableSignals;
             p.voff = voff; p.vcount = m.nv(); p.loff = loff; p.lcount = m.nl();
             if (m.enableCilia) anyCilia = true;
             if (m.enableSignals) anySignals = true;
