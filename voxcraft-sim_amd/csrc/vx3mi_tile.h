@@ -17,25 +17,27 @@
 namespace vx3mi {
 
 // Tile configurations: NT threads, up to KL links and KV voxels per thread,
-// MINB workgroups per compute unit for the register budget.  The large tile
+// MINW the minimum waves per SIMD the register budget must allow (the second
+// __launch_bounds__ argument).  The large tile
 // uses ~158 KB of LDS (one workgroup per compute unit); the FP32 one runs 768
 // threads, 12 waves, so each SIMD holds three waves of at most 168 VGPRs (the
 // kernel is VALU-issue bound, and a third wave hides the LDS and barrier
 // waits: +33 % over 512 threads holding four links each at two waves).  The
 // small tile uses half the LDS (two workgroups per compute unit, so one can
-// compute while the other waits at a barrier).  The batch's autotuner picks one.
+// compute while the other waits at a barrier), 384 threads each, again three
+// waves per SIMD.  The batch's autotuner picks one.
 template <typename T, int SMALL> struct TileCfg;
 template <> struct TileCfg<float, 0> {
-    static constexpr int NT = 768, KL = 3, KV = 2, LMAX = 2048, VMAX = 1536, RMAX = 128, MINB = 1;
+    static constexpr int NT = 768, KL = 3, KV = 2, LMAX = 2048, VMAX = 1536, RMAX = 128, MINW = 1;
 };
 template <> struct TileCfg<float, 1> {
-    static constexpr int NT = 256, KL = 4, KV = 3, LMAX = 1024, VMAX = 768, RMAX = 64, MINB = 2;
+    static constexpr int NT = 384, KL = 3, KV = 2, LMAX = 1024, VMAX = 768, RMAX = 64, MINW = 3;
 };
 template <> struct TileCfg<double, 0> {
-    static constexpr int NT = 512, KL = 2, KV = 2, LMAX = 1024, VMAX = 768, RMAX = 64, MINB = 2;
+    static constexpr int NT = 512, KL = 2, KV = 2, LMAX = 1024, VMAX = 768, RMAX = 64, MINW = 2;
 };
 template <> struct TileCfg<double, 1> {
-    static constexpr int NT = 256, KL = 2, KV = 2, LMAX = 512, VMAX = 384, RMAX = 32, MINB = 2;
+    static constexpr int NT = 256, KL = 2, KV = 2, LMAX = 512, VMAX = 384, RMAX = 32, MINW = 2;
 };
 
 // LDS robot table of a tile
@@ -70,7 +72,7 @@ __device__ __forceinline__ int tile_slot(const int *pre, int n, int x) {
 // LDSMAT: the batch has at most kTileMatCache voxel and link materials, which
 // are read from LDS in the step loop instead of global memory
 template <typename T, int SMALL, int LDSMAT>
-__global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINB)) void k_tile(Dev<T> a, const int4 *tiles, const int *tileRobots,
+__global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINW)) void k_tile(Dev<T> a, const int4 *tiles, const int *tileRobots,
                                                                 int nsteps, int parity0, int nLinkMats, int nVoxMats) {
     using C = TileCfg<T, SMALL>;
     constexpr int NT = C::NT, KL = C::KL, KV = C::KV;

@@ -479,7 +479,7 @@ class Batch:
         _check(lib().vx3mi_batch_recommended_dt(self._h, _ptr(d, C.c_double)))
         return d
 
-    CANDIDATES = ("tiled-768", "tiled-256x2", "persistent-128", "persistent-256", "persistent-512", "flat-graph")
+    CANDIDATES = ("tiled-768", "tiled-384x2", "persistent-128", "persistent-256", "persistent-512", "flat-graph")
 
     def autotune(self, trial_steps: int = 100) -> dict:
         """Times every launch configuration on trial steps (state restored after)
