@@ -23,15 +23,15 @@ namespace vx3mi {
 // threads, 12 waves, so each SIMD holds three waves of at most 168 VGPRs (the
 // kernel is VALU-issue bound, and a third wave hides the LDS and barrier
 // waits: +33 % over 512 threads holding four links each at two waves).  The
-// small tile uses half the LDS (two workgroups per compute unit, so one can
-// compute while the other waits at a barrier), 384 threads each, again three
-// waves per SIMD.  The batch's autotuner picks one.
+// small tile stays within 78 KB of LDS, so two workgroups fit a compute unit's
+// 160 KB (one computes while the other waits at a barrier), 384 threads each,
+// again three waves per SIMD.  The batch's autotuner picks one.
 template <typename T, int SMALL> struct TileCfg;
 template <> struct TileCfg<float, 0> {
     static constexpr int NT = 768, KL = 3, KV = 2, LMAX = 2048, VMAX = 1536, RMAX = 128, MINW = 1;
 };
 template <> struct TileCfg<float, 1> {
-    static constexpr int NT = 384, KL = 3, KV = 2, LMAX = 1024, VMAX = 768, RMAX = 64, MINW = 3;
+    static constexpr int NT = 384, KL = 3, KV = 2, LMAX = 960, VMAX = 768, RMAX = 64, MINW = 3;
 };
 template <> struct TileCfg<double, 0> {
     static constexpr int NT = 512, KL = 2, KV = 2, LMAX = 1024, VMAX = 768, RMAX = 64, MINW = 2;
@@ -56,7 +56,7 @@ template <int RMAX> struct TileRobots {
 
 // material tables of at most this many entries (large / small tile) are
 // copied into LDS at launch; the small tile keeps within half the LDS
-template <int SMALL> struct TileMatCache { static constexpr int N = SMALL ? 16 : 24; };
+template <int SMALL> struct TileMatCache { static constexpr int N = SMALL ? 12 : 24; };
 
 // largest j with pre[j] <= x (pre ascending, pre[0] = 0, pre[n] = total)
 __device__ __forceinline__ int tile_slot(const int *pre, int n, int x) {
