@@ -669,3 +669,57 @@ def test_targets_fp64_bit_exact(mode):
         assert b.hits().tolist() == [o.collision_count() for o in oracles]
     assert b.hits()[0] > 0
     b.close()
+
+
+def _ragged_problems():
+    """Robots of very different sizes in one batch: one voxel (no links), two
+    voxels (one link), a 1 x 1 x 40 bar standing on the floor, a 3 x 3 x 3
+    cube and a random robot."""
+    base = dict(configs.config0(), phase=None)
+    one = dict(base, dims=(1, 1, 1), cells=np.ones(1, dtype=np.int8))
+    two = dict(base, dims=(2, 1, 1), cells=np.ones(2, dtype=np.int8), phase=np.array([0.0, 0.5]))
+    bar = dict(base, dims=(1, 1, 40), cells=np.ones(40, dtype=np.int8), phase=np.linspace(0, 1, 40))
+    cube = dict(base, dims=(3, 3, 3), cells=np.ones(27, dtype=np.int8), phase=np.linspace(0, 1, 27))
+    return [one, two, bar, cube, small_problem(seed=9)]
+
+
+@pytest.mark.parametrize("mode,tile,threads", [(vx3mi.MODE_PLAIN, 0, 0), (vx3mi.MODE_FLAT, 0, 0),
+                                               (vx3mi.MODE_PERSISTENT, 0, 128), (vx3mi.MODE_TILED, 0, 0),
+                                               (vx3mi.MODE_TILED, 1, 0)])
+def test_ragged_batch_every_mode_fp64(mode, tile, threads):
+    """Ragged batches (a voxel without links, a single link, a 40-voxel chain)
+    give the oracle's bits in every launch mode; the robots' clocks advance
+    independently (each its own recommended dt)."""
+    log(f"ragged mode {mode} tile {tile}")
+    probs = _ragged_problems()
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
+    run_mode = mode
+    if mode in (vx3mi.MODE_TILED, vx3mi.MODE_PERSISTENT):
+        b.configure(mode, tile, threads)
+        run_mode = vx3mi.MODE_AUTO
+    oracles = [O.OracleModel(p) for p in probs]
+    b.step(500, mode=run_mode)
+    for r, o in enumerate(oracles):
+        o.step(500)
+        s, ref = b.state(r), o.state()
+        for k in ("pos", "quat", "linmom", "angmom"):
+            assert np.array_equal(s[k], ref[k]), (r, k)
+    assert np.array_equal(b.time(), np.array([o.time for o in oracles]))
+    b.close()
+
+
+def test_robot_beyond_tile_capacity_runs_elsewhere():
+    """A robot larger than the large tile (2,352 voxels > 1,536) makes the tiled
+    mode unavailable (configure raises) and the automatic choice runs it in
+    another mode, still bit for bit in FP64."""
+    big = dict(configs.config0(), dims=(14, 14, 12), cells=np.ones(14 * 14 * 12, dtype=np.int8), phase=None)
+    probs = [big, small_problem(seed=10)]
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP64)
+    with pytest.raises(vx3mi.VX3MIError):
+        b.configure(vx3mi.MODE_TILED, 0, 0)
+    b.step(100)
+    for r, p in enumerate(probs):
+        o = O.OracleModel(p)
+        o.step(100)
+        assert np.array_equal(b.state(r)["pos"], o.state()["pos"]), r
+    b.close()
