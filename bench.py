@@ -63,6 +63,18 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
+def cpu_model() -> str:
+    """The host CPU's model name (/proc/cpuinfo), or "unknown"."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(problems, nsteps: int, omp: bool = False):
     """Reference CPU Voxelyze (oracle/_ref, built from the reference sources) on
     a sample of robots; falls back to the FP64 oracle port when absent.  omp:
@@ -345,7 +357,7 @@ def main() -> int:
         svox = sum(int((np.asarray(p["cells"]) > 0).sum()) for p in sample)
         cpu = {"value": cv, "unit": "voxel-steps/s", "cores": 1, "kind": kind,
                "sample": f"first {len(sample)} robots of the workload ({svox} voxels) x {args.cpu_steps} steps, "
-                         "single thread"}
+                         "single thread", "cpu_model": cpu_model()}
         log(f"bench: CPU baseline ({kind}) {cv:.3e} voxel-steps/s")
         osteps = max(1, args.cpu_steps // 5)
         ov, _ = cpu_baseline(sample, osteps, omp=True)
