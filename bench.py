@@ -223,14 +223,21 @@ def parity_report(c1_steps: int = 20000, c1_window: int = 1000, every: int = 50)
         ep, eq = max(ep, float(dp.max())), max(eq, float(dq.max()))
         ep999 = max(ep999, float(np.quantile(dp, 0.999)))
         over = max(over, int(np.sum((dp > 1e-5) | (dq > 1e-5))))
-    b32.step(c1_steps - c1_window)
-    b64.step(c1_steps - c1_window)
+    rest = c1_steps - c1_window
+    b32.step(rest)
+    ms32 = b32.last_step_ms()  # HIP events on the batch stream around the steps
+    b64.step(rest)
+    ms64 = b64.last_step_ms()
+    nv1 = int(len(dp))
     out["c1_fp32_vs_fp64"] = {"e_pos_first_1000": ep, "e_q_first_1000": eq,
                               "e_pos_p999_first_1000": ep999,
                               "most_voxels_over_1e-5_at_a_sample": over, "voxels": int(len(dp)),
                               f"com_after_{c1_steps}": com_rel(b32.com()[0], c32, b64.com()[0], c64),
                               "diverged": [int(b32.diverged()[0]), int(b64.diverged()[0])],
-                              "collision_overflow": [int(b32.collision_overflow()[0]), int(b64.collision_overflow()[0])]}
+                              "collision_overflow": [int(b32.collision_overflow()[0]), int(b64.collision_overflow()[0])],
+                              "voxel_steps_per_s": {"fp32": nv1 * rest / (ms32 * 1e-3), "fp64": nv1 * rest / (ms64 * 1e-3),
+                                                    "mode": "flat (hipGraph chunks), collisions on",
+                                                    "steps": rest}}
     b32.close()
     b64.close()
     # config 3: differentiable FP32 vs the oracle's differentiable mode, 2000 steps
