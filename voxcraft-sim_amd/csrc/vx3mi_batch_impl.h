@@ -56,6 +56,8 @@ template <typename T> struct Dev {
     int *ccnt[2];          // bucket counts, one table per step parity
     int *ctab[2];          // bucket entries (kBucketCap per bucket): surface indices
     T4<T> *contactW;
+    int *fcount;           // per surface voxel: partners found this step
+    int *flist;            // per surface voxel: kMaxContacts partner surface indices (unordered)
     long tableBuckets;
     const VoxelCilia *cilia;  // per-voxel cilia, or nullptr when no robot has EnableCilia
     // removal (SecondaryExperiment): step count at whose end a voxel / link was
@@ -264,6 +266,7 @@ template <typename T> __device__ __forceinline__ void col_insert(const Dev<T> &a
     const int v = a.surf[s];
     RobotTime *rt = a.rt + a.vrob[v];
     const RobotParams &rp = a.rp[a.vrob[v]];
+    a.fcount[s] = 0;
     if (!rp.collisions || rt->diverged || !col_active(a, v)) return;
     const V3<T> p = abs_position(a, v, a.pq0[v]);
     const double h = rp.cellSize;
@@ -276,6 +279,61 @@ template <typename T> __device__ __forceinline__ void col_insert(const Dev<T> &a
     else atomicOr(&rt->overflow, 1);
 }
 
+// One neighbouring cell (c = 0..26, offsets -1..1 in x fastest) of surface
+// voxel s: the partners in its bucket that are inside the contact envelope,
+// not directly linked (is_neighbor depth 1, VX3_VoxelyzeKernel.cu:651-680,
+// :701-703) and not both fixed, appended to s's list.  27 work-items per voxel
+// keep the dependent bucket / cell / position loads of one voxel in parallel.
+template <typename T> __device__ __forceinline__ void col_find(const Dev<T> &a, int s, int c, int parity) {
+    const int v = a.surf[s];
+    RobotTime *rt = a.rt + a.vrob[v];
+    const RobotParams &rp = a.rp[a.vrob[v]];
+    if (!rp.collisions || rt->diverged || !col_active(a, v)) return;
+    const int dx = c % 3 - 1, dy = (c / 3) % 3 - 1, dz = c / 9 - 1;
+    const int4 cs = a.scell[s];
+    const int key = (int)(cell_hash(cs.x + dx, cs.y + dy, cs.z + dz) & (unsigned)rp.mask);
+    const long b = (long)rp.toff + key;
+    const int n = min(a.ccnt[parity][b], kBucketCap);
+    if (n == 0) return;
+    const VoxelMat &mi = a.vmats[a.vmat[v]];
+    const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v];
+    const V3<T> pi = abs_position(a, v, qi0);
+    const T bsaI = base_size_average<T>(mi.nomSize, (float)qi1.w, mi.alphaCTE);
+    const int pki = a.lat[v];
+    for (int k = 0; k < n; k++) {
+        const int sj = a.ctab[parity][b * kBucketCap + k];
+        if (sj == s) continue;
+        const int4 cj = a.scell[sj];
+        if (cj.x != cs.x + dx || cj.y != cs.y + dy || cj.z != cs.z + dz) continue;  // hash alias
+        const int vj = a.surf[sj];
+        // a link joins lattice neighbours only: a unit lattice offset in direction d
+        // (X_POS..Z_NEG) is linked when the voxel's link slot d is present
+        const int pkj = a.lat[vj];
+        const int lx = (pkj & 1023) - (pki & 1023), ly = ((pkj >> 10) & 1023) - ((pki >> 10) & 1023),
+                  lz = (pkj >> 20) - (pki >> 20);
+        if (abs(lx) + abs(ly) + abs(lz) == 1) {
+            const int d = lx ? (lx > 0 ? 0 : 1) : (ly ? (ly > 0 ? 2 : 3) : (lz > 0 ? 4 : 5));
+            if (a.vlinks[6 * v + d] >= 0) continue;
+        }
+        const VoxelMat &mj = a.vmats[a.vmat[vj]];
+        if (mi.fixed && mj.fixed) continue;
+        const T4<T> qj0 = a.pq0[vj], qj1 = a.pq1[vj];
+        const V3<T> pj = abs_position(a, vj, qj0);
+        V3<T> diff = pi - pj;
+        if constexpr (!kAbsolutePositions<T>) {
+            // lattice offset and displacement difference kept apart in float
+            const T L = (T)mi.nomSize;
+            diff = V3<T>{(T)(-lx) * L + (qi0.x - qj0.x), (T)(-ly) * L + (qi0.y - qj0.y), (T)(-lz) * L + (qi0.z - qj0.z)};
+        }
+        const T bsaJ = base_size_average<T>(mj.nomSize, (float)qj1.w, mj.alphaCTE);
+        if (!in_contact_envelope(diff, bsaI, bsaJ)) continue;
+        const int slot = atomicAdd(&a.fcount[s], 1);
+        if (slot < kMaxContacts) a.flist[(long)s * kMaxContacts + slot] = sj;
+        else atomicOr(&rt->overflow, 2);
+    }
+}
+
+// The contact force of surface voxel s from the partners col_find listed.
 template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity) {
     const int v = a.surf[s];
     RobotTime *rt = a.rt + a.vrob[v];
@@ -285,51 +343,9 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
     const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v];
     const V3<T> pi = abs_position(a, v, qi0);
     const T bsaI = base_size_average<T>(mi.nomSize, (float)qi1.w, mi.alphaCTE);
-    // directly linked voxels never collide (is_neighbor depth 1, :651-680, :701-703)
-    int nb[6];
-#pragma unroll
-    for (int d = 0; d < 6; d++) {
-        const int l = a.vlinks[6 * v + d];
-        nb[d] = l < 0 ? -1 : ((d & 1) ? a.lends[l].x : a.lends[l].y);
-    }
-    const int4 c = a.scell[s];
     int found[kMaxContacts];
-    int nf = 0;
-    for (int dz = -1; dz <= 1; dz++)
-        for (int dy = -1; dy <= 1; dy++)
-            for (int dx = -1; dx <= 1; dx++) {
-                const int key = (int)(cell_hash(c.x + dx, c.y + dy, c.z + dz) & (unsigned)rp.mask);
-                const long b = (long)rp.toff + key;
-                const int n = min(a.ccnt[parity][b], kBucketCap);
-                for (int k = 0; k < n; k++) {
-                    const int sj = a.ctab[parity][b * kBucketCap + k];
-                    if (sj == s) continue;
-                    const int4 cj = a.scell[sj];
-                    if (cj.x != c.x + dx || cj.y != c.y + dy || cj.z != c.z + dz) continue;  // hash alias
-                    const int vj = a.surf[sj];
-                    bool linked = false;
-#pragma unroll
-                    for (int d = 0; d < 6; d++) linked |= nb[d] == vj;
-                    if (linked) continue;
-                    const VoxelMat &mj = a.vmats[a.vmat[vj]];
-                    if (mi.fixed && mj.fixed) continue;
-                    const T4<T> qj0 = a.pq0[vj], qj1 = a.pq1[vj];
-                    const V3<T> pj = abs_position(a, vj, qj0);
-                    V3<T> diff = pi - pj;
-                    if constexpr (!kAbsolutePositions<T>) {
-                        // lattice offset and displacement difference kept apart in float
-                        const int pki = a.lat[v], pkj = a.lat[vj];
-                        const T L = (T)mi.nomSize;
-                        diff = V3<T>{(T)((pki & 1023) - (pkj & 1023)) * L + (qi0.x - qj0.x),
-                                     (T)(((pki >> 10) & 1023) - ((pkj >> 10) & 1023)) * L + (qi0.y - qj0.y),
-                                     (T)((pki >> 20) - (pkj >> 20)) * L + (qi0.z - qj0.z)};
-                    }
-                    const T bsaJ = base_size_average<T>(mj.nomSize, (float)qj1.w, mj.alphaCTE);
-                    if (!in_contact_envelope(diff, bsaI, bsaJ)) continue;
-                    if (nf < kMaxContacts) found[nf++] = sj;
-                    else atomicOr(&rt->overflow, 2);
-                }
-            }
+    const int nf = min(a.fcount[s], kMaxContacts);
+    for (int k = 0; k < nf; k++) found[k] = a.flist[(long)s * kMaxContacts + k];
     // ascending partner index: the order in which the serialized pair loop of
     // the reference accumulates this voxel's contact force
     for (int i = 1; i < nf; i++) {
@@ -380,6 +396,12 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
 template <typename T> __global__ __launch_bounds__(256) void k_col_insert(Dev<T> a, int s0, int n, int parity) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) col_insert(a, s0 + i, parity);
+}
+
+// 27 work-items per surface voxel, one per neighbouring cell
+template <typename T> __global__ __launch_bounds__(256) void k_col_find(Dev<T> a, int s0, int n, int parity) {
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < 27L * n) col_find(a, s0 + (int)(i / 27), (int)(i % 27), parity);
 }
 
 // contact forces; also empties the other parity's table for the next step
@@ -456,6 +478,8 @@ template <typename T, int NT> __global__ __launch_bounds__(NT) void k_persistent
         if (col) {
             for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_insert(a, i, parity);
             for (int b = threadIdx.x; b <= rp.mask; b += blockDim.x) a.ccnt[parity ^ 1][(long)rp.toff + b] = 0;
+            __syncthreads();
+            for (int i = threadIdx.x; i < 27 * rp.scount; i += blockDim.x) col_find(a, rp.soff + i / 27, i % 27, parity);
             __syncthreads();
             for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_contact(a, i, parity);
         }
@@ -824,13 +848,17 @@ template <typename T> struct Batch : BatchBase {
         d.contact = nullptr;
         d.tableBuckets = buckets;
         if (anyCollisions) {
-            int *surfd, *c0, *c1, *t0, *t1;
+            int *surfd, *c0, *c1, *t0, *t1, *fc, *fl;
             int4 *scd;
             T4<T> *cw;
             if ((rc = alloc(&surfd, S)) || (rc = alloc(&scd, S)) || (rc = alloc(&c0, buckets)) ||
                 (rc = alloc(&c1, buckets)) || (rc = alloc(&t0, buckets * kBucketCap)) ||
-                (rc = alloc(&t1, buckets * kBucketCap)) || (rc = alloc(&cw, V)))
+                (rc = alloc(&t1, buckets * kBucketCap)) || (rc = alloc(&cw, V)) || (rc = alloc(&fc, S)) ||
+                (rc = alloc(&fl, (size_t)S * kMaxContacts)))
                 return rc;
+            HIP_TRY(hipMemset(fc, 0, sizeof(int) * S));
+            d.fcount = fc;
+            d.flist = fl;
             HIP_TRY(hipMemcpy(surfd, surf.data(), sizeof(int) * S, hipMemcpyHostToDevice));
             HIP_TRY(hipMemset(c0, 0, sizeof(int) * buckets));
             HIP_TRY(hipMemset(c1, 0, sizeof(int) * buckets));
@@ -1005,6 +1033,7 @@ template <typename T> struct Batch : BatchBase {
 
     void launch_flat_step(int parity) {
         if (anyCollisions) k_col_insert<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
+        if (anyCollisions) k_col_find<T><<<(int)((27L * S + 255) / 256), 256, 0, stream>>>(d, 0, S, parity);
         if (L) k_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, 0, L, parity);
         if (anyCollisions) k_col_contact<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
         if (anyForceField) k_force_field<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
@@ -1014,7 +1043,7 @@ template <typename T> struct Batch : BatchBase {
         if (anyRemoval) k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
     }
     int kernels_per_flat_step() const {
-        return 2 + (anyCollisions ? 2 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0) +
+        return 2 + (anyCollisions ? 3 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0) +
                (anyDetach ? 1 : 0);
     }
 
@@ -1040,11 +1069,13 @@ template <typename T> struct Batch : BatchBase {
                 for (int r = 0; r < R; r++) {
                     const RobotParams &p = hrp[r];
                     if (p.scount) k_col_insert<T><<<(p.scount + 255) / 256, 256, 0, stream>>>(d, p.soff, p.scount, parity);
+                    if (p.scount)
+                        k_col_find<T><<<(int)((27L * p.scount + 255) / 256), 256, 0, stream>>>(d, p.soff, p.scount, parity);
                     if (p.lcount) k_links<T><<<(p.lcount + 255) / 256, 256, 0, stream>>>(d, p.loff, p.lcount, parity);
                     if (p.scount) k_col_contact<T><<<(p.scount + 255) / 256, 256, 0, stream>>>(d, p.soff, p.scount, parity);
                     if (anyForceField) k_force_field<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
                     k_voxels<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
-                    launches += 2 + (p.scount ? 2 : 0) + (anyForceField ? 1 : 0);
+                    launches += 2 + (p.scount ? 3 : 0) + (anyForceField ? 1 : 0);
                 }
                 // the end-of-step phases for the whole batch (each touches only its
                 // robot's voxels and links)
