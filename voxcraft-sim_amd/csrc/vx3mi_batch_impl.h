@@ -56,6 +56,8 @@ template <typename T> struct Dev {
     int *ccnt[2];          // bucket counts, one table per step parity
     int *ctab[2];          // bucket entries (kBucketCap per bucket): surface indices
     T4<T> *contactW;
+    int4 *sinfo;           // per surface voxel this step: voxel, lattice index, fixed
+    T4<T> *spos;           // per surface voxel this step: position field xyz, base size average
     int *fcount;           // per surface voxel: partners found this step
     int *flist;            // per surface voxel: kMaxContacts partner surface indices (unordered)
     long tableBuckets;
@@ -273,6 +275,12 @@ template <typename T> __device__ __forceinline__ void col_insert(const Dev<T> &a
     const int cx = (int)floor((double)p.x / h), cy = (int)floor((double)p.y / h), cz = (int)floor((double)p.z / h);
     const int key = (int)(cell_hash(cx, cy, cz) & (unsigned)rp.mask);
     a.scell[s] = int4{cx, cy, cz, key};
+    {  // the record the search reads for this voxel as a partner
+        const VoxelMat &m = a.vmats[a.vmat[v]];
+        const T4<T> q0 = a.pq0[v];
+        a.sinfo[s] = int4{v, a.lat[v], m.fixed, 0};
+        a.spos[s] = T4<T>{q0.x, q0.y, q0.z, base_size_average<T>(m.nomSize, (float)a.pq1[v].w, m.alphaCTE)};
+    }
     const long b = (long)rp.toff + key;
     const int slot = atomicAdd(&a.ccnt[parity][b], 1);
     if (slot < kBucketCap) a.ctab[parity][b * kBucketCap + slot] = s;
@@ -305,28 +313,27 @@ template <typename T> __device__ __forceinline__ void col_find(const Dev<T> &a, 
         if (sj == s) continue;
         const int4 cj = a.scell[sj];
         if (cj.x != cs.x + dx || cj.y != cs.y + dy || cj.z != cs.z + dz) continue;  // hash alias
-        const int vj = a.surf[sj];
+        const int4 ij = a.sinfo[sj];
+        const T4<T> pj = a.spos[sj];
         // a link joins lattice neighbours only: a unit lattice offset in direction d
         // (X_POS..Z_NEG) is linked when the voxel's link slot d is present
-        const int pkj = a.lat[vj];
+        const int pkj = ij.y;
         const int lx = (pkj & 1023) - (pki & 1023), ly = ((pkj >> 10) & 1023) - ((pki >> 10) & 1023),
                   lz = (pkj >> 20) - (pki >> 20);
         if (abs(lx) + abs(ly) + abs(lz) == 1) {
             const int d = lx ? (lx > 0 ? 0 : 1) : (ly ? (ly > 0 ? 2 : 3) : (lz > 0 ? 4 : 5));
             if (a.vlinks[6 * v + d] >= 0) continue;
         }
-        const VoxelMat &mj = a.vmats[a.vmat[vj]];
-        if (mi.fixed && mj.fixed) continue;
-        const T4<T> qj0 = a.pq0[vj], qj1 = a.pq1[vj];
-        const V3<T> pj = abs_position(a, vj, qj0);
-        V3<T> diff = pi - pj;
-        if constexpr (!kAbsolutePositions<T>) {
+        if (mi.fixed && ij.z) continue;
+        V3<T> diff;
+        if constexpr (kAbsolutePositions<T>) {
+            diff = pi - V3<T>{pj.x, pj.y, pj.z};
+        } else {
             // lattice offset and displacement difference kept apart in float
             const T L = (T)mi.nomSize;
-            diff = V3<T>{(T)(-lx) * L + (qi0.x - qj0.x), (T)(-ly) * L + (qi0.y - qj0.y), (T)(-lz) * L + (qi0.z - qj0.z)};
+            diff = V3<T>{(T)(-lx) * L + (qi0.x - pj.x), (T)(-ly) * L + (qi0.y - pj.y), (T)(-lz) * L + (qi0.z - pj.z)};
         }
-        const T bsaJ = base_size_average<T>(mj.nomSize, (float)qj1.w, mj.alphaCTE);
-        if (!in_contact_envelope(diff, bsaI, bsaJ)) continue;
+        if (!in_contact_envelope(diff, bsaI, pj.w)) continue;
         const int slot = atomicAdd(&a.fcount[s], 1);
         if (slot < kMaxContacts) a.flist[(long)s * kMaxContacts + slot] = sj;
         else atomicOr(&rt->overflow, 2);
@@ -849,13 +856,15 @@ template <typename T> struct Batch : BatchBase {
         d.tableBuckets = buckets;
         if (anyCollisions) {
             int *surfd, *c0, *c1, *t0, *t1, *fc, *fl;
-            int4 *scd;
-            T4<T> *cw;
+            int4 *scd, *sid;
+            T4<T> *cw, *spd;
             if ((rc = alloc(&surfd, S)) || (rc = alloc(&scd, S)) || (rc = alloc(&c0, buckets)) ||
                 (rc = alloc(&c1, buckets)) || (rc = alloc(&t0, buckets * kBucketCap)) ||
                 (rc = alloc(&t1, buckets * kBucketCap)) || (rc = alloc(&cw, V)) || (rc = alloc(&fc, S)) ||
-                (rc = alloc(&fl, (size_t)S * kMaxContacts)))
+                (rc = alloc(&fl, (size_t)S * kMaxContacts)) || (rc = alloc(&sid, S)) || (rc = alloc(&spd, S)))
                 return rc;
+            d.sinfo = sid;
+            d.spos = spd;
             HIP_TRY(hipMemset(fc, 0, sizeof(int) * S));
             d.fcount = fc;
             d.flist = fl;
