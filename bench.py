@@ -256,6 +256,27 @@ def parity_report(c1_steps: int = 20000, c1_window: int = 1000, every: int = 50)
     return out
 
 
+def config1_only(args) -> int:
+    """The 64x64x16 robot (45,800 voxels) with collisions, FP32, flat mode: warmup, then
+    args.steps timed steps with HIP events on the batch stream."""
+    m = vx3mi.Model.from_problem(configs.config1())
+    b = vx3mi.Batch([m], vx3mi.FP32)
+    b.step(args.warmup, mode=vx3mi.MODE_FLAT)
+    b.sync()
+    b.step(args.steps, mode=vx3mi.MODE_FLAT)
+    b.sync()
+    ms = b.last_step_ms()
+    nv = b.nv
+    out = {"metric": "voxel-steps/s", "value": nv * args.steps / (ms * 1e-3), "unit": "voxel-steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms / args.steps,
+           "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
+           "config": {"workload": "one 64x64x16 robot, collisions on, flat mode (hipGraph chunks)", "voxels": nv},
+           "diverged": int(b.diverged()[0]), "collision_overflow": int(b.collision_overflow()[0])}
+    b.close()
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -277,11 +298,16 @@ def main() -> int:
                     help="after the timed region, copy a device buffer of this size once (a copy kernel of "
                          "known size, to calibrate HBM byte counters)")
     ap.add_argument("--adjoint-budget", type=float, default=16.0 * 2**30)
+    ap.add_argument("--workload", choices=["batched", "config1"], default="batched",
+                    help="config1: only the 64x64x16 robot with collisions (flat mode), for profiling the "
+                         "collision kernels; prints its own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); robots shard across GPUs "
               "as independent batches, which this build does not launch.", file=sys.stderr)
         return 2
+    if args.workload == "config1":
+        return config1_only(args)
 
     mode = {"auto": vx3mi.MODE_AUTO, "tiled": vx3mi.MODE_TILED, "persistent": vx3mi.MODE_PERSISTENT,
             "flat": vx3mi.MODE_FLAT}[args.mode]
