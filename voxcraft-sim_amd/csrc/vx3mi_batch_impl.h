@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <climits>
 #include <map>
 #include <cstring>
 #include <new>
@@ -54,7 +55,7 @@ template <typename T> struct Dev {
     const int *surf;       // surface voxel ids, CSR by robot
     int4 *scell;           // per surface voxel: cell coordinates and bucket
     int *ccnt[2];          // bucket counts, one table per step parity
-    int *ctab[2];          // bucket entries (kBucketCap per bucket): surface indices
+    int4 *ctab[2];         // bucket entries (kBucketCap per bucket): surface index, cell coordinates
     T4<T> *contactW;
     int4 *sinfo;           // per surface voxel this step: voxel, lattice index, fixed
     T4<T> *spos;           // per surface voxel this step: position field xyz, base size average
@@ -283,7 +284,7 @@ template <typename T> __device__ __forceinline__ void col_insert(const Dev<T> &a
     }
     const long b = (long)rp.toff + key;
     const int slot = atomicAdd(&a.ccnt[parity][b], 1);
-    if (slot < kBucketCap) a.ctab[parity][b * kBucketCap + slot] = s;
+    if (slot < kBucketCap) a.ctab[parity][b * kBucketCap + slot] = int4{s, cx, cy, cz};
     else atomicOr(&rt->overflow, 1);
 }
 
@@ -309,10 +310,10 @@ template <typename T> __device__ __forceinline__ void col_find(const Dev<T> &a, 
     const T bsaI = base_size_average<T>(mi.nomSize, (float)qi1.w, mi.alphaCTE);
     const int pki = a.lat[v];
     for (int k = 0; k < n; k++) {
-        const int sj = a.ctab[parity][b * kBucketCap + k];
+        const int4 e = a.ctab[parity][b * kBucketCap + k];
+        const int sj = e.x;
         if (sj == s) continue;
-        const int4 cj = a.scell[sj];
-        if (cj.x != cs.x + dx || cj.y != cs.y + dy || cj.z != cs.z + dz) continue;  // hash alias
+        if (e.y != cs.x + dx || e.z != cs.y + dy || e.w != cs.z + dz) continue;  // hash alias
         const int4 ij = a.sinfo[sj];
         const T4<T> pj = a.spos[sj];
         // a link joins lattice neighbours only: a unit lattice offset in direction d
@@ -350,25 +351,17 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
     const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v];
     const V3<T> pi = abs_position(a, v, qi0);
     const T bsaI = base_size_average<T>(mi.nomSize, (float)qi1.w, mi.alphaCTE);
-    int found[kMaxContacts];
     const int nf = min(a.fcount[s], kMaxContacts);
-    for (int k = 0; k < nf; k++) found[k] = a.flist[(long)s * kMaxContacts + k];
-    // ascending partner index: the order in which the serialized pair loop of
-    // the reference accumulates this voxel's contact force
-    for (int i = 1; i < nf; i++) {
-        const int x = found[i];
-        int j = i - 1;
-        while (j >= 0 && found[j] > x) { found[j + 1] = found[j]; j--; }
-        found[j + 1] = x;
-    }
+    const int *list = a.flist + (long)s * kMaxContacts;  // unordered, each partner once
     if (a.vtarget) {
         // a target touching a non-target: collisionCount (each pair once), and
         // with signals the non-target voxel receives a forced 100 (:711-726)
         const int ti = a.vtarget[v];
         bool hitTarget = false;
         for (int k = 0; k < nf; k++) {
-            if (a.vtarget[a.surf[found[k]]] == ti) continue;
-            if (s < found[k]) atomicAdd(&rt->collisionCount, 1);
+            const int sj = list[k];
+            if (a.vtarget[a.surf[sj]] == ti) continue;
+            if (s < sj) atomicAdd(&rt->collisionCount, 1);
             hitTarget = true;
         }
         if (hitTarget && !ti && a.sig && rp.signals) {
@@ -379,8 +372,20 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
     }
     const V3<T> velI = V3<T>{a.mo0[v].x, a.mo0[v].y, a.mo0[v].z} * (T)mi.massInv;
     V3<T> f = {0, 0, 0};
+    // ascending partner index: the order in which the serialized pair loop of
+    // the reference accumulates this voxel's contact force.  The list is short
+    // (usually empty), so each next partner is found by a scan of the list
+    // instead of sorting a copy in private memory.
+    int prev = -1;
     for (int k = 0; k < nf; k++) {
-        const int vj = a.surf[found[k]];
+        int sj = INT_MAX;
+        for (int i = 0; i < nf; i++) {
+            const int x = list[i];
+            if (x > prev && x < sj) sj = x;
+        }
+        if (sj == INT_MAX) break;  // (entries are distinct; never taken)
+        prev = sj;
+        const int vj = a.surf[sj];
         const VoxelMat &mj = a.vmats[a.vmat[vj]];
         const T4<T> qj0 = a.pq0[vj], qj1 = a.pq1[vj];
         V3<T> diff;
@@ -855,8 +860,8 @@ template <typename T> struct Batch : BatchBase {
         d.contact = nullptr;
         d.tableBuckets = buckets;
         if (anyCollisions) {
-            int *surfd, *c0, *c1, *t0, *t1, *fc, *fl;
-            int4 *scd, *sid;
+            int *surfd, *c0, *c1, *fc, *fl;
+            int4 *scd, *sid, *t0, *t1;
             T4<T> *cw, *spd;
             if ((rc = alloc(&surfd, S)) || (rc = alloc(&scd, S)) || (rc = alloc(&c0, buckets)) ||
                 (rc = alloc(&c1, buckets)) || (rc = alloc(&t0, buckets * kBucketCap)) ||
