@@ -20,6 +20,7 @@ _LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
 
 VXO_CPU_SEMANTICS = 1
 VXO_NO_TEMPERATURE = 2
+VXO_ALL_PAIRS = 4
 
 
 class _Material(C.Structure):
@@ -125,6 +126,8 @@ def lib():
         L.vxo_get_signals.argtypes = [vp, dp]
         L.vxo_collision_count.restype = C.c_long
         L.vxo_collision_count.argtypes = [vp]
+        L.vxo_contact_pairs.restype = C.c_int
+        L.vxo_contact_pairs.argtypes = [vp, ip, C.c_int]
         L.vxo_get_link_state.argtypes = [vp, dp, fp]
         L.vxo_get_temperature.argtypes = [vp, fp, dp]
         L.vxo_set_temperature.argtypes = [vp, fp]
@@ -296,6 +299,16 @@ class OracleModel:
         o = np.zeros(self.nv)
         self._L.vxo_get_signals(self._h, _ptr(o, C.c_double))
         return o
+
+    def contact_pairs(self) -> np.ndarray:
+        """(voxel1, voxel2) of every pair in contact in the last step, in the
+        order the oracle applied them (ascending voxel1 surface index, then
+        ascending voxel2)."""
+        n = self._L.vxo_contact_pairs(self._h, None, 0)
+        out = np.zeros((n, 2), dtype=np.int32)
+        if n:
+            self._L.vxo_contact_pairs(self._h, _ptr(out, C.c_int), n)
+        return out
 
     def link_state(self):
         s = np.zeros((self.nl, 9)); st = np.zeros(self.nl, dtype=np.float32)

@@ -17,6 +17,9 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 
 typedef struct { double x, y, z; } v3;
 typedef struct { double w, x, y, z; } q4;
@@ -101,6 +104,8 @@ struct vxo_model {
     double *sLocal, *sLocalDt, *sValue, *sActive, *sInactive, *sNextPulse;
     double *outValue, *outTime;
     long collisionCount;
+    /* the pairs in contact in the last step's collision phase (voxel1, voxel2) */
+    int *pairs, npairs, pairCap;
     /* links */
     int *lneg, *lpos, *laxis;
     lmat *lm;
@@ -710,7 +715,7 @@ void vxo_free(vxo_model *m) {
                     m->ext_dof, m->ext_t, m->ext_r, m->ext_q, m->ext_f, m->ext_m,
                     m->ff_op[0], m->ff_op[1], m->ff_op[2], m->ff_val[0], m->ff_val[1], m->ff_val[2],
                     m->ciliaBase, m->ciliaShift, m->vremoved, m->lremoved, m->sLocal, m->sLocalDt, m->sValue,
-                    m->sActive, m->sInactive, m->sNextPulse, m->outValue, m->outTime};
+                    m->sActive, m->sInactive, m->sNextPulse, m->outValue, m->outTime, m->pairs};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(m);
 }
@@ -955,24 +960,28 @@ static inline long long cell_key(long long cx, long long cy, long long cz) {
  * voxel2 = surface[second], first > second (gpu_update_attach :833-843). */
 static void signal_receive(vxo_model *m, int v, const vmat *mt, double value, double activeTime, int force);
 
-static void collide_pair(vxo_model *m, int v1, int v2) {
+/* The contact force of the pair (voxel1 = v1, voxel2 = v2) if it is in contact:
+ * the tests of handle_collision_attachment (VX3_VoxelyzeKernel.cu:682-729) in
+ * its order, then VX3_Collision::updateContactForce (VX3_Collision.cu:17-31).
+ * Returns 0 (no effect at all) for pairs the reference skips. */
+static int pair_force(const vxo_model *m, int v1, int v2, v3 *out) {
     const vmat *m1 = &m->vm[m->vmat_of[v1]], *m2 = &m->vm[m->vmat_of[v2]];
-    if (m1->fixed && m2->fixed) return;
+    if (m1->fixed && m2->fixed) return 0;
     v3 diff = vsub(m->pos[v1], m->pos[v2]);
     double bsa1 = base_size_average(m, v1), bsa2 = base_size_average(m, v2);
     double watch = (bsa1 + bsa2) * COLLISION_ENVELOPE_RADIUS;
-    if (diff.x > watch || diff.x < -watch) return;
-    if (diff.y > watch || diff.y < -watch) return;
-    if (diff.z > watch || diff.z < -watch) return;
-    if (vlen(diff) > watch) return;
+    if (diff.x > watch || diff.x < -watch) return 0;
+    if (diff.y > watch || diff.y < -watch) return 0;
+    if (diff.z > watch || diff.z < -watch) return 0;
+    if (vlen(diff) > watch) return 0;
     /* is_neighbor(voxel1, voxel2, NULL, 1) :651-680 */
     for (int d = 0; d < 6; d++) {
         int l = m->vlinks[v1][d];
         if (l < 0) continue;
         int other = m->lneg[l] == v1 ? m->lpos[l] : m->lneg[l];
-        if (other == v2) return;
+        if (other == v2) return 0;
     }
-    if (!m->enableCollision) return;
+    if (!m->enableCollision) return 0;
     float pk1 = (float)(2 * m1->E * m1->nomSize), pk2 = (float)(2 * m2->E * m2->nomSize);
     double penStiff = 2.0f / (1.0f / pk1 + 1.0f / pk2);
     double dampC = 0.5f * (m1->zetaC * m1->c2xSqMxExS + m2->zetaC * m2->c2xSqMxExS);
@@ -987,10 +996,26 @@ static void collide_pair(vxo_model *m, int v1, int v2) {
         double rv = (vel1.x * unit.x + vel1.y * unit.y + vel1.z * unit.z) - (vel2.x * unit.x + vel2.y * unit.y + vel2.z * unit.z);
         force = vscale(unit, penStiff * relDist + dampC * rv);
     }
+    *out = force;
+    return 1;
+}
+
+/* The effects of a pair in contact, in the reference's order: the contact
+ * forces (voxel1 gains it, voxel2 loses it), then for a target touching a
+ * non-target collisionCount and, with signals, a forced 100 received by the
+ * non-target voxel (:711-726). */
+static void apply_pair(vxo_model *m, int v1, int v2, v3 force) {
+    if (m->npairs == m->pairCap) {
+        m->pairCap = m->pairCap ? 2 * m->pairCap : 1024;
+        m->pairs = (int *)realloc(m->pairs, sizeof(int) * 2 * (size_t)m->pairCap);
+        if (!m->pairs) abort();
+    }
+    m->pairs[2 * m->npairs] = v1;
+    m->pairs[2 * m->npairs + 1] = v2;
+    m->npairs++;
     m->contact[v1] = vadd(m->contact[v1], force);
     m->contact[v2] = vsub(m->contact[v2], force);
-    /* a target touching a non-target: collisionCount, and with signals the
-     * non-target voxel receives a forced 100 (:711-726) */
+    const vmat *m1 = &m->vm[m->vmat_of[v1]], *m2 = &m->vm[m->vmat_of[v2]];
     if (m1->isTarget != m2->isTarget) {
         m->collisionCount++;
         if (m->enableSignals) {
@@ -1000,12 +1025,31 @@ static void collide_pair(vxo_model *m, int v1, int v2) {
     }
 }
 
-/* updateAttach all-pairs branch VX3_VoxelyzeKernel.cu:448-458.  The uniform
- * grid only prunes pairs farther apart than any watch distance; the pairs are
- * visited in the same (first, second) order as the nested all-pairs loop. */
+static void collide_pair(vxo_model *m, int v1, int v2) {
+    v3 f;
+    if (pair_force(m, v1, v2, &f)) apply_pair(m, v1, v2, f);
+}
+
+/* updateAttach all-pairs branch VX3_VoxelyzeKernel.cu:448-458, literally: every
+ * surface pair (first > second) in nested-loop order (gpu_update_attach
+ * :833-843 serialised).  O(S^2): the check of the pruned search below. */
+static void update_collisions_all_pairs(vxo_model *m) {
+    m->npairs = 0;
+    for (int first = 1; first < m->ns; first++)
+        for (int second = 0; second < first; second++) collide_pair(m, m->surface[first], m->surface[second]);
+}
+
+/* The same pairs in the same (first, second) order, pruned by a uniform grid
+ * whose cell edge is the largest watch distance (pairs in non-adjacent cells
+ * are farther apart than any watch distance, so the reference skips them
+ * without an effect).  The candidate search and the pair forces run in
+ * parallel over `first`; the effects are applied serially in the all-pairs
+ * order, so the result does not depend on the thread count. */
+typedef struct { int v1, v2; v3 f; } pairhit;
 static void update_collisions(vxo_model *m) {
     int ns = m->ns;
-    if (ns < 2) return;
+    m->npairs = 0;
+    if (ns < 2 || !m->enableCollision) return;  /* pair_force has no effect without EnableCollision */
     double h = 0;
     for (int i = 0; i < ns; i++) {
         double b = base_size_average(m, m->surface[i]);
@@ -1024,34 +1068,66 @@ static void update_collisions(vxo_model *m) {
         ent[i].s = i;
     }
     qsort(ent, ns, sizeof(cellent), cellent_cmp);
-    int cap = 64, cnt;
-    int *cand = (int *)xcalloc(cap, sizeof(int));
-    for (int first = 1; first < ns; first++) {
-        cnt = 0;
-        for (int dx = -1; dx <= 1; dx++)
-            for (int dy = -1; dy <= 1; dy++)
-                for (int dz = -1; dz <= 1; dz++) {
-                    long long key = cell_key(cc[first][0] + dx, cc[first][1] + dy, cc[first][2] + dz);
-                    int lo = 0, hi = ns;
-                    while (lo < hi) {
-                        int mid = (lo + hi) / 2;
-                        if (ent[mid].key < key) lo = mid + 1; else hi = mid;
+    int nt = 1;
+#ifdef _OPENMP
+    nt = omp_get_max_threads();
+#endif
+    pairhit **hits = (pairhit **)xcalloc(nt, sizeof(pairhit *));
+    int *nhit = (int *)xcalloc(nt, sizeof(int));
+#pragma omp parallel num_threads(nt)
+    {
+        int t = 0;
+#ifdef _OPENMP
+        t = omp_get_thread_num();
+#endif
+        /* a contiguous block of `first` per thread, so that the blocks in thread
+         * order are the all-pairs order */
+        long lo_f = 1 + (long)(ns - 1) * t / nt, hi_f = 1 + (long)(ns - 1) * (t + 1) / nt;
+        int cap = 64, hcap = 64, cnt, nh = 0;
+        int *cand = (int *)xcalloc(cap, sizeof(int));
+        pairhit *hb = (pairhit *)xcalloc(hcap, sizeof(pairhit));
+        for (long first = lo_f; first < hi_f; first++) {
+            cnt = 0;
+            for (int dx = -1; dx <= 1; dx++)
+                for (int dy = -1; dy <= 1; dy++)
+                    for (int dz = -1; dz <= 1; dz++) {
+                        long long key = cell_key(cc[first][0] + dx, cc[first][1] + dy, cc[first][2] + dz);
+                        int lo = 0, hi = ns;
+                        while (lo < hi) {
+                            int mid = (lo + hi) / 2;
+                            if (ent[mid].key < key) lo = mid + 1; else hi = mid;
+                        }
+                        for (int k = lo; k < ns && ent[k].key == key; k++) {
+                            if (ent[k].s >= first) continue;
+                            if (cnt == cap) { cap *= 2; cand = (int *)realloc(cand, cap * sizeof(int)); if (!cand) abort(); }
+                            cand[cnt++] = ent[k].s;
+                        }
                     }
-                    for (int k = lo; k < ns && ent[k].key == key; k++) {
-                        if (ent[k].s >= first) continue;
-                        if (cnt == cap) { cap *= 2; cand = (int *)realloc(cand, cap * sizeof(int)); if (!cand) abort(); }
-                        cand[cnt++] = ent[k].s;
-                    }
-                }
-        /* ascending second, as in the all-pairs order */
-        for (int a = 1; a < cnt; a++) {
-            int x = cand[a], b = a - 1;
-            while (b >= 0 && cand[b] > x) { cand[b + 1] = cand[b]; b--; }
-            cand[b + 1] = x;
+            /* ascending second, as in the all-pairs order */
+            for (int a = 1; a < cnt; a++) {
+                int x = cand[a], b = a - 1;
+                while (b >= 0 && cand[b] > x) { cand[b + 1] = cand[b]; b--; }
+                cand[b + 1] = x;
+            }
+            for (int a = 0; a < cnt; a++) {
+                v3 f;
+                int v1 = m->surface[first], v2 = m->surface[cand[a]];
+                if (!pair_force(m, v1, v2, &f)) continue;
+                if (nh == hcap) { hcap *= 2; hb = (pairhit *)realloc(hb, hcap * sizeof(pairhit)); if (!hb) abort(); }
+                hb[nh].v1 = v1; hb[nh].v2 = v2; hb[nh].f = f;
+                nh++;
+            }
         }
-        for (int a = 0; a < cnt; a++) collide_pair(m, m->surface[first], m->surface[cand[a]]);
+        free(cand);
+        hits[t] = hb;
+        nhit[t] = nh;
     }
-    free(cand);
+    for (int t = 0; t < nt; t++) {
+        for (int k = 0; k < nhit[t]; k++) apply_pair(m, hits[t][k].v1, hits[t][k].v2, hits[t][k].f);
+        free(hits[t]);
+    }
+    free(hits);
+    free(nhit);
     free(cc);
     free(ent);
 }
@@ -1326,7 +1402,8 @@ int vxo_step(vxo_model *m, float dt, int nsteps, int flags) {
         for (int l = 0; l < m->nl; l++)
             if (m->strain[l] > 100) { diverged = 1; break; }
         if (diverged) return s;
-        update_collisions(m);
+        if (flags & VXO_ALL_PAIRS) update_collisions_all_pairs(m);
+        else update_collisions(m);
         update_voxels(m, d, flags & VXO_CPU_SEMANTICS);
         /* the end of the step: detached links no longer carry signals (the
          * reference detaches before the voxel update), removal comes last */
@@ -1386,6 +1463,15 @@ void vxo_get_removed(const vxo_model *m, int *voxel_step, int *link_step) {
 }
 
 long vxo_collision_count(const vxo_model *m) { return m->collisionCount; }
+
+int vxo_contact_pairs(const vxo_model *m, int *v1v2, int cap) {
+    if (v1v2)
+        for (int i = 0; i < m->npairs && i < cap; i++) {
+            v1v2[2 * i] = m->pairs[2 * i];
+            v1v2[2 * i + 1] = m->pairs[2 * i + 1];
+        }
+    return m->npairs;
+}
 
 void vxo_get_signals(const vxo_model *m, double *local) {
     memcpy(local, m->sLocal, sizeof(double) * m->nv);

@@ -99,6 +99,7 @@ typedef struct vxo_model vxo_model;
 /* flags for vxo_step */
 #define VXO_CPU_SEMANTICS 1  /* CVoxelyze (src/old) integrator: no angMom reset under static friction */
 #define VXO_NO_TEMPERATURE 2 /* skip the temperature update (caller drives temperatures) */
+#define VXO_ALL_PAIRS 4      /* collisions by the literal all-pairs loop (VX3_VoxelyzeKernel.cu:448-458), O(S^2) */
 
 vxo_model *vxo_build(const vxo_spec *spec);
 void vxo_free(vxo_model *m);
@@ -154,6 +155,10 @@ void vxo_get_signals(const vxo_model *m, double *local);
 /* collisionCount: contacts between a target and a non-target voxel, summed over
  * the steps (VX3_VoxelyzeKernel.cu:711-726) */
 long vxo_collision_count(const vxo_model *m);
+/* the pairs in contact in the last step's collision phase, in the order their
+ * effects were applied: (voxel1, voxel2) per pair, at most cap pairs written;
+ * returns the number of pairs */
+int vxo_contact_pairs(const vxo_model *m, int *v1v2, int cap);
 void vxo_get_link_state(const vxo_model *m, double *state9, float *strain);
 void vxo_get_temperature(const vxo_model *m, float *temp, double *rest_length);
 void vxo_set_temperature(vxo_model *m, const float *temp);

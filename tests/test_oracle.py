@@ -276,6 +276,84 @@ def test_cpu_semantics_flag_only_changes_static_friction_rotation():
     assert np.max(np.abs(a.state()["pos"] - b.state()["pos"])) <= 1e-12
 
 
+def test_static_friction_zeroes_angular_momentum():
+    """VX3_Voxel.cu:259-264: a voxel held by static floor friction ends every step
+    with zero angular momentum.  A 3x3x1 slab resting on the floor, a twisting and
+    tipping moment on its centre voxel: under VX3 semantics every voxel with the
+    static-friction flag and a non-negative floor penetration has angMom exactly
+    0 after each step, so the twisted voxel turns only by the one step of
+    rotation each step's moment gives it; under CVoxelyze semantics (no reset)
+    the same voxel carries angular momentum and turns faster until the links
+    balance the moment."""
+    p = quiet((3, 3, 1), grav_acc=-9.81, floor_enabled=1, bond_damp_z=0.1, slow_damp_z=1.0,
+              materials=[dict(E=1e6, rho=1000.0, cte=0.0, mu_s=2.0, mu_k=0.5)])
+    vx3, cpu = O.OracleModel(p), O.OracleModel(p)
+    for m in (vx3, cpu):
+        m.set_external(4, dof=0, moment=(2e-6, 0.0, 5e-6))
+    held = 0
+    for k in range(200):
+        vx3.step(1)
+        cpu.step(1, flags=O.VXO_CPU_SEMANTICS)
+        s, c = vx3.state(), cpu.state()
+        vf, _ = vx3.flags()
+        static = ((vf & 8) != 0) & (s["pos"][:, 2] <= 0.0)  # FLOOR_STATIC_FRICTION, floorPenetration >= 0
+        held += int(static.sum())
+        assert np.all(s["angmom"][static] == 0.0), k
+        if k < 10:  # spinning up: CVoxelyze keeps the angular momentum, VX3 drops it
+            assert static[4] and np.linalg.norm(c["angmom"][4]) > 1e-11, k
+            assert k == 0 or c["quat"][4][3] > s["quat"][4][3] * 1.05, k
+    assert held >= 9 * 150  # the slab rests in static friction for most of the run
+
+
+def test_contacts_grid_equals_all_pairs():
+    """The oracle's grid-pruned contact search against the literal all-pairs loop
+    of updateAttach (VX3_VoxelyzeKernel.cu:448-458): a 10x10x4 block squeezed to
+    0.8 of its spacing (every lattice diagonal inside the contact envelope) and
+    jostled by 0.06 voxel, so that pairs straddle the grid cells in every
+    direction.  Same pairs in the same order, bit-identical states."""
+    p = quiet((10, 10, 4), enable_collision=1, col_damp_z=0.8, bond_damp_z=0.1)
+    a, b = O.OracleModel(p), O.OracleModel(p)
+    rng = np.random.default_rng(7)
+    pos = a.state()["pos"]
+    pos = 0.8 * (pos - pos.mean(axis=0)) + rng.normal(scale=0.0006, size=pos.shape)
+    lm = rng.normal(scale=1e-6, size=pos.shape)
+    for m in (a, b):
+        m.set_state(pos=pos, linmom=lm)
+    h = 1.25 * 0.01 * 1.0001  # the search's cell edge (largest watch distance)
+    crossing = total = 0
+    for k in range(20):
+        a.step(1)
+        b.step(1, flags=O.VXO_ALL_PAIRS)
+        pa, pb = a.contact_pairs(), b.contact_pairs()
+        assert len(pa) > 0 and np.array_equal(pa, pb), k
+        total += len(pa)
+        cells = np.floor(a.state()["pos"] / h).astype(np.int64)
+        crossing += int(np.any(cells[pa[:, 0]] != cells[pa[:, 1]], axis=1).sum())
+        sa, sb = a.state(), b.state()
+        for key in ("pos", "quat", "linmom", "angmom"):
+            assert np.array_equal(sa[key], sb[key]), (k, key)
+    assert total > 5000 and crossing > 4000
+
+
+def test_divergence_stops_the_step():
+    """doTimeStep returns false once a link's axial strain exceeds 100
+    (VX3_VoxelyzeKernel.cu:271-280): a bar driven at 40x its stable timestep
+    blows up and the oracle stops advancing it; at the stable timestep it runs."""
+    p = quiet((3, 1, 1))
+    m = O.OracleModel(p)
+    ps = m.state()["pos"]
+    ps[2, 0] += 1e-4
+    m.set_state(pos=ps)
+    dt = m.step_dt()
+    done = m.step(2000, 40 * dt)
+    assert done < 2000
+    t = m.time
+    assert m.step(10, 40 * dt) == 0 and m.time == t  # stays stopped
+    ok = O.OracleModel(p)
+    ok.set_state(pos=ps)
+    assert ok.step(2000, dt) == 2000
+
+
 def test_force_field_constant_force_and_operand_order():
     """A force field F_x = 3 - 1 (mtSUB: left minus right) on a free body with
     no gravity, floor or damping: every voxel follows the exact discrete

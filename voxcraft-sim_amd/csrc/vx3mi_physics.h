@@ -582,6 +582,11 @@ VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T>
         smallAngle = false;
         flags &= ~LF_LOCAL_VELOCITY_VALID;
     }
+#ifdef VX3MI_TEST_SMALL_ANGLE
+    // host test builds only: replaces the decision by a given one (the CPU
+    // attribution of FP32-vs-FP64 differences to this branch)
+    VX3MI_TEST_SMALL_ANGLE(smallAngle, flags);
+#endif
     if (smallAngle) {
         if constexpr (std::is_same<T, double>::value) pos2.x -= restLen;
         else pos2.x = ext;
