@@ -196,6 +196,27 @@ constexpr int VF_FLOOR_STATIC_FRICTION = 1 << 3;
 constexpr int LF_LOCAL_VELOCITY_VALID = 1;
 constexpr int LF_SMALL_ANGLE = 2;
 
+// ---- discrete decisions of the piecewise equations ------------------------------
+// Each branch point of the reference's equations where a continuous quantity is
+// compared with a threshold.  A host test build may define
+// VX3MI_TEST_DECISION(kind, b) before including this header to observe or
+// replace the decision b (a bool lvalue) -- the attribution of FP32-vs-FP64
+// differences to branches; otherwise it expands to nothing.
+enum DecisionKind {
+    DEC_SMALL_ANGLE = 0,  // link small-angle formulation with hysteresis (VX3_Link.cu:117-131)
+    DEC_ROTVEC_SERIES,    // ToRotationVector square-root form below 2.4e-3 (VX3_Quat3D.h:347-359)
+    DEC_POSX_LINEAR,      // FromAngleToPosX first-order form below 1.732e-2 (VX3_Quat3D.h:384-400)
+    DEC_FLOOR,            // floor contact, floorPenetration() >= 0 (VX3_Voxel.cu:188, 399-426)
+    DEC_STATIC_BREAK,     // static friction broken, |F_xy|^2 > (mu_s N)^2 (VX3_Voxel.cu:409-415)
+    DEC_STATIC_MAKE,      // static friction entered, hKE + work <= 0 (VX3_Voxel.cu:202-209)
+    DEC_FLOOR_AFTER,      // floor contact after the move, the angMom reset (VX3_Voxel.cu:259-264)
+    DEC_CONTACT,          // collision pair in contact, relDist > 0 (VX3_Collision.cu:17-31)
+    DEC_KINDS
+};
+#ifndef VX3MI_TEST_DECISION
+#define VX3MI_TEST_DECISION(kind, b) ((void)0)
+#endif
+
 // ---- quaternion algebra (VX3_Quat3D.h:217-222, 439-447, 465-474) -----------
 template <typename T> VX_HD Q4<T> qmul(Q4<T> a, Q4<T> f) {
     Q4<T> r;
@@ -323,7 +344,9 @@ template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q) {
     if constexpr (std::is_same<T, double>::value) {
         if (q.w >= 1.0 || q.w <= -1.0) return v3<T>(0, 0, 0);
         double sl = 1.0 - q.w * q.w;
-        if (sl < kSLTHRESH_ACOS2SQRT) {
+        bool series = sl < kSLTHRESH_ACOS2SQRT;
+        VX3MI_TEST_DECISION(DEC_ROTVEC_SERIES, series);
+        if (series) {
             double s = sqrt((2 - 2 * q.w) / sl);
             return v3<T>(q.x * 2.0 * s, q.y * 2.0 * s, q.z * 2.0 * s);
         }
@@ -333,8 +356,10 @@ template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q) {
         float s2 = q.x * q.x + q.y * q.y + q.z * q.z;  // == 1 - w^2 for a unit quaternion
         if (s2 == 0.0f || q.w <= -1.0f) return v3<T>(0, 0, 0);
         float k;
+        bool series = s2 < (float)kSLTHRESH_ACOS2SQRT;
+        VX3MI_TEST_DECISION(DEC_ROTVEC_SERIES, series);
         // 2 sqrt((2 - 2w) / (1 - w^2)) = 2 sqrt(2) / sqrt(1 + w)
-        if (s2 < (float)kSLTHRESH_ACOS2SQRT) k = 2.8284271247461903f * frsqrt(1.0f + q.w);
+        if (series) k = 2.8284271247461903f * frsqrt(1.0f + q.w);
         else {
             float s = fsqrt(s2);
             k = 2.0f * atan2f(s, q.w) * frcp(s);
@@ -389,7 +414,9 @@ template <typename T> VX_HD Q4<T> from_angle_to_posx(V3<T> rf) {
         zx = rf.z / rf.x;
     }
     const T sa = (T)kSMALL_ANGLE_RAD;
-    if (yx < sa && yx > -sa && zx < sa && zx > -sa) {
+    bool linear = yx < sa && yx > -sa && zx < sa && zx > -sa;
+    VX3MI_TEST_DECISION(DEC_POSX_LINEAR, linear);
+    if (linear) {
         q.x = 0;
         q.y = (T)0.5 * zx;
         q.z = (T)-0.5 * yx;
@@ -573,20 +600,15 @@ VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T>
     float extendPerc;
     if constexpr (std::is_same<T, double>::value) extendPerc = (float)(mabs((T)1 - pos2.x / restLen));
     else extendPerc = mabs(ext * frcp(restLen));
-    bool smallAngle = (flags & LF_SMALL_ANGLE) != 0;
-    if (!smallAngle && smallTurn < kSA_BOND_BEND_RAD && extendPerc < kSA_BOND_EXT_PERC) {
-        smallAngle = true;
-        flags &= ~LF_LOCAL_VELOCITY_VALID;
-    } else if (smallAngle && (smallTurn > kHYSTERESIS_FACTOR * kSA_BOND_BEND_RAD ||
-                              extendPerc > kHYSTERESIS_FACTOR * kSA_BOND_EXT_PERC)) {
+    const bool wasSmall = (flags & LF_SMALL_ANGLE) != 0;
+    bool smallAngle = wasSmall;
+    if (!smallAngle && smallTurn < kSA_BOND_BEND_RAD && extendPerc < kSA_BOND_EXT_PERC) smallAngle = true;
+    else if (smallAngle && (smallTurn > kHYSTERESIS_FACTOR * kSA_BOND_BEND_RAD ||
+                            extendPerc > kHYSTERESIS_FACTOR * kSA_BOND_EXT_PERC))
         smallAngle = false;
-        flags &= ~LF_LOCAL_VELOCITY_VALID;
-    }
-#ifdef VX3MI_TEST_SMALL_ANGLE
-    // host test builds only: replaces the decision by a given one (the CPU
-    // attribution of FP32-vs-FP64 differences to this branch)
-    VX3MI_TEST_SMALL_ANGLE(smallAngle, flags);
-#endif
+    VX3MI_TEST_DECISION(DEC_SMALL_ANGLE, smallAngle);
+    // a change of formulation invalidates the local velocity (:117-131)
+    if (smallAngle != wasSmall) flags &= ~LF_LOCAL_VELOCITY_VALID;
     if (smallAngle) {
         if constexpr (std::is_same<T, double>::value) pos2.x -= restLen;
         else pos2.x = ext;
@@ -769,42 +791,46 @@ VX_HD void voxel_step(const VoxelMat &vm, const VoxelExt *exts, float dtf, int l
     tf.x += field.x; tf.y += field.y; tf.z += field.z;
 
     V3<T> cur = tf;
-    if (flags & VF_FLOOR_ENABLED) {
-        float pen = floor_penetration<T>(vm, temp, iz, s.u.z);
-        if (pen >= 0) {
-            float normalForce = vm.penK * pen;
-            cur.z += normalForce - vm.cDampT * vel.z;
-            if (flags & VF_FLOOR_STATIC_FRICTION) {
-                float sfs = (float)(cur.x * cur.x + cur.y * cur.y);
-                float ffs = (vm.muS * normalForce) * (vm.muS * normalForce);
-                if (sfs > ffs) flags &= ~VF_FLOOR_STATIC_FRICTION;
-            } else {
-                V3<T> hv = v3<T>(vel.x, vel.y, 0);
-                T l = len(hv);
-                V3<T> hn = hv;
-                if (l > 0) {
-                    if constexpr (std::is_same<T, float>::value) hn = hv * frcp(l);
-                    else hn = v3<T>(hv.x / l, hv.y / l, hv.z / l);
-                }
-                cur = cur - hn * (T)(vm.muK * normalForce);
-            }
+    // floorForce (:399-426) and the friction test of timeStep (:198-209) both
+    // act while the voxel touches the floor at its position entering the step
+    const float pen = floor_penetration<T>(vm, temp, iz, s.u.z);
+    bool onFloor = (flags & VF_FLOOR_ENABLED) && pen >= 0;
+    VX3MI_TEST_DECISION(DEC_FLOOR, onFloor);
+    if (onFloor) {
+        float normalForce = vm.penK * pen;
+        cur.z += normalForce - vm.cDampT * vel.z;
+        if (flags & VF_FLOOR_STATIC_FRICTION) {
+            float sfs = (float)(cur.x * cur.x + cur.y * cur.y);
+            float ffs = (vm.muS * normalForce) * (vm.muS * normalForce);
+            bool slip = sfs > ffs;
+            VX3MI_TEST_DECISION(DEC_STATIC_BREAK, slip);
+            if (slip) flags &= ~VF_FLOOR_STATIC_FRICTION;
         } else {
-            flags &= ~VF_FLOOR_STATIC_FRICTION;
+            V3<T> hv = v3<T>(vel.x, vel.y, 0);
+            T l = len(hv);
+            V3<T> hn = hv;
+            if (l > 0) {
+                if constexpr (std::is_same<T, float>::value) hn = hv * frcp(l);
+                else hn = v3<T>(hv.x / l, hv.y / l, hv.z / l);
+            }
+            cur = cur - hn * (T)(vm.muK * normalForce);
         }
+    } else {
+        flags &= ~VF_FLOOR_STATIC_FRICTION;
     }
     V3<T> fric = cur - tf;
     V3<T> p = s.p + cur * dt;
     V3<T> tr = p * (dt * (T)vm.massInv);
-    if ((flags & VF_FLOOR_ENABLED) && floor_penetration<T>(vm, temp, iz, s.u.z) >= 0) {
+    if (onFloor) {
         T work = fric.x * tr.x + fric.y * tr.y;
         T hKe = (T)0.5 * (T)vm.massInv * (p.x * p.x + p.y * p.y);
-        if (hKe + work <= 0) flags |= VF_FLOOR_STATIC_FRICTION;
+        bool stick = hKe + work <= 0;
+        VX3MI_TEST_DECISION(DEC_STATIC_MAKE, stick);
+        if (stick) flags |= VF_FLOOR_STATIC_FRICTION;
         if (flags & VF_FLOOR_STATIC_FRICTION) {
             p.x = p.y = 0;
             tr.x = tr.y = 0;
         }
-    } else {
-        flags &= ~VF_FLOOR_STATIC_FRICTION;
     }
     s.u = s.u + tr;
     s.p = p;
@@ -841,9 +867,9 @@ VX_HD void voxel_step(const VoxelMat &vm, const VoxelExt *exts, float dtf, int l
         s.q.w *= n; s.q.x *= n; s.q.y *= n; s.q.z *= n;
     }
     // VX3-only: no rotation while held by static floor friction (VX3_Voxel.cu:259-264)
-    if ((flags & VF_FLOOR_ENABLED) && floor_penetration<T>(vm, temp, iz, s.u.z) >= 0) {
-        if (flags & VF_FLOOR_STATIC_FRICTION) h = v3<T>(0, 0, 0);
-    }
+    bool onFloorAfter = (flags & VF_FLOOR_ENABLED) && floor_penetration<T>(vm, temp, iz, s.u.z) >= 0;
+    VX3MI_TEST_DECISION(DEC_FLOOR_AFTER, onFloorAfter);
+    if (onFloorAfter && (flags & VF_FLOOR_STATIC_FRICTION)) h = v3<T>(0, 0, 0);
     s.h = h;
     s.flags = flags;
 }
@@ -887,7 +913,9 @@ VX_HD V3<T> contact_force(V3<T> diff, T bsaI, T bsaJ, V3<T> velI, V3<T> velJ, fl
     const T nomDist = (bsaI + bsaJ) * (T)kCOLLISION_ENVELOPE_RADIUS;
     const T l = len(offset);
     const T relDist = nomDist - l;
-    if (!(relDist > 0)) return v3<T>(0, 0, 0);
+    bool touch = relDist > 0;
+    VX3MI_TEST_DECISION(DEC_CONTACT, touch);
+    if (!touch) return v3<T>(0, 0, 0);
     const V3<T> unit = l > 0 ? v3<T>(offset.x / l, offset.y / l, offset.z / l) : offset;
     const T rv = (velI.x * unit.x + velI.y * unit.y + velI.z * unit.z) - (velJ.x * unit.x + velJ.y * unit.y + velJ.z * unit.z);
     return unit * (penStiff * relDist + dampC * rv);
