@@ -234,7 +234,6 @@ def parity_report(c1_steps: int = 20000, c1_window: int = 1000, every: int = 50)
                               "most_voxels_over_1e-5_at_a_sample": over, "voxels": int(len(dp)),
                               f"com_after_{c1_steps}": com_rel(b32.com()[0], c32, b64.com()[0], c64),
                               "diverged": [int(b32.diverged()[0]), int(b64.diverged()[0])],
-                              "collision_overflow": [int(b32.collision_overflow()[0]), int(b64.collision_overflow()[0])],
                               "voxel_steps_per_s": {"fp32": nv1 * rest / (ms32 * 1e-3), "fp64": nv1 * rest / (ms64 * 1e-3),
                                                     "mode": "flat (hipGraph chunks), collisions on",
                                                     "steps": rest}}
@@ -271,7 +270,7 @@ def config1_only(args) -> int:
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms / args.steps,
            "higher_is_better": True, "dtype": "fp32", "data": "synthetic",
            "config": {"workload": "one 64x64x16 robot, collisions on, flat mode (hipGraph chunks)", "voxels": nv},
-           "diverged": int(b.diverged()[0]), "collision_overflow": int(b.collision_overflow()[0])}
+           "diverged": int(b.diverged()[0])}
     b.close()
     print(json.dumps(out), flush=True)
     return 0

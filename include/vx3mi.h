@@ -204,8 +204,10 @@ int vx3mi_batch_recommended_dt(vx3mi_batch *b, double *out);
 
 /* doTimeStep (VX3_VoxelyzeKernel.cu:237-359) nsteps times for every robot.
  * dt < 0: each robot uses (float)(DtFrac * recommendedTimeStep), as
- * doTimeStep(-1).  dt >= 0: every robot uses dt.  Returns VX3MI_OK,
- * VX3MI_DIVERGED (some robot stopped; it is not advanced further) or an
+ * doTimeStep(-1).  dt >= 0: every robot uses dt (dt = 0 only counts the
+ * steps, as doTimeStep(0) does).  A robot whose link strain passes 100 stops
+ * (doTimeStep returns false, VX3_VoxelyzeKernel.cu:271-280) and is not
+ * advanced further; vx3mi_batch_diverged reports it.  Returns VX3MI_OK or an
  * error.  Asynchronous on the batch stream except for the return code of
  * launch errors; vx3mi_batch_sync waits. */
 int vx3mi_step(vx3mi_batch *b, int nsteps, float dt, int mode);
@@ -245,12 +247,24 @@ int vx3mi_get_link_flags(vx3mi_batch *b, int robot, int *flags, float *strain);
 /* per-robot simulated time (currentTime), diverged flag, step count */
 int vx3mi_batch_time(vx3mi_batch *b, double *out);
 int vx3mi_batch_diverged(vx3mi_batch *b, int *out);
-/* per robot: non-zero when the collision grid dropped a contact since creation
- * (bit 0: a grid bucket held more than 32 surface voxels; bit 1: a voxel had
- * more than 64 contacts in one step); results then differ from the all-pairs
- * reference and are no longer deterministic.  Returns 1 when any robot
- * overflowed, 0 otherwise, negative on error. */
-int vx3mi_batch_collision_overflow(vx3mi_batch *b, int *out);
+/* Contact pairs, a diagnostic of the collision phase (the pairs the
+ * reference's all-pairs loop finds in contact, VX3_VoxelyzeKernel.cu:448-458,
+ * 682-729): with cap > 0 every later vx3mi_step call records the pairs of
+ * surface voxels inside each other's contact envelope in its steps (at most
+ * cap pairs per call); cap 0 stops recording.  Error when no robot has
+ * collisions. */
+int vx3mi_batch_record_contacts(vx3mi_batch *b, int cap);
+/* per robot: the steps whose collision phase rebuilt the robot's candidate
+ * lists (every surface voxel lists the surface voxels within its largest
+ * contact envelope plus a skin of a quarter voxel; they are rebuilt, through a
+ * counting-sort grid, once some surface voxel has moved more than half the
+ * skin).  out: n ints. */
+int vx3mi_batch_listings(vx3mi_batch *b, int *out);
+/* The pairs recorded by the last vx3mi_step call for one robot (robot < 0:
+ * all, batch voxel indices): voxel index pairs, the smaller first, sorted.
+ * Returns the number of pairs (at most cap are written) or VX3MI_E_LIMIT when
+ * the call found more pairs than the record holds. */
+int vx3mi_batch_contact_pairs(vx3mi_batch *b, int robot, int *pairs2, int cap);
 /* updateCurrentCenterOfMass (VX3_VoxelyzeKernel.cu:477-493) per robot: out 3n */
 int vx3mi_center_of_mass(vx3mi_batch *b, double *out);
 

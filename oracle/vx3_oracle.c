@@ -45,6 +45,13 @@ static const float SA_BOND_EXT_PERC = 0.50f;
 /* link flag bits: src/old/types.h:6-8 (LOCAL_VELOCITY_VALID) + smallAngle */
 #define LF_LOCAL_VELOCITY_VALID 1
 #define LF_SMALL_ANGLE 2
+/* branches the last update took (recorded for comparisons, read by nothing):
+ * ToRotationVector of angle1 / angle2 by acos (above the 2.4e-3 threshold),
+ * FromAngleToPosX by its exact form (above 1.732e-2) */
+#define LF_ROTVEC_ACOS1 4
+#define LF_ROTVEC_ACOS2 8
+#define LF_POSX_EXACT 16
+#define LF_DECISIONS (LF_ROTVEC_ACOS1 | LF_ROTVEC_ACOS2 | LF_POSX_EXACT)
 
 typedef struct {
     double nomSize;
@@ -162,13 +169,15 @@ static inline v3 qrotinv(q4 q, v3 f) {
                tw * q.z + tx * q.y - ty * q.x + tz * q.w);
 }
 /* VX3_Quat3D.h:347-359 */
-static inline v3 q_to_rotvec(q4 q) {
+static inline v3 q_to_rotvec(q4 q, int *acos_branch) {
+    if (acos_branch) *acos_branch = 0;
     if (q.w >= 1.0 || q.w <= -1.0) return v3_(0, 0, 0);
     double sl = 1.0 - q.w * q.w;
     if (sl < Q_SLTHRESH_ACOS2SQRT) {
         double s = sqrt((2 - 2 * q.w) / sl);
         return v3_(q.x * 2.0 * s, q.y * 2.0 * s, q.z * 2.0 * s);
     }
+    if (acos_branch) *acos_branch = 1;
     double a = vxm_acos(q.w), d = sqrt(sl);
     return v3_(q.x * 2.0 * a / d, q.y * 2.0 * a / d, q.z * 2.0 * a / d);
 }
@@ -191,8 +200,9 @@ static inline q4 q_from_rotvec(v3 in) {
     return r;
 }
 /* VX3_Quat3D.h:384-432; q is the identity on entry (VX3_Link.cu:103,120) */
-static inline q4 q_from_angle_to_posx(v3 rf) {
+static inline q4 q_from_angle_to_posx(v3 rf, int *exact) {
     q4 q = {1, 0, 0, 0};
+    if (exact) *exact = 0;
     if (rf.x == 0 && rf.y == 0 && rf.z == 0) return q;
     double yx = rf.y / rf.x, zx = rf.z / rf.x;
     if (yx < Q_SMALL_ANGLE_RAD && yx > -Q_SMALL_ANGLE_RAD && zx < Q_SMALL_ANGLE_RAD && zx > -Q_SMALL_ANGLE_RAD) {
@@ -202,6 +212,7 @@ static inline q4 q_from_angle_to_posx(v3 rf) {
         q.w = 1 + 0.5 * (-q.y * q.y - q.z * q.z);
         return q;
     }
+    if (exact) *exact = 1;
     v3 n = rf;
     double l = sqrt(n.x * n.x + n.y * n.y + n.z * n.z);
     if (l > 0) {
@@ -853,17 +864,19 @@ static void link_update(vxo_model *m, int l) {
         smallAngle = 0;
         flags &= ~LF_LOCAL_VELOCITY_VALID;
     }
+    int exact = 0, acos1, acos2;
     if (smallAngle) {
         pos2.x -= restLen;
     } else {
-        angle1 = q_from_angle_to_posx(pos2);
+        angle1 = q_from_angle_to_posx(pos2, &exact);
         totalRot = qmul(angle1, totalRot);
         angle2 = qmul(angle1, angle2);
         pos2 = v3_(vlen(pos2) - restLen, 0, 0);
     }
-    v3 a1v = q_to_rotvec(angle1);
-    v3 a2v = q_to_rotvec(angle2);
-    flags = smallAngle ? (flags | LF_SMALL_ANGLE) : (flags & ~LF_SMALL_ANGLE);
+    v3 a1v = q_to_rotvec(angle1, &acos1);
+    v3 a2v = q_to_rotvec(angle2, &acos2);
+    flags = (flags & ~(LF_SMALL_ANGLE | LF_DECISIONS)) | (smallAngle ? LF_SMALL_ANGLE : 0) |
+            (acos1 ? LF_ROTVEC_ACOS1 : 0) | (acos2 ? LF_ROTVEC_ACOS2 : 0) | (exact ? LF_POSX_EXACT : 0);
     m->pos2[l] = pos2;
     m->a1v[l] = a1v;
     m->a2v[l] = a2v;
@@ -1309,7 +1322,7 @@ static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
                 orient = m->ext_q[v];
                 angMom = v3_(0, 0, 0);
             } else {
-                v3 rv = q_to_rotvec(orient);
+                v3 rv = q_to_rotvec(orient, NULL);
                 if (ed & 8) { rv.x = 0; angMom.x = 0; }
                 if (ed & 16) { rv.y = 0; angMom.y = 0; }
                 if (ed & 32) { rv.z = 0; angMom.z = 0; }
@@ -1692,11 +1705,11 @@ static void diff_link(vxo_diff *d, int l) {
     if (smallAngle) {
         pos2.x -= restLen;
     } else {
-        angle1 = q_from_angle_to_posx(pos2);
+        angle1 = q_from_angle_to_posx(pos2, NULL);
         angle2 = qmul(angle1, angle2);
         pos2 = v3_(vlen(pos2) - restLen, 0, 0);
     }
-    v3 a1v = q_to_rotvec(angle1), a2v = q_to_rotvec(angle2);
+    v3 a1v = q_to_rotvec(angle1, NULL), a2v = q_to_rotvec(angle2, NULL);
     flags = smallAngle ? (flags | LF_SMALL_ANGLE) : (flags & ~LF_SMALL_ANGLE);
     d->pos2[l] = pos2;
     d->a1v[l] = a1v;

@@ -25,6 +25,7 @@ sys.path.insert(0, os.path.join(HERE, "golden"))
 import make_diff_fd as FD  # noqa: E402
 
 FIX = np.load(os.path.join(HERE, "golden", "diff_fd_4x4x4.npz"))
+FIX_C4 = np.load(os.path.join(HERE, "golden", "diff_fd_c4_robot0.npz"))
 
 
 def log(msg):
@@ -173,3 +174,61 @@ def test_torch_function_gradients():
     rel_psi = np.linalg.norm(psi_t.grad.numpy() - FIX["grad_psi"]) / np.linalg.norm(FIX["grad_psi"])
     rel_l = np.linalg.norm(l_t.grad.numpy() - FIX["grad_log10E"]) / np.linalg.norm(FIX["grad_log10E"])
     assert rel_psi < 1e-4 and rel_l < 1e-4
+
+
+# ---- configuration 4: the batched adjoint workload ---------------------------------
+@pytest.mark.parametrize("precision", [vx3mi.FP64, vx3mi.FP32])
+def test_config4_robot_subset_matches_finite_differences(precision):
+    """The 4x4x4 part of configuration 4's robot 0 (52 voxels of its largest
+    component there, the robot's own phases and moduli), 200 steps: adjoint
+    gradients against the FP64 central differences of the oracle."""
+    prob, psi, log10E = FD.c4_subset_problem()
+    assert np.array_equal(np.asarray(prob["cells"]), FIX_C4["cells"])
+    m, d = make(prob, precision, dt=float(FIX_C4["dt"]))
+    E = 10.0 ** log10E
+    d.set_params(psi / (2 * math.pi), E)
+    d.forward(int(FIX_C4["nsteps"]))
+    gpos = np.zeros((m.nv, 3))
+    gpos[:, 0] = -1.0 / m.nv  # one material: equal masses
+    gphase, gE = d.backward(grad_pos=gpos)
+    g_psi, g_l = gphase / (2 * math.pi), gE * E * math.log(10.0)
+    rel_psi = np.linalg.norm(g_psi - FIX_C4["grad_psi"]) / np.linalg.norm(FIX_C4["grad_psi"])
+    rel_l = np.linalg.norm(g_l - FIX_C4["grad_log10E"]) / np.linalg.norm(FIX_C4["grad_log10E"])
+    log(f"config4 subset fp{precision}: rel err phase {rel_psi:.3e} log10E {rel_l:.3e}")
+    assert rel_psi < 1e-4 and rel_l < 1e-4
+
+
+def _com_x_grad(nv):
+    g = np.zeros((nv, 3))
+    g[:, 0] = -1.0 / nv
+    return g
+
+
+@pytest.mark.parametrize("nrobots,steps", [(16, 400), (256, 150)])
+def test_config4_batched_equals_single(nrobots, steps):
+    """Configuration 4 (dims 6..16, per-voxel phase and modulus): the first 16
+    robots over 400 steps with checkpoints every 64 and recompute, and the full
+    256-robot batch over 150 steps; every robot's final state and gradients of
+    its own centre-of-mass loss equal a run of that robot alone, bit for bit
+    (16 robots: all of them; 256: four spread over the batch)."""
+    probs = configs.config4()[:nrobots]
+    models = [vx3mi.Model.from_problem(p) for p in probs]
+    params = [(np.asarray(p["phase"]), np.asarray(p["voxel_E"], np.float64)) for p in probs]
+    K = 64
+    both = vx3mi.Diff(models, vx3mi.FP32, dt=float(FIX["dt"]), checkpoint_every=K)
+    both.set_params(np.concatenate([a for a, _ in params]), np.concatenate([b for _, b in params]))
+    both.forward(steps)
+    off = both.voxel_offsets
+    g = np.concatenate([_com_x_grad(m.nv) for m in models])
+    gb = both.backward(grad_pos=g)
+    log(f"config4 {nrobots} robots, {both.nv} voxels: info {both.info()}")
+    check = range(nrobots) if nrobots <= 16 else (0, 85, 170, nrobots - 1)
+    for r in check:
+        one = vx3mi.Diff(models[r], vx3mi.FP32, dt=float(FIX["dt"]), checkpoint_every=K)
+        one.set_params(*params[r])
+        one.forward(steps)
+        assert np.array_equal(one.state()["pos"], both.state(r)["pos"]), r
+        go = one.backward(grad_pos=g[off[r]:off[r + 1]])
+        assert np.array_equal(go[0], gb[0][off[r]:off[r + 1]]), r
+        assert np.array_equal(go[1], gb[1][off[r]:off[r + 1]]), r
+        assert np.all(np.isfinite(go[0])) and np.linalg.norm(go[0]) > 0, r

@@ -21,18 +21,23 @@ struct RobotParams {
     int ffOff[3], ffLen[3];  // force-field postfix programs in the formula table (length 0: none)
     int voff, vcount, loff, lcount;
     // collision broadphase: surface voxels [soff, soff+scount) of the surface
-    // list, hash table of (mask+1) buckets at bucket offset toff, cell size
+    // list; the rebuild grid's bucket table of (mask+1) buckets at toff (mask+2
+    // prefix entries), its cell edge = the candidate radius listR; candidate
+    // lists are rebuilt when a surface voxel moved more than sqrt(staleD2)
     int soff, scount, toff, mask;
-    double cellSize;
+    int listLinked;    // lists keep linked lattice neighbours (the robot can lose links or voxels)
+    float staleD2;
+    double listR;
 };
 
-// fixed-capacity buckets of the collision hash grid, and the most contacts one
-// voxel may have in one step (both overflows are reported, never ignored)
-// (the grid cell is one envelope diameter, up to 1.74 voxel lengths with thermal
-// expansion: a compressed body can put ~15 surface voxels in one cell and ~60
-// within one envelope)
-constexpr int kBucketCap = 32;
-constexpr int kMaxContacts = 64;
+// Candidate lists: every surface voxel lists (ascending) the surface voxels of
+// its robot within listR = the largest contact envelope + kSkin voxel lengths,
+// found through a counting-sort grid whenever a surface voxel of the robot has
+// moved more than kSkin / 2 (less a margin) since the last listing.  A voxel
+// with more than kMaxCand candidates searches the grid itself, and its robot
+// rebuilds the grid every step while it does.
+constexpr int kMaxCand = 48;
+constexpr double kSkin = 0.25;
 
 struct RobotTime {
     double time[2];
@@ -40,8 +45,9 @@ struct RobotTime {
     float dt;
     int diverged;
     int steps;
-    int overflow;  // bit 0: a collision grid bucket overflowed; bit 1: a voxel's contact list did
+    int stale[2];        // candidate lists to be rebuilt before the contact phase of the step of this parity
     int collisionCount;  // target / non-target contacts summed over the steps (VX3_VoxelyzeKernel.cu:711-726)
+    int listings;        // steps whose collision phase rebuilt the robot's candidate lists
     int pad;
 };
 
@@ -65,7 +71,7 @@ struct BatchBase {
     bool anyCollisions = false;
     bool anyRemoval = false;      // some robot can lose voxels (SecondaryExperiment)
     bool anyDetach = false;       // some robot can lose links (EnableDetach with failing materials)
-    long tableBuckets = 0;
+    long tableBuckets = 0;         // rebuild-grid prefix entries of the batch
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     std::vector<void *> allocs;
@@ -117,6 +123,10 @@ struct BatchBase {
     virtual int autotune(int trialSteps, double *ms, int *choice) = 0;
     // fixes the launch configuration VX3MI_MODE_AUTO uses
     virtual int configure(int mode, int tile, int threads) = 0;
+    // contact pairs of the collision phases of the next step calls (cap pairs
+    // per call; 0 stops), and those of the last call for one robot
+    virtual int record_contacts(int cap) = 0;
+    virtual int contact_pairs(int robot, int *out, int cap) = 0;
     RobotTime *d_rt = nullptr;
 };
 

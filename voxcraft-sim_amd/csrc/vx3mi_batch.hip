@@ -161,14 +161,25 @@ int vx3mi_batch_hits(vx3mi_batch *b, int *out) {
     return VX3MI_OK;
 }
 
-int vx3mi_batch_collision_overflow(vx3mi_batch *b, int *out) {
+int vx3mi_batch_listings(vx3mi_batch *b, int *out) {
     if (!b || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
     std::vector<RobotTime> rt;
     int rc = read_rt(b, rt);
     if (rc) return rc;
-    int any = 0;
-    for (int r = 0; r < b->b->R; r++) { out[r] = rt[r].overflow; any |= rt[r].overflow; }
-    return any ? 1 : 0;
+    for (int r = 0; r < b->b->R; r++) out[r] = rt[r].listings;
+    return VX3MI_OK;
+}
+
+int vx3mi_batch_record_contacts(vx3mi_batch *b, int cap) {
+    if (!b || !b->b) { set_error("batch is NULL"); return VX3MI_E_ARG; }
+    return b->b->record_contacts(cap);
+}
+
+int vx3mi_batch_contact_pairs(vx3mi_batch *b, int robot, int *pairs2, int cap) {
+    int rc = check_robot(b, robot);
+    if (rc) return rc;
+    if (!pairs2 && cap > 0) { set_error("pairs is NULL"); return VX3MI_E_ARG; }
+    return b->b->contact_pairs(robot, pairs2, cap);
 }
 
 int vx3mi_center_of_mass(vx3mi_batch *b, double *out) {

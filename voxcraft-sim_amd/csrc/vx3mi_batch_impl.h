@@ -51,17 +51,22 @@ template <typename T> struct Dev {
     const RobotParams *rp;
     RobotTime *rt;
     const T4<T> *contact;  // per-voxel contact force, or nullptr without collisions
-    // collision broadphase
+    // collision broadphase (candidate lists, see the collision section)
     const int *surf;       // surface voxel ids, CSR by robot
-    int4 *scell;           // per surface voxel: cell coordinates and bucket
-    int *ccnt[2];          // bucket counts, one table per step parity
-    int4 *ctab[2];         // bucket entries (kBucketCap per bucket): surface index, cell coordinates
+    const int *vsurf;      // voxel -> its index in surf, -1 when not listed
+    T4<T> *bpos;           // per voxel: position field at the last listing (surface voxels)
+    int *anyStale;         // per step parity: some robot's lists are stale
+    int4 *scell;           // per surface voxel: grid cell and bucket at the last listing
+    int *srank;            // per surface voxel: rank within its bucket (rebuild)
+    int *ccnt;             // bucket counts (rebuild; zero between rebuilds)
+    int *cstart;           // bucket starts within the robot's records, mask + 2 per robot
+    int *srec;             // robot-relative surface indices in bucket order, CSR by robot
+    int *clist;            // per surface voxel: kMaxCand candidate surface indices, ascending
+    int *ccount;           // per surface voxel: candidates (> kMaxCand: search the grid)
     T4<T> *contactW;
-    int4 *sinfo;           // per surface voxel this step: voxel, lattice index, fixed
-    T4<T> *spos;           // per surface voxel this step: position field xyz, base size average
-    int *fcount;           // per surface voxel: partners found this step
-    int *flist;            // per surface voxel: kMaxContacts partner surface indices (unordered)
-    long tableBuckets;
+    int2 *pairRec;         // when recording: pairs in contact (voxel ids, smaller surface index first)
+    int *pairN;
+    int pairCap;
     const VoxelCilia *cilia;  // per-voxel cilia, or nullptr when no robot has EnableCilia
     // removal (SecondaryExperiment): step count at whose end a voxel / link was
     // removed (0 present), or nullptr when no robot can lose voxels
@@ -114,6 +119,9 @@ template <typename T> __device__ __forceinline__ void link_update(const Dev<T> &
     // divergence (VX3_VoxelyzeKernel.cu:271-280): any link, not one sampled link
     if (o.strain > 100.0f) atomicOr(&rt->diverged, 1);
 }
+
+template <typename T> __device__ __forceinline__ void col_track(const Dev<T> &a, int v, const V3<T> &u, RobotTime *rt,
+                                                                const RobotParams &rp, int parity);
 
 template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> &a, int v, int parity) {
     const int r = a.vrob[v];
@@ -190,6 +198,7 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
     a.pq1[v] = T4<T>{s.q.x, s.q.y, s.q.z, (T)temp};
     a.mo0[v] = T4<T>{s.p.x, s.p.y, s.p.z, s.h.x};
     a.mo1[v] = T4<T>{s.h.y, s.h.z, (T)s.flags, (T)0};
+    if (rp.collisions) col_track(a, v, s.u, rt, rp, parity);
     // the voxel's own signal section ends a full timeStep (not one of a voxel
     // with all six degrees of freedom fixed, VX3_Voxel.cu:166-172)
     if (a.sig && rp.signals && dt != 0.0f && !(vm.ext >= 0 && a.exts[vm.ext].dof == DOF_ALL)) {
@@ -238,19 +247,21 @@ template <typename T> __global__ __launch_bounds__(256) void k_force_field(Dev<T
     a.field[v] = T4<T>{f.x, f.y, f.z, (T)0};
 }
 
-// ---- collisions: hashed uniform grid over surface voxels ----------------------
+// ---- collisions: candidate lists from a counting-sort grid --------------------
 // Replaces the all-pairs surface loop (updateAttach VX3_VoxelyzeKernel.cu:448-458,
-// gpu_update_attach :833-843).  The cell edge is the largest envelope
-// diameter, so every pair within an envelope lies in adjacent cells.
+// gpu_update_attach :833-843).  Every surface voxel keeps the ascending list of
+// its robot's surface voxels within listR (the largest contact envelope plus a
+// skin of kSkin voxel lengths).  The lists are built from a uniform grid of
+// cell edge listR by a counting sort -- count per bucket, prefix sum, scatter
+// in bucket order -- whenever a surface voxel of the robot has moved more than
+// half the skin since the last listing, and each step the contact phase tests
+// every listed candidate against the envelope exactly.  A pair inside the
+// envelope now was within listR when the lists were built (each end moved less
+// than half the skin since), so the partners are exactly those of the all-pairs
+// loop, in ascending order, whenever the lists were built: results do not
+// depend on the rebuild schedule.
 __device__ __forceinline__ unsigned cell_hash(int cx, int cy, int cz) {
     return ((unsigned)cx * 73856093u) ^ ((unsigned)cy * 19349663u) ^ ((unsigned)cz * 83492791u);
-}
-
-template <typename T> __device__ __forceinline__ V3<T> abs_position(const Dev<T> &a, int v, const T4<T> &q0) {
-    if constexpr (kAbsolutePositions<T>) return V3<T>{q0.x, q0.y, q0.z};
-    const int pk = a.lat[v];
-    const T L = (T)a.vmats[a.vmat[v]].nomSize;
-    return V3<T>{(T)(pk & 1023) * L + q0.x, (T)((pk >> 10) & 1023) * L + q0.y, (T)(pk >> 20) * L + q0.z};
 }
 
 // a voxel of the surface list that currently takes part in collisions: with
@@ -265,161 +276,252 @@ template <typename T> __device__ __forceinline__ bool col_active(const Dev<T> &a
     return surface;
 }
 
-template <typename T> __device__ __forceinline__ void col_insert(const Dev<T> &a, int s, int parity) {
+// v and w are directly linked (is_neighbor depth 1, VX3_VoxelyzeKernel.cu:651-680,
+// :701-703): a link joins lattice neighbours only, so a unit lattice offset in
+// direction d (X_POS..Z_NEG) is linked when v's link slot d is present
+template <typename T> __device__ __forceinline__ bool col_linked(const Dev<T> &a, int v, int lx, int ly, int lz) {
+    if (abs(lx) + abs(ly) + abs(lz) != 1) return false;
+    const int d = lx ? (lx > 0 ? 0 : 1) : (ly ? (ly > 0 ? 2 : 3) : (lz > 0 ? 4 : 5));
+    return a.vlinks[6 * v + d] >= 0;
+}
+
+// lattice offset of w from v
+__device__ __forceinline__ int3 lat_offset(int pkv, int pkw) {
+    return int3{(pkw & 1023) - (pkv & 1023), ((pkw >> 10) & 1023) - ((pkv >> 10) & 1023), (pkw >> 20) - (pkv >> 20)};
+}
+
+// pos_v - pos_w: absolute positions in double; in float the lattice offset and
+// the displacement difference are kept apart
+template <typename T> __device__ __forceinline__ V3<T> col_diff(const T4<T> &pv, const T4<T> &pw, int3 o, T L) {
+    if constexpr (kAbsolutePositions<T>) return V3<T>{pv.x - pw.x, pv.y - pw.y, pv.z - pw.z};
+    return V3<T>{(T)(-o.x) * L + (pv.x - pw.x), (T)(-o.y) * L + (pv.y - pw.y), (T)(-o.z) * L + (pv.z - pw.z)};
+}
+
+// rebuild pass 1 (robots whose lists are stale): the cell of each surface voxel,
+// its rank within its bucket, its position field at the listing
+template <typename T> __device__ __forceinline__ void col_count(const Dev<T> &a, int s, int parity) {
     const int v = a.surf[s];
-    RobotTime *rt = a.rt + a.vrob[v];
-    const RobotParams &rp = a.rp[a.vrob[v]];
-    a.fcount[s] = 0;
-    if (!rp.collisions || rt->diverged || !col_active(a, v)) return;
-    const V3<T> p = abs_position(a, v, a.pq0[v]);
-    const double h = rp.cellSize;
-    const int cx = (int)floor((double)p.x / h), cy = (int)floor((double)p.y / h), cz = (int)floor((double)p.z / h);
+    const int r = a.vrob[v];
+    const RobotTime *rt = a.rt + r;
+    if (!rt->stale[parity] || rt->diverged) return;
+    const RobotParams &rp = a.rp[r];
+    const T4<T> q0 = a.pq0[v];
+    const V3<double> p = position_d<T>(a.lat[v], a.vmats[a.vmat[v]].nomSize, V3<T>{q0.x, q0.y, q0.z});
+    const double h = rp.listR;
+    const int cx = (int)floor(p.x / h), cy = (int)floor(p.y / h), cz = (int)floor(p.z / h);
     const int key = (int)(cell_hash(cx, cy, cz) & (unsigned)rp.mask);
     a.scell[s] = int4{cx, cy, cz, key};
-    {  // the record the search reads for this voxel as a partner
-        const VoxelMat &m = a.vmats[a.vmat[v]];
-        const T4<T> q0 = a.pq0[v];
-        a.sinfo[s] = int4{v, a.lat[v], m.fixed, 0};
-        a.spos[s] = T4<T>{q0.x, q0.y, q0.z, base_size_average<T>(m.nomSize, (float)a.pq1[v].w, m.alphaCTE)};
-    }
-    const long b = (long)rp.toff + key;
-    const int slot = atomicAdd(&a.ccnt[parity][b], 1);
-    if (slot < kBucketCap) a.ctab[parity][b * kBucketCap + slot] = int4{s, cx, cy, cz};
-    else atomicOr(&rt->overflow, 1);
+    a.srank[s] = atomicAdd(&a.ccnt[rp.toff + key], 1);
+    a.bpos[v] = q0;
 }
 
-// One neighbouring cell (c = 0..26, offsets -1..1 in x fastest) of surface
-// voxel s: the partners in its bucket that are inside the contact envelope,
-// not directly linked (is_neighbor depth 1, VX3_VoxelyzeKernel.cu:651-680,
-// :701-703) and not both fixed, appended to s's list.  27 work-items per voxel
-// keep the dependent bucket / cell / position loads of one voxel in parallel.
-template <typename T> __device__ __forceinline__ void col_find(const Dev<T> &a, int s, int c, int parity) {
+// rebuild pass 2, one workgroup of blockDim threads per robot: exclusive prefix
+// sum of the robot's bucket counts into cstart (mask + 2 entries, the last the
+// total), counts zeroed for the next rebuild
+template <typename T> __device__ __forceinline__ void col_scan(const Dev<T> &a, int r, int *wsum) {
+    const RobotParams &rp = a.rp[r];
+    const int n = rp.mask + 1, nt = blockDim.x, tid = threadIdx.x;
+    const int chunk = (n + nt - 1) / nt, b0 = min(n, tid * chunk), b1 = min(n, b0 + chunk);
+    int *cnt = a.ccnt + rp.toff, *start = a.cstart + rp.toff;
+    int sum = 0;
+    for (int b = b0; b < b1; b++) sum += cnt[b];
+    wsum[tid] = sum;
+    __syncthreads();
+    if (tid == 0) {  // nt partial sums, exclusive
+        int acc = 0;
+        for (int i = 0; i < nt; i++) { const int x = wsum[i]; wsum[i] = acc; acc += x; }
+        start[n] = acc;
+    }
+    __syncthreads();
+    int acc = wsum[tid];
+    for (int b = b0; b < b1; b++) {
+        const int c = cnt[b];
+        start[b] = acc;
+        acc += c;
+        cnt[b] = 0;
+    }
+}
+
+// rebuild pass 3: surface indices (robot-relative) in bucket order
+template <typename T> __device__ __forceinline__ void col_scatter(const Dev<T> &a, int s, int parity) {
     const int v = a.surf[s];
-    RobotTime *rt = a.rt + a.vrob[v];
-    const RobotParams &rp = a.rp[a.vrob[v]];
-    if (!rp.collisions || rt->diverged || !col_active(a, v)) return;
-    const int dx = c % 3 - 1, dy = (c / 3) % 3 - 1, dz = c / 9 - 1;
+    const int r = a.vrob[v];
+    const RobotTime *rt = a.rt + r;
+    if (!rt->stale[parity] || rt->diverged) return;
+    const RobotParams &rp = a.rp[r];
+    a.srec[rp.soff + a.cstart[rp.toff + a.scell[s].w] + a.srank[s]] = s - rp.soff;
+}
+
+// calls f(sj) for every surface voxel sj of the robot in the 27 grid cells
+// around surface voxel s (the grid of the last listing)
+template <typename T, typename F> __device__ __forceinline__ void col_grid_each(const Dev<T> &a, const RobotParams &rp, int s, F &&f) {
     const int4 cs = a.scell[s];
-    const int key = (int)(cell_hash(cs.x + dx, cs.y + dy, cs.z + dz) & (unsigned)rp.mask);
-    const long b = (long)rp.toff + key;
-    const int n = min(a.ccnt[parity][b], kBucketCap);
-    if (n == 0) return;
-    const VoxelMat &mi = a.vmats[a.vmat[v]];
-    const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v];
-    const V3<T> pi = abs_position(a, v, qi0);
-    const T bsaI = base_size_average<T>(mi.nomSize, (float)qi1.w, mi.alphaCTE);
-    const int pki = a.lat[v];
-    for (int k = 0; k < n; k++) {
-        const int4 e = a.ctab[parity][b * kBucketCap + k];
-        const int sj = e.x;
-        if (sj == s) continue;
-        if (e.y != cs.x + dx || e.z != cs.y + dy || e.w != cs.z + dz) continue;  // hash alias
-        const int4 ij = a.sinfo[sj];
-        const T4<T> pj = a.spos[sj];
-        // a link joins lattice neighbours only: a unit lattice offset in direction d
-        // (X_POS..Z_NEG) is linked when the voxel's link slot d is present
-        const int pkj = ij.y;
-        const int lx = (pkj & 1023) - (pki & 1023), ly = ((pkj >> 10) & 1023) - ((pki >> 10) & 1023),
-                  lz = (pkj >> 20) - (pki >> 20);
-        if (abs(lx) + abs(ly) + abs(lz) == 1) {
-            const int d = lx ? (lx > 0 ? 0 : 1) : (ly ? (ly > 0 ? 2 : 3) : (lz > 0 ? 4 : 5));
-            if (a.vlinks[6 * v + d] >= 0) continue;
+    for (int c = 0; c < 27; c++) {
+        const int cx = cs.x + c % 3 - 1, cy = cs.y + (c / 3) % 3 - 1, cz = cs.z + c / 9 - 1;
+        const int key = (int)(cell_hash(cx, cy, cz) & (unsigned)rp.mask);
+        const int b0 = a.cstart[rp.toff + key], b1 = a.cstart[rp.toff + key + 1];
+        for (int k = b0; k < b1; k++) {
+            const int sj = rp.soff + a.srec[rp.soff + k];
+            if (sj == s) continue;
+            const int4 cj = a.scell[sj];
+            if (cj.x != cx || cj.y != cy || cj.z != cz) continue;  // another cell of the same bucket
+            f(sj);
         }
-        if (mi.fixed && ij.z) continue;
-        V3<T> diff;
-        if constexpr (kAbsolutePositions<T>) {
-            diff = pi - V3<T>{pj.x, pj.y, pj.z};
-        } else {
-            // lattice offset and displacement difference kept apart in float
-            const T L = (T)mi.nomSize;
-            diff = V3<T>{(T)(-lx) * L + (qi0.x - pj.x), (T)(-ly) * L + (qi0.y - pj.y), (T)(-lz) * L + (qi0.z - pj.z)};
-        }
-        if (!in_contact_envelope(diff, bsaI, pj.w)) continue;
-        const int slot = atomicAdd(&a.fcount[s], 1);
-        if (slot < kMaxContacts) a.flist[(long)s * kMaxContacts + slot] = sj;
-        else atomicOr(&rt->overflow, 2);
     }
 }
 
-// The contact force of surface voxel s from the partners col_find listed.
+// rebuild pass 4: the candidate list of surface voxel s, ascending: surface
+// voxels of its robot in the 27 cells around it within listR, not both fixed,
+// not directly linked unless the robot can lose links or voxels (the contact
+// phase tests the current links then); more than kMaxCand: ccount = their
+// number and the voxel searches the grid itself
+template <typename T> __device__ __forceinline__ void col_list(const Dev<T> &a, int s, int parity) {
+    const int v = a.surf[s];
+    const int r = a.vrob[v];
+    const RobotTime *rt = a.rt + r;
+    if (!rt->stale[parity] || rt->diverged) return;
+    const RobotParams &rp = a.rp[r];
+    const VoxelMat &mi = a.vmats[a.vmat[v]];
+    const T4<T> pv = a.pq0[v];
+    const int pkv = a.lat[v];
+    const T L = (T)mi.nomSize, R = (T)rp.listR;
+    int *list = a.clist + (long)s * kMaxCand;
+    int n = 0;
+    col_grid_each(a, rp, s, [&](int sj) {
+        const int w = a.surf[sj];
+        const int3 o = lat_offset(pkv, a.lat[w]);
+        if (!rp.listLinked && col_linked(a, v, o.x, o.y, o.z)) return;
+        if (mi.fixed && a.vmats[a.vmat[w]].fixed) return;
+        const V3<T> d = col_diff<T>(pv, a.pq0[w], o, L);
+        if (d.x * d.x + d.y * d.y + d.z * d.z > R * R) return;
+        if (n < kMaxCand) {  // insertion in ascending order
+            int k = n;
+            while (k > 0 && list[k - 1] > sj) { list[k] = list[k - 1]; k--; }
+            list[k] = sj;
+        }
+        n++;
+    });
+    a.ccount[s] = n;
+}
+
+// The contact force of surface voxel s from the partners inside its envelope,
+// summed in ascending partner index (the order in which the serialized pair loop
+// of the reference accumulates this voxel's contact force), with the target
+// hits, the forced signal, and the pair record when one is kept.  Clears this
+// step's listing flags (the rebuild passes have run).
 template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity) {
     const int v = a.surf[s];
-    RobotTime *rt = a.rt + a.vrob[v];
-    const RobotParams &rp = a.rp[a.vrob[v]];
+    const int r = a.vrob[v];
+    RobotTime *rt = a.rt + r;
+    const RobotParams &rp = a.rp[r];
+    if (s == rp.soff && rt->stale[parity]) {
+        rt->listings += 1;
+        rt->stale[parity] = 0;
+    }
+    if (s == 0 && a.anyStale) a.anyStale[parity] = 0;
     if (!rp.collisions || rt->diverged || !col_active(a, v)) return;
     const VoxelMat &mi = a.vmats[a.vmat[v]];
-    const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v];
-    const V3<T> pi = abs_position(a, v, qi0);
+    const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v], mi0 = a.mo0[v];
+    const int pkv = a.lat[v];
+    const T L = (T)mi.nomSize;
     const T bsaI = base_size_average<T>(mi.nomSize, (float)qi1.w, mi.alphaCTE);
-    const int nf = min(a.fcount[s], kMaxContacts);
-    const int *list = a.flist + (long)s * kMaxContacts;  // unordered, each partner once
-    if (a.vtarget) {
-        // a target touching a non-target: collisionCount (each pair once), and
-        // with signals the non-target voxel receives a forced 100 (:711-726)
-        const int ti = a.vtarget[v];
-        bool hitTarget = false;
-        for (int k = 0; k < nf; k++) {
-            const int sj = list[k];
-            if (a.vtarget[a.surf[sj]] == ti) continue;
+    const V3<T> velI = V3<T>{mi0.x, mi0.y, mi0.z} * (T)mi.massInv;
+    const int ti = a.vtarget ? a.vtarget[v] : 0;
+    bool hitTarget = false;
+    V3<T> f = {0, 0, 0};
+    // partner sj when it is in contact now (links: whether to test the current
+    // links; lists leave out linked neighbours unless the robot can lose links)
+    auto partner = [&](int sj, bool links) -> bool {
+        const int w = a.surf[sj];
+        if (!col_active(a, w)) return false;
+        const int3 o = lat_offset(pkv, a.lat[w]);
+        if (links && col_linked(a, v, o.x, o.y, o.z)) return false;
+        const VoxelMat &mj = a.vmats[a.vmat[w]];
+        if (mi.fixed && mj.fixed) return false;
+        const T4<T> qj0 = a.pq0[w];
+        const V3<T> diff = col_diff<T>(qi0, qj0, o, L);
+        const T bsaJ = base_size_average<T>(mj.nomSize, (float)a.pq1[w].w, mj.alphaCTE);
+        if (!in_contact_envelope(diff, bsaI, bsaJ)) return false;
+        if (a.vtarget && a.vtarget[w] != ti) {  // a target touching a non-target (:711-726)
             if (s < sj) atomicAdd(&rt->collisionCount, 1);
             hitTarget = true;
         }
-        if (hitTarget && !ti && a.sig && rp.signals) {
-            VoxelSignal sg = a.sig[v];
-            signal_receive(sg, a.sigMat[v], 100, rt->time[parity], true);
-            a.sig[v] = sg;
+        if (a.pairRec && s < sj) {
+            const int k = atomicAdd(a.pairN, 1);
+            if (k < a.pairCap) a.pairRec[k] = int2{v, w};
+        }
+        const T4<T> mj0 = a.mo0[w];
+        const V3<T> velJ = V3<T>{mj0.x, mj0.y, mj0.z} * (T)mj.massInv;
+        f = f + contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
+        return true;
+    };
+    const int nc = a.ccount[s];
+    if (nc <= kMaxCand) {
+        const int *list = a.clist + (long)s * kMaxCand;
+        for (int k = 0; k < nc; k++) partner(list[k], rp.listLinked != 0);
+    } else {
+        // more candidates than a list holds: ascending partners straight from the
+        // grid, which is rebuilt every step while this lasts
+        rt->stale[parity ^ 1] = 1;
+        if (a.anyStale) a.anyStale[parity ^ 1] = 1;
+        int prev = -1;
+        for (;;) {
+            int next = INT_MAX;
+            col_grid_each(a, rp, s, [&](int sj) {
+                if (sj > prev && sj < next) next = sj;
+            });
+            if (next == INT_MAX) break;
+            partner(next, true);
+            prev = next;
         }
     }
-    const V3<T> velI = V3<T>{a.mo0[v].x, a.mo0[v].y, a.mo0[v].z} * (T)mi.massInv;
-    V3<T> f = {0, 0, 0};
-    // ascending partner index: the order in which the serialized pair loop of
-    // the reference accumulates this voxel's contact force.  The list is short
-    // (usually empty), so each next partner is found by a scan of the list
-    // instead of sorting a copy in private memory.
-    int prev = -1;
-    for (int k = 0; k < nf; k++) {
-        int sj = INT_MAX;
-        for (int i = 0; i < nf; i++) {
-            const int x = list[i];
-            if (x > prev && x < sj) sj = x;
-        }
-        if (sj == INT_MAX) break;  // (entries are distinct; never taken)
-        prev = sj;
-        const int vj = a.surf[sj];
-        const VoxelMat &mj = a.vmats[a.vmat[vj]];
-        const T4<T> qj0 = a.pq0[vj], qj1 = a.pq1[vj];
-        V3<T> diff;
-        if constexpr (kAbsolutePositions<T>) {
-            diff = pi - V3<T>{qj0.x, qj0.y, qj0.z};
-        } else {
-            const int pki = a.lat[v], pkj = a.lat[vj];
-            const T L = (T)mi.nomSize;
-            diff = V3<T>{(T)((pki & 1023) - (pkj & 1023)) * L + (qi0.x - qj0.x),
-                         (T)(((pki >> 10) & 1023) - ((pkj >> 10) & 1023)) * L + (qi0.y - qj0.y),
-                         (T)((pki >> 20) - (pkj >> 20)) * L + (qi0.z - qj0.z)};
-        }
-        const T bsaJ = base_size_average<T>(mj.nomSize, (float)qj1.w, mj.alphaCTE);
-        const V3<T> velJ = V3<T>{a.mo0[vj].x, a.mo0[vj].y, a.mo0[vj].z} * (T)mj.massInv;
-        f = f + contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
+    // with signals a non-target voxel touching a target receives a forced 100
+    if (hitTarget && !ti && a.sig && rp.signals) {
+        VoxelSignal sg = a.sig[v];
+        signal_receive(sg, a.sigMat[v], 100, rt->time[parity], true);
+        a.sig[v] = sg;
     }
     a.contactW[v] = T4<T>{f.x, f.y, f.z, 0};
 }
 
-template <typename T> __global__ __launch_bounds__(256) void k_col_insert(Dev<T> a, int s0, int n, int parity) {
+// a moved surface voxel marks its robot's lists stale for the next step
+template <typename T> __device__ __forceinline__ void col_track(const Dev<T> &a, int v, const V3<T> &u, RobotTime *rt,
+                                                                const RobotParams &rp, int parity) {
+    if (!a.vsurf || a.vsurf[v] < 0) return;
+    const T4<T> b = a.bpos[v];
+    const T dx = u.x - b.x, dy = u.y - b.y, dz = u.z - b.z;
+    if (dx * dx + dy * dy + dz * dz > (T)rp.staleD2) {
+        rt->stale[parity ^ 1] = 1;
+        a.anyStale[parity ^ 1] = 1;
+    }
+}
+
+// rebuild kernels: every surface voxel of the batch, robots whose lists are
+// stale (all = 0: only when some robot is)
+template <typename T> __global__ __launch_bounds__(256) void k_col_count(Dev<T> a, int s0, int n, int parity, int all) {
+    if (!all && !a.anyStale[parity]) return;
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) col_insert(a, s0 + i, parity);
+    if (i < n) col_count(a, s0 + i, parity);
 }
-
-// 27 work-items per surface voxel, one per neighbouring cell
-template <typename T> __global__ __launch_bounds__(256) void k_col_find(Dev<T> a, int s0, int n, int parity) {
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < 27L * n) col_find(a, s0 + (int)(i / 27), (int)(i % 27), parity);
+template <typename T> __global__ __launch_bounds__(1024) void k_col_scan(Dev<T> a, int r0, int parity, int all) {
+    __shared__ int wsum[1024];
+    if (!all && !a.anyStale[parity]) return;
+    const int r = r0 + blockIdx.x;
+    if (!a.rt[r].stale[parity] || a.rt[r].diverged || a.rp[r].scount == 0) return;
+    col_scan(a, r, wsum);
 }
-
-// contact forces; also empties the other parity's table for the next step
+template <typename T> __global__ __launch_bounds__(256) void k_col_scatter(Dev<T> a, int s0, int n, int parity, int all) {
+    if (!all && !a.anyStale[parity]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) col_scatter(a, s0 + i, parity);
+}
+template <typename T> __global__ __launch_bounds__(256) void k_col_list(Dev<T> a, int s0, int n, int parity, int all) {
+    if (!all && !a.anyStale[parity]) return;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) col_list(a, s0 + i, parity);
+}
 template <typename T> __global__ __launch_bounds__(256) void k_col_contact(Dev<T> a, int s0, int n, int parity) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    for (long b = i; b < a.tableBuckets; b += (long)gridDim.x * blockDim.x) a.ccnt[parity ^ 1][b] = 0;
     if (i < n) col_contact(a, s0 + i, parity);
 }
 
@@ -481,23 +583,52 @@ template <typename T> __global__ __launch_bounds__(256) void k_detach(Dev<T> a) 
     a.vlinks[6 * e.y + 2 * axis + 1] = -1;  // and pVPos's NEG-direction link
 }
 
-// One workgroup per robot; nsteps steps with two barrier-separated phases.
+template <typename T> __global__ __launch_bounds__(256) void k_mark_stale(RobotTime *rt, int R, int *anyStale) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r < R) rt[r].stale[0] = rt[r].stale[1] = 1;
+    if (r == 0 && anyStale) anyStale[0] = anyStale[1] = 1;
+}
+
+// nsteps steps with dt = 0: the step count advances, time and the previous dt
+// stay (an odd count carries them, and pending listings, to the other parity)
+template <typename T> __global__ __launch_bounds__(256) void k_zero_steps(RobotTime *rt, int R, int nsteps, int parity, int *anyStale) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r == 0 && anyStale && (nsteps & 1) && anyStale[parity]) anyStale[parity ^ 1] = 1;
+    if (r >= R || rt[r].diverged) return;
+    RobotTime &t = rt[r];
+    if (nsteps & 1) {
+        t.time[parity ^ 1] = t.time[parity];
+        t.prevDt[parity ^ 1] = t.prevDt[parity];
+        t.stale[parity ^ 1] |= t.stale[parity];
+    }
+    t.steps += nsteps;
+}
+
+// One workgroup per robot; nsteps steps with barrier-separated phases (the
+// collision listing when it is stale, the contact forces, links, voxels).
 template <typename T, int NT> __global__ __launch_bounds__(NT) void k_persistent(Dev<T> a, int nsteps, int parity0) {
-    const RobotParams rp = a.rp[blockIdx.x];
-    const bool col = rp.collisions && rp.scount > 0 && a.contact != nullptr;
+    __shared__ int wsum[NT];
+    const int r = blockIdx.x;
+    const RobotParams rp = a.rp[r];
+    const bool col = rp.collisions && rp.scount > 0 && a.contactW != nullptr;
     for (int s = 0; s < nsteps; s++) {
         const int parity = (parity0 + s) & 1;
         if (col) {
-            for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_insert(a, i, parity);
-            for (int b = threadIdx.x; b <= rp.mask; b += blockDim.x) a.ccnt[parity ^ 1][(long)rp.toff + b] = 0;
+            if (a.rt[r].stale[parity] && !a.rt[r].diverged) {
+                for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += NT) col_count(a, i, parity);
+                __syncthreads();
+                col_scan(a, r, wsum);
+                __syncthreads();
+                for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += NT) col_scatter(a, i, parity);
+                __syncthreads();
+                for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += NT) col_list(a, i, parity);
+            }
             __syncthreads();
-            for (int i = threadIdx.x; i < 27 * rp.scount; i += blockDim.x) col_find(a, rp.soff + i / 27, i % 27, parity);
-            __syncthreads();
-            for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_contact(a, i, parity);
+            for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += NT) col_contact(a, i, parity);
         }
-        for (int l = rp.loff + threadIdx.x; l < rp.loff + rp.lcount; l += blockDim.x) link_update(a, l, parity);
+        for (int l = rp.loff + threadIdx.x; l < rp.loff + rp.lcount; l += NT) link_update(a, l, parity);
         __syncthreads();
-        for (int v = rp.voff + threadIdx.x; v < rp.voff + rp.vcount; v += blockDim.x) voxel_update(a, v, parity);
+        for (int v = rp.voff + threadIdx.x; v < rp.voff + rp.vcount; v += NT) voxel_update(a, v, parity);
         __syncthreads();
     }
 }
@@ -806,6 +937,7 @@ template <typename T> struct Batch : BatchBase {
         for (int r = 0; r < R; r++) {
             std::memset(&rt[r], 0, sizeof(RobotTime));
             rt[r].dt = defaultDt[r];
+            rt[r].stale[0] = rt[r].stale[1] = 1;  // candidate lists are built by the first step
         }
         curDt = defaultDt;
         HIP_TRY(hipMemcpy(d_rt, rt.data(), sizeof(RobotTime) * R, hipMemcpyHostToDevice));
@@ -841,16 +973,25 @@ template <typename T> struct Batch : BatchBase {
             if (p.scount) {
                 for (int v : rs) surf.push_back(p.voff + v);
                 int hsz = 64;
-                while (hsz < 2 * p.scount) hsz <<= 1;
+                while (hsz < p.scount) hsz <<= 1;
                 p.toff = (int)buckets;
                 p.mask = hsz - 1;
-                buckets += hsz;
-                // largest envelope diameter: 2 * 0.625 * max base size (thermal growth bounded by
-                // the largest |temperature| times the largest CTE of the robot)
-                float amax = 0.0f;
-                for (const VoxelMat &vm : m.vmats) amax = std::max(amax, std::fabs(vm.alphaCTE));
-                const double tmax = std::max(std::fabs((double)m.initialTemp), std::fabs(m.tempAmplitude));
-                p.cellSize = 2 * kCOLLISION_ENVELOPE_RADIUS * m.latticeDim * (1.0 + tmax * amax) * 1.0001;
+                buckets += hsz + 2;
+                // the largest base size: thermal factor 1 + T * CTE over the
+                // temperatures the robot takes (the import temperature and, with
+                // actuation, -A .. A or -A .. 0 without EnableExpansion)
+                double tlo = m.initialTemp, thi = m.initialTemp;
+                if (m.varyTemp) {
+                    tlo = std::min(tlo, -std::fabs(m.tempAmplitude));
+                    thi = std::max(thi, m.enableExpansion ? std::fabs(m.tempAmplitude) : 0.0);
+                }
+                double fmax = 1.0;
+                for (const VoxelMat &vm : m.vmats)
+                    fmax = std::max({fmax, 1.0 + (double)vm.alphaCTE * tlo, 1.0 + (double)vm.alphaCTE * thi});
+                const double L0 = m.latticeDim;
+                p.listR = (2 * kCOLLISION_ENVELOPE_RADIUS * L0 * fmax + kSkin * L0) * 1.0001;
+                p.staleD2 = (float)((0.49 * kSkin * L0) * (0.49 * kSkin * L0));
+                p.listLinked = (m.enableDetach || m.secondaryExperiment) ? 1 : 0;
                 anyCollisions = true;
             }
         }
@@ -858,27 +999,35 @@ template <typename T> struct Batch : BatchBase {
         tableBuckets = buckets;
         HIP_TRY(hipMemcpy(d_rp, hrp.data(), sizeof(RobotParams) * R, hipMemcpyHostToDevice));
         d.contact = nullptr;
-        d.tableBuckets = buckets;
+        d.contactW = nullptr;
+        d.vsurf = nullptr;
+        d.anyStale = nullptr;
+        d.pairRec = nullptr;
+        d.pairN = nullptr;
+        d.pairCap = 0;
         if (anyCollisions) {
-            int *surfd, *c0, *c1, *fc, *fl;
-            int4 *scd, *sid, *t0, *t1;
-            T4<T> *cw, *spd;
-            if ((rc = alloc(&surfd, S)) || (rc = alloc(&scd, S)) || (rc = alloc(&c0, buckets)) ||
-                (rc = alloc(&c1, buckets)) || (rc = alloc(&t0, buckets * kBucketCap)) ||
-                (rc = alloc(&t1, buckets * kBucketCap)) || (rc = alloc(&cw, V)) || (rc = alloc(&fc, S)) ||
-                (rc = alloc(&fl, (size_t)S * kMaxContacts)) || (rc = alloc(&sid, S)) || (rc = alloc(&spd, S)))
+            int *surfd, *vsurfd, *anyd, *srankd, *ccntd, *cstartd, *srecd, *clistd, *ccountd;
+            int4 *scd;
+            T4<T> *cw, *bpd;
+            if ((rc = alloc(&surfd, S)) || (rc = alloc(&vsurfd, V)) || (rc = alloc(&anyd, 2)) || (rc = alloc(&scd, S)) ||
+                (rc = alloc(&srankd, S)) || (rc = alloc(&ccntd, buckets)) || (rc = alloc(&cstartd, buckets)) ||
+                (rc = alloc(&srecd, S)) || (rc = alloc(&clistd, (size_t)S * kMaxCand)) || (rc = alloc(&ccountd, S)) ||
+                (rc = alloc(&cw, V)) || (rc = alloc(&bpd, V)))
                 return rc;
-            d.sinfo = sid;
-            d.spos = spd;
-            HIP_TRY(hipMemset(fc, 0, sizeof(int) * S));
-            d.fcount = fc;
-            d.flist = fl;
+            std::vector<int> vs((size_t)V, -1);
+            for (int i = 0; i < S; i++) vs[surf[i]] = i;
+            const int two[2] = {1, 1};
             HIP_TRY(hipMemcpy(surfd, surf.data(), sizeof(int) * S, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemset(c0, 0, sizeof(int) * buckets));
-            HIP_TRY(hipMemset(c1, 0, sizeof(int) * buckets));
+            HIP_TRY(hipMemcpy(vsurfd, vs.data(), sizeof(int) * V, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(anyd, two, sizeof(two), hipMemcpyHostToDevice));
+            HIP_TRY(hipMemset(ccntd, 0, sizeof(int) * buckets));
+            HIP_TRY(hipMemset(cstartd, 0, sizeof(int) * buckets));
+            HIP_TRY(hipMemset(ccountd, 0, sizeof(int) * S));
             HIP_TRY(hipMemset(cw, 0, sizeof(T4<T>) * V));
+            HIP_TRY(hipMemset(bpd, 0, sizeof(T4<T>) * V));
             HIP_TRY(hipDeviceSynchronize());
-            d.surf = surfd; d.scell = scd; d.ccnt[0] = c0; d.ccnt[1] = c1; d.ctab[0] = t0; d.ctab[1] = t1;
+            d.surf = surfd; d.vsurf = vsurfd; d.anyStale = anyd; d.scell = scd; d.srank = srankd;
+            d.ccnt = ccntd; d.cstart = cstartd; d.srec = srecd; d.clist = clistd; d.ccount = ccountd; d.bpos = bpd;
             d.contactW = cw;
             d.contact = cw;
         }
@@ -1045,11 +1194,23 @@ template <typename T> struct Batch : BatchBase {
         return VX3MI_OK;
     }
 
+    // the collision phase of one step for surface voxels [s0, s0 + n) (robots
+    // [r0, r0 + nr)): the listing passes (which return at once unless some
+    // robot's lists are stale; all: check the robots' own flags only), then
+    // the contact forces
+    void launch_collisions(int s0, int n, int r0, int nr, int parity, int all) {
+        const int g = (n + 255) / 256;
+        k_col_count<T><<<g, 256, 0, stream>>>(d, s0, n, parity, all);
+        k_col_scan<T><<<nr, 1024, 0, stream>>>(d, r0, parity, all);
+        k_col_scatter<T><<<g, 256, 0, stream>>>(d, s0, n, parity, all);
+        k_col_list<T><<<g, 256, 0, stream>>>(d, s0, n, parity, all);
+        k_col_contact<T><<<g, 256, 0, stream>>>(d, s0, n, parity);
+    }
+    static constexpr int kCollisionLaunches = 5;
+
     void launch_flat_step(int parity) {
-        if (anyCollisions) k_col_insert<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
-        if (anyCollisions) k_col_find<T><<<(int)((27L * S + 255) / 256), 256, 0, stream>>>(d, 0, S, parity);
+        if (anyCollisions) launch_collisions(0, S, 0, R, parity, 0);
         if (L) k_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, 0, L, parity);
-        if (anyCollisions) k_col_contact<T><<<(S + 255) / 256, 256, 0, stream>>>(d, 0, S, parity);
         if (anyForceField) k_force_field<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         k_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         if (anyDetach) k_detach<T><<<(d.nDetCand + 255) / 256, 256, 0, stream>>>(d);
@@ -1057,13 +1218,18 @@ template <typename T> struct Batch : BatchBase {
         if (anyRemoval) k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
     }
     int kernels_per_flat_step() const {
-        return 2 + (anyCollisions ? 3 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0) +
+        return 2 + (anyCollisions ? kCollisionLaunches : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0) +
                (anyDetach ? 1 : 0);
     }
 
     int step(int nsteps, float dt, int mode) override {
         if (nsteps <= 0) return VX3MI_OK;
-        if (dt == 0.0f) return VX3MI_OK;  // doTimeStep(0) only advances the step counter
+        if (dt == 0.0f) {  // doTimeStep(0) only advances the step counter (VX3_VoxelyzeKernel.cu:241-243)
+            k_zero_steps<T><<<(R + 255) / 256, 256, 0, stream>>>(d_rt, R, nsteps, parity, d.anyStale);
+            HIP_TRY(hipGetLastError());
+            parity = (parity + nsteps) & 1;
+            return VX3MI_OK;
+        }
         int rc = set_dt(dt);
         if (rc) return rc;
         if (mode == VX3MI_MODE_TILED && !plans[tileSel].ok) {
@@ -1075,6 +1241,7 @@ template <typename T> struct Batch : BatchBase {
             set_error("force fields, material removal, detach and signals run in the plain and flat modes only");
             return VX3MI_E_ARG;
         }
+        if (d.pairRec) HIP_TRY(hipMemsetAsync(d.pairN, 0, sizeof(int), stream));  // (not inside a graph)
         HIP_TRY(hipEventRecord(ev0, stream));
         launches = 0;
         lastMode = mode;
@@ -1082,14 +1249,11 @@ template <typename T> struct Batch : BatchBase {
             for (int s = 0; s < nsteps; s++) {
                 for (int r = 0; r < R; r++) {
                     const RobotParams &p = hrp[r];
-                    if (p.scount) k_col_insert<T><<<(p.scount + 255) / 256, 256, 0, stream>>>(d, p.soff, p.scount, parity);
-                    if (p.scount)
-                        k_col_find<T><<<(int)((27L * p.scount + 255) / 256), 256, 0, stream>>>(d, p.soff, p.scount, parity);
+                    if (p.scount) launch_collisions(p.soff, p.scount, r, 1, parity, 1);
                     if (p.lcount) k_links<T><<<(p.lcount + 255) / 256, 256, 0, stream>>>(d, p.loff, p.lcount, parity);
-                    if (p.scount) k_col_contact<T><<<(p.scount + 255) / 256, 256, 0, stream>>>(d, p.soff, p.scount, parity);
                     if (anyForceField) k_force_field<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
                     k_voxels<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
-                    launches += 2 + (p.scount ? 3 : 0) + (anyForceField ? 1 : 0);
+                    launches += 2 + (p.scount ? kCollisionLaunches : 0) + (anyForceField ? 1 : 0);
                 }
                 // the end-of-step phases for the whole batch (each touches only its
                 // robot's voxels and links)
@@ -1211,6 +1375,16 @@ template <typename T> struct Batch : BatchBase {
         HIP_TRY(hipMemcpy(d.pq1, a1.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.mo0, b0.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(d.mo1, b1.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
+        return mark_stale();
+    }
+
+    // the candidate lists of every robot are rebuilt by the next step (after the
+    // state was replaced)
+    int mark_stale() {
+        if (!anyCollisions) return VX3MI_OK;
+        k_mark_stale<T><<<(R + 255) / 256, 256, 0, stream>>>(d_rt, R, d.anyStale);
+        HIP_TRY(hipGetLastError());
+        HIP_TRY(hipStreamSynchronize(stream));
         return VX3MI_OK;
     }
 
@@ -1225,11 +1399,8 @@ template <typename T> struct Batch : BatchBase {
                                  {d.mo1, sizeof(T4<T>) * V}, {d.ls0, sizeof(T4<T>) * L}, {d.ls1, sizeof(T4<T>) * L},
                                  {d.ls2, sizeof(T4<T>) * L}, {d.lf0, sizeof(T4<T>) * L}, {d.lf1, sizeof(T4<T>) * L},
                                  {d.lf2, sizeof(T4<T>) * L}, {d_rt, sizeof(RobotTime) * R}};
-        if (anyCollisions) {
-            bufs.push_back({d.ccnt[0], sizeof(int) * (size_t)tableBuckets});
-            bufs.push_back({d.ccnt[1], sizeof(int) * (size_t)tableBuckets});
-            bufs.push_back({d.contactW, sizeof(T4<T>) * V});
-        }
+        // (collision lists need no snapshot: they are rebuilt after each restore,
+        // and a step's partners do not depend on when the lists were built)
         if (anySignals) bufs.push_back({d.sig, sizeof(VoxelSignal) * (size_t)V});
         if (anyRemoval || anyDetach) {
             bufs.push_back({d.vlinks, sizeof(int) * 6 * (size_t)V});
@@ -1251,8 +1422,9 @@ template <typename T> struct Batch : BatchBase {
                 off += (b.bytes + 255) & ~(size_t)255;
             }
             HIP_TRY(hipStreamSynchronize(stream));
-            return VX3MI_OK;
+            return save ? VX3MI_OK : mark_stale();
         };
+        const std::vector<float> curDt0 = curDt;  // the dt the snapshot holds
         const int parity0 = parity, tile0 = tileSel, threads0 = persistThreads, mode0 = autoMode;
         const int modes[kCandidates] = {VX3MI_MODE_TILED, VX3MI_MODE_TILED, VX3MI_MODE_PERSISTENT,
                                         VX3MI_MODE_PERSISTENT, VX3MI_MODE_PERSISTENT, VX3MI_MODE_FLAT};
@@ -1279,6 +1451,7 @@ template <typename T> struct Batch : BatchBase {
         }
         if (rc == VX3MI_DIVERGED) rc = VX3MI_OK;
         (void)hipFree(snap);
+        curDt = curDt0;
         if (rc != VX3MI_OK) {
             tileSel = tile0; persistThreads = threads0; autoMode = mode0; parity = parity0;
             return rc;
@@ -1292,6 +1465,50 @@ template <typename T> struct Batch : BatchBase {
         choice[1] = tileSel;
         choice[2] = persistThreads;
         return VX3MI_OK;
+    }
+
+    int2 *pairBuf = nullptr;
+    int *pairNBuf = nullptr;
+    int pairAlloc = 0;
+    int record_contacts(int cap) override {
+        if (cap < 0) { set_error("negative capacity"); return VX3MI_E_ARG; }
+        if (cap > 0 && !anyCollisions) { set_error("no robot of the batch has collisions"); return VX3MI_E_ARG; }
+        if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }  // it holds the launch arguments
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (cap > pairAlloc) {
+            int rc;
+            if ((rc = alloc(&pairBuf, cap)) || (!pairNBuf && (rc = alloc(&pairNBuf, 1)))) return rc;
+            pairAlloc = cap;
+        }
+        d.pairRec = cap ? pairBuf : nullptr;
+        d.pairN = cap ? pairNBuf : nullptr;
+        d.pairCap = cap;
+        if (cap) HIP_TRY(hipMemset(pairNBuf, 0, sizeof(int)));
+        return VX3MI_OK;
+    }
+
+    int contact_pairs(int robot, int *out, int cap) override {
+        if (!d.pairRec) { set_error("contacts are not being recorded"); return VX3MI_E_ARG; }
+        HIP_TRY(hipStreamSynchronize(stream));
+        int n = 0;
+        HIP_TRY(hipMemcpy(&n, d.pairN, sizeof(int), hipMemcpyDeviceToHost));
+        if (n > d.pairCap) { set_error("more contact pairs than the record holds"); return VX3MI_E_LIMIT; }
+        std::vector<int2> h((size_t)std::max(n, 1));
+        if (n) HIP_TRY(hipMemcpy(h.data(), d.pairRec, sizeof(int2) * n, hipMemcpyDeviceToHost));
+        int v0, v1;
+        range(robot, v0, v1);
+        std::vector<std::pair<int, int>> pr;
+        for (int k = 0; k < n; k++) {
+            const int a = h[k].x, b = h[k].y;
+            if (a < v0 || a >= v1) continue;
+            pr.emplace_back(std::min(a, b) - v0, std::max(a, b) - v0);
+        }
+        std::sort(pr.begin(), pr.end());
+        for (int k = 0; k < (int)pr.size() && k < cap; k++) {
+            out[2 * k] = pr[k].first;
+            out[2 * k + 1] = pr[k].second;
+        }
+        return (int)pr.size();
     }
 
     int configure(int mode, int tile, int threads) override {

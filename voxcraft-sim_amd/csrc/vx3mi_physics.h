@@ -195,6 +195,13 @@ constexpr int VF_FLOOR_ENABLED = 1 << 2;
 constexpr int VF_FLOOR_STATIC_FRICTION = 1 << 3;
 constexpr int LF_LOCAL_VELOCITY_VALID = 1;
 constexpr int LF_SMALL_ANGLE = 2;
+// branches the last update took, kept in the flag word for comparisons (no
+// equation reads them): ToRotationVector of angle1 / angle2 by acos (above the
+// 2.4e-3 threshold), FromAngleToPosX by its exact form (above 1.732e-2)
+constexpr int LF_ROTVEC_ACOS1 = 4;
+constexpr int LF_ROTVEC_ACOS2 = 8;
+constexpr int LF_POSX_EXACT = 16;
+constexpr int LF_DECISIONS = LF_ROTVEC_ACOS1 | LF_ROTVEC_ACOS2 | LF_POSX_EXACT;
 
 // ---- discrete decisions of the piecewise equations ------------------------------
 // Each branch point of the reference's equations where a continuous quantity is
@@ -339,8 +346,10 @@ template <typename T> VX_HD T qdiv(T a, T b) {
     else return a / b;
 }
 
-// ToRotationVector, VX3_Quat3D.h:347-359
-template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q) {
+// ToRotationVector, VX3_Quat3D.h:347-359; *acos_branch (when given): 1 when the
+// angle is above the threshold of the square-root form
+template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q, int *acos_branch = nullptr) {
+    if (acos_branch) *acos_branch = 0;
     if constexpr (std::is_same<T, double>::value) {
         if (q.w >= 1.0 || q.w <= -1.0) return v3<T>(0, 0, 0);
         double sl = 1.0 - q.w * q.w;
@@ -350,6 +359,7 @@ template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q) {
             double s = sqrt((2 - 2 * q.w) / sl);
             return v3<T>(q.x * 2.0 * s, q.y * 2.0 * s, q.z * 2.0 * s);
         }
+        if (acos_branch) *acos_branch = 1;
         double a = vxm_acos(q.w), d = sqrt(sl);
         return v3<T>(q.x * 2.0 * a / d, q.y * 2.0 * a / d, q.z * 2.0 * a / d);
     } else {
@@ -361,6 +371,7 @@ template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q) {
         // 2 sqrt((2 - 2w) / (1 - w^2)) = 2 sqrt(2) / sqrt(1 + w)
         if (series) k = 2.8284271247461903f * frsqrt(1.0f + q.w);
         else {
+            if (acos_branch) *acos_branch = 1;
             float s = fsqrt(s2);
             k = 2.0f * atan2f(s, q.w) * frcp(s);
         }
@@ -400,9 +411,10 @@ template <typename T> VX_HD Q4<T> from_rotvec(V3<T> in) {
 }
 
 // FromAngleToPosX, VX3_Quat3D.h:384-432 (the target starts as the identity,
-// VX3_Link.cu:103)
-template <typename T> VX_HD Q4<T> from_angle_to_posx(V3<T> rf) {
+// VX3_Link.cu:103); *exact (when given): 1 unless the first-order form is taken
+template <typename T> VX_HD Q4<T> from_angle_to_posx(V3<T> rf, int *exact = nullptr) {
     Q4<T> q = {1, 0, 0, 0};
+    if (exact) *exact = 0;
     if (rf.x == 0 && rf.y == 0 && rf.z == 0) return q;
     T yx, zx;
     if constexpr (std::is_same<T, float>::value) {
@@ -423,6 +435,7 @@ template <typename T> VX_HD Q4<T> from_angle_to_posx(V3<T> rf) {
         q.w = (T)1 + (T)0.5 * (-q.y * q.y - q.z * q.z);
         return q;
     }
+    if (exact) *exact = 1;
     if constexpr (std::is_same<T, float>::value) {
         // the same rotation without trigonometry: for the unit vector n = rf/l
         // at angle theta from +x, cos(theta/2) = sqrt((1 + n.x)/2) and the axis
@@ -609,11 +622,12 @@ VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T>
     VX3MI_TEST_DECISION(DEC_SMALL_ANGLE, smallAngle);
     // a change of formulation invalidates the local velocity (:117-131)
     if (smallAngle != wasSmall) flags &= ~LF_LOCAL_VELOCITY_VALID;
+    int exact = 0, acos1, acos2;
     if (smallAngle) {
         if constexpr (std::is_same<T, double>::value) pos2.x -= restLen;
         else pos2.x = ext;
     } else {
-        angle1 = from_angle_to_posx(pos2);
+        angle1 = from_angle_to_posx(pos2, &exact);
         angle2 = qmul(angle1, angle2);
         if constexpr (std::is_same<T, double>::value) {
             pos2 = v3<T>(len(pos2) - restLen, 0, 0);
@@ -624,9 +638,10 @@ VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T>
             pos2 = v3<T>(num * frcp(l + restLen), 0, 0);
         }
     }
-    V3<T> a1v = to_rotvec(angle1);
-    V3<T> a2v = to_rotvec(angle2);
-    flags = smallAngle ? (flags | LF_SMALL_ANGLE) : (flags & ~LF_SMALL_ANGLE);
+    V3<T> a1v = to_rotvec(angle1, &acos1);
+    V3<T> a2v = to_rotvec(angle2, &acos2);
+    flags = (flags & ~(LF_SMALL_ANGLE | LF_DECISIONS)) | (smallAngle ? LF_SMALL_ANGLE : 0) |
+            (acos1 ? LF_ROTVEC_ACOS1 : 0) | (acos2 ? LF_ROTVEC_ACOS2 : 0) | (exact ? LF_POSX_EXACT : 0);
     st.pos2 = pos2;
     st.a1v = a1v;
     st.a2v = a2v;

@@ -165,7 +165,9 @@ def lib():
         "vx3mi_batch_hits": (C.c_int, [vp, ip]),
         "vx3mi_model_cilia": (C.c_int, [vp, dp, dp]),
         "vx3mi_model_material": (C.c_int, [vp, C.c_int, C.POINTER(_Material)]),
-        "vx3mi_batch_collision_overflow": (C.c_int, [vp, ip]),
+        "vx3mi_batch_record_contacts": (C.c_int, [vp, C.c_int]),
+        "vx3mi_batch_listings": (C.c_int, [vp, ip]),
+        "vx3mi_batch_contact_pairs": (C.c_int, [vp, C.c_int, ip, C.c_int]),
         "vx3mi_batch_time": (C.c_int, [vp, dp]),
         "vx3mi_batch_diverged": (C.c_int, [vp, ip]),
         "vx3mi_center_of_mass": (C.c_int, [vp, dp]),
@@ -425,11 +427,27 @@ class Batch:
         _check(lib().vx3mi_get_voxel_flags(self._h, int(robot), _ptr(f, C.c_int)))
         return f
 
-    def collision_overflow(self) -> np.ndarray:
-        """Per robot: 1 when the collision grid dropped a contact (see the header)."""
+    def record_contacts(self, cap: int = 1 << 20):
+        """Record the pairs in contact during each later step call (cap pairs per
+        call; 0 stops recording)."""
+        _check(lib().vx3mi_batch_record_contacts(self._h, int(cap)))
+
+    def listings(self) -> np.ndarray:
+        """Per robot: the steps whose collision phase rebuilt the candidate lists."""
         out = np.zeros(self.nrobot, dtype=np.int32)
-        _check(min(0, lib().vx3mi_batch_collision_overflow(self._h, _ptr(out, C.c_int))))
+        _check(lib().vx3mi_batch_listings(self._h, _ptr(out, C.c_int)))
         return out
+
+    def contact_pairs(self, robot: int = -1) -> np.ndarray:
+        """(n, 2) voxel index pairs in contact in the last step call, smaller
+        index first, sorted (robot-local indices, batch indices for robot -1)."""
+        n = lib().vx3mi_batch_contact_pairs(self._h, int(robot), None, 0)
+        if n < 0:
+            _check(n)
+        out = np.zeros((max(n, 1), 2), dtype=np.int32)
+        if n:
+            _check(min(0, lib().vx3mi_batch_contact_pairs(self._h, int(robot), _ptr(out, C.c_int), n)))
+        return out[:n]
 
     def link_flags(self, robot: int = -1):
         """(flags, axial strain) per link: bit 0 local velocity valid, bit 1 small angle."""
