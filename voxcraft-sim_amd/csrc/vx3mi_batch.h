@@ -48,7 +48,7 @@ struct RobotTime {
     int stale[2];        // candidate lists to be rebuilt before the contact phase of the step of this parity
     int collisionCount;  // target / non-target contacts summed over the steps (VX3_VoxelyzeKernel.cu:711-726)
     int listings;        // steps whose collision phase rebuilt the robot's candidate lists
-    int pad;
+    int divergedNow;     // a link of this step passed the divergence strain (the voxel update sets diverged)
 };
 
 #define HIP_TRY(expr)                                                                          \

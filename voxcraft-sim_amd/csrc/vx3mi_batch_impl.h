@@ -116,8 +116,11 @@ template <typename T> __device__ __forceinline__ void link_update(const Dev<T> &
     a.lf0[l] = T4<T>{o.fN.x, o.fN.y, o.fN.z, o.mN.x};
     a.lf1[l] = T4<T>{o.mN.y, o.mN.z, o.fP.x, o.fP.y};
     a.lf2[l] = T4<T>{o.fP.z, o.mP.x, o.mP.y, o.mP.z};
-    // divergence (VX3_VoxelyzeKernel.cu:271-280): any link, not one sampled link
-    if (o.strain > 100.0f) atomicOr(&rt->diverged, 1);
+    // divergence (VX3_VoxelyzeKernel.cu:271-280): any link, not one sampled link;
+    // kept apart from `diverged` until the voxel update, so that work running
+    // beside the links in the same step (the contact phase) sees the flag of
+    // the earlier steps only
+    if (o.strain > 100.0f) atomicOr(&rt->divergedNow, 1);
 }
 
 template <typename T> __device__ __forceinline__ void col_track(const Dev<T> &a, int v, const V3<T> &u, RobotTime *rt,
@@ -127,6 +130,10 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
     const int r = a.vrob[v];
     RobotTime *rt = a.rt + r;
     if (rt->diverged) return;
+    if (rt->divergedNow) {  // doTimeStep returns before the voxels move (:271-280)
+        rt->diverged = 1;
+        return;
+    }
     const RobotParams &rp = a.rp[r];
     const float dt = rt->dt;
     const double tnext = rt->time[parity] + (double)dt;  // currentTime += dt (:352)
@@ -370,11 +377,13 @@ template <typename T, typename F> __device__ __forceinline__ void col_grid_each(
     }
 }
 
-// rebuild pass 4: the candidate list of surface voxel s, ascending: surface
-// voxels of its robot in the 27 cells around it within listR, not both fixed,
-// not directly linked unless the robot can lose links or voxels (the contact
-// phase tests the current links then); more than kMaxCand: ccount = their
-// number and the voxel searches the grid itself
+// rebuild pass 4: the candidate list of surface voxel s, ascending (voxel ids:
+// the surface list of a robot is in voxel order, so this is the surface order
+// of the reference's pair loop): surface voxels of its robot in the 27 cells
+// around it within listR, not both fixed, not directly linked unless the robot
+// can lose links or voxels (the contact phase tests the current links then);
+// more than kMaxCand: ccount = their number and the voxel searches the grid
+// itself
 template <typename T> __device__ __forceinline__ void col_list(const Dev<T> &a, int s, int parity) {
     const int v = a.surf[s];
     const int r = a.vrob[v];
@@ -396,8 +405,8 @@ template <typename T> __device__ __forceinline__ void col_list(const Dev<T> &a, 
         if (d.x * d.x + d.y * d.y + d.z * d.z > R * R) return;
         if (n < kMaxCand) {  // insertion in ascending order
             int k = n;
-            while (k > 0 && list[k - 1] > sj) { list[k] = list[k - 1]; k--; }
-            list[k] = sj;
+            while (k > 0 && list[k - 1] > w) { list[k] = list[k - 1]; k--; }
+            list[k] = w;
         }
         n++;
     });
@@ -407,19 +416,26 @@ template <typename T> __device__ __forceinline__ void col_list(const Dev<T> &a, 
 // The contact force of surface voxel s from the partners inside its envelope,
 // summed in ascending partner index (the order in which the serialized pair loop
 // of the reference accumulates this voxel's contact force), with the target
-// hits, the forced signal, and the pair record when one is kept.  Clears this
-// step's listing flags (the rebuild passes have run).
-template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity) {
+// hits, the forced signal, and the pair record when one is kept.  kContactLanes
+// consecutive work-items (lane 0..) share a voxel: each evaluates every
+// kContactLanes-th candidate, and the forces are added in list order.  Clears
+// this step's listing flags (the rebuild passes have run).
+constexpr int kContactLanes = 8;
+template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity, int lane) {
     const int v = a.surf[s];
     const int r = a.vrob[v];
     RobotTime *rt = a.rt + r;
     const RobotParams &rp = a.rp[r];
-    if (s == rp.soff && rt->stale[parity]) {
+    if (lane == 0 && s == rp.soff && rt->stale[parity]) {
         rt->listings += 1;
         rt->stale[parity] = 0;
     }
-    if (s == 0 && a.anyStale) a.anyStale[parity] = 0;
-    if (!rp.collisions || rt->diverged || !col_active(a, v)) return;
+    if (lane == 0 && s == 0 && a.anyStale) a.anyStale[parity] = 0;
+    if (!rp.collisions || rt->diverged) return;
+    if (!col_active(a, v)) {
+        if (lane == 0) a.contactW[v] = T4<T>{0, 0, 0, 0};
+        return;
+    }
     const VoxelMat &mi = a.vmats[a.vmat[v]];
     const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v], mi0 = a.mo0[v];
     const int pkv = a.lat[v];
@@ -427,39 +443,50 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
     const T bsaI = base_size_average<T>(mi.nomSize, (float)qi1.w, mi.alphaCTE);
     const V3<T> velI = V3<T>{mi0.x, mi0.y, mi0.z} * (T)mi.massInv;
     const int ti = a.vtarget ? a.vtarget[v] : 0;
-    bool hitTarget = false;
-    V3<T> f = {0, 0, 0};
-    // partner sj when it is in contact now (links: whether to test the current
-    // links; lists leave out linked neighbours unless the robot can lose links)
-    auto partner = [&](int sj, bool links) -> bool {
-        const int w = a.surf[sj];
-        if (!col_active(a, w)) return false;
+    int hit = 0;
+    // partner w when it is in contact now, its force in fw (links: whether to
+    // test the current links; lists leave out linked neighbours unless the
+    // robot can lose links)
+    auto partner = [&](int w, bool links, V3<T> &fw) -> int {
+        if (!col_active(a, w)) return 0;
         const int3 o = lat_offset(pkv, a.lat[w]);
-        if (links && col_linked(a, v, o.x, o.y, o.z)) return false;
+        if (links && col_linked(a, v, o.x, o.y, o.z)) return 0;
         const VoxelMat &mj = a.vmats[a.vmat[w]];
-        if (mi.fixed && mj.fixed) return false;
+        if (mi.fixed && mj.fixed) return 0;
         const T4<T> qj0 = a.pq0[w];
         const V3<T> diff = col_diff<T>(qi0, qj0, o, L);
         const T bsaJ = base_size_average<T>(mj.nomSize, (float)a.pq1[w].w, mj.alphaCTE);
-        if (!in_contact_envelope(diff, bsaI, bsaJ)) return false;
+        if (!in_contact_envelope(diff, bsaI, bsaJ)) return 0;
         if (a.vtarget && a.vtarget[w] != ti) {  // a target touching a non-target (:711-726)
-            if (s < sj) atomicAdd(&rt->collisionCount, 1);
-            hitTarget = true;
+            if (v < w) atomicAdd(&rt->collisionCount, 1);
+            hit = 1;
         }
-        if (a.pairRec && s < sj) {
+        if (a.pairRec && v < w) {
             const int k = atomicAdd(a.pairN, 1);
             if (k < a.pairCap) a.pairRec[k] = int2{v, w};
         }
         const T4<T> mj0 = a.mo0[w];
         const V3<T> velJ = V3<T>{mj0.x, mj0.y, mj0.z} * (T)mj.massInv;
-        f = f + contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
-        return true;
+        fw = contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
+        return 1;
     };
+    V3<T> f = {0, 0, 0};
     const int nc = a.ccount[s];
     if (nc <= kMaxCand) {
         const int *list = a.clist + (long)s * kMaxCand;
-        for (int k = 0; k < nc; k++) partner(list[k], rp.listLinked != 0);
-    } else {
+        const bool links = rp.listLinked != 0;
+        for (int k0 = 0; k0 < nc; k0 += kContactLanes) {
+            V3<T> fk = {0, 0, 0};
+            int in = 0;
+            if (k0 + lane < nc) in = partner(list[k0 + lane], links, fk);
+            for (int j = 0; j < kContactLanes && k0 + j < nc; j++) {  // list order
+                const int inj = __shfl(in, j, kContactLanes);
+                const V3<T> fj = {__shfl(fk.x, j, kContactLanes), __shfl(fk.y, j, kContactLanes),
+                                  __shfl(fk.z, j, kContactLanes)};
+                if (inj) f = f + fj;
+            }
+        }
+    } else if (lane == 0) {
         // more candidates than a list holds: ascending partners straight from the
         // grid, which is rebuilt every step while this lasts
         rt->stale[parity ^ 1] = 1;
@@ -471,12 +498,15 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
                 if (sj > prev && sj < next) next = sj;
             });
             if (next == INT_MAX) break;
-            partner(next, true);
+            V3<T> fw;
+            if (partner(a.surf[next], true, fw)) f = f + fw;
             prev = next;
         }
     }
+    for (int o = 1; o < kContactLanes; o <<= 1) hit |= __shfl_xor(hit, o, kContactLanes);
+    if (lane != 0) return;
     // with signals a non-target voxel touching a target receives a forced 100
-    if (hitTarget && !ti && a.sig && rp.signals) {
+    if (hit && !ti && a.sig && rp.signals) {
         VoxelSignal sg = a.sig[v];
         signal_receive(sg, a.sigMat[v], 100, rt->time[parity], true);
         a.sig[v] = sg;
@@ -503,17 +533,16 @@ template <typename T> __global__ __launch_bounds__(256) void k_col_count(Dev<T> 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) col_count(a, s0 + i, parity);
 }
+// prefix sum and scatter, one workgroup per robot
 template <typename T> __global__ __launch_bounds__(1024) void k_col_scan(Dev<T> a, int r0, int parity, int all) {
     __shared__ int wsum[1024];
     if (!all && !a.anyStale[parity]) return;
     const int r = r0 + blockIdx.x;
-    if (!a.rt[r].stale[parity] || a.rt[r].diverged || a.rp[r].scount == 0) return;
+    const RobotParams &rp = a.rp[r];
+    if (!a.rt[r].stale[parity] || a.rt[r].diverged || rp.scount == 0) return;
     col_scan(a, r, wsum);
-}
-template <typename T> __global__ __launch_bounds__(256) void k_col_scatter(Dev<T> a, int s0, int n, int parity, int all) {
-    if (!all && !a.anyStale[parity]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) col_scatter(a, s0 + i, parity);
+    __syncthreads();
+    for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_scatter(a, i, parity);
 }
 template <typename T> __global__ __launch_bounds__(256) void k_col_list(Dev<T> a, int s0, int n, int parity, int all) {
     if (!all && !a.anyStale[parity]) return;
@@ -521,8 +550,22 @@ template <typename T> __global__ __launch_bounds__(256) void k_col_list(Dev<T> a
     if (i < n) col_list(a, s0 + i, parity);
 }
 template <typename T> __global__ __launch_bounds__(256) void k_col_contact(Dev<T> a, int s0, int n, int parity) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) col_contact(a, s0 + i, parity);
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;  // kContactLanes per surface voxel
+    if (i < n * kContactLanes) col_contact(a, s0 + i / kContactLanes, parity, i % kContactLanes);
+}
+
+// the contact phase and the link updates of one step side by side in one launch
+// (they read the state entering the step and write disjoint outputs): blocks
+// [0, nbl) update links, the rest compute contact forces
+template <typename T>
+__global__ __launch_bounds__(256) void k_contact_links(Dev<T> a, int nl, int nbl, int ns, int parity) {
+    if ((int)blockIdx.x < nbl) {
+        const int l = blockIdx.x * blockDim.x + threadIdx.x;
+        if (l < nl) link_update(a, l, parity);
+    } else {
+        const int i = (blockIdx.x - nbl) * blockDim.x + threadIdx.x;
+        if (i < ns * kContactLanes) col_contact(a, i / kContactLanes, parity, i % kContactLanes);
+    }
 }
 
 template <typename T> __global__ __launch_bounds__(256) void k_links(Dev<T> a, int l0, int n, int parity) {
@@ -624,7 +667,8 @@ template <typename T, int NT> __global__ __launch_bounds__(NT) void k_persistent
                 for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += NT) col_list(a, i, parity);
             }
             __syncthreads();
-            for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += NT) col_contact(a, i, parity);
+            for (int i = threadIdx.x; i < rp.scount * kContactLanes; i += NT)
+                col_contact(a, rp.soff + i / kContactLanes, parity, i % kContactLanes);
         }
         for (int l = rp.loff + threadIdx.x; l < rp.loff + rp.lcount; l += NT) link_update(a, l, parity);
         __syncthreads();
@@ -1195,22 +1239,29 @@ template <typename T> struct Batch : BatchBase {
     }
 
     // the collision phase of one step for surface voxels [s0, s0 + n) (robots
-    // [r0, r0 + nr)): the listing passes (which return at once unless some
-    // robot's lists are stale; all: check the robots' own flags only), then
-    // the contact forces
-    void launch_collisions(int s0, int n, int r0, int nr, int parity, int all) {
+    // [r0, r0 + nr)): the listing passes -- count, prefix sum + scatter, lists;
+    // they return at once unless some robot's lists are stale (all: check the
+    // robots' own flags only) -- then the contact forces
+    void launch_listing(int s0, int n, int r0, int nr, int parity, int all) {
         const int g = (n + 255) / 256;
         k_col_count<T><<<g, 256, 0, stream>>>(d, s0, n, parity, all);
         k_col_scan<T><<<nr, 1024, 0, stream>>>(d, r0, parity, all);
-        k_col_scatter<T><<<g, 256, 0, stream>>>(d, s0, n, parity, all);
         k_col_list<T><<<g, 256, 0, stream>>>(d, s0, n, parity, all);
-        k_col_contact<T><<<g, 256, 0, stream>>>(d, s0, n, parity);
     }
-    static constexpr int kCollisionLaunches = 5;
+    void launch_collisions(int s0, int n, int r0, int nr, int parity, int all) {
+        launch_listing(s0, n, r0, nr, parity, all);
+        k_col_contact<T><<<(n * kContactLanes + 255) / 256, 256, 0, stream>>>(d, s0, n, parity);
+    }
+    static constexpr int kCollisionLaunches = 4;  // plain mode; the flat step runs the contacts beside the links
 
     void launch_flat_step(int parity) {
-        if (anyCollisions) launch_collisions(0, S, 0, R, parity, 0);
-        if (L) k_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, 0, L, parity);
+        if (anyCollisions) {
+            launch_listing(0, S, 0, R, parity, 0);
+            const int nbl = (L + 255) / 256, nbc = (S * kContactLanes + 255) / 256;
+            k_contact_links<T><<<nbl + nbc, 256, 0, stream>>>(d, L, nbl, S, parity);
+        } else if (L) {
+            k_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, 0, L, parity);
+        }
         if (anyForceField) k_force_field<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         k_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         if (anyDetach) k_detach<T><<<(d.nDetCand + 255) / 256, 256, 0, stream>>>(d);
@@ -1218,7 +1269,7 @@ template <typename T> struct Batch : BatchBase {
         if (anyRemoval) k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
     }
     int kernels_per_flat_step() const {
-        return 2 + (anyCollisions ? kCollisionLaunches : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0) +
+        return 2 + (anyCollisions ? kCollisionLaunches - 1 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0) +
                (anyDetach ? 1 : 0);
     }
 

@@ -102,7 +102,7 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINW))
         tr.prevDt[0][tid] = t.prevDt[0];
         tr.prevDt[1][tid] = t.prevDt[1];
         tr.dt[tid] = t.dt;
-        tr.diverged[tid] = t.diverged;
+        tr.diverged[tid] = t.diverged | t.divergedNow;
         tr.steps[tid] = t.steps;
         tr.tempAmp[tid] = p.tempAmp;
         tr.tempPeriod[tid] = std::is_same<T, float>::value ? p.tempInvPeriod : p.tempPeriod;
