@@ -7,12 +7,16 @@ largest face-connected component, two materials, random actuation phases,
 FP32 state, collisions off, seed 0.  A *step* is one timestep of every robot
 of the batch (VX3_VoxelyzeKernel::doTimeStep for all 1024 simulations).
 
-Printed: one JSON line with voxel-steps/s of the batched persistent kernel
+Printed: one JSON line with voxel-steps/s of the autotuned batched kernel
 (value), the same workload on the plain path (one robot per launch sequence,
 as the reference launches per simulation) and the ratio, the relative error
-of the FP32 run against the FP64 oracle on a sample of robots, the HBM
-roofline of the dominant kernel, and the reference CPU Voxelyze timed on a
-bounded sample of the same robots on this host.
+of the FP32 run against the FP64 oracle on a sample of robots, the roofline of
+the dominant kernel (its binding roof, VALU issue, from counted instructions
+and the timed launch; its measured HBM bytes beside it), the same workload
+with collisions on, per-configuration parity and config-1 HBM traffic, the
+adjoint's cost, and the reference CPU Voxelyze timed on a bounded sample of
+the same robots on this host.  Counter passes run this script as a child
+under rocprofv3 (--no-counters skips them).
 """
 from __future__ import annotations
 
@@ -44,19 +48,122 @@ def algorithmic_bytes_per_voxel_step(nv: int, nl: int, fp64: bool = False) -> fl
     return 144.0 + 100.0 * nl / nv
 
 
-def measured_traffic(kernel: str, robots: int):
-    """HBM bytes per launch of the tiled kernel on this workload, from the
-    committed counter summary (FETCH_SIZE + WRITE_SIZE of one launch, calibrated
-    on a copy of known size); None for another kernel or workload.  The tiled
-    kernel reads and writes the batch state once per launch, so the figure does
-    not depend on the steps per launch."""
-    path = os.path.join(ROOT, "profiles", "k_tile_fp32_batched_1024robots_counters.md")
-    if kernel != "k_tile<float, 0>" or robots != 1024 or not os.path.exists(path):
+# VALU issue roof: every compute unit has four SIMDs, each issues one wave64
+# VALU instruction per four cycles (16 lanes wide; packed FP32 is not used);
+# MI355X: 256 compute units at a peak engine clock of 2.4 GHz
+VALU_ISSUE_PEAK_G = 256 * 4 * 2.4 / 4  # G wave-instructions per second
+COPY_PROBE_MB = 256
+
+
+def rocprof_counters(counters, child_args, timeout=120.0):
+    """Runs this script as a child under `rocprofv3 --pmc <counters>` (one pass)
+    and returns {kernel name: [per-dispatch {counter: value}]} from its CSV, or
+    None when rocprofv3 is missing or the run fails."""
+    import csv
+    import shutil
+    import tempfile
+
+    exe = shutil.which("rocprofv3") or ("/opt/rocm/bin/rocprofv3" if os.path.exists("/opt/rocm/bin/rocprofv3") else None)
+    if exe is None:
         return None
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return float(line.split(":")[1])
-    return None
+    with tempfile.TemporaryDirectory(dir=os.environ.get("TMPDIR", "/tmp")) as d:
+        cmd = [exe, "--pmc", *counters, "--output-format", "csv", "-d", d, "-o", "run", "--",
+               sys.executable, os.path.abspath(__file__), "--counters-child", *child_args]
+        try:
+            r = subprocess.run(cmd, capture_output=True, timeout=timeout, cwd=d)
+        except subprocess.TimeoutExpired:
+            log(f"bench: rocprofv3 --pmc {' '.join(counters)} timed out")
+            return None
+        files = [os.path.join(dp, f) for dp, _, fs in os.walk(d) for f in fs if f.endswith("counter_collection.csv")]
+        if r.returncode != 0 or not files:
+            log(f"bench: rocprofv3 --pmc {' '.join(counters)} failed (rc {r.returncode})")
+            return None
+        out = {}
+        disp = {}
+        with open(files[0]) as f:
+            for row in csv.DictReader(f):
+                key = (row["Dispatch_Id"], row["Kernel_Name"])
+                disp.setdefault(key, {})
+                disp[key][row["Counter_Name"]] = disp[key].get(row["Counter_Name"], 0.0) + float(row["Counter_Value"])
+        for (did, name), vals in sorted(disp.items(), key=lambda kv: int(kv[0][0])):
+            out.setdefault(name, []).append(vals)
+        return out
+
+
+def kernel_rows(rows, prefix):
+    return [v for k, vs in rows.items() if k.split("(")[0].replace("void ", "").startswith(prefix) for v in vs]
+
+
+def copy_calibration(rows, counter):
+    """Bytes per counted KiB of `counter`, from the largest copy dispatch (the
+    COPY_PROBE_MB device-to-device copy at the child's end)."""
+    copies = [v[counter] for k, vs in rows.items() if "copyBuffer" in k for v in vs]
+    if not copies or max(copies) <= 0:
+        return None
+    return COPY_PROBE_MB * 2**20 / max(copies)
+
+
+def counter_run(kernel_prefix, child_args):
+    """HBM bytes (FETCH_SIZE + WRITE_SIZE, calibrated on the copy) and VALU wave
+    instructions per dispatch of the kernels whose names start with kernel_prefix,
+    from three counter passes; values are summed over those dispatches and
+    divided by their number."""
+    res = {}
+    for counters in (["FETCH_SIZE"], ["WRITE_SIZE"], ["SQ_INSTS_VALU", "SQ_WAVES"]):
+        rows = rocprof_counters(counters, child_args)
+        if rows is None:
+            return None
+        ks = kernel_rows(rows, kernel_prefix)
+        if not ks:
+            return None
+        for c in counters:
+            res[c] = sum(v[c] for v in ks) / len(ks)
+            if c in ("FETCH_SIZE", "WRITE_SIZE"):
+                f = copy_calibration(rows, c)
+                if f is None:
+                    return None
+                res[c + "_bytes"] = res[c] * f
+                res[c + "_calibration"] = f
+        res["dispatches"] = len(ks)
+    res["traffic"] = res["FETCH_SIZE_bytes"] + res["WRITE_SIZE_bytes"]
+    return res
+
+
+def copy_probe(mb: int = COPY_PROBE_MB) -> None:
+    """One device-to-device hipMemcpy (the runtime's copy kernel) of a known size:
+    the reference dispatch that calibrates the byte counters."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = C.c_size_t(mb * 2**20)
+    src, dst = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(src), nbytes) == 0 and hip.hipMalloc(C.byref(dst), nbytes) == 0
+    assert hip.hipMemset(src, 1, nbytes) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(dst, src, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+
+
+def counters_child(args) -> int:
+    """The process the counter passes profile: the batched workload (or config 1)
+    in the given launch configuration, one launch of args.steps steps after one
+    warm-up launch, then the calibration copy."""
+    if args.workload == "config1":
+        b = vx3mi.Batch([vx3mi.Model.from_problem(configs.config1())], vx3mi.FP32)
+        b.step(args.warmup, mode=vx3mi.MODE_FLAT)
+        b.step(args.steps, mode=vx3mi.MODE_FLAT)
+    else:
+        probs = configs.config2(seed=0, n=args.robots, collisions=args.collisions)
+        b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP32)
+        b.configure(args.child_mode, args.child_tile, args.child_threads)
+        b.step(args.steps, mode=vx3mi.MODE_AUTO)  # warm-up launch (code load)
+        b.step(args.steps, mode=vx3mi.MODE_AUTO)
+    b.sync()
+    copy_probe()
+    b.close()
+    return 0
 
 
 def log(msg: str) -> None:
@@ -276,6 +383,95 @@ def config1_only(args) -> int:
     return 0
 
 
+def launch_roofline(mode, tile, threads, nv, nl, steps_per_launch, launch_s, args, collisions):
+    """The dominant kernel's roofline.  The tiled kernel keeps the batch state in
+    registers and LDS for a whole launch, so HBM does not bound it: its binding
+    roof is VALU issue (achieved = VALU wave instructions per launch, counted,
+    / launch duration, timed; peak VALU_ISSUE_PEAK_G).  Beside it: the bytes a
+    step that keeps nothing on chip would move (algorithmic, per voxel 144 B +
+    100 B per link) and the HBM bytes the launch moves (FETCH_SIZE +
+    WRITE_SIZE, calibrated on a copy of known size)."""
+    kernel = {vx3mi.MODE_TILED: "vx3mi::k_tile", vx3mi.MODE_PERSISTENT: "vx3mi::k_persistent"}.get(mode)
+    bpv = algorithmic_bytes_per_voxel_step(nv, nl)
+    alg = bpv * nv * steps_per_launch / launch_s / 1e9
+    out = {"kernel": kernel, "steps_per_launch": steps_per_launch, "launch_ms": launch_s * 1e3,
+           "bytes_per_voxel_step": bpv, "algorithmic_gbs": alg}
+    cnt = None
+    if kernel and not args.no_counters:
+        child = ["--steps", str(int(steps_per_launch)), "--robots", str(args.robots), "--collisions", str(collisions),
+                 "--child-mode", str(mode), "--child-tile", str(tile), "--child-threads", str(threads)]
+        cnt = counter_run(kernel, child)
+    if cnt is None:
+        out.update(bound="valu-issue", achieved=None, peak=VALU_ISSUE_PEAK_G, unit="G VALU wave-instructions/s",
+                   frac=None, traffic=None, note="counters not collected")
+        return out
+    rate = cnt["SQ_INSTS_VALU"] / launch_s / 1e9
+    out.update(bound="valu-issue", achieved=rate, peak=VALU_ISSUE_PEAK_G, unit="G VALU wave-instructions/s",
+               frac=rate / VALU_ISSUE_PEAK_G, traffic=cnt["traffic"],
+               hbm={"achieved": cnt["traffic"] / launch_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": cnt["traffic"] / launch_s / 1e9 / HBM_PEAK_GBS},
+               valu_per_wave_step=cnt["SQ_INSTS_VALU"] / cnt["SQ_WAVES"] / steps_per_launch,
+               counters={k: cnt[k] for k in ("FETCH_SIZE", "WRITE_SIZE", "FETCH_SIZE_calibration",
+                                             "WRITE_SIZE_calibration", "SQ_INSTS_VALU", "SQ_WAVES", "dispatches")},
+               note="achieved: counted VALU wave instructions per launch / timed launch duration; peak: 256 CUs "
+                    "x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction; traffic: HBM bytes per launch "
+                    "(FETCH_SIZE + WRITE_SIZE x their copy calibration); algorithmic_gbs: the bytes of a step "
+                    "that keeps no state on chip, which the tiled kernel beats")
+    return out
+
+
+def config1_traffic(step_s: float, steps: int = 100):
+    """Config 1 (one 64x64x16 robot, collisions on, flat mode): HBM bytes per step
+    of all its kernels (FETCH_SIZE + WRITE_SIZE over `steps` steps, calibrated on
+    the copy), against the HBM peak at the timed step duration step_s."""
+    child = ["--workload", "config1", "--steps", str(steps), "--warmup", "0"]
+    per = {}
+    for c in ("FETCH_SIZE", "WRITE_SIZE"):
+        rows = rocprof_counters([c], child)
+        if rows is None:
+            return None
+        ks = kernel_rows(rows, "vx3mi::k_")
+        f = copy_calibration(rows, c)
+        if not ks or f is None:
+            return None
+        per[c] = sum(v[c] for v in ks) * f / steps
+    traffic = per["FETCH_SIZE"] + per["WRITE_SIZE"]
+    nv, nl = configs.counts(configs.config1())
+    alg = algorithmic_bytes_per_voxel_step(nv, nl) * nv
+    return {"bound": "hbm", "achieved": traffic / step_s / 1e9, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": traffic / step_s / 1e9 / HBM_PEAK_GBS, "traffic_per_step": traffic,
+            "read_per_step": per["FETCH_SIZE"], "written_per_step": per["WRITE_SIZE"],
+            "algorithmic_per_step": alg, "algorithmic_gbs": alg / step_s / 1e9, "step_us": step_s * 1e6,
+            "note": "achieved: measured HBM bytes per step (all kernels of the step) / timed step duration"}
+
+
+def collisions_line(args, steps: int = 2000):
+    """The batched workload with collisions on (the reference's default,
+    VX3_SimulationManager.cu:328): autotuned launch configuration, timed steps,
+    candidate-list rebuilds, and the dominant kernel's roofline."""
+    probs = configs.config2(seed=0, n=args.robots, collisions=1)
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP32)
+    tune = b.autotune(args.tune_steps)
+    b.step(args.warmup, mode=vx3mi.MODE_AUTO)
+    b.sync()
+    l0 = b.listings().copy()
+    b.step(steps, mode=vx3mi.MODE_AUTO)
+    b.sync()
+    ms, launches = b.last_step_ms(), b.last_step_launches()
+    lst = b.listings() - l0
+    out = {"value": b.nv * steps / (ms * 1e-3), "unit": "voxel-steps/s", "steps": steps,
+           "ms_per_step": ms / steps, "voxels": b.nv, "autotune": tune,
+           "listings_per_robot_per_1000_steps": float(lst.mean()) * 1000 / steps,
+           "diverged": int(b.diverged().sum()),
+           "workload": "the batched workload with collisions on (EnableCollision, the reference default)"}
+    mode = tune["mode"]
+    per = steps / max(1, launches) if mode in (vx3mi.MODE_TILED, vx3mi.MODE_PERSISTENT) else 1
+    out["roofline"] = launch_roofline(mode, tune["tile"], tune["threads"], b.nv, b.nl, per,
+                                      ms * 1e-3 / max(1, launches), args, collisions=1)
+    b.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -293,9 +489,14 @@ def main() -> int:
     ap.add_argument("--tune-steps", type=int, default=200)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-parity", action="store_true")
-    ap.add_argument("--copy-probe-mb", type=int, default=0,
-                    help="after the timed region, copy a device buffer of this size once (a copy kernel of "
-                         "known size, to calibrate HBM byte counters)")
+    ap.add_argument("--no-counters", action="store_true",
+                    help="skip the rocprofv3 counter passes (HBM bytes, VALU instructions of the dominant kernel)")
+    ap.add_argument("--no-collisions-line", action="store_true")
+    ap.add_argument("--counters-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--collisions", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--child-mode", type=int, default=vx3mi.MODE_TILED, help=argparse.SUPPRESS)
+    ap.add_argument("--child-tile", type=int, default=0, help=argparse.SUPPRESS)
+    ap.add_argument("--child-threads", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--adjoint-budget", type=float, default=16.0 * 2**30)
     ap.add_argument("--workload", choices=["batched", "config1"], default="batched",
                     help="config1: only the 64x64x16 robot with collisions (flat mode), for profiling the "
@@ -305,6 +506,8 @@ def main() -> int:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); robots shard across GPUs "
               "as independent batches, which this build does not launch.", file=sys.stderr)
         return 2
+    if args.counters_child:
+        return counters_child(args)
     if args.workload == "config1":
         return config1_only(args)
 
@@ -325,9 +528,6 @@ def main() -> int:
             f"threads {tune['threads']}")
     else:
         chosen = mode
-    kernel = {vx3mi.MODE_TILED: "k_tile<float, %d>" % (tune["tile"] if tune else 0),
-              vx3mi.MODE_PERSISTENT: "k_persistent<float, %d>" % (tune["threads"] if tune else 256),
-              vx3mi.MODE_FLAT: "k_links<float> + k_voxels<float>"}.get(chosen, "?")
     # warmup
     batch.step(args.warmup, mode=mode)
     batch.sync()
@@ -343,22 +543,6 @@ def main() -> int:
     log(f"bench: batched {args.steps} steps: device {dev_ms:.1f} ms, wall {wall * 1e3:.1f} ms, "
         f"{launches} launches, {value:.3e} voxel-steps/s, diverged robots {div}")
 
-    if args.copy_probe_mb > 0:
-        # one device-to-device hipMemcpy (the runtime's copy kernel) of a known size
-        import ctypes as C
-
-        hip = C.CDLL("libamdhip64.so")
-        nbytes = C.c_size_t(args.copy_probe_mb * 2**20)
-        src, dst = C.c_void_p(), C.c_void_p()
-        assert hip.hipMalloc(C.byref(src), nbytes) == 0 and hip.hipMalloc(C.byref(dst), nbytes) == 0
-        assert hip.hipMemset(src, 1, nbytes) == 0
-        assert hip.hipDeviceSynchronize() == 0
-        assert hip.hipMemcpy(dst, src, nbytes, 3) == 0  # hipMemcpyDeviceToDevice
-        assert hip.hipDeviceSynchronize() == 0
-        hip.hipFree(src)
-        hip.hipFree(dst)
-        log(f"bench: copy probe {args.copy_probe_mb} MiB device to device")
-
     # plain path: one robot per launch sequence
     plain = vx3mi.Batch(models, vx3mi.FP32)
     plain.step(2, mode=vx3mi.MODE_PLAIN)
@@ -370,12 +554,20 @@ def main() -> int:
     plain.close()
     log(f"bench: plain {args.plain_steps} steps: {plain_ms:.1f} ms, {plain_value:.3e} voxel-steps/s")
 
-    # roofline of the dominant (persistent) kernel
-    bpv = algorithmic_bytes_per_voxel_step(nv, nl)
+    # roofline of the dominant kernel: its launch duration from the HIP events
+    # above; its VALU instructions and HBM bytes per launch from counter passes
+    # over launches of the same shape in a child process
+    tile = tune["tile"] if tune else 0
+    threads = tune["threads"] if tune else 256
     steps_per_launch = args.steps / max(1, launches)
-    bytes_per_launch = bpv * nv * steps_per_launch
     avg_launch_s = dev_ms * 1e-3 / max(1, launches)
-    achieved = bytes_per_launch / avg_launch_s / 1e9
+    roofline = launch_roofline(chosen, tile, threads, nv, nl, steps_per_launch, avg_launch_s, args, collisions=0)
+    log(f"bench: roofline {roofline}")
+
+    collisions_on = None
+    if not args.no_collisions_line:
+        collisions_on = collisions_line(args)
+        log(f"bench: batched with collisions on: {collisions_on}")
 
     err = None
     if args.check_robots > 0:
@@ -404,6 +596,11 @@ def main() -> int:
         t0 = time.perf_counter()
         parity = parity_report()
         log(f"bench: parity per config ({time.perf_counter() - t0:.1f}s): {parity}")
+        c1 = parity["c1_fp32_vs_fp64"]
+        if not args.no_counters:
+            step_s = c1["voxels"] / c1["voxel_steps_per_s"]["fp32"]
+            c1["roofline"] = config1_traffic(step_s)
+            log(f"bench: config 1 HBM traffic {c1['roofline']}")
 
     adjoint = None
     if not args.no_adjoint:
@@ -444,14 +641,8 @@ def main() -> int:
         "plain_value": plain_value,
         "batched_over_plain": value / plain_value,
         "fp32_rel_err_vs_fp64": err,
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(kernel, args.robots),
-                     "bytes_per_voxel_step": bpv, "kernel": kernel,
-                     "steps_per_launch": steps_per_launch,
-                     "note": "achieved counts the bytes a step that keeps no state on chip moves (per voxel "
-                             "144 B + 100 B per link); the tiled kernel keeps the state in registers and LDS "
-                             "for a whole launch, so it can exceed the HBM peak while its real HBM bytes per "
-                             "launch are `traffic`"},
+        "roofline": roofline,
+        "collisions_on": collisions_on,
         "cpu_baseline": cpu,
         "adjoint": adjoint,
         "autotune": tune,

@@ -71,7 +71,13 @@ def test_autotune_restores_state_and_auto_runs_the_choice():
 
 
 def test_configure_rejects_an_impossible_tile():
-    probs = [configs.config1(seed=3, dims=(12, 12, 6))] * 2  # collisions on: no tiles
+    probs = [configs.config1(seed=3, dims=(12, 12, 6))] * 2  # collisions on: the large tile only
     b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP32)
+    b.configure(vx3mi.MODE_TILED, 0, 0)
     with pytest.raises(vx3mi.VX3MIError):
-        b.configure(vx3mi.MODE_TILED, 0, 0)
+        b.configure(vx3mi.MODE_TILED, 1, 0)
+    big = [configs.config1(seed=3, dims=(20, 20, 8))] * 2  # beyond a collision tile's 1,248 voxels
+    b = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in big], vx3mi.FP32)
+    for tile in (0, 1):
+        with pytest.raises(vx3mi.VX3MIError):
+            b.configure(vx3mi.MODE_TILED, tile, 0)

@@ -369,25 +369,59 @@ def test_config1_fp32_first_1000_steps():
     assert batch.diverged().sum() == 0
 
 
-def test_collision_modes_bitwise_identical():
-    """Collisions on: plain, flat and persistent kernels agree bit for bit."""
-    probs = configs.config2(seed=8, n=6, collisions=1)
+@pytest.mark.parametrize("precision", [vx3mi.FP32, vx3mi.FP64])
+def test_collision_modes_bitwise_identical(precision):
+    """Collisions on: plain, flat, persistent and tiled kernels agree bit for bit
+    (states, contact pairs of the last step, target hits), over 150 steps from a
+    jostled state in which the candidate lists are rebuilt several times (FP64:
+    robots of at most 7^3 cells, the FP64 collision tile holds 640 voxels)."""
+    probs = configs.config2(seed=8, n=6, hi=12 if precision == vx3mi.FP32 else 7, collisions=1)
     models = [vx3mi.Model.from_problem(p) for p in probs]
     rng = np.random.default_rng(1)
-    states = []
-    base = vx3mi.Batch(models, vx3mi.FP32)
+    out = []
+    base = vx3mi.Batch(models, precision)
     pos = base.state()["pos"] + rng.normal(scale=1.5e-3, size=(base.nv, 3))
     base.close()
-    for mode in (vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT, vx3mi.MODE_PERSISTENT):
-        b = vx3mi.Batch(models, vx3mi.FP32)
+    for mode in (vx3mi.MODE_PLAIN, vx3mi.MODE_FLAT, vx3mi.MODE_PERSISTENT, vx3mi.MODE_TILED):
+        b = vx3mi.Batch(models, precision)
         b.set_state(pos=pos)
-        b.step(150, mode=mode)
-        states.append(b.state())
+        b.step(149, mode=mode)
+        b.record_contacts()
+        b.step(1, mode=mode)
+        out.append((b.state(), b.contact_pairs(), b.listings()))
         assert b.diverged().sum() == 0
         b.close()
-    for s in states[1:]:
+    log(f"listings per robot {out[0][2].tolist()}, pairs in the last step {len(out[0][1])}")
+    assert out[0][2].min() > 1 and len(out[0][1]) > 0
+    for st, pr, lst in out[1:]:
         for k in ("pos", "quat", "linmom", "angmom"):
-            assert np.array_equal(s[k], states[0][k]), k
+            assert np.array_equal(st[k], out[0][0][k]), k
+        assert np.array_equal(pr, out[0][1])
+
+
+def test_tiled_collisions_fp64_match_oracle():
+    """Collisions on, FP64, tiled kernel (contacts from LDS poses and momenta,
+    candidate lists rebuilt inside the launch): every robot of a jostled batch
+    equals the oracle bit for bit after 200 steps (two launches of 100)."""
+    probs = configs.config2(seed=9, n=5, hi=7, collisions=1)
+    models = [vx3mi.Model.from_problem(p) for p in probs]
+    b = vx3mi.Batch(models, vx3mi.FP64)
+    rng = np.random.default_rng(2)
+    refs = []
+    for r, p in enumerate(probs):
+        ref = O.OracleModel(p)
+        pos = ref.state()["pos"] + rng.normal(scale=1.5e-3, size=(ref.nv, 3))
+        ref.set_state(pos=pos)
+        b.set_state(r, pos=pos)
+        refs.append(ref)
+    for _ in range(2):
+        b.step(100, mode=vx3mi.MODE_TILED)
+    assert b.diverged().sum() == 0
+    for r, ref in enumerate(refs):
+        ref.step(200)
+        s, o = b.state(r), ref.state()
+        for k in ("pos", "quat", "linmom", "angmom"):
+            assert np.array_equal(s[k], o[k]), (r, k)
 
 
 def test_tiled_full_batch_matches_flat():

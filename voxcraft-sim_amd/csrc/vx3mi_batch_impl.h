@@ -1143,11 +1143,12 @@ template <typename T> struct Batch : BatchBase {
     // least one tile per workgroup slot of the device (compute units x
     // workgroups per unit), robots sorted by link count, each placed in the
     // least-loaded tile that still has room.
-    template <int SMALL> int plan_tiles_cfg() {
-        using C = TileCfg<T, SMALL>;
+    template <int SMALL, int COL> int plan_tiles_cfg() {
+        using C = typename TileSel<T, SMALL, COL>::C;
         TilePlan &P = plans[SMALL];
         P.ok = false;
-        if (anyCollisions || anyForceField || anyCilia || plainFlatOnly() || R < 2) return VX3MI_OK;
+        if (anyForceField || anyCilia || plainFlatOnly() || R < 2) return VX3MI_OK;
+        if (anyCollisions && SMALL) return VX3MI_OK;  // collision tiles: the large configuration
         long totL = 0, totV = 0;
         for (const RobotParams &p : hrp) {
             if (p.vcount > C::VMAX || p.lcount > C::LMAX) return VX3MI_OK;
@@ -1199,7 +1200,9 @@ template <typename T> struct Batch : BatchBase {
 
     int plan_tiles() {
         int rc;
-        if ((rc = plan_tiles_cfg<0>()) || (rc = plan_tiles_cfg<1>())) return rc;
+        if (anyCollisions) rc = plan_tiles_cfg<0, 1>();
+        else rc = plan_tiles_cfg<0, 0>();
+        if (rc || (rc = plan_tiles_cfg<1, 0>())) return rc;
         tileOK = plans[0].ok || plans[1].ok;
         tileSel = plans[0].ok ? 0 : 1;
         autoMode = tileOK ? VX3MI_MODE_TILED
@@ -1212,12 +1215,16 @@ template <typename T> struct Batch : BatchBase {
         const int cap = tileSel ? TileMatCache<1>::N : TileMatCache<0>::N;
         const bool cache = nlm <= cap && nvm <= cap;
         const TilePlan &P = plans[tileSel];
-        if (tileSel == 0) {
-            if (cache) k_tile<T, 0, 1><<<P.ntiles, TileCfg<T, 0>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
-            else k_tile<T, 0, 0><<<P.ntiles, TileCfg<T, 0>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
+        if (anyCollisions) {
+            constexpr int NTC = TileColCfg<T>::NT;
+            if (cache) k_tile<T, 0, 1, 1><<<P.ntiles, NTC, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
+            else k_tile<T, 0, 0, 1><<<P.ntiles, NTC, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
+        } else if (tileSel == 0) {
+            if (cache) k_tile<T, 0, 1, 0><<<P.ntiles, TileCfg<T, 0>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
+            else k_tile<T, 0, 0, 0><<<P.ntiles, TileCfg<T, 0>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
         } else {
-            if (cache) k_tile<T, 1, 1><<<P.ntiles, TileCfg<T, 1>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
-            else k_tile<T, 1, 0><<<P.ntiles, TileCfg<T, 1>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
+            if (cache) k_tile<T, 1, 1, 0><<<P.ntiles, TileCfg<T, 1>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
+            else k_tile<T, 1, 0, 0><<<P.ntiles, TileCfg<T, 1>::NT, 0, stream>>>(d, P.d_tiles, P.d_tileRobots, k, par, nlm, nvm);
         }
     }
     void launch_persistent(int k, int par) {
@@ -1284,7 +1291,7 @@ template <typename T> struct Batch : BatchBase {
         int rc = set_dt(dt);
         if (rc) return rc;
         if (mode == VX3MI_MODE_TILED && !plans[tileSel].ok) {
-            set_error("tiled mode needs collisions, force fields, cilia and removal off and every robot within one tile");
+            set_error("tiled mode needs force fields, cilia, signals, detach and removal off and every robot within one tile");
             return VX3MI_E_ARG;
         }
         if (mode == VX3MI_MODE_AUTO) mode = autoMode;
@@ -1564,7 +1571,7 @@ template <typename T> struct Batch : BatchBase {
 
     int configure(int mode, int tile, int threads) override {
         if (mode == VX3MI_MODE_TILED && (tile < 0 || tile > 1 || !plans[tile].ok)) {
-            set_error("this tile variant cannot hold every robot of the batch (or collisions are on)");
+            set_error("this tile variant cannot hold every robot of the batch (batches with collisions: the large tile)");
             return VX3MI_E_ARG;
         }
         if (threads != 0 && threads != 128 && threads != 256 && threads != 512) {

@@ -40,6 +40,19 @@ template <> struct TileCfg<double, 1> {
     static constexpr int NT = 256, KL = 2, KV = 2, LMAX = 512, VMAX = 384, RMAX = 32, MINW = 2;
 };
 
+// Collision tiles (large configuration): the partner momenta of the contact
+// phase sit in LDS beside the poses, so the capacities are smaller (every
+// robot of the batched workload still fits: at most 1,861 links, 1,077 voxels).
+template <typename T> struct TileColCfg;
+template <> struct TileColCfg<float> {
+    static constexpr int NT = 768, KL = 3, KV = 2, LMAX = 1920, VMAX = 1248, RMAX = 128, MINW = 1;
+};
+template <> struct TileColCfg<double> {
+    static constexpr int NT = 512, KL = 2, KV = 2, LMAX = 896, VMAX = 640, RMAX = 64, MINW = 2;
+};
+template <typename T, int SMALL, int COL> struct TileSel { using C = TileCfg<T, SMALL>; };
+template <typename T, int SMALL> struct TileSel<T, SMALL, 1> { using C = TileColCfg<T>; };
+
 // LDS robot table of a tile
 template <int RMAX> struct TileRobots {
     int rid[RMAX];
@@ -52,6 +65,9 @@ template <int RMAX> struct TileRobots {
     int steps[RMAX];
     double tempAmp[RMAX], tempPeriod[RMAX];  // actuation of each robot (FP32: 1 / period)
     int tempFlags[RMAX];                     // varyTemp | enableExpansion << 1
+    int col[RMAX];                           // the robot has collisions (surface voxels in the lists)
+    int spre[RMAX + 1];                      // prefix of the robots' surface voxel counts (collision tiles)
+    int anyStale;                            // some robot of the tile lists its candidates this step
 };
 
 // material tables of at most this many entries (large / small tile) are
@@ -71,15 +87,22 @@ __device__ __forceinline__ int tile_slot(const int *pre, int n, int x) {
 
 // LDSMAT: the batch has at most kTileMatCache voxel and link materials, which
 // are read from LDS in the step loop instead of global memory
-template <typename T, int SMALL, int LDSMAT>
-__global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINW)) void k_tile(Dev<T> a, const int4 *tiles, const int *tileRobots,
-                                                                int nsteps, int parity0, int nLinkMats, int nVoxMats) {
-    using C = TileCfg<T, SMALL>;
+// COL: collision tiles.  Each step, robots whose candidate lists are stale
+// list them first (the tile publishes its poses to global memory and runs the
+// listing passes of the flat path); the contact forces are computed at the
+// start of the voxel phase, from the poses and momenta entering the step (LDS)
+// and the lists, summed in list order as in k_col_contact.
+template <typename T, int SMALL, int LDSMAT, int COL>
+__global__ __launch_bounds__((TileSel<T, SMALL, COL>::C::NT), (TileSel<T, SMALL, COL>::C::MINW)) void k_tile(
+    Dev<T> a, const int4 *tiles, const int *tileRobots, int nsteps, int parity0, int nLinkMats, int nVoxMats) {
+    using C = typename TileSel<T, SMALL, COL>::C;
     constexpr int NT = C::NT, KL = C::KL, KV = C::KV;
     constexpr int LMAX = C::LMAX, VMAX = C::VMAX, RMAX = C::RMAX;
-    static_assert(KL * NT >= LMAX && KV * NT >= VMAX && RMAX <= NT, "a tile's threads must cover its capacity");
+    static_assert(KL * NT >= LMAX && KV * NT >= VMAX && RMAX <= NT && NT <= 3 * LMAX,
+                  "a tile's threads must cover its capacity");
     __shared__ T sPose[8][VMAX];    // position field xyz, qw, qx, qy, qz, temperature
     __shared__ T sForce[12][LMAX];  // forceNeg, momentNeg, forcePos, momentPos
+    __shared__ T sMom[COL ? 3 : 1][COL ? VMAX : 1];  // linear momentum entering the step (contacts)
     __shared__ TileRobots<RMAX> tr;
     __shared__ LinkMat sLM[LDSMAT ? TileMatCache<SMALL>::N : 1];
     __shared__ VoxelMat sVM[LDSMAT ? TileMatCache<SMALL>::N : 1];
@@ -107,6 +130,8 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINW))
         tr.tempAmp[tid] = p.tempAmp;
         tr.tempPeriod[tid] = std::is_same<T, float>::value ? p.tempInvPeriod : p.tempPeriod;
         tr.tempFlags[tid] = (p.varyTemp ? 1 : 0) | (p.enableExpansion ? 2 : 0);
+        tr.col[tid] = COL && p.collisions && p.scount > 0;
+        tr.spre[tid + 1] = COL ? p.scount : 0;
     }
     if (LDSMAT) {
         for (int i = tid; i < nLinkMats; i += NT) sLM[i] = a.lmats[i];
@@ -116,9 +141,11 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINW))
     if (tid == 0) {
         tr.vpre[0] = 0;
         tr.lpre[0] = 0;
+        tr.spre[0] = 0;
         for (int j = 0; j < nr; j++) {
             tr.vpre[j + 1] += tr.vpre[j];
             tr.lpre[j + 1] += tr.lpre[j];
+            tr.spre[j + 1] += tr.spre[j];
         }
     }
     __syncthreads();
@@ -180,12 +207,123 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINW))
             vs[k].flags = (int)m1.z;
             sPose[0][lv] = q0.x; sPose[1][lv] = q0.y; sPose[2][lv] = q0.z; sPose[3][lv] = q0.w;
             sPose[4][lv] = q1.x; sPose[5][lv] = q1.y; sPose[6][lv] = q1.z; sPose[7][lv] = q1.w;
+            if constexpr (COL) { sMom[0][lv] = m0.x; sMom[1][lv] = m0.y; sMom[2][lv] = m0.z; }
         }
     }
     __syncthreads();
 
+    // surface voxel i of the tile (collision robots in tile order): its global
+    // surface index and its robot's slot
+    auto tile_surface = [&](int i, int &j) {
+        j = tile_slot(tr.spre, nr, i);
+        return a.rp[tr.rid[j]].soff + (i - tr.spre[j]);
+    };
+    // the contact force on owned voxel gv (tile index lv, robot slot j), from the
+    // listed candidates in contact now: col_contact's partner test and force with
+    // the poses and momenta of the tile (the robots of a tile lose neither links
+    // nor voxels, and have no signals)
+    auto contact = [&](int gv, int lv, int j, int pkv, const VoxelMat &mi, int parity) -> V3<T> {
+        V3<T> f = {0, 0, 0};
+        const int s = a.vsurf[gv];
+        if (s < 0) return f;
+        const int r = tr.rid[j];
+        RobotTime *rt = a.rt + r;
+        const RobotParams &rp = a.rp[r];
+        if (s == rp.soff && rt->stale[parity]) {
+            rt->listings += 1;
+            rt->stale[parity] = 0;
+        }
+        const int lbase = tr.vpre[j] - tr.voffG[j];  // tile index of a voxel of the robot = global id + lbase
+        const T4<T> qi0 = {sPose[0][lv], sPose[1][lv], sPose[2][lv], sPose[3][lv]};
+        const T L = (T)mi.nomSize;
+        const T bsaI = base_size_average<T>(mi.nomSize, (float)sPose[7][lv], mi.alphaCTE);
+        const V3<T> velI = V3<T>{sMom[0][lv], sMom[1][lv], sMom[2][lv]} * (T)mi.massInv;
+        const int ti = a.vtarget ? a.vtarget[gv] : 0;
+        auto partner = [&](int w, bool links, V3<T> &fw) -> bool {
+            const int lw = w + lbase;
+            const int3 o = lat_offset(pkv, a.lat[w]);
+            if (links && col_linked(a, gv, o.x, o.y, o.z)) return false;
+            const VoxelMat &mj = LDSMAT ? sVM[a.vmat[w]] : a.vmats[a.vmat[w]];
+            if (mi.fixed && mj.fixed) return false;
+            const T4<T> qj0 = {sPose[0][lw], sPose[1][lw], sPose[2][lw], sPose[3][lw]};
+            const V3<T> diff = col_diff<T>(qi0, qj0, o, L);
+            const T bsaJ = base_size_average<T>(mj.nomSize, (float)sPose[7][lw], mj.alphaCTE);
+            if (!in_contact_envelope(diff, bsaI, bsaJ)) return false;
+            if (a.vtarget && a.vtarget[w] != ti && gv < w) atomicAdd(&rt->collisionCount, 1);
+            if (a.pairRec && gv < w) {
+                const int k = atomicAdd(a.pairN, 1);
+                if (k < a.pairCap) a.pairRec[k] = int2{gv, w};
+            }
+            const V3<T> velJ = V3<T>{sMom[0][lw], sMom[1][lw], sMom[2][lw]} * (T)mj.massInv;
+            fw = contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
+            return true;
+        };
+        const int nc = a.ccount[s];
+        if (nc <= kMaxCand) {
+            const int *list = a.clist + (long)s * kMaxCand;
+            for (int k = 0; k < nc; k++) {
+                V3<T> fw;
+                if (partner(list[k], false, fw)) f = f + fw;
+            }
+        } else {  // more candidates than a list holds: the grid of this step, ascending
+            rt->stale[parity ^ 1] = 1;
+            int prev = -1;
+            for (;;) {
+                int next = INT_MAX;
+                col_grid_each(a, rp, s, [&](int sj) {
+                    if (sj > prev && sj < next) next = sj;
+                });
+                if (next == INT_MAX) break;
+                V3<T> fw;
+                if (partner(a.surf[next], true, fw)) f = f + fw;
+                prev = next;
+            }
+        }
+        return f;
+    };
+
     for (int s = 0; s < nsteps; s++) {
         const int parity = (parity0 + s) & 1;
+        if constexpr (COL) {
+            // ---- candidate lists of the robots whose lists are stale (rare)
+            if (tid == 0) tr.anyStale = 0;
+            __syncthreads();
+            if (tid < nr && tr.col[tid] && !tr.diverged[tid] && a.rt[tr.rid[tid]].stale[parity]) tr.anyStale = 1;
+            __syncthreads();
+            if (tr.anyStale) {
+                auto listing = [&](int j) { return tr.col[j] && !tr.diverged[j] && a.rt[tr.rid[j]].stale[parity]; };
+#pragma unroll
+                for (int k = 0; k < KV; k++) {  // the listing passes read poses from global memory
+                    const int lv = tid + k * NT;
+                    if (vGlob[k] < 0 || !listing(vSlot[k])) continue;
+                    a.pq0[vGlob[k]] = T4<T>{sPose[0][lv], sPose[1][lv], sPose[2][lv], sPose[3][lv]};
+                    a.pq1[vGlob[k]] = T4<T>{sPose[4][lv], sPose[5][lv], sPose[6][lv], sPose[7][lv]};
+                }
+                __syncthreads();
+                for (int i = tid; i < tr.spre[nr]; i += NT) {
+                    int j;
+                    const int si = tile_surface(i, j);
+                    if (listing(j)) col_count(a, si, parity);
+                }
+                __syncthreads();
+                int *wsum = reinterpret_cast<int *>(&sForce[0][0]);  // free until the link phase
+                for (int j = 0; j < nr; j++) {
+                    if (!listing(j)) continue;  // (the same for every thread)
+                    const int r = tr.rid[j];
+                    col_scan(a, r, wsum);
+                    __syncthreads();
+                    const RobotParams &rp = a.rp[r];
+                    for (int i = rp.soff + tid; i < rp.soff + rp.scount; i += NT) col_scatter(a, i, parity);
+                    __syncthreads();
+                }
+                for (int i = tid; i < tr.spre[nr]; i += NT) {
+                    int j;
+                    const int si = tile_surface(i, j);
+                    if (listing(j)) col_list(a, si, parity);
+                }
+                __syncthreads();
+            }
+        }
         // ---- links
 #pragma unroll
         for (int k = 0; k < KL; k++) {
@@ -210,7 +348,19 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINW))
             if (o.strain > 100.0f) tr.diverged[j] = 1;
         }
         __syncthreads();
-        // ---- voxels
+        // ---- voxels (contact forces first, from the poses entering the step)
+        V3<T> cf[KV];
+#pragma unroll
+        for (int k = 0; k < KV; k++) {
+            cf[k] = V3<T>{0, 0, 0};
+            if constexpr (COL) {
+                const int lv = tid + k * NT;
+                if (vGlob[k] < 0 || !tr.col[vSlot[k]] || tr.diverged[vSlot[k]]) continue;
+                const VoxelMat &vm = LDSMAT ? sVM[vMat[k]] : a.vmats[vMat[k]];
+                cf[k] = contact(vGlob[k], lv, vSlot[k], vLat[k], vm, parity);
+            }
+        }
+        if constexpr (COL) __syncthreads();
 #pragma unroll
         for (int k = 0; k < KV; k++) {
             const int lv = tid + k * NT;
@@ -237,7 +387,7 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINW))
             }
             float temp = (float)sPose[7][lv];
             const int tf = tr.tempFlags[j];
-            voxel_step<T>(vm, a.exts, dt, vLat[k], temp, fs, ms, V3<T>{0, 0, 0}, V3<T>{0, 0, 0}, vs[k]);
+            voxel_step<T>(vm, a.exts, dt, vLat[k], temp, fs, ms, cf[k], V3<T>{0, 0, 0}, vs[k]);
             if ((tf & 1) && !(vm.thermalOnAfter > tnext)) {
                 if constexpr (std::is_same<T, float>::value)
                     temp = actuation_temperature_f32(tr.tempAmp[j], tr.tempPeriod[j], tnext, (double)vPhase[k], tf >> 1);
@@ -246,6 +396,10 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINW))
             }
             sPose[0][lv] = vs[k].u.x; sPose[1][lv] = vs[k].u.y; sPose[2][lv] = vs[k].u.z; sPose[3][lv] = vs[k].q.w;
             sPose[4][lv] = vs[k].q.x; sPose[5][lv] = vs[k].q.y; sPose[6][lv] = vs[k].q.z; sPose[7][lv] = (T)temp;
+            if constexpr (COL) {
+                sMom[0][lv] = vs[k].p.x; sMom[1][lv] = vs[k].p.y; sMom[2][lv] = vs[k].p.z;
+                if (tr.col[j]) col_track(a, vGlob[k], vs[k].u, a.rt + tr.rid[j], a.rp[tr.rid[j]], parity);
+            }
         }
         __syncthreads();
     }
@@ -277,6 +431,8 @@ __global__ __launch_bounds__((TileCfg<T, SMALL>::NT), (TileCfg<T, SMALL>::MINW))
         t.diverged = tr.diverged[tid];
         t.steps = tr.steps[tid];
     }
+    // the flat path's hint that some robot may be stale (per-robot flags decide)
+    if (COL && blockIdx.x == 0 && tid == 0) a.anyStale[0] = a.anyStale[1] = 1;
 }
 
 }  // namespace vx3mi
