@@ -254,6 +254,15 @@ int vx3mi_batch_diverged(vx3mi_batch *b, int *out);
  * cap pairs per call); cap 0 stops recording.  Error when no robot has
  * collisions. */
 int vx3mi_batch_record_contacts(vx3mi_batch *b, int cap);
+/* No counterpart in the reference: the same contact record for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving loops).
+ * The record lives in the library's static per-GPU queue of at most 1,048,576
+ * pairs (cap beyond that: VX3MI_E_LIMIT); the call allocates nothing.  Call it
+ * with the batch's device current.  Every batch recording this way on one GPU
+ * shares that GPU's queue: step calls of such batches that overlap race, and
+ * each step call starts the record afresh (the queue keeps the last call's
+ * pairs until then).  Flat-mode steps capture their graph again after it. */
+int vx3mi_batch_record_contacts_static(vx3mi_batch *b, int cap);
 /* per robot: the steps whose collision phase rebuilt the robot's candidate
  * lists (every surface voxel lists the surface voxels within its largest
  * contact envelope plus a skin of a quarter voxel; they are rebuilt, through a

@@ -83,6 +83,7 @@ struct BatchBase {
     // graph of a chunk of flat steps
     hipGraphExec_t graph = nullptr;
     int graphSteps = 0, graphParity = -1;
+    bool graphStale = false;  // the launch arguments changed: capture the flat graph again
     virtual ~BatchBase() {
         if (graph) (void)hipGraphExecDestroy(graph);
         for (void *p : allocs) (void)hipFree(p);
@@ -126,6 +127,9 @@ struct BatchBase {
     // contact pairs of the collision phases of the next step calls (cap pairs
     // per call; 0 stops), and those of the last call for one robot
     virtual int record_contacts(int cap) = 0;
+    // record into the given device queue (pairs) and counter, or stop (nullptr);
+    // allocates nothing
+    virtual int record_contacts_at(int2 *rec, int *counter, int cap) = 0;
     virtual int contact_pairs(int robot, int *out, int cap) = 0;
     RobotTime *d_rt = nullptr;
 };

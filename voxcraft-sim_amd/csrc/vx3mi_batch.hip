@@ -1,6 +1,7 @@
 // vx3mi_batch.hip -- C ABI of device batches (include/vx3mi.h).
 #include <hip/hip_runtime.h>
 
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -8,6 +9,23 @@
 #include "vx3mi_batch.h"
 
 using namespace vx3mi;
+
+// Allocation-free contact record (vx3mi_batch_record_contacts_static): one
+// device queue per GPU in a file-scope array, here `scratch` -- a counter in its
+// first 256 bytes, then at most kPairScratch pairs (8 MiB in all).  In host
+// code `scratch` names a host-side stand-in; its device address is taken once
+// per device with hipGetSymbolAddress and only that address is handed to
+// kernels.  Every batch on a GPU that records this way shares that GPU's queue.
+// (A static __device__ variable that host code names through HIP_SYMBOL is
+// given an external device symbol by the compiler, which the runtime looks up;
+// marking it `used` would keep that symbol local, and the lookup would fail.)
+constexpr int kPairScratch = 1 << 20;
+static __device__ __align__(256) int2 scratch[32 + kPairScratch];
+namespace {
+constexpr int kMaxDevices = 64;
+void *g_scratch[kMaxDevices];
+std::mutex g_scratchLock;
+}  // namespace
 
 
 extern "C" {
@@ -173,6 +191,37 @@ int vx3mi_batch_listings(vx3mi_batch *b, int *out) {
 int vx3mi_batch_record_contacts(vx3mi_batch *b, int cap) {
     if (!b || !b->b) { set_error("batch is NULL"); return VX3MI_E_ARG; }
     return b->b->record_contacts(cap);
+}
+
+int vx3mi_batch_record_contacts_static(vx3mi_batch *b, int cap) {
+    if (!b || !b->b) { set_error("batch is NULL"); return VX3MI_E_ARG; }
+    if (cap < 0 || cap > kPairScratch) {
+        set_error("contact record capacity must be 0.." + std::to_string(kPairScratch) + " pairs");
+        return cap < 0 ? VX3MI_E_ARG : VX3MI_E_LIMIT;
+    }
+    if (cap == 0) return b->b->record_contacts_at(nullptr, nullptr, 0);
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess || dev < 0 || dev >= kMaxDevices) {
+        set_error(std::string("hipGetDevice: ") + (e != hipSuccess ? hipGetErrorString(e) : "device index out of range"));
+        return VX3MI_E_HIP;
+    }
+    void *base = nullptr;
+    {
+        std::lock_guard<std::mutex> lock(g_scratchLock);
+        if (!g_scratch[dev]) {  // this device's copy of the array, with the device current
+            void *p = nullptr;
+            e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+            if (e != hipSuccess || !p) {
+                set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+                return VX3MI_E_HIP;
+            }
+            g_scratch[dev] = p;
+        }
+        base = g_scratch[dev];
+    }
+    return b->b->record_contacts_at(reinterpret_cast<int2 *>(static_cast<char *>(base) + 256), static_cast<int *>(base),
+                                    cap);
 }
 
 int vx3mi_batch_contact_pairs(vx3mi_batch *b, int robot, int *pairs2, int cap) {

@@ -626,6 +626,8 @@ template <typename T> __global__ __launch_bounds__(256) void k_detach(Dev<T> a) 
     a.vlinks[6 * e.y + 2 * axis + 1] = -1;  // and pVPos's NEG-direction link
 }
 
+template <typename T = int> __global__ __launch_bounds__(1) void k_zero_int(int *p) { *p = 0; }
+
 template <typename T> __global__ __launch_bounds__(256) void k_mark_stale(RobotTime *rt, int R, int *anyStale) {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r < R) rt[r].stale[0] = rt[r].stale[1] = 1;
@@ -1299,7 +1301,15 @@ template <typename T> struct Batch : BatchBase {
             set_error("force fields, material removal, detach and signals run in the plain and flat modes only");
             return VX3MI_E_ARG;
         }
-        if (d.pairRec) HIP_TRY(hipMemsetAsync(d.pairN, 0, sizeof(int), stream));  // (not inside a graph)
+        if (d.pairRec) {  // each call records its own steps (a fill kernel: no memset node under capture)
+            k_zero_int<<<1, 1, 0, stream>>>(d.pairN);
+            HIP_TRY(hipGetLastError());
+        }
+        if (graphStale) {
+            if (graph) (void)hipGraphExecDestroy(graph);
+            graph = nullptr;
+            graphStale = false;
+        }
         HIP_TRY(hipEventRecord(ev0, stream));
         launches = 0;
         lastMode = mode;
@@ -1531,17 +1541,27 @@ template <typename T> struct Batch : BatchBase {
     int record_contacts(int cap) override {
         if (cap < 0) { set_error("negative capacity"); return VX3MI_E_ARG; }
         if (cap > 0 && !anyCollisions) { set_error("no robot of the batch has collisions"); return VX3MI_E_ARG; }
-        if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }  // it holds the launch arguments
         HIP_TRY(hipStreamSynchronize(stream));
         if (cap > pairAlloc) {
             int rc;
             if ((rc = alloc(&pairBuf, cap)) || (!pairNBuf && (rc = alloc(&pairNBuf, 1)))) return rc;
             pairAlloc = cap;
         }
-        d.pairRec = cap ? pairBuf : nullptr;
-        d.pairN = cap ? pairNBuf : nullptr;
+        return record_contacts_at(cap ? pairBuf : nullptr, cap ? pairNBuf : nullptr, cap);
+    }
+
+    int record_contacts_at(int2 *rec, int *counter, int cap) override {
+        if (cap < 0 || (cap > 0 && (!rec || !counter))) { set_error("invalid contact record"); return VX3MI_E_ARG; }
+        if (cap > 0 && !anyCollisions) { set_error("no robot of the batch has collisions"); return VX3MI_E_ARG; }
+        d.pairRec = cap ? rec : nullptr;
+        d.pairN = cap ? counter : nullptr;
         d.pairCap = cap;
-        if (cap) HIP_TRY(hipMemset(pairNBuf, 0, sizeof(int)));
+        graphStale = true;  // the flat graph holds the launch arguments
+        if (cap) {
+            k_zero_int<<<1, 1, 0, stream>>>(counter);
+            HIP_TRY(hipGetLastError());
+        }
+        HIP_TRY(hipStreamSynchronize(stream));
         return VX3MI_OK;
     }
 

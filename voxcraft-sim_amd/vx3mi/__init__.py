@@ -166,6 +166,7 @@ def lib():
         "vx3mi_model_cilia": (C.c_int, [vp, dp, dp]),
         "vx3mi_model_material": (C.c_int, [vp, C.c_int, C.POINTER(_Material)]),
         "vx3mi_batch_record_contacts": (C.c_int, [vp, C.c_int]),
+        "vx3mi_batch_record_contacts_static": (C.c_int, [vp, C.c_int]),
         "vx3mi_batch_listings": (C.c_int, [vp, ip]),
         "vx3mi_batch_contact_pairs": (C.c_int, [vp, C.c_int, ip, C.c_int]),
         "vx3mi_batch_time": (C.c_int, [vp, dp]),
@@ -431,6 +432,10 @@ class Batch:
         """Record the pairs in contact during each later step call (cap pairs per
         call; 0 stops recording)."""
         _check(lib().vx3mi_batch_record_contacts(self._h, int(cap)))
+
+    def record_contacts_static(self, cap: int = 1 << 20):
+        """record_contacts into the library's static per-GPU queue (allocates nothing)."""
+        _check(lib().vx3mi_batch_record_contacts_static(self._h, int(cap)))
 
     def listings(self) -> np.ndarray:
         """Per robot: the steps whose collision phase rebuilt the candidate lists."""
