@@ -472,6 +472,50 @@ def collisions_line(args, steps: int = 2000):
     return out
 
 
+def adjoint_report(args) -> dict:
+    """The adjoint's cost on the 8x8x8 differentiable robot (2000 steps) and on
+    256 batched robots (5000 steps), against the differentiable forward and
+    against the product timestep on the same robots."""
+    # the reverse sweep's workgroup size (512 or 1024 threads per robot) is
+    # chosen by timing both on each workload
+    best = {}
+    for name, probs, steps, K in (("c3", [configs.config3()], 2000, 100), ("c4", configs.config4(), 5000, 0)):
+        for rt in (512, 1024):
+            c = adjoint_cost(probs, steps, K, args.adjoint_budget, reverse_threads=rt)
+            log(f"bench: adjoint {name} reverse threads {rt}: forward {c['forward_ms']:.1f} ms, backward "
+                f"{c['backward_ms']:.1f} ms, ratio {c['adjoint_over_forward']:.2f}")
+            if name not in best or c["backward_ms"] < best[name]["backward_ms"]:
+                best[name] = c
+    c3, c4 = best["c3"], best["c4"]
+    # the same robots through the product timestep (autotuned, FP32, rigid floor):
+    # the adjoint's cost per voxel-step against the fastest forward of the library
+    for c, probs, steps in ((c3, [configs.config3()], 2000), (c4, configs.config4(), 5000)):
+        pb = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP32)
+        pt = pb.autotune(50)
+        pb.step(min(steps, 1000))
+        pb.sync()
+        pb.step(steps)
+        pb.sync()
+        fwd_ms = pb.last_step_ms()
+        pb.close()
+        c["product_forward_ms"] = fwd_ms
+        c["product_forward_mode"] = pt["mode"]
+        c["adjoint_over_product_forward"] = c["backward_ms"] / fwd_ms
+        c["diff_forward_over_product_forward"] = c["forward_ms"] / fwd_ms
+    log(f"bench: adjoint 8x8x8 2000 steps K=100: forward {c3['forward_ms']:.1f} ms, backward "
+        f"{c3['backward_ms']:.1f} ms, ratio {c3['adjoint_over_forward']:.2f}")
+    log(f"bench: batched adjoint 256 robots 5000 steps K={c4['checkpoint_every']}: forward {c4['forward_ms']:.1f} ms, "
+        f"backward {c4['backward_ms']:.1f} ms, ratio {c4['adjoint_over_forward']:.2f}")
+    for name, c in (("8x8x8", c3), ("256 robots", c4)):
+        log(f"bench: adjoint {name}: product forward {c['product_forward_ms']:.1f} ms (mode "
+            f"{c['product_forward_mode']}), backward / product forward {c['adjoint_over_product_forward']:.2f}")
+    return {"single_8x8x8": c3, "batched_256": c4,
+            "note": "backward = recompute from checkpoints + reverse sweep; FP32; softplus floor; collisions off; "
+                    "adjoint_over_forward: against the differentiable forward (checkpoints, softplus floor, "
+                    "per-voxel moduli); adjoint_over_product_forward: against the product timestep on the "
+                    "same robots (autotuned launch configuration)"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -498,9 +542,9 @@ def main() -> int:
     ap.add_argument("--child-tile", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--child-threads", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--adjoint-budget", type=float, default=16.0 * 2**30)
-    ap.add_argument("--workload", choices=["batched", "config1"], default="batched",
+    ap.add_argument("--workload", choices=["batched", "config1", "adjoint"], default="batched",
                     help="config1: only the 64x64x16 robot with collisions (flat mode), for profiling the "
-                         "collision kernels; prints its own line")
+                         "collision kernels; adjoint: only the adjoint cost lines; each prints its own line")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); robots shard across GPUs "
@@ -510,6 +554,9 @@ def main() -> int:
         return counters_child(args)
     if args.workload == "config1":
         return config1_only(args)
+    if args.workload == "adjoint":
+        print(json.dumps({"workload": "adjoint", "adjoint": adjoint_report(args)}), flush=True)
+        return 0
 
     mode = {"auto": vx3mi.MODE_AUTO, "tiled": vx3mi.MODE_TILED, "persistent": vx3mi.MODE_PERSISTENT,
             "flat": vx3mi.MODE_FLAT}[args.mode]
@@ -602,25 +649,7 @@ def main() -> int:
             c1["roofline"] = config1_traffic(step_s)
             log(f"bench: config 1 HBM traffic {c1['roofline']}")
 
-    adjoint = None
-    if not args.no_adjoint:
-        # the reverse sweep's workgroup size (512 or 1024 threads per robot) is
-        # chosen by timing both on each workload
-        best = {}
-        for name, probs, steps, K in (("c3", [configs.config3()], 2000, 100), ("c4", configs.config4(), 5000, 0)):
-            for rt in (512, 1024):
-                c = adjoint_cost(probs, steps, K, args.adjoint_budget, reverse_threads=rt)
-                log(f"bench: adjoint {name} reverse threads {rt}: forward {c['forward_ms']:.1f} ms, backward "
-                    f"{c['backward_ms']:.1f} ms, ratio {c['adjoint_over_forward']:.2f}")
-                if name not in best or c["backward_ms"] < best[name]["backward_ms"]:
-                    best[name] = c
-        c3, c4 = best["c3"], best["c4"]
-        log(f"bench: adjoint 8x8x8 2000 steps K=100: forward {c3['forward_ms']:.1f} ms, backward "
-            f"{c3['backward_ms']:.1f} ms, ratio {c3['adjoint_over_forward']:.2f}")
-        log(f"bench: batched adjoint 256 robots 5000 steps K={c4['checkpoint_every']}: forward {c4['forward_ms']:.1f} ms, "
-            f"backward {c4['backward_ms']:.1f} ms, ratio {c4['adjoint_over_forward']:.2f}")
-        adjoint = {"single_8x8x8": c3, "batched_256": c4,
-                   "note": "backward = recompute from checkpoints + reverse sweep; FP32; softplus floor; collisions off"}
+    adjoint = None if args.no_adjoint else adjoint_report(args)
 
     out = {
         "metric": "voxel-steps/s batched vs plain @<=1e-5 rel err to FP64; adjoint/forward cost",
