@@ -71,6 +71,7 @@ typedef struct {
     double signal_value_decay; /* signalValueDecay (a VXA without it: 0.9) */
     double signal_time_delay;  /* signalTimeDelay (s; a VXA without it: 0.03) */
     double inactive_period;    /* inactivePeriod (s; a VXA without it: 0.03) */
+    int sticky;                /* Sticky: touching voxels of this material attach (with enable_attach) */
 } vx3mi_material;
 
 /* Lattice description: the information content of a merged VXA (base.vxa +
@@ -99,6 +100,8 @@ typedef struct {
     int enable_signals;            /* Simulator/EnableSignals */
     int enable_detach;             /* Simulator/AttachDetach/EnableDetach: failed links leave their voxels */
     double max_dist_pairs;         /* Simulator/MaxDistInVoxelLengthsToCountAsPair (0: no close-pair counting) */
+    int enable_attach;             /* Simulator/AttachDetach/EnableAttach: touching sticky voxels link up */
+    int safety_guard;              /* Simulator/AttachDetach/SafetyGuard (a VXA without it: 500), 0..65535 */
 } vx3mi_lattice;
 
 /* Settings a model carries besides topology and materials, read back by
@@ -115,6 +118,7 @@ typedef struct {
     int has_stop_formula, has_fitness_formula;
     int enable_cilia, secondary_experiment, enable_signals, enable_detach;
     double max_dist_pairs;         /* MaxDistInVoxelLengthsToCountAsPair */
+    int enable_attach, safety_guard; /* AttachDetach/EnableAttach, SafetyGuard */
 } vx3mi_settings;
 
 typedef struct vx3mi_model vx3mi_model;
@@ -186,6 +190,13 @@ int vx3mi_model_set_external(vx3mi_model *m, int voxel, int dof, const double *t
  * targetCloseness and numClosePairs are 0; formulas using angle are rejected.
  * Batches with force fields run in the plain and flat modes. */
 int vx3mi_model_set_force_field(vx3mi_model *m, int axis, const char *formula_xml);
+/* AttachCondition Condition_k (k = 0..4, VX3_SimulationManager.cu:334-343): a
+ * MathTree XML as for force fields (empty or NULL removes it).  After each voxel
+ * update a voxel may attach when every condition given is > 0
+ * (VX3_VoxelyzeKernel.cu:607-621); variables x, y, z (the voxel's new position),
+ * hit (collisionCount), t (the step's start time), num_voxel; formulas using
+ * angle, targetCloseness or numClosePairs are rejected. */
+int vx3mi_model_set_attach_condition(vx3mi_model *m, int k, const char *formula_xml);
 /* the voxel's external (*dof = -1 and zeros when it has none) */
 int vx3mi_model_get_external(const vx3mi_model *m, int voxel, int *dof, double *translation3,
                              double *rotation3, double *force3, double *moment3);
@@ -274,6 +285,19 @@ int vx3mi_batch_listings(vx3mi_batch *b, int *out);
  * Returns the number of pairs (at most cap are written) or VX3MI_E_LIMIT when
  * the call found more pairs than the record holds. */
 int vx3mi_batch_contact_pairs(vx3mi_batch *b, int robot, int *pairs2, int cap);
+/* Attach state of one robot (replaces reading VX3_Voxel::enableAttach,
+ * VX3_Voxel.h:302, and the links handle_collision_attachment appends to
+ * d_v_links, VX3_VoxelyzeKernel.cu:729-831): voxel_flags (robot voxel count
+ * ints, may be NULL) gets each voxel's enableAttach flag; for the links made so
+ * far, in the order they were made, link_neg / link_pos / link_axis get the
+ * negative end, positive end (robot voxel indices) and axis, new_left the
+ * updates left whose forces are scaled by 0.01 (isNewLink); at most cap links
+ * are written (the arrays may be NULL with cap 0).  overflow (may be NULL) is
+ * set when a voxel had more than 8 attach candidates of lower index in one step
+ * or the robot's pool of link entries ran out; the candidates beyond were not
+ * considered.  Returns the number of links made. */
+int vx3mi_batch_attach_state(vx3mi_batch *b, int robot, int *voxel_flags, int *link_neg, int *link_pos,
+                             int *link_axis, int *new_left, int cap, int *overflow);
 /* updateCurrentCenterOfMass (VX3_VoxelyzeKernel.cu:477-493) per robot: out 3n */
 int vx3mi_center_of_mass(vx3mi_batch *b, double *out);
 

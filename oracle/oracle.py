@@ -31,7 +31,8 @@ class _Material(C.Structure):
                 ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double),
                 ("ss_points", C.c_int), ("ss_strain", C.c_float * 32), ("ss_stress", C.c_float * 32),
                 ("is_pacemaker", C.c_int), ("pacemaker_period", C.c_double), ("signal_decay", C.c_double),
-                ("signal_delay", C.c_double), ("inactive_period", C.c_double), ("is_target", C.c_int)]
+                ("signal_delay", C.c_double), ("inactive_period", C.c_double), ("is_target", C.c_int),
+                ("sticky", C.c_int)]
 
 
 # MathTree token codes (vx3_oracle.h VXO_MT_*) and variable order
@@ -65,7 +66,8 @@ class _Spec(C.Structure):
                 ("temp_base", C.c_double), ("temp_amplitude", C.c_double), ("temp_period", C.c_double),
                 ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int),
                 ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double)),
-                ("enable_signals", C.c_int), ("enable_detach", C.c_int)]
+                ("enable_signals", C.c_int), ("enable_detach", C.c_int),
+                ("enable_attach", C.c_int), ("safety_guard", C.c_int)]
 
 
 class _DiffOpts(C.Structure):
@@ -79,13 +81,13 @@ SPEC_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=
                      floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
                      temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None,
                      enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None, enable_signals=0,
-                     enable_detach=0)
+                     enable_detach=0, enable_attach=0, safety_guard=500)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, thermal_on_after=0.0,
                          cilia=0.0, cilia_on_after=0.0, remove_after=0.0, ss_data=None,
                          is_pacemaker=0, pacemaker_period=0.0, signal_decay=0.9, signal_delay=0.03,
-                         inactive_period=0.03, is_target=0)
+                         inactive_period=0.03, is_target=0, sticky=0)
 
 
 def build_library(force: bool = False) -> str:
@@ -136,6 +138,8 @@ def lib():
         L.vxo_get_contact.argtypes = [vp, dp]
         L.vxo_set_external.argtypes = [vp, C.c_int, C.c_int, dp, dp, fp, fp]
         L.vxo_set_force_field.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
+        L.vxo_set_attach_condition.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
+        L.vxo_get_attach.argtypes = [vp, ip, ip]
         L.vxo_get_link_curve.restype = C.c_int
         L.vxo_get_link_curve.argtypes = [vp, C.c_int, fp, fp]
         L.vxo_get_voxel_material.argtypes = [vp, C.c_int, fp]
@@ -180,7 +184,7 @@ class OracleModel:
                                       m["cilia"], m["cilia_on_after"], m["remove_after"], npts,
                                       (C.c_float * 32)(*ss[0]), (C.c_float * 32)(*ss[1]), int(m["is_pacemaker"]),
                                       m["pacemaker_period"], m["signal_decay"], m["signal_delay"], m["inactive_period"],
-                                      int(m["is_target"]))
+                                      int(m["is_target"]), int(m["sticky"]))
         self._phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         self._E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
         self._cb, self._cs = [None if p[k] is None else np.ascontiguousarray(p[k], dtype=np.float64).reshape(-1)
@@ -193,19 +197,27 @@ class OracleModel:
                      p["temp_base"], p["temp_amplitude"], p["temp_period"],
                      int(p["enable_cilia"]), int(p["secondary_experiment"]),
                      _ptr(self._cb, C.c_double), _ptr(self._cs, C.c_double), int(p["enable_signals"]),
-                     int(p["enable_detach"]))
+                     int(p["enable_detach"]), int(p["enable_attach"]), int(p["safety_guard"]))
         self._L = lib()
         self._h = self._L.vxo_build(C.byref(spec))
         self.nv = self._L.vxo_num_voxels(self._h)
-        self.nl = self._L.vxo_num_links(self._h)
         self.ns = self._L.vxo_num_surface(self._h)
         for e in p.get("externals") or []:
             self.set_external(**e)
         for axis, expr in (p.get("force_field") or {}).items():
             self.set_force_field("xyz".index(axis), expr)
+        for k, expr in enumerate(p.get("attach_conditions") or []):
+            if expr is not None:
+                self.set_attach_condition(k, expr)
 
-    def set_force_field(self, axis: int, expr) -> None:
-        """Force-field formula of one axis as a nested tuple, e.g.
+    @property
+    def nl(self) -> int:
+        """Links so far (attach adds links as the model steps)."""
+        return self._L.vxo_num_links(self._h)
+
+    @staticmethod
+    def _tokens(expr):
+        """A formula as a nested tuple, e.g.
         ("mtMUL", ("mtCONST", 0.01), ("mtSIN", ("mtVAR", "t"))), linearised in
         the reference's token order: breadth-first over the formula's subtree,
         reversed, END last (VX3_SimulationManager::ParseMathTree)."""
@@ -224,9 +236,24 @@ class OracleModel:
                 vals.append(0.0)
         ops.append(MT_CODES["mtEND"])
         vals.append(0.0)
-        o = np.asarray(ops, dtype=np.int32)
-        v = np.asarray(vals, dtype=np.float64)
+        return np.asarray(ops, dtype=np.int32), np.asarray(vals, dtype=np.float64)
+
+    def set_force_field(self, axis: int, expr) -> None:
+        """Force-field formula of one axis (a nested tuple, see _tokens)."""
+        o, v = self._tokens(expr)
         self._L.vxo_set_force_field(self._h, int(axis), len(o), _ptr(o, C.c_int), _ptr(v, C.c_double))
+
+    def set_attach_condition(self, k: int, expr) -> None:
+        """AttachCondition Condition_k (a nested tuple, see _tokens)."""
+        o, v = self._tokens(expr)
+        self._L.vxo_set_attach_condition(self._h, int(k), len(o), _ptr(o, C.c_int), _ptr(v, C.c_double))
+
+    def attach_state(self):
+        """Per voxel the enableAttach flag, per link the updates left whose
+        forces are scaled by 0.01."""
+        vf = np.zeros(self.nv, dtype=np.int32); ln = np.zeros(max(1, self.nl), dtype=np.int32)
+        self._L.vxo_get_attach(self._h, _ptr(vf, C.c_int), _ptr(ln, C.c_int))
+        return vf, ln[:self.nl]
 
     def link_curve(self, l: int) -> np.ndarray:
         st, ss = np.zeros(72, dtype=np.float32), np.zeros(72, dtype=np.float32)
@@ -298,6 +325,12 @@ class OracleModel:
         """Each voxel's localSignal."""
         o = np.zeros(self.nv)
         self._L.vxo_get_signals(self._h, _ptr(o, C.c_double))
+        return o
+
+    def contact(self) -> np.ndarray:
+        """Each voxel's contact force of the last step."""
+        o = np.zeros((self.nv, 3))
+        self._L.vxo_get_contact(self._h, _ptr(o, C.c_double))
         return o
 
     def contact_pairs(self) -> np.ndarray:

@@ -61,7 +61,7 @@ typedef struct {
     int fixed, isMeasured;
     double thermalOnAfter;
     double cilia, ciliaOnAfter, removeAfter;
-    int isPaceMaker, isTarget;
+    int isPaceMaker, isTarget, sticky;
     double paceMakerPeriod, signalDecay, signalDelay, inactivePeriod;
     /* stress-strain model: CVX_Material linear / strainData / stressData */
     int linear, npts;
@@ -115,6 +115,8 @@ struct vxo_model {
     int *pairs, npairs, pairCap;
     /* links */
     int *lneg, *lpos, *laxis;
+    int lcap;              /* capacity of the per-link arrays (links attach at run time) */
+    int *newLink;          /* isNewLink: updates left whose forces are scaled by 0.01 */
     lmat *lm;
     v3 *pos2, *a1v, *a2v;
     float *strain, *maxStrain, *strainOffset, *tArea, *tStrainSum;
@@ -129,6 +131,13 @@ struct vxo_model {
     int enableCollision, enableExpansion, varyTemp;
     int enableCilia, secondary, enableSignals, enableDetach;
     double tempAmplitude, tempPeriod;
+    /* attach: EnableAttach, SafetyGuard, per-voxel enableAttach (VX3_Voxel.h:302,
+     * true until the first voxel update), AttachCondition Condition_0..4 */
+    int enableAttach, safetyGuard;
+    int *vattach;
+    int ac_n[5];
+    int *ac_op[5];
+    double *ac_val[5];
 };
 
 /* ------------------------------------------------------------------ math */
@@ -534,6 +543,8 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->secondary = sp->secondary_experiment;
     m->enableSignals = sp->enable_signals;
     m->enableDetach = sp->enable_detach;
+    m->enableAttach = sp->enable_attach;
+    m->safetyGuard = sp->safety_guard;
 
     int nv = 0;
     for (int i = 0; i < ncell; i++)
@@ -587,6 +598,7 @@ vxo_model *vxo_build(const vxo_spec *sp) {
         vm->signalDelay = in->signal_delay;
         vm->inactivePeriod = in->inactive_period;
         vm->isTarget = in->is_target;
+        vm->sticky = in->sticky;
         vmat_derive(vm);
     }
 
@@ -615,6 +627,8 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->ciliaBase = (v3 *)xcalloc(nv, sizeof(v3));
     m->ciliaShift = (v3 *)xcalloc(nv, sizeof(v3));
     m->vremoved = (int *)xcalloc(nv, sizeof(int));
+    m->vattach = (int *)xcalloc(nv, sizeof(int));
+    for (int i = 0; i < nv; i++) m->vattach[i] = 1;
     double **sig[8] = {&m->sLocal, &m->sLocalDt, &m->sValue, &m->sActive, &m->sInactive, &m->sNextPulse,
                        &m->outValue, &m->outTime};
     for (int k = 0; k < 8; k++) *sig[k] = (double *)xcalloc(nv, sizeof(double));
@@ -674,6 +688,9 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     }
     free(cellvox);
     m->nl = nl;
+    m->lcap = nlmax;
+    m->newLink = (int *)xcalloc(nlmax, sizeof(int));
+    nl = nlmax;  /* the arrays below hold lcap links */
     m->lm = (lmat *)xcalloc(nl, sizeof(lmat));
     m->pos2 = (v3 *)xcalloc(nl, sizeof(v3));
     m->a1v = (v3 *)xcalloc(nl, sizeof(v3));
@@ -690,6 +707,7 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->mNeg = (v3 *)xcalloc(nl, sizeof(v3));
     m->mPos = (v3 *)xcalloc(nl, sizeof(v3));
     m->lremoved = (int *)xcalloc(nl > 0 ? nl : 1, sizeof(int));
+    nl = m->nl;
     for (int l = 0; l < nl; l++) {
         /* combinedMaterial(voxel1 = new voxel, voxel2 = neighbour): symmetric */
         lmat_combine(&m->lm[l], &m->vm[m->vmat_of[m->lpos[l]]], &m->vm[m->vmat_of[m->lneg[l]]]);
@@ -726,7 +744,9 @@ void vxo_free(vxo_model *m) {
                     m->ext_dof, m->ext_t, m->ext_r, m->ext_q, m->ext_f, m->ext_m,
                     m->ff_op[0], m->ff_op[1], m->ff_op[2], m->ff_val[0], m->ff_val[1], m->ff_val[2],
                     m->ciliaBase, m->ciliaShift, m->vremoved, m->lremoved, m->sLocal, m->sLocalDt, m->sValue,
-                    m->sActive, m->sInactive, m->sNextPulse, m->outValue, m->outTime, m->pairs};
+                    m->sActive, m->sInactive, m->sNextPulse, m->outValue, m->outTime, m->pairs, m->newLink,
+                    m->vattach, m->ac_op[0], m->ac_op[1], m->ac_op[2], m->ac_op[3], m->ac_op[4],
+                    m->ac_val[0], m->ac_val[1], m->ac_val[2], m->ac_val[3], m->ac_val[4]};
     for (size_t i = 0; i < sizeof(ptrs) / sizeof(ptrs[0]); i++) free(ptrs[i]);
     free(m);
 }
@@ -770,6 +790,21 @@ static double mt_eval_fifo(const int *op, const double *val, int n, const double
         v[vc++] = r;
     }
     return 0.0;
+}
+
+void vxo_set_attach_condition(vxo_model *m, int k, int n, const int *ops, const double *values) {
+    if (k < 0 || k > 4) return;
+    free(m->ac_op[k]);
+    free(m->ac_val[k]);
+    m->ac_n[k] = n > 0 ? n : 0;
+    m->ac_op[k] = (int *)xcalloc(m->ac_n[k], sizeof(int));
+    m->ac_val[k] = (double *)xcalloc(m->ac_n[k], sizeof(double));
+    for (int i = 0; i < m->ac_n[k]; i++) { m->ac_op[k][i] = ops[i]; m->ac_val[k][i] = values[i]; }
+}
+
+void vxo_get_attach(const vxo_model *m, int *voxel_flags, int *link_new) {
+    if (voxel_flags) memcpy(voxel_flags, m->vattach, sizeof(int) * m->nv);
+    if (link_new) memcpy(link_new, m->newLink, sizeof(int) * m->nl);
 }
 
 void vxo_set_force_field(vxo_model *m, int axis, int n, const int *ops, const double *values) {
@@ -939,10 +974,23 @@ static void link_update(vxo_model *m, int l) {
     }
     fPos = qrotinv(angle2, fPos);
     mPos = qrotinv(angle2, mPos);
-    m->fNeg[l] = to_axis_original(fNeg, axis);
-    m->fPos[l] = to_axis_original(fPos, axis);
-    m->mNeg[l] = to_axis_original(mNeg, axis);
-    m->mPos[l] = to_axis_original(mPos, axis);
+    fNeg = to_axis_original(fNeg, axis);
+    fPos = to_axis_original(fPos, axis);
+    mNeg = to_axis_original(mNeg, axis);
+    mPos = to_axis_original(mPos, axis);
+    /* a link made by attach: its first SafetyGuard updates are scaled by 0.01
+     * (VX3_Link.cu:206-213) */
+    if (m->newLink[l]) {
+        fNeg = vscale(fNeg, 0.01);
+        fPos = vscale(fPos, 0.01);
+        mNeg = vscale(mNeg, 0.01);
+        mPos = vscale(mPos, 0.01);
+        m->newLink[l] -= 1;
+    }
+    m->fNeg[l] = fNeg;
+    m->fPos[l] = fPos;
+    m->mNeg[l] = mNeg;
+    m->mPos[l] = mPos;
     m->lflags[l] = flags;
 }
 
@@ -955,6 +1003,39 @@ static void update_links(vxo_model *m) {
         if (m->vm[m->vmat_of[m->lpos[l]]].fixed && m->vm[m->vmat_of[m->lneg[l]]].fixed) continue;
         link_update(m, l);
     }
+}
+
+/* room for one more link in every per-link array (attach appends links) */
+static void *grow(void *p, size_t old, size_t cap, size_t sz) {
+    char *q = (char *)realloc(p, cap * sz);
+    if (!q) abort();
+    memset(q + old * sz, 0, (cap - old) * sz);
+    return q;
+}
+static void link_reserve(vxo_model *m) {
+    if (m->nl < m->lcap) return;
+    size_t o = (size_t)m->lcap, c = o ? 2 * o : 64;
+    m->lneg = (int *)grow(m->lneg, o, c, sizeof(int));
+    m->lpos = (int *)grow(m->lpos, o, c, sizeof(int));
+    m->laxis = (int *)grow(m->laxis, o, c, sizeof(int));
+    m->newLink = (int *)grow(m->newLink, o, c, sizeof(int));
+    m->lm = (lmat *)grow(m->lm, o, c, sizeof(lmat));
+    m->pos2 = (v3 *)grow(m->pos2, o, c, sizeof(v3));
+    m->a1v = (v3 *)grow(m->a1v, o, c, sizeof(v3));
+    m->a2v = (v3 *)grow(m->a2v, o, c, sizeof(v3));
+    m->strain = (float *)grow(m->strain, o, c, sizeof(float));
+    m->maxStrain = (float *)grow(m->maxStrain, o, c, sizeof(float));
+    m->strainOffset = (float *)grow(m->strainOffset, o, c, sizeof(float));
+    m->tArea = (float *)grow(m->tArea, o, c, sizeof(float));
+    m->tStrainSum = (float *)grow(m->tStrainSum, o, c, sizeof(float));
+    m->restLen = (double *)grow(m->restLen, o, c, sizeof(double));
+    m->lflags = (int *)grow(m->lflags, o, c, sizeof(int));
+    m->fNeg = (v3 *)grow(m->fNeg, o, c, sizeof(v3));
+    m->fPos = (v3 *)grow(m->fPos, o, c, sizeof(v3));
+    m->mNeg = (v3 *)grow(m->mNeg, o, c, sizeof(v3));
+    m->mPos = (v3 *)grow(m->mPos, o, c, sizeof(v3));
+    m->lremoved = (int *)grow(m->lremoved, o ? o : 1, c, sizeof(int));
+    m->lcap = (int)c;
 }
 
 /* ------------------------------------------------------------- collisions */
@@ -977,7 +1058,23 @@ static void signal_receive(vxo_model *m, int v, const vmat *mt, double value, do
  * the tests of handle_collision_attachment (VX3_VoxelyzeKernel.cu:682-729) in
  * its order, then VX3_Collision::updateContactForce (VX3_Collision.cu:17-31).
  * Returns 0 (no effect at all) for pairs the reference skips. */
-static int pair_force(const vxo_model *m, int v1, int v2, v3 *out) {
+/* is_neighbor (VX3_VoxelyzeKernel.cu:651-680): b is reached from a over at
+ * most depth links without going straight back over the link just taken */
+static int linked_within(const vxo_model *m, int a, int b, int incoming, int depth) {
+    if (a == b) return 1;
+    if (depth <= 0) return 0;
+    for (int d = 0; d < 6; d++) {
+        int l = m->vlinks[a][d];
+        if (l < 0 || l == incoming) continue;
+        int o = m->lneg[l] == a ? m->lpos[l] : m->lneg[l];
+        if (linked_within(m, o, b, l, depth - 1)) return 1;
+    }
+    return 0;
+}
+
+/* The tests every effect of a pair passes first (:684-703): not both fixed,
+ * inside the watch distance, not directly linked. */
+static int pair_near(const vxo_model *m, int v1, int v2) {
     const vmat *m1 = &m->vm[m->vmat_of[v1]], *m2 = &m->vm[m->vmat_of[v2]];
     if (m1->fixed && m2->fixed) return 0;
     v3 diff = vsub(m->pos[v1], m->pos[v2]);
@@ -987,14 +1084,25 @@ static int pair_force(const vxo_model *m, int v1, int v2, v3 *out) {
     if (diff.y > watch || diff.y < -watch) return 0;
     if (diff.z > watch || diff.z < -watch) return 0;
     if (vlen(diff) > watch) return 0;
-    /* is_neighbor(voxel1, voxel2, NULL, 1) :651-680 */
-    for (int d = 0; d < 6; d++) {
-        int l = m->vlinks[v1][d];
-        if (l < 0) continue;
-        int other = m->lneg[l] == v1 ? m->lpos[l] : m->lneg[l];
-        if (other == v2) return 0;
-    }
-    if (!m->enableCollision) return 0;
+    return !linked_within(m, v1, v2, -1, 1);
+}
+
+/* A near pair may attach (:729-746): attach on, both voxels' enableAttach,
+ * neither fixed, one material, sticky, and no path of at most 5 links between
+ * them. */
+static int attach_candidate(const vxo_model *m, int v1, int v2) {
+    if (!m->enableAttach || !m->vattach[v1] || !m->vattach[v2]) return 0;
+    const vmat *m1 = &m->vm[m->vmat_of[v1]], *m2 = &m->vm[m->vmat_of[v2]];
+    if (m1->fixed || m2->fixed) return 0;
+    if (m->vmat_of[v1] != m->vmat_of[v2]) return 0;
+    if (!m1->sticky) return 0;
+    return !linked_within(m, v1, v2, -1, 5);
+}
+
+/* VX3_Collision::updateContactForce (VX3_Collision.cu:17-31) of a near pair */
+static v3 contact_force_of(const vxo_model *m, int v1, int v2) {
+    const vmat *m1 = &m->vm[m->vmat_of[v1]], *m2 = &m->vm[m->vmat_of[v2]];
+    double bsa1 = base_size_average(m, v1), bsa2 = base_size_average(m, v2);
     float pk1 = (float)(2 * m1->E * m1->nomSize), pk2 = (float)(2 * m2->E * m2->nomSize);
     double penStiff = 2.0f / (1.0f / pk1 + 1.0f / pk2);
     double dampC = 0.5f * (m1->zetaC * m1->c2xSqMxExS + m2->zetaC * m2->c2xSqMxExS);
@@ -1009,8 +1117,7 @@ static int pair_force(const vxo_model *m, int v1, int v2, v3 *out) {
         double rv = (vel1.x * unit.x + vel1.y * unit.y + vel1.z * unit.z) - (vel2.x * unit.x + vel2.y * unit.y + vel2.z * unit.z);
         force = vscale(unit, penStiff * relDist + dampC * rv);
     }
-    *out = force;
-    return 1;
+    return force;
 }
 
 /* The effects of a pair in contact, in the reference's order: the contact
@@ -1038,18 +1145,87 @@ static void apply_pair(vxo_model *m, int v1, int v2, v3 force) {
     }
 }
 
-static void collide_pair(vxo_model *m, int v1, int v2) {
-    v3 f;
-    if (pair_force(m, v1, v2, &f)) apply_pair(m, v1, v2, f);
+/* what a surface pair does this step: its contact force (contact) and whether
+ * it is an attach candidate */
+typedef struct { int v1, v2, contact, attach; v3 f; } pairhit;
+static int pair_eval(const vxo_model *m, int v1, int v2, pairhit *h) {
+    h->v1 = v1;
+    h->v2 = v2;
+    if (!pair_near(m, v1, v2)) return 0;
+    h->contact = m->enableCollision;  /* with EnableCollision a near pair is in contact */
+    if (h->contact) h->f = contact_force_of(m, v1, v2);
+    h->attach = attach_candidate(m, v1, v2);
+    return h->contact || h->attach;
+}
+
+/* The link of an attach candidate (handle_collision_attachment :748-830 with the
+ * VX3_Link constructor and reset, VX3_Link.cu:31-76): the axis is the dominant
+ * component of voxel2's offset in voxel1's frame; voxel1 takes the slot facing
+ * voxel2 and voxel2 the opposite one, and the voxel holding a NEG-direction
+ * slot is the link's positive end.  Made only when both slots are free; the
+ * pair's contact force (f, when it had one) is given back.  Returns 1 when a
+ * link was made. */
+static int attach_pair(vxo_model *m, int v1, int v2, const v3 *f) {
+    v3 e = vsub(m->pos[v1], m->pos[v2]);
+    v3 ea = qrotinv(m->orient[v1], v3_(-e.x, -e.y, -e.z));
+    double ax = fabs(ea.x), ay = fabs(ea.y), az = fabs(ea.z);
+    int axis, neg1; /* neg1: voxel1 takes its NEG-direction slot (reverseOrder) */
+    if (ax >= ay && ax >= az) { axis = 0; neg1 = ea.x < 0; }
+    else if (ay >= ax && ay >= az) { axis = 1; neg1 = ea.y < 0; }
+    else { axis = 2; neg1 = ea.z < 0; }
+    int d1 = 2 * axis + (neg1 ? 1 : 0), d2 = 2 * axis + (neg1 ? 0 : 1);
+    if (m->vlinks[v1][d1] >= 0 || m->vlinks[v2][d2] >= 0) return 0;
+    link_reserve(m);
+    int l = m->nl++;
+    int neg = neg1 ? v2 : v1, pos = neg1 ? v1 : v2;
+    m->lneg[l] = neg;
+    m->lpos[l] = pos;
+    m->laxis[l] = axis;
+    /* combinedMaterial(positive end, negative end): one material here */
+    lmat_combine(&m->lm[l], &m->vm[m->vmat_of[pos]], &m->vm[m->vmat_of[neg]]);
+    m->lflags[l] = LF_SMALL_ANGLE;
+    /* transverse info at creation with zero Poisson's ratio (VX3_Voxel.cu:478-513) */
+    float sN = (float)m->vm[m->vmat_of[neg]].nomSize, sP = (float)m->vm[m->vmat_of[pos]].nomSize;
+    m->tArea[l] = 0.5f * (sN * sN + sP * sP);
+    m->tStrainSum[l] = 0.0f;
+    update_rest_length(m, l);
+    m->newLink[l] = m->safetyGuard;
+    m->vlinks[v1][d1] = l;
+    m->vlinks[v2][d2] = l;
+    if (f) {
+        m->contact[v1] = vsub(m->contact[v1], *f);
+        m->contact[v2] = vsub(m->contact[v2], v3_(-f->x, -f->y, -f->z));
+    }
+    return 1;
+}
+
+/* The effects of the step's pairs, given in the all-pairs order: every contact
+ * first, then the attach candidates one at a time.  Returns 1 when a link was
+ * made (the surface changes). */
+static int apply_hits(vxo_model *m, pairhit *h, long n) {
+    int made = 0;
+    for (long k = 0; k < n; k++)
+        if (h[k].contact) apply_pair(m, h[k].v1, h[k].v2, h[k].f);
+    for (long k = 0; k < n; k++)
+        if (h[k].attach) made |= attach_pair(m, h[k].v1, h[k].v2, h[k].contact ? &h[k].f : NULL);
+    return made;
 }
 
 /* updateAttach all-pairs branch VX3_VoxelyzeKernel.cu:448-458, literally: every
  * surface pair (first > second) in nested-loop order (gpu_update_attach
  * :833-843 serialised).  O(S^2): the check of the pruned search below. */
-static void update_collisions_all_pairs(vxo_model *m) {
+static int update_collisions_all_pairs(vxo_model *m) {
     m->npairs = 0;
+    long n = 0, cap = 64;
+    pairhit *h = (pairhit *)xcalloc(cap, sizeof(pairhit));
     for (int first = 1; first < m->ns; first++)
-        for (int second = 0; second < first; second++) collide_pair(m, m->surface[first], m->surface[second]);
+        for (int second = 0; second < first; second++) {
+            if (n == cap) { cap *= 2; h = (pairhit *)realloc(h, cap * sizeof(pairhit)); if (!h) abort(); }
+            if (pair_eval(m, m->surface[first], m->surface[second], &h[n])) n++;
+        }
+    int made = apply_hits(m, h, n);
+    free(h);
+    return made;
 }
 
 /* The same pairs in the same (first, second) order, pruned by a uniform grid
@@ -1058,11 +1234,11 @@ static void update_collisions_all_pairs(vxo_model *m) {
  * without an effect).  The candidate search and the pair forces run in
  * parallel over `first`; the effects are applied serially in the all-pairs
  * order, so the result does not depend on the thread count. */
-typedef struct { int v1, v2; v3 f; } pairhit;
-static void update_collisions(vxo_model *m) {
+static int update_collisions(vxo_model *m) {
     int ns = m->ns;
     m->npairs = 0;
-    if (ns < 2 || !m->enableCollision) return;  /* pair_force has no effect without EnableCollision */
+    /* a pair has no effect without EnableCollision or EnableAttach */
+    if (ns < 2 || !(m->enableCollision || m->enableAttach)) return 0;
     double h = 0;
     for (int i = 0; i < ns; i++) {
         double b = base_size_average(m, m->surface[i]);
@@ -1123,26 +1299,30 @@ static void update_collisions(vxo_model *m) {
                 cand[b + 1] = x;
             }
             for (int a = 0; a < cnt; a++) {
-                v3 f;
-                int v1 = m->surface[first], v2 = m->surface[cand[a]];
-                if (!pair_force(m, v1, v2, &f)) continue;
                 if (nh == hcap) { hcap *= 2; hb = (pairhit *)realloc(hb, hcap * sizeof(pairhit)); if (!hb) abort(); }
-                hb[nh].v1 = v1; hb[nh].v2 = v2; hb[nh].f = f;
-                nh++;
+                if (pair_eval(m, m->surface[first], m->surface[cand[a]], &hb[nh])) nh++;
             }
         }
         free(cand);
         hits[t] = hb;
         nhit[t] = nh;
     }
+    long total = 0;
+    for (int t = 0; t < nt; t++) total += nhit[t];
+    pairhit *all = (pairhit *)xcalloc(total, sizeof(pairhit));
+    long at = 0;
     for (int t = 0; t < nt; t++) {
-        for (int k = 0; k < nhit[t]; k++) apply_pair(m, hits[t][k].v1, hits[t][k].v2, hits[t][k].f);
+        memcpy(all + at, hits[t], sizeof(pairhit) * nhit[t]);
+        at += nhit[t];
         free(hits[t]);
     }
+    int made = apply_hits(m, all, total);
+    free(all);
     free(hits);
     free(nhit);
     free(cc);
     free(ent);
+    return made;
 }
 
 /* ---------------------------------------------------------------- signals */
@@ -1353,6 +1533,23 @@ static void update_voxels(vxo_model *m, float dt, int cpu_semantics) {
     }
 }
 
+/* gpu_update_voxels VX3_VoxelyzeKernel.cu:607-621: after its update every
+ * present, non-fixed voxel re-evaluates enableAttach, true when each of the
+ * five AttachCondition formulas is > 0, at its new position with the step's
+ * start time and the collision count so far (angle, closeness and pairs 0) */
+static void update_attach_flags(vxo_model *m) {
+    if (!m->enableAttach) return;
+    for (int v = 0; v < m->nv; v++) {
+        if (m->vremoved[v] || m->vm[m->vmat_of[v]].fixed) continue;
+        const double vars[9] = {m->pos[v].x, m->pos[v].y, m->pos[v].z, (double)m->collisionCount, m->time,
+                                0.0, 0.0, 0.0, (double)m->nv};
+        int ok = 1;
+        for (int k = 0; k < 5; k++)
+            if (m->ac_n[k] && !(mt_eval_fifo(m->ac_op[k], m->ac_val[k], m->ac_n[k], vars) > 0)) ok = 0;
+        m->vattach[v] = ok;
+    }
+}
+
 /* VX3_VoxelyzeKernel::removeVoxels VX3_VoxelyzeKernel.cu:365-399, called at
  * the end of doTimeStep before currentTime advances when SecondaryExperiment
  * is set (:336-349): every voxel of a material whose
@@ -1415,14 +1612,15 @@ int vxo_step(vxo_model *m, float dt, int nsteps, int flags) {
         for (int l = 0; l < m->nl; l++)
             if (m->strain[l] > 100) { diverged = 1; break; }
         if (diverged) return s;
-        if (flags & VXO_ALL_PAIRS) update_collisions_all_pairs(m);
-        else update_collisions(m);
+        int attached = (flags & VXO_ALL_PAIRS) ? update_collisions_all_pairs(m) : update_collisions(m);
         update_voxels(m, d, flags & VXO_CPU_SEMANTICS);
+        update_attach_flags(m);
         /* the end of the step: detached links no longer carry signals (the
-         * reference detaches before the voxel update), removal comes last */
+         * reference detaches before the voxel update), removal comes last; a
+         * new link, a detach or a removal regenerates the surface list */
         int detached = m->enableDetach ? detach_links(m) : 0;
         if (m->enableSignals) signal_deliver(m);
-        if (m->secondary || detached) remove_voxels(m, detached);
+        if (m->secondary || detached || attached) remove_voxels(m, detached || attached);
         m->time += d;
         m->steps++;
     }

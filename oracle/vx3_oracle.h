@@ -19,6 +19,7 @@
  *   removal      src/VX3/VX3_VoxelyzeKernel.cu:336-399, 495-513 (SecondaryExperiment)
  *   signals      src/VX3/VX3_Voxel.cu:271-348 (EnableSignals)
  *   detach       src/VX3/VX3_VoxelyzeKernel.cu:461-475, 946-968 (EnableDetach)
+ *   attach       src/VX3/VX3_VoxelyzeKernel.cu:593-621, 651-831, VX3_Link.cu:31-76, 206-213 (EnableAttach)
  *   CoM          src/VX3/VX3_VoxelyzeKernel.cu:477-493
  *
  * Deliberate (documented) deviations from the racy reference:
@@ -31,6 +32,17 @@
  *     (propagate, pacemaker, decay), then receives the signals its neighbours
  *     sent in that step, in its link direction order X_POS..Z_NEG: the
  *     interleaving in which a voxel's own section precedes what it receives.
+ *   - attach (VX3_VoxelyzeKernel.cu:729-831): every surface pair first adds its
+ *     contact force and is tested against the topology entering the step (the
+ *     depth-5 neighbour search included); then, serially in the all-pairs order
+ *     (first > second), a candidate whose two link slots are still free gets
+ *     its link and gives its contact force back.  The reference runs these
+ *     threads concurrently with unsynchronised slot writes; this is the
+ *     interleaving in which every test precedes every creation and creations
+ *     happen one at a time in pair order.  Attach needs EnableAttach (the
+ *     reference's pair loop also runs for EnableCollision alone, with per-voxel
+ *     flags whose formulas default to an uninitialised value), and a condition
+ *     that is not given does not constrain.
  */
 #ifndef VX3_ORACLE_H
 #define VX3_ORACLE_H
@@ -66,6 +78,7 @@ typedef struct {
     double signal_delay;     /* signalTimeDelay (s) */
     double inactive_period;  /* inactivePeriod (s) */
     int is_target;           /* isTarget */
+    int sticky;              /* Sticky: touching voxels of this material may attach (with enable_attach) */
 } vxo_material;
 
 /* Lattice + simulator + environment description. */
@@ -92,6 +105,8 @@ typedef struct {
     const double *cilia_shift;  /* per-voxel ShiftCiliaForce or NULL */
     int enable_signals;         /* Simulator/EnableSignals */
     int enable_detach;          /* Simulator/AttachDetach/EnableDetach */
+    int enable_attach;          /* Simulator/AttachDetach/EnableAttach */
+    int safety_guard;           /* Simulator/AttachDetach/SafetyGuard: updates a new link's forces are scaled by 0.01 */
 } vxo_spec;
 
 typedef struct vxo_model vxo_model;
@@ -137,6 +152,14 @@ enum { VXO_MT_END, VXO_MT_CONST, VXO_MT_E, VXO_MT_PI, VXO_MT_VAR, VXO_MT_ADD, VX
        VXO_MT_POW, VXO_MT_SQRT, VXO_MT_SIN, VXO_MT_COS, VXO_MT_TAN, VXO_MT_ATAN, VXO_MT_LOG, VXO_MT_INT, VXO_MT_ABS,
        VXO_MT_NOT, VXO_MT_GT, VXO_MT_LT, VXO_MT_AND, VXO_MT_OR, VXO_MT_NORMALCDF };
 void vxo_set_force_field(vxo_model *m, int axis, int n, const int *ops, const double *values);
+/* AttachCondition Condition_k (k = 0..4, VX3_SimulationManager.cu:334-343), tokens
+ * as for force fields.  A voxel may attach when every condition it has is > 0
+ * (a condition never set does not constrain). */
+void vxo_set_attach_condition(vxo_model *m, int k, int n, const int *ops, const double *values);
+/* attach state: per voxel the enableAttach flag of the last voxel update (1
+ * before the first), per link the updates left whose forces are scaled by 0.01
+ * (isNewLink) */
+void vxo_get_attach(const vxo_model *m, int *voxel_flags, int *link_new);
 
 /* stress-strain curve of a non-linear link material (n <= VXO_MAX_CURVE_POINTS
  * points, 0 when linear) */

@@ -188,6 +188,29 @@ def test_history_frames_follow_removal():
     b.close()
 
 
+def test_history_frames_list_attached_links():
+    """.history with attach: the links attach makes join the frame's link list
+    (after the lattice links) from the frame of the step that makes them."""
+    from test_attach_oracle import run_until_links, stacked_cubes
+    prob = stacked_cubes()
+    ref = O.OracleModel(prob)
+    k, _ = run_until_links(ref, 3000)  # the links come in step k + 1 (1-based): frame index k
+    m0 = vx3mi.Model.from_problem(prob)
+    dt = float(np.float32(m0.settings()["dt_frac"] * m0.recommended_dt()))
+    text = W.write_vxa(prob, stop=("mtSUB", ("mtVAR", "t"), ("mtCONST", (k + 60) * dt)), record=(1, 1, 1))
+    m = vx3mi.Model.from_xml(text)
+    assert m.settings()["enable_attach"] == 1
+    b = vx3mi.Batch([m], vx3mi.FP64)
+    log("runner: history with attach")
+    res, hist, _ = vx3mi.run(b, [m])
+    frames = re.findall(r"<<<Step(\d+) Time:[^>]*>>>(.*?)<<<>>>\|\[\[\[\d+\]\]\](.*?)\[\[\[\]\]\]", hist[0])
+    assert len(frames) > 3
+    counts = {(int(j) >= k, vox.count(";"), links.count(";")) for j, vox, links in frames}
+    assert counts == {(False, 16, 24), (True, 16, 28)}
+    assert len(b.attach_state(0)["links"]) == 4
+    b.close()
+
+
 def test_runner_hits_and_target_closeness():
     """Stop formula on hit (collisionCount): the run stops at the first
     iteration whose count exceeds 20; fitness targetCloseness and the report's

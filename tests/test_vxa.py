@@ -230,7 +230,9 @@ BAD = {
     "layer_len": DEMO_LIKE.replace("<![CDATA[0]]>", "<![CDATA[00]]>"),
     "zlib": DEMO_LIKE.replace('Compression="ASCII_READABLE"', 'Compression="ZLIB"'),
     "few_layers": DEMO_LIKE.replace("<Z_Voxels>2</Z_Voxels>", "<Z_Voxels>3</Z_Voxels>"),
-    "attach": DEMO_LIKE.replace("<Simulator>", "<Simulator><AttachDetach><EnableAttach>true</EnableAttach></AttachDetach>"),
+    "attach_angle": DEMO_LIKE.replace("<Simulator>", "<Simulator><AttachDetach><EnableAttach>true</EnableAttach>"
+                                                     "<AttachCondition><Condition_2><mtVAR>angle</mtVAR></Condition_2>"
+                                                     "</AttachCondition></AttachDetach>"),
     "bc_count": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs></Boundary_Conditions>"
                                           "</Environment><VXC "),
     "bc_mesh": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs><FRegion><PrimType>3"
@@ -251,6 +253,37 @@ BAD = {
 def test_malformed_or_unsupported_input_is_a_parse_error(case):
     with pytest.raises(vx3mi.VX3MIError, match="error -2"):
         vx3mi.Model.from_xml(BAD[case])
+
+
+def test_attach_settings_sticky_and_conditions_are_read():
+    # VX3_SimulationManager.cu:329-343 (EnableAttach, SafetyGuard, AttachCondition)
+    # and VX_Object.cpp:1436 (Sticky)
+    text = DEMO_LIKE.replace("<Simulator>", "<Simulator><AttachDetach><EnableAttach>1</EnableAttach>"
+                                            "<SafetyGuard>37</SafetyGuard><AttachCondition><Condition_0>"
+                                            "<mtGREATERTHAN><mtVAR>t</mtVAR><mtCONST>0.5</mtCONST></mtGREATERTHAN>"
+                                            "</Condition_0></AttachCondition></AttachDetach>")
+    text = text.replace("<MatModel>0</MatModel>", "<MatModel>0</MatModel><Sticky>1</Sticky>", 1)
+    m = vx3mi.Model.from_xml(text)
+    st = m.settings()
+    assert st["enable_attach"] == 1 and st["safety_guard"] == 37
+    assert m.material(1)["sticky"] == 1
+    # without the keys: attach off, the reference's SafetyGuard default
+    st0 = vx3mi.Model.from_xml(DEMO_LIKE).settings()
+    assert st0["enable_attach"] == 0 and st0["safety_guard"] == 500
+
+
+def test_attach_model_limits():
+    base = dict(dims=(2, 1, 1), cells=[1, 1], materials=[dict(sticky=1)], enable_attach=1)
+    with pytest.raises(vx3mi.VX3MIError, match="SafetyGuard"):
+        vx3mi.Model.from_problem(dict(base, safety_guard=70000))
+    with pytest.raises(vx3mi.VX3MIError, match="per-voxel"):
+        vx3mi.Model.from_problem(dict(base, voxel_E=[1e6, 2e6]))
+    with pytest.raises(vx3mi.VX3MIError, match="Poisson"):
+        vx3mi.Model.from_problem(dict(base, volume_effects=1, materials=[dict(sticky=1, nu=0.3)]))
+    m = vx3mi.Model.from_problem(base)
+    with pytest.raises(vx3mi.VX3MIError, match="angle"):
+        m.set_attach_condition(1, ("mtVAR", "angle"))
+    m.set_attach_condition(4, ("mtLESSTHAN", ("mtVAR", "z"), ("mtCONST", 0.05)))
 
 
 def test_zero_force_field_and_collision_false_are_accepted():

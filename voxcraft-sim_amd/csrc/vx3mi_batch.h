@@ -28,6 +28,12 @@ struct RobotParams {
     int listLinked;    // lists keep linked lattice neighbours (the robot can lose links or voxels)
     float staleD2;
     double listR;
+    // attach (EnableAttach): SafetyGuard, the AttachCondition programs in the
+    // formula table (length 0: no condition), and the robot's pool of link
+    // entries [poff, poff + pcount) for links made at run time
+    int attach, safetyGuard;
+    int acOff[5], acLen[5];
+    int poff, pcount;
 };
 
 // Candidate lists: every surface voxel lists (ascending) the surface voxels of
@@ -49,7 +55,12 @@ struct RobotTime {
     int collisionCount;  // target / non-target contacts summed over the steps (VX3_VoxelyzeKernel.cu:711-726)
     int listings;        // steps whose collision phase rebuilt the robot's candidate lists
     int divergedNow;     // a link of this step passed the divergence strain (the voxel update sets diverged)
+    int attachUsed;      // pool entries handed out so far (attach)
+    int attachOverflow;  // a voxel had more attach candidates than it keeps (kAttachPerVoxel), or the pool ran out
 };
+
+// attach candidates a surface voxel keeps per step (partners of lower index)
+constexpr int kAttachPerVoxel = 8;
 
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
@@ -71,6 +82,8 @@ struct BatchBase {
     bool anyCollisions = false;
     bool anyRemoval = false;      // some robot can lose voxels (SecondaryExperiment)
     bool anyDetach = false;       // some robot can lose links (EnableDetach with failing materials)
+    bool anyAttach = false;       // some robot can make links (EnableAttach with a sticky material)
+    int LP = 0;                   // link entries: the L lattice links, then the attach pools
     long tableBuckets = 0;         // rebuild-grid prefix entries of the batch
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -131,6 +144,16 @@ struct BatchBase {
     // allocates nothing
     virtual int record_contacts_at(int2 *rec, int *counter, int cap) = 0;
     virtual int contact_pairs(int robot, int *out, int cap) = 0;
+    // attach state of one robot: per voxel the enableAttach flag, and the links
+    // made so far in the order they were made (robot voxel indices; per link the
+    // negative end, positive end, axis and the updates left at 0.01 force);
+    // returns the number of links made (at most cap written), *overflow set when
+    // a candidate was lost
+    virtual int attach_state(int robot, int *vflags, int *neg, int *pos, int *axis, int *newLeft, int cap,
+                             int *overflow) = 0;
+    // the robot's present attached links: negative end, positive end (robot
+    // voxel indices), axis, and the step count at whose end each was made
+    virtual int attach_links(int robot, std::vector<int4> &out) = 0;
     RobotTime *d_rt = nullptr;
 };
 

@@ -231,6 +231,14 @@ int vx3mi_batch_contact_pairs(vx3mi_batch *b, int robot, int *pairs2, int cap) {
     return b->b->contact_pairs(robot, pairs2, cap);
 }
 
+int vx3mi_batch_attach_state(vx3mi_batch *b, int robot, int *voxel_flags, int *link_neg, int *link_pos,
+                             int *link_axis, int *new_left, int cap, int *overflow) {
+    if (!b) { set_error("batch is NULL"); return VX3MI_E_ARG; }
+    if (robot < 0 || robot >= b->b->R) { set_error("robot index out of range"); return VX3MI_E_ARG; }
+    if (cap < 0) { set_error("negative capacity"); return VX3MI_E_ARG; }
+    return b->b->attach_state(robot, voxel_flags, link_neg, link_pos, link_axis, new_left, cap, overflow);
+}
+
 int vx3mi_center_of_mass(vx3mi_batch *b, double *out) {
     if (!b || !out) { set_error("NULL argument"); return VX3MI_E_ARG; }
     return b->b->com(out);

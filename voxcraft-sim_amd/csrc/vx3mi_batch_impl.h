@@ -38,8 +38,8 @@ template <typename T> struct Dev {
     T4<T> *pq0, *pq1, *mo0, *mo1;
     const int *lat, *vmat, *vrob;
     const T *phase;
-    int *vlinks;           // edited only by removal
-    const int2 *lends, *lmeta;
+    int *vlinks;           // edited by removal, detach and attach
+    int2 *lends, *lmeta;   // written for the link entries attach hands out
     T4<T> *ls0, *ls1, *ls2;
     T4<T> *lf0, *lf1, *lf2;
     const VoxelMat *vmats;
@@ -79,6 +79,14 @@ template <typename T> struct Dev {
     VoxelSignal *sig;         // per-voxel signal state, or nullptr when no robot has EnableSignals
     const SignalMat *sigMat;  // per-voxel signal constants
     const int *vtarget;       // per-voxel isTarget, or nullptr when no voxel is a target
+    // attach (nullptr when no robot has EnableAttach): per voxel the palette
+    // material it attaches with (-1: never) and the link material of such a
+    // link, and its enableAttach flag; per surface voxel this step's candidates
+    const int2 *vatt;
+    int *vflagA;
+    int *acand;               // kAttachPerVoxel partner voxels per surface voxel, ascending
+    int *acount;              // candidates found (beyond kAttachPerVoxel: not kept)
+    int *lmade;               // per link entry: the step count at whose end attach made it (0: lattice)
 };
 
 // ------------------------------------------------------------------ device
@@ -87,6 +95,11 @@ template <typename T> __device__ __forceinline__ V3<double> position_d(int lat, 
     if constexpr (kAbsolutePositions<T>) return V3<double>{u.x, u.y, u.z};
     return V3<double>{(lat & 1023) * nomSize + (double)u.x, ((lat >> 10) & 1023) * nomSize + (double)u.y,
                       (lat >> 20) * nomSize + (double)u.z};
+}
+
+// lattice offset of w from v
+__device__ __forceinline__ int3 lat_offset(int pkv, int pkw) {
+    return int3{(pkw & 1023) - (pkv & 1023), ((pkw >> 10) & 1023) - ((pkv >> 10) & 1023), (pkw >> 20) - (pkv >> 20)};
 }
 
 template <typename T> __device__ __forceinline__ void link_update(const Dev<T> &a, int l, int parity) {
@@ -108,8 +121,26 @@ template <typename T> __device__ __forceinline__ void link_update(const Dev<T> &
     st.a2v = {s1.z, s1.w, s2.x};
     st.maxStrain = (float)s2.y;
     st.flags = (int)s2.z;
+    if constexpr (!kAbsolutePositions<T>) {
+        // the displacement form takes the ends one voxel apart along the axis;
+        // an attached link adds the rest of its ends' lattice offset
+        if (st.flags & LF_ATTACHED) {
+            const int3 lo = lat_offset(a.lat[e.x], a.lat[e.y]);
+            const T L = (T)lm.nomSize;
+            du.x += (T)(lo.x - (axis == 0)) * L;
+            du.y += (T)(lo.y - (axis == 1)) * L;
+            du.z += (T)(lo.z - (axis == 2)) * L;
+        }
+    }
     LinkOut<T> o;
     link_forces<T>(lm, a.curves, axis, du, qn, qp, (float)n1.w, (float)p1.w, rt->prevDt[parity], st, o);
+    // isNewLink (VX3_Link.cu:206-213): the first SafetyGuard updates of a link
+    // made by attach are scaled by 0.01, a failed link's zero forces aside
+    if ((st.flags >> LF_NEW_SHIFT) > 0 && !(lm.epsFail != -1.0f && st.maxStrain > lm.epsFail)) {
+        const T k = (T)0.01;
+        o.fN = o.fN * k; o.fP = o.fP * k; o.mN = o.mN * k; o.mP = o.mP * k;
+        st.flags -= 1 << LF_NEW_SHIFT;
+    }
     a.ls0[l] = T4<T>{st.pos2.x, st.pos2.y, st.pos2.z, st.a1v.x};
     a.ls1[l] = T4<T>{st.a1v.y, st.a1v.z, st.a2v.x, st.a2v.y};
     a.ls2[l] = T4<T>{st.a2v.z, (T)st.maxStrain, (T)st.flags, (T)o.strain};
@@ -205,7 +236,7 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
     a.pq1[v] = T4<T>{s.q.x, s.q.y, s.q.z, (T)temp};
     a.mo0[v] = T4<T>{s.p.x, s.p.y, s.p.z, s.h.x};
     a.mo1[v] = T4<T>{s.h.y, s.h.z, (T)s.flags, (T)0};
-    if (rp.collisions) col_track(a, v, s.u, rt, rp, parity);
+    if (rp.collisions || rp.attach) col_track(a, v, s.u, rt, rp, parity);  // attach uses the candidate lists too
     // the voxel's own signal section ends a full timeStep (not one of a voxel
     // with all six degrees of freedom fixed, VX3_Voxel.cu:166-172)
     if (a.sig && rp.signals && dt != 0.0f && !(vm.ext >= 0 && a.exts[vm.ext].dof == DOF_ALL)) {
@@ -292,10 +323,19 @@ template <typename T> __device__ __forceinline__ bool col_linked(const Dev<T> &a
     return a.vlinks[6 * v + d] >= 0;
 }
 
-// lattice offset of w from v
-__device__ __forceinline__ int3 lat_offset(int pkv, int pkw) {
-    return int3{(pkw & 1023) - (pkv & 1023), ((pkw >> 10) & 1023) - ((pkv >> 10) & 1023), (pkw >> 20) - (pkv >> 20)};
+// v and w share a link in any slot: the test for robots whose links can join
+// voxels that are not lattice neighbours (attach)
+template <typename T> __device__ __forceinline__ bool col_linked_any(const Dev<T> &a, int v, int w) {
+    for (int d = 0; d < 6; d++) {
+        const int l = a.vlinks[6 * v + d];
+        if (l >= 0) {
+            const int2 e = a.lends[l];
+            if (e.x == w || e.y == w) return true;
+        }
+    }
+    return false;
 }
+
 
 // pos_v - pos_w: absolute positions in double; in float the lattice offset and
 // the displacement difference are kept apart
@@ -450,7 +490,7 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
     auto partner = [&](int w, bool links, V3<T> &fw) -> int {
         if (!col_active(a, w)) return 0;
         const int3 o = lat_offset(pkv, a.lat[w]);
-        if (links && col_linked(a, v, o.x, o.y, o.z)) return 0;
+        if (links && (rp.attach ? col_linked_any(a, v, w) : col_linked(a, v, o.x, o.y, o.z))) return 0;
         const VoxelMat &mj = a.vmats[a.vmat[w]];
         if (mi.fixed && mj.fixed) return 0;
         const T4<T> qj0 = a.pq0[w];
@@ -566,6 +606,212 @@ __global__ __launch_bounds__(256) void k_contact_links(Dev<T> a, int nl, int nbl
         const int i = (blockIdx.x - nbl) * blockDim.x + threadIdx.x;
         if (i < ns * kContactLanes) col_contact(a, i / kContactLanes, parity, i % kContactLanes);
     }
+}
+
+// ---- attach (handle_collision_attachment VX3_VoxelyzeKernel.cu:729-831) -------
+// Each step, after the contact forces: every surface voxel lists its attach
+// candidates among its partners of lower index (k_attach_find, in parallel,
+// from the topology entering the step); then one workgroup per robot walks the
+// candidates in the order of the reference's pair loop and makes the links
+// whose slots are still free, one at a time (k_attach_walk).  The reference
+// runs these tests and link creations concurrently with unsynchronised slot
+// writes; this is the interleaving in which every test precedes every creation
+// and creations happen in pair order.
+
+// is_neighbor(from, to, NULL, 5) (:651-680): `to` is reached from `from` over at
+// most five links without stepping straight back over the link just taken;
+// depth first in slot order with an explicit stack
+template <typename T> __device__ bool linked_within5(const Dev<T> &a, int from, int to) {
+    if (from == to) return true;
+    int node[5], inc[5], dir[5];
+    int top = 0;
+    node[0] = from;
+    inc[0] = -1;
+    dir[0] = 0;
+    while (top >= 0) {
+        if (dir[top] == 6) {
+            top--;
+            continue;
+        }
+        const int l = a.vlinks[6 * node[top] + dir[top]];
+        dir[top]++;
+        if (l < 0 || l == inc[top]) continue;
+        const int2 e = a.lends[l];
+        const int o = e.x == node[top] ? e.y : e.x;
+        if (o == to) return true;
+        if (top < 4) {
+            top++;
+            node[top] = o;
+            inc[top] = l;
+            dir[top] = 0;
+        }
+    }
+    return false;
+}
+
+// the contact force on v from w, the term col_contact adds for the pair
+template <typename T> __device__ V3<T> pair_contact(const Dev<T> &a, int v, int w) {
+    const VoxelMat &mi = a.vmats[a.vmat[v]], &mj = a.vmats[a.vmat[w]];
+    const T4<T> mi0 = a.mo0[v], mj0 = a.mo0[w];
+    const T bsaI = base_size_average<T>(mi.nomSize, (float)a.pq1[v].w, mi.alphaCTE);
+    const T bsaJ = base_size_average<T>(mj.nomSize, (float)a.pq1[w].w, mj.alphaCTE);
+    const V3<T> diff = col_diff<T>(a.pq0[v], a.pq0[w], lat_offset(a.lat[v], a.lat[w]), (T)mi.nomSize);
+    const V3<T> velI = V3<T>{mi0.x, mi0.y, mi0.z} * (T)mi.massInv;
+    const V3<T> velJ = V3<T>{mj0.x, mj0.y, mj0.z} * (T)mj.massInv;
+    return contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
+}
+
+// The attach candidates of surface voxel v = surf[s] as voxel1 (:684-746):
+// partners w < v inside the contact envelope, not linked, both with
+// enableAttach, of v's sticky material (neither fixed), and with no path of at
+// most five links between them; ascending, the first kAttachPerVoxel kept.
+template <typename T> __device__ void attach_find(const Dev<T> &a, int s) {
+    const int v = a.surf[s];
+    const int r = a.vrob[v];
+    const RobotParams &rp = a.rp[r];
+    int n = 0;
+    if (rp.attach && !a.rt[r].diverged && a.vflagA[v] && a.vatt[v].x >= 0 && col_active(a, v)) {
+        const int mat = a.vatt[v].x;
+        const VoxelMat &mi = a.vmats[a.vmat[v]];
+        const T4<T> qi0 = a.pq0[v];
+        const int pkv = a.lat[v];
+        const T L = (T)mi.nomSize;
+        const T bsaI = base_size_average<T>(mi.nomSize, (float)a.pq1[v].w, mi.alphaCTE);
+        auto test = [&](int w) {
+            if (w >= v || !a.vflagA[w] || a.vatt[w].x != mat || !col_active(a, w)) return;
+            const VoxelMat &mj = a.vmats[a.vmat[w]];
+            const V3<T> diff = col_diff<T>(qi0, a.pq0[w], lat_offset(pkv, a.lat[w]), L);
+            const T bsaJ = base_size_average<T>(mj.nomSize, (float)a.pq1[w].w, mj.alphaCTE);
+            if (!in_contact_envelope(diff, bsaI, bsaJ)) return;
+            if (col_linked_any(a, v, w) || linked_within5(a, v, w)) return;
+            if (n < kAttachPerVoxel) a.acand[(long)s * kAttachPerVoxel + n] = w;
+            n++;
+        };
+        const int nc = a.ccount[s];
+        if (nc <= kMaxCand) {
+            const int *list = a.clist + (long)s * kMaxCand;
+            for (int k = 0; k < nc; k++) test(list[k]);
+        } else {  // ascending partners from the grid, as the contact phase takes them
+            int prev = -1;
+            for (;;) {
+                int next = INT_MAX;
+                col_grid_each(a, rp, s, [&](int sj) {
+                    if (sj > prev && sj < next) next = sj;
+                });
+                if (next == INT_MAX) break;
+                test(a.surf[next]);
+                prev = next;
+            }
+        }
+    }
+    a.acount[s] = n;
+}
+
+template <typename T> __global__ __launch_bounds__(256) void k_attach_find(Dev<T> a, int s0, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) attach_find(a, s0 + i);
+}
+
+// The link of candidate pair (v1, v2) when both slots are free (:748-830, the
+// VX3_Link constructor and reset VX3_Link.cu:31-76): the axis is the dominant
+// component of voxel2's offset in voxel1's frame, voxel1 takes the slot facing
+// voxel2 and voxel2 the opposite one, and the voxel with the NEG-direction slot
+// is the positive end.  The entry comes from the robot's pool (entries of links
+// that left are reused once every entry was handed out); the link starts from
+// the reset state with isNewLink = SafetyGuard; the pair's contact force is
+// given back (:819-825).
+template <typename T>
+__device__ void attach_pair(const Dev<T> &a, const RobotParams &rp, RobotTime *rt, int r, int v1, int v2) {
+    const T4<T> p1 = a.pq0[v1], p1b = a.pq1[v1];
+    const Q4<T> q1 = {p1.w, p1b.x, p1b.y, p1b.z};
+    const V3<T> e = col_diff<T>(p1, a.pq0[v2], lat_offset(a.lat[v1], a.lat[v2]), (T)a.vmats[a.vmat[v1]].nomSize);
+    const V3<T> ea = qrotinv(q1, V3<T>{-e.x, -e.y, -e.z});
+    const T ax = mabs(ea.x), ay = mabs(ea.y), az = mabs(ea.z);
+    int axis;
+    bool neg1;  // voxel1 takes its NEG-direction slot
+    if (ax >= ay && ax >= az) { axis = 0; neg1 = ea.x < 0; }
+    else if (ay >= ax && ay >= az) { axis = 1; neg1 = ea.y < 0; }
+    else { axis = 2; neg1 = ea.z < 0; }
+    const int d1 = 2 * axis + (neg1 ? 1 : 0), d2 = 2 * axis + (neg1 ? 0 : 1);
+    if (a.vlinks[6 * v1 + d1] >= 0 || a.vlinks[6 * v2 + d2] >= 0) return;
+    int l = -1;
+    if (rt->attachUsed < rp.pcount) {
+        l = rp.poff + rt->attachUsed;
+        rt->attachUsed += 1;
+    } else {
+        for (int k = 0; k < rp.pcount && l < 0; k++)
+            if (a.lrem[rp.poff + k]) l = rp.poff + k;
+    }
+    if (l < 0) {
+        rt->attachOverflow = 1;
+        return;
+    }
+    a.lends[l] = int2{neg1 ? v2 : v1, neg1 ? v1 : v2};
+    a.lmeta[l] = int2{axis | (a.vatt[v1].y << 2), r};
+    const T4<T> z4 = {0, 0, 0, 0};
+    a.ls0[l] = z4;
+    a.ls1[l] = z4;
+    a.ls2[l] = T4<T>{0, 0, (T)(LF_SMALL_ANGLE | LF_ATTACHED | (rp.safetyGuard << LF_NEW_SHIFT)), 0};
+    a.lf0[l] = z4;
+    a.lf1[l] = z4;
+    a.lf2[l] = z4;
+    a.lrem[l] = 0;
+    a.lmade[l] = rt->steps + 1;  // the count the voxel update of this step leaves
+    a.vlinks[6 * v1 + d1] = l;
+    a.vlinks[6 * v2 + d2] = l;
+    if (rp.collisions) {
+        const V3<T> f = pair_contact(a, v1, v2);
+        const T4<T> c1 = a.contactW[v1], c2 = a.contactW[v2];
+        a.contactW[v1] = T4<T>{c1.x - f.x, c1.y - f.y, c1.z - f.z, c1.w};
+        a.contactW[v2] = T4<T>{c2.x - (-f.x), c2.y - (-f.y), c2.z - (-f.z), c2.w};
+    }
+}
+
+// one workgroup per robot: its surface voxels' candidates in ascending voxel1
+// (then voxel2) order, made by one work-item
+template <typename T> __global__ __launch_bounds__(256) void k_attach_walk(Dev<T> a, int r0) {
+    const int r = r0 + blockIdx.x;
+    const RobotParams &rp = a.rp[r];
+    RobotTime *rt = a.rt + r;
+    if (!rp.attach || rp.scount == 0 || rt->diverged) return;
+    const int end = rp.soff + rp.scount;
+    for (int base = rp.soff; base < end; base += 256) {
+        const int s = base + (int)threadIdx.x;
+        const int c = s < end ? a.acount[s] : 0;
+        if (!__syncthreads_or(c > 0)) continue;
+        if (threadIdx.x == 0) {
+            for (int i = base; i < end && i < base + 256; i++) {
+                const int n = a.acount[i];
+                if (n > kAttachPerVoxel) rt->attachOverflow = 1;
+                for (int k = 0; k < n && k < kAttachPerVoxel; k++)
+                    attach_pair(a, rp, rt, r, a.surf[i], a.acand[(long)i * kAttachPerVoxel + k]);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// gpu_update_voxels (:607-621): after the voxel update every present, non-fixed
+// voxel of an attach robot re-evaluates enableAttach, true when each
+// AttachCondition given is > 0, at its new position with the step's start time
+// and the collision count (angle, closeness and pairs are not available)
+template <typename T> __global__ __launch_bounds__(256) void k_attach_flags(Dev<T> a, int v0, int n, int parity) {
+    const int v = v0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= v0 + n) return;
+    const int r = a.vrob[v];
+    const RobotParams &rp = a.rp[r];
+    const RobotTime *rt = a.rt + r;
+    if (!rp.attach || rt->diverged || a.vrem[v]) return;
+    const VoxelMat &vm = a.vmats[a.vmat[v]];
+    if (vm.fixed) return;
+    const T4<T> q0 = a.pq0[v];
+    const V3<double> p = position_d<T>(a.lat[v], vm.nomSize, V3<T>{q0.x, q0.y, q0.z});
+    const double vars[MV_COUNT] = {p.x, p.y, p.z, (double)rt->collisionCount, rt->time[parity], 0.0, 0.0, 0.0,
+                                   (double)rp.vcount};
+    int ok = 1;
+    for (int k = 0; k < 5; k++)
+        if (rp.acLen[k] && !(formula_eval(a.formulas + rp.acOff[k], rp.acLen[k], vars) > 0)) ok = 0;
+    a.vflagA[v] = ok;
 }
 
 template <typename T> __global__ __launch_bounds__(256) void k_links(Dev<T> a, int l0, int n, int parity) {
@@ -767,7 +1013,7 @@ template <typename T> struct Batch : BatchBase {
     bool anySignals = false;      // some robot has EnableSignals
     // plain and flat modes only: force fields (their own kernel), removal and
     // signals (kernels between steps)
-    bool plainFlatOnly() const { return anyForceField || anyRemoval || anySignals || anyDetach; }
+    bool plainFlatOnly() const { return anyForceField || anyRemoval || anySignals || anyDetach || anyAttach; }
     // launch configuration used by VX3MI_MODE_AUTO (set by the autotuner)
     int autoMode = 0, tileSel = 0, persistThreads = 256;
     int nLinkMats = 0;  // entries of the device link material table
@@ -794,6 +1040,8 @@ template <typename T> struct Batch : BatchBase {
         std::vector<VoxelCilia> cilia;
         std::vector<SignalMat> sigMat;
         std::vector<int> vtarget;
+        std::vector<int2> vatt;          // attach material and link material per voxel
+        std::vector<char> poolDetach;    // per robot: its attach links can fail and detach
         bool anyTarget = false;
         std::vector<int> remCand, detCand;
         std::vector<double> remAt;
@@ -843,6 +1091,21 @@ template <typename T> struct Batch : BatchBase {
                 if (!post.empty()) anyForceField = true;
                 formulas.insert(formulas.end(), post.begin(), post.end());
             }
+            p.attach = 0;
+            p.safetyGuard = m.safetyGuard;
+            if (m.enableAttach)
+                for (int lmi : m.attachLmat) p.attach |= lmi >= 0;
+            for (int k = 0; k < 5; k++) {
+                std::vector<MtToken> post;
+                if (!m.attachCond[k].empty() && !mt_postfix(m.attachCond[k], post)) {
+                    set_error("malformed attach condition");
+                    return VX3MI_E_ARG;
+                }
+                p.acOff[k] = (int)formulas.size();
+                p.acLen[k] = p.attach ? (int)post.size() : 0;
+                if (p.attach) formulas.insert(formulas.end(), post.begin(), post.end());
+            }
+            if (p.attach) anyAttach = true;
             std::vector<int> vmRemap(m.vmats.size()), lmRemap(m.lmats.size());
             for (size_t k = 0; k < m.vmats.size(); k++) vmRemap[k] = vm_index(m.vmats[k], m.vmatE[k], -1);
             const int extOff = (int)exts.size();
@@ -907,7 +1170,16 @@ template <typename T> struct Batch : BatchBase {
                     remCand.push_back(voff + v);
                     remAt.push_back(pm.remove_after);
                 }
+                const int alm = p.attach ? m.attachLmat[m.vmat[v]] : -1;
+                vatt.push_back(alm < 0 ? int2{-1, 0} : int2{m.vpal[v], lmRemap[alm]});
             }
+            // attach pool: a voxel holds six links, so a robot holds at most
+            // 3 nv links at once and at most 3 nv - nl made by attach
+            p.pcount = p.attach ? std::max(0, 3 * m.nv() - m.nl()) : 0;
+            bool poolCanFail = false;
+            for (int lmi : m.attachLmat)
+                if (lmi >= 0 && m.lmats[lmi].epsFail != -1.0f) poolCanFail = true;
+            poolDetach.push_back(p.attach && m.enableDetach && poolCanFail);
             for (int l = 0; l < m.nl(); l++) {
                 lends.push_back(int2{voff + m.lneg[l], voff + m.lpos[l]});
                 lmeta.push_back(int2{m.laxis[l] | (lmRemap[m.lmat[l]] << 2), r});
@@ -918,6 +1190,22 @@ template <typename T> struct Batch : BatchBase {
         }
         V = (int)lat.size();
         L = (int)lends.size();
+        // the attach pools follow the lattice links of every robot; an entry
+        // not handed out yet is marked removed (-1)
+        LP = L;
+        for (int r = 0; r < n; r++) {
+            RobotParams &p = hrp[r];
+            p.poff = LP;
+            LP += p.pcount;
+            int lm0 = 0;  // any valid link material for the entries not handed out
+            for (int v = p.voff; p.attach && v < p.voff + p.vcount; v++)
+                if (vatt[v].x >= 0) { lm0 = vatt[v].y; break; }
+            for (int k = 0; k < p.pcount; k++) {
+                lends.push_back(int2{p.voff, p.voff});
+                lmeta.push_back(int2{lm0 << 2, r});
+                if (poolDetach[r]) detCand.push_back(p.poff + k);
+            }
+        }
         nLinkMats = (int)lmats.size();
         anyRemoval = !remCand.empty();
         anyDetach = !detCand.empty();
@@ -934,22 +1222,22 @@ template <typename T> struct Batch : BatchBase {
         int rc = VX3MI_OK;
         if ((rc = alloc(&pq0d, V)) || (rc = alloc(&pq1d, V)) || (rc = alloc(&mo0d, V)) || (rc = alloc(&mo1d, V)) ||
             (rc = alloc(&latd, V)) || (rc = alloc(&vmatd, V)) || (rc = alloc(&vrobd, V)) || (rc = alloc(&phased, V)) ||
-            (rc = alloc(&vlinksd, (size_t)6 * V)) || (rc = alloc(&lendsd, L)) || (rc = alloc(&lmetad, L)) ||
-            (rc = alloc(&ls0, L)) || (rc = alloc(&ls1, L)) || (rc = alloc(&ls2, L)) || (rc = alloc(&lf0, L)) ||
-            (rc = alloc(&lf1, L)) || (rc = alloc(&lf2, L)) || (rc = alloc(&d_vmats, hvmats.size())) ||
+            (rc = alloc(&vlinksd, (size_t)6 * V)) || (rc = alloc(&lendsd, LP)) || (rc = alloc(&lmetad, LP)) ||
+            (rc = alloc(&ls0, LP)) || (rc = alloc(&ls1, LP)) || (rc = alloc(&ls2, LP)) || (rc = alloc(&lf0, LP)) ||
+            (rc = alloc(&lf1, LP)) || (rc = alloc(&lf2, LP)) || (rc = alloc(&d_vmats, hvmats.size())) ||
             (rc = alloc(&d_exts, exts.size())) || (rc = alloc(&d_curves, curves.size())) ||
             (rc = alloc(&d_formulas, formulas.size())) ||
             (rc = alloc(&d_lmats, lmats.size())) || (rc = alloc(&d_vmatE, vmatE.size())) || (rc = alloc(&d_rp, R)) ||
             (rc = alloc(&d_rt, R)) || (rc = alloc(&d_out, (size_t)3 * R)))
             return rc;
-        std::vector<T4<T>> zeros((size_t)std::max(V, L), T4<T>{0, 0, 0, 0});
+        std::vector<T4<T>> zeros((size_t)std::max(V, LP), T4<T>{0, 0, 0, 0});
         std::vector<T4<T>> mo1((size_t)V, T4<T>{0, 0, 0, 0});
         for (int r = 0; r < n; r++) {
             const Model &m = models[r]->m;
             const int fl = VF_FLOOR_STATIC_FRICTION | (m.floorEnabled ? VF_FLOOR_ENABLED : 0);
             for (int v = 0; v < m.nv(); v++) mo1[hrp[r].voff + v].z = (T)fl;
         }
-        std::vector<T4<T>> ls2h((size_t)L, T4<T>{0, 0, (T)LF_SMALL_ANGLE, 0});
+        std::vector<T4<T>> ls2h((size_t)LP, T4<T>{0, 0, (T)LF_SMALL_ANGLE, 0});
         HIP_TRY(hipMemcpy(pq0d, pq0.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(pq1d, pq1.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(mo0d, zeros.data(), sizeof(T4<T>) * V, hipMemcpyHostToDevice));
@@ -959,15 +1247,15 @@ template <typename T> struct Batch : BatchBase {
         HIP_TRY(hipMemcpy(vrobd, vrob.data(), sizeof(int) * V, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(phased, phase.data(), sizeof(T) * V, hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(vlinksd, vlinks.data(), sizeof(int) * 6 * V, hipMemcpyHostToDevice));
-        if (L) {
-            HIP_TRY(hipMemcpy(lendsd, lends.data(), sizeof(int2) * L, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(lmetad, lmeta.data(), sizeof(int2) * L, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(ls0, zeros.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(ls1, zeros.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(ls2, ls2h.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(lf0, zeros.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(lf1, zeros.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
-            HIP_TRY(hipMemcpy(lf2, zeros.data(), sizeof(T4<T>) * L, hipMemcpyHostToDevice));
+        if (LP) {
+            HIP_TRY(hipMemcpy(lendsd, lends.data(), sizeof(int2) * LP, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(lmetad, lmeta.data(), sizeof(int2) * LP, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(ls0, zeros.data(), sizeof(T4<T>) * LP, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(ls1, zeros.data(), sizeof(T4<T>) * LP, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(ls2, ls2h.data(), sizeof(T4<T>) * LP, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(lf0, zeros.data(), sizeof(T4<T>) * LP, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(lf1, zeros.data(), sizeof(T4<T>) * LP, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(lf2, zeros.data(), sizeof(T4<T>) * LP, hipMemcpyHostToDevice));
             HIP_TRY(hipMemcpy(d_lmats, lmats.data(), sizeof(LinkMat) * lmats.size(), hipMemcpyHostToDevice));
         }
         HIP_TRY(hipMemcpy(d_vmats, hvmats.data(), sizeof(VoxelMat) * hvmats.size(), hipMemcpyHostToDevice));
@@ -998,7 +1286,7 @@ template <typename T> struct Batch : BatchBase {
             // with removal the list also holds the voxels linked to a removable
             // one (they become surface when it goes), in voxel order
             std::vector<int> rs;
-            if (m.enableCollision) {
+            if (m.enableCollision || p.attach) {  // attach finds its pairs through the same lists
                 std::vector<char> in((size_t)m.nv(), 0);
                 for (int v : m.surface) in[v] = 1;
                 if (m.enableDetach)  // the ends of links that can fail
@@ -1037,7 +1325,7 @@ template <typename T> struct Batch : BatchBase {
                 const double L0 = m.latticeDim;
                 p.listR = (2 * kCOLLISION_ENVELOPE_RADIUS * L0 * fmax + kSkin * L0) * 1.0001;
                 p.staleD2 = (float)((0.49 * kSkin * L0) * (0.49 * kSkin * L0));
-                p.listLinked = (m.enableDetach || m.secondaryExperiment) ? 1 : 0;
+                p.listLinked = (m.enableDetach || m.secondaryExperiment || p.attach) ? 1 : 0;
                 anyCollisions = true;
             }
         }
@@ -1104,15 +1392,17 @@ template <typename T> struct Batch : BatchBase {
         d.nRemCand = 0;
         d.detCand = nullptr;
         d.nDetCand = 0;
-        if (anyRemoval || anyDetach) {
+        if (anyRemoval || anyDetach || anyAttach) {
             int *vr, *lr, *rc0, *dc;
             double *ra;
             const int nc = (int)remCand.size(), nd = (int)detCand.size();
-            if ((rc = alloc(&vr, V)) || (rc = alloc(&lr, L)) || (rc = alloc(&rc0, nc)) || (rc = alloc(&ra, nc)) ||
+            if ((rc = alloc(&vr, V)) || (rc = alloc(&lr, LP)) || (rc = alloc(&rc0, nc)) || (rc = alloc(&ra, nc)) ||
                 (rc = alloc(&dc, nd)))
                 return rc;
             HIP_TRY(hipMemset(vr, 0, sizeof(int) * V));
-            HIP_TRY(hipMemset(lr, 0, sizeof(int) * std::max(L, 1)));
+            std::vector<int> lr0((size_t)std::max(LP, 1), 0);
+            std::fill(lr0.begin() + L, lr0.begin() + LP, -1);  // pool entries not handed out
+            HIP_TRY(hipMemcpy(lr, lr0.data(), sizeof(int) * std::max(LP, 1), hipMemcpyHostToDevice));
             if (nc) HIP_TRY(hipMemcpy(rc0, remCand.data(), sizeof(int) * nc, hipMemcpyHostToDevice));
             if (nc) HIP_TRY(hipMemcpy(ra, remAt.data(), sizeof(double) * nc, hipMemcpyHostToDevice));
             if (nd) HIP_TRY(hipMemcpy(dc, detCand.data(), sizeof(int) * nd, hipMemcpyHostToDevice));
@@ -1126,6 +1416,21 @@ template <typename T> struct Batch : BatchBase {
             if ((rc = alloc(&td, V))) return rc;
             HIP_TRY(hipMemcpy(td, vtarget.data(), sizeof(int) * V, hipMemcpyHostToDevice));
             d.vtarget = td;
+        }
+        d.vatt = nullptr;
+        d.vflagA = d.acand = d.acount = d.lmade = nullptr;
+        if (anyAttach) {
+            int2 *vad;
+            int *vfd, *acd, *acn, *lmd;
+            if ((rc = alloc(&vad, V)) || (rc = alloc(&vfd, V)) || (rc = alloc(&acd, (size_t)std::max(S, 1) * kAttachPerVoxel)) ||
+                (rc = alloc(&acn, std::max(S, 1))) || (rc = alloc(&lmd, LP)))
+                return rc;
+            HIP_TRY(hipMemset(lmd, 0, sizeof(int) * std::max(LP, 1)));
+            const std::vector<int> ones((size_t)V, 1);  // enableAttach is true until the first voxel update
+            HIP_TRY(hipMemcpy(vad, vatt.data(), sizeof(int2) * V, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemcpy(vfd, ones.data(), sizeof(int) * V, hipMemcpyHostToDevice));
+            HIP_TRY(hipMemset(acn, 0, sizeof(int) * std::max(S, 1)));
+            d.vatt = vad; d.vflagA = vfd; d.acand = acd; d.acount = acn; d.lmade = lmd;
         }
         d.sig = nullptr;
         d.sigMat = nullptr;
@@ -1266,20 +1571,28 @@ template <typename T> struct Batch : BatchBase {
     void launch_flat_step(int parity) {
         if (anyCollisions) {
             launch_listing(0, S, 0, R, parity, 0);
-            const int nbl = (L + 255) / 256, nbc = (S * kContactLanes + 255) / 256;
-            k_contact_links<T><<<nbl + nbc, 256, 0, stream>>>(d, L, nbl, S, parity);
-        } else if (L) {
-            k_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, 0, L, parity);
+            const int nbl = (LP + 255) / 256, nbc = (S * kContactLanes + 255) / 256;
+            k_contact_links<T><<<nbl + nbc, 256, 0, stream>>>(d, LP, nbl, S, parity);
+        } else if (LP) {
+            k_links<T><<<(LP + 255) / 256, 256, 0, stream>>>(d, 0, LP, parity);
         }
+        if (anyAttach) launch_attach(0, S, 0, R);
         if (anyForceField) k_force_field<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         k_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
+        if (anyAttach) k_attach_flags<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         if (anyDetach) k_detach<T><<<(d.nDetCand + 255) / 256, 256, 0, stream>>>(d);
         if (anySignals) k_signal_deliver<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V);
         if (anyRemoval) k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
     }
     int kernels_per_flat_step() const {
         return 2 + (anyCollisions ? kCollisionLaunches - 1 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0) +
-               (anyDetach ? 1 : 0);
+               (anyDetach ? 1 : 0) + (anyAttach ? 3 : 0);
+    }
+    // the attach phase of surface voxels [s0, s0 + n) (robots [r0, r0 + nr)):
+    // candidates, then the links made robot by robot
+    void launch_attach(int s0, int n, int r0, int nr) {
+        if (n) k_attach_find<T><<<(n + 255) / 256, 256, 0, stream>>>(d, s0, n);
+        k_attach_walk<T><<<nr, 256, 0, stream>>>(d, r0);
     }
 
     int step(int nsteps, float dt, int mode) override {
@@ -1319,9 +1632,13 @@ template <typename T> struct Batch : BatchBase {
                     const RobotParams &p = hrp[r];
                     if (p.scount) launch_collisions(p.soff, p.scount, r, 1, parity, 1);
                     if (p.lcount) k_links<T><<<(p.lcount + 255) / 256, 256, 0, stream>>>(d, p.loff, p.lcount, parity);
+                    if (p.pcount) k_links<T><<<(p.pcount + 255) / 256, 256, 0, stream>>>(d, p.poff, p.pcount, parity);
+                    if (p.attach) launch_attach(p.soff, p.scount, r, 1);
                     if (anyForceField) k_force_field<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
                     k_voxels<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
-                    launches += 2 + (p.scount ? kCollisionLaunches : 0) + (anyForceField ? 1 : 0);
+                    if (p.attach) k_attach_flags<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
+                    launches += 2 + (p.scount ? kCollisionLaunches : 0) + (anyForceField ? 1 : 0) + (p.pcount ? 1 : 0) +
+                                (p.attach ? 3 : 0);
                 }
                 // the end-of-step phases for the whole batch (each touches only its
                 // robot's voxels and links)
@@ -1464,16 +1781,23 @@ template <typename T> struct Batch : BatchBase {
         // snapshot of everything a step changes
         struct Buf { void *p; size_t bytes; };
         std::vector<Buf> bufs = {{d.pq0, sizeof(T4<T>) * V}, {d.pq1, sizeof(T4<T>) * V}, {d.mo0, sizeof(T4<T>) * V},
-                                 {d.mo1, sizeof(T4<T>) * V}, {d.ls0, sizeof(T4<T>) * L}, {d.ls1, sizeof(T4<T>) * L},
-                                 {d.ls2, sizeof(T4<T>) * L}, {d.lf0, sizeof(T4<T>) * L}, {d.lf1, sizeof(T4<T>) * L},
-                                 {d.lf2, sizeof(T4<T>) * L}, {d_rt, sizeof(RobotTime) * R}};
+                                 {d.mo1, sizeof(T4<T>) * V}, {d.ls0, sizeof(T4<T>) * LP}, {d.ls1, sizeof(T4<T>) * LP},
+                                 {d.ls2, sizeof(T4<T>) * LP}, {d.lf0, sizeof(T4<T>) * LP}, {d.lf1, sizeof(T4<T>) * LP},
+                                 {d.lf2, sizeof(T4<T>) * LP}, {d_rt, sizeof(RobotTime) * R}};
         // (collision lists need no snapshot: they are rebuilt after each restore,
         // and a step's partners do not depend on when the lists were built)
         if (anySignals) bufs.push_back({d.sig, sizeof(VoxelSignal) * (size_t)V});
-        if (anyRemoval || anyDetach) {
+        if (anyRemoval || anyDetach || anyAttach) {
             bufs.push_back({d.vlinks, sizeof(int) * 6 * (size_t)V});
             bufs.push_back({d.vrem, sizeof(int) * (size_t)V});
-            bufs.push_back({d.lrem, sizeof(int) * (size_t)L});
+            bufs.push_back({d.lrem, sizeof(int) * (size_t)LP});
+        }
+        if (anyAttach) {
+            bufs.push_back({d.lends, sizeof(int2) * (size_t)LP});
+            bufs.push_back({d.lmeta, sizeof(int2) * (size_t)LP});
+            bufs.push_back({d.vflagA, sizeof(int) * (size_t)V});
+            if (d.contactW) bufs.push_back({d.contactW, sizeof(T4<T>) * (size_t)V});
+            bufs.push_back({d.lmade, sizeof(int) * (size_t)LP});
         }
         size_t total = 0;
         for (const Buf &b : bufs) total += (b.bytes + 255) & ~(size_t)255;
@@ -1563,6 +1887,63 @@ template <typename T> struct Batch : BatchBase {
         }
         HIP_TRY(hipStreamSynchronize(stream));
         return VX3MI_OK;
+    }
+
+    int attach_links(int robot, std::vector<int4> &out) override {
+        out.clear();
+        const RobotParams &p = hrp[robot];
+        if (!p.attach) return VX3MI_OK;
+        HIP_TRY(hipStreamSynchronize(stream));
+        RobotTime rt;
+        HIP_TRY(hipMemcpy(&rt, d_rt + robot, sizeof(RobotTime), hipMemcpyDeviceToHost));
+        const int used = rt.attachUsed;
+        if (!used) return VX3MI_OK;
+        std::vector<int2> e((size_t)used), mt((size_t)used);
+        std::vector<int> rem((size_t)used), made((size_t)used);
+        HIP_TRY(hipMemcpy(e.data(), d.lends + p.poff, sizeof(int2) * used, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(mt.data(), d.lmeta + p.poff, sizeof(int2) * used, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(rem.data(), d.lrem + p.poff, sizeof(int) * used, hipMemcpyDeviceToHost));
+        HIP_TRY(hipMemcpy(made.data(), d.lmade + p.poff, sizeof(int) * used, hipMemcpyDeviceToHost));
+        for (int k = 0; k < used; k++)
+            if (!rem[k]) out.push_back(int4{e[k].x - p.voff, e[k].y - p.voff, mt[k].x & 3, made[k]});
+        return VX3MI_OK;
+    }
+
+    int attach_state(int robot, int *vflags, int *neg, int *pos, int *axis, int *newLeft, int cap,
+                     int *overflow) override {
+        const RobotParams &p = hrp[robot];
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (vflags) {
+            if (p.attach) HIP_TRY(hipMemcpy(vflags, d.vflagA + p.voff, sizeof(int) * p.vcount, hipMemcpyDeviceToHost));
+            else std::fill(vflags, vflags + p.vcount, 1);
+        }
+        if (overflow) *overflow = 0;
+        if (!p.attach) return 0;
+        RobotTime rt;
+        HIP_TRY(hipMemcpy(&rt, d_rt + robot, sizeof(RobotTime), hipMemcpyDeviceToHost));
+        if (overflow) *overflow = rt.attachOverflow;
+        const int used = rt.attachUsed;
+        std::vector<int2> e((size_t)std::max(used, 1)), mt((size_t)std::max(used, 1));
+        std::vector<T4<T>> s2((size_t)std::max(used, 1));
+        std::vector<int> rem((size_t)std::max(used, 1));
+        if (used) {
+            HIP_TRY(hipMemcpy(e.data(), d.lends + p.poff, sizeof(int2) * used, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(mt.data(), d.lmeta + p.poff, sizeof(int2) * used, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(s2.data(), d.ls2 + p.poff, sizeof(T4<T>) * used, hipMemcpyDeviceToHost));
+            HIP_TRY(hipMemcpy(rem.data(), d.lrem + p.poff, sizeof(int) * used, hipMemcpyDeviceToHost));
+        }
+        int n = 0;  // present links in pool order (the order made, unless entries were reused)
+        for (int k = 0; k < used; k++) {
+            if (rem[k]) continue;
+            if (n < cap) {
+                if (neg) neg[n] = e[k].x - p.voff;
+                if (pos) pos[n] = e[k].y - p.voff;
+                if (axis) axis[n] = mt[k].x & 3;
+                if (newLeft) newLeft[n] = (int)s2[k].z >> LF_NEW_SHIFT;
+            }
+            n++;
+        }
+        return n;
     }
 
     int contact_pairs(int robot, int *out, int cap) override {

@@ -157,8 +157,8 @@ template <typename T> struct DiffBatch : DiffBase {
                 }
             bool removal = false;
             for (const vx3mi_material &pm : m.palette) removal |= m.secondaryExperiment && pm.remove_after > 0;
-            if (m.enableCilia || removal || m.enableSignals || m.enableDetach) {
-                set_error("differentiable mode: cilia, signals, detach and material removal are not supported");
+            if (m.enableCilia || removal || m.enableSignals || m.enableDetach || m.enableAttach) {
+                set_error("differentiable mode: cilia, signals, detach, attach and material removal are not supported");
                 return VX3MI_E_ARG;
             }
             RobotParams &p = hrp[r];

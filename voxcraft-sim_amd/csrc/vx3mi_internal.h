@@ -66,6 +66,12 @@ struct Model {
     int enableCilia = 0;             // Simulator/EnableCilia
     int enableSignals = 0;           // Simulator/EnableSignals
     int enableDetach = 0;            // Simulator/AttachDetach/EnableDetach
+    int enableAttach = 0;            // Simulator/AttachDetach/EnableAttach
+    int safetyGuard = 500;           // Simulator/AttachDetach/SafetyGuard
+    std::vector<MtToken> attachCond[5];  // AttachCondition/Condition_0..4 (empty: none)
+    // per voxel material: the link material (lmats index) a link between two
+    // voxels of it gets when they attach, -1 when they cannot (not sticky, fixed)
+    std::vector<int> attachLmat;
     double maxDistPairs = 0.0;       // Simulator/MaxDistInVoxelLengthsToCountAsPair
     std::vector<double> ciliaBase, ciliaShift;  // 3 per voxel (empty: zeros)
     int nv() const { return (int)ix.size(); }

@@ -279,6 +279,24 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
     m.enableSignals = sp->enable_signals ? 1 : 0;
     m.enableDetach = sp->enable_detach ? 1 : 0;
     m.maxDistPairs = sp->max_dist_pairs;
+    m.enableAttach = sp->enable_attach ? 1 : 0;
+    m.safetyGuard = sp->safety_guard;
+    if (m.enableAttach) {
+        if (sp->safety_guard < 0 || sp->safety_guard > kMaxSafetyGuard) {
+            set_error("SafetyGuard must lie in 0..65535");
+            return VX3MI_E_ARG;
+        }
+        if (sp->voxel_E) {  // every voxel is a material of its own: nothing could attach
+            set_error("EnableAttach with a per-voxel Young's modulus is not supported");
+            return VX3MI_E_ARG;
+        }
+        for (int k = 0; k < sp->nmat; k++)
+            if (sp->mats[k].sticky && sp->volume_effects && sp->mats[k].poissons_ratio != 0.0f) {
+                // a new link's transverse area would follow its voxels' current strains
+                set_error("EnableAttach with a sticky material of non-zero Poisson's ratio under volume effects is not supported");
+                return VX3MI_E_ARG;
+            }
+    }
     m.secondaryExperiment = sp->secondary_experiment ? 1 : 0;
 
     // UpdateMatTemps (VX_Sim.cpp:467-483) via UpdateCurTemp (VX_Environment.cpp:327-346)
@@ -373,6 +391,34 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
     // links: created when the later voxel of a pair is added (addVoxel's
     // X_POS..Z_NEG loop only finds lower neighbours), pVNeg = lower coordinate
     std::map<std::pair<int, int>, int> lmatIndex;
+    // the link material between voxel materials mn (negative end) and mp
+    // (positive end), made on first use
+    auto link_mat = [&](int mn, int mp) -> int {
+        std::pair<int, int> key(mn, mp);
+        auto it = lmatIndex.find(key);
+        if (it != lmatIndex.end()) return it->second;
+        // combinedMaterial(voxel1 = new voxel, voxel2 = neighbour); the result is symmetric
+        std::vector<CurvePt> curve;
+        LinkMat l = combine(vm[mp], vm[mn], curve);
+        if (!curve.empty()) {
+            l.curve = ((int)m.curves.size() << 8) | (int)curve.size();
+            m.curves.insert(m.curves.end(), curve.begin(), curve.end());
+        }
+        const MatDerived &dn = vm[mn], &dp = vm[mp];
+        l.nomN = dn.nomSize; l.nomP = dp.nomSize;
+        l.alphaN = dn.alpha; l.alphaP = dp.alpha;
+        l.dmNumN = 2 * dn.sqrtMass * dn.zI;
+        l.dmNumP = 2 * dp.sqrtMass * dp.zI;
+        // transverse info frozen at creation (VX3_Link.cu:147-150; all strains zero)
+        float sN = (float)dn.nomSize, sP = (float)dp.nomSize;
+        l.tArea = 0.5f * (sN * sN + sP * sP);
+        l.tStrainSum = 0.0f;
+        l.fixedBoth = (dn.fixed && dp.fixed) ? 1 : 0;
+        const int lmi = (int)m.lmats.size();
+        m.lmats.push_back(l);
+        lmatIndex[key] = lmi;
+        return lmi;
+    };
     for (int v = 0; v < nv; v++) {
         int x = m.ix[v], y = m.iy[v], z = m.iz[v];
         int nb[3] = {x > 0 ? cellvox[(x - 1) + (long)nx * (y + (long)ny * z)] : -1,
@@ -387,36 +433,16 @@ static int build_model(const vx3mi_lattice *sp, Model &m) {
             m.laxis.push_back(a);
             m.vlinks[(size_t)v * 6 + 2 * a + 1] = li;
             m.vlinks[(size_t)n * 6 + 2 * a] = li;
-            std::pair<int, int> key(m.vmat[n], m.vmat[v]);
-            auto it = lmatIndex.find(key);
-            int lmi;
-            if (it == lmatIndex.end()) {
-                // combinedMaterial(voxel1 = new voxel, voxel2 = neighbour); the result is symmetric
-                std::vector<CurvePt> curve;
-                LinkMat l = combine(vm[m.vmat[v]], vm[m.vmat[n]], curve);
-                if (!curve.empty()) {
-                    l.curve = ((int)m.curves.size() << 8) | (int)curve.size();
-                    m.curves.insert(m.curves.end(), curve.begin(), curve.end());
-                }
-                const MatDerived &dn = vm[m.vmat[n]], &dp = vm[m.vmat[v]];
-                l.nomN = dn.nomSize; l.nomP = dp.nomSize;
-                l.alphaN = dn.alpha; l.alphaP = dp.alpha;
-                l.dmNumN = 2 * dn.sqrtMass * dn.zI;
-                l.dmNumP = 2 * dp.sqrtMass * dp.zI;
-                // transverse info frozen at creation (VX3_Link.cu:147-150; all strains zero)
-                float sN = (float)dn.nomSize, sP = (float)dp.nomSize;
-                l.tArea = 0.5f * (sN * sN + sP * sP);
-                l.tStrainSum = 0.0f;
-                l.fixedBoth = (dn.fixed && dp.fixed) ? 1 : 0;
-                lmi = (int)m.lmats.size();
-                m.lmats.push_back(l);
-                lmatIndex[key] = lmi;
-            } else {
-                lmi = it->second;
-            }
-            m.lmat.push_back(lmi);
+            m.lmat.push_back(link_mat(m.vmat[n], m.vmat[v]));
         }
     }
+    // the link material two voxels of one sticky material get when they attach
+    // (VX3_Link constructor, VX3_Link.cu:31-57: combinedMaterial of the two ends)
+    m.attachLmat.assign(vm.size(), -1);
+    if (m.enableAttach)
+        for (int c = 0; c < sp->nmat; c++)
+            if (sp->mats[c].sticky && palToVm[c] >= 0 && !vm[palToVm[c]].fixed)
+                m.attachLmat[palToVm[c]] = link_mat(palToVm[c], palToVm[c]);
     for (const MatDerived &d : vm) {
         m.vmats.push_back(to_voxel_mat(d));
         m.vmatE.push_back(d.E);
@@ -516,6 +542,8 @@ int vx3mi_model_settings(const vx3mi_model *m, vx3mi_settings *o) {
     o->enable_signals = M.enableSignals;
     o->enable_detach = M.enableDetach;
     o->max_dist_pairs = M.maxDistPairs;
+    o->enable_attach = M.enableAttach;
+    o->safety_guard = M.safetyGuard;
     return VX3MI_OK;
 }
 

@@ -202,6 +202,12 @@ constexpr int LF_ROTVEC_ACOS1 = 4;
 constexpr int LF_ROTVEC_ACOS2 = 8;
 constexpr int LF_POSX_EXACT = 16;
 constexpr int LF_DECISIONS = LF_ROTVEC_ACOS1 | LF_ROTVEC_ACOS2 | LF_POSX_EXACT;
+// a link made by attach (its ends are not lattice neighbours), and the updates
+// left whose forces are scaled by 0.01 (isNewLink, VX3_Link.cu:206-213) in the
+// bits from LF_NEW_SHIFT; the flags travel as a float, exact below 2^24
+constexpr int LF_ATTACHED = 32;
+constexpr int LF_NEW_SHIFT = 8;
+constexpr int kMaxSafetyGuard = 65535;
 
 // ---- discrete decisions of the piecewise equations ------------------------------
 // Each branch point of the reference's equations where a continuous quantity is

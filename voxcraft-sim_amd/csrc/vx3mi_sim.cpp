@@ -131,12 +131,15 @@ std::string history_header(const Model &m) {
 // one .history frame (VX3_SimulationManager.cu:61-111) from the robot's
 // positions, orientations, temperatures and link strains.  With removal
 // (vrem / lrem: the step count at whose end a voxel / link went, 0 present;
-// stampNow: the step count after the frame's step) the voxels listed are those
-// of the surface list regenerated at the start of the frame's step that are
-// still present, and removed links are left out.
+// stampNow: the step count after the frame's step) or attach (att: the links
+// attach made that are present, with the step count at whose end each was
+// made) the voxels listed are those of the surface list regenerated at the
+// start of the frame's step that are still present, removed links are left
+// out and attached links follow the lattice links.
 std::string history_frame(const Model &m, long long j, double t, const double *pos, const double *q,
                           const float *temp, const float *strain, const int *vrem = nullptr,
-                          const int *lrem = nullptr, long long stampNow = 0, const double *signal = nullptr) {
+                          const int *lrem = nullptr, long long stampNow = 0, const double *signal = nullptr,
+                          const std::vector<int4> *att = nullptr) {
     std::string s;
     const double vs = 1 / 0.001;
     auto link_at = [&](int v, int d) {  // the voxel's link in direction d now, -1 none
@@ -144,18 +147,22 @@ std::string history_frame(const Model &m, long long j, double t, const double *p
         return (l >= 0 && lrem && lrem[l]) ? -1 : l;
     };
     std::vector<int> listed;
-    if (vrem) {
+    if (vrem || att) {
+        std::vector<int> extra((size_t)m.nv(), 0);  // slots attach had filled when the frame's step began
+        if (att)
+            for (const int4 &a : *att)
+                if (a.w < stampNow) { extra[a.x]++; extra[a.y]++; }
         for (int v = 0; v < m.nv(); v++) {
-            if (vrem[v]) continue;
-            bool surface = false;
+            if (vrem && vrem[v]) continue;
+            int filled = extra[v];
             for (int d = 0; d < 6; d++) {
                 const int l = m.vlinks[(size_t)v * 6 + d];
-                surface |= l < 0 || (lrem[l] != 0 && lrem[l] < stampNow);
+                filled += l >= 0 && !(lrem && lrem[l] != 0 && lrem[l] < stampNow);
             }
-            if (surface) listed.push_back(v);
+            if (filled < 6) listed.push_back(v);
         }
     }
-    const std::vector<int> &voxels = vrem ? listed : m.surface;
+    const std::vector<int> &voxels = (vrem || att) ? listed : m.surface;
     if (m.recordVoxel) {
         appendf(s, "<<<Step%lld Time:%f>>>", j, t);
         for (int v : voxels) {
@@ -199,6 +206,12 @@ std::string history_frame(const Model &m, long long j, double t, const double *p
             appendf(s, "%.4f,%.4f,%.4f,", pos[3 * b], pos[3 * b + 1], pos[3 * b + 2]);
             s += ";";
         }
+        if (att)
+            for (const int4 &l : *att) {
+                appendf(s, "%.4f,%.4f,%.4f,", pos[3 * l.y], pos[3 * l.y + 1], pos[3 * l.y + 2]);
+                appendf(s, "%.4f,%.4f,%.4f,", pos[3 * l.x], pos[3 * l.x + 1], pos[3 * l.x + 2]);
+                s += ";";
+            }
         s += "[[[]]]";
     }
     s += "\n";
@@ -226,6 +239,7 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
     std::vector<double> pos, quat;
     std::vector<float> temp, strain;
     std::vector<int> vrem, lrem;
+    std::vector<int4> attached;
     std::vector<double> signal;
     int rc = b.com(com.data());
     if (rc) return rc;
@@ -400,8 +414,13 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
                     if ((rc = b.get_signals(r, signal.data()))) return rc;
                     sg = signal.data();
                 }
+                const std::vector<int4> *at = nullptr;
+                if (b.anyAttach && m.enableAttach) {
+                    if ((rc = b.attach_links(r, attached))) return rc;
+                    at = &attached;
+                }
                 const std::string f = history_frame(m, x.j - 1, x.t, pos.data(), quat.data(), temp.data(), strain.data(),
-                                                    vr, lr, (long long)steps0[r] + x.j, sg);
+                                                    vr, lr, (long long)steps0[r] + x.j, sg, at);
                 history(user, r, f.data(), f.size());
             }
         }

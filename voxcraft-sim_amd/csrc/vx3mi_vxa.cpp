@@ -300,6 +300,7 @@ struct VxaContent {
     std::vector<vx3mi_material> mats;
     std::vector<MtToken> stop, fitness;
     std::vector<MtToken> forceField[3];
+    std::vector<MtToken> attachCond[5];
     std::string rawPrint;
     int savePositions = 0;
     int recordStepSize = 0, recordVoxel = 1, recordLink = 0;
@@ -509,8 +510,10 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
         // VX3 keys (VX3_SimulationManager.cu:323-376); EnableCollision defaults to true
         const XNode *ad = sim->child("AttachDetach");
         L.enable_collision = load_bool(ad, "EnableCollision", true);
-        if (load_bool(ad, "EnableAttach", false)) { err = "EnableAttach is not supported"; return VX3MI_E_PARSE; }
+        L.enable_attach = load_bool(ad, "EnableAttach", false);
         L.enable_detach = load_bool(ad, "EnableDetach", false);
+        L.safety_guard = 500;
+        load(ad, "SafetyGuard", L.safety_guard);
         int v;
         if (load(sim, "EnableExpansion", v)) L.enable_expansion = v;
         if (load(sim, "EnableCilia", v)) L.enable_cilia = v != 0;
@@ -549,8 +552,25 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
                 c.forceField[a] = f;
             }
         }
+        // AttachDetach/AttachCondition/Condition_0..4 (VX3_SimulationManager.cu:334-343)
+        for (int k = 0; k < 5; k++) {
+            const std::string name = "Condition_" + std::to_string(k);
+            std::vector<MtToken> f, post;
+            if ((rc = parse_formula(path(ad, {"AttachCondition", name.c_str()}), f, err)) != VX3MI_OK) return rc;
+            if (f.empty()) continue;
+            if (mt_vars_used(f) & ((1u << MV_ANGLE) | (1u << MV_CLOSENESS) | (1u << MV_PAIRS))) {
+                err = "attach conditions that use the recent angle, targetCloseness or numClosePairs are not supported";
+                return VX3MI_E_PARSE;
+            }
+            if (!mt_postfix(f, post)) {
+                err = "attach condition nested too deeply";
+                return VX3MI_E_PARSE;
+            }
+            c.attachCond[k] = f;
+        }
     } else {
         L.enable_collision = 1;
+        L.safety_guard = 500;
     }
     // ---- environment (CVX_Environment ctor defaults, src/old/VX_Environment.cpp:22-35)
     L.grav_acc = -9.81f;
@@ -673,6 +693,7 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
                 load(mech, "uDynamic", m.u_dynamic);
                 if (load(mech, "isTarget", iv)) m.is_target = iv != 0;
                 if (load(mech, "Fixed", iv)) m.fixed = iv != 0;
+                if (load(mech, "Sticky", iv)) m.sticky = iv != 0;
                 if (load(mech, "isMeasured", iv)) m.is_measured = iv;
                 load(mech, "TurnOnThermalExpansionAfterThisManySeconds", m.thermal_on_after);
                 load(mech, "Cilia", m.cilia);
@@ -804,6 +825,7 @@ int model_from_docs(const char *baseText, const char *vxdText, vx3mi_model **out
         m.stopFormula = c.stop;
         m.fitnessFormula = c.fitness;
         for (int a = 0; a < 3; a++) m.forceField[a] = c.forceField[a];
+        for (int k = 0; k < 5; k++) m.attachCond[k] = c.attachCond[k];
         m.secondaryExperiment = c.secondaryExperiment;
         m.reinitInitialPositionAfter = c.reinitAfter;
         m.recordStepSize = c.recordStepSize;
@@ -856,6 +878,30 @@ int vx3mi_model_set_force_field(vx3mi_model *m, int axis, const char *formula_xm
     }
     if (!f.empty() && !mt_postfix(f, post)) { set_error("force-field formula nested too deeply"); return VX3MI_E_PARSE; }
     m->m.forceField[axis] = f;
+    return VX3MI_OK;
+}
+
+int vx3mi_model_set_attach_condition(vx3mi_model *m, int k, const char *formula_xml) {
+    if (!m || k < 0 || k > 4) { set_error("vx3mi_model_set_attach_condition: bad model or condition index"); return VX3MI_E_ARG; }
+    std::vector<MtToken> f, post;
+    try {
+        if (formula_xml && *formula_xml) {
+            const std::string wrapped = std::string("<f>") + formula_xml + "</f>";
+            XNode doc = XmlReader(wrapped.c_str()).parse_document();
+            std::string err;
+            const int rc = parse_formula(doc.child("f"), f, err);
+            if (rc != VX3MI_OK) { set_error(err); return rc; }
+        }
+    } catch (const XmlError &e) {
+        set_error("XML: " + e.msg);
+        return VX3MI_E_PARSE;
+    }
+    if (!f.empty() && (mt_vars_used(f) & ((1u << MV_ANGLE) | (1u << MV_CLOSENESS) | (1u << MV_PAIRS)))) {
+        set_error("attach conditions that use the recent angle, targetCloseness or numClosePairs are not supported");
+        return VX3MI_E_PARSE;
+    }
+    if (!f.empty() && !mt_postfix(f, post)) { set_error("attach condition nested too deeply"); return VX3MI_E_PARSE; }
+    m->m.attachCond[k] = f;
     return VX3MI_OK;
 }
 

@@ -32,13 +32,13 @@ PROBLEM_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp
                         floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
                         temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None,
                         enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None, enable_signals=0,
-                        enable_detach=0, max_dist_pairs=0.0)
+                        enable_detach=0, max_dist_pairs=0.0, enable_attach=0, safety_guard=500)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, is_target=0, thermal_on_after=0.0,
                          cilia=0.0, cilia_on_after=0.0, remove_after=0.0, ss_data=None,
                          is_pacemaker=0, pacemaker_period=0.0, signal_decay=0.9, signal_delay=0.03,
-                         inactive_period=0.03, rgba=(0.5, 0.5, 0.5, 1.0))
+                         inactive_period=0.03, rgba=(0.5, 0.5, 0.5, 1.0), sticky=0)
 
 
 class VX3MIError(RuntimeError):
@@ -66,7 +66,7 @@ class _Material(C.Structure):
                 ("cilia", C.c_double), ("cilia_on_after", C.c_double), ("remove_after", C.c_double),
                 ("ss_points", C.c_int), ("ss_strain", C.c_float * 32), ("ss_stress", C.c_float * 32),
                 ("is_pacemaker", C.c_int), ("pacemaker_period", C.c_double), ("signal_value_decay", C.c_double),
-                ("signal_time_delay", C.c_double), ("inactive_period", C.c_double)]
+                ("signal_time_delay", C.c_double), ("inactive_period", C.c_double), ("sticky", C.c_int)]
 
 
 class _Settings(C.Structure):
@@ -77,7 +77,8 @@ class _Settings(C.Structure):
                 ("record_step_size", C.c_int), ("record_voxel", C.c_int), ("record_link", C.c_int),
                 ("has_stop_formula", C.c_int), ("has_fitness_formula", C.c_int),
                 ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int), ("enable_signals", C.c_int),
-                ("enable_detach", C.c_int), ("max_dist_pairs", C.c_double)]
+                ("enable_detach", C.c_int), ("max_dist_pairs", C.c_double), ("enable_attach", C.c_int),
+                ("safety_guard", C.c_int)]
 
 
 FORMULA_STOP, FORMULA_FITNESS = 0, 1
@@ -111,7 +112,8 @@ class _Lattice(C.Structure):
                 ("vary_temp", C.c_int), ("temp_base", C.c_double), ("temp_amplitude", C.c_double),
                 ("temp_period", C.c_double), ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int),
                 ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double)),
-                ("enable_signals", C.c_int), ("enable_detach", C.c_int), ("max_dist_pairs", C.c_double)]
+                ("enable_signals", C.c_int), ("enable_detach", C.c_int), ("max_dist_pairs", C.c_double),
+                ("enable_attach", C.c_int), ("safety_guard", C.c_int)]
 
 
 def library_path() -> str:
@@ -148,6 +150,8 @@ def lib():
         "vx3mi_model_link_constants": (C.c_int, [vp, C.c_int, fp]),
         "vx3mi_model_link_curve": (C.c_int, [vp, C.c_int, fp, fp]),
         "vx3mi_model_set_force_field": (C.c_int, [vp, C.c_int, C.c_char_p]),
+        "vx3mi_model_set_attach_condition": (C.c_int, [vp, C.c_int, C.c_char_p]),
+        "vx3mi_batch_attach_state": (C.c_int, [vp, C.c_int, ip, ip, ip, ip, ip, C.c_int, ip]),
         "vx3mi_model_set_external": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp, dp]),
         "vx3mi_model_get_external": (C.c_int, [vp, C.c_int, ip, dp, dp, dp, dp]),
         "vx3mi_batch_create": (C.c_int, [pvp, C.c_int, C.c_int, pvp]),
@@ -242,7 +246,7 @@ class Model:
                                 r, g, b, a, float(m["cilia"]), float(m["cilia_on_after"]), float(m["remove_after"]),
                                 len(ss[0]), (C.c_float * 32)(*ss[0]), (C.c_float * 32)(*ss[1]), int(m["is_pacemaker"]),
                                 float(m["pacemaker_period"]), float(m["signal_decay"]), float(m["signal_delay"]),
-                                float(m["inactive_period"]))
+                                float(m["inactive_period"]), int(m["sticky"]))
         phase = None if p["phase"] is None else np.ascontiguousarray(p["phase"], dtype=np.float64)
         E = None if p["voxel_E"] is None else np.ascontiguousarray(p["voxel_E"], dtype=np.float32)
         cb, cs = [None if p[k] is None else np.ascontiguousarray(p[k], dtype=np.float64).reshape(-1)
@@ -254,7 +258,8 @@ class Model:
                         int(p["temp_enabled"]), int(p["vary_temp"]), p["temp_base"], p["temp_amplitude"],
                         p["temp_period"], int(p["enable_cilia"]), int(p["secondary_experiment"]),
                         _ptr(cb, C.c_double), _ptr(cs, C.c_double), int(p["enable_signals"]),
-                        int(p["enable_detach"]), float(p["max_dist_pairs"]))
+                        int(p["enable_detach"]), float(p["max_dist_pairs"]), int(p["enable_attach"]),
+                        int(p["safety_guard"]))
         h = C.c_void_p()
         _check(lib().vx3mi_model_from_lattice(C.byref(spec), C.byref(h)))
         m = cls(h.value)
@@ -262,7 +267,17 @@ class Model:
             m.set_external(**e)
         for axis, expr in (p.get("force_field") or {}).items():
             m.set_force_field(axis, expr)
+        for k, expr in enumerate(p.get("attach_conditions") or []):
+            if expr is not None:
+                m.set_attach_condition(k, expr)
         return m
+
+    def set_attach_condition(self, k: int, expr) -> None:
+        """AttachCondition Condition_k (0..4) as a nested tuple (the format of
+        vx3mi.vxa.formula_xml) or MathTree XML text; None removes it."""
+        from .vxa import formula_xml
+        text = "" if expr is None else (expr if isinstance(expr, str) else formula_xml(expr))
+        _check(lib().vx3mi_model_set_attach_condition(self._h, int(k), text.encode()))
 
     def set_force_field(self, axis, expr) -> None:
         """Force-field formula of axis ("x", "y", "z" or 0..2) as a nested tuple
@@ -515,6 +530,23 @@ class Batch:
 
     def configure(self, mode: int, tile: int = 0, threads: int = 0):
         _check(lib().vx3mi_batch_configure(self._h, int(mode), int(tile), int(threads)))
+
+    def attach_state(self, robot: int = 0) -> dict:
+        """Per voxel the enableAttach flag; the links attach made (robot voxel
+        indices: negative end, positive end, axis, updates left at 0.01 force);
+        overflow when an attach candidate was lost."""
+        nv = self.models[robot].nv
+        vf = np.zeros(nv, dtype=np.int32)
+        ov = C.c_int()
+        n = _check(lib().vx3mi_batch_attach_state(self._h, int(robot), _ptr(vf, C.c_int), None, None, None, None, 0,
+                                                  C.byref(ov)), allow=range(0, 1 << 30))
+        links = np.zeros((n, 4), dtype=np.int32)
+        cols = [np.zeros(max(n, 1), dtype=np.int32) for _ in range(4)]
+        if n:
+            _check(lib().vx3mi_batch_attach_state(self._h, int(robot), None, *[_ptr(c, C.c_int) for c in cols], n,
+                                                  C.byref(ov)), allow=range(0, 1 << 30))
+            links = np.stack([c[:n] for c in cols], axis=1)
+        return dict(flags=vf, links=links, overflow=int(ov.value))
 
     def eval_link_forces(self) -> np.ndarray:
         out = np.zeros((self.nl, 12))

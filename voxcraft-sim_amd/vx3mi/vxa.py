@@ -42,6 +42,17 @@ def formula_xml(expr) -> str:
     return _el(op, "".join(formula_xml(e) for e in expr[1:]))
 
 
+def _attach(p: dict) -> str:
+    """EnableAttach, SafetyGuard and AttachCondition/Condition_k (problem key
+    "attach_conditions": up to five formulas, None for none)."""
+    if not p.get("enable_attach"):
+        return ""
+    conds = p.get("attach_conditions") or []
+    body = "".join(_el(f"Condition_{k}", formula_xml(c)) for k, c in enumerate(conds) if c is not None)
+    return (_el("EnableAttach", "1") + _el("SafetyGuard", str(int(p.get("safety_guard", 500))))
+            + (_el("AttachCondition", body) if body else ""))
+
+
 def _simulator(p: dict, stop=None, fitness=None, record=None) -> str:
     parts = [
         _el("Integration", _el("DtFrac", _num(p["dt_frac"]))),
@@ -49,7 +60,7 @@ def _simulator(p: dict, stop=None, fitness=None, record=None) -> str:
             + _el("SlowDampingZ", _num(p["slow_damp_z"]))),
         _el("Features", _el("VolumeEffectsEnabled", str(int(p["volume_effects"])))),
         _el("AttachDetach", _el("EnableCollision", str(int(p["enable_collision"])))
-            + (_el("EnableDetach", "1") if p.get("enable_detach") else "")),
+            + (_el("EnableDetach", "1") if p.get("enable_detach") else "") + _attach(p)),
         _el("EnableExpansion", str(int(p["enable_expansion"]))),
     ]
     if p.get("enable_cilia"):
@@ -126,6 +137,8 @@ def _palette(mats: Sequence[dict]) -> str:
             mech += _el("Cilia", _num(m["cilia"]))
         if m["is_target"]:
             mech += _el("isTarget", "1")
+        if m["sticky"]:
+            mech += _el("Sticky", "1")
         if m["cilia_on_after"]:
             mech += _el("TurnOnCiliaAfterThisManySeconds", _num(m["cilia_on_after"]))
         if m["remove_after"]:
