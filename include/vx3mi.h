@@ -186,16 +186,18 @@ int vx3mi_model_set_external(vx3mi_model *m, int voxel, int dof, const double *t
  * "<mtMUL><mtCONST>0.01</mtCONST><mtVAR>t</mtVAR></mtMUL>"; empty or NULL
  * removes it), added to every voxel's force as VX3_Voxel::timeStep does
  * (src/VX3/VX3_Voxel.cu:177-183, parsed as VX3_SimulationManager::ParseMathTree,
- * VX3_SimulationManager.cu:157-275).  Variables x, y, z, t, num_voxel; hit,
- * targetCloseness and numClosePairs are 0; formulas using angle are rejected.
- * Batches with force fields run in the plain and flat modes. */
+ * VX3_SimulationManager.cu:157-275).  Variables x, y, z (the voxel's position),
+ * t, hit (collisionCount), num_voxel, and angle, targetCloseness and
+ * numClosePairs from the CycleStep samples of doTimeStep (every
+ * int(TempPeriod / dt) steps, VX3_VoxelyzeKernel.cu:314-334, 545-562), taken on
+ * the device.  Batches with force fields run in the plain and flat modes. */
 int vx3mi_model_set_force_field(vx3mi_model *m, int axis, const char *formula_xml);
 /* AttachCondition Condition_k (k = 0..4, VX3_SimulationManager.cu:334-343): a
  * MathTree XML as for force fields (empty or NULL removes it).  After each voxel
  * update a voxel may attach when every condition given is > 0
  * (VX3_VoxelyzeKernel.cu:607-621); variables x, y, z (the voxel's new position),
- * hit (collisionCount), t (the step's start time), num_voxel; formulas using
- * angle, targetCloseness or numClosePairs are rejected. */
+ * hit (collisionCount), t (the step's start time), num_voxel, and angle,
+ * targetCloseness, numClosePairs from the last CycleStep samples. */
 int vx3mi_model_set_attach_condition(vx3mi_model *m, int k, const char *formula_xml);
 /* the voxel's external (*dof = -1 and zeros when it has none) */
 int vx3mi_model_get_external(const vx3mi_model *m, int voxel, int *dof, double *translation3,

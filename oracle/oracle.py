@@ -67,7 +67,7 @@ class _Spec(C.Structure):
                 ("enable_cilia", C.c_int), ("secondary_experiment", C.c_int),
                 ("cilia_base", C.POINTER(C.c_double)), ("cilia_shift", C.POINTER(C.c_double)),
                 ("enable_signals", C.c_int), ("enable_detach", C.c_int),
-                ("enable_attach", C.c_int), ("safety_guard", C.c_int)]
+                ("enable_attach", C.c_int), ("safety_guard", C.c_int), ("max_dist_pairs", C.c_double)]
 
 
 class _DiffOpts(C.Structure):
@@ -81,7 +81,7 @@ SPEC_DEFAULTS = dict(lattice_dim=0.01, dt_frac=0.9, bond_damp_z=0.1, col_damp_z=
                      floor_enabled=1, temp_enabled=1, vary_temp=0, temp_base=25.0,
                      temp_amplitude=0.0, temp_period=0.1, phase=None, voxel_E=None,
                      enable_cilia=0, secondary_experiment=0, cilia_base=None, cilia_shift=None, enable_signals=0,
-                     enable_detach=0, enable_attach=0, safety_guard=500)
+                     enable_detach=0, enable_attach=0, safety_guard=500, max_dist_pairs=0.0)
 MATERIAL_DEFAULTS = dict(E=1e6, rho=1000.0, nu=0.0, cte=0.0, mu_s=0.0, mu_k=0.0, fail_stress=-1.0,
                          plastic_mod=0.0, yield_stress=0.0,
                          fixed=0, is_measured=1, thermal_on_after=0.0,
@@ -140,6 +140,7 @@ def lib():
         L.vxo_set_force_field.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         L.vxo_set_attach_condition.argtypes = [vp, C.c_int, C.c_int, C.POINTER(C.c_int), dp]
         L.vxo_get_attach.argtypes = [vp, ip, ip]
+        L.vxo_get_samples.argtypes = [vp, dp]
         L.vxo_get_link_curve.restype = C.c_int
         L.vxo_get_link_curve.argtypes = [vp, C.c_int, fp, fp]
         L.vxo_get_voxel_material.argtypes = [vp, C.c_int, fp]
@@ -197,7 +198,8 @@ class OracleModel:
                      p["temp_base"], p["temp_amplitude"], p["temp_period"],
                      int(p["enable_cilia"]), int(p["secondary_experiment"]),
                      _ptr(self._cb, C.c_double), _ptr(self._cs, C.c_double), int(p["enable_signals"]),
-                     int(p["enable_detach"]), int(p["enable_attach"]), int(p["safety_guard"]))
+                     int(p["enable_detach"]), int(p["enable_attach"]), int(p["safety_guard"]),
+                     float(p["max_dist_pairs"]))
         self._L = lib()
         self._h = self._L.vxo_build(C.byref(spec))
         self.nv = self._L.vxo_num_voxels(self._h)
@@ -247,6 +249,13 @@ class OracleModel:
         """AttachCondition Condition_k (a nested tuple, see _tokens)."""
         o, v = self._tokens(expr)
         self._L.vxo_set_attach_condition(self._h, int(k), len(o), _ptr(o, C.c_int), _ptr(v, C.c_double))
+
+    def samples(self) -> dict:
+        """The last CycleStep samples: recentAngle, targetCloseness,
+        numClosePairs, angleSampleTimes and the sampled centre of mass."""
+        o = np.zeros(7)
+        self._L.vxo_get_samples(self._h, _ptr(o, C.c_double))
+        return dict(angle=o[0], closeness=o[1], pairs=int(o[2]), count=int(o[3]), com=o[4:7].copy())
 
     def attach_state(self):
         """Per voxel the enableAttach flag, per link the updates left whose

@@ -135,6 +135,12 @@ struct vxo_model {
      * true until the first voxel update), AttachCondition Condition_0..4 */
     int enableAttach, safetyGuard;
     int *vattach;
+    /* CycleStep samples (VX3_VoxelyzeKernel.cu:314-334): centre of mass now and
+     * at the last two samples, recentAngle, targetCloseness, numClosePairs */
+    double maxDistPairs;
+    v3 comCur, comHist[2];
+    double recentAngle, closeness;
+    int closePairs, angleSamples;
     int ac_n[5];
     int *ac_op[5];
     double *ac_val[5];
@@ -545,6 +551,7 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     m->enableDetach = sp->enable_detach;
     m->enableAttach = sp->enable_attach;
     m->safetyGuard = sp->safety_guard;
+    m->maxDistPairs = sp->max_dist_pairs;
 
     int nv = 0;
     for (int i = 0; i < ncell; i++)
@@ -732,6 +739,9 @@ vxo_model *vxo_build(const vxo_spec *sp) {
     }
     for (v = 0; v < nv; v++) m->temp[v] = t0;
     for (int l = 0; l < nl; l++) update_rest_length(m, l);
+    double c[3];
+    vxo_center_of_mass(m, c);  /* InitializeCenterOfMass */
+    m->comCur = v3_(c[0], c[1], c[2]);
     return m;
 }
 
@@ -800,6 +810,14 @@ void vxo_set_attach_condition(vxo_model *m, int k, int n, const int *ops, const 
     m->ac_op[k] = (int *)xcalloc(m->ac_n[k], sizeof(int));
     m->ac_val[k] = (double *)xcalloc(m->ac_n[k], sizeof(double));
     for (int i = 0; i < m->ac_n[k]; i++) { m->ac_op[k][i] = ops[i]; m->ac_val[k][i] = values[i]; }
+}
+
+void vxo_get_samples(const vxo_model *m, double *out7) {
+    out7[0] = m->recentAngle;
+    out7[1] = m->closeness;
+    out7[2] = m->closePairs;
+    out7[3] = m->angleSamples;
+    out7[4] = m->comCur.x; out7[5] = m->comCur.y; out7[6] = m->comCur.z;
 }
 
 void vxo_get_attach(const vxo_model *m, int *voxel_flags, int *link_new) {
@@ -1431,7 +1449,8 @@ static void voxel_update(vxo_model *m, int v, float dtf, int cpu_semantics) {
     /* force field (VX3_Voxel.cu:177-183) at the position entering the step and
      * currentTime; hit is collisionCount; angle, closeness and pairs are 0; absent
      * trees add +0.0 as the reference's constant-zero defaults do */
-    const double vars[9] = {pos.x, pos.y, pos.z, (double)m->collisionCount, m->time, 0.0, 0.0, 0.0, (double)m->nv};
+    const double vars[9] = {pos.x, pos.y, pos.z, (double)m->collisionCount, m->time, m->recentAngle, m->closeness,
+                            (double)m->closePairs, (double)m->nv};
     tf.x += m->ff_n[0] ? mt_eval_fifo(m->ff_op[0], m->ff_val[0], m->ff_n[0], vars) : 0.0;
     tf.y += m->ff_n[1] ? mt_eval_fifo(m->ff_op[1], m->ff_val[1], m->ff_n[1], vars) : 0.0;
     tf.z += m->ff_n[2] ? mt_eval_fifo(m->ff_op[2], m->ff_val[2], m->ff_n[2], vars) : 0.0;
@@ -1536,18 +1555,59 @@ static void update_voxels(vxo_model *m, float dt, int cpu_semantics) {
 /* gpu_update_voxels VX3_VoxelyzeKernel.cu:607-621: after its update every
  * present, non-fixed voxel re-evaluates enableAttach, true when each of the
  * five AttachCondition formulas is > 0, at its new position with the step's
- * start time and the collision count so far (angle, closeness and pairs 0) */
+ * start time, the collision count so far and the last CycleStep samples */
 static void update_attach_flags(vxo_model *m) {
     if (!m->enableAttach) return;
     for (int v = 0; v < m->nv; v++) {
         if (m->vremoved[v] || m->vm[m->vmat_of[v]].fixed) continue;
         const double vars[9] = {m->pos[v].x, m->pos[v].y, m->pos[v].z, (double)m->collisionCount, m->time,
-                                0.0, 0.0, 0.0, (double)m->nv};
+                                m->recentAngle, m->closeness, (double)m->closePairs, (double)m->nv};
         int ok = 1;
         for (int k = 0; k < 5; k++)
             if (m->ac_n[k] && !(mt_eval_fifo(m->ac_op[k], m->ac_val[k], m->ac_n[k], vars) > 0)) ok = 0;
         m->vattach[v] = ok;
     }
+}
+
+/* The CycleStep block of doTimeStep (VX3_VoxelyzeKernel.cu:314-334) with
+ * updateCurrentCenterOfMass (:477-493) and computeTargetCloseness (:545-562):
+ * every CycleStep = int(TempPeriod / dt) steps (at least 1; the reference divides
+ * by zero below that) the centre of mass is sampled, recentAngle is the angle
+ * between the last two displacements of the sampled centre of mass (0 for the
+ * first two samples or a repeated point), and with MaxDistInVoxelLengthsToCountAsPair
+ * set, targetCloseness sums 1 / distance over the target voxel pairs and
+ * numClosePairs counts those closer than that many voxel lengths. */
+static void cycle_sample(vxo_model *m, float dt) {
+    int cycle = (int)(m->tempPeriod / (double)dt);
+    if (cycle < 1) cycle = 1;
+    if ((m->steps + 1) % cycle != 0) return;  /* CurStepCount counts this step */
+    m->angleSamples++;
+    m->comHist[0] = m->comHist[1];
+    m->comHist[1] = m->comCur;
+    double c[3];
+    vxo_center_of_mass(m, c);
+    m->comCur = v3_(c[0], c[1], c[2]);
+    v3 A = m->comHist[0], B = m->comHist[1], C = m->comCur;
+    int bc = B.x == C.x && B.y == C.y && B.z == C.z, ab = A.x == B.x && A.y == B.y && A.z == B.z;
+    if (bc || ab || m->angleSamples < 3) {
+        m->recentAngle = 0;
+    } else {
+        v3 u = vsub(B, A), w = vsub(C, B);
+        m->recentAngle = vxm_acos((u.x * w.x + u.y * w.y + u.z * w.z) / (vlen(u) * vlen(w)));
+    }
+    if (m->maxDistPairs == 0) return;
+    double R = m->maxDistPairs * m->voxSize, ret = 0;
+    m->closePairs = 0;
+    for (int i = 0; i < m->nv; i++) {
+        if (!m->vm[m->vmat_of[i]].isTarget) continue;
+        for (int j = i + 1; j < m->nv; j++) {
+            if (!m->vm[m->vmat_of[j]].isTarget) continue;
+            double d = vlen(vsub(m->pos[i], m->pos[j]));
+            if (d < R) m->closePairs++;
+            ret += 1 / d;
+        }
+    }
+    m->closeness = ret;
 }
 
 /* VX3_VoxelyzeKernel::removeVoxels VX3_VoxelyzeKernel.cu:365-399, called at
@@ -1606,7 +1666,10 @@ int vxo_step(vxo_model *m, float dt, int nsteps, int flags) {
         if (!(flags & VXO_NO_TEMPERATURE)) update_temperature(m);
         float d = dt;
         if (d < 0) d = vxo_step_dt(m);
-        if (d == 0) continue;
+        if (d == 0) {  /* CurStepCount still counts the step (:239-241) */
+            m->steps++;
+            continue;
+        }
         update_links(m);
         int diverged = 0;
         for (int l = 0; l < m->nl; l++)
@@ -1615,6 +1678,7 @@ int vxo_step(vxo_model *m, float dt, int nsteps, int flags) {
         int attached = (flags & VXO_ALL_PAIRS) ? update_collisions_all_pairs(m) : update_collisions(m);
         update_voxels(m, d, flags & VXO_CPU_SEMANTICS);
         update_attach_flags(m);
+        cycle_sample(m, d);
         /* the end of the step: detached links no longer carry signals (the
          * reference detaches before the voxel update), removal comes last; a
          * new link, a detach or a removal regenerates the surface list */

@@ -107,6 +107,7 @@ typedef struct {
     int enable_detach;          /* Simulator/AttachDetach/EnableDetach */
     int enable_attach;          /* Simulator/AttachDetach/EnableAttach */
     int safety_guard;           /* Simulator/AttachDetach/SafetyGuard: updates a new link's forces are scaled by 0.01 */
+    double max_dist_pairs;      /* Simulator/MaxDistInVoxelLengthsToCountAsPair (0: targetCloseness not computed) */
 } vxo_spec;
 
 typedef struct vxo_model vxo_model;
@@ -160,6 +161,11 @@ void vxo_set_attach_condition(vxo_model *m, int k, int n, const int *ops, const 
  * before the first), per link the updates left whose forces are scaled by 0.01
  * (isNewLink) */
 void vxo_get_attach(const vxo_model *m, int *voxel_flags, int *link_new);
+/* the CycleStep samples of doTimeStep (VX3_VoxelyzeKernel.cu:314-334, 545-562),
+ * which force fields and attach conditions read as angle, targetCloseness and
+ * numClosePairs: out7 = recentAngle, targetCloseness, numClosePairs,
+ * angleSampleTimes, the sampled centre of mass (3) */
+void vxo_get_samples(const vxo_model *m, double *out7);
 
 /* stress-strain curve of a non-linear link material (n <= VXO_MAX_CURVE_POINTS
  * points, 0 when linear) */

@@ -230,9 +230,9 @@ BAD = {
     "layer_len": DEMO_LIKE.replace("<![CDATA[0]]>", "<![CDATA[00]]>"),
     "zlib": DEMO_LIKE.replace('Compression="ASCII_READABLE"', 'Compression="ZLIB"'),
     "few_layers": DEMO_LIKE.replace("<Z_Voxels>2</Z_Voxels>", "<Z_Voxels>3</Z_Voxels>"),
-    "attach_angle": DEMO_LIKE.replace("<Simulator>", "<Simulator><AttachDetach><EnableAttach>true</EnableAttach>"
-                                                     "<AttachCondition><Condition_2><mtVAR>angle</mtVAR></Condition_2>"
-                                                     "</AttachCondition></AttachDetach>"),
+    "attach_bad_var": DEMO_LIKE.replace("<Simulator>", "<Simulator><AttachDetach><EnableAttach>true</EnableAttach>"
+                                                       "<AttachCondition><Condition_2><mtVAR>speed</mtVAR></Condition_2>"
+                                                       "</AttachCondition></AttachDetach>"),
     "bc_count": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs></Boundary_Conditions>"
                                           "</Environment><VXC "),
     "bc_mesh": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs><FRegion><PrimType>3"
@@ -242,10 +242,6 @@ BAD = {
     "bad_token": DEMO_LIKE.replace("<mtSUB>", "<mtFOO>").replace("</mtSUB>", "</mtFOO>"),
     "bad_var": DEMO_LIKE.replace("<mtVAR>t</mtVAR>", "<mtVAR>speed</mtVAR>"),
     "missing_operand": DEMO_LIKE.replace("<mtCONST>1</mtCONST>", ""),
-    "force_field_angle": DEMO_LIKE.replace("<Simulator>", "<Simulator><ForceField><x_forcefield><mtVAR>angle</mtVAR>"
-                                                          "</x_forcefield></ForceField>"),
-    "force_field_closeness": DEMO_LIKE.replace("<Simulator>", "<Simulator><ForceField><x_forcefield><mtVAR>"
-                                                              "targetCloseness</mtVAR></x_forcefield></ForceField>"),
 }
 
 
@@ -281,9 +277,17 @@ def test_attach_model_limits():
     with pytest.raises(vx3mi.VX3MIError, match="Poisson"):
         vx3mi.Model.from_problem(dict(base, volume_effects=1, materials=[dict(sticky=1, nu=0.3)]))
     m = vx3mi.Model.from_problem(base)
-    with pytest.raises(vx3mi.VX3MIError, match="angle"):
-        m.set_attach_condition(1, ("mtVAR", "angle"))
+    m.set_attach_condition(1, ("mtVAR", "angle"))
     m.set_attach_condition(4, ("mtLESSTHAN", ("mtVAR", "z"), ("mtCONST", 0.05)))
+
+
+def test_sampled_variables_are_accepted_in_force_fields_and_conditions():
+    # angle, targetCloseness and numClosePairs are the CycleStep samples
+    # (VX3_VoxelyzeKernel.cu:314-334, 545-562) that every force-field evaluation reads
+    for var in ("angle", "targetCloseness", "numClosePairs"):
+        text = DEMO_LIKE.replace("<Simulator>", f"<Simulator><ForceField><x_forcefield><mtVAR>{var}</mtVAR>"
+                                                "</x_forcefield></ForceField>")
+        vx3mi.Model.from_xml(text)
 
 
 def test_zero_force_field_and_collision_false_are_accepted():

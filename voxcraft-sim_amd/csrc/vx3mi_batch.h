@@ -34,6 +34,12 @@ struct RobotParams {
     int attach, safetyGuard;
     int acOff[5], acLen[5];
     int poff, pcount;
+    // CycleStep samples on the device (a force field or attach condition reads
+    // angle, targetCloseness or numClosePairs): the robot's target voxels
+    // [tgOff, tgOff + tgCount) of the target list and the pair radius
+    // MaxDistInVoxelLengthsToCountAsPair * voxel size (0: closeness not computed)
+    int sample, tgOff, tgCount;
+    double pairR;
 };
 
 // Candidate lists: every surface voxel lists (ascending) the surface voxels of
@@ -55,6 +61,12 @@ struct RobotTime {
     int collisionCount;  // target / non-target contacts summed over the steps (VX3_VoxelyzeKernel.cu:711-726)
     int listings;        // steps whose collision phase rebuilt the robot's candidate lists
     int divergedNow;     // a link of this step passed the divergence strain (the voxel update sets diverged)
+    // the CycleStep samples (VX3_VoxelyzeKernel.cu:314-334, 545-562): centre of
+    // mass now and at the last two samples, recentAngle, targetCloseness,
+    // numClosePairs, angleSampleTimes
+    double scom[3], shist[2][3];
+    double recentAngle, closeness;
+    int closePairs, angleSamples;
     int attachUsed;      // pool entries handed out so far (attach)
     int attachOverflow;  // a voxel had more attach candidates than it keeps (kAttachPerVoxel), or the pool ran out
 };

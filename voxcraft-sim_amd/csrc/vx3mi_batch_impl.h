@@ -87,6 +87,7 @@ template <typename T> struct Dev {
     int *acand;               // kAttachPerVoxel partner voxels per surface voxel, ascending
     int *acount;              // candidates found (beyond kAttachPerVoxel: not kept)
     int *lmade;               // per link entry: the step count at whose end attach made it (0: lattice)
+    const int *targets;       // target voxels of robots with device samples, CSR by robot (RobotParams::tgOff)
 };
 
 // ------------------------------------------------------------------ device
@@ -280,7 +281,8 @@ template <typename T> __global__ __launch_bounds__(256) void k_force_field(Dev<T
     if (!rt->diverged && (rp.ffLen[0] | rp.ffLen[1] | rp.ffLen[2])) {
         const T4<T> q0 = a.pq0[v];
         const V3<double> p = position_d<T>(a.lat[v], a.vmats[a.vmat[v]].nomSize, V3<T>{q0.x, q0.y, q0.z});
-        f = force_field<T>(a.formulas, rp.ffOff, rp.ffLen, p.x, p.y, p.z, rt->time[parity], rp.vcount, rt->collisionCount);
+        f = force_field<T>(a.formulas, rp.ffOff, rp.ffLen, p.x, p.y, p.z, rt->time[parity], rp.vcount, rt->collisionCount,
+                           rt->recentAngle, rt->closeness, rt->closePairs);
     }
     a.field[v] = T4<T>{f.x, f.y, f.z, (T)0};
 }
@@ -793,8 +795,8 @@ template <typename T> __global__ __launch_bounds__(256) void k_attach_walk(Dev<T
 
 // gpu_update_voxels (:607-621): after the voxel update every present, non-fixed
 // voxel of an attach robot re-evaluates enableAttach, true when each
-// AttachCondition given is > 0, at its new position with the step's start time
-// and the collision count (angle, closeness and pairs are not available)
+// AttachCondition given is > 0, at its new position with the step's start time,
+// the collision count and the last CycleStep samples
 template <typename T> __global__ __launch_bounds__(256) void k_attach_flags(Dev<T> a, int v0, int n, int parity) {
     const int v = v0 + blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= v0 + n) return;
@@ -806,12 +808,83 @@ template <typename T> __global__ __launch_bounds__(256) void k_attach_flags(Dev<
     if (vm.fixed) return;
     const T4<T> q0 = a.pq0[v];
     const V3<double> p = position_d<T>(a.lat[v], vm.nomSize, V3<T>{q0.x, q0.y, q0.z});
-    const double vars[MV_COUNT] = {p.x, p.y, p.z, (double)rt->collisionCount, rt->time[parity], 0.0, 0.0, 0.0,
-                                   (double)rp.vcount};
+    const double vars[MV_COUNT] = {p.x, p.y, p.z, (double)rt->collisionCount, rt->time[parity], rt->recentAngle,
+                                   rt->closeness, (double)rt->closePairs, (double)rp.vcount};
     int ok = 1;
     for (int k = 0; k < 5; k++)
         if (rp.acLen[k] && !(formula_eval(a.formulas + rp.acOff[k], rp.acLen[k], vars) > 0)) ok = 0;
     a.vflagA[v] = ok;
+}
+
+// The CycleStep block of doTimeStep (VX3_VoxelyzeKernel.cu:314-334) for robots
+// whose force fields or attach conditions read its samples: every CycleStep =
+// int(TempPeriod / dt) steps (at least 1; the reference divides by zero below)
+// the centre of mass is sampled (updateCurrentCenterOfMass :477-493, summed in
+// voxel order as the reference's loop), recentAngle is the angle between the
+// last two displacements of the sampled centre of mass (0 for the first two
+// samples or a repeated point), and with a pair radius targetCloseness sums
+// 1 / distance over the target pairs and numClosePairs counts those closer
+// (computeTargetCloseness :545-562).  One work-item per robot.
+template <typename T> __global__ __launch_bounds__(64) void k_cycle_sample(Dev<T> a, int r0, int nr) {
+    const int r = r0 + blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= r0 + nr) return;
+    const RobotParams &rp = a.rp[r];
+    RobotTime *rt = a.rt + r;
+    if (!rp.sample || rt->diverged) return;
+    int cycle = (int)(rp.tempPeriod / (double)rt->dt);
+    if (cycle < 1) cycle = 1;
+    if (rt->steps % cycle != 0) return;  // the step count includes this step (CurStepCount)
+    rt->angleSamples += 1;
+    double s[3] = {0, 0, 0}, total = 0;
+    for (int v = rp.voff; v < rp.voff + rp.vcount; v++) {
+        const VoxelMat &vm = a.vmats[a.vmat[v]];
+        if (!vm.isMeasured) continue;
+        const T4<T> q = a.pq0[v];
+        const V3<double> p = position_d<T>(a.lat[v], vm.nomSize, V3<T>{q.x, q.y, q.z});
+        const double w = vm.mass;
+        s[0] = s[0] + p.x * w;
+        s[1] = s[1] + p.y * w;
+        s[2] = s[2] + p.z * w;
+        total += w;
+    }
+    double A[3], B[3], C[3];
+    for (int k = 0; k < 3; k++) {
+        A[k] = rt->shist[1][k];
+        B[k] = rt->scom[k];
+        C[k] = total == 0 ? 0.0 : s[k] / total;
+        rt->shist[0][k] = A[k];
+        rt->shist[1][k] = B[k];
+        rt->scom[k] = C[k];
+    }
+    const bool bc = B[0] == C[0] && B[1] == C[1] && B[2] == C[2], ab = A[0] == B[0] && A[1] == B[1] && A[2] == B[2];
+    if (bc || ab || rt->angleSamples < 3) {
+        rt->recentAngle = 0;
+    } else {
+        const double u[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, w[3] = {C[0] - B[0], C[1] - B[1], C[2] - B[2]};
+        const double d = u[0] * w[0] + u[1] * w[1] + u[2] * w[2];
+        const double lu = sqrt(u[0] * u[0] + u[1] * u[1] + u[2] * u[2]);
+        const double lw = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        rt->recentAngle = vxm_acos(d / (lu * lw));
+    }
+    if (rp.pairR == 0) return;
+    double ret = 0;
+    int pairs = 0;
+    for (int i = 0; i < rp.tgCount; i++) {
+        const int vi = a.targets[rp.tgOff + i];
+        const T4<T> qi = a.pq0[vi];
+        const V3<double> pi = position_d<T>(a.lat[vi], a.vmats[a.vmat[vi]].nomSize, V3<T>{qi.x, qi.y, qi.z});
+        for (int j = i + 1; j < rp.tgCount; j++) {
+            const int vj = a.targets[rp.tgOff + j];
+            const T4<T> qj = a.pq0[vj];
+            const V3<double> pj = position_d<T>(a.lat[vj], a.vmats[a.vmat[vj]].nomSize, V3<T>{qj.x, qj.y, qj.z});
+            const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+            const double dist = sqrt(dx * dx + dy * dy + dz * dz);
+            if (dist < rp.pairR) pairs++;
+            ret += 1 / dist;
+        }
+    }
+    rt->closeness = ret;
+    rt->closePairs = pairs;
 }
 
 template <typename T> __global__ __launch_bounds__(256) void k_links(Dev<T> a, int l0, int n, int parity) {
@@ -1011,6 +1084,7 @@ template <typename T> struct Batch : BatchBase {
     bool anyForceField = false;   // some robot has a force field (plain and flat modes only)
     bool anyCilia = false;        // some robot has EnableCilia (not in the tiles)
     bool anySignals = false;      // some robot has EnableSignals
+    bool anySample = false;       // some robot's formulas read the CycleStep samples (angle, closeness, pairs)
     // plain and flat modes only: force fields (their own kernel), removal and
     // signals (kernels between steps)
     bool plainFlatOnly() const { return anyForceField || anyRemoval || anySignals || anyDetach || anyAttach; }
@@ -1041,6 +1115,7 @@ template <typename T> struct Batch : BatchBase {
         std::vector<SignalMat> sigMat;
         std::vector<int> vtarget;
         std::vector<int2> vatt;          // attach material and link material per voxel
+        std::vector<int> targets;        // target voxels of robots with device samples
         std::vector<char> poolDetach;    // per robot: its attach links can fail and detach
         bool anyTarget = false;
         std::vector<int> remCand, detCand;
@@ -1106,6 +1181,19 @@ template <typename T> struct Batch : BatchBase {
                 if (p.attach) formulas.insert(formulas.end(), post.begin(), post.end());
             }
             if (p.attach) anyAttach = true;
+            // device CycleStep samples when a force field or attach condition reads them
+            const unsigned sampledVars = (1u << MV_ANGLE) | (1u << MV_CLOSENESS) | (1u << MV_PAIRS);
+            unsigned used = 0;
+            for (int ax = 0; ax < 3; ax++) used |= mt_vars_used(m.forceField[ax]);
+            for (int k = 0; p.attach && k < 5; k++) used |= mt_vars_used(m.attachCond[k]);
+            p.sample = (used & sampledVars) ? 1 : 0;
+            p.pairR = m.maxDistPairs * m.latticeDim;
+            p.tgOff = (int)targets.size();
+            if (p.sample)
+                for (int v = 0; v < m.nv(); v++)
+                    if (m.palette[m.vpal[v] - 1].is_target) targets.push_back(voff + v);
+            p.tgCount = (int)targets.size() - p.tgOff;
+            if (p.sample) anySample = true;
             std::vector<int> vmRemap(m.vmats.size()), lmRemap(m.lmats.size());
             for (size_t k = 0; k < m.vmats.size(); k++) vmRemap[k] = vm_index(m.vmats[k], m.vmatE[k], -1);
             const int extOff = (int)exts.size();
@@ -1272,6 +1360,19 @@ template <typename T> struct Batch : BatchBase {
             std::memset(&rt[r], 0, sizeof(RobotTime));
             rt[r].dt = defaultDt[r];
             rt[r].stale[0] = rt[r].stale[1] = 1;  // candidate lists are built by the first step
+            // InitializeCenterOfMass: the sampled centre of mass starts at the initial one
+            const Model &m = models[r]->m;
+            double sc[3] = {0, 0, 0}, total = 0;
+            for (int v = 0; v < m.nv(); v++) {
+                const VoxelMat &vm = m.vmats[m.vmat[v]];
+                if (!vm.isMeasured) continue;
+                const double L0 = vm.nomSize, w = vm.mass;
+                sc[0] = sc[0] + (m.ix[v] * L0) * w;
+                sc[1] = sc[1] + (m.iy[v] * L0) * w;
+                sc[2] = sc[2] + (m.iz[v] * L0) * w;
+                total += w;
+            }
+            for (int k = 0; k < 3; k++) rt[r].scom[k] = total == 0 ? 0.0 : sc[k] / total;
         }
         curDt = defaultDt;
         HIP_TRY(hipMemcpy(d_rt, rt.data(), sizeof(RobotTime) * R, hipMemcpyHostToDevice));
@@ -1416,6 +1517,13 @@ template <typename T> struct Batch : BatchBase {
             if ((rc = alloc(&td, V))) return rc;
             HIP_TRY(hipMemcpy(td, vtarget.data(), sizeof(int) * V, hipMemcpyHostToDevice));
             d.vtarget = td;
+        }
+        d.targets = nullptr;
+        if (anySample && !targets.empty()) {
+            int *tg;
+            if ((rc = alloc(&tg, targets.size()))) return rc;
+            HIP_TRY(hipMemcpy(tg, targets.data(), sizeof(int) * targets.size(), hipMemcpyHostToDevice));
+            d.targets = tg;
         }
         d.vatt = nullptr;
         d.vflagA = d.acand = d.acount = d.lmade = nullptr;
@@ -1580,13 +1688,14 @@ template <typename T> struct Batch : BatchBase {
         if (anyForceField) k_force_field<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         k_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
         if (anyAttach) k_attach_flags<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V, parity);
+        if (anySample) k_cycle_sample<T><<<(R + 63) / 64, 64, 0, stream>>>(d, 0, R);
         if (anyDetach) k_detach<T><<<(d.nDetCand + 255) / 256, 256, 0, stream>>>(d);
         if (anySignals) k_signal_deliver<T><<<(V + 255) / 256, 256, 0, stream>>>(d, 0, V);
         if (anyRemoval) k_removal<T><<<(d.nRemCand + 255) / 256, 256, 0, stream>>>(d, parity);
     }
     int kernels_per_flat_step() const {
         return 2 + (anyCollisions ? kCollisionLaunches - 1 : 0) + (anyForceField ? 1 : 0) + (anyRemoval ? 1 : 0) + (anySignals ? 1 : 0) +
-               (anyDetach ? 1 : 0) + (anyAttach ? 3 : 0);
+               (anyDetach ? 1 : 0) + (anyAttach ? 3 : 0) + (anySample ? 1 : 0);
     }
     // the attach phase of surface voxels [s0, s0 + n) (robots [r0, r0 + nr)):
     // candidates, then the links made robot by robot
@@ -1637,8 +1746,9 @@ template <typename T> struct Batch : BatchBase {
                     if (anyForceField) k_force_field<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
                     k_voxels<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
                     if (p.attach) k_attach_flags<T><<<(p.vcount + 255) / 256, 256, 0, stream>>>(d, p.voff, p.vcount, parity);
+                    if (p.sample) k_cycle_sample<T><<<1, 64, 0, stream>>>(d, r, 1);
                     launches += 2 + (p.scount ? kCollisionLaunches : 0) + (anyForceField ? 1 : 0) + (p.pcount ? 1 : 0) +
-                                (p.attach ? 3 : 0);
+                                (p.attach ? 3 : 0) + (p.sample ? 1 : 0);
                 }
                 // the end-of-step phases for the whole batch (each touches only its
                 // robot's voxels and links)

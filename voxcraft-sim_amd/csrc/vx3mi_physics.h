@@ -773,8 +773,8 @@ template <typename T> VX_HD T fixed_coordinate(int c, double nomSize, double t) 
 // rejected when read.
 template <typename T>
 VX_HD V3<T> force_field(const MtToken *f, const int *off, const int *len, double x, double y, double z, double t,
-                        int numVoxels, int hit) {
-    const double vars[MV_COUNT] = {x, y, z, (double)hit, t, 0.0, 0.0, 0.0, (double)numVoxels};
+                        int numVoxels, int hit, double angle = 0.0, double closeness = 0.0, int pairs = 0) {
+    const double vars[MV_COUNT] = {x, y, z, (double)hit, t, angle, closeness, (double)pairs, (double)numVoxels};
     V3<T> r = {0, 0, 0};
     if (len[0]) r.x = (T)formula_eval(f + off[0], len[0], vars);
     if (len[1]) r.y = (T)formula_eval(f + off[1], len[1], vars);

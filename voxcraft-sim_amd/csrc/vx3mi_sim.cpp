@@ -18,7 +18,9 @@
 // were taken when it stopped).  collisionCount (hit) comes from the device;
 // targetCloseness and numClosePairs are computed at the CycleStep samples
 // (computeTargetCloseness, VX3_VoxelyzeKernel.cu:545-562) from the target
-// voxels' positions when MaxDistInVoxelLengthsToCountAsPair is set.
+// voxels' positions when MaxDistInVoxelLengthsToCountAsPair is set.  A robot
+// whose force fields or attach conditions read the samples has them taken on
+// the device (k_cycle_sample), and the runner uses those.
 #include <algorithm>
 #include <cmath>
 #include <cstdarg>
@@ -380,7 +382,19 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
             }
             x.j += seg;
             x.hit = rt[r].collisionCount;
-            if (x.needCycle && x.j % x.cycle == 0) {
+            if (b.hrp[r].sample && x.j % x.cycle == 0) {
+                // the device took this sample (its force fields or attach conditions read it)
+                const RobotTime &t = rt[r];
+                std::memcpy(x.hist[0], t.shist[0], sizeof(x.hist[0]));
+                std::memcpy(x.hist[1], t.shist[1], sizeof(x.hist[1]));
+                std::memcpy(x.com, t.scom, sizeof(x.com));
+                x.recentAngle = t.recentAngle;
+                x.angleSamples = t.angleSamples;
+                if (x.m->maxDistPairs != 0) {
+                    x.closeness = t.closeness;
+                    x.pairs = t.closePairs;
+                }
+            } else if (x.needCycle && x.j % x.cycle == 0) {
                 cycle_sample(x, &com[3 * r]);
                 if (x.m->maxDistPairs != 0) {
                     pos.resize((size_t)3 * x.m->nv());

@@ -540,11 +540,6 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
                 std::vector<MtToken> f, post;
                 if ((rc = parse_formula(ff->child(names[a]), f, err)) != VX3MI_OK) return rc;
                 if (f.empty() || (f.size() == 2 && f[0].op == MT_CONST && f[0].value == 0.0)) continue;
-                if (mt_vars_used(f) & ((1u << MV_ANGLE) | (1u << MV_CLOSENESS) | (1u << MV_PAIRS))) {
-                    // sampled inside doTimeStep every CycleStep (VX3_VoxelyzeKernel.cu:314-334)
-                    err = "force fields that use the recent angle, targetCloseness or numClosePairs are not supported";
-                    return VX3MI_E_PARSE;
-                }
                 if (!mt_postfix(f, post)) {
                     err = "force-field formula nested too deeply";
                     return VX3MI_E_PARSE;
@@ -558,10 +553,6 @@ int read_vxa(const XNode &doc, VxaContent &c, std::string &err) {
             std::vector<MtToken> f, post;
             if ((rc = parse_formula(path(ad, {"AttachCondition", name.c_str()}), f, err)) != VX3MI_OK) return rc;
             if (f.empty()) continue;
-            if (mt_vars_used(f) & ((1u << MV_ANGLE) | (1u << MV_CLOSENESS) | (1u << MV_PAIRS))) {
-                err = "attach conditions that use the recent angle, targetCloseness or numClosePairs are not supported";
-                return VX3MI_E_PARSE;
-            }
             if (!mt_postfix(f, post)) {
                 err = "attach condition nested too deeply";
                 return VX3MI_E_PARSE;
@@ -872,10 +863,6 @@ int vx3mi_model_set_force_field(vx3mi_model *m, int axis, const char *formula_xm
         set_error("XML: " + e.msg);
         return VX3MI_E_PARSE;
     }
-    if (!f.empty() && (mt_vars_used(f) & ((1u << MV_ANGLE) | (1u << MV_CLOSENESS) | (1u << MV_PAIRS)))) {
-        set_error("force fields that use the recent angle, targetCloseness or numClosePairs are not supported");
-        return VX3MI_E_PARSE;
-    }
     if (!f.empty() && !mt_postfix(f, post)) { set_error("force-field formula nested too deeply"); return VX3MI_E_PARSE; }
     m->m.forceField[axis] = f;
     return VX3MI_OK;
@@ -894,10 +881,6 @@ int vx3mi_model_set_attach_condition(vx3mi_model *m, int k, const char *formula_
         }
     } catch (const XmlError &e) {
         set_error("XML: " + e.msg);
-        return VX3MI_E_PARSE;
-    }
-    if (!f.empty() && (mt_vars_used(f) & ((1u << MV_ANGLE) | (1u << MV_CLOSENESS) | (1u << MV_PAIRS)))) {
-        set_error("attach conditions that use the recent angle, targetCloseness or numClosePairs are not supported");
         return VX3MI_E_PARSE;
     }
     if (!f.empty() && !mt_postfix(f, post)) { set_error("attach condition nested too deeply"); return VX3MI_E_PARSE; }
