@@ -370,6 +370,25 @@ def test_config1_fp32_first_1000_steps():
 
 
 @pytest.mark.parametrize("precision", [vx3mi.FP32, vx3mi.FP64])
+def test_flat_graph_chunks_agree(precision):
+    """Flat steps run as 1000-step graphs, 64-step graphs or single launches
+    give the same bits (collisions on, both step parities)."""
+    probs = configs.config2(seed=5, n=6, collisions=1)
+    models = [vx3mi.Model.from_problem(p) for p in probs]
+    a, b = vx3mi.Batch(models, precision), vx3mi.Batch(models, precision)
+    a.step(1, mode=vx3mi.MODE_FLAT)
+    b.step(1, mode=vx3mi.MODE_FLAT)
+    a.step(2130, mode=vx3mi.MODE_FLAT)         # two 1000-step graphs, two 64-step graphs, 2 single steps
+    for n in [63] + [64] * 32 + [19]:           # odd starts: single steps, 64-step graphs
+        b.step(n, mode=vx3mi.MODE_FLAT)
+    assert a.last_step_launches() > 0
+    sa, sb = a.state(), b.state()
+    for key in ("pos", "quat", "linmom", "angmom"):
+        np.testing.assert_array_equal(sa[key], sb[key])
+    assert not a.diverged().any()
+
+
+@pytest.mark.parametrize("precision", [vx3mi.FP32, vx3mi.FP64])
 def test_collision_modes_bitwise_identical(precision):
     """Collisions on: plain, flat, persistent and tiled kernels agree bit for bit
     (states, contact pairs of the last step, target hits), over 150 steps from a

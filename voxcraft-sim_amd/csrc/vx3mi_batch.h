@@ -105,12 +105,21 @@ struct BatchBase {
     std::vector<float> curDt;      // dt uploaded to the device
     std::vector<int> hlat, hvmat;
     std::vector<VoxelMat> hvmats;
-    // graph of a chunk of flat steps
-    hipGraphExec_t graph = nullptr;
-    int graphSteps = 0, graphParity = -1;
-    bool graphStale = false;  // the launch arguments changed: capture the flat graph again
+    // graphs of flat steps: chunks of kGraphChunks[k] steps (even, so the step
+    // parity returns to its start), each captured for the parity it starts at
+    static constexpr int kGraphChunks[2] = {1000, 64};
+    hipGraphExec_t graphs[2] = {nullptr, nullptr};
+    int graphParity[2] = {-1, -1};
+    bool graphStale = false;  // the launch arguments changed: capture the flat graphs again
+    void drop_graphs() {
+        for (int k = 0; k < 2; k++) {
+            if (graphs[k]) (void)hipGraphExecDestroy(graphs[k]);
+            graphs[k] = nullptr;
+            graphParity[k] = -1;
+        }
+    }
     virtual ~BatchBase() {
-        if (graph) (void)hipGraphExecDestroy(graph);
+        drop_graphs();
         for (void *p : allocs) (void)hipFree(p);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);

@@ -568,20 +568,19 @@ template <typename T> __device__ __forceinline__ void col_track(const Dev<T> &a,
     }
 }
 
-// rebuild kernels: every surface voxel of the batch, robots whose lists are
-// stale (all = 0: only when some robot is)
-template <typename T> __global__ __launch_bounds__(256) void k_col_count(Dev<T> a, int s0, int n, int parity, int all) {
-    if (!all && !a.anyStale[parity]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) col_count(a, s0 + i, parity);
-}
-// prefix sum and scatter, one workgroup per robot
+// rebuild kernels for robots whose lists are stale (all = 0: they return at
+// once unless some robot is).  The grid: one workgroup per robot counts its
+// surface voxels into buckets, takes the prefix sum and scatters (one launch
+// fewer per step than a batch-wide count pass; a listing comes every ~100
+// steps, the launch every step)
 template <typename T> __global__ __launch_bounds__(1024) void k_col_scan(Dev<T> a, int r0, int parity, int all) {
     __shared__ int wsum[1024];
     if (!all && !a.anyStale[parity]) return;
     const int r = r0 + blockIdx.x;
     const RobotParams &rp = a.rp[r];
     if (!a.rt[r].stale[parity] || a.rt[r].diverged || rp.scount == 0) return;
+    for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_count(a, i, parity);
+    __syncthreads();
     col_scan(a, r, wsum);
     __syncthreads();
     for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_scatter(a, i, parity);
@@ -1661,12 +1660,11 @@ template <typename T> struct Batch : BatchBase {
     }
 
     // the collision phase of one step for surface voxels [s0, s0 + n) (robots
-    // [r0, r0 + nr)): the listing passes -- count, prefix sum + scatter, lists;
-    // they return at once unless some robot's lists are stale (all: check the
-    // robots' own flags only) -- then the contact forces
+    // [r0, r0 + nr)): the listing passes -- count + prefix sum + scatter per
+    // robot, lists per voxel; they return at once unless some robot's lists are
+    // stale (all: check the robots' own flags only) -- then the contact forces
     void launch_listing(int s0, int n, int r0, int nr, int parity, int all) {
         const int g = (n + 255) / 256;
-        k_col_count<T><<<g, 256, 0, stream>>>(d, s0, n, parity, all);
         k_col_scan<T><<<nr, 1024, 0, stream>>>(d, r0, parity, all);
         k_col_list<T><<<g, 256, 0, stream>>>(d, s0, n, parity, all);
     }
@@ -1674,7 +1672,7 @@ template <typename T> struct Batch : BatchBase {
         launch_listing(s0, n, r0, nr, parity, all);
         k_col_contact<T><<<(n * kContactLanes + 255) / 256, 256, 0, stream>>>(d, s0, n, parity);
     }
-    static constexpr int kCollisionLaunches = 4;  // plain mode; the flat step runs the contacts beside the links
+    static constexpr int kCollisionLaunches = 3;  // plain mode; the flat step runs the contacts beside the links
 
     void launch_flat_step(int parity) {
         if (anyCollisions) {
@@ -1728,8 +1726,7 @@ template <typename T> struct Batch : BatchBase {
             HIP_TRY(hipGetLastError());
         }
         if (graphStale) {
-            if (graph) (void)hipGraphExecDestroy(graph);
-            graph = nullptr;
+            drop_graphs();
             graphStale = false;
         }
         HIP_TRY(hipEventRecord(ev0, stream));
@@ -1782,22 +1779,25 @@ template <typename T> struct Batch : BatchBase {
                 launches++;
             }
         } else {
-            const int chunk = 64;  // steps per captured graph (even: parity returns to its start)
+            // 1000-step graphs while at least that many steps remain, then 64-step
+            // ones, then single steps
             int s = 0;
-            if (nsteps >= chunk) {
-                if (!graph || graphParity != parity) {
-                    if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
-                    hipGraph_t g;
+            for (int g = 0; g < 2; g++) {
+                const int chunk = kGraphChunks[g];
+                if (nsteps - s < chunk) continue;
+                if (!graphs[g] || graphParity[g] != parity) {
+                    if (graphs[g]) { (void)hipGraphExecDestroy(graphs[g]); graphs[g] = nullptr; }
+                    hipGraph_t cg;
                     HIP_TRY(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
                     for (int k = 0; k < chunk; k++) launch_flat_step((parity + k) & 1);
-                    HIP_TRY(hipStreamEndCapture(stream, &g));
-                    HIP_TRY(hipGraphInstantiate(&graph, g, nullptr, nullptr, 0));
-                    HIP_TRY(hipGraphDestroy(g));
-                    graphParity = parity;
-                    graphSteps = chunk;
+                    HIP_TRY(hipStreamEndCapture(stream, &cg));
+                    const hipError_t e = hipGraphInstantiate(&graphs[g], cg, nullptr, nullptr, 0);
+                    (void)hipGraphDestroy(cg);
+                    HIP_TRY(e);
+                    graphParity[g] = parity;
                 }
                 for (; s + chunk <= nsteps; s += chunk) {
-                    HIP_TRY(hipGraphLaunch(graph, stream));
+                    HIP_TRY(hipGraphLaunch(graphs[g], stream));
                     launches += (long)kernels_per_flat_step() * chunk;
                 }
             }
@@ -1962,7 +1962,7 @@ template <typename T> struct Batch : BatchBase {
         tileSel = modes[best] == VX3MI_MODE_TILED ? tiles[best] : (plans[0].ok ? 0 : 1);
         persistThreads = threads[best] ? threads[best] : 256;
         parity = parity0;
-        if (graph) { (void)hipGraphExecDestroy(graph); graph = nullptr; }
+        drop_graphs();
         choice[0] = autoMode;
         choice[1] = tileSel;
         choice[2] = persistThreads;

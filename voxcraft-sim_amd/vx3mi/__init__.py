@@ -19,7 +19,8 @@ __all__ = ["lib", "Model", "Batch", "VX3MIError", "FP32", "FP64", "MODE_AUTO", "
            "MODE_PERSISTENT", "MODE_TILED", "Diff", "run", "FORMULA_STOP", "FORMULA_FITNESS", "PROBLEM_DEFAULTS", "MATERIAL_DEFAULTS", "library_path"]
 
 _PKG = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-_LIB_PATH = os.path.join(_PKG, "lib", "libvx3mi.so")
+# the in-tree build; VX3MI_LIB names another build of the same library
+_LIB_PATH = os.environ.get("VX3MI_LIB") or os.path.join(_PKG, "lib", "libvx3mi.so")
 
 FP32, FP64 = 32, 64
 MODE_AUTO, MODE_PLAIN, MODE_FLAT, MODE_PERSISTENT, MODE_TILED = 0, 1, 2, 3, 4
