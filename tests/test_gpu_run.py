@@ -77,6 +77,89 @@ def test_stop_formula_fitness_and_history():
     assert "|[[[" in hist[0] and "[[[]]]" in hist[0]
 
 
+def render_frame(prob, ref, j, t, L=0.01):
+    """A .history frame (VX3_SimulationManager.cu:61-111) rendered from the oracle's
+    state with the reference's printf formats and its cornerOffset
+    (VX3_Voxel.cu:147-159: per axis (1 + axialStrain(posLink)) of the link on
+    that side, float; half the thermally scaled size), for a robot without
+    failing links, signals or removal."""
+    vs = 1 / 0.001
+    st = ref.state()
+    topo = ref.topology()
+    temp, _ = ref.temperature()
+    _, strain = ref.link_state()
+    pal = np.asarray(prob["cells"]).reshape(-1)
+    pal = pal[pal > 0]
+    E = np.array([m.get("E", 1e6) for m in prob["materials"]], dtype=np.float32)
+    cte = np.array([m.get("cte", 0.0) for m in prob["materials"]], dtype=np.float32)
+    slot = {}
+    for l, (n, p_, a) in enumerate(zip(topo["link_neg"], topo["link_pos"], topo["link_axis"])):
+        slot[(int(n), 2 * int(a))] = l
+        slot[(int(p_), 2 * int(a) + 1)] = l
+    out = ["<<<Step%d Time:%f>>>" % (j, t)]
+    for v in topo["surface"]:
+        v = int(v)
+        p = st["pos"][v]
+        q = st["quat"][v]
+        out.append("%.1f,%.1f,%.1f," % (p[0] * vs, p[1] * vs, p[2] * vs))
+        w = q[0]
+        out.append("%.1f,%.2f,%.2f,%.2f," % (2.0 * math.acos(1 if w > 1 else w) * 57.29577951308232, q[1], q[2], q[3]))
+        m = pal[v] - 1
+        factor = np.float32(np.float32(1) + np.float32(temp[v]) * cte[m])
+        half = 0.5 * (L * float(factor))
+        offs = []
+        for pos_link in (False, True):
+            for i in range(3):
+                l = slot.get((v, 2 * i + (0 if pos_link else 1)))
+                if l is None:
+                    s_ = 1.0 if pos_link else -1.0
+                else:
+                    ratio = np.float32(E[pal[topo["link_pos"][l]] - 1] / E[pal[topo["link_neg"][l]] - 1])
+                    two = np.float32(2.0)
+                    a = (two * strain[l] * ratio / (np.float32(1) + ratio)) if pos_link else (two * strain[l] / (np.float32(1) + ratio))
+                    s_ = float(np.float32(np.float32(1) + np.float32(a))) * (1 if pos_link else -1)
+                offs.append(float(np.float32(half * s_)))
+        out.append("%.1f,%.1f,%.1f,%.1f,%.1f,%.1f," % tuple(o * vs for o in offs))
+        out.append("%d," % pal[v])
+        out.append("%.1f," % 0.0)
+        out.append(";")
+    out.append("<<<>>>")
+    out.append("|[[[%d]]]" % j)
+    for n, p_ in zip(topo["link_neg"], topo["link_pos"]):
+        a, b = st["pos"][p_], st["pos"][n]
+        out.append("%.4f,%.4f,%.4f," % tuple(a) + "%.4f,%.4f,%.4f," % tuple(b) + ";")
+    out.append("[[[]]]")
+    return "".join(out)
+
+
+def test_history_frames_byte_for_byte():
+    """Every frame of an FP64 run (voxels and links) equals the frame rendered
+    from the oracle at that iteration with the reference's formats."""
+    prob = dict(configs.BASE, dims=(3, 3, 2), cells=np.array([1, 2, 1, 2, 1, 2, 1, 2, 1, 1, 1, 2, 2, 1, 2, 1, 2, 1],
+                                                          dtype=np.int8), phase=None, enable_collision=0,
+                materials=[dict(configs.SOFT, E=1e6, cte=0.01), dict(configs.SOFT, E=1e7, cte=0.01)])
+    m0 = vx3mi.Model.from_problem(prob)
+    dt = float(np.float32(m0.settings()["dt_frac"] * m0.recommended_dt()))
+    text = W.write_vxa(prob, stop=("mtSUB", ("mtVAR", "t"), ("mtCONST", 40.5 * dt)), record=(2, 1, 1))
+    m = vx3mi.Model.from_xml(text)
+    b = vx3mi.Batch([m], vx3mi.FP64)
+    log("runner: history bytes")
+    res, hist, _ = vx3mi.run(b, [m])
+    real = res[0]["real_stepsize"]
+    lines = [x for x in hist[0].split("\n") if x.startswith("<<<Step")]
+    assert len(lines) >= 3
+    ref = O.OracleModel(prob)
+    done, t = 0, 0.0
+    for line in lines:
+        j = int(line[len("<<<Step"):line.index(" ")])
+        assert j % real == 0
+        while done < j + 1:
+            ref.step(1)
+            done += 1
+        assert line == render_frame(prob, ref, j, ref.time), f"frame {j}"
+    b.close()
+
+
 def test_robots_stop_independently_in_one_batch():
     """Three robots with different stop times run in one batch; each one's
     results equal those of the robot run alone."""

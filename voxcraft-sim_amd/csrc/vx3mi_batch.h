@@ -152,7 +152,7 @@ struct BatchBase {
     virtual int recommended(double *out) = 0;
     // one robot's absolute positions, orientations, temperatures and link
     // strains (robot voxel / link order)
-    virtual int frame(int robot, double *pos, double *quat, float *temp, float *strain) = 0;
+    virtual int frame(int robot, double *pos, double *quat, float *temp, float *strain, float *maxStrain = nullptr) = 0;
     // times the launch configurations on trial steps and keeps the fastest for
     // VX3MI_MODE_AUTO; the batch state is left as it was
     virtual int autotune(int trialSteps, double *ms, int *choice) = 0;

@@ -2100,7 +2100,7 @@ template <typename T> struct Batch : BatchBase {
         return VX3MI_OK;
     }
 
-    int frame(int robot, double *pos, double *quat, float *temp, float *strain) override {
+    int frame(int robot, double *pos, double *quat, float *temp, float *strain, float *maxStrain) override {
         if (robot < 0 || robot >= R) { set_error("robot index out of range"); return VX3MI_E_ARG; }
         const RobotParams &p = hrp[robot];
         std::vector<T4<T>> a0((size_t)p.vcount), a1((size_t)p.vcount), l2((size_t)p.lcount);
@@ -2121,8 +2121,10 @@ template <typename T> struct Batch : BatchBase {
             if (quat) { quat[4 * i] = a0[i].w; quat[4 * i + 1] = a1[i].x; quat[4 * i + 2] = a1[i].y; quat[4 * i + 3] = a1[i].z; }
             if (temp) temp[i] = (float)a1[i].w;
         }
-        for (int l = 0; l < p.lcount; l++)
+        for (int l = 0; l < p.lcount; l++) {
             if (strain) strain[l] = (float)l2[l].w;
+            if (maxStrain) maxStrain[l] = (float)l2[l].y;
+        }
         return VX3MI_OK;
     }
 

@@ -139,7 +139,7 @@ std::string history_header(const Model &m) {
 // start of the frame's step that are still present, removed links are left
 // out and attached links follow the lattice links.
 std::string history_frame(const Model &m, long long j, double t, const double *pos, const double *q,
-                          const float *temp, const float *strain, const int *vrem = nullptr,
+                          const float *temp, const float *strain, const float *maxStrain, const int *vrem = nullptr,
                           const int *lrem = nullptr, long long stampNow = 0, const double *signal = nullptr,
                           const std::vector<int4> *att = nullptr) {
     std::string s;
@@ -180,8 +180,9 @@ std::string history_frame(const Model &m, long long j, double t, const double *p
                 const bool posLink = corner == 1;
                 for (int i = 0; i < 3; i++) {
                     const int l = link_at(v, 2 * i + (posLink ? 0 : 1));
+                    const float epsFail = l >= 0 ? m.lmats[m.lmat[l]].epsFail : -1.0f;
                     double st;
-                    if (l >= 0) {
+                    if (l >= 0 && !(epsFail != -1.0f && maxStrain[l] > epsFail)) {  // present, not failed
                         const float ratio = m.vmatE[m.vmat[m.lpos[l]]] / m.vmatE[m.vmat[m.lneg[l]]];
                         const float a = posLink ? 2.0f * strain[l] * ratio / (1.0f + ratio) : 2.0f * strain[l] / (1.0f + ratio);
                         st = (1 + a) * (posLink ? 1 : -1);
@@ -239,7 +240,7 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
     std::vector<RobotRun> run((size_t)n);
     std::vector<double> com((size_t)3 * n);
     std::vector<double> pos, quat;
-    std::vector<float> temp, strain;
+    std::vector<float> temp, strain, maxStrain;
     std::vector<int> vrem, lrem;
     std::vector<int4> attached;
     std::vector<double> signal;
@@ -413,7 +414,8 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
                 quat.resize((size_t)4 * m.nv());
                 temp.resize((size_t)m.nv());
                 strain.resize((size_t)std::max(1, m.nl()));
-                if ((rc = b.frame(r, pos.data(), quat.data(), temp.data(), strain.data()))) return rc;
+                maxStrain.resize(strain.size());
+                if ((rc = b.frame(r, pos.data(), quat.data(), temp.data(), strain.data(), maxStrain.data()))) return rc;
                 const int *vr = nullptr, *lr = nullptr;
                 if (b.anyRemoval && m.secondaryExperiment) {
                     vrem.resize((size_t)m.nv());
@@ -434,7 +436,7 @@ extern "C" int vx3mi_run(vx3mi_batch *bh, vx3mi_model *const *models, int n, lon
                     at = &attached;
                 }
                 const std::string f = history_frame(m, x.j - 1, x.t, pos.data(), quat.data(), temp.data(), strain.data(),
-                                                    vr, lr, (long long)steps0[r] + x.j, sg, at);
+                                                    maxStrain.data(), vr, lr, (long long)steps0[r] + x.j, sg, at);
                 history(user, r, f.data(), f.size());
             }
         }
