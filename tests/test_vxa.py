@@ -235,8 +235,10 @@ BAD = {
                                                        "</AttachCondition></AttachDetach>"),
     "bc_count": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs></Boundary_Conditions>"
                                           "</Environment><VXC "),
-    "bc_mesh": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs><FRegion><PrimType>3"
-                                         "</PrimType></FRegion></Boundary_Conditions></Environment><VXC "),
+    "bc_mesh_index": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs><FRegion><PrimType>3"
+                                               "</PrimType><CMesh><Vertices><Vertex><Vx>1</Vx></Vertex></Vertices>"
+                                               "<Facets><Facet><V0>0</V0><V1>1</V1><V2>0</V2></Facet></Facets></CMesh>"
+                                               "</FRegion></Boundary_Conditions></Environment><VXC "),
     "bc_no_prim": DEMO_LIKE.replace("<VXC ", "<Environment><Boundary_Conditions><NumBCs>1</NumBCs><FRegion><X>0</X>"
                                             "</FRegion></Boundary_Conditions></Environment><VXC "),
     "bad_token": DEMO_LIKE.replace("<mtSUB>", "<mtFOO>").replace("</mtSUB>", "</mtFOO>"),
@@ -360,6 +362,67 @@ def test_displacements_and_overlapping_regions():
         assert e["force"][0] == float(expect), (v, e["force"][0], expect)
         assert e["moment"][1] == (float(np.float32(0.2 / 16)) if in2 else 0.0)
     assert len(ext) == 16 + 16 - 4
+
+
+CUBE_MESH = (
+    [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)],
+    [(0, 2, 1), (0, 3, 2), (4, 5, 6), (4, 6, 7), (0, 1, 5), (0, 5, 4), (3, 7, 6), (3, 6, 2), (0, 4, 7), (0, 7, 3),
+     (1, 2, 6), (1, 6, 5)])
+TETRA_MESH = ([(0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1)], [(0, 2, 1), (0, 1, 3), (0, 3, 2), (1, 2, 3)])
+
+
+def _solid(n):
+    return dict(configs.config0(), dims=(n, n, n), cells=np.ones(n ** 3, dtype=np.int8), phase=None)
+
+
+def _mesh_points(n, X, dX, size=1.0):
+    """Voxel centres of an n^3 workspace mapped into mesh coordinates as
+    CP_Mesh::IsIn maps them (region origin X, uniform scale min(dX * Env / size))."""
+    idx = np.stack(np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij"), -1).reshape(-1, 3)[:, ::-1]
+    centre = (idx + 0.5) / n  # workspace fractions, x fastest like the voxel order
+    scale = min(d / size for d in dX)
+    return (centre - np.asarray(X)) / scale
+
+
+def test_mesh_regions_select_the_voxels_whose_centres_lie_inside():
+    """Mesh regions (PrimType 3, CP_Mesh::IsIn, src/old/VX_FRegion.cpp:668-686):
+    the mesh is scaled uniformly into the region's box and a voxel is in when its
+    centre is inside the closed surface (x-ray parity, CMesh::IsInside).  Region
+    offsets keep the rays off the facets' shared edges (a ray through an edge
+    meets both facets, as in the reference).  A unit cube mesh: the voxels whose
+    mapped centres lie in the open unit cube; a corner tetrahedron: those with
+    every coordinate positive and x + y + z < 1."""
+    X, dX = (0.25, 0.2, 0.3), (0.5, 0.5, 0.5)
+    cube = vx3mi.Model.from_xml(W.write_vxa(dict(configs.config0(), bc_regions=[
+        dict(PrimType=3, X=X[0], Y=X[1], Z=X[2], dX=dX[0], dY=dX[1], dZ=dX[2], DofFixed=7, mesh=CUBE_MESH)])))
+    ext = _externals(cube)
+    pts = _mesh_points(4, X, dX)
+    assert np.abs(pts[:, 1] - pts[:, 2]).min() > 1e-6  # the x faces are split along y = z
+    want = [v for v in range(64) if ((pts[v] > 0) & (pts[v] < 1)).all()]
+    assert len(want) == 8 and sorted(ext) == want
+    assert all(e["dof"] == 7 for e in ext.values())
+    X, dX = (0.037, 0.021, 0.013), (0.93, 0.91, 0.95)
+    tet = vx3mi.Model.from_xml(W.write_vxa(dict(_solid(6), bc_regions=[
+        dict(PrimType=3, X=X[0], Y=X[1], Z=X[2], dX=dX[0], dY=dX[1], dZ=dX[2], DofFixed=1, ForceZ=-3.0,
+             mesh=TETRA_MESH)])))
+    ext = _externals(tet)
+    pts = _mesh_points(6, X, dX)
+    assert np.abs(pts[:, 1] + pts[:, 2] - 1).min() > 1e-6
+    want = [v for v in range(216) if (pts[v] > 0).all() and pts[v].sum() < 1]
+    assert len(want) > 20 and sorted(ext) == want
+    for e in ext.values():  # the force is shared by the voxels the region touches
+        assert e["force"][2] == float(np.float32(-3.0 / len(want)))
+
+
+def test_mesh_region_scales_by_its_smallest_ratio():
+    """The mesh keeps its aspect: a unit cube mesh in a region of 0.5 x 1 x 1
+    workspace fractions is scaled by the smallest ratio (0.5) and covers the
+    first half of the workspace along every axis from the region's origin."""
+    m = vx3mi.Model.from_xml(W.write_vxa(dict(configs.config0(), bc_regions=[
+        dict(PrimType=3, X=0.01, Y=0.02, Z=0.03, dX=0.5, dY=1.0, dZ=1.0, DofFixed=63, mesh=CUBE_MESH)])))
+    ext = _externals(m)
+    lat = m.topology()["lattice"]
+    assert sorted(ext) == [v for v in range(64) if all(c in (0, 1) for c in lat[v])]
 
 
 def test_legacy_fixed_regions_and_binary_fixed_tag():

@@ -9,6 +9,7 @@
 //   environment CVX_Environment::ReadXML (src/old/VX_Environment.cpp:101-171)
 //   palette    CVXC_Material::ReadXML (src/VXA/VX_Object.cpp:1373-1475)
 //   structure  CVXC_Structure::ReadXML (src/VXA/VX_Object.cpp:1760-1958)
+#include <algorithm>
 #include <cctype>
 #include <cmath>
 #include <cstdlib>
@@ -281,11 +282,14 @@ std::vector<std::string> split_commas(const std::string &s) {
 }
 
 // One boundary-condition region (CVX_FRegion, src/old/VX_FRegion.cpp:165-219):
-// a box, cylinder or sphere in workspace fractions, the degrees of freedom it
-// fixes and the displacement / force it applies.
+// a box, cylinder, sphere or triangle mesh in workspace fractions, the degrees
+// of freedom it fixes and the displacement / force it applies.
 struct BcRegion {
-    int prim = 0;  // PRIM_BOX 0, PRIM_CYLINDER 1, PRIM_SPHERE 2
+    int prim = 0;  // PRIM_BOX 0, PRIM_CYLINDER 1, PRIM_SPHERE 2, PRIM_MESH 3
     double X = 0, Y = 0, Z = 0, dX = 0, dY = 0, dZ = 0, radius = 0;
+    std::vector<double> mv;  // mesh vertices (x, y, z each)
+    std::vector<int> mf;     // mesh facets (three vertex indices each)
+    double bbMin[3] = {0, 0, 0}, bbSize[3] = {0, 0, 0};
     int dof = 0;
     double force[3] = {0, 0, 0}, torque[3] = {0, 0, 0}, disp[3] = {0, 0, 0}, angDisp[3] = {0, 0, 0};
 };
@@ -312,7 +316,7 @@ struct VxaContent {
 // CVX_FRegion::ReadXML (src/old/VX_FRegion.cpp:165-219)
 int read_region(const XNode *fr, BcRegion &b, std::string &err) {
     if (!load(fr, "PrimType", b.prim)) { err = "FRegion without PrimType"; return VX3MI_E_PARSE; }
-    if (b.prim < 0 || b.prim > 2) { err = "only box, cylinder and sphere boundary-condition regions are supported"; return VX3MI_E_PARSE; }
+    if (b.prim < 0 || b.prim > 3) { err = "boundary-condition region PrimType must be 0..3"; return VX3MI_E_PARSE; }
     load(fr, "X", b.X); load(fr, "Y", b.Y); load(fr, "Z", b.Z);
     load(fr, "dX", b.dX); load(fr, "dY", b.dY); load(fr, "dZ", b.dZ);
     load(fr, "Radius", b.radius);
@@ -328,7 +332,119 @@ int read_region(const XNode *fr, BcRegion &b, std::string &err) {
         load(fr, dk[k], b.disp[k]);
         load(fr, ak[k], b.angDisp[k]);
     }
+    if (b.prim == 3) {
+        // CMesh::ReadXML (src/old/Mesh.cpp:133-217): vertices Vx, Vy, Vz and facets
+        // V0, V1, V2 (absent values 0), then the bounding box
+        const XNode *mesh = fr->child("CMesh");
+        if (const XNode *vs = mesh ? mesh->child("Vertices") : nullptr)
+            for (const XNode &vx : vs->kids) {
+                if (vx.name != "Vertex") continue;
+                double x = 0, y = 0, z = 0;
+                load(&vx, "Vx", x); load(&vx, "Vy", y); load(&vx, "Vz", z);
+                b.mv.insert(b.mv.end(), {x, y, z});
+            }
+        if (const XNode *fs = mesh ? mesh->child("Facets") : nullptr)
+            for (const XNode &fc : fs->kids) {
+                if (fc.name != "Facet") continue;
+                int i0 = 0, i1 = 0, i2 = 0;
+                load(&fc, "V0", i0); load(&fc, "V1", i1); load(&fc, "V2", i2);
+                b.mf.insert(b.mf.end(), {i0, i1, i2});
+            }
+        const int nvert = (int)b.mv.size() / 3;
+        for (int i : b.mf)
+            if (i < 0 || i >= nvert) {  // the reference would read past its vertex list
+                err = "mesh region facet with a vertex index out of range";
+                return VX3MI_E_PARSE;
+            }
+        if (nvert) {  // CMesh::UpdateBoundingBox (Mesh.cpp:673-689)
+            double lo[3] = {b.mv[0], b.mv[1], b.mv[2]}, hi[3] = {b.mv[0], b.mv[1], b.mv[2]};
+            for (int i = 0; i < nvert; i++)
+                for (int k = 0; k < 3; k++) {
+                    const double x = b.mv[3 * i + k];
+                    lo[k] = lo[k] < x ? lo[k] : x;
+                    hi[k] = hi[k] > x ? hi[k] : x;
+                }
+            for (int k = 0; k < 3; k++) { b.bbMin[k] = lo[k]; b.bbSize[k] = hi[k] - lo[k]; }
+        }
+    }
     return VX3MI_OK;
+}
+
+// ---- mesh regions: the x-ray parity test of CMesh::IsInside -------------------
+struct D3 { double x, y, z; };
+D3 d3sub(D3 a, D3 b) { return D3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+double d3dot(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+D3 d3cross(D3 a, D3 b) { return D3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+D3 d3norm(D3 a) {  // Vec3D::Normalized: unchanged when of zero length
+    const double l = std::sqrt(a.x * a.x + a.y * a.y + a.z * a.z);
+    return l > 0 ? D3{a.x / l, a.y / l, a.z / l} : a;
+}
+double det3(D3 v0, D3 v1, D3 v2) {  // CMesh::Det (Mesh.cpp:877-884)
+    return v0.x * (v1.y * v2.z - v1.z * v2.y) + v0.y * (v1.z * v2.x - v1.x * v2.z) + v0.z * (v1.x * v2.y - v1.y * v2.x);
+}
+
+// CMesh::IntersectXRay (Mesh.cpp:887-931): where the x-parallel ray at (y, z)
+// meets the facet's plane, accepted by the two edge tests the reference makes
+bool xray_hits(D3 a, D3 b, D3 c, double y, double z, D3 &p) {
+    const double minX = a.x < b.x ? (a.x < c.x ? a.x : c.x) : (b.x < c.x ? b.x : c.x) - 1.0;
+    const D3 o = {minX, y, z};
+    const D3 n = d3norm(d3cross(d3sub(b, a), d3sub(c, a)));
+    const double dn = 1.0 * n.x + 0.0 * n.y + 0.0 * n.z;
+    if (std::fabs(dn) < 1E-5) return false;  // parallel
+    const double dist = -d3dot(d3sub(o, a), n) / dn;
+    p = D3{o.x + 1.0 * dist, o.y + 0.0 * dist, o.z + 0.0 * dist};
+    const double V1 = d3dot(d3cross(d3sub(b, a), d3sub(p, a)), n);
+    const double V2 = d3dot(d3cross(d3sub(c, b), d3sub(p, b)), n);
+    return V1 >= 0 && V2 >= 0;
+}
+
+// CMesh::InsideTri (Mesh.cpp:851-874): p projects into the triangle
+bool inside_tri(D3 p, D3 v0, D3 v1, D3 v2) {
+    const D3 xax = d3norm(d3sub(v1, v0));
+    const D3 zax = d3norm(d3cross(d3sub(v2, v0), xax));
+    const D3 yax = d3norm(d3cross(zax, xax));
+    const D3 p0 = {0, 0, 1};
+    const D3 p1 = {d3dot(d3sub(v1, v0), xax), d3dot(d3sub(v1, v0), yax), 1};
+    const D3 p2 = {d3dot(d3sub(v2, v0), xax), d3dot(d3sub(v2, v0), yax), 1};
+    const D3 pt = {d3dot(d3sub(p, v0), xax), d3dot(d3sub(p, v0), yax), 1};
+    const double e0 = det3(p0, p1, pt), e1 = det3(p1, p2, pt), e2 = det3(p2, p0, pt);
+    return (e0 <= 0 && e1 <= 0 && e2 <= 0) || (e0 >= 0 && e1 >= 0 && e2 >= 0);
+}
+
+// CP_Mesh::IsIn (src/old/VX_FRegion.cpp:668-686) with CMesh::IsInside,
+// FillTriLayer and FillTriLine (Mesh.cpp:787-804, 1046-1116): the point is
+// mapped into mesh coordinates (region origin, uniform scale of the smallest
+// region-to-mesh ratio, mesh bounding-box corner), and is inside when an odd
+// number of the facets crossing its z plane and y line meet the x ray at or
+// before it
+bool mesh_is_in(const BcRegion &b, const double P[3], const double Env[3]) {
+    if (b.mv.empty()) return false;
+    double sc[3];
+    for (int k = 0; k < 3; k++) sc[k] = (k == 0 ? b.dX : k == 1 ? b.dY : b.dZ) * (Env[k] / b.bbSize[k]);
+    double s = sc[0];
+    if (sc[1] < s) s = sc[1];
+    if (sc[2] < s) s = sc[2];
+    const double o[3] = {b.X * Env[0], b.Y * Env[1], b.Z * Env[2]};
+    const D3 pt = {(P[0] - o[0]) / s + b.bbMin[0], (P[1] - o[1]) / s + b.bbMin[1], (P[2] - o[2]) / s + b.bbMin[2]};
+    auto vert = [&](int i) { return D3{b.mv[3 * i], b.mv[3 * i + 1], b.mv[3 * i + 2]}; };
+    std::vector<double> hits;
+    for (size_t f = 0; f + 2 < b.mf.size(); f += 3) {
+        const D3 a = vert(b.mf[f]), c1 = vert(b.mf[f + 1]), c2 = vert(b.mf[f + 2]);
+        const bool below = a.z <= pt.z && c1.z <= pt.z && c2.z <= pt.z;  // FillTriLayer
+        const bool above = a.z >= pt.z && c1.z >= pt.z && c2.z >= pt.z;
+        if (below || above) continue;
+        if (a.y < pt.y && c1.y < pt.y && c2.y < pt.y) continue;  // FillTriLine
+        if (a.y > pt.y && c1.y > pt.y && c2.y > pt.y) continue;
+        D3 p;
+        if (xray_hits(a, c1, c2, pt.y, pt.z, p) && inside_tri(p, a, c1, c2)) hits.push_back(p.x);
+    }
+    std::sort(hits.begin(), hits.end());
+    int count = 0;
+    for (double x : hits) {
+        if (pt.x < x) break;
+        count++;
+    }
+    return count % 2 == 1;
 }
 
 // the first n FRegion children of a region list (CVX_Environment::ReadXML,
@@ -349,8 +465,10 @@ int read_regions(const XNode *list, const char *countKey, std::vector<BcRegion> 
 }
 
 // CPrimitive::IsTouching(P, Dist, Envelope) of the box / cylinder / sphere
-// (src/old/VX_FRegion.cpp:420-428, 494-524, 601-605)
+// (src/old/VX_FRegion.cpp:420-428, 494-524, 601-605); a mesh is touched where
+// its IsIn holds (:690-698)
 bool region_touches(const BcRegion &b, const double P[3], const double D[3], const double Env[3]) {
+    if (b.prim == 3) return mesh_is_in(b, P, Env);
     if (b.prim == 0) {
         const double B0[3] = {b.X, b.Y, b.Z}, B1[3] = {b.dX, b.dY, b.dZ};
         for (int k = 0; k < 3; k++) {

@@ -101,15 +101,23 @@ REGION_KEYS = ("PrimType", "X", "Y", "Z", "dX", "dY", "dZ", "Radius", "DofFixed"
 
 def boundary_conditions_xml(regions: Sequence[dict]) -> str:
     """<Boundary_Conditions> with one <FRegion> per dict (keys as in
-    CVX_FRegion::ReadXML: PrimType 0 box / 1 cylinder / 2 sphere, X..dZ and Radius
-    in workspace fractions, DofFixed bits, Force*, Torque*, Displace*,
-    AngDisplace*); an empty list writes nothing."""
+    CVX_FRegion::ReadXML: PrimType 0 box / 1 cylinder / 2 sphere / 3 mesh, X..dZ
+    and Radius in workspace fractions, DofFixed bits, Force*, Torque*, Displace*,
+    AngDisplace*; a mesh region's "mesh": (vertices, facets) as a <CMesh>); an
+    empty list writes nothing."""
     if not regions:
         return ""
     body = _el("NumBCs", str(len(regions)))
     for r in regions:
-        body += _el("FRegion", "".join(_el(k, _num(r[k]) if isinstance(r[k], float) else str(r[k]))
-                                       for k in REGION_KEYS if k in r))
+        inner = "".join(_el(k, _num(r[k]) if isinstance(r[k], float) else str(r[k])) for k in REGION_KEYS if k in r)
+        if r.get("mesh") is not None:
+            verts, facets = r["mesh"]
+            inner += _el("CMesh", _el("Vertices", "".join(
+                _el("Vertex", _el("Vx", _num(float(x))) + _el("Vy", _num(float(y))) + _el("Vz", _num(float(z))))
+                for x, y, z in verts)) + _el("Facets", "".join(
+                    _el("Facet", _el("V0", str(int(a))) + _el("V1", str(int(b))) + _el("V2", str(int(c))))
+                    for a, b, c in facets)))
+        body += _el("FRegion", inner)
     return _el("Boundary_Conditions", body)
 
 
