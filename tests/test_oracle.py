@@ -265,6 +265,30 @@ def test_two_sphere_contact_force():
     assert np.array_equal(a.state()["linmom"], b.state()["linmom"])
 
 
+def test_volume_effects_keep_the_transverse_info_of_creation():
+    """VX3_Link::updateForces leaves updateTransverseInfo commented out
+    (VX3_Link.cu:145-149): with volume effects the transverse strain sum and
+    area stay those of the link's creation (zero strain: 0 and L^2).  A bar of
+    two voxels (Poisson 0.3) stretched by 2 % then has the axial force
+    eHat (1 - nu) strain L^2; with the transverse strains of the stretch
+    (-nu strain per voxel and axis, summed over two axes) the stress would be
+    lower by eHat nu * 2 nu strain, 26 % of it."""
+    nu, E, L = 0.3, 1e6, 0.01
+    p = quiet((2, 1, 1), volume_effects=1, materials=[dict(E=E, rho=1000.0, cte=0.0, nu=nu)])
+    m = O.OracleModel(p)
+    pos = m.state()["pos"]
+    pos[1, 0] = L * 1.02
+    m.set_state(pos=pos)
+    f = m.update_links()[0]
+    strain = np.float32(np.float32(0.02 * L) / np.float32(L))
+    e_hat = np.float32(E) / ((np.float32(1) - np.float32(2) * np.float32(nu)) * (np.float32(1) + np.float32(nu)))
+    frozen = float(e_hat * (np.float32(1) - np.float32(nu)) * strain) * (L * L)
+    updated = float(e_hat * ((np.float32(1) - np.float32(nu)) * strain + np.float32(nu) * (-2 * nu * strain))) * (L * L)
+    assert f[0] == pytest.approx(frozen, rel=1e-5)  # forceNeg.x pulls the negative end along +x
+    assert abs(f[0] - updated) > 0.2 * abs(frozen)
+    assert f[3] == -f[0]
+
+
 def test_cpu_semantics_flag_only_changes_static_friction_rotation():
     """VX3 zeroes angular momentum of voxels held by static floor friction
     (VX3_Voxel.cu:259-264); CVoxelyze does not.  Without a floor they agree."""
