@@ -240,7 +240,7 @@ def _squeezed_block(factor, dims=(6, 6, 5)):
     return prob, ref, pos
 
 
-@pytest.mark.parametrize("mode", [vx3mi.MODE_FLAT, vx3mi.MODE_PERSISTENT])
+@pytest.mark.parametrize("mode", [vx3mi.MODE_FLAT, vx3mi.MODE_PERSISTENT, vx3mi.MODE_TILED])
 @pytest.mark.parametrize("factor", [0.45, 0.8])
 def test_dense_pack_fp64_bit_exact(factor, mode):
     """Squeezed blocks: at 0.45 of the spacing the lists overflow and every step
@@ -456,6 +456,30 @@ def test_tiled_full_batch_matches_flat():
     for k in ("pos", "quat", "linmom", "angmom"):
         assert np.array_equal(out[0][0][k], out[1][0][k]), k
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def test_tiled_full_batch_with_collisions_matches_flat():
+    """The whole 1024-robot workload with collisions on (the reference default):
+    the collision tiles (packed candidate lists, listing inside the launch) and
+    the flat kernels agree bit for bit over 600 steps in which the robots list
+    their candidates again; so do the contact pairs of the last step."""
+    probs = configs.config2(seed=0, n=1024, collisions=1)
+    models = [vx3mi.Model.from_problem(p) for p in probs]
+    out = []
+    for mode in (vx3mi.MODE_FLAT, vx3mi.MODE_TILED):
+        b = vx3mi.Batch(models, vx3mi.FP32)
+        b.step(599, mode=mode)
+        b.record_contacts()
+        b.step(1, mode=mode)
+        out.append((b.state(), b.time(), b.contact_pairs(), b.listings()))
+        assert b.diverged().sum() == 0
+        b.close()
+    log(f"listings {int(out[0][3].sum())} (flat) {int(out[1][3].sum())} (tiled), pairs {len(out[0][2])}")
+    assert out[0][3].sum() > 1024 and len(out[0][2]) > 0
+    for k in ("pos", "quat", "linmom", "angmom"):
+        assert np.array_equal(out[0][0][k], out[1][0][k]), k
+    assert np.array_equal(out[0][1], out[1][1])
+    assert np.array_equal(out[0][2], out[1][2])
 
 
 def _fixed_bottom():

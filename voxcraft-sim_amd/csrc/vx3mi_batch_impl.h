@@ -63,6 +63,7 @@ template <typename T> struct Dev {
     int *srec;             // robot-relative surface indices in bucket order, CSR by robot
     int *clist;            // per surface voxel: kMaxCand candidate surface indices, ascending
     int *ccount;           // per surface voxel: candidates (> kMaxCand: search the grid)
+    int *tcand;            // collision tiles (k_tile): packed candidates, kMaxCand x KV x NT words per tile
     T4<T> *contactW;
     int2 *pairRec;         // when recording: pairs in contact (voxel ids, smaller surface index first)
     int *pairN;
@@ -1606,6 +1607,7 @@ template <typename T> struct Batch : BatchBase {
         P.ntiles = (int)tiles.size();
         int rc;
         if ((rc = alloc(&P.d_tiles, tiles.size())) || (rc = alloc(&P.d_tileRobots, trobots.size()))) return rc;
+        if (COL && (rc = alloc(&d.tcand, tiles.size() * kMaxCand * C::KV * C::NT))) return rc;
         HIP_TRY(hipMemcpy(P.d_tiles, tiles.data(), sizeof(int4) * tiles.size(), hipMemcpyHostToDevice));
         HIP_TRY(hipMemcpy(P.d_tileRobots, trobots.data(), sizeof(int) * trobots.size(), hipMemcpyHostToDevice));
         P.ok = true;
