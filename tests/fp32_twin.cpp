@@ -233,7 +233,7 @@ template <typename T> void contact_forces(Twin &t, Side<T> &s, const std::vector
             }
             const T bsaJ = base_size_average<T>(mj.nomSize, s.temp[vj], mj.alphaCTE);
             const V3<T> velJ = b.p * (T)mj.massInv;
-            fsum = fsum + contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
+            fsum = fsum + contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penKinv, mj.penKinv, mi.cDampT, mj.cDampT);
         }
         s.contact[v] = fsum;
         g_ctx = nullptr;

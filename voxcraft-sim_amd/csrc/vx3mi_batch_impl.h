@@ -365,9 +365,10 @@ template <typename T> __device__ __forceinline__ void col_count(const Dev<T> &a,
     a.bpos[v] = q0;
 }
 
-// rebuild pass 2, one workgroup of blockDim threads per robot: exclusive prefix
-// sum of the robot's bucket counts into cstart (mask + 2 entries, the last the
-// total), counts zeroed for the next rebuild
+// rebuild pass 2, one workgroup of blockDim threads (a multiple of 64, at most
+// 1024) per robot: exclusive prefix sum of the robot's bucket counts into cstart
+// (mask + 2 entries, the last the total), counts zeroed for the next rebuild;
+// wsum holds 16 ints of LDS
 template <typename T> __device__ __forceinline__ void col_scan(const Dev<T> &a, int r, int *wsum) {
     const RobotParams &rp = a.rp[r];
     const int n = rp.mask + 1, nt = blockDim.x, tid = threadIdx.x;
@@ -375,15 +376,23 @@ template <typename T> __device__ __forceinline__ void col_scan(const Dev<T> &a, 
     int *cnt = a.ccnt + rp.toff, *start = a.cstart + rp.toff;
     int sum = 0;
     for (int b = b0; b < b1; b++) sum += cnt[b];
-    wsum[tid] = sum;
+    // exclusive prefix of the per-thread sums: an inclusive scan inside each
+    // wave, then the wave totals (at most 16) in order
+    const int lane = tid & 63, wv = tid >> 6, nw = (nt + 63) >> 6;
+    int incl = sum;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(incl, o, 64);
+        if (lane >= o) incl += y;
+    }
+    if (lane == 63 || tid == nt - 1) wsum[wv] = incl;
     __syncthreads();
-    if (tid == 0) {  // nt partial sums, exclusive
+    if (tid == 0) {
         int acc = 0;
-        for (int i = 0; i < nt; i++) { const int x = wsum[i]; wsum[i] = acc; acc += x; }
+        for (int i = 0; i < nw; i++) { const int x = wsum[i]; wsum[i] = acc; acc += x; }
         start[n] = acc;
     }
     __syncthreads();
-    int acc = wsum[tid];
+    int acc = wsum[wv] + incl - sum;
     for (int b = b0; b < b1; b++) {
         const int c = cnt[b];
         start[b] = acc;
@@ -510,7 +519,7 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
         }
         const T4<T> mj0 = a.mo0[w];
         const V3<T> velJ = V3<T>{mj0.x, mj0.y, mj0.z} * (T)mj.massInv;
-        fw = contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
+        fw = contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penKinv, mj.penKinv, mi.cDampT, mj.cDampT);
         return 1;
     };
     V3<T> f = {0, 0, 0};
@@ -575,7 +584,7 @@ template <typename T> __device__ __forceinline__ void col_track(const Dev<T> &a,
 // fewer per step than a batch-wide count pass; a listing comes every ~100
 // steps, the launch every step)
 template <typename T> __global__ __launch_bounds__(1024) void k_col_scan(Dev<T> a, int r0, int parity, int all) {
-    __shared__ int wsum[1024];
+    __shared__ int wsum[16];  // wave totals of col_scan
     if (!all && !a.anyStale[parity]) return;
     const int r = r0 + blockIdx.x;
     const RobotParams &rp = a.rp[r];
@@ -660,7 +669,7 @@ template <typename T> __device__ V3<T> pair_contact(const Dev<T> &a, int v, int 
     const V3<T> diff = col_diff<T>(a.pq0[v], a.pq0[w], lat_offset(a.lat[v], a.lat[w]), (T)mi.nomSize);
     const V3<T> velI = V3<T>{mi0.x, mi0.y, mi0.z} * (T)mi.massInv;
     const V3<T> velJ = V3<T>{mj0.x, mj0.y, mj0.z} * (T)mj.massInv;
-    return contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
+    return contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penKinv, mj.penKinv, mi.cDampT, mj.cDampT);
 }
 
 // The attach candidates of surface voxel v = surf[s] as voxel1 (:684-746):
@@ -971,7 +980,7 @@ template <typename T> __global__ __launch_bounds__(256) void k_zero_steps(RobotT
 // One workgroup per robot; nsteps steps with barrier-separated phases (the
 // collision listing when it is stale, the contact forces, links, voxels).
 template <typename T, int NT> __global__ __launch_bounds__(NT) void k_persistent(Dev<T> a, int nsteps, int parity0) {
-    __shared__ int wsum[NT];
+    __shared__ int wsum[16];  // wave totals of col_scan
     const int r = blockIdx.x;
     const RobotParams rp = a.rp[r];
     const bool col = rp.collisions && rp.scount > 0 && a.contactW != nullptr;

@@ -142,6 +142,7 @@ VoxelMat to_voxel_mat(const MatDerived &d) {
     v.cDampT = d.zC * d.c2M;
     v.gravF = -d.mass * 9.80665f * d.gravMult;
     v.penK = (float)(2 * d.E * d.nomSize);
+    v.penKinv = 1.0f / v.penK;
     v.muS = d.muS;
     v.muK = d.muK;
     v.fixed = d.fixed;

@@ -74,6 +74,7 @@ struct VoxelMat {
     float cDampT;   // collisionDampingTranslateC
     float gravF;    // gravityForce()
     float penK;     // penetrationStiffness()
+    float penKinv;  // 1 / penK, rounded as the contact force's float division rounds it
     float muS, muK;
     int fixed, isMeasured;
     float zetaI, zetaG, zetaC;  // damping ratios (the differentiable mode derives its own constants)
@@ -926,9 +927,9 @@ VX_HD bool in_contact_envelope(V3<T> diff, T bsaI, T bsaJ) {
 }
 
 template <typename T>
-VX_HD V3<T> contact_force(V3<T> diff, T bsaI, T bsaJ, V3<T> velI, V3<T> velJ, float pkI, float pkJ, float cdI,
+VX_HD V3<T> contact_force(V3<T> diff, T bsaI, T bsaJ, V3<T> velI, V3<T> velJ, float ipkI, float ipkJ, float cdI,
                           float cdJ) {
-    const T penStiff = (T)(2.0f / (1.0f / pkI + 1.0f / pkJ));
+    const T penStiff = (T)(2.0f / (ipkI + ipkJ));  // 2 / (1 / pkI + 1 / pkJ), the reciprocals from the materials
     const T dampC = (T)(0.5f * (cdI + cdJ));
     const V3<T> offset = v3<T>(-diff.x, -diff.y, -diff.z);
     const T nomDist = (bsaI + bsaJ) * (T)kCOLLISION_ENVELOPE_RADIUS;

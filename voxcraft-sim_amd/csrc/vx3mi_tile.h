@@ -251,13 +251,15 @@ __global__ __launch_bounds__((TileSel<T, SMALL, COL>::C::NT), (TileSel<T, SMALL,
         j = tile_slot(tr.spre, nr, i);
         return a.rp[tr.rid[j]].soff + (i - tr.spre[j]);
     };
-    // packed candidates of the tile's voxels: per candidate rank c, owned voxel
-    // slot k and thread one word holding the partner's tile index (11 bits), its
-    // material (5 bits; the materials are in LDS) and its lattice offset from
-    // the voxel (5 signed bits per axis); the threads of a wave read consecutive
-    // words, and a tile's entries stay in its XCD's L2
-    const size_t cbase = COL ? (size_t)blockIdx.x * kMaxCand * KV * NT : 0;
-    auto cand_at = [&](int c, int k) -> size_t { return cbase + ((size_t)c * KV + k) * NT + tid; };
+    // packed candidates of the tile's voxels: one word per candidate holding the
+    // partner's tile index (11 bits), its material (5 bits; the materials are in
+    // LDS) and its lattice offset from the voxel (5 signed bits per axis); four
+    // candidates of a voxel per int4, laid out by (rank / 4, owned voxel slot k,
+    // thread), so the threads of a wave read consecutive int4s, and a tile's
+    // entries stay in its XCD's L2
+    int4 *const tc4 = reinterpret_cast<int4 *>(a.tcand);
+    const size_t cbase = COL ? (size_t)blockIdx.x * (kMaxCand / 4) * KV * NT : 0;
+    auto cand4_at = [&](int c4, int k) -> size_t { return cbase + ((size_t)c4 * KV + k) * NT + tid; };
     // owned voxel k's packed entries from its candidate list, and its position
     // field at the listing (at the start of a launch and after each listing)
     auto pack = [&](int k) {
@@ -273,12 +275,20 @@ __global__ __launch_bounds__((TileSel<T, SMALL, COL>::C::NT), (TileSel<T, SMALL,
         if (!LDSMAT || n > kMaxCand) return;
         const int *list = a.clist + (long)s * kMaxCand;
         const int lbase = tr.vpre[vSlot(k)] - tr.voffG[vSlot(k)];
+        int4 q = {0, 0, 0, 0};
         for (int c = 0; c < n; c++) {
             const int w = list[c];
             const int3 o = lat_offset(vLat[k], a.lat[w]);
             if (o.x < -16 || o.x > 15 || o.y < -16 || o.y > 15 || o.z < -16 || o.z > 15) return;
-            a.tcand[cand_at(c, k)] = (w + lbase) | (a.vmat[w] << 11) | ((o.x & 31) << 16) | ((o.y & 31) << 21) |
-                                     ((o.z & 31) << 26);
+            const int e = (w + lbase) | (a.vmat[w] << 11) | ((o.x & 31) << 16) | ((o.y & 31) << 21) | ((o.z & 31) << 26);
+            if ((c & 3) == 0) q.x = e;
+            else if ((c & 3) == 1) q.y = e;
+            else if ((c & 3) == 2) q.z = e;
+            else q.w = e;
+            if ((c & 3) == 3 || c == n - 1) {
+                tc4[cand4_at(c >> 2, k)] = q;
+                q = int4{0, 0, 0, 0};
+            }
         }
         vCand[k] = n;
     };
@@ -311,7 +321,7 @@ __global__ __launch_bounds__((TileSel<T, SMALL, COL>::C::NT), (TileSel<T, SMALL,
                 if (q < a.pairCap) a.pairRec[q] = int2{gv, w};
             }
             const V3<T> velJ = V3<T>{sMom[0][lw], sMom[1][lw], sMom[2][lw]} * (T)mj.massInv;
-            fw = contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penK, mj.penK, mi.cDampT, mj.cDampT);
+            fw = contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penKinv, mj.penKinv, mi.cDampT, mj.cDampT);
             return true;
         };
         // a partner by voxel id (lists read in place); links: test the current
@@ -322,13 +332,31 @@ __global__ __launch_bounds__((TileSel<T, SMALL, COL>::C::NT), (TileSel<T, SMALL,
             return partner(w + lbase, o, LDSMAT ? sVM[a.vmat[w]] : a.vmats[a.vmat[w]], fw);
         };
         if constexpr (LDSMAT) {
-            if (vCand[k] >= 0) {
-                for (int c = 0; c < vCand[k]; c++) {
-                    const int e = a.tcand[cand_at(c, k)];
-                    const unsigned u = (unsigned)e;
-                    const int3 o = {(int)(u << 11) >> 27, (int)(u << 6) >> 27, (int)(u << 1) >> 27};
-                    V3<T> fw;
-                    if (partner(e & 2047, o, sVM[(e >> 11) & 31], fw)) f = f + fw;
+            const int n = vCand[k];
+            if (n >= 0) {
+                // four candidates per load; the first twelve requested at once,
+                // each later load two batches ahead of its use (three registers
+                // in rotation, so no load result is copied before it arrives)
+                auto batch = [&](const int4 &E, int c0) {
+                    for (int q = 0; q < 4 && c0 + q < n; q++) {
+                        const int e = q == 0 ? E.x : q == 1 ? E.y : q == 2 ? E.z : E.w;
+                        const unsigned u = (unsigned)e;
+                        const int3 o = {(int)(u << 11) >> 27, (int)(u << 6) >> 27, (int)(u << 1) >> 27};
+                        V3<T> fw;
+                        if (partner(e & 2047, o, sVM[(e >> 11) & 31], fw)) f = f + fw;
+                    }
+                };
+                int4 R0 = {0, 0, 0, 0}, R1 = {0, 0, 0, 0}, R2 = {0, 0, 0, 0};
+                if (n > 0) R0 = tc4[cand4_at(0, k)];
+                if (n > 4) R1 = tc4[cand4_at(1, k)];
+                if (n > 8) R2 = tc4[cand4_at(2, k)];
+                for (int c0 = 0; c0 < n; c0 += 12) {
+                    batch(R0, c0);
+                    if (c0 + 12 < n) R0 = tc4[cand4_at((c0 >> 2) + 3, k)];
+                    batch(R1, c0 + 4);
+                    if (c0 + 16 < n) R1 = tc4[cand4_at((c0 >> 2) + 4, k)];
+                    batch(R2, c0 + 8);
+                    if (c0 + 20 < n) R2 = tc4[cand4_at((c0 >> 2) + 5, k)];
                 }
                 return f;
             }
