@@ -18,6 +18,21 @@ double qdist(Q4<float> a, Q4<double> b) {
 }
 }  // namespace
 
+// worst absolute difference of the float sine of a phase in cycles from the
+// double sine of the same argument, over n phases spread over [0, 1) and the
+// float neighbours of the quarter points
+extern "C" double check_sin_cycles(int n) {
+    double worst = 0;
+    const float k = 2 * 3.1415926f;
+    for (int i = 0; i < n + 8; i++) {
+        float c = i < n ? (float)((i + 0.5) / n) : 0.25f * (float)((i - n) / 2);
+        if (i >= n && (i - n) % 2) c = std::nextafterf(c, 2.0f);
+        const double want = std::sin((double)k * (double)c);
+        worst = std::fmax(worst, std::fabs((double)sin_cycles_f32(c) - want));
+    }
+    return worst;
+}
+
 // worst absolute differences over n random inputs: out[0] from_rotvec,
 // out[1] from_angle_to_posx, out[2] to_rotvec (relative to |v|)
 extern "C" void check_primitives(unsigned long long seed, int n, double *out) {

@@ -52,3 +52,10 @@ def test_fp32_rotation_primitives_agree_with_fp64(tmp_path):
     res = (C.c_double * 3)()
     lib.check_primitives(C.c_ulonglong(7), 200000, res)
     assert res[0] < 1e-6 and res[1] < 1e-6 and res[2] < 1e-6, list(res)  # summed over 4 components: ~1 ulp each
+    # the actuation sine of the FP32 path: within the 3.0e-7 argument error of
+    # its period-1 folding (2 pi - 2 * 3.1415926f) plus float roundings of
+    # sin(2 * 3.1415926f * c); a fold to the wrong side or a dropped series term
+    # is off by 1e-2 or more
+    lib.check_sin_cycles.restype = C.c_double
+    worst = lib.check_sin_cycles(1 << 20)
+    assert worst < 4e-7, worst

@@ -504,6 +504,22 @@ template <typename T> VX_HD V3<T> to_axis_original(V3<T> v, int axis) {
 }
 
 // ---- temperature (gpu_update_temperature VX3_VoxelyzeKernel.cu:625-650) -----
+// sin(K c) in float, K = 2 * 3.1415926f (the reference's constant), for a phase
+// c counted in cycles: c is folded into [-1/4, 1/4] with the symmetries of a
+// sine of period 1 (r = c - rint(c), then 1/2 - r or -1/2 - r beyond a quarter;
+// K differs from 2 pi by 3.0e-7, so the folded argument is off by at most that,
+// as the reduction of the cycle count to [0, 1) already is), and sin x
+// on |x| <= K / 4 is its odd Taylor series to the x^13 term (remainder below
+// (pi/2)^15 / 15! < 7e-10).  About a third of the instructions of sinf.
+VX_HD float sin_cycles_f32(float c) {
+    float r = c - rintf(c);
+    if (r > 0.25f) r = 0.5f - r;
+    else if (r < -0.25f) r = -0.5f - r;
+    const float x = (2 * 3.1415926f) * r, x2 = x * x;
+    return x * (1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f +
+               x2 * (1.0f / 362880.0f + x2 * (-1.0f / 39916800.0f + x2 * (1.0f / 6227020800.0f)))))));
+}
+
 // Returns the new temperature of a voxel at time t (the caller skips fixed
 // voxels and voxels whose TurnOnThermalExpansion time has not come).
 // FP32 path: the cycle count t / period + phase is formed in double as
@@ -511,7 +527,7 @@ template <typename T> VX_HD V3<T> to_axis_original(V3<T> v, int axis) {
 VX_HD float actuation_temperature_f32(double amp, double invPeriod, double t, double phase, int enableExpansion) {
     double c = t * invPeriod + phase;
     c -= floor(c);
-    float Tc = (float)amp * sinf((2 * 3.1415926f) * (float)c);
+    float Tc = (float)amp * sin_cycles_f32((float)c);
     if (!enableExpansion && Tc > 0) Tc = 0;
     return Tc;
 }
