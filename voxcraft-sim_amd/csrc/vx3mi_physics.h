@@ -596,8 +596,11 @@ VX_HD float curve_stress(const LinkMat &lm, const CurvePt *c, float strain, floa
 // VX3_Link.cu:80-83).  curves: the stress-strain curve table (read only for
 // non-linear materials).  The strain offset of a non-linear link that backed
 // off its curve, maxStrain - stress(maxStrain) / E (:239-242), is a function
-// of maxStrain alone and is recomputed instead of stored.
-template <typename T>
+// of maxStrain alone and is recomputed instead of stored.  LINK_FRAME: the
+// four end vectors are left in the link's frame (x along the link; the caller
+// applies toAxisOriginal, an exact permutation with sign changes, where the
+// axis is known at compile time).
+template <typename T, bool LINK_FRAME = false>
 VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T> du, Q4<T> qn, Q4<T> qp, float tN,
                        float tP, float prevDt, LinkState<T> &st, LinkOut<T> &out) {
     V3<T> oldPos2 = st.pos2, oldA1v = st.a1v, oldA2v = st.a2v;
@@ -737,10 +740,17 @@ VX_HD void link_forces(const LinkMat &lm, const CurvePt *curves, int axis, V3<T>
     }
     fP = qrotinv(angle2, fP);
     mP = qrotinv(angle2, mP);
-    out.fN = to_axis_original(fN, axis);
-    out.fP = to_axis_original(fP, axis);
-    out.mN = to_axis_original(mN, axis);
-    out.mP = to_axis_original(mP, axis);
+    if constexpr (LINK_FRAME) {
+        out.fN = fN;
+        out.fP = fP;
+        out.mN = mN;
+        out.mP = mP;
+    } else {
+        out.fN = to_axis_original(fN, axis);
+        out.fP = to_axis_original(fP, axis);
+        out.mN = to_axis_original(mN, axis);
+        out.mP = to_axis_original(mP, axis);
+    }
     st.flags = flags;
 }
 

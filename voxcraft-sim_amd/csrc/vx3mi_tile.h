@@ -116,7 +116,7 @@ __global__ __launch_bounds__((TileSel<T, SMALL, COL>::C::NT), (TileSel<T, SMALL,
                   "a tile's threads must cover its capacity");
     static_assert(RMAX <= 128 && VMAX <= 2048, "robot slots take 7 bits, packed candidates 11");
     __shared__ T sPose[8][VMAX];    // position field xyz, qw, qx, qy, qz, temperature
-    __shared__ T sForce[12][LMAX];  // forceNeg, momentNeg, forcePos, momentPos
+    __shared__ T sForce[12][LMAX];  // forceNeg, momentNeg, forcePos, momentPos (in the link frame)
     __shared__ T sMom[COL ? 3 : 1][COL ? VMAX : 1];  // linear momentum entering the step (contacts)
     __shared__ TileRobots<RMAX, COL> tr;
     __shared__ LinkMat sLM[LDSMAT ? TileMatCache<SMALL>::N : 1];
@@ -458,7 +458,7 @@ __global__ __launch_bounds__((TileSel<T, SMALL, COL>::C::NT), (TileSel<T, SMALL,
             const Q4<T> qn = {sPose[3][n], sPose[4][n], sPose[5][n], sPose[6][n]};
             const Q4<T> qp = {sPose[3][p], sPose[4][p], sPose[5][p], sPose[6][p]};
             LinkOut<T> o;
-            link_forces<T>(lm, a.curves, lInfo[k] & 3, du, qn, qp, (float)sPose[7][n], (float)sPose[7][p], tr.prevDt[parity][j],
+            link_forces<T, true>(lm, a.curves, lInfo[k] & 3, du, qn, qp, (float)sPose[7][n], (float)sPose[7][p], tr.prevDt[parity][j],
                            ls[k], o);
             sForce[0][ll] = o.fN.x; sForce[1][ll] = o.fN.y; sForce[2][ll] = o.fN.z;
             sForce[3][ll] = o.mN.x; sForce[4][ll] = o.mN.y; sForce[5][ll] = o.mN.z;
@@ -503,8 +503,9 @@ __global__ __launch_bounds__((TileSel<T, SMALL, COL>::C::NT), (TileSel<T, SMALL,
                 const int l = (vLinks[k][d >> 1] >> (16 * (d & 1))) & 0xffff;
                 if (l == 0xffff) continue;
                 const int o = (d & 1) ? 6 : 0;  // NEG direction: this voxel is the pVPos end
-                fs = fs + V3<T>{sForce[o + 0][l], sForce[o + 1][l], sForce[o + 2][l]};
-                ms = ms + V3<T>{sForce[o + 3][l], sForce[o + 4][l], sForce[o + 5][l]};
+                // link-frame end vectors back to the lattice frame (axis d / 2, known here)
+                fs = fs + to_axis_original(V3<T>{sForce[o + 0][l], sForce[o + 1][l], sForce[o + 2][l]}, d >> 1);
+                ms = ms + to_axis_original(V3<T>{sForce[o + 3][l], sForce[o + 4][l], sForce[o + 5][l]}, d >> 1);
             }
             float temp = (float)sPose[7][lv];
             const int tf = tr.tempFlags[j];
