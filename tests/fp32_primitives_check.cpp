@@ -33,6 +33,23 @@ extern "C" double check_sin_cycles(int n) {
     return worst;
 }
 
+// worst relative difference of the float atan2 of the rotation-vector angle
+// from the double atan2, over n angles spread over (0, pi] on the unit circle
+// (the unit quaternions it sees) and at scales 1e-3 .. 1e3 off it
+extern "C" double check_atan2(int n) {
+    double worst = 0;
+    for (int i = 0; i < n; i++) {
+        const double a = M_PI * (i + 1.0) / n;
+        for (double sc : {1.0, 1e-3, 1e3}) {
+            const float y = (float)(sc * std::sin(a)), x = (float)(sc * std::cos(a));
+            if (y < 0 || (x == 0 && y == 0)) continue;
+            const double want = std::atan2((double)y, (double)x);
+            worst = std::fmax(worst, std::fabs((double)atan2_nonneg_f32(y, x) - want) / want);
+        }
+    }
+    return worst;
+}
+
 // worst absolute differences over n random inputs: out[0] from_rotvec,
 // out[1] from_angle_to_posx, out[2] to_rotvec (relative to |v|)
 extern "C" void check_primitives(unsigned long long seed, int n, double *out) {

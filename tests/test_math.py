@@ -59,3 +59,8 @@ def test_fp32_rotation_primitives_agree_with_fp64(tmp_path):
     lib.check_sin_cycles.restype = C.c_double
     worst = lib.check_sin_cycles(1 << 20)
     assert worst < 4e-7, worst
+    # the rotation-vector angle of the FP32 path: a few float ulps of atan2 over
+    # the half circle (an octant folded the wrong way is off by 1e-1 or more)
+    lib.check_atan2.restype = C.c_double
+    worst = lib.check_atan2(1 << 20)
+    assert worst < 5e-7, worst

@@ -347,6 +347,28 @@ VX_HD float frsqrt(float x) {
     return 1.0f / sqrtf(x);
 #endif
 }
+// atan2(y, x) in float for y >= 0 and (x, y) != (0, 0), in [0, pi]: the ratio
+// of the smaller to the larger of y and |x| (in [0, 1]) is reduced below
+// tan(pi/8) by atan(t) = pi/4 + atan((t - 1)/(t + 1)), formed from the two
+// magnitudes with one reciprocal; atan(u) = u + u^3 P(u^2) with P of degree 4
+// fitted by least squares (relative error weight) on Chebyshev nodes of
+// [0, tan^2(pi/8)], within 6.3e-8 relative in float; then the octant and the
+// sign of x.  The rotation-vector angle of the FP32 path (to_rotvec).
+VX_HD float atan2_nonneg_f32(float y, float x) {
+    const float ax = mabs(x);
+    const bool swap = y > ax;
+    const float num = swap ? ax : y, den = swap ? y : ax;
+    const bool hi = num > 0.41421356f * den;
+    const float u = (hi ? num - den : num) * frcp(hi ? num + den : den);
+    const float z = u * u;
+    const float p = -0.33333313f + z * (0.19998442f + z * (-0.14243065f + z * (0.10590773f + z * -0.06071304f)));
+    float r = u + (u * z) * p;
+    if (hi) r += 0.78539816f;
+    if (swap) r = 1.57079633f - r;
+    if (x < 0) r = 3.14159265f - r;
+    return r;
+}
+
 // a / b: exact in double, through the reciprocal in float
 template <typename T> VX_HD T qdiv(T a, T b) {
     if constexpr (std::is_same<T, float>::value) return a * frcp(b);
@@ -380,7 +402,7 @@ template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q, int *acos_branch = nullptr)
         else {
             if (acos_branch) *acos_branch = 1;
             float s = fsqrt(s2);
-            k = 2.0f * atan2f(s, q.w) * frcp(s);
+            k = 2.0f * atan2_nonneg_f32(s, q.w) * frcp(s);
         }
         return v3<T>(q.x * k, q.y * k, q.z * k);
     }
