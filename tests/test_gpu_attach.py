@@ -169,3 +169,38 @@ def test_autotune_keeps_the_attach_state():
     for key in ("pos", "quat", "linmom", "angmom"):
         np.testing.assert_array_equal(a.state(0)[key], b.state(0)[key])
     np.testing.assert_array_equal(a.attach_state(0)["links"], b.attach_state(0)["links"])
+
+
+def test_attach_overflow_is_reported():
+    """More attach candidates than a surface voxel keeps in one step (8 partners
+    of lower index): a checkerboard of unlinked sticky voxels squeezed to 0.6 of
+    the lattice spacing, up to 9 such partners each.  The robot's overflow flag
+    is set; the links made are distinct, join voxels inside the envelope at the
+    step's start, use each voxel slot once, and are no more than the oracle
+    (which keeps every candidate) makes in that step."""
+    d = 6
+    z, y, x = np.meshgrid(np.arange(d), np.arange(d), np.arange(d), indexing="ij")
+    cells = (((x + y + z) % 2) == 0).astype(np.int8).reshape(-1)
+    prob = dict(dims=(d, d, d), cells=cells, materials=[dict(E=1e6, rho=1000, sticky=1)], enable_attach=1,
+                lattice_dim=L0, floor_enabled=0, grav_acc=0.0, slow_damp_z=0.0, vary_temp=0)
+    ref = O.OracleModel(prob)
+    pos = ref.state()["pos"]
+    pos = pos.mean(axis=0) + 0.6 * (pos - pos.mean(axis=0))
+    ref.set_state(pos=pos)
+    nl0 = ref.nl
+    ref.step(1)
+    b = vx3mi.Batch([vx3mi.Model.from_problem(prob)], vx3mi.FP64)
+    b.set_state(0, pos=pos)
+    b.step(1, mode=vx3mi.MODE_FLAT)
+    st = b.attach_state(0)
+    links = st["links"]
+    log(f"overflow {st['overflow']}, links {len(links)} (oracle {ref.nl - nl0})")
+    assert st["overflow"] == 1
+    assert 0 < len(links) <= ref.nl - nl0
+    pairs = {(int(n), int(p)) for n, p in links[:, :2]}
+    assert len(pairs) == len(links)
+    dist = np.linalg.norm(pos[links[:, 0]] - pos[links[:, 1]], axis=1)
+    assert (dist < 2 * 0.625 * L0).all()
+    slots = [(int(n), 2 * int(a)) for n, _, a, _ in links] + [(int(p), 2 * int(a) + 1) for _, p, a, _ in links]
+    assert len(set(slots)) == len(slots)
+    assert not b.diverged().any()
