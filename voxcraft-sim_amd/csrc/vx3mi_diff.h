@@ -32,6 +32,10 @@ template <typename T> VX_HD V3<T> cross(V3<T> a, V3<T> b) {
 }
 template <typename T> VX_HD Q4<T> qadd(Q4<T> a, Q4<T> b) { Q4<T> r = {a.w + b.w, a.x + b.x, a.y + b.y, a.z + b.z}; return r; }
 template <typename T> VX_HD V3<T> neg(V3<T> a) { return v3<T>(-a.x, -a.y, -a.z); }
+// quotients of T: exact in double, through the hardware reciprocal in float
+// (about 1 ulp; the FP32 differentiable mode is held to the FP64 one by
+// tolerance, its gradients to finite differences at 1e-4)
+template <typename T> VX_HD T ddiv(T a, T b) { return qdiv(a, b); }
 VX_HD float matan(float a) { return atanf(a); }
 VX_HD double matan(double a) { return atan(a); }
 VX_HD float mexp(float a) { return expf(a); }
@@ -85,9 +89,9 @@ VX_HD T d_temperature(const DiffOpts &o, double t, double phase, T *dTdPhase) {
     } else {
         double c = t / o.tempPeriod + phase;
         c -= floor(c);
-        const float a = (2 * 3.1415926f) * (float)c;
-        val = (float)o.tempAmp * sinf(a);
-        der = (float)o.tempAmp * (2 * 3.1415926f) * cosf(a);
+        // sin and cos of 2 * 3.1415926f * c (cos as the sine a quarter cycle on)
+        val = (float)o.tempAmp * sin_cycles_f32((float)c);
+        der = (float)o.tempAmp * (2 * 3.1415926f) * sin_cycles_f32((float)c + 0.25f);
     }
     if (!o.enableExpansion && val > 0) return 0;
     *dTdPhase = der;
@@ -131,10 +135,10 @@ template <typename T> VX_HD V3<T> d_rotvec(Q4<T> q) {
     const T s2 = q.x * q.x + q.y * q.y + q.z * q.z;
     if (q.w <= (T)-1) return v3<T>(0, 0, 0);
     T k;
-    if (s2 < (T)kSLTHRESH_ACOS2SQRT) k = (T)2 * msqrt((T)2 / ((T)1 + q.w));
+    if (s2 < (T)kSLTHRESH_ACOS2SQRT) k = (T)2 * msqrt(ddiv((T)2, (T)1 + q.w));
     else {
         const T s = msqrt(s2);
-        k = (T)2 * matan2(s, q.w) / s;
+        k = ddiv((T)2 * matan2_nonneg(s, q.w), s);
     }
     return v3<T>(q.x * k, q.y * k, q.z * k);
 }
@@ -144,14 +148,14 @@ template <typename T> VX_HD void d_rotvec_vjp(Q4<T> q, V3<T> ob, Q4<T> &qb) {
     const V3<T> v = v3<T>(q.x, q.y, q.z);
     T k, dkdw, dkds_over_s;  // dk/ds / s
     if (s2 < (T)kSLTHRESH_ACOS2SQRT) {
-        k = (T)2 * msqrt((T)2 / ((T)1 + q.w));
-        dkdw = -k / ((T)2 * ((T)1 + q.w));
+        k = (T)2 * msqrt(ddiv((T)2, (T)1 + q.w));
+        dkdw = ddiv(-k, (T)2 * ((T)1 + q.w));
         dkds_over_s = 0;
     } else {
-        const T s = msqrt(s2), at = matan2(s, q.w), r2 = s2 + q.w * q.w;
-        k = (T)2 * at / s;
-        dkdw = (T)-2 / r2;
-        dkds_over_s = (T)2 * (q.w / (r2 * s) - at / s2) / s;
+        const T s = msqrt(s2), at = matan2_nonneg(s, q.w), r2 = s2 + q.w * q.w;
+        k = ddiv((T)2 * at, s);
+        dkdw = ddiv((T)-2, r2);
+        dkds_over_s = ddiv((T)2 * (ddiv(q.w, r2 * s) - ddiv(at, s2)), s);
     }
     const T kb = dot(ob, v);
     qb.x += k * ob.x + kb * dkds_over_s * q.x;
@@ -183,7 +187,7 @@ template <typename T> VX_HD void from_rotvec_vjp(V3<T> in, Q4<T> ob, V3<T> &inb)
 template <typename T> VX_HD Q4<T> d_angle_to_posx(V3<T> rf) {
     Q4<T> q = {1, 0, 0, 0};
     if (rf.x == 0 && rf.y == 0 && rf.z == 0) return q;
-    const T yx = rf.y / rf.x, zx = rf.z / rf.x;
+    const T yx = ddiv(rf.y, rf.x), zx = ddiv(rf.z, rf.x);
     const T sa = (T)kSMALL_ANGLE_RAD;
     if (yx < sa && yx > -sa && zx < sa && zx > -sa) {
         q.y = (T)0.5 * zx;
@@ -192,40 +196,40 @@ template <typename T> VX_HD Q4<T> d_angle_to_posx(V3<T> rf) {
         return q;
     }
     const T rho = msqrt(rf.y * rf.y + rf.z * rf.z);
-    const T theta = matan2(rho, rf.x);
+    const T theta = matan2_nonneg(rho, rf.x);
     if (theta > (T)(kPI - kDISCARD_ANGLE_RAD)) { q.w = 0; q.y = 1; return q; }
     const T h = (T)0.5 * theta, sn = msin(h);
     q.w = mcos(h);
-    q.y = rf.z / rho * sn;
-    q.z = -rf.y / rho * sn;
+    q.y = ddiv(rf.z, rho) * sn;
+    q.z = -ddiv(rf.y, rho) * sn;
     return q;
 }
 template <typename T> VX_HD void d_angle_to_posx_vjp(V3<T> rf, Q4<T> qb, V3<T> &rb) {
     if (rf.x == 0 && rf.y == 0 && rf.z == 0) return;
-    const T yx = rf.y / rf.x, zx = rf.z / rf.x;
+    const T yx = ddiv(rf.y, rf.x), zx = ddiv(rf.z, rf.x);
     const T sa = (T)kSMALL_ANGLE_RAD;
     if (yx < sa && yx > -sa && zx < sa && zx > -sa) {
         const T qy = (T)0.5 * zx, qz = (T)-0.5 * yx;
         const T qyb = qb.y - qb.w * qy, qzb = qb.z - qb.w * qz;
         const T zxb = (T)0.5 * qyb, yxb = (T)-0.5 * qzb;
-        rb.x += -(yxb * yx + zxb * zx) / rf.x;
-        rb.y += yxb / rf.x;
-        rb.z += zxb / rf.x;
+        rb.x += ddiv(-(yxb * yx + zxb * zx), rf.x);
+        rb.y += ddiv(yxb, rf.x);
+        rb.z += ddiv(zxb, rf.x);
         return;
     }
     const T rho2 = rf.y * rf.y + rf.z * rf.z, rho = msqrt(rho2);
-    const T theta = matan2(rho, rf.x);
+    const T theta = matan2_nonneg(rho, rf.x);
     if (theta > (T)(kPI - kDISCARD_ANGLE_RAD)) return;
     const T h = (T)0.5 * theta, sn = msin(h), cs = mcos(h);
-    const T ey = rf.y / rho, ez = rf.z / rho;
+    const T ey = ddiv(rf.y, rho), ez = ddiv(rf.z, rho);
     const T thb = qb.w * (-sn * (T)0.5) + (qb.y * ez - qb.z * ey) * (cs * (T)0.5);
     const T ezb = qb.y * sn, eyb = -qb.z * sn;
     const T r2 = rho2 + rf.x * rf.x;
-    const T rhob = thb * rf.x / r2;
-    rb.x += -thb * rho / r2;
+    const T rhob = ddiv(thb * rf.x, r2);
+    rb.x += ddiv(-thb * rho, r2);
     const T rho3 = rho2 * rho;
-    rb.y += rhob * ey + (eyb * rf.z * rf.z - ezb * rf.y * rf.z) / rho3;
-    rb.z += rhob * ez + (ezb * rf.y * rf.y - eyb * rf.y * rf.z) / rho3;
+    rb.y += rhob * ey + ddiv(eyb * rf.z * rf.z - ezb * rf.y * rf.z, rho3);
+    rb.z += rhob * ez + ddiv(ezb * rf.y * rf.y - eyb * rf.y * rf.z, rho3);
 }
 
 // ---- link -------------------------------------------------------------------------
@@ -264,8 +268,8 @@ VX_HD void d_link_forward(const DLink<T> &c, V3<T> du, Q4<T> qn, Q4<T> qp, T tN,
     e.pos2 = v3<T>(L + rel.x, rel.y, rel.z);
     e.ext = rel.x + e.lmr;
 
-    const float smallTurn = (float)((mabs(e.pos2.z) + mabs(e.pos2.y)) / e.pos2.x);
-    const float extendPerc = (float)mabs(e.ext / e.restLen);
+    const float smallTurn = (float)ddiv(mabs(e.pos2.z) + mabs(e.pos2.y), e.pos2.x);
+    const float extendPerc = (float)mabs(ddiv(e.ext, e.restLen));
     bool small = (flags & LF_SMALL_ANGLE) != 0;
     if (!small && smallTurn < kSA_BOND_BEND_RAD && extendPerc < kSA_BOND_EXT_PERC) {
         small = true;
@@ -286,11 +290,11 @@ VX_HD void d_link_forward(const DLink<T> &c, V3<T> du, Q4<T> qn, Q4<T> qp, T tN,
         e.Bf = qmul(e.G, e.B);
         const T l = len(e.pos2);
         const T num = e.ext * (e.pos2.x + e.restLen) + e.pos2.y * e.pos2.y + e.pos2.z * e.pos2.z;
-        e.P = v3<T>(num / (l + e.restLen), 0, 0);
+        e.P = v3<T>(ddiv(num, l + e.restLen), 0, 0);
         e.A1v = d_rotvec(e.G);
     }
     e.A2v = d_rotvec(e.Bf);
-    e.strain = e.P.x / e.restLen;
+    e.strain = ddiv(e.P.x, e.restLen);
     const T stress = c.E * e.strain;
     const V3<T> P = e.P, a1 = e.A1v, a2 = e.A2v;
     V3<T> fN = v3<T>(stress * c.tArea, c.b1 * P.y - c.b2 * (a1.z + a2.z), c.b1 * P.z + c.b2 * (a1.y + a2.y));
@@ -302,7 +306,7 @@ VX_HD void d_link_forward(const DLink<T> &c, V3<T> du, Q4<T> qn, Q4<T> qp, T tN,
         const V3<T> dP = (P - P0) * (T)0.5, dA1 = (a1 - A10) * (T)0.5, dA2 = (a2 - A20) * (T)0.5;
         const V3<T> pc = v3<T>(c.sqA1 * dP.x, c.sqB1 * dP.y - c.sqB2xFMp * (dA1.z + dA2.z),
                                c.sqB1 * dP.z + c.sqB2xFMp * (dA1.y + dA2.y));
-        const T dmN = c.dmNumN / prevDt, dmP = c.dmNumP / prevDt;
+        const T dmN = ddiv(c.dmNumN, prevDt), dmP = ddiv(c.dmNumP, prevDt);
         fN = fN + pc * dmN;
         fP = fP - pc * dmP;
         const V3<T> tNv = v3<T>(-c.sqA2xIp * (dA2.x - dA1.x), c.sqB2xFMp * dP.z + c.sqB3xIp * ((T)2 * dA1.y + dA2.y),
@@ -369,7 +373,7 @@ VX_HD void d_link_vjp(const DLink<T> &c, const DLinkEval<T> &e, Q4<T> qn, Q4<T> 
     V3<T> Pbar = Pb, a1b = A1b, a2b = A2b;
     if (e.lvv) {
         const V3<T> dP = (e.P - P0) * (T)0.5, dA1 = (e.A1v - A10) * (T)0.5, dA2 = (e.A2v - A20) * (T)0.5;
-        const T dmN = c.dmNumN / prevDt, dmP = c.dmNumP / prevDt;
+        const T dmN = ddiv(c.dmNumN, prevDt), dmP = ddiv(c.dmNumP, prevDt);
         const V3<T> tNb = mN1b * (-(T)0.5 * dmN), tPb = mP1b * (-(T)0.5 * dmP);
         const V3<T> pcb = fN1b * dmN - fP1b * dmP;
         V3<T> dPb = v3<T>(0, 0, 0), dA1b = v3<T>(0, 0, 0), dA2b = v3<T>(0, 0, 0);
@@ -427,8 +431,8 @@ VX_HD void d_link_vjp(const DLink<T> &c, const DLinkEval<T> &e, Q4<T> qn, Q4<T> 
             mPb0.y * (-c.b2 * P.z - c.b3 * (a1.y + (T)2 * a2.y)) + mPb0.z * (c.b2 * P.y - c.b3 * (a1.z + (T)2 * a2.z));
     // stress = E strain, strain = P.x / restLen
     const T strainb = stressb * c.E;
-    Pbar.x += strainb / e.restLen;
-    T restLenb = -strainb * P.x / (e.restLen * e.restLen);
+    Pbar.x += ddiv(strainb, e.restLen);
+    T restLenb = ddiv(-strainb * P.x, e.restLen * e.restLen);
     // rotation vectors
     Q4<T> Bb = {0, 0, 0, 0};
     d_rotvec_vjp(e.Bf, a2b, Bfb);
@@ -444,15 +448,15 @@ VX_HD void d_link_vjp(const DLink<T> &c, const DLinkEval<T> &e, Q4<T> qn, Q4<T> 
         qmul_vjp(e.G, e.B, Bfb, Gb, Bb);
         const T l = len(e.pos2), den = l + e.restLen;
         const T num = e.ext * (e.pos2.x + e.restLen) + e.pos2.y * e.pos2.y + e.pos2.z * e.pos2.z;
-        const T numb = Pbar.x / den;
-        const T denb = -Pbar.x * num / (den * den);
+        const T numb = ddiv(Pbar.x, den);
+        const T denb = ddiv(-Pbar.x * num, den * den);
         const T lb = denb;
         restLenb += denb + numb * e.ext;
         extb += numb * (e.pos2.x + e.restLen);
         pos2b.x += numb * e.ext;
         pos2b.y += (T)2 * numb * e.pos2.y;
         pos2b.z += (T)2 * numb * e.pos2.z;
-        if (l > 0) pos2b = pos2b + e.pos2 * (lb / l);
+        if (l > 0) pos2b = pos2b + e.pos2 * ddiv(lb, l);
         d_angle_to_posx_vjp(e.pos2, Gb, pos2b);
     }
     // ext = rel.x + lmr; pos2 = (L + rel.x, rel.y, rel.z)
@@ -488,9 +492,9 @@ template <typename T> struct DVoxState {
 
 template <typename T> VX_HD T softplus(T x) { return x > 0 ? x + mlog1p(mexp(-x)) : mlog1p(mexp(x)); }
 template <typename T> VX_HD T sigmoid(T x) {
-    if (x >= 0) return (T)1 / ((T)1 + mexp(-x));
+    if (x >= 0) return ddiv((T)1, (T)1 + mexp(-x));
     const T e = mexp(x);
-    return e / ((T)1 + e);
+    return ddiv(e, (T)1 + e);
 }
 
 // Forward of one voxel (temp = temperature of this step, F/M the link sums in
@@ -505,11 +509,11 @@ VX_HD void d_voxel_forward(const DVox<T> &c, const DiffOpts &o, T dt, T temp, V3
     tf.z += c.grav;
     if (o.floorEnabled) {
         const T pen = (c.L * (temp * c.alpha) * (T)0.5 - (T)c.iz * c.L) - s.u.z;
-        const T x = pen / (T)o.floorW;
+        const T x = ddiv(pen, (T)o.floorW);
         const T N = (T)(o.floorK * o.floorW) * softplus(x);
         tf.z += N - c.cDampT * vel.z * sigmoid(x);
         const T r = msqrt(vel.x * vel.x + vel.y * vel.y + (T)(o.fricV * o.fricV));
-        const T g = c.muK * N / r;
+        const T g = ddiv(c.muK * N, r);
         tf.x -= g * vel.x;
         tf.y -= g * vel.y;
     }
@@ -518,7 +522,7 @@ VX_HD void d_voxel_forward(const DVox<T> &c, const DiffOpts &o, T dt, T temp, V3
     V3<T> tm = qrot(s.q, M) - s.h * (c.IInv * c.gDampR);
     out.h = s.h + tm * dt;
     const Q4<T> q1 = qmul(from_rotvec(out.h * (dt * c.IInv)), s.q);
-    const T n = (T)1 / msqrt(q1.w * q1.w + q1.x * q1.x + q1.y * q1.y + q1.z * q1.z);
+    const T n = ddiv((T)1, msqrt(q1.w * q1.w + q1.x * q1.x + q1.y * q1.y + q1.z * q1.z));
     out.q = Q4<T>{q1.w * n, q1.x * n, q1.y * n, q1.z * n};
 }
 
@@ -547,12 +551,12 @@ VX_HD void d_voxel_vjp(const DVox<T> &c, const DiffOpts &o, T dt, T temp, V3<T> 
     T pen = 0, x = 0, N = 0, sg = 0, r0 = 1, g = 0;
     if (o.floorEnabled) {
         pen = (c.L * (temp * c.alpha) * (T)0.5 - (T)c.iz * c.L) - s.u.z;
-        x = pen / (T)o.floorW;
+        x = ddiv(pen, (T)o.floorW);
         N = (T)(o.floorK * o.floorW) * softplus(x);
         sg = sigmoid(x);
         tf.z += N - c.cDampT * vel.z * sg;
         r0 = msqrt(vel.x * vel.x + vel.y * vel.y + (T)(o.fricV * o.fricV));
-        g = c.muK * N / r0;
+        g = ddiv(c.muK * N, r0);
         tf.x -= g * vel.x;
         tf.y -= g * vel.y;
     }
@@ -562,7 +566,7 @@ VX_HD void d_voxel_vjp(const DVox<T> &c, const DiffOpts &o, T dt, T temp, V3<T> 
     const V3<T> rv = h1 * (dt * c.IInv);
     const Q4<T> dq = from_rotvec(rv);
     const Q4<T> q1 = qmul(dq, s.q);
-    const T nn = msqrt(q1.w * q1.w + q1.x * q1.x + q1.y * q1.y + q1.z * q1.z), ni = (T)1 / nn;
+    const T nn = msqrt(q1.w * q1.w + q1.x * q1.x + q1.y * q1.y + q1.z * q1.z), ni = ddiv((T)1, nn);
     const Q4<T> qh = {q1.w * ni, q1.x * ni, q1.y * ni, q1.z * ni};
 
     // reverse: normalisation
@@ -591,17 +595,17 @@ VX_HD void d_voxel_vjp(const DVox<T> &c, const DiffOpts &o, T dt, T temp, V3<T> 
         velb.x -= g * tfb.x;
         velb.y -= g * tfb.y;
         const T gb = -(vel.x * tfb.x + vel.y * tfb.y);
-        T Nb = gb * c.muK / r0;
-        const T rb = -gb * g / r0;
-        velb.x += rb * vel.x / r0;
-        velb.y += rb * vel.y / r0;
+        T Nb = ddiv(gb * c.muK, r0);
+        const T rb = ddiv(-gb * g, r0);
+        velb.x += ddiv(rb * vel.x, r0);
+        velb.y += ddiv(rb * vel.y, r0);
         // normal force and floor damping
         Nb += tfb.z;
         velb.z -= c.cDampT * sg * tfb.z;
         const T sgb = -c.cDampT * vel.z * tfb.z;
         cDampTb -= vel.z * sg * tfb.z;
         const T xb = Nb * (T)(o.floorK * o.floorW) * sg + sgb * sg * ((T)1 - sg);
-        const T penb = xb / (T)o.floorW;
+        const T penb = ddiv(xb, (T)o.floorW);
         ub.z -= penb;
         tempb += penb * c.L * c.alpha * (T)0.5;
     }

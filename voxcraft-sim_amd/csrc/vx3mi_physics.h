@@ -369,6 +369,10 @@ VX_HD float atan2_nonneg_f32(float y, float x) {
     return r;
 }
 
+// atan2(a, b) for a >= 0: the function above in float, libm's in double
+VX_HD float matan2_nonneg(float a, float b) { return atan2_nonneg_f32(a, b); }
+VX_HD double matan2_nonneg(double a, double b) { return atan2(a, b); }
+
 // a / b: exact in double, through the reciprocal in float
 template <typename T> VX_HD T qdiv(T a, T b) {
     if constexpr (std::is_same<T, float>::value) return a * frcp(b);
