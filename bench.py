@@ -276,7 +276,7 @@ def adjoint_cost(problems, nsteps: int, checkpoint_every: int, budget: float, pr
             "adjoint_over_forward": info["backward_ms"] / info["forward_ms"],
             "recompute_ms": info["recompute_ms"], "reverse_ms": info["reverse_ms"],
             "checkpoint_every": info["checkpoint_every"], "checkpoint_bytes": info["checkpoint_bytes"],
-            "reverse_threads": reverse_threads or 512,
+            "reverse_threads": reverse_threads or 512,  # -1: the flat sweep (one item per thread, all robots)
             "tape_bytes": info["tape_bytes"], "hbm_budget_bytes": budget,
             "forward_voxel_steps_per_s": off[-1] * nsteps / (info["forward_ms"] * 1e-3)}
 
@@ -476,20 +476,31 @@ def adjoint_report(args) -> dict:
     """The adjoint's cost on the 8x8x8 differentiable robot (2000 steps) and on
     256 batched robots (5000 steps), against the differentiable forward and
     against the product timestep on the same robots."""
-    # the reverse sweep's workgroup size (512 or 1024 threads per robot) is
-    # chosen by timing both on each workload
+    # the reverse sweep (512 or 1024 threads per robot, or flat: every robot's
+    # reverse step at once, one item per thread) is chosen by timing each on
+    # each workload
     best = {}
     for name, probs, steps, K in (("c3", [configs.config3()], 2000, 100), ("c4", configs.config4(), 5000, 0)):
-        for rt in (512, 1024):
+        for rt in (512, 1024, -1):
             c = adjoint_cost(probs, steps, K, args.adjoint_budget, reverse_threads=rt)
             log(f"bench: adjoint {name} reverse threads {rt}: forward {c['forward_ms']:.1f} ms, backward "
                 f"{c['backward_ms']:.1f} ms, ratio {c['adjoint_over_forward']:.2f}")
             if name not in best or c["backward_ms"] < best[name]["backward_ms"]:
                 best[name] = c
     c3, c4 = best["c3"], best["c4"]
+    # the 256 robots again with a budget sized for the MI355X's 288 GB: the whole
+    # run's tape (164 GB) stays in HBM, so the backward is the reverse sweep alone
+    try:
+        c4w = adjoint_cost(configs.config4(), 5000, 0, args.adjoint_tape_budget, reverse_threads=-1)
+        log(f"bench: adjoint c4 whole tape ({c4w['tape_bytes'] / 1e9:.0f} GB, budget "
+            f"{args.adjoint_tape_budget / 1e9:.0f} GB): forward {c4w['forward_ms']:.1f} ms, backward "
+            f"{c4w['backward_ms']:.1f} ms, ratio {c4w['adjoint_over_forward']:.2f}")
+    except vx3mi.VX3MIError as e:  # not enough free device memory for the whole tape
+        log(f"bench: adjoint c4 whole tape skipped: {e}")
+        c4w = None
     # the same robots through the product timestep (autotuned, FP32, rigid floor):
     # the adjoint's cost per voxel-step against the fastest forward of the library
-    for c, probs, steps in ((c3, [configs.config3()], 2000), (c4, configs.config4(), 5000)):
+    for c, probs, steps in ((c3, [configs.config3()], 2000), (c4, configs.config4(), 5000)):  # (c4w: below)
         pb = vx3mi.Batch([vx3mi.Model.from_problem(p) for p in probs], vx3mi.FP32)
         pt = pb.autotune(50)
         pb.step(min(steps, 1000))
@@ -509,7 +520,10 @@ def adjoint_report(args) -> dict:
     for name, c in (("8x8x8", c3), ("256 robots", c4)):
         log(f"bench: adjoint {name}: product forward {c['product_forward_ms']:.1f} ms (mode "
             f"{c['product_forward_mode']}), backward / product forward {c['adjoint_over_product_forward']:.2f}")
-    return {"single_8x8x8": c3, "batched_256": c4,
+    if c4w is not None:
+        c4w["product_forward_ms"] = c4["product_forward_ms"]
+        c4w["adjoint_over_product_forward"] = c4w["backward_ms"] / c4["product_forward_ms"]
+    return {"single_8x8x8": c3, "batched_256": c4, "batched_256_whole_tape": c4w,
             "note": "backward = recompute from checkpoints + reverse sweep; FP32; softplus floor; collisions off; "
                     "adjoint_over_forward: against the differentiable forward (checkpoints, softplus floor, "
                     "per-voxel moduli); adjoint_over_product_forward: against the product timestep on the "
@@ -542,6 +556,7 @@ def main() -> int:
     ap.add_argument("--child-tile", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--child-threads", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--adjoint-budget", type=float, default=16.0 * 2**30)
+    ap.add_argument("--adjoint-tape-budget", type=float, default=192e9)
     ap.add_argument("--workload", choices=["batched", "config1", "adjoint"], default="batched",
                     help="config1: only the 64x64x16 robot with collisions (flat mode), for profiling the "
                          "collision kernels; adjoint: only the adjoint cost lines; each prints its own line")

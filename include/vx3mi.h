@@ -382,7 +382,9 @@ typedef struct {
     double e_max;            /* modulus bound for the default timestep (Pa) */
     int checkpoint_every;    /* K; <= 0: chosen under hbm_budget */
     double hbm_budget;       /* device bytes for checkpoints + tape when K is chosen */
-    int reverse_threads;     /* threads per robot of the reverse sweep: FP32 512 or 1024, FP64 256; 0 = 512 / 256 */
+    int reverse_threads;     /* threads per robot of the reverse sweep: FP32 512 or 1024, FP64 256; 0 = 512 / 256;
+                                -1 = flat: each reverse step of every robot at once, one voxel or link per
+                                thread over the device (batches whose robots differ much in size) */
 } vx3mi_diff_opts;
 /* defaults: k 1e5, w 1e-4, friction speed 1e-3, dt from e_max 1e7, K chosen, budget 16 GiB,
  * reverse threads 0 */

@@ -19,17 +19,22 @@ double qdist(Q4<float> a, Q4<double> b) {
 }  // namespace
 
 // worst absolute difference of the float sine of a phase in cycles from the
-// double sine of the same argument, over n phases spread over [0, 1) and the
-// float neighbours of the quarter points
-extern "C" double check_sin_cycles(int n) {
-    double worst = 0;
+// double sine of the same float argument 2 * 3.1415926f * c (what sinf of the
+// float path's argument approximates), over n phases spread over [0, 1.25) and
+// the float neighbours of the quarter points; out[1]: the mean signed
+// difference (a bias of the reduction would show here)
+extern "C" double check_sin_cycles(int n, double *mean) {
+    double worst = 0, sum = 0;
     const float k = 2 * 3.1415926f;
-    for (int i = 0; i < n + 8; i++) {
-        float c = i < n ? (float)((i + 0.5) / n) : 0.25f * (float)((i - n) / 2);
+    for (int i = 0; i < n + 10; i++) {
+        float c = i < n ? (float)(1.25 * (i + 0.5) / n) : 0.25f * (float)((i - n) / 2);
         if (i >= n && (i - n) % 2) c = std::nextafterf(c, 2.0f);
-        const double want = std::sin((double)k * (double)c);
-        worst = std::fmax(worst, std::fabs((double)sin_cycles_f32(c) - want));
+        const double want = std::sin((double)(k * c));
+        const double e = (double)sin_cycles_f32(c) - want;
+        worst = std::fmax(worst, std::fabs(e));
+        sum += e;
     }
+    *mean = sum / (n + 10);
     return worst;
 }
 

@@ -145,17 +145,40 @@ def test_batched_checkpointed_equals_whole_tape():
 
 
 def test_reverse_sweep_width_does_not_change_gradients():
-    """512 and 1024 threads per robot in the reverse sweep give the same bits
-    (each work item is one thread's; the gather order is fixed)."""
+    """512 and 1024 threads per robot in the reverse sweep, and the flat sweep
+    (every robot's reverse step at once, one item per thread, -1), give the
+    same bits (each work item is one thread's; the gather order is fixed)."""
     prob = configs.config3(dims=(5, 4, 4))
     m = vx3mi.Model.from_problem(prob)
     out = []
-    for rt in (512, 1024):
+    for rt in (512, 1024, -1):
         d = vx3mi.Diff(m, vx3mi.FP32, dt=float(FIX["dt"]), checkpoint_every=17, reverse_threads=rt)
         d.set_params(np.asarray(prob["phase"]), np.asarray(prob["voxel_E"], np.float64))
         d.forward(150)
         out.append(d.backward(grad_pos=np.ones((m.nv, 3))))
+    for o in out[1:]:
+        assert np.array_equal(out[0][0], o[0]) and np.array_equal(out[0][1], o[1])
+
+
+@pytest.mark.parametrize("precision", [vx3mi.FP32, vx3mi.FP64])
+def test_flat_reverse_sweep_equals_per_robot_sweep(precision):
+    """A batch of robots of different sizes (the config-4 draw): the flat reverse
+    sweep gives the per-robot sweep's gradients bit for bit, with recompute from
+    checkpoints (K = 13 over 60 steps)."""
+    probs = configs.config4(n=6, seed=3)
+    models = [vx3mi.Model.from_problem(p) for p in probs]
+    phase = np.concatenate([np.asarray(p["phase"], np.float64) for p in probs])
+    E = np.concatenate([np.asarray(p["voxel_E"], np.float64) for p in probs])
+    out = []
+    for rt in (0, -1):
+        d = vx3mi.Diff(models, precision, checkpoint_every=13, reverse_threads=rt)
+        d.set_params(phase, E)
+        d.forward(60)
+        g = np.random.default_rng(4).normal(size=(d.nv, 3))
+        out.append(d.backward(grad_pos=g))
+        d.close()
     assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][1], out[1][1])
+    assert np.abs(out[0][0]).max() > 0
 
 
 def test_torch_function_gradients():

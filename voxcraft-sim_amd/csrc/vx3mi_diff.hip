@@ -30,8 +30,8 @@ int vx3mi_diff_create(vx3mi_model *const *models, int n, int precision, const vx
     vx3mi_diff_default_opts(&o);
     if (opts) o = *opts;
     const int rt = o.reverse_threads;
-    if (rt != 0 && !(precision == VX3MI_FP32 ? (rt == 512 || rt == 1024) : rt == 256)) {
-        set_error("reverse_threads must be 0, or 512 / 1024 (FP32), 256 (FP64)");
+    if (rt != 0 && rt != -1 && !(precision == VX3MI_FP32 ? (rt == 512 || rt == 1024) : rt == 256)) {
+        set_error("reverse_threads must be 0, -1 (flat), or 512 / 1024 (FP32), 256 (FP64)");
         return VX3MI_E_ARG;
     }
     if (!(o.floor_stiffness > 0) || !(o.floor_width > 0) || !(o.friction_speed > 0) || !(o.e_max > 0)) {

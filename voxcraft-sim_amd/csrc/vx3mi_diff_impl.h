@@ -96,6 +96,26 @@ __global__ __launch_bounds__(NT) void k_diff_reverse(DD<T> d, const RobotParams 
     }
 }
 
+// the same reverse step n over every robot at once, one item per thread, two
+// launches in stream order instead of the workgroup barriers: the voxel phase
+// (gather of step n + 1, voxel adjoint of step n) ...
+template <typename T>
+__global__ __launch_bounds__(256) void k_diff_rev_voxels(DD<T> d, int n, int segStart, int gather, const T4<T> *tapeV) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= d.V) return;
+    if (gather) diff_gather_adjoint(d, v, d.times[n + 1]);
+    diff_voxel_rev(d, v, tapeV + ((size_t)(n - segStart) * d.V + v) * 5);
+}
+// ... and the link phase (forward recompute of the link from the tape, its adjoint)
+template <typename T>
+__global__ __launch_bounds__(256) void k_diff_rev_links(DD<T> d, int n, int segStart, const T4<T> *tapeV,
+                                                         const T4<T> *tapeL) {
+    const int l = blockIdx.x * blockDim.x + threadIdx.x;
+    if (l >= d.L) return;
+    const size_t ts = (size_t)(n - segStart);
+    diff_link_rev(d, l, n > 0 ? d.dt : (T)0, tapeV + ts * d.V * 5, tapeL + (ts * d.L + l) * 3);
+}
+
 template <typename T> __global__ __launch_bounds__(256) void k_diff_finish(DD<T> d, T *gE) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v < d.V) gE[v] = diff_finish_voxel(d, v);
@@ -425,23 +445,28 @@ template <typename T> struct DiffBatch : DiffBase {
         // reverse sweep of segment s + 2, the last reader of that tape, is done
         T4<T> *bufV[2] = {tapeV, tapeV2}, *bufL[2] = {tapeL, tapeL2};
         const bool recompute = !(tapeValid && nseg == 1);
-        const bool wide = optsIn.reverse_threads == DiffThreads<T>::REV_WIDE && DiffThreads<T>::REV_WIDE != DiffThreads<T>::REV;
-        HIP_TRY(hipStreamWaitEvent(rstream, ev0, 0));
+        // flat: every robot's reverse step at once over the whole device (a robot
+        // larger than the others no longer sets the pace of the sweep)
+        const bool flat = optsIn.reverse_threads < 0;
+        const bool wide = flat || (optsIn.reverse_threads == DiffThreads<T>::REV_WIDE &&
+                                   DiffThreads<T>::REV_WIDE != DiffThreads<T>::REV);
+        hipStream_t rs = rstream;
+        HIP_TRY(hipStreamWaitEvent(rs, ev0, 0));
         auto recompute_seg = [&](int seg) -> int {
             const int s0 = seg * K, len = std::min(K, steps - s0);
-            if (seg + 2 < nseg) HIP_TRY(hipStreamWaitEvent(rstream, segEv[4 * (seg + 2) + 3], 0));
-            HIP_TRY(hipEventRecord(segEv[4 * seg], rstream));
-            k_diff_restore<T><<<std::max(nb, 1), 256, 0, rstream>>>(d, ckV, ckL, seg);
+            if (seg + 2 < nseg) HIP_TRY(hipStreamWaitEvent(rs, segEv[4 * (seg + 2) + 3], 0));
+            HIP_TRY(hipEventRecord(segEv[4 * seg], rs));
+            k_diff_restore<T><<<std::max(nb, 1), 256, 0, rs>>>(d, ckV, ckL, seg);
             for (int c = 0; c < len; c += chunk) {
                 const int k = std::min(chunk, len - c);
                 if (wide)  // a wide reverse fills the compute units: recompute at full width too
-                    k_diff_forward<T, DiffThreads<T>::FWD><<<R, DiffThreads<T>::FWD, 0, rstream>>>(
+                    k_diff_forward<T, DiffThreads<T>::FWD><<<R, DiffThreads<T>::FWD, 0, rs>>>(
                         d, d_rp, s0 + c, k, K, nullptr, nullptr, bufV[seg & 1], bufL[seg & 1], s0);
                 else
-                    k_diff_forward<T, DiffThreads<T>::RECOMP><<<R, DiffThreads<T>::RECOMP, 0, rstream>>>(
+                    k_diff_forward<T, DiffThreads<T>::RECOMP><<<R, DiffThreads<T>::RECOMP, 0, rs>>>(
                         d, d_rp, s0 + c, k, K, nullptr, nullptr, bufV[seg & 1], bufL[seg & 1], s0);
             }
-            HIP_TRY(hipEventRecord(segEv[4 * seg + 1], rstream));
+            HIP_TRY(hipEventRecord(segEv[4 * seg + 1], rs));
             return VX3MI_OK;
         };
         int rc;
@@ -451,7 +476,13 @@ template <typename T> struct DiffBatch : DiffBase {
             if (recompute) HIP_TRY(hipStreamWaitEvent(stream, segEv[4 * seg + 1], 0));
             HIP_TRY(hipEventRecord(segEv[4 * seg + 2], stream));
             const int gatherFirst = seg != nseg - 1;
-            if (wide)
+            if (flat) {
+                for (int n = s0 + len - 1; n >= s0; n--) {
+                    const int gather = n < s0 + len - 1 || gatherFirst;
+                    k_diff_rev_voxels<T><<<(V + 255) / 256, 256, 0, stream>>>(d, n, s0, gather, bufV[seg & 1]);
+                    if (L) k_diff_rev_links<T><<<(L + 255) / 256, 256, 0, stream>>>(d, n, s0, bufV[seg & 1], bufL[seg & 1]);
+                }
+            } else if (wide)
                 k_diff_reverse<T, DiffThreads<T>::REV_WIDE><<<R, DiffThreads<T>::REV_WIDE, 0, stream>>>(
                     d, d_rp, s0, len, gatherFirst, bufV[seg & 1], bufL[seg & 1]);
             else

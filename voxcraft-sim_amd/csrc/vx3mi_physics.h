@@ -353,7 +353,14 @@ VX_HD float frsqrt(float x) {
 // magnitudes with one reciprocal; atan(u) = u + u^3 P(u^2) with P of degree 4
 // fitted by least squares (relative error weight) on Chebyshev nodes of
 // [0, tan^2(pi/8)], within 6.3e-8 relative in float; then the octant and the
-// sign of x.  The rotation-vector angle of the FP32 path (to_rotvec).
+// sign of x, with pi/4, pi/2 and pi each split into a float and the float of
+// its remainder (rounded constants alone bias the angle by +0.3 ulp on
+// average).  Within 1.9e-7 relative (about 2.5 ulp): the rotation-vector angle
+// of the FP32 differentiable mode, whose gradients are held to finite
+// differences.  The product timestep keeps atan2f there: with this function its
+// FP32 run of config 1 shows orientation excursions above 1e-5 that no branch
+// of the reference's piecewise formulas explains (a correctly rounded atan2
+// shows none), so the product path gives up the 3.5 % it saved.
 VX_HD float atan2_nonneg_f32(float y, float x) {
     const float ax = mabs(x);
     const bool swap = y > ax;
@@ -363,9 +370,9 @@ VX_HD float atan2_nonneg_f32(float y, float x) {
     const float z = u * u;
     const float p = -0.33333313f + z * (0.19998442f + z * (-0.14243065f + z * (0.10590773f + z * -0.06071304f)));
     float r = u + (u * z) * p;
-    if (hi) r += 0.78539816f;
-    if (swap) r = 1.57079633f - r;
-    if (x < 0) r = 3.14159265f - r;
+    if (hi) r = 0.785398126f + (r + 3.77489471e-08f);   // pi/4 +
+    if (swap) r = 1.57079625f - (r - 7.54978942e-08f);  // pi/2 -
+    if (x < 0) r = 3.14159250f - (r - 1.50995788e-07f); // pi -
     return r;
 }
 
@@ -406,7 +413,7 @@ template <typename T> VX_HD V3<T> to_rotvec(Q4<T> q, int *acos_branch = nullptr)
         else {
             if (acos_branch) *acos_branch = 1;
             float s = fsqrt(s2);
-            k = 2.0f * atan2_nonneg_f32(s, q.w) * frcp(s);
+            k = 2.0f * atan2f(s, q.w) * frcp(s);
         }
         return v3<T>(q.x * k, q.y * k, q.z * k);
     }
@@ -531,17 +538,19 @@ template <typename T> VX_HD V3<T> to_axis_original(V3<T> v, int axis) {
 
 // ---- temperature (gpu_update_temperature VX3_VoxelyzeKernel.cu:625-650) -----
 // sin(K c) in float, K = 2 * 3.1415926f (the reference's constant), for a phase
-// c counted in cycles: c is folded into [-1/4, 1/4] with the symmetries of a
-// sine of period 1 (r = c - rint(c), then 1/2 - r or -1/2 - r beyond a quarter;
-// K differs from 2 pi by 3.0e-7, so the folded argument is off by at most that,
-// as the reduction of the cycle count to [0, 1) already is), and sin x
-// on |x| <= K / 4 is its odd Taylor series to the x^13 term (remainder below
-// (pi/2)^15 / 15! < 7e-10).  About a third of the instructions of sinf.
+// c in [0, 1.25) counted in cycles: the argument x = K c is formed in float, as
+// the float path always formed it, and brought into [-pi/2, pi/2] by pi - x or
+// x - 2 pi with pi split into a float and the float of its remainder (Cody and
+// Waite's reduction: the subtraction of the float part is exact here), then sin
+// is its odd Taylor series to the x^13 term (remainder below (pi/2)^15 / 15! <
+// 7e-10).  Within about 1 ulp of the sine of the same float argument, with no
+// bias; about a third of the instructions of sinf.
 VX_HD float sin_cycles_f32(float c) {
-    float r = c - rintf(c);
-    if (r > 0.25f) r = 0.5f - r;
-    else if (r < -0.25f) r = -0.5f - r;
-    const float x = (2 * 3.1415926f) * r, x2 = x * x;
+    constexpr float kPiHi = 3.14159274101257324f, kPiLo = -8.74227766e-08f;  // pi = kPiHi + kPiLo
+    float x = (2 * 3.1415926f) * c;
+    if (x > 4.71238898f) x = (x - 2 * kPiHi) - 2 * kPiLo;  // sin(x - 2 pi)
+    else if (x > 1.57079633f) x = (kPiHi - x) + kPiLo;     // sin(pi - x)
+    const float x2 = x * x;
     return x * (1.0f + x2 * (-1.0f / 6.0f + x2 * (1.0f / 120.0f + x2 * (-1.0f / 5040.0f +
                x2 * (1.0f / 362880.0f + x2 * (-1.0f / 39916800.0f + x2 * (1.0f / 6227020800.0f)))))));
 }
