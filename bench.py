@@ -491,6 +491,8 @@ def adjoint_report(args) -> dict:
     # the 256 robots again with a budget sized for the MI355X's 288 GB: the whole
     # run's tape (164 GB) stays in HBM, so the backward is the reverse sweep alone
     try:
+        if args.adjoint_tape_budget <= 0:
+            raise vx3mi.VX3MIError("--adjoint-tape-budget 0")
         c4w = adjoint_cost(configs.config4(), 5000, 0, args.adjoint_tape_budget, reverse_threads=-1)
         log(f"bench: adjoint c4 whole tape ({c4w['tape_bytes'] / 1e9:.0f} GB, budget "
             f"{args.adjoint_tape_budget / 1e9:.0f} GB): forward {c4w['forward_ms']:.1f} ms, backward "
@@ -556,7 +558,8 @@ def main() -> int:
     ap.add_argument("--child-tile", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--child-threads", type=int, default=0, help=argparse.SUPPRESS)
     ap.add_argument("--adjoint-budget", type=float, default=16.0 * 2**30)
-    ap.add_argument("--adjoint-tape-budget", type=float, default=192e9)
+    ap.add_argument("--adjoint-tape-budget", type=float, default=192e9,
+                    help="HBM budget of the whole-tape adjoint line (0: skip it)")
     ap.add_argument("--workload", choices=["batched", "config1", "adjoint"], default="batched",
                     help="config1: only the 64x64x16 robot with collisions (flat mode), for profiling the "
                          "collision kernels; adjoint: only the adjoint cost lines; each prints its own line")
