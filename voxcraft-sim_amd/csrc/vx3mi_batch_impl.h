@@ -583,17 +583,70 @@ template <typename T> __device__ __forceinline__ void col_track(const Dev<T> &a,
 // surface voxels into buckets, takes the prefix sum and scatters (one launch
 // fewer per step than a batch-wide count pass; a listing comes every ~100
 // steps, the launch every step)
+// passes 1 and 3 for one robot known to be listing (the checks of col_count /
+// col_scatter done once), four surface voxels per thread at a time so that
+// their loads are in flight together: one workgroup lists a robot of any size,
+// 40 voxels per thread for a 64x64x16 body.  The same cells and records as the
+// per-voxel passes (the ranks inside a bucket may differ, which the ascending
+// lists do not see).
+template <typename T> __device__ __forceinline__ void col_count_robot(const Dev<T> &a, const RobotParams &rp) {
+    const double h = rp.listR;
+    const int end = rp.soff + rp.scount, nt = blockDim.x;
+    for (int i0 = rp.soff + (int)threadIdx.x; i0 < end; i0 += 4 * nt) {
+        int v[4], pk[4];
+        T4<T> q[4];
+        double L[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) v[k] = i0 + k * nt < end ? a.surf[i0 + k * nt] : -1;
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (v[k] < 0) continue;
+            q[k] = a.pq0[v[k]];
+            pk[k] = a.lat[v[k]];
+            L[k] = a.vmats[a.vmat[v[k]]].nomSize;
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (v[k] < 0) continue;
+            const int s = i0 + k * nt;
+            const V3<double> p = position_d<T>(pk[k], L[k], V3<T>{q[k].x, q[k].y, q[k].z});
+            const int cx = (int)floor(p.x / h), cy = (int)floor(p.y / h), cz = (int)floor(p.z / h);
+            const int key = (int)(cell_hash(cx, cy, cz) & (unsigned)rp.mask);
+            a.scell[s] = int4{cx, cy, cz, key};
+            a.srank[s] = atomicAdd(&a.ccnt[rp.toff + key], 1);
+            a.bpos[v[k]] = q[k];
+        }
+    }
+}
+template <typename T> __device__ __forceinline__ void col_scatter_robot(const Dev<T> &a, const RobotParams &rp) {
+    const int end = rp.soff + rp.scount, nt = blockDim.x;
+    for (int i0 = rp.soff + (int)threadIdx.x; i0 < end; i0 += 4 * nt) {
+        int key[4], rank[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (i0 + k * nt >= end) continue;
+            key[k] = a.scell[i0 + k * nt].w;
+            rank[k] = a.srank[i0 + k * nt];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (i0 + k * nt >= end) continue;
+            a.srec[rp.soff + a.cstart[rp.toff + key[k]] + rank[k]] = i0 + k * nt - rp.soff;
+        }
+    }
+}
+
 template <typename T> __global__ __launch_bounds__(1024) void k_col_scan(Dev<T> a, int r0, int parity, int all) {
     __shared__ int wsum[16];  // wave totals of col_scan
     if (!all && !a.anyStale[parity]) return;
     const int r = r0 + blockIdx.x;
     const RobotParams &rp = a.rp[r];
     if (!a.rt[r].stale[parity] || a.rt[r].diverged || rp.scount == 0) return;
-    for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_count(a, i, parity);
+    col_count_robot(a, rp);
     __syncthreads();
     col_scan(a, r, wsum);
     __syncthreads();
-    for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += blockDim.x) col_scatter(a, i, parity);
+    col_scatter_robot(a, rp);
 }
 template <typename T> __global__ __launch_bounds__(256) void k_col_list(Dev<T> a, int s0, int n, int parity, int all) {
     if (!all && !a.anyStale[parity]) return;
