@@ -472,7 +472,7 @@ template <typename T> __device__ __forceinline__ void col_list(const Dev<T> &a, 
 // consecutive work-items (lane 0..) share a voxel: each evaluates every
 // kContactLanes-th candidate, and the forces are added in list order.  Clears
 // this step's listing flags (the rebuild passes have run).
-constexpr int kContactLanes = 8;
+constexpr int kContactLanes = 4;
 template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity, int lane) {
     const int v = a.surf[s];
     const int r = a.vrob[v];
