@@ -60,7 +60,8 @@ template <typename T> struct Dev {
     int *srank;            // per surface voxel: rank within its bucket (rebuild)
     int *ccnt;             // bucket counts (rebuild; zero between rebuilds)
     int *cstart;           // bucket starts within the robot's records, mask + 2 per robot
-    int *srec;             // robot-relative surface indices in bucket order, CSR by robot
+    int4 *grec;            // a robot's surface voxels in bucket order, CSR by robot: cell at the listing, voxel id
+    T4<T> *gpos;           // with grec: position field at the listing, lattice site in the fourth word
     int *clist;            // per surface voxel: kMaxCand candidate surface indices, ascending
     int *ccount;           // per surface voxel: candidates (> kMaxCand: search the grid)
     int *tcand;            // collision tiles (k_tile): packed candidates, kMaxCand x KV x NT words per tile
@@ -340,6 +341,17 @@ template <typename T> __device__ __forceinline__ bool col_linked_any(const Dev<T
 }
 
 
+// a lattice site carried in the fourth word of a position record: the bits of
+// the packed site in float, its value in double (exact below 2^53)
+template <typename T> __device__ __forceinline__ T lat_word(int lat) {
+    if constexpr (sizeof(T) == 4) return __int_as_float(lat);
+    else return (T)lat;
+}
+template <typename T> __device__ __forceinline__ int word_lat(T w) {
+    if constexpr (sizeof(T) == 4) return __float_as_int(w);
+    else return (int)w;
+}
+
 // pos_v - pos_w: absolute positions in double; in float the lattice offset and
 // the displacement difference are kept apart
 template <typename T> __device__ __forceinline__ V3<T> col_diff(const T4<T> &pv, const T4<T> &pw, int3 o, T L) {
@@ -368,36 +380,69 @@ template <typename T> __device__ __forceinline__ void col_count(const Dev<T> &a,
 // rebuild pass 2, one workgroup of blockDim threads (a multiple of 64, at most
 // 1024) per robot: exclusive prefix sum of the robot's bucket counts into cstart
 // (mask + 2 entries, the last the total), counts zeroed for the next rebuild;
-// wsum holds 16 ints of LDS
-template <typename T> __device__ __forceinline__ void col_scan(const Dev<T> &a, int r, int *wsum) {
+// wsum holds 16 ints of LDS.  WIDE: the buckets go in tiles of four per thread
+// (the table of a robot starts on a multiple of four entries and holds a power
+// of two, at least 64), so that a wave reads and writes consecutive 16-byte
+// words -- a robot of any size; otherwise each thread takes a run of
+// consecutive buckets, fewer registers for tables of a few buckets per thread
+template <bool WIDE, typename T> __device__ __forceinline__ void col_scan(const Dev<T> &a, int r, int *wsum) {
     const RobotParams &rp = a.rp[r];
-    const int n = rp.mask + 1, nt = blockDim.x, tid = threadIdx.x;
-    const int chunk = (n + nt - 1) / nt, b0 = min(n, tid * chunk), b1 = min(n, b0 + chunk);
-    int *cnt = a.ccnt + rp.toff, *start = a.cstart + rp.toff;
-    int sum = 0;
-    for (int b = b0; b < b1; b++) sum += cnt[b];
-    // exclusive prefix of the per-thread sums: an inclusive scan inside each
-    // wave, then the wave totals (at most 16) in order
+    const int nt = blockDim.x, tid = threadIdx.x;
     const int lane = tid & 63, wv = tid >> 6, nw = (nt + 63) >> 6;
-    int incl = sum;
-    for (int o = 1; o < 64; o <<= 1) {
-        const int y = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += y;
-    }
-    if (lane == 63 || tid == nt - 1) wsum[wv] = incl;
-    __syncthreads();
-    if (tid == 0) {
-        int acc = 0;
-        for (int i = 0; i < nw; i++) { const int x = wsum[i]; wsum[i] = acc; acc += x; }
-        start[n] = acc;
-    }
-    __syncthreads();
-    int acc = wsum[wv] + incl - sum;
-    for (int b = b0; b < b1; b++) {
-        const int c = cnt[b];
-        start[b] = acc;
-        acc += c;
-        cnt[b] = 0;
+    // inclusive scan of x over the wave
+    auto wave_scan = [&](int x) {
+        for (int o = 1; o < 64; o <<= 1) {
+            const int y = __shfl_up(x, o, 64);
+            if (lane >= o) x += y;
+        }
+        return x;
+    };
+    if constexpr (WIDE) {
+        const int n4 = (rp.mask + 1) >> 2;
+        int4 *cnt = reinterpret_cast<int4 *>(a.ccnt + rp.toff);
+        int4 *start = reinterpret_cast<int4 *>(a.cstart + rp.toff);
+        int carry = 0;
+        for (int t0 = 0; t0 < n4; t0 += nt) {
+            const int i = t0 + tid;
+            const int4 c = i < n4 ? cnt[i] : int4{0, 0, 0, 0};
+            const int sum = c.x + c.y + c.z + c.w, incl = wave_scan(sum);
+            if (lane == 63 || tid == nt - 1) wsum[wv] = incl;
+            __syncthreads();
+            int e = carry + incl - sum;  // the wave totals before this wave, in order
+            for (int w = 0; w < nw; w++) {
+                const int x = wsum[w];
+                if (w < wv) e += x;
+                carry += x;
+            }
+            if (i < n4) {
+                start[i] = int4{e, e + c.x, e + c.x + c.y, e + c.x + c.y + c.z};
+                cnt[i] = int4{0, 0, 0, 0};
+            }
+            __syncthreads();  // wsum is rewritten by the next tile
+        }
+        if (tid == 0) start[n4].x = carry;
+    } else {
+        const int n = rp.mask + 1;
+        const int chunk = (n + nt - 1) / nt, b0 = min(n, tid * chunk), b1 = min(n, b0 + chunk);
+        int *cnt = a.ccnt + rp.toff, *start = a.cstart + rp.toff;
+        int sum = 0;
+        for (int b = b0; b < b1; b++) sum += cnt[b];
+        const int incl = wave_scan(sum);
+        if (lane == 63 || tid == nt - 1) wsum[wv] = incl;
+        __syncthreads();
+        if (tid == 0) {  // exclusive prefix of the wave totals
+            int acc = 0;
+            for (int i = 0; i < nw; i++) { const int x = wsum[i]; wsum[i] = acc; acc += x; }
+            start[n] = acc;
+        }
+        __syncthreads();
+        int acc = wsum[wv] + incl - sum;
+        for (int b = b0; b < b1; b++) {
+            const int c = cnt[b];
+            start[b] = acc;
+            acc += c;
+            cnt[b] = 0;
+        }
     }
 }
 
@@ -408,23 +453,29 @@ template <typename T> __device__ __forceinline__ void col_scatter(const Dev<T> &
     const RobotTime *rt = a.rt + r;
     if (!rt->stale[parity] || rt->diverged) return;
     const RobotParams &rp = a.rp[r];
-    a.srec[rp.soff + a.cstart[rp.toff + a.scell[s].w] + a.srank[s]] = s - rp.soff;
+    const int4 c = a.scell[s];
+    const int k = rp.soff + a.cstart[rp.toff + c.w] + a.srank[s];
+    const T4<T> q = a.bpos[v];
+    a.grec[k] = int4{c.x, c.y, c.z, v};
+    a.gpos[k] = T4<T>{q.x, q.y, q.z, lat_word<T>(a.lat[v])};
 }
 
-// calls f(sj) for every surface voxel sj of the robot in the 27 grid cells
-// around surface voxel s (the grid of the last listing)
-template <typename T, typename F> __device__ __forceinline__ void col_grid_each(const Dev<T> &a, const RobotParams &rp, int s, F &&f) {
+// calls f(w, pw) for every surface voxel w of the robot in the 27 grid cells
+// around surface voxel s (voxel v), from the grid of the last listing; pw is
+// w's record of that listing (position field, lattice site), loaded with POS
+template <bool POS, typename T, typename F>
+__device__ __forceinline__ void col_grid_each(const Dev<T> &a, const RobotParams &rp, int s, int v, F &&f) {
     const int4 cs = a.scell[s];
     for (int c = 0; c < 27; c++) {
         const int cx = cs.x + c % 3 - 1, cy = cs.y + (c / 3) % 3 - 1, cz = cs.z + c / 9 - 1;
         const int key = (int)(cell_hash(cx, cy, cz) & (unsigned)rp.mask);
         const int b0 = a.cstart[rp.toff + key], b1 = a.cstart[rp.toff + key + 1];
-        for (int k = b0; k < b1; k++) {
-            const int sj = rp.soff + a.srec[rp.soff + k];
-            if (sj == s) continue;
-            const int4 cj = a.scell[sj];
-            if (cj.x != cx || cj.y != cy || cj.z != cz) continue;  // another cell of the same bucket
-            f(sj);
+        for (int k = rp.soff + b0; k < rp.soff + b1; k++) {
+            const int4 g = a.grec[k];
+            T4<T> pw{};
+            if constexpr (POS) pw = a.gpos[k];  // in flight with the cell
+            if (g.w == v || g.x != cx || g.y != cy || g.z != cz) continue;  // itself, another cell of the bucket
+            f(g.w, pw);
         }
     }
 }
@@ -448,12 +499,11 @@ template <typename T> __device__ __forceinline__ void col_list(const Dev<T> &a, 
     const T L = (T)mi.nomSize, R = (T)rp.listR;
     int *list = a.clist + (long)s * kMaxCand;
     int n = 0;
-    col_grid_each(a, rp, s, [&](int sj) {
-        const int w = a.surf[sj];
-        const int3 o = lat_offset(pkv, a.lat[w]);
+    col_grid_each<true>(a, rp, s, v, [&](int w, const T4<T> &pw) {  // pw: w's position now
+        const int3 o = lat_offset(pkv, word_lat<T>(pw.w));
         if (!rp.listLinked && col_linked(a, v, o.x, o.y, o.z)) return;
         if (mi.fixed && a.vmats[a.vmat[w]].fixed) return;
-        const V3<T> d = col_diff<T>(pv, a.pq0[w], o, L);
+        const V3<T> d = col_diff<T>(pv, pw, o, L);
         if (d.x * d.x + d.y * d.y + d.z * d.z > R * R) return;
         if (n < kMaxCand) {  // insertion in ascending order
             int k = n;
@@ -546,12 +596,12 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
         int prev = -1;
         for (;;) {
             int next = INT_MAX;
-            col_grid_each(a, rp, s, [&](int sj) {
-                if (sj > prev && sj < next) next = sj;
+            col_grid_each<false>(a, rp, s, v, [&](int w, const T4<T> &) {
+                if (w > prev && w < next) next = w;
             });
             if (next == INT_MAX) break;
             V3<T> fw;
-            if (partner(a.surf[next], true, fw)) f = f + fw;
+            if (partner(next, true, fw)) f = f + fw;
             prev = next;
         }
     }
@@ -621,17 +671,28 @@ template <typename T> __device__ __forceinline__ void col_count_robot(const Dev<
 template <typename T> __device__ __forceinline__ void col_scatter_robot(const Dev<T> &a, const RobotParams &rp) {
     const int end = rp.soff + rp.scount, nt = blockDim.x;
     for (int i0 = rp.soff + (int)threadIdx.x; i0 < end; i0 += 4 * nt) {
-        int key[4], rank[4];
+        int4 c[4];
+        int rank[4], v[4], lat[4], at[4];
+        T4<T> q[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (i0 + k * nt >= end) continue;
-            key[k] = a.scell[i0 + k * nt].w;
+            c[k] = a.scell[i0 + k * nt];
             rank[k] = a.srank[i0 + k * nt];
+            v[k] = a.surf[i0 + k * nt];
         }
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             if (i0 + k * nt >= end) continue;
-            a.srec[rp.soff + a.cstart[rp.toff + key[k]] + rank[k]] = i0 + k * nt - rp.soff;
+            at[k] = rp.soff + a.cstart[rp.toff + c[k].w] + rank[k];
+            q[k] = a.bpos[v[k]];
+            lat[k] = a.lat[v[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            if (i0 + k * nt >= end) continue;
+            a.grec[at[k]] = int4{c[k].x, c[k].y, c[k].z, v[k]};
+            a.gpos[at[k]] = T4<T>{q[k].x, q[k].y, q[k].z, lat_word<T>(lat[k])};
         }
     }
 }
@@ -644,7 +705,7 @@ template <typename T> __global__ __launch_bounds__(1024) void k_col_scan(Dev<T> 
     if (!a.rt[r].stale[parity] || a.rt[r].diverged || rp.scount == 0) return;
     col_count_robot(a, rp);
     __syncthreads();
-    col_scan(a, r, wsum);
+    col_scan<true>(a, r, wsum);
     __syncthreads();
     col_scatter_robot(a, rp);
 }
@@ -759,11 +820,11 @@ template <typename T> __device__ void attach_find(const Dev<T> &a, int s) {
             int prev = -1;
             for (;;) {
                 int next = INT_MAX;
-                col_grid_each(a, rp, s, [&](int sj) {
-                    if (sj > prev && sj < next) next = sj;
+                col_grid_each<false>(a, rp, s, v, [&](int w, const T4<T> &) {
+                    if (w > prev && w < next) next = w;
                 });
                 if (next == INT_MAX) break;
-                test(a.surf[next]);
+                test(next);
                 prev = next;
             }
         }
@@ -1043,7 +1104,7 @@ template <typename T, int NT> __global__ __launch_bounds__(NT) void k_persistent
             if (a.rt[r].stale[parity] && !a.rt[r].diverged) {
                 for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += NT) col_count(a, i, parity);
                 __syncthreads();
-                col_scan(a, r, wsum);
+                col_scan<true>(a, r, wsum);
                 __syncthreads();
                 for (int i = rp.soff + threadIdx.x; i < rp.soff + rp.scount; i += NT) col_scatter(a, i, parity);
                 __syncthreads();
@@ -1473,7 +1534,7 @@ template <typename T> struct Batch : BatchBase {
                 while (hsz < p.scount) hsz <<= 1;
                 p.toff = (int)buckets;
                 p.mask = hsz - 1;
-                buckets += hsz + 2;
+                buckets += hsz + 4;  // the total after the last bucket; tables on 16 bytes
                 // the largest base size: thermal factor 1 + T * CTE over the
                 // temperatures the robot takes (the import temperature and, with
                 // actuation, -A .. A or -A .. 0 without EnableExpansion)
@@ -1503,12 +1564,12 @@ template <typename T> struct Batch : BatchBase {
         d.pairN = nullptr;
         d.pairCap = 0;
         if (anyCollisions) {
-            int *surfd, *vsurfd, *anyd, *srankd, *ccntd, *cstartd, *srecd, *clistd, *ccountd;
-            int4 *scd;
-            T4<T> *cw, *bpd;
+            int *surfd, *vsurfd, *anyd, *srankd, *ccntd, *cstartd, *clistd, *ccountd;
+            int4 *scd, *grecd;
+            T4<T> *cw, *bpd, *gposd;
             if ((rc = alloc(&surfd, S)) || (rc = alloc(&vsurfd, V)) || (rc = alloc(&anyd, 2)) || (rc = alloc(&scd, S)) ||
                 (rc = alloc(&srankd, S)) || (rc = alloc(&ccntd, buckets)) || (rc = alloc(&cstartd, buckets)) ||
-                (rc = alloc(&srecd, S)) || (rc = alloc(&clistd, (size_t)S * kMaxCand)) || (rc = alloc(&ccountd, S)) ||
+                (rc = alloc(&grecd, S)) || (rc = alloc(&gposd, S)) || (rc = alloc(&clistd, (size_t)S * kMaxCand)) || (rc = alloc(&ccountd, S)) ||
                 (rc = alloc(&cw, V)) || (rc = alloc(&bpd, V)))
                 return rc;
             std::vector<int> vs((size_t)V, -1);
@@ -1524,7 +1585,7 @@ template <typename T> struct Batch : BatchBase {
             HIP_TRY(hipMemset(bpd, 0, sizeof(T4<T>) * V));
             HIP_TRY(hipDeviceSynchronize());
             d.surf = surfd; d.vsurf = vsurfd; d.anyStale = anyd; d.scell = scd; d.srank = srankd;
-            d.ccnt = ccntd; d.cstart = cstartd; d.srec = srecd; d.clist = clistd; d.ccount = ccountd; d.bpos = bpd;
+            d.ccnt = ccntd; d.cstart = cstartd; d.grec = grecd; d.gpos = gposd; d.clist = clistd; d.ccount = ccountd; d.bpos = bpd;
             d.contactW = cw;
             d.contact = cw;
         }

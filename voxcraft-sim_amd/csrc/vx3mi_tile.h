@@ -376,12 +376,12 @@ __global__ __launch_bounds__((TileSel<T, SMALL, COL>::C::NT), (TileSel<T, SMALL,
             int prev = -1;
             for (;;) {
                 int next = INT_MAX;
-                col_grid_each(a, rp, s, [&](int sj) {
-                    if (sj > prev && sj < next) next = sj;
+                col_grid_each<false>(a, rp, s, -1, [&](int w, const T4<T> &) {
+                    if (w > prev && w < next && w != gv) next = w;
                 });
                 if (next == INT_MAX) break;
                 V3<T> fw;
-                if (partner_id(a.surf[next], true, fw)) f = f + fw;
+                if (partner_id(next, true, fw)) f = f + fw;
                 prev = next;
             }
         }
@@ -417,7 +417,7 @@ __global__ __launch_bounds__((TileSel<T, SMALL, COL>::C::NT), (TileSel<T, SMALL,
                 for (int j = 0; j < nr; j++) {
                     if (!listing(j)) continue;  // (the same for every thread)
                     const int r = tr.rid[j];
-                    col_scan(a, r, wsum);
+                    col_scan<false>(a, r, wsum);  // at most VMAX surface voxels: a few buckets per thread
                     __syncthreads();
                     const RobotParams &rp = a.rp[r];
                     for (int i = rp.soff + tid; i < rp.soff + rp.scount; i += NT) col_scatter(a, i, parity);
