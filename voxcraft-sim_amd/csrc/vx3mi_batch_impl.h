@@ -56,6 +56,7 @@ template <typename T> struct Dev {
     const int *vsurf;      // voxel -> its index in surf, -1 when not listed
     T4<T> *bpos;           // per voxel: position field at the last listing (surface voxels)
     int *anyStale;         // per step parity: some robot's lists are stale
+    int *ctick;            // per robot: workgroups done with their part of the count pass (zero between listings)
     int4 *scell;           // per surface voxel: grid cell and bucket at the last listing
     int *srank;            // per surface voxel: rank within its bucket (rebuild)
     int *ccnt;             // bucket counts (rebuild; zero between rebuilds)
@@ -163,6 +164,14 @@ template <typename T> __device__ __forceinline__ void col_track(const Dev<T> &a,
 template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> &a, int v, int parity) {
     const int r = a.vrob[v];
     RobotTime *rt = a.rt + r;
+    {  // this step's listing flags: the lists are built and read by now
+        const RobotParams &rp = a.rp[r];
+        if (v == rp.voff && rp.scount > 0 && rt->stale[parity]) {
+            rt->listings += 1;
+            rt->stale[parity] = 0;
+        }
+        if (v == 0 && a.anyStale) a.anyStale[parity] = 0;
+    }
     if (rt->diverged) return;
     if (rt->divergedNow) {  // doTimeStep returns before the voxels move (:271-280)
         rt->diverged = 1;
@@ -520,19 +529,19 @@ template <typename T> __device__ __forceinline__ void col_list(const Dev<T> &a, 
 // of the reference accumulates this voxel's contact force), with the target
 // hits, the forced signal, and the pair record when one is kept.  kContactLanes
 // consecutive work-items (lane 0..) share a voxel: each evaluates every
-// kContactLanes-th candidate, and the forces are added in list order.  Clears
-// this step's listing flags (the rebuild passes have run).
+// kContactLanes-th candidate, and the forces are added in list order.  LIST: in
+// a step whose listing passes have run (the grid is rebuilt), lane 0 builds the
+// voxel's candidate list first and then takes every candidate itself (the list
+// is its own writes; a listing comes every ~100 steps).  The voxel update of the
+// step clears the listing flags.
 constexpr int kContactLanes = 4;
-template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity, int lane) {
+template <bool LIST, typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity, int lane) {
     const int v = a.surf[s];
     const int r = a.vrob[v];
     RobotTime *rt = a.rt + r;
     const RobotParams &rp = a.rp[r];
-    if (lane == 0 && s == rp.soff && rt->stale[parity]) {
-        rt->listings += 1;
-        rt->stale[parity] = 0;
-    }
-    if (lane == 0 && s == 0 && a.anyStale) a.anyStale[parity] = 0;
+    const bool listed = LIST && rt->stale[parity] && !rt->diverged;  // the same for the lanes of a voxel
+    if (listed && lane == 0) col_list(a, s, parity);
     if (!rp.collisions || rt->diverged) return;
     if (!col_active(a, v)) {
         if (lane == 0) a.contactW[v] = T4<T>{0, 0, 0, 0};
@@ -573,8 +582,17 @@ template <typename T> __device__ __forceinline__ void col_contact(const Dev<T> &
         return 1;
     };
     V3<T> f = {0, 0, 0};
-    const int nc = a.ccount[s];
-    if (nc <= kMaxCand) {
+    const int nc = a.ccount[s];  // when listed, lane 0's own write (the other lanes leave it then)
+    if (nc <= kMaxCand && listed) {
+        if (lane == 0) {
+            const int *list = a.clist + (long)s * kMaxCand;
+            const bool links = rp.listLinked != 0;
+            for (int k = 0; k < nc; k++) {
+                V3<T> fk;
+                if (partner(list[k], links, fk)) f = f + fk;
+            }
+        }
+    } else if (nc <= kMaxCand) {
         const int *list = a.clist + (long)s * kMaxCand;
         const bool links = rp.listLinked != 0;
         for (int k0 = 0; k0 < nc; k0 += kContactLanes) {
@@ -629,20 +647,22 @@ template <typename T> __device__ __forceinline__ void col_track(const Dev<T> &a,
 }
 
 // rebuild kernels for robots whose lists are stale (all = 0: they return at
-// once unless some robot is).  The grid: one workgroup per robot counts its
-// surface voxels into buckets, takes the prefix sum and scatters (one launch
-// fewer per step than a batch-wide count pass; a listing comes every ~100
-// steps, the launch every step)
+// once unless some robot is).  The grid: the workgroups of a robot count its
+// surface voxels into buckets, kCountPerGroup each, and the last of them to
+// finish takes the prefix sum and scatters (one launch fewer per step than a
+// separate count pass; a listing comes every ~100 steps, the launch every step)
 // passes 1 and 3 for one robot known to be listing (the checks of col_count /
 // col_scatter done once), four surface voxels per thread at a time so that
-// their loads are in flight together: one workgroup lists a robot of any size,
-// 40 voxels per thread for a 64x64x16 body.  The same cells and records as the
+// their loads are in flight together: the count of surface voxels [s0, s1), the
+// scatter of the whole robot (one workgroup scatters a robot of any size, 40
+// voxels per thread for a 64x64x16 body).  The same cells and records as the
 // per-voxel passes (the ranks inside a bucket may differ, which the ascending
 // lists do not see).
-template <typename T> __device__ __forceinline__ void col_count_robot(const Dev<T> &a, const RobotParams &rp) {
+constexpr int kCountPerGroup = 4096;  // four surface voxels per thread of a 1024-thread workgroup
+template <typename T> __device__ __forceinline__ void col_count_robot(const Dev<T> &a, const RobotParams &rp, int s0, int s1) {
     const double h = rp.listR;
-    const int end = rp.soff + rp.scount, nt = blockDim.x;
-    for (int i0 = rp.soff + (int)threadIdx.x; i0 < end; i0 += 4 * nt) {
+    const int end = s1, nt = blockDim.x;
+    for (int i0 = s0 + (int)threadIdx.x; i0 < end; i0 += 4 * nt) {
         int v[4], pk[4];
         T4<T> q[4];
         double L[4];
@@ -697,26 +717,37 @@ template <typename T> __device__ __forceinline__ void col_scatter_robot(const De
     }
 }
 
+// blockIdx.x: robot r0 + x; blockIdx.y: its part of the count pass
 template <typename T> __global__ __launch_bounds__(1024) void k_col_scan(Dev<T> a, int r0, int parity, int all) {
     __shared__ int wsum[16];  // wave totals of col_scan
+    __shared__ int lastPart;
     if (!all && !a.anyStale[parity]) return;
     const int r = r0 + blockIdx.x;
     const RobotParams &rp = a.rp[r];
     if (!a.rt[r].stale[parity] || a.rt[r].diverged || rp.scount == 0) return;
-    col_count_robot(a, rp);
+    const int parts = (rp.scount + kCountPerGroup - 1) / kCountPerGroup, part = blockIdx.y;
+    if (part >= parts) return;
+    col_count_robot(a, rp, rp.soff + part * kCountPerGroup, rp.soff + min(rp.scount, (part + 1) * kCountPerGroup));
+    if (parts > 1) {
+        // the workgroup that finishes its part last goes on with the whole
+        // robot: every workgroup releases its cells, ranks and positions before
+        // taking its ticket, the last one acquires them after
+        __threadfence();
+        __syncthreads();
+        if (threadIdx.x == 0) lastPart = atomicAdd(&a.ctick[r], 1) == parts - 1;
+        __syncthreads();
+        if (!lastPart) return;
+        __threadfence();
+        if (threadIdx.x == 0) a.ctick[r] = 0;  // for the next listing (a later launch)
+    }
     __syncthreads();
     col_scan<true>(a, r, wsum);
     __syncthreads();
     col_scatter_robot(a, rp);
 }
-template <typename T> __global__ __launch_bounds__(256) void k_col_list(Dev<T> a, int s0, int n, int parity, int all) {
-    if (!all && !a.anyStale[parity]) return;
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) col_list(a, s0 + i, parity);
-}
 template <typename T> __global__ __launch_bounds__(256) void k_col_contact(Dev<T> a, int s0, int n, int parity) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;  // kContactLanes per surface voxel
-    if (i < n * kContactLanes) col_contact(a, s0 + i / kContactLanes, parity, i % kContactLanes);
+    if (i < n * kContactLanes) col_contact<true>(a, s0 + i / kContactLanes, parity, i % kContactLanes);
 }
 
 // the contact phase and the link updates of one step side by side in one launch
@@ -729,7 +760,7 @@ __global__ __launch_bounds__(256) void k_contact_links(Dev<T> a, int nl, int nbl
         if (l < nl) link_update(a, l, parity);
     } else {
         const int i = (blockIdx.x - nbl) * blockDim.x + threadIdx.x;
-        if (i < ns * kContactLanes) col_contact(a, i / kContactLanes, parity, i % kContactLanes);
+        if (i < ns * kContactLanes) col_contact<true>(a, i / kContactLanes, parity, i % kContactLanes);
     }
 }
 
@@ -1112,7 +1143,7 @@ template <typename T, int NT> __global__ __launch_bounds__(NT) void k_persistent
             }
             __syncthreads();
             for (int i = threadIdx.x; i < rp.scount * kContactLanes; i += NT)
-                col_contact(a, rp.soff + i / kContactLanes, parity, i % kContactLanes);
+                col_contact<false>(a, rp.soff + i / kContactLanes, parity, i % kContactLanes);  // listed above
         }
         for (int l = rp.loff + threadIdx.x; l < rp.loff + rp.lcount; l += NT) link_update(a, l, parity);
         __syncthreads();
@@ -1560,17 +1591,18 @@ template <typename T> struct Batch : BatchBase {
         d.contactW = nullptr;
         d.vsurf = nullptr;
         d.anyStale = nullptr;
+        d.ctick = nullptr;
         d.pairRec = nullptr;
         d.pairN = nullptr;
         d.pairCap = 0;
         if (anyCollisions) {
-            int *surfd, *vsurfd, *anyd, *srankd, *ccntd, *cstartd, *clistd, *ccountd;
+            int *surfd, *vsurfd, *anyd, *srankd, *ccntd, *cstartd, *clistd, *ccountd, *tickd;
             int4 *scd, *grecd;
             T4<T> *cw, *bpd, *gposd;
             if ((rc = alloc(&surfd, S)) || (rc = alloc(&vsurfd, V)) || (rc = alloc(&anyd, 2)) || (rc = alloc(&scd, S)) ||
                 (rc = alloc(&srankd, S)) || (rc = alloc(&ccntd, buckets)) || (rc = alloc(&cstartd, buckets)) ||
                 (rc = alloc(&grecd, S)) || (rc = alloc(&gposd, S)) || (rc = alloc(&clistd, (size_t)S * kMaxCand)) || (rc = alloc(&ccountd, S)) ||
-                (rc = alloc(&cw, V)) || (rc = alloc(&bpd, V)))
+                (rc = alloc(&cw, V)) || (rc = alloc(&bpd, V)) || (rc = alloc(&tickd, R)))
                 return rc;
             std::vector<int> vs((size_t)V, -1);
             for (int i = 0; i < S; i++) vs[surf[i]] = i;
@@ -1583,8 +1615,9 @@ template <typename T> struct Batch : BatchBase {
             HIP_TRY(hipMemset(ccountd, 0, sizeof(int) * S));
             HIP_TRY(hipMemset(cw, 0, sizeof(T4<T>) * V));
             HIP_TRY(hipMemset(bpd, 0, sizeof(T4<T>) * V));
+            HIP_TRY(hipMemset(tickd, 0, sizeof(int) * R));
             HIP_TRY(hipDeviceSynchronize());
-            d.surf = surfd; d.vsurf = vsurfd; d.anyStale = anyd; d.scell = scd; d.srank = srankd;
+            d.surf = surfd; d.vsurf = vsurfd; d.anyStale = anyd; d.ctick = tickd; d.scell = scd; d.srank = srankd;
             d.ccnt = ccntd; d.cstart = cstartd; d.grec = grecd; d.gpos = gposd; d.clist = clistd; d.ccount = ccountd; d.bpos = bpd;
             d.contactW = cw;
             d.contact = cw;
@@ -1784,24 +1817,26 @@ template <typename T> struct Batch : BatchBase {
         return VX3MI_OK;
     }
 
-    // the collision phase of one step for surface voxels [s0, s0 + n) (robots
-    // [r0, r0 + nr)): the listing passes -- count + prefix sum + scatter per
-    // robot, lists per voxel; they return at once unless some robot's lists are
-    // stale (all: check the robots' own flags only) -- then the contact forces
-    void launch_listing(int s0, int n, int r0, int nr, int parity, int all) {
-        const int g = (n + 255) / 256;
-        k_col_scan<T><<<nr, 1024, 0, stream>>>(d, r0, parity, all);
-        k_col_list<T><<<g, 256, 0, stream>>>(d, s0, n, parity, all);
+    // the listing passes of robots [r0, r0 + nr) -- count, prefix sum and
+    // scatter; they return at once unless some robot's lists are stale (all:
+    // check the robots' own flags only) -- the lists themselves are built by the
+    // contact kernels that follow
+    void launch_listing(int r0, int nr, int parity, int all) {
+        int parts = 1;  // count-pass workgroups of the largest robot
+        for (int r = r0; r < r0 + nr; r++) parts = std::max(parts, (hrp[r].scount + kCountPerGroup - 1) / kCountPerGroup);
+        k_col_scan<T><<<dim3(nr, parts), 1024, 0, stream>>>(d, r0, parity, all);
     }
+    // the collision phase of one step for surface voxels [s0, s0 + n) (robots
+    // [r0, r0 + nr)): the listing passes, then the lists and contact forces
     void launch_collisions(int s0, int n, int r0, int nr, int parity, int all) {
-        launch_listing(s0, n, r0, nr, parity, all);
+        launch_listing(r0, nr, parity, all);
         k_col_contact<T><<<(n * kContactLanes + 255) / 256, 256, 0, stream>>>(d, s0, n, parity);
     }
-    static constexpr int kCollisionLaunches = 3;  // plain mode; the flat step runs the contacts beside the links
+    static constexpr int kCollisionLaunches = 2;  // plain mode; the flat step runs the contacts beside the links
 
     void launch_flat_step(int parity) {
         if (anyCollisions) {
-            launch_listing(0, S, 0, R, parity, 0);
+            launch_listing(0, R, parity, 0);
             const int nbl = (LP + 255) / 256, nbc = (S * kContactLanes + 255) / 256;
             k_contact_links<T><<<nbl + nbc, 256, 0, stream>>>(d, LP, nbl, S, parity);
         } else if (LP) {
