@@ -43,13 +43,19 @@ struct RobotParams {
 };
 
 // Candidate lists: every surface voxel lists (ascending) the surface voxels of
-// its robot within listR = the largest contact envelope + kSkin voxel lengths,
-// found through a counting-sort grid whenever a surface voxel of the robot has
-// moved more than kSkin / 2 (less a margin) since the last listing.  A voxel
-// with more than kMaxCand candidates searches the grid itself, and its robot
-// rebuilds the grid every step while it does.
+// its robot within listR = the largest contact envelope + a skin of kSkin voxel
+// lengths, found through a counting-sort grid whenever a surface voxel of the
+// robot has moved more than half the skin (less a margin) since the last
+// listing.  A voxel with more than kMaxCand candidates searches the grid itself,
+// and its robot rebuilds the grid every step while it does.  Robots of at most
+// kSmallRobotVoxels voxels (those a collision tile holds) take the thinner skin
+// kSkinSmall: their listing runs inside the tile and costs little, and every
+// step's contact phase tests fewer candidates.  The partners, and so the
+// results, do not depend on the skin.
 constexpr int kMaxCand = 48;
 constexpr double kSkin = 0.25;
+constexpr double kSkinSmall = 0.15;
+constexpr int kSmallRobotVoxels = 1280;
 
 struct RobotTime {
     double time[2];

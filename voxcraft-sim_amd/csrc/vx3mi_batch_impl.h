@@ -302,7 +302,7 @@ template <typename T> __global__ __launch_bounds__(256) void k_force_field(Dev<T
 // Replaces the all-pairs surface loop (updateAttach VX3_VoxelyzeKernel.cu:448-458,
 // gpu_update_attach :833-843).  Every surface voxel keeps the ascending list of
 // its robot's surface voxels within listR (the largest contact envelope plus a
-// skin of kSkin voxel lengths).  The lists are built from a uniform grid of
+// skin of kSkin or, for small robots, kSkinSmall voxel lengths).  The lists are built from a uniform grid of
 // cell edge listR by a counting sort -- count per bucket, prefix sum, scatter
 // in bucket order -- whenever a surface voxel of the robot has moved more than
 // half the skin since the last listing, and each step the contact phase tests
@@ -1578,8 +1578,9 @@ template <typename T> struct Batch : BatchBase {
                 for (const VoxelMat &vm : m.vmats)
                     fmax = std::max({fmax, 1.0 + (double)vm.alphaCTE * tlo, 1.0 + (double)vm.alphaCTE * thi});
                 const double L0 = m.latticeDim;
-                p.listR = (2 * kCOLLISION_ENVELOPE_RADIUS * L0 * fmax + kSkin * L0) * 1.0001;
-                p.staleD2 = (float)((0.49 * kSkin * L0) * (0.49 * kSkin * L0));
+                const double skin = m.nv() <= kSmallRobotVoxels ? kSkinSmall : kSkin;
+                p.listR = (2 * kCOLLISION_ENVELOPE_RADIUS * L0 * fmax + skin * L0) * 1.0001;
+                p.staleD2 = (float)((0.49 * skin * L0) * (0.49 * skin * L0));
                 p.listLinked = (m.enableDetach || m.secondaryExperiment || p.attach) ? 1 : 0;
                 anyCollisions = true;
             }
