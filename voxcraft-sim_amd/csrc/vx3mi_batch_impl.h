@@ -107,18 +107,20 @@ __device__ __forceinline__ int3 lat_offset(int pkv, int pkw) {
 }
 
 template <typename T> __device__ __forceinline__ void link_update(const Dev<T> &a, int l, int parity) {
+    // the link's own words and its ends' poses are loaded before the checks
+    // (each check returns early, and a load after it would wait for it)
     const int2 meta = a.lmeta[l];
+    const int2 e = a.lends[l];
+    const T4<T> s0 = a.ls0[l], s1 = a.ls1[l], s2 = a.ls2[l];
+    const T4<T> n0 = a.pq0[e.x], n1 = a.pq1[e.x], p0 = a.pq0[e.y], p1 = a.pq1[e.y];
     RobotTime *rt = a.rt + meta.y;
     if (rt->diverged) return;
     if (a.lrem && a.lrem[l]) return;  // removed link (gpu_update_links :570-571)
     const LinkMat &lm = a.lmats[meta.x >> 2];
     if (lm.fixedBoth) return;  // gpu_update_links VX3_VoxelyzeKernel.cu:572-573
     const int axis = meta.x & 3;
-    const int2 e = a.lends[l];
-    const T4<T> n0 = a.pq0[e.x], n1 = a.pq1[e.x], p0 = a.pq0[e.y], p1 = a.pq1[e.y];
     V3<T> du = {p0.x - n0.x, p0.y - n0.y, p0.z - n0.z};
     Q4<T> qn = {n0.w, n1.x, n1.y, n1.z}, qp = {p0.w, p1.x, p1.y, p1.z};
-    const T4<T> s0 = a.ls0[l], s1 = a.ls1[l], s2 = a.ls2[l];
     LinkState<T> st;
     st.pos2 = {s0.x, s0.y, s0.z};
     st.a1v = {s0.w, s1.x, s1.y};
@@ -161,8 +163,43 @@ template <typename T> __device__ __forceinline__ void link_update(const Dev<T> &
 template <typename T> __device__ __forceinline__ void col_track(const Dev<T> &a, int v, const V3<T> &u, RobotTime *rt,
                                                                 const RobotParams &rp, int parity);
 
+// an empty asm that takes x as it stands here: a load issued earlier stays in
+// flight until this point, and the arithmetic on x is not moved above it (the
+// FP32 code contracts multiplies and adds within a basic block, so the blocks
+// the arithmetic sits in decide its rounding)
+template <typename T> __device__ __forceinline__ void pin(T &x) { asm volatile("" : "+v"(x)); }
+template <typename T> __device__ __forceinline__ void pin(T4<T> &x) {
+    pin(x.x);
+    pin(x.y);
+    pin(x.z);
+    pin(x.w);
+}
+
 template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> &a, int v, int parity) {
+    // the loads that need only v (and the link forces, once the link ids are in)
+    // are issued before the robot's checks: each check returns early, and a load
+    // after it would wait for the loads before it.  The values are pinned where
+    // the arithmetic starts, which keeps it in the blocks it had with the loads
+    // there (the same rounding as the tiled kernel's voxel update)
     const int r = a.vrob[v];
+    const int mat = a.vmat[v];
+    int vl[6];
+#pragma unroll
+    for (int d = 0; d < 6; d++) vl[d] = a.vlinks[6 * v + d];
+    T4<T> q0 = a.pq0[v], q1 = a.pq1[v], m0 = a.mo0[v], m1 = a.mo1[v];
+    T4<T> lfA[6], lfB[6];  // force and moment words of each present link at this voxel's end
+#pragma unroll
+    for (int d = 0; d < 6; d++) {
+        const int l = vl[d];
+        if (l < 0) continue;
+        if (d & 1) {  // link in a NEG direction: this voxel is its pVPos end
+            lfA[d] = a.lf1[l];
+            lfB[d] = a.lf2[l];
+        } else {
+            lfA[d] = a.lf0[l];
+            lfB[d] = a.lf1[l];
+        }
+    }
     RobotTime *rt = a.rt + r;
     {  // this step's listing flags: the lists are built and read by now
         const RobotParams &rp = a.rp[r];
@@ -186,27 +223,28 @@ template <typename T> __device__ __forceinline__ void voxel_update(const Dev<T> 
         rt->steps += 1;
     }
     if (a.vrem && a.vrem[v]) return;  // removed voxel (gpu_update_voxels :586-587)
-    const VoxelMat &vm = a.vmats[a.vmat[v]];
+    const VoxelMat &vm = a.vmats[mat];
     if (vm.fixed) return;  // gpu_update_voxels :588-589
 
-    const int *vl = a.vlinks + 6 * v;
     V3<T> fs = {0, 0, 0}, ms = {0, 0, 0};
     bool surface = false;  // fewer than six links (VX3_Voxel::updateSurface, VX3_Voxel.cu:515-524)
 #pragma unroll
     for (int d = 0; d < 6; d++) {
-        const int l = vl[d];
-        if (l < 0) { surface = true; continue; }
-        if (d & 1) {  // link in a NEG direction: this voxel is its pVPos end
-            const T4<T> f1 = a.lf1[l], f2 = a.lf2[l];
-            fs = fs + V3<T>{f1.z, f1.w, f2.x};
-            ms = ms + V3<T>{f2.y, f2.z, f2.w};
+        if (vl[d] < 0) { surface = true; continue; }
+        pin(lfA[d]);
+        pin(lfB[d]);
+        if (d & 1) {
+            fs = fs + V3<T>{lfA[d].z, lfA[d].w, lfB[d].x};
+            ms = ms + V3<T>{lfB[d].y, lfB[d].z, lfB[d].w};
         } else {
-            const T4<T> f0 = a.lf0[l], f1 = a.lf1[l];
-            fs = fs + V3<T>{f0.x, f0.y, f0.z};
-            ms = ms + V3<T>{f0.w, f1.x, f1.y};
+            fs = fs + V3<T>{lfA[d].x, lfA[d].y, lfA[d].z};
+            ms = ms + V3<T>{lfA[d].w, lfB[d].x, lfB[d].y};
         }
     }
-    const T4<T> q0 = a.pq0[v], q1 = a.pq1[v], m0 = a.mo0[v], m1 = a.mo1[v];
+    pin(q0);
+    pin(q1);
+    pin(m0);
+    pin(m1);
     VoxelState<T> s;
     s.u = {q0.x, q0.y, q0.z};
     s.q = {q0.w, q1.x, q1.y, q1.z};
@@ -536,8 +574,17 @@ template <typename T> __device__ __forceinline__ void col_list(const Dev<T> &a, 
 // step clears the listing flags.
 constexpr int kContactLanes = 4;
 template <bool LIST, typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity, int lane) {
+    // the loads that need only s or v are issued before the robot's checks (each
+    // returns early, and a load after it would wait for it): the voxel's pose,
+    // and its list as it stands with this lane's first candidate (a listing step
+    // rebuilds the list below, and lane 0 then reads it again)
     const int v = a.surf[s];
+    const int ncPre = a.ccount[s];
+    const int *list = a.clist + (long)s * kMaxCand;
+    const int wFirst = list[lane];
     const int r = a.vrob[v];
+    const int matI = a.vmat[v], pkv = a.lat[v];
+    const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v], mi0 = a.mo0[v];
     RobotTime *rt = a.rt + r;
     const RobotParams &rp = a.rp[r];
     const bool listed = LIST && rt->stale[parity] && !rt->diverged;  // the same for the lanes of a voxel
@@ -547,9 +594,7 @@ template <bool LIST, typename T> __device__ __forceinline__ void col_contact(con
         if (lane == 0) a.contactW[v] = T4<T>{0, 0, 0, 0};
         return;
     }
-    const VoxelMat &mi = a.vmats[a.vmat[v]];
-    const T4<T> qi0 = a.pq0[v], qi1 = a.pq1[v], mi0 = a.mo0[v];
-    const int pkv = a.lat[v];
+    const VoxelMat &mi = a.vmats[matI];
     const T L = (T)mi.nomSize;
     const T bsaI = base_size_average<T>(mi.nomSize, (float)qi1.w, mi.alphaCTE);
     const V3<T> velI = V3<T>{mi0.x, mi0.y, mi0.z} * (T)mi.massInv;
@@ -582,10 +627,10 @@ template <bool LIST, typename T> __device__ __forceinline__ void col_contact(con
         return 1;
     };
     V3<T> f = {0, 0, 0};
-    const int nc = a.ccount[s];  // when listed, lane 0's own write (the other lanes leave it then)
+    // when listed, lane 0's own write (the other lanes leave it then)
+    const int nc = listed ? a.ccount[s] : ncPre;
     if (nc <= kMaxCand && listed) {
         if (lane == 0) {
-            const int *list = a.clist + (long)s * kMaxCand;
             const bool links = rp.listLinked != 0;
             for (int k = 0; k < nc; k++) {
                 V3<T> fk;
@@ -593,12 +638,15 @@ template <bool LIST, typename T> __device__ __forceinline__ void col_contact(con
             }
         }
     } else if (nc <= kMaxCand) {
-        const int *list = a.clist + (long)s * kMaxCand;
         const bool links = rp.listLinked != 0;
+        int w = wFirst;
         for (int k0 = 0; k0 < nc; k0 += kContactLanes) {
+            const int kn = k0 + kContactLanes + lane;
+            const int wn = kn < nc ? list[kn] : 0;  // the next batch in flight
             V3<T> fk = {0, 0, 0};
             int in = 0;
-            if (k0 + lane < nc) in = partner(list[k0 + lane], links, fk);
+            if (k0 + lane < nc) in = partner(w, links, fk);
+            w = wn;
             for (int j = 0; j < kContactLanes && k0 + j < nc; j++) {  // list order
                 const int inj = __shfl(in, j, kContactLanes);
                 const V3<T> fj = {__shfl(fk.x, j, kContactLanes), __shfl(fk.y, j, kContactLanes),
