@@ -604,14 +604,20 @@ template <bool LIST, typename T> __device__ __forceinline__ void col_contact(con
     // test the current links; lists leave out linked neighbours unless the
     // robot can lose links)
     auto partner = [&](int w, bool links, V3<T> &fw) -> int {
+        // w's words are loaded before the tests that return early (and pinned
+        // where the arithmetic on them starts, which keeps its rounding)
+        const int latW = a.lat[w], matW = a.vmat[w];
+        T4<T> qj0 = a.pq0[w], mj0 = a.mo0[w];
+        T tj = a.pq1[w].w;
         if (!col_active(a, w)) return 0;
-        const int3 o = lat_offset(pkv, a.lat[w]);
+        const int3 o = lat_offset(pkv, latW);
         if (links && (rp.attach ? col_linked_any(a, v, w) : col_linked(a, v, o.x, o.y, o.z))) return 0;
-        const VoxelMat &mj = a.vmats[a.vmat[w]];
+        const VoxelMat &mj = a.vmats[matW];
         if (mi.fixed && mj.fixed) return 0;
-        const T4<T> qj0 = a.pq0[w];
+        pin(qj0);
         const V3<T> diff = col_diff<T>(qi0, qj0, o, L);
-        const T bsaJ = base_size_average<T>(mj.nomSize, (float)a.pq1[w].w, mj.alphaCTE);
+        pin(tj);
+        const T bsaJ = base_size_average<T>(mj.nomSize, (float)tj, mj.alphaCTE);
         if (!in_contact_envelope(diff, bsaI, bsaJ)) return 0;
         if (a.vtarget && a.vtarget[w] != ti) {  // a target touching a non-target (:711-726)
             if (v < w) atomicAdd(&rt->collisionCount, 1);
@@ -621,7 +627,7 @@ template <bool LIST, typename T> __device__ __forceinline__ void col_contact(con
             const int k = atomicAdd(a.pairN, 1);
             if (k < a.pairCap) a.pairRec[k] = int2{v, w};
         }
-        const T4<T> mj0 = a.mo0[w];
+        pin(mj0);
         const V3<T> velJ = V3<T>{mj0.x, mj0.y, mj0.z} * (T)mj.massInv;
         fw = contact_force(diff, bsaI, bsaJ, velI, velJ, mi.penKinv, mj.penKinv, mi.cDampT, mj.cDampT);
         return 1;
