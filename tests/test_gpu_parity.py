@@ -226,6 +226,35 @@ def test_contact_pairs_equal_the_oracle(mode):
     assert total > 1000 and 1 < lst < 80  # rebuilt more than once, not every step
 
 
+@pytest.mark.parametrize("mode", [vx3mi.MODE_FLAT, vx3mi.MODE_PLAIN])
+def test_contact_pairs_large_body_several_count_workgroups(mode):
+    """A 40x40x8 config-1 body (8,946 voxels, 8,222 surface voxels): its count
+    pass runs in three workgroups, the last of which takes the prefix sum and the
+    scatter, and the lists are built inside the contact kernel.  FP64 from a
+    jostled state: the pairs in contact equal the oracle's all-pairs contacts at
+    every step and the states stay bit-identical."""
+    prob = configs.config1(seed=2, dims=(40, 40, 8))
+    ref, pos = _perturbed(prob)
+    model = vx3mi.Model.from_problem(prob)
+    assert model.ns > 2 * 4096
+    batch = vx3mi.Batch([model], vx3mi.FP64)
+    batch.set_state(pos=pos)
+    batch.record_contacts()
+    total = 0
+    for k in range(60):
+        batch.step(1, mode=mode)
+        ref.step(1)
+        g, o = batch.contact_pairs(0), _oracle_pairs(ref)
+        assert np.array_equal(g, o), (k, len(g), len(o))
+        total += len(g)
+    s, r = batch.state(), ref.state()
+    for key in ("pos", "quat", "linmom", "angmom"):
+        assert np.array_equal(s[key], r[key]), key
+    lst = int(batch.listings()[0])
+    log(f"large body: contact pairs over 60 steps {total}, listings {lst}")
+    assert total > 1000 and 1 < lst < 60
+
+
 def _squeezed_block(factor, dims=(6, 6, 5)):
     """A solid block squeezed to `factor` of its lattice spacing about its centre
     (no actuation, no gravity): at 0.45 every surface voxel has more candidates
