@@ -54,6 +54,8 @@ template <> struct TileColCfg<float> {
 template <> struct TileColCfg<double> {
     static constexpr int NT = 512, KL = 2, KV = 2, LMAX = 896, VMAX = 640, RMAX = 64, MINW = 2;
 };
+// the thinner list skin goes to the robots the FP32 collision tile can hold
+static_assert(kSmallRobotVoxels == TileColCfg<float>::VMAX, "small-robot skin threshold");
 template <typename T, int SMALL, int COL> struct TileSel { using C = TileCfg<T, SMALL>; };
 template <typename T, int SMALL> struct TileSel<T, SMALL, 1> { using C = TileColCfg<T>; };
 
