@@ -511,9 +511,10 @@ template <typename T> __device__ __forceinline__ void col_scatter(const Dev<T> &
 // around surface voxel s (voxel v), from the grid of the last listing; pw is
 // w's record of that listing (position field, lattice site), loaded with POS
 template <bool POS, typename T, typename F>
-__device__ __forceinline__ void col_grid_each(const Dev<T> &a, const RobotParams &rp, int s, int v, F &&f) {
+__device__ __forceinline__ void col_grid_each(const Dev<T> &a, const RobotParams &rp, int s, int v, F &&f, int c0 = 0,
+                                              int cstep = 1) {
     const int4 cs = a.scell[s];
-    for (int c = 0; c < 27; c++) {
+    for (int c = c0; c < 27; c += cstep) {  // the cells c0, c0 + cstep, ...
         const int cx = cs.x + c % 3 - 1, cy = cs.y + (c / 3) % 3 - 1, cz = cs.z + c / 9 - 1;
         const int key = (int)(cell_hash(cx, cy, cz) & (unsigned)rp.mask);
         const int b0 = a.cstart[rp.toff + key], b1 = a.cstart[rp.toff + key + 1];
@@ -534,32 +535,41 @@ __device__ __forceinline__ void col_grid_each(const Dev<T> &a, const RobotParams
 // can lose links or voxels (the contact phase tests the current links then);
 // more than kMaxCand: ccount = their number and the voxel searches the grid
 // itself
+// The candidates of surface voxel s (voxel v) in the cells c0, c0 + cstep, ...
+// of the 27, in ascending order into out[0 .. cap); returns their number (also
+// beyond cap).  out is global or LDS memory.
+template <typename T>
+__device__ __forceinline__ int col_list_cells(const Dev<T> &a, const RobotParams &rp, int s, int v, int c0, int cstep,
+                                              int *out, int cap) {
+    const VoxelMat &mi = a.vmats[a.vmat[v]];
+    const T4<T> pv = a.pq0[v];
+    const int pkv = a.lat[v];
+    const T L = (T)mi.nomSize, R = (T)rp.listR;
+    int n = 0;
+    col_grid_each<true>(
+        a, rp, s, v,
+        [&](int w, const T4<T> &pw) {  // pw: w's position now
+            const int3 o = lat_offset(pkv, word_lat<T>(pw.w));
+            if (!rp.listLinked && col_linked(a, v, o.x, o.y, o.z)) return;
+            if (mi.fixed && a.vmats[a.vmat[w]].fixed) return;
+            const V3<T> d = col_diff<T>(pv, pw, o, L);
+            if (d.x * d.x + d.y * d.y + d.z * d.z > R * R) return;
+            if (n < cap) {  // insertion in ascending order
+                int k = n;
+                while (k > 0 && out[k - 1] > w) { out[k] = out[k - 1]; k--; }
+                out[k] = w;
+            }
+            n++;
+        },
+        c0, cstep);
+    return n;
+}
 template <typename T> __device__ __forceinline__ void col_list(const Dev<T> &a, int s, int parity) {
     const int v = a.surf[s];
     const int r = a.vrob[v];
     const RobotTime *rt = a.rt + r;
     if (!rt->stale[parity] || rt->diverged) return;
-    const RobotParams &rp = a.rp[r];
-    const VoxelMat &mi = a.vmats[a.vmat[v]];
-    const T4<T> pv = a.pq0[v];
-    const int pkv = a.lat[v];
-    const T L = (T)mi.nomSize, R = (T)rp.listR;
-    int *list = a.clist + (long)s * kMaxCand;
-    int n = 0;
-    col_grid_each<true>(a, rp, s, v, [&](int w, const T4<T> &pw) {  // pw: w's position now
-        const int3 o = lat_offset(pkv, word_lat<T>(pw.w));
-        if (!rp.listLinked && col_linked(a, v, o.x, o.y, o.z)) return;
-        if (mi.fixed && a.vmats[a.vmat[w]].fixed) return;
-        const V3<T> d = col_diff<T>(pv, pw, o, L);
-        if (d.x * d.x + d.y * d.y + d.z * d.z > R * R) return;
-        if (n < kMaxCand) {  // insertion in ascending order
-            int k = n;
-            while (k > 0 && list[k - 1] > w) { list[k] = list[k - 1]; k--; }
-            list[k] = w;
-        }
-        n++;
-    });
-    a.ccount[s] = n;
+    a.ccount[s] = col_list_cells(a, a.rp[r], s, v, 0, 1, a.clist + (long)s * kMaxCand, kMaxCand);
 }
 
 // The contact force of surface voxel s from the partners inside its envelope,
@@ -573,7 +583,49 @@ template <typename T> __device__ __forceinline__ void col_list(const Dev<T> &a, 
 // is its own writes; a listing comes every ~100 steps).  The voxel update of the
 // step clears the listing flags.
 constexpr int kContactLanes = 4;
-template <bool LIST, typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity, int lane) {
+
+// The list of surface voxel s (voxel v) built by its kContactLanes contact lanes
+// (lane = 0 ..; seg: kMaxCand ints of LDS for the voxel): lane j walks the cells
+// j, j + kContactLanes, ... into its part of seg, in ascending order; then each
+// lane ranks its candidates among all of them and writes them to their places
+// in the list row -- the list and count col_list makes.  When a lane finds more
+// than its part holds, lane 0 makes the whole list alone.  Returns the count to
+// every lane.
+template <typename T>
+__device__ __forceinline__ int col_list_lanes(const Dev<T> &a, const RobotParams &rp, int s, int v, int lane, int *seg) {
+    constexpr int PART = kMaxCand / kContactLanes;
+    int *list = a.clist + (long)s * kMaxCand;
+    const int own = col_list_cells(a, rp, s, v, lane, kContactLanes, seg + lane * PART, PART);
+    int cnt[kContactLanes], total = 0;
+    bool over = false;
+#pragma unroll
+    for (int j = 0; j < kContactLanes; j++) {
+        cnt[j] = __shfl(own, j, kContactLanes);
+        total += cnt[j];
+        over |= cnt[j] > PART;
+    }
+    if (over) {  // the same for the voxel's lanes
+        if (lane == 0) total = col_list_cells(a, rp, s, v, 0, 1, list, kMaxCand);
+        total = __shfl(total, 0, kContactLanes);
+    } else {
+        __threadfence_block();  // the parts of the other lanes
+        for (int k = 0; k < own; k++) {
+            const int e = seg[lane * PART + k];
+            int rank = k;
+#pragma unroll
+            for (int j = 0; j < kContactLanes; j++) {
+                if (j == lane) continue;
+                for (int m = 0; m < cnt[j]; m++) rank += seg[j * PART + m] < e;
+            }
+            list[rank] = e;
+        }
+    }
+    if (lane == 0) a.ccount[s] = total;
+    __threadfence_block();  // the list, for the other lanes
+    return total;
+}
+
+template <bool LIST, typename T> __device__ __forceinline__ void col_contact(const Dev<T> &a, int s, int parity, int lane, int *seg) {
     // the loads that need only s or v are issued before the robot's checks (each
     // returns early, and a load after it would wait for it): the voxel's pose,
     // and its list as it stands with this lane's first candidate (a listing step
@@ -588,7 +640,7 @@ template <bool LIST, typename T> __device__ __forceinline__ void col_contact(con
     RobotTime *rt = a.rt + r;
     const RobotParams &rp = a.rp[r];
     const bool listed = LIST && rt->stale[parity] && !rt->diverged;  // the same for the lanes of a voxel
-    if (listed && lane == 0) col_list(a, s, parity);
+    const int nc = listed ? col_list_lanes(a, rp, s, v, lane, seg) : ncPre;
     if (!rp.collisions || rt->diverged) return;
     if (!col_active(a, v)) {
         if (lane == 0) a.contactW[v] = T4<T>{0, 0, 0, 0};
@@ -633,19 +685,9 @@ template <bool LIST, typename T> __device__ __forceinline__ void col_contact(con
         return 1;
     };
     V3<T> f = {0, 0, 0};
-    // when listed, lane 0's own write (the other lanes leave it then)
-    const int nc = listed ? a.ccount[s] : ncPre;
-    if (nc <= kMaxCand && listed) {
-        if (lane == 0) {
-            const bool links = rp.listLinked != 0;
-            for (int k = 0; k < nc; k++) {
-                V3<T> fk;
-                if (partner(list[k], links, fk)) f = f + fk;
-            }
-        }
-    } else if (nc <= kMaxCand) {
+    if (nc <= kMaxCand) {
         const bool links = rp.listLinked != 0;
-        int w = wFirst;
+        int w = listed ? list[lane] : wFirst;  // a listing step has just written the list
         for (int k0 = 0; k0 < nc; k0 += kContactLanes) {
             const int kn = k0 + kContactLanes + lane;
             const int wn = kn < nc ? list[kn] : 0;  // the next batch in flight
@@ -801,7 +843,9 @@ template <typename T> __global__ __launch_bounds__(1024) void k_col_scan(Dev<T> 
 }
 template <typename T> __global__ __launch_bounds__(256) void k_col_contact(Dev<T> a, int s0, int n, int parity) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;  // kContactLanes per surface voxel
-    if (i < n * kContactLanes) col_contact<true>(a, s0 + i / kContactLanes, parity, i % kContactLanes);
+    __shared__ int seg[256 / kContactLanes * kMaxCand];  // a listing step's parts of each voxel's list
+    if (i < n * kContactLanes)
+        col_contact<true>(a, s0 + i / kContactLanes, parity, i % kContactLanes, seg + threadIdx.x / kContactLanes * kMaxCand);
 }
 
 // the contact phase and the link updates of one step side by side in one launch
@@ -809,12 +853,14 @@ template <typename T> __global__ __launch_bounds__(256) void k_col_contact(Dev<T
 // [0, nbl) update links, the rest compute contact forces
 template <typename T>
 __global__ __launch_bounds__(256) void k_contact_links(Dev<T> a, int nl, int nbl, int ns, int parity) {
+    __shared__ int seg[256 / kContactLanes * kMaxCand];  // a listing step's parts of each voxel's list
     if ((int)blockIdx.x < nbl) {
         const int l = blockIdx.x * blockDim.x + threadIdx.x;
         if (l < nl) link_update(a, l, parity);
     } else {
         const int i = (blockIdx.x - nbl) * blockDim.x + threadIdx.x;
-        if (i < ns * kContactLanes) col_contact<true>(a, i / kContactLanes, parity, i % kContactLanes);
+        if (i < ns * kContactLanes)
+            col_contact<true>(a, i / kContactLanes, parity, i % kContactLanes, seg + threadIdx.x / kContactLanes * kMaxCand);
     }
 }
 
@@ -1197,7 +1243,7 @@ template <typename T, int NT> __global__ __launch_bounds__(NT) void k_persistent
             }
             __syncthreads();
             for (int i = threadIdx.x; i < rp.scount * kContactLanes; i += NT)
-                col_contact<false>(a, rp.soff + i / kContactLanes, parity, i % kContactLanes);  // listed above
+                col_contact<false>(a, rp.soff + i / kContactLanes, parity, i % kContactLanes, nullptr);  // listed above
         }
         for (int l = rp.loff + threadIdx.x; l < rp.loff + rp.lcount; l += NT) link_update(a, l, parity);
         __syncthreads();
